@@ -1,0 +1,91 @@
+// Internal interface between the HIP kernel files and the C ABI (capi.hip).
+// Written for gfx950 (MI355X): wave64, 256 CUs in 8 XCDs, 160 KB LDS per CU.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstddef>
+
+namespace amb {
+
+// ---------------------------------------------------------------------------
+// Field storage.  Arithmetic is always FP32; StoreBF16 keeps the iteration
+// fields as bfloat16 (round to nearest even on store).
+// ---------------------------------------------------------------------------
+struct StoreF32 {
+    typedef float T;
+    static __device__ __forceinline__ float ld(const float* p, size_t i) { return p[i]; }
+    static __device__ __forceinline__ void st(float* p, size_t i, float v) { p[i] = v; }
+};
+
+struct StoreBF16 {
+    typedef uint16_t T;
+    static __device__ __forceinline__ float ld(const uint16_t* p, size_t i) {
+        return __uint_as_float(uint32_t(p[i]) << 16);
+    }
+    static __device__ __forceinline__ uint16_t rne(float v) {
+        uint32_t u = __float_as_uint(v);
+        if ((u & 0x7f800000u) == 0x7f800000u) return uint16_t((u >> 16) | ((u & 0xffffu) ? 0x40u : 0u));
+        u += 0x7fffu + ((u >> 16) & 1u);
+        return uint16_t(u >> 16);
+    }
+    static __device__ __forceinline__ void st(uint16_t* p, size_t i, float v) { p[i] = rne(v); }
+};
+
+// Field indices inside the 17-field state block (C ABI order).
+enum Field {
+    F_U = 0, F_UBAR = 1, F_VX = 2, F_VY = 3, F_VZ = 4, F_VBX = 5, F_VBY = 6, F_VBZ = 7,
+    F_PX = 8, F_PY = 9, F_PZ = 10, F_QXX = 11, F_QYY = 12, F_QZZ = 13, F_QXY = 14,
+    F_QXZ = 15, F_QYZ = 16, N_FIELDS = 17
+};
+
+struct Geometry {
+    int w, h, d;
+    __host__ __device__ size_t n() const { return size_t(w) * size_t(h) * size_t(d); }
+};
+
+struct SweepParams {
+    float lambda, alpha0, alpha1, tau, sigma, theta;
+    int data_term;   // 0 L1, 1 L2
+    int zchunk;      // planes per block along z
+};
+
+// Launchers (stream-ordered, asynchronous).  `fields` points at 17 arrays of
+// the storage type (void* so both storages share the signature).
+hipError_t launch_init(hipStream_t s, int storage, void* const* fields, const float* f, Geometry g);
+hipError_t launch_dual_sweep(hipStream_t s, int storage, void* const* fields, Geometry g,
+                             const SweepParams& p);
+hipError_t launch_primal_sweep(hipStream_t s, int storage, void* const* fields, const float* f,
+                               Geometry g, const SweepParams& p);
+hipError_t launch_convert(hipStream_t s, int storage, const void* src, void* dst, size_t n,
+                          int to_storage);
+
+// DCT / Poisson / divergence / epilogue (dct.hip)
+struct CosTables;   // device-resident cosine tables for one geometry
+hipError_t launch_divergence(hipStream_t s, const float* vx, const float* vy, const float* vz,
+                             float* out, Geometry g);
+hipError_t launch_dct_reference_pass(hipStream_t s, const float* in, float* out,
+                                     const double* table, Geometry g, int axis, int inverse,
+                                     int last);
+hipError_t launch_dct_mfma_pass(hipStream_t s, const float* in, float* out, const float* table,
+                                const uint16_t* table_hi, const uint16_t* table_lo, Geometry g,
+                                int axis, int inverse, int last, int precision,
+                                const float* eig_x, const float* eig_y, const float* eig_z,
+                                int fuse_poisson, int quirk);
+hipError_t launch_poisson_divide(hipStream_t s, const float* in, float* out, const float* cx,
+                                 const float* cy, const float* cz, Geometry g, int quirk);
+hipError_t launch_epilogue(hipStream_t s, const float* u, float* l, const float* mask, float* r,
+                           size_t n, int mask_shading, int set_negative_zero,
+                           int add_background_back);
+
+// FP64 energy / gap (energy.hip).  partials: at least energy_partials_size(g) doubles.
+size_t energy_partials_size(Geometry g);
+hipError_t launch_energy(hipStream_t s, int storage, void* const* fields, const float* f,
+                         Geometry g, double lambda, double alpha0, double alpha1, int data_term,
+                         double* partials, double* out10);
+
+// Host-side cos(pi t) from the mathematics (exact reduction modulo 2, octant
+// folding, libm cos/sin on |pi r| <= pi/4).
+double host_cospi(double t);
+
+}  // namespace amb

@@ -1,0 +1,850 @@
+// C ABI of the MI355X TGV-DCT deshade path (declared in include/ambient_amd.h).
+//
+// Host orchestration of the reference launchers' work, device-resident:
+//   tgv2_l1_deshade_launch  (reference tgv_deshade/tgv2_l1_deshade.cu:33-303)
+//   tgv2_l1/l2_launch       (tgv_cuda/tgv2_l1.cu:64-321, tgv2_l2.cu:60-316)
+//   DCT Poisson solve       (tgv_deshade/TGVDeshadeProcessor.cpp:386-408)
+//   deshade composition     (TGVDeshadeProcessor.cpp:71-95, 122-166, 224-282)
+// The reference allocates 32 device arrays per call and copies every
+// intermediate back to the host; here a context keeps 17 state fields + f +
+// 5 scratch volumes resident and all steps run on the context's stream.
+#include "amb_internal.hpp"
+#include "../../include/ambient_amd.h"
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <cstdlib>
+#include <string>
+#include <vector>
+
+using namespace amb;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+amb_status fail(amb_status st, const std::string& msg) {
+    g_last_error = msg;
+    return st;
+}
+
+amb_status fail_hip(hipError_t e, const char* what, int line) {
+    char buf[512];
+    std::snprintf(buf, sizeof(buf), "HIP error %d (%s) in %s [capi.hip:%d]", int(e),
+                  hipGetErrorString(e), what, line);
+    g_last_error = buf;
+    return e == hipErrorOutOfMemory ? AMB_ERR_OUT_OF_MEMORY : AMB_ERR_HIP;
+}
+
+#define AMB_HIP(call)                                                     \
+    do {                                                                  \
+        hipError_t e_ = (call);                                           \
+        if (e_ != hipSuccess) return fail_hip(e_, #call, __LINE__);       \
+    } while (0)
+
+#define AMB_TRY(call)                                                     \
+    do {                                                                  \
+        amb_status s_ = (call);                                           \
+        if (s_ != AMB_OK) return s_;                                      \
+    } while (0)
+
+constexpr int N_SCRATCH = 5;
+constexpr int N_PROF = 8;
+enum { P_DUAL = 0, P_PRIMAL = 1, P_DCT = 2, P_POISSON = 3, P_DIV = 4, P_EPI = 5, P_INIT = 6,
+       P_ENERGY = 7 };
+
+struct AxisTables {
+    int n = 0;
+    double* fwd64 = nullptr;     // [m][k] cospi((m+0.5)k/n)
+    double* inv64 = nullptr;     // [m][k] cospi((k+0.5)m/n)
+    float* fwd32 = nullptr;      // FP32 copies for the MFMA passes, [k][m] layout
+    float* inv32 = nullptr;
+    uint16_t* fwd_hi = nullptr;  // bf16 splits, [k][m]
+    uint16_t* fwd_lo = nullptr;
+    uint16_t* inv_hi = nullptr;
+    uint16_t* inv_lo = nullptr;
+    float* eig = nullptr;        // float(cospi(float(k)/float(n))), n + 1 entries
+};
+
+}  // namespace
+
+struct amb_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    // state
+    Geometry g{0, 0, 0};
+    int storage = 0;
+    size_t cap_bytes = 0;      // bytes of the fields block
+    size_t cap_n = 0;          // voxels the float buffers can hold
+    char* block = nullptr;     // fields
+    void* fields[N_FIELDS] = {};
+    void** fields_dev = nullptr;   // device copy of the field pointer table
+    float* f = nullptr;
+    float* scratch[N_SCRATCH] = {};
+    char* fblock = nullptr;    // f + scratch
+    double* partials = nullptr;
+    size_t partials_cap = 0;
+    double* energy_out = nullptr;
+    bool have_state = false;
+    SweepParams sp{};
+    amb_tgv_params params{};
+    // cosine tables per axis, keyed by extent
+    AxisTables axes[3];
+    // profiling
+    bool prof_on = false;
+    std::vector<hipEvent_t> pool;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> pending[N_PROF];
+    double prof_ms[N_PROF] = {};
+    uint64_t prof_n[N_PROF] = {};
+    // host staging for callbacks
+    std::vector<float> cb_u, cb_v[3];
+};
+
+namespace {
+
+size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+amb_status set_device(amb_context* c) {
+    AMB_HIP(hipSetDevice(c->device));
+    return AMB_OK;
+}
+
+hipEvent_t take_event(amb_context* c) {
+    if (!c->pool.empty()) {
+        hipEvent_t e = c->pool.back();
+        c->pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+struct ProfScope {
+    amb_context* c;
+    int cls;
+    hipEvent_t a = nullptr;
+    ProfScope(amb_context* c_, int cls_) : c(c_), cls(cls_) {
+        if (c->prof_on) {
+            a = take_event(c);
+            if (a) (void)hipEventRecord(a, c->stream);
+        }
+    }
+    ~ProfScope() {
+        if (a) {
+            hipEvent_t b = take_event(c);
+            if (b) {
+                (void)hipEventRecord(b, c->stream);
+                c->pending[cls].push_back({a, b});
+            } else {
+                c->pool.push_back(a);
+            }
+        }
+    }
+};
+
+amb_status drain_profile(amb_context* c) {
+    AMB_HIP(hipStreamSynchronize(c->stream));
+    for (int k = 0; k < N_PROF; ++k) {
+        for (auto& pr : c->pending[k]) {
+            float ms = 0.f;
+            AMB_HIP(hipEventElapsedTime(&ms, pr.first, pr.second));
+            c->prof_ms[k] += ms;
+            c->prof_n[k] += 1;
+            c->pool.push_back(pr.first);
+            c->pool.push_back(pr.second);
+        }
+        c->pending[k].clear();
+    }
+    return AMB_OK;
+}
+
+amb_status check_geometry(uint32_t w, uint32_t h, uint32_t d) {
+    if (w == 0 || h == 0 || d == 0) return fail(AMB_ERR_INVALID_ARGUMENT, "zero-sized volume");
+    const double n = double(w) * double(h) * double(d);
+    if (n > 2147483647.0) return fail(AMB_ERR_INVALID_ARGUMENT, "volume exceeds 2^31-1 voxels");
+    if (w > 65536 || h > 65536 || d > 65536)
+        return fail(AMB_ERR_INVALID_ARGUMENT, "extent exceeds 65536");
+    return AMB_OK;
+}
+
+amb_status check_params(const amb_tgv_params* p) {
+    if (!p) return fail(AMB_ERR_INVALID_ARGUMENT, "params is NULL");
+    if (p->data_term > 1) return fail(AMB_ERR_INVALID_ARGUMENT, "data_term must be L1 or L2");
+    if (p->storage > 1) return fail(AMB_ERR_INVALID_ARGUMENT, "storage must be F32 or BF16");
+    if (p->dct_precision > 2) return fail(AMB_ERR_INVALID_ARGUMENT, "unknown dct_precision");
+    if (!(p->alpha0 > 0.f) || !(p->alpha1 > 0.f))
+        return fail(AMB_ERR_INVALID_ARGUMENT, "alpha0 and alpha1 must be positive");
+    if (!(p->tau > 0.f) || !(p->sigma > 0.f))
+        return fail(AMB_ERR_INVALID_ARGUMENT, "tau and sigma must be positive");
+    if (!(p->lambda >= 0.f)) return fail(AMB_ERR_INVALID_ARGUMENT, "lambda must be >= 0");
+    return AMB_OK;
+}
+
+// Allocate (or reuse) the state block for geometry g and the storage type.
+amb_status ensure_buffers(amb_context* c, Geometry g, int storage) {
+    const size_t n = g.n();
+    const size_t elem = storage == 1 ? 2 : 4;
+    const size_t fstride = align_up(n * elem, 256);
+    const size_t need = fstride * N_FIELDS;
+    if (need > c->cap_bytes) {
+        if (c->block) AMB_HIP(hipFree(c->block));
+        c->block = nullptr;
+        c->cap_bytes = 0;
+        AMB_HIP(hipMalloc(reinterpret_cast<void**>(&c->block), need));
+        c->cap_bytes = need;
+    }
+    for (int k = 0; k < N_FIELDS; ++k) c->fields[k] = c->block + size_t(k) * fstride;
+    const size_t fl = align_up(n * 4, 256);
+    if (n > c->cap_n) {
+        if (c->fblock) AMB_HIP(hipFree(c->fblock));
+        c->fblock = nullptr;
+        c->cap_n = 0;
+        AMB_HIP(hipMalloc(reinterpret_cast<void**>(&c->fblock), fl * (1 + N_SCRATCH)));
+        c->cap_n = n;
+    }
+    c->f = reinterpret_cast<float*>(c->fblock);
+    for (int k = 0; k < N_SCRATCH; ++k)
+        c->scratch[k] = reinterpret_cast<float*>(c->fblock + fl * size_t(1 + k));
+    if (!c->fields_dev) AMB_HIP(hipMalloc(reinterpret_cast<void**>(&c->fields_dev), sizeof(void*) * N_FIELDS));
+    AMB_HIP(hipMemcpyAsync(c->fields_dev, c->fields, sizeof(void*) * N_FIELDS,
+                           hipMemcpyHostToDevice, c->stream));
+    const size_t pn = energy_partials_size(g);
+    if (pn > c->partials_cap) {
+        if (c->partials) AMB_HIP(hipFree(c->partials));
+        c->partials = nullptr;
+        AMB_HIP(hipMalloc(reinterpret_cast<void**>(&c->partials), pn * sizeof(double)));
+        c->partials_cap = pn;
+    }
+    if (!c->energy_out) AMB_HIP(hipMalloc(reinterpret_cast<void**>(&c->energy_out), 16 * sizeof(double)));
+    c->g = g;
+    c->storage = storage;
+    return AMB_OK;
+}
+
+void free_axis(AxisTables& t) {
+    void* ptrs[] = {t.fwd64, t.inv64, t.fwd32, t.inv32, t.fwd_hi, t.fwd_lo, t.inv_hi, t.inv_lo, t.eig};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    t = AxisTables();
+}
+
+uint16_t host_bf16_rne(float v) {
+    uint32_t u;
+    std::memcpy(&u, &v, 4);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return uint16_t(u >> 16);
+}
+float host_bf16_to_float(uint16_t h) {
+    uint32_t u = uint32_t(h) << 16;
+    float v;
+    std::memcpy(&v, &u, 4);
+    return v;
+}
+
+// Cosine tables for one axis of extent n (built once per extent, host FP64).
+amb_status ensure_axis(amb_context* c, int axis, int n) {
+    AxisTables& t = c->axes[axis];
+    if (t.n == n) return AMB_OK;
+    free_axis(t);
+    const size_t nn = size_t(n) * n;
+    std::vector<double> f64(nn), i64v(nn);
+    std::vector<float> f32(nn), i32(nn);
+    std::vector<uint16_t> fh(nn), fl(nn), ih(nn), il(nn);
+    const double size = double(float(n));
+    for (int m = 0; m < n; ++m)
+        for (int k = 0; k < n; ++k) {
+            f64[size_t(m) * n + k] = host_cospi((double(m) + 0.5) * double(k) / size);
+            i64v[size_t(m) * n + k] = host_cospi((double(k) + 0.5) * double(m) / size);
+        }
+    // MFMA tables, [k][m] (row = output index, contiguous over the summed index)
+    for (int k = 0; k < n; ++k)
+        for (int m = 0; m < n; ++m) {
+            const double cf = f64[size_t(m) * n + k];
+            // inverse: out[k] = in[0] + 2 sum_{m>=1} in[m] cos  -> fold the factor 2 and the
+            // m = 0 term into the matrix (weights 1 and 2)
+            const double ci = (m == 0 ? 1.0 : 2.0) * i64v[size_t(m) * n + k];
+            f32[size_t(k) * n + m] = float(cf);
+            i32[size_t(k) * n + m] = float(ci);
+            const uint16_t h1 = host_bf16_rne(float(cf));
+            fh[size_t(k) * n + m] = h1;
+            fl[size_t(k) * n + m] = host_bf16_rne(float(cf - double(host_bf16_to_float(h1))));
+            const uint16_t h2 = host_bf16_rne(float(ci));
+            ih[size_t(k) * n + m] = h2;
+            il[size_t(k) * n + m] = host_bf16_rne(float(ci - double(host_bf16_to_float(h2))));
+        }
+    std::vector<float> eig(size_t(n) + 1);
+    for (int k = 0; k <= n; ++k) eig[k] = float(host_cospi(double(float(k) / float(n))));
+    AMB_HIP(hipMalloc(reinterpret_cast<void**>(&t.fwd64), nn * 8));
+    AMB_HIP(hipMalloc(reinterpret_cast<void**>(&t.inv64), nn * 8));
+    AMB_HIP(hipMalloc(reinterpret_cast<void**>(&t.fwd32), nn * 4));
+    AMB_HIP(hipMalloc(reinterpret_cast<void**>(&t.inv32), nn * 4));
+    AMB_HIP(hipMalloc(reinterpret_cast<void**>(&t.fwd_hi), nn * 2));
+    AMB_HIP(hipMalloc(reinterpret_cast<void**>(&t.fwd_lo), nn * 2));
+    AMB_HIP(hipMalloc(reinterpret_cast<void**>(&t.inv_hi), nn * 2));
+    AMB_HIP(hipMalloc(reinterpret_cast<void**>(&t.inv_lo), nn * 2));
+    AMB_HIP(hipMalloc(reinterpret_cast<void**>(&t.eig), (size_t(n) + 1) * 4));
+    AMB_HIP(hipMemcpy(t.fwd64, f64.data(), nn * 8, hipMemcpyHostToDevice));
+    AMB_HIP(hipMemcpy(t.inv64, i64v.data(), nn * 8, hipMemcpyHostToDevice));
+    AMB_HIP(hipMemcpy(t.fwd32, f32.data(), nn * 4, hipMemcpyHostToDevice));
+    AMB_HIP(hipMemcpy(t.inv32, i32.data(), nn * 4, hipMemcpyHostToDevice));
+    AMB_HIP(hipMemcpy(t.fwd_hi, fh.data(), nn * 2, hipMemcpyHostToDevice));
+    AMB_HIP(hipMemcpy(t.fwd_lo, fl.data(), nn * 2, hipMemcpyHostToDevice));
+    AMB_HIP(hipMemcpy(t.inv_hi, ih.data(), nn * 2, hipMemcpyHostToDevice));
+    AMB_HIP(hipMemcpy(t.inv_lo, il.data(), nn * 2, hipMemcpyHostToDevice));
+    AMB_HIP(hipMemcpy(t.eig, eig.data(), (size_t(n) + 1) * 4, hipMemcpyHostToDevice));
+    t.n = n;
+    return AMB_OK;
+}
+
+amb_status ensure_tables(amb_context* c, Geometry g) {
+    AMB_TRY(ensure_axis(c, 0, g.w));
+    AMB_TRY(ensure_axis(c, 1, g.h));
+    AMB_TRY(ensure_axis(c, 2, g.d));
+    return AMB_OK;
+}
+
+hipMemcpyKind kind_in(amb_memory mem) {
+    return mem == AMB_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+}
+hipMemcpyKind kind_out(amb_memory mem) {
+    return mem == AMB_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+}
+
+int choose_zchunk(const Geometry& g) {
+    const char* env = std::getenv("AMB_ZCHUNK");
+    if (env) {
+        const int v = std::atoi(env);
+        if (v > 0) return v;
+    }
+    const long nxy = long((g.w + 63) / 64) * long((g.h + 3) / 4);
+    // aim for >= 4096 blocks (16 per CU) while keeping chunks >= 8 planes
+    long zc = (long(g.d) * nxy) / 4096;
+    if (zc < 8) zc = 8;
+    if (zc > 64) zc = 64;
+    if (zc > g.d) zc = g.d;
+    return int(zc);
+}
+
+// u (and optionally v) of the state as FP32 into dst (device), converting BF16.
+amb_status state_field_to_float(amb_context* c, int field, float* dst) {
+    if (c->storage == 0) {
+        AMB_HIP(hipMemcpyAsync(dst, c->fields[field], c->g.n() * 4, hipMemcpyDeviceToDevice,
+                               c->stream));
+    } else {
+        AMB_HIP(launch_convert(c->stream, 1, c->fields[field], dst, c->g.n(), 0));
+    }
+    return AMB_OK;
+}
+
+const float* state_field_f32_view(amb_context* c, int field, int scratch_idx) {
+    if (c->storage == 0) return static_cast<const float*>(c->fields[field]);
+    if (launch_convert(c->stream, 1, c->fields[field], c->scratch[scratch_idx], c->g.n(), 0) !=
+        hipSuccess)
+        return nullptr;
+    return c->scratch[scratch_idx];
+}
+
+amb_status copy_out(amb_context* c, amb_memory mem, float* dst, const float* src_dev) {
+    if (!dst) return AMB_OK;
+    AMB_HIP(hipMemcpyAsync(dst, src_dev, c->g.n() * 4, kind_out(mem), c->stream));
+    return AMB_OK;
+}
+
+amb_status copy_field_out(amb_context* c, amb_memory mem, float* dst, int field) {
+    if (!dst) return AMB_OK;
+    if (c->storage == 0) {
+        AMB_HIP(hipMemcpyAsync(dst, c->fields[field], c->g.n() * 4, kind_out(mem), c->stream));
+    } else if (mem == AMB_MEM_DEVICE) {
+        AMB_HIP(launch_convert(c->stream, 1, c->fields[field], dst, c->g.n(), 0));
+    } else {
+        AMB_HIP(launch_convert(c->stream, 1, c->fields[field], c->scratch[4], c->g.n(), 0));
+        AMB_HIP(hipMemcpyAsync(dst, c->scratch[4], c->g.n() * 4, hipMemcpyDeviceToHost, c->stream));
+    }
+    return AMB_OK;
+}
+
+amb_status begin(amb_context* c, amb_memory mem, const float* f, uint32_t w, uint32_t h,
+                 uint32_t d, const amb_tgv_params* p) {
+    AMB_TRY(check_geometry(w, h, d));
+    AMB_TRY(check_params(p));
+    if (!f) return fail(AMB_ERR_INVALID_ARGUMENT, "f is NULL");
+    AMB_TRY(set_device(c));
+    const Geometry g{int(w), int(h), int(d)};
+    AMB_TRY(ensure_buffers(c, g, int(p->storage)));
+    AMB_HIP(hipMemcpyAsync(c->f, f, g.n() * 4, kind_in(mem), c->stream));
+    {
+        ProfScope ps(c, P_INIT);
+        AMB_HIP(launch_init(c->stream, c->storage, c->fields, c->f, g));
+    }
+    c->params = *p;
+    c->sp.lambda = p->lambda; c->sp.alpha0 = p->alpha0; c->sp.alpha1 = p->alpha1;
+    c->sp.tau = p->tau; c->sp.sigma = p->sigma; c->sp.theta = p->theta;
+    c->sp.data_term = int(p->data_term);
+    c->sp.zchunk = choose_zchunk(g);
+    c->have_state = true;
+    return AMB_OK;
+}
+
+amb_status sweep(amb_context* c, int which) {
+    if (which == 0) {
+        ProfScope ps(c, P_DUAL);
+        AMB_HIP(launch_dual_sweep(c->stream, c->storage, c->fields, c->g, c->sp));
+    } else {
+        ProfScope ps(c, P_PRIMAL);
+        AMB_HIP(launch_primal_sweep(c->stream, c->storage, c->fields, c->f, c->g, c->sp));
+    }
+    return AMB_OK;
+}
+
+// The reference loop with its callback placement (tgv2_l1_deshade.cu:169-272,
+// callback rule tgv2_common.cu:381-383).
+amb_status run(amb_context* c, uint32_t iterations, uint32_t interval, amb_iteration_callback cb,
+               void* user, bool with_v) {
+    for (uint32_t it = 0; it < iterations; ++it) {
+        AMB_TRY(sweep(c, 0));
+        AMB_TRY(sweep(c, 1));
+        if (cb && interval > 0 && it > 0 && it % interval == 0) {
+            const size_t n = c->g.n();
+            c->cb_u.resize(n);
+            AMB_TRY(copy_field_out(c, AMB_MEM_HOST, c->cb_u.data(), F_U));
+            if (with_v)
+                for (int k = 0; k < 3; ++k) {
+                    c->cb_v[k].resize(n);
+                    AMB_TRY(copy_field_out(c, AMB_MEM_HOST, c->cb_v[k].data(), F_VX + k));
+                }
+            AMB_HIP(hipStreamSynchronize(c->stream));
+            const int stop = cb(it, iterations, c->cb_u.data(), with_v ? c->cb_v[0].data() : nullptr,
+                                with_v ? c->cb_v[1].data() : nullptr,
+                                with_v ? c->cb_v[2].data() : nullptr, user);
+            if (stop) break;
+        }
+    }
+    return AMB_OK;
+}
+
+// One separable 3-pass transform from src into dst (src is overwritten as scratch).
+amb_status transform3(amb_context* c, float* src, float* tmp, float* dst, int inverse,
+                      int precision) {
+    const Geometry g = c->g;
+    AMB_TRY(ensure_tables(c, g));
+    float* bufs[4] = {src, tmp, src, dst};   // x: src->tmp, y: tmp->src, z: src->dst
+    for (int axis = 0; axis < 3; ++axis) {
+        ProfScope ps(c, P_DCT);
+        const AxisTables& t = c->axes[axis];
+        const int last = axis == 2;
+        if (precision == AMB_DCT_REFERENCE) {
+            AMB_HIP(launch_dct_reference_pass(c->stream, bufs[axis], bufs[axis + 1],
+                                              inverse ? t.inv64 : t.fwd64, g, axis, inverse, last));
+        } else {
+            AMB_HIP(launch_dct_mfma_pass(c->stream, bufs[axis], bufs[axis + 1],
+                                         inverse ? t.inv32 : t.fwd32, inverse ? t.inv_hi : t.fwd_hi,
+                                         inverse ? t.inv_lo : t.fwd_lo, g, axis, inverse, last,
+                                         precision, nullptr, nullptr, nullptr, 0, 0));
+        }
+    }
+    return AMB_OK;
+}
+
+amb_status poisson(amb_context* c, const float* in, float* out, uint32_t flags) {
+    AMB_TRY(ensure_tables(c, c->g));
+    ProfScope ps(c, P_POISSON);
+    AMB_HIP(launch_poisson_divide(c->stream, in, out, c->axes[0].eig, c->axes[1].eig,
+                                  c->axes[2].eig, c->g, (flags & AMB_FLAG_POISSON_INDEX_QUIRK) ? 1 : 0));
+    return AMB_OK;
+}
+
+// l = iDCT(P(DCT(B-div(vx, vy, vz)))) into scratch[1]; vx.. are device FP32.
+amb_status integrate(amb_context* c, const float* vx, const float* vy, const float* vz,
+                     int precision, uint32_t flags, float** result) {
+    float* s0 = c->scratch[0];
+    float* s1 = c->scratch[1];
+    float* s2 = c->scratch[2];
+    {
+        ProfScope ps(c, P_DIV);
+        AMB_HIP(launch_divergence(c->stream, vx, vy, vz, s0, c->g));
+    }
+    AMB_TRY(transform3(c, s0, s1, s2, 0, precision));     // DCT -> s2
+    AMB_TRY(poisson(c, s2, s0, flags));                    // -> s0
+    AMB_TRY(transform3(c, s0, s2, s1, 1, precision));     // iDCT -> s1
+    *result = s1;
+    return AMB_OK;
+}
+
+// Upload three host/device volumes into scratch[2..4] (device FP32 views).
+amb_status stage3(amb_context* c, amb_memory mem, const float* a, const float* b, const float* d,
+                  const float** out) {
+    const float* in[3] = {a, b, d};
+    for (int k = 0; k < 3; ++k) {
+        if (!in[k]) {
+            if (k == 2 && c->g.d == 1) {
+                out[k] = c->scratch[2 + k];   // unused for depth 1
+                continue;
+            }
+            return fail(AMB_ERR_INVALID_ARGUMENT, "gradient component is NULL");
+        }
+        if (mem == AMB_MEM_DEVICE) {
+            out[k] = in[k];
+        } else {
+            AMB_HIP(hipMemcpyAsync(c->scratch[2 + k], in[k], c->g.n() * 4, hipMemcpyHostToDevice,
+                                   c->stream));
+            out[k] = c->scratch[2 + k];
+        }
+    }
+    return AMB_OK;
+}
+
+amb_status geometry_only(amb_context* c, uint32_t w, uint32_t h, uint32_t d) {
+    AMB_TRY(check_geometry(w, h, d));
+    AMB_TRY(set_device(c));
+    const Geometry g{int(w), int(h), int(d)};
+    // keep the iteration state if the geometry matches, else (re)allocate FP32
+    if (!(c->have_state && c->g.w == g.w && c->g.h == g.h && c->g.d == g.d)) {
+        AMB_TRY(ensure_buffers(c, g, c->have_state ? c->storage : 0));
+        c->have_state = false;
+    }
+    return AMB_OK;
+}
+
+}  // namespace
+
+// ===========================================================================
+extern "C" {
+
+AMB_API const char* amb_last_error(void) { return g_last_error.c_str(); }
+AMB_API const char* amb_version(void) { return "ambient_amd 0.1 gfx950"; }
+
+AMB_API void amb_tgv_params_default(amb_tgv_params* p) {
+    if (!p) return;
+    std::memset(p, 0, sizeof(*p));
+    p->lambda = 1.0f;
+    p->alpha0 = 2.0f;
+    p->alpha1 = 1.0f;
+    p->tau = float(1.0 / std::sqrt(20.0));   // tgv2_l1_deshade.cu:148 (Pixel tau = 1.0 / sqrt(20.0))
+    p->sigma = p->tau;
+    p->theta = 1.0f;
+    p->iterations = 10000;
+    p->data_term = AMB_DATA_L1;
+    p->storage = AMB_STORAGE_F32;
+    p->dct_precision = AMB_DCT_REFERENCE;
+    p->callback_interval = 0;
+    p->flags = AMB_FLAG_SET_NEGATIVE_ZERO;
+}
+
+AMB_API amb_status amb_context_create(int device, amb_context** out) {
+    if (!out) return fail(AMB_ERR_INVALID_ARGUMENT, "ctx is NULL");
+    *out = nullptr;
+    int count = 0;
+    AMB_HIP(hipGetDeviceCount(&count));
+    if (device < 0 || device >= count) return fail(AMB_ERR_INVALID_ARGUMENT, "no such device");
+    AMB_HIP(hipSetDevice(device));
+    amb_context* c = new amb_context();
+    c->device = device;
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete c;
+        return fail_hip(e, "hipStreamCreateWithFlags", __LINE__);
+    }
+    *out = c;
+    return AMB_OK;
+}
+
+AMB_API amb_status amb_context_destroy(amb_context* c) {
+    if (!c) return AMB_OK;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    for (int k = 0; k < N_PROF; ++k)
+        for (auto& pr : c->pending[k]) {
+            (void)hipEventDestroy(pr.first);
+            (void)hipEventDestroy(pr.second);
+        }
+    for (hipEvent_t e : c->pool) (void)hipEventDestroy(e);
+    for (auto& t : c->axes) free_axis(t);
+    if (c->block) (void)hipFree(c->block);
+    if (c->fblock) (void)hipFree(c->fblock);
+    if (c->fields_dev) (void)hipFree(c->fields_dev);
+    if (c->partials) (void)hipFree(c->partials);
+    if (c->energy_out) (void)hipFree(c->energy_out);
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return AMB_OK;
+}
+
+AMB_API amb_status amb_device_alloc(amb_context* c, size_t bytes, void** ptr) {
+    if (!c || !ptr) return fail(AMB_ERR_INVALID_ARGUMENT, "NULL argument");
+    AMB_TRY(set_device(c));
+    AMB_HIP(hipMalloc(ptr, bytes));
+    return AMB_OK;
+}
+
+AMB_API amb_status amb_device_free(amb_context* c, void* ptr) {
+    if (!c) return fail(AMB_ERR_INVALID_ARGUMENT, "ctx is NULL");
+    AMB_TRY(set_device(c));
+    AMB_HIP(hipStreamSynchronize(c->stream));
+    AMB_HIP(hipFree(ptr));
+    return AMB_OK;
+}
+
+AMB_API amb_status amb_memcpy(amb_context* c, void* dst, const void* src, size_t bytes, int kind) {
+    if (!c || (!dst && bytes) || (!src && bytes)) return fail(AMB_ERR_INVALID_ARGUMENT, "NULL argument");
+    AMB_TRY(set_device(c));
+    const hipMemcpyKind k = kind == 0 ? hipMemcpyHostToDevice
+                          : kind == 1 ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
+    AMB_HIP(hipMemcpyAsync(dst, src, bytes, k, c->stream));
+    AMB_HIP(hipStreamSynchronize(c->stream));
+    return AMB_OK;
+}
+
+AMB_API amb_status amb_synchronize(amb_context* c) {
+    if (!c) return fail(AMB_ERR_INVALID_ARGUMENT, "ctx is NULL");
+    AMB_TRY(set_device(c));
+    AMB_HIP(hipStreamSynchronize(c->stream));
+    return AMB_OK;
+}
+
+AMB_API void* amb_context_stream(amb_context* c) { return c ? static_cast<void*>(c->stream) : nullptr; }
+
+AMB_API amb_status amb_tgv2_begin(amb_context* c, amb_memory mem, const float* f, uint32_t w,
+                                  uint32_t h, uint32_t d, const amb_tgv_params* p) {
+    if (!c) return fail(AMB_ERR_INVALID_ARGUMENT, "ctx is NULL");
+    AMB_TRY(begin(c, mem, f, w, h, d, p));
+    AMB_HIP(hipStreamSynchronize(c->stream));
+    return AMB_OK;
+}
+
+AMB_API amb_status amb_tgv2_set_state(amb_context* c, amb_memory mem, const float* const* fields) {
+    if (!c || !fields) return fail(AMB_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (!c->have_state) return fail(AMB_ERR_INVALID_ARGUMENT, "no iteration state: call amb_tgv2_begin");
+    AMB_TRY(set_device(c));
+    const size_t n = c->g.n();
+    for (int k = 0; k < N_FIELDS; ++k) {
+        if (!fields[k]) return fail(AMB_ERR_INVALID_ARGUMENT, "field pointer is NULL");
+        if (c->storage == 0) {
+            AMB_HIP(hipMemcpyAsync(c->fields[k], fields[k], n * 4, kind_in(mem), c->stream));
+        } else {
+            const float* src = fields[k];
+            if (mem == AMB_MEM_HOST) {
+                AMB_HIP(hipMemcpyAsync(c->scratch[4], fields[k], n * 4, hipMemcpyHostToDevice, c->stream));
+                src = c->scratch[4];
+            }
+            AMB_HIP(launch_convert(c->stream, 1, src, c->fields[k], n, 1));
+        }
+    }
+    AMB_HIP(hipStreamSynchronize(c->stream));
+    return AMB_OK;
+}
+
+AMB_API amb_status amb_tgv2_get_state(amb_context* c, amb_memory mem, float* const* fields) {
+    if (!c || !fields) return fail(AMB_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (!c->have_state) return fail(AMB_ERR_INVALID_ARGUMENT, "no iteration state");
+    AMB_TRY(set_device(c));
+    for (int k = 0; k < N_FIELDS; ++k) {
+        AMB_TRY(copy_field_out(c, mem, fields[k], k));
+        if (c->storage == 1 && mem == AMB_MEM_HOST) AMB_HIP(hipStreamSynchronize(c->stream));
+    }
+    AMB_HIP(hipStreamSynchronize(c->stream));
+    return AMB_OK;
+}
+
+AMB_API amb_status amb_tgv2_iterate(amb_context* c, uint32_t iterations) {
+    if (!c) return fail(AMB_ERR_INVALID_ARGUMENT, "ctx is NULL");
+    if (!c->have_state) return fail(AMB_ERR_INVALID_ARGUMENT, "no iteration state: call amb_tgv2_begin");
+    AMB_TRY(set_device(c));
+    AMB_TRY(run(c, iterations, 0, nullptr, nullptr, false));
+    AMB_HIP(hipStreamSynchronize(c->stream));
+    return AMB_OK;
+}
+
+AMB_API amb_status amb_tgv2_sweep(amb_context* c, int which) {
+    if (!c) return fail(AMB_ERR_INVALID_ARGUMENT, "ctx is NULL");
+    if (!c->have_state) return fail(AMB_ERR_INVALID_ARGUMENT, "no iteration state: call amb_tgv2_begin");
+    if (which != 0 && which != 1) return fail(AMB_ERR_INVALID_ARGUMENT, "which must be 0 or 1");
+    AMB_TRY(set_device(c));
+    AMB_TRY(sweep(c, which));
+    AMB_HIP(hipStreamSynchronize(c->stream));
+    return AMB_OK;
+}
+
+AMB_API amb_status amb_tgv2_deshade(amb_context* c, amb_memory mem, const float* f, uint32_t w,
+                                    uint32_t h, uint32_t d, const amb_tgv_params* p, float* u,
+                                    float* vx, float* vy, float* vz, amb_iteration_callback cb,
+                                    void* user) {
+    if (!c) return fail(AMB_ERR_INVALID_ARGUMENT, "ctx is NULL");
+    AMB_TRY(begin(c, mem, f, w, h, d, p));
+    AMB_TRY(run(c, p->iterations, p->callback_interval, cb, user, true));
+    AMB_TRY(copy_field_out(c, mem, u, F_U));
+    AMB_TRY(copy_field_out(c, mem, vx, F_VX));
+    AMB_TRY(copy_field_out(c, mem, vy, F_VY));
+    AMB_TRY(copy_field_out(c, mem, vz, F_VZ));
+    AMB_HIP(hipStreamSynchronize(c->stream));
+    return AMB_OK;
+}
+
+AMB_API amb_status amb_tgv2_denoise(amb_context* c, amb_memory mem, const float* f, uint32_t w,
+                                    uint32_t h, uint32_t d, const amb_tgv_params* p, float* u,
+                                    amb_iteration_callback cb, void* user) {
+    if (!c) return fail(AMB_ERR_INVALID_ARGUMENT, "ctx is NULL");
+    AMB_TRY(begin(c, mem, f, w, h, d, p));
+    AMB_TRY(run(c, p->iterations, p->callback_interval, cb, user, false));
+    AMB_TRY(copy_field_out(c, mem, u, F_U));
+    AMB_HIP(hipStreamSynchronize(c->stream));
+    return AMB_OK;
+}
+
+AMB_API amb_status amb_divergence(amb_context* c, amb_memory mem, const float* dx, const float* dy,
+                                  const float* dz, uint32_t w, uint32_t h, uint32_t d, float* out) {
+    if (!c || !out) return fail(AMB_ERR_INVALID_ARGUMENT, "NULL argument");
+    AMB_TRY(geometry_only(c, w, h, d));
+    const float* v[3];
+    AMB_TRY(stage3(c, mem, dx, dy, dz, v));
+    {
+        ProfScope ps(c, P_DIV);
+        AMB_HIP(launch_divergence(c->stream, v[0], v[1], v[2], c->scratch[0], c->g));
+    }
+    AMB_TRY(copy_out(c, mem, out, c->scratch[0]));
+    AMB_HIP(hipStreamSynchronize(c->stream));
+    return AMB_OK;
+}
+
+static amb_status transform_entry(amb_context* c, amb_memory mem, const float* image, uint32_t w,
+                                  uint32_t h, uint32_t d, uint32_t precision, float* out,
+                                  int inverse) {
+    if (!c || !image || !out) return fail(AMB_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (precision > 2) return fail(AMB_ERR_INVALID_ARGUMENT, "unknown dct_precision");
+    AMB_TRY(geometry_only(c, w, h, d));
+    AMB_HIP(hipMemcpyAsync(c->scratch[0], image, c->g.n() * 4, kind_in(mem), c->stream));
+    AMB_TRY(transform3(c, c->scratch[0], c->scratch[1], c->scratch[2], inverse, int(precision)));
+    AMB_TRY(copy_out(c, mem, out, c->scratch[2]));
+    AMB_HIP(hipStreamSynchronize(c->stream));
+    return AMB_OK;
+}
+
+AMB_API amb_status amb_cosine_transform(amb_context* c, amb_memory mem, const float* image,
+                                        uint32_t w, uint32_t h, uint32_t d, uint32_t precision,
+                                        float* out) {
+    return transform_entry(c, mem, image, w, h, d, precision, out, 0);
+}
+
+AMB_API amb_status amb_inverse_cosine_transform(amb_context* c, amb_memory mem, const float* image,
+                                                uint32_t w, uint32_t h, uint32_t d,
+                                                uint32_t precision, float* out) {
+    return transform_entry(c, mem, image, w, h, d, precision, out, 1);
+}
+
+AMB_API amb_status amb_solve_poisson_in_cosine_domain(amb_context* c, amb_memory mem,
+                                                      const float* image, uint32_t w, uint32_t h,
+                                                      uint32_t d, uint32_t flags, float* out) {
+    if (!c || !image || !out) return fail(AMB_ERR_INVALID_ARGUMENT, "NULL argument");
+    AMB_TRY(geometry_only(c, w, h, d));
+    AMB_HIP(hipMemcpyAsync(c->scratch[0], image, c->g.n() * 4, kind_in(mem), c->stream));
+    AMB_TRY(poisson(c, c->scratch[0], c->scratch[1], flags));
+    AMB_TRY(copy_out(c, mem, out, c->scratch[1]));
+    AMB_HIP(hipStreamSynchronize(c->stream));
+    return AMB_OK;
+}
+
+AMB_API amb_status amb_integrate_gradients_poisson(amb_context* c, amb_memory mem, const float* vx,
+                                                   const float* vy, const float* vz, uint32_t w,
+                                                   uint32_t h, uint32_t d, uint32_t precision,
+                                                   uint32_t flags, float* shading) {
+    if (!c || !shading) return fail(AMB_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (precision > 2) return fail(AMB_ERR_INVALID_ARGUMENT, "unknown dct_precision");
+    AMB_TRY(geometry_only(c, w, h, d));
+    const float* v[3];
+    AMB_TRY(stage3(c, mem, vx, vy, vz, v));
+    float* l = nullptr;
+    AMB_TRY(integrate(c, v[0], v[1], v[2], int(precision), flags, &l));
+    AMB_TRY(copy_out(c, mem, shading, l));
+    AMB_HIP(hipStreamSynchronize(c->stream));
+    return AMB_OK;
+}
+
+AMB_API amb_status amb_tgv2_deshade_pipeline(amb_context* c, amb_memory mem, const float* f,
+                                             uint32_t w, uint32_t h, uint32_t d,
+                                             const amb_tgv_params* p, const float* mask, float* u,
+                                             float* shading, float* deshaded,
+                                             amb_iteration_callback cb, void* user) {
+    if (!c) return fail(AMB_ERR_INVALID_ARGUMENT, "ctx is NULL");
+    AMB_TRY(begin(c, mem, f, w, h, d, p));
+    AMB_TRY(run(c, p->iterations, p->callback_interval, cb, user, true));
+    // v as FP32 device views (BF16 storage: converted into scratch[2..4])
+    const float* v[3];
+    for (int k = 0; k < 3; ++k) {
+        if (c->storage == 0) {
+            v[k] = static_cast<const float*>(c->fields[F_VX + k]);
+        } else {
+            AMB_HIP(launch_convert(c->stream, 1, c->fields[F_VX + k], c->scratch[2 + k], c->g.n(), 0));
+            v[k] = c->scratch[2 + k];
+        }
+    }
+    float* l = nullptr;
+    AMB_TRY(integrate(c, v[0], v[1], v[2], int(p->dct_precision), p->flags, &l));   // l in scratch[1]
+    // u as FP32 (scratch[2] for BF16), mask into scratch[3], r into scratch[4]
+    const float* uf = static_cast<const float*>(c->fields[F_U]);
+    if (c->storage == 1) {
+        AMB_HIP(launch_convert(c->stream, 1, c->fields[F_U], c->scratch[2], c->g.n(), 0));
+        uf = c->scratch[2];
+    }
+    const float* m = nullptr;
+    if (mask) {
+        if (mem == AMB_MEM_DEVICE) {
+            m = mask;
+        } else {
+            AMB_HIP(hipMemcpyAsync(c->scratch[3], mask, c->g.n() * 4, hipMemcpyHostToDevice, c->stream));
+            m = c->scratch[3];
+        }
+    }
+    {
+        ProfScope ps(c, P_EPI);
+        AMB_HIP(launch_epilogue(c->stream, uf, l, m, c->scratch[4], c->g.n(),
+                                (p->flags & AMB_FLAG_MASK_SHADING) ? 1 : 0,
+                                (p->flags & AMB_FLAG_SET_NEGATIVE_ZERO) ? 1 : 0,
+                                (p->flags & AMB_FLAG_ADD_BACKGROUND_BACK) ? 1 : 0));
+    }
+    AMB_TRY(copy_out(c, mem, u, uf));
+    AMB_TRY(copy_out(c, mem, shading, l));
+    AMB_TRY(copy_out(c, mem, deshaded, c->scratch[4]));
+    AMB_HIP(hipStreamSynchronize(c->stream));
+    return AMB_OK;
+}
+
+AMB_API amb_status amb_tgv2_energy(amb_context* c, double out[10]) {
+    if (!c || !out) return fail(AMB_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (!c->have_state) return fail(AMB_ERR_INVALID_ARGUMENT, "no iteration state");
+    AMB_TRY(set_device(c));
+    {
+        ProfScope ps(c, P_ENERGY);
+        AMB_HIP(launch_energy(c->stream, c->storage, c->fields_dev, c->f, c->g, c->params.lambda,
+                              c->params.alpha0, c->params.alpha1, int(c->params.data_term),
+                              c->partials, c->energy_out));
+    }
+    AMB_HIP(hipMemcpyAsync(out, c->energy_out, 10 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    AMB_HIP(hipStreamSynchronize(c->stream));
+    return AMB_OK;
+}
+
+AMB_API amb_status amb_profile_enable(amb_context* c, int enable) {
+    if (!c) return fail(AMB_ERR_INVALID_ARGUMENT, "ctx is NULL");
+    c->prof_on = enable != 0;
+    return AMB_OK;
+}
+
+AMB_API amb_status amb_profile_read(amb_context* c, int cls, double* ms, uint64_t* n) {
+    if (!c || cls < 0 || cls >= N_PROF) return fail(AMB_ERR_INVALID_ARGUMENT, "bad argument");
+    AMB_TRY(set_device(c));
+    AMB_TRY(drain_profile(c));
+    if (ms) *ms = c->prof_ms[cls];
+    if (n) *n = c->prof_n[cls];
+    return AMB_OK;
+}
+
+AMB_API amb_status amb_profile_reset(amb_context* c) {
+    if (!c) return fail(AMB_ERR_INVALID_ARGUMENT, "ctx is NULL");
+    AMB_TRY(set_device(c));
+    AMB_TRY(drain_profile(c));
+    for (int k = 0; k < N_PROF; ++k) { c->prof_ms[k] = 0.0; c->prof_n[k] = 0; }
+    return AMB_OK;
+}
+
+}  // extern "C"

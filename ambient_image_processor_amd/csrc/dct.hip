@@ -1,0 +1,197 @@
+// DCT Poisson bias solve for gfx950: B-divergence, separable DCT-II/III
+// passes, Laplacian-eigenvalue divide, and the deshade epilogue.
+//
+// Reference: cuda_image_operations/divergence.cu:26-56, cosine_transform.cu:142-223,
+// solve_poisson_in_cosine_domain.cu:27-54, inverse_cosine_transform.cu:79-164,
+// subtract.cu:27, multiply.cu:25, clamp_negative_values.cu:27, add.cu:27,
+// invert.cu:26; composition tgv_deshade/TGVDeshadeProcessor.cpp:71-95,122-166,386-408.
+//
+// Built with -ffp-contract=off: every fused multiply-add is written as fma.
+#include "amb_internal.hpp"
+
+#include <cmath>
+
+namespace amb {
+
+double host_cospi(double t) {
+    t = std::fabs(t);
+    double r = std::fmod(t, 2.0);
+    if (r > 1.0) r = 2.0 - r;
+    double sgn = 1.0;
+    if (r > 0.5) { r = 1.0 - r; sgn = -1.0; }
+    if (r == 0.5) return 0.0;
+    const double v = (r <= 0.25) ? std::cos(M_PI * r) : std::sin(M_PI * (0.5 - r));
+    return sgn * v;
+}
+
+namespace {
+
+__device__ __forceinline__ void axis_info(const Geometry& g, int axis, size_t& stride, int& n) {
+    stride = axis == 0 ? 1 : (axis == 1 ? size_t(g.w) : size_t(g.w) * g.h);
+    n = axis == 0 ? g.w : (axis == 1 ? g.h : g.d);
+}
+
+// A14: g = (B_x v_x + B_y v_y) + B_z v_z (B_z only when depth > 1),
+// B a = -a[i] + a[i-1], -a[0] at i = 0 (tgv_cuda/tgv_common.cu:133-197).
+__global__ void divergence_kernel(const float* __restrict__ vx, const float* __restrict__ vy,
+                                  const float* __restrict__ vz, float* __restrict__ out,
+                                  Geometry g) {
+    const size_t n = g.n();
+    const size_t plane = size_t(g.w) * g.h;
+    const size_t stride = size_t(gridDim.x) * blockDim.x;
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const int x = int(i % g.w);
+        const int y = int((i / g.w) % g.h);
+        const int z = int(i / plane);
+        const float bx = x > 0 ? -vx[i] + vx[i - 1] : -vx[i];
+        const float by = y > 0 ? -vy[i] + vy[i - g.w] : -vy[i];
+        float r = bx + by;
+        if (g.d > 1) {
+            const float bz = z > 0 ? -vz[i] + vz[i - plane] : -vz[i];
+            r = r + bz;
+        }
+        out[i] = r;
+    }
+}
+
+// A15/A17 in the reference's numerics: one thread per output element;
+// term = (double)x[m] * cos64 added into the FP32 accumulator (fma in double,
+// rounded to float each term), reference term order.
+//   forward:  acc = sum_{m=0}^{n-1} in[m] * T[m][k]; last pass: acc *= 8
+//   inverse:  acc = sum_{m=1}^{n-1} in[m] * T[m][k]; acc *= 2; acc += in[0];
+//             last pass: acc = acc / (W*H*D*8.0) in double
+// T[m][k] = cospi((m + 0.5) k / n) forward, cospi((k + 0.5) m / n) inverse.
+template <bool INVERSE>
+__global__ void dct_reference_kernel(const float* __restrict__ in, float* __restrict__ out,
+                                     const double* __restrict__ table, Geometry g, int axis,
+                                     int last) {
+    size_t st;
+    int n;
+    axis_info(g, axis, st, n);
+    const size_t total = g.n();
+    const double vol8 = double(size_t(g.w) * g.h * g.d) * 8.0;
+    const size_t stride = size_t(gridDim.x) * blockDim.x;
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < total; i += stride) {
+        const int k = int((i / st) % size_t(n));
+        const size_t base = i - size_t(k) * st;
+        float acc = 0.0f;
+        if (!INVERSE) {
+            for (int m = 0; m < n; ++m)
+                acc = float(fma(double(in[base + size_t(m) * st]), table[size_t(m) * n + k], double(acc)));
+            if (last) acc = acc * 8.0f;
+        } else {
+            for (int m = 1; m < n; ++m)
+                acc = float(fma(double(in[base + size_t(m) * st]), table[size_t(m) * n + k], double(acc)));
+            acc = acc * 2.0f;
+            acc = acc + in[base];
+            if (last) acc = float(double(acc) / vol8);
+        }
+        out[i] = acc;
+    }
+}
+
+// Voxel coordinates for the Poisson divide.  quirk != 0 reproduces the
+// reference's float arithmetic (solve_poisson_in_cosine_domain.cu:36-40),
+// which differs from the exact coordinates only for linear index >= 2^24
+// (y may then reach H; the cy table has H + 1 entries).
+__device__ __forceinline__ void poisson_coords(size_t i, const Geometry& g, int quirk, int& x,
+                                               int& y, int& z) {
+    const unsigned wh = unsigned(g.w) * unsigned(g.h);
+    if (!quirk) {
+        x = int(i % g.w); y = int((i / g.w) % g.h); z = int(i / wh);
+        return;
+    }
+    const float zf = floorf(float(unsigned(i) / wh));
+    int rest = int(float(int(i)) - zf * float(wh));
+    const float yf = floorf(float(unsigned(rest) / unsigned(g.w)));
+    rest = int(float(rest) - yf * float(unsigned(g.w)));
+    x = rest; y = int(yf); z = int(zf);
+}
+
+// A16: X / (6.0 - 2.0*cx[x] - 2.0*cy[y] - 2.0*cz[z]), (0,0,0) -> 0; cx etc. are the
+// reference's FP32 cospif(x / W) values, the denominator and quotient are double.
+__global__ void poisson_kernel(const float* __restrict__ in, float* __restrict__ out,
+                               const float* __restrict__ cx, const float* __restrict__ cy,
+                               const float* __restrict__ cz, Geometry g, int quirk) {
+    const size_t total = g.n();
+    const size_t stride = size_t(gridDim.x) * blockDim.x;
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < total; i += stride) {
+        int x, y, z;
+        poisson_coords(i, g, quirk, x, y, z);
+        if (x == 0 && y == 0 && z == 0) { out[i] = 0.0f; continue; }
+        const double den = 6.0 - 2.0 * double(cx[x]) - 2.0 * double(cy[y]) - 2.0 * double(cz[z]);
+        out[i] = float(double(in[i]) / den);
+    }
+}
+
+// A19 epilogue: r = u - l ; mask ; clamp ; add-back (see ambient_amd.h flags).
+__global__ void epilogue_kernel(const float* __restrict__ u, float* __restrict__ l,
+                                const float* __restrict__ mask, float* __restrict__ r, size_t n,
+                                int mask_shading, int set_negative_zero, int add_background_back) {
+    const size_t stride = size_t(gridDim.x) * blockDim.x;
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const float uv = u[i];
+        float ri = uv - l[i];
+        if (mask) {
+            const float m = mask[i];
+            if (mask_shading) l[i] = l[i] * m;
+            ri = ri * m;
+        }
+        if (set_negative_zero && ri < 0.0f) ri = 0.0f;
+        if (add_background_back && mask) {
+            const float bg = float(1.0 - double(mask[i]));
+            ri = ri + uv * bg;
+        }
+        r[i] = ri;
+    }
+}
+
+inline int elem_blocks(size_t n) {
+    size_t b = (n + 255) / 256;
+    if (b > 16384) b = 16384;
+    return int(b ? b : 1);
+}
+
+}  // namespace
+
+hipError_t launch_divergence(hipStream_t s, const float* vx, const float* vy, const float* vz,
+                             float* out, Geometry g) {
+    hipLaunchKernelGGL(divergence_kernel, dim3(elem_blocks(g.n())), dim3(256), 0, s, vx, vy, vz,
+                       out, g);
+    return hipGetLastError();
+}
+
+hipError_t launch_dct_reference_pass(hipStream_t s, const float* in, float* out,
+                                     const double* table, Geometry g, int axis, int inverse,
+                                     int last) {
+    if (inverse)
+        hipLaunchKernelGGL(dct_reference_kernel<true>, dim3(elem_blocks(g.n())), dim3(256), 0, s,
+                           in, out, table, g, axis, last);
+    else
+        hipLaunchKernelGGL(dct_reference_kernel<false>, dim3(elem_blocks(g.n())), dim3(256), 0, s,
+                           in, out, table, g, axis, last);
+    return hipGetLastError();
+}
+
+hipError_t launch_poisson_divide(hipStream_t s, const float* in, float* out, const float* cx,
+                                 const float* cy, const float* cz, Geometry g, int quirk) {
+    hipLaunchKernelGGL(poisson_kernel, dim3(elem_blocks(g.n())), dim3(256), 0, s, in, out, cx, cy,
+                       cz, g, quirk);
+    return hipGetLastError();
+}
+
+hipError_t launch_epilogue(hipStream_t s, const float* u, float* l, const float* mask, float* r,
+                           size_t n, int mask_shading, int set_negative_zero,
+                           int add_background_back) {
+    hipLaunchKernelGGL(epilogue_kernel, dim3(elem_blocks(n)), dim3(256), 0, s, u, l, mask, r, n,
+                       mask_shading, set_negative_zero, add_background_back);
+    return hipGetLastError();
+}
+
+hipError_t launch_dct_mfma_pass(hipStream_t, const float*, float*, const float*, const uint16_t*,
+                                const uint16_t*, Geometry, int, int, int, int, const float*,
+                                const float*, const float*, int, int) {
+    return hipErrorNotSupported;
+}
+
+}  // namespace amb

@@ -1,0 +1,335 @@
+"""ctypes binding of libambient_amd.so (the C ABI in include/ambient_amd.h).
+
+The product path has no CPU fallback: if the HIP library is not built, or no
+GPU is visible, every entry point raises.  Volumes are numpy float32 arrays of
+shape (D, H, W) (x fastest); device buffers are raw pointers (ints) obtained
+from Context.alloc().
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from ctypes import POINTER, c_double, c_float, c_int, c_size_t, c_uint32, c_uint64, c_void_p
+
+import numpy as np
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libambient_amd.so")
+
+MEM_HOST, MEM_DEVICE = 0, 1
+DATA_L1, DATA_L2 = 0, 1
+STORAGE_F32, STORAGE_BF16 = 0, 1
+DCT_REFERENCE, DCT_F32, DCT_BF16X3 = 0, 1, 2
+FLAG_SET_NEGATIVE_ZERO = 0x1
+FLAG_MASK_SHADING = 0x2
+FLAG_ADD_BACKGROUND_BACK = 0x4
+FLAG_POISSON_INDEX_QUIRK = 0x8
+N_FIELDS = 17
+FIELD_NAMES = ["u", "ubar", "vx", "vy", "vz", "vbx", "vby", "vbz", "px", "py", "pz",
+               "qxx", "qyy", "qzz", "qxy", "qxz", "qyz"]
+PROFILE_CLASSES = ["dual_sweep", "primal_sweep", "dct", "poisson", "divergence", "epilogue",
+                   "init", "energy"]
+
+# Every function include/ambient_amd.h declares (checked by the CPU test suite).
+EXPORTS = [
+    "amb_context_create", "amb_context_destroy", "amb_last_error", "amb_version",
+    "amb_tgv_params_default", "amb_device_alloc", "amb_device_free", "amb_memcpy",
+    "amb_synchronize", "amb_context_stream", "amb_tgv2_deshade", "amb_tgv2_denoise",
+    "amb_divergence", "amb_cosine_transform", "amb_inverse_cosine_transform",
+    "amb_solve_poisson_in_cosine_domain", "amb_integrate_gradients_poisson",
+    "amb_tgv2_deshade_pipeline", "amb_tgv2_energy", "amb_tgv2_begin", "amb_tgv2_set_state",
+    "amb_tgv2_get_state", "amb_tgv2_iterate", "amb_tgv2_sweep", "amb_profile_enable",
+    "amb_profile_read", "amb_profile_reset",
+]
+
+
+class AmbError(RuntimeError):
+    def __init__(self, status, message):
+        super().__init__(f"amb status {status}: {message}")
+        self.status = status
+
+
+class Params(ctypes.Structure):
+    _fields_ = [("lambda_", c_float), ("alpha0", c_float), ("alpha1", c_float),
+                ("tau", c_float), ("sigma", c_float), ("theta", c_float),
+                ("iterations", c_uint32), ("data_term", c_uint32), ("storage", c_uint32),
+                ("dct_precision", c_uint32), ("callback_interval", c_uint32),
+                ("flags", c_uint32)]
+
+
+CALLBACK = ctypes.CFUNCTYPE(c_int, c_uint32, c_uint32, POINTER(c_float), POINTER(c_float),
+                            POINTER(c_float), POINTER(c_float), c_void_p)
+
+_lib = None
+
+
+def lib():
+    """Load the HIP library; raises if it has not been built (no fallback)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
+                          "(hipcc --offload-arch=gfx950); there is no CPU fallback")
+    L = ctypes.CDLL(LIB_PATH)
+    vp, u32, mem = c_void_p, c_uint32, c_int
+    fp = c_void_p   # float* passed as void*
+    sig = {
+        "amb_context_create": [c_int, POINTER(c_void_p)],
+        "amb_context_destroy": [vp],
+        "amb_tgv_params_default": [POINTER(Params)],
+        "amb_device_alloc": [vp, c_size_t, POINTER(c_void_p)],
+        "amb_device_free": [vp, vp],
+        "amb_memcpy": [vp, vp, vp, c_size_t, c_int],
+        "amb_synchronize": [vp],
+        "amb_context_stream": [vp],
+        "amb_tgv2_deshade": [vp, mem, fp, u32, u32, u32, POINTER(Params), fp, fp, fp, fp,
+                             CALLBACK, vp],
+        "amb_tgv2_denoise": [vp, mem, fp, u32, u32, u32, POINTER(Params), fp, CALLBACK, vp],
+        "amb_divergence": [vp, mem, fp, fp, fp, u32, u32, u32, fp],
+        "amb_cosine_transform": [vp, mem, fp, u32, u32, u32, u32, fp],
+        "amb_inverse_cosine_transform": [vp, mem, fp, u32, u32, u32, u32, fp],
+        "amb_solve_poisson_in_cosine_domain": [vp, mem, fp, u32, u32, u32, u32, fp],
+        "amb_integrate_gradients_poisson": [vp, mem, fp, fp, fp, u32, u32, u32, u32, u32, fp],
+        "amb_tgv2_deshade_pipeline": [vp, mem, fp, u32, u32, u32, POINTER(Params), fp, fp, fp,
+                                      fp, CALLBACK, vp],
+        "amb_tgv2_energy": [vp, POINTER(c_double)],
+        "amb_tgv2_begin": [vp, mem, fp, u32, u32, u32, POINTER(Params)],
+        "amb_tgv2_set_state": [vp, mem, POINTER(c_void_p)],
+        "amb_tgv2_get_state": [vp, mem, POINTER(c_void_p)],
+        "amb_tgv2_iterate": [vp, u32],
+        "amb_tgv2_sweep": [vp, c_int],
+        "amb_profile_enable": [vp, c_int],
+        "amb_profile_read": [vp, c_int, POINTER(c_double), POINTER(c_uint64)],
+        "amb_profile_reset": [vp],
+    }
+    for name, args in sig.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = c_void_p if name == "amb_context_stream" else (
+            None if name == "amb_tgv_params_default" else c_int)
+    L.amb_last_error.restype = ctypes.c_char_p
+    L.amb_version.restype = ctypes.c_char_p
+    _lib = L
+    return L
+
+
+def _check(status):
+    if status != 0:
+        raise AmbError(status, lib().amb_last_error().decode())
+
+
+def default_params(**kw) -> Params:
+    p = Params()
+    lib().amb_tgv_params_default(ctypes.byref(p))
+    for k, v in kw.items():
+        setattr(p, "lambda_" if k in ("lam", "lambda") else k, v)
+    return p
+
+
+def _f32(a):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    return a
+
+
+def _ptr(a):
+    if a is None:
+        return None
+    if isinstance(a, int):
+        return c_void_p(a)
+    return a.ctypes.data_as(c_void_p)
+
+
+def _dims(shape):
+    d, h, w = shape
+    return c_uint32(w), c_uint32(h), c_uint32(d)
+
+
+class Context:
+    """An amb_context: device state + stream on one GPU."""
+
+    def __init__(self, device: int = 0):
+        self._h = c_void_p()
+        _check(lib().amb_context_create(device, ctypes.byref(self._h)))
+
+    def close(self):
+        if self._h:
+            lib().amb_context_destroy(self._h)
+            self._h = c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- device memory ----------------------------------------------------
+    def alloc(self, nbytes: int) -> int:
+        p = c_void_p()
+        _check(lib().amb_device_alloc(self._h, nbytes, ctypes.byref(p)))
+        return p.value
+
+    def free(self, ptr: int):
+        _check(lib().amb_device_free(self._h, c_void_p(ptr)))
+
+    def upload(self, ptr: int, a: np.ndarray):
+        a = np.ascontiguousarray(a)
+        _check(lib().amb_memcpy(self._h, c_void_p(ptr), _ptr(a), a.nbytes, 0))
+
+    def download(self, ptr: int, shape, dtype=np.float32) -> np.ndarray:
+        out = np.empty(shape, dtype=dtype)
+        _check(lib().amb_memcpy(self._h, _ptr(out), c_void_p(ptr), out.nbytes, 1))
+        return out
+
+    def synchronize(self):
+        _check(lib().amb_synchronize(self._h))
+
+    # ---- TGV2 iteration ---------------------------------------------------
+    @staticmethod
+    def _callback(fn, shape, with_v):
+        if fn is None:
+            return CALLBACK(), None
+        n = int(np.prod(shape))
+
+        def tramp(it, count, u, vx, vy, vz, user):
+            U = np.ctypeslib.as_array(u, (n,)).reshape(shape).copy()
+            if with_v:
+                V = [np.ctypeslib.as_array(p, (n,)).reshape(shape).copy() for p in (vx, vy, vz)]
+                return int(bool(fn(it, count, U, *V)))
+            return int(bool(fn(it, count, U)))
+        cb = CALLBACK(tramp)
+        return cb, cb
+
+    def tgv2_deshade(self, f, params: Params, callback=None):
+        """amb_tgv2_deshade on host arrays -> (u, vx, vy, vz)."""
+        f = _f32(f)
+        outs = [np.empty_like(f) for _ in range(4)]
+        cb, keep = self._callback(callback, f.shape, True)
+        _check(lib().amb_tgv2_deshade(self._h, MEM_HOST, _ptr(f), *_dims(f.shape),
+                                      ctypes.byref(params), *[_ptr(o) for o in outs], cb, None))
+        del keep
+        return tuple(outs)
+
+    def tgv2_denoise(self, f, params: Params, callback=None):
+        f = _f32(f)
+        u = np.empty_like(f)
+        cb, keep = self._callback(callback, f.shape, False)
+        _check(lib().amb_tgv2_denoise(self._h, MEM_HOST, _ptr(f), *_dims(f.shape),
+                                      ctypes.byref(params), _ptr(u), cb, None))
+        del keep
+        return u
+
+    def deshade_pipeline(self, f, params: Params, mask=None, callback=None):
+        """amb_tgv2_deshade_pipeline on host arrays -> (u, shading, deshaded)."""
+        f = _f32(f)
+        m = _f32(mask) if mask is not None else None
+        u, l, r = (np.empty_like(f) for _ in range(3))
+        cb, keep = self._callback(callback, f.shape, True)
+        _check(lib().amb_tgv2_deshade_pipeline(self._h, MEM_HOST, _ptr(f), *_dims(f.shape),
+                                               ctypes.byref(params), _ptr(m), _ptr(u), _ptr(l),
+                                               _ptr(r), cb, None))
+        del keep
+        return u, l, r
+
+    def deshade_pipeline_device(self, f_dev: int, shape, params: Params, u_dev=None, l_dev=None,
+                                r_dev=None, mask_dev=None):
+        _check(lib().amb_tgv2_deshade_pipeline(self._h, MEM_DEVICE, c_void_p(f_dev), *_dims(shape),
+                                               ctypes.byref(params), _ptr(mask_dev), _ptr(u_dev),
+                                               _ptr(l_dev), _ptr(r_dev), CALLBACK(), None))
+
+    def begin(self, f, params: Params):
+        f = _f32(f)
+        self.shape = f.shape
+        _check(lib().amb_tgv2_begin(self._h, MEM_HOST, _ptr(f), *_dims(f.shape), ctypes.byref(params)))
+
+    def begin_device(self, f_dev: int, shape, params: Params):
+        self.shape = tuple(shape)
+        _check(lib().amb_tgv2_begin(self._h, MEM_DEVICE, c_void_p(f_dev), *_dims(shape),
+                                    ctypes.byref(params)))
+
+    def set_state(self, fields: np.ndarray):
+        fields = np.ascontiguousarray(fields, dtype=np.float32)
+        assert fields.shape[0] == N_FIELDS
+        arr = (c_void_p * N_FIELDS)(*[fields[k].ctypes.data for k in range(N_FIELDS)])
+        _check(lib().amb_tgv2_set_state(self._h, MEM_HOST, arr))
+
+    def get_state(self) -> np.ndarray:
+        out = np.empty((N_FIELDS,) + tuple(self.shape), dtype=np.float32)
+        arr = (c_void_p * N_FIELDS)(*[out[k].ctypes.data for k in range(N_FIELDS)])
+        _check(lib().amb_tgv2_get_state(self._h, MEM_HOST, arr))
+        return out
+
+    def iterate(self, n: int):
+        _check(lib().amb_tgv2_iterate(self._h, n))
+
+    def sweep(self, which: int):
+        _check(lib().amb_tgv2_sweep(self._h, which))
+
+    def energy(self) -> np.ndarray:
+        out = (c_double * 10)()
+        _check(lib().amb_tgv2_energy(self._h, out))
+        return np.array(out[:])
+
+    # ---- DCT Poisson ------------------------------------------------------
+    def divergence(self, vx, vy, vz):
+        vx, vy, vz = _f32(vx), _f32(vy), _f32(vz)
+        out = np.empty_like(vx)
+        _check(lib().amb_divergence(self._h, MEM_HOST, _ptr(vx), _ptr(vy), _ptr(vz),
+                                    *_dims(vx.shape), _ptr(out)))
+        return out
+
+    def cosine_transform(self, a, precision=DCT_REFERENCE):
+        a = _f32(a)
+        out = np.empty_like(a)
+        _check(lib().amb_cosine_transform(self._h, MEM_HOST, _ptr(a), *_dims(a.shape), precision,
+                                          _ptr(out)))
+        return out
+
+    def inverse_cosine_transform(self, a, precision=DCT_REFERENCE):
+        a = _f32(a)
+        out = np.empty_like(a)
+        _check(lib().amb_inverse_cosine_transform(self._h, MEM_HOST, _ptr(a), *_dims(a.shape),
+                                                  precision, _ptr(out)))
+        return out
+
+    def solve_poisson(self, a, index_quirk=False):
+        a = _f32(a)
+        out = np.empty_like(a)
+        _check(lib().amb_solve_poisson_in_cosine_domain(
+            self._h, MEM_HOST, _ptr(a), *_dims(a.shape),
+            FLAG_POISSON_INDEX_QUIRK if index_quirk else 0, _ptr(out)))
+        return out
+
+    def integrate_gradients_poisson(self, vx, vy, vz, precision=DCT_REFERENCE, flags=0):
+        vx, vy, vz = _f32(vx), _f32(vy), _f32(vz)
+        out = np.empty_like(vx)
+        _check(lib().amb_integrate_gradients_poisson(self._h, MEM_HOST, _ptr(vx), _ptr(vy),
+                                                     _ptr(vz), *_dims(vx.shape), precision, flags,
+                                                     _ptr(out)))
+        return out
+
+    def integrate_gradients_poisson_device(self, vx, vy, vz, shape, out, precision=DCT_REFERENCE,
+                                           flags=0):
+        _check(lib().amb_integrate_gradients_poisson(self._h, MEM_DEVICE, c_void_p(vx),
+                                                     c_void_p(vy), c_void_p(vz), *_dims(shape),
+                                                     precision, flags, c_void_p(out)))
+
+    # ---- profiling ----------------------------------------------------------
+    def profile(self, enable=True):
+        _check(lib().amb_profile_enable(self._h, int(enable)))
+
+    def profile_reset(self):
+        _check(lib().amb_profile_reset(self._h))
+
+    def profile_read(self):
+        res = {}
+        for k, name in enumerate(PROFILE_CLASSES):
+            ms, n = c_double(), c_uint64()
+            _check(lib().amb_profile_read(self._h, k, ctypes.byref(ms), ctypes.byref(n)))
+            res[name] = (ms.value, n.value)
+        return res

@@ -1,0 +1,172 @@
+"""Host-side mirror of the reference's launcher and processor interface for the
+TGV-DCT deshade path, on top of the gfx950 C ABI.
+
+Same names, argument order and meaning as the reference's C++ templates
+(float instantiation).  Differences: volumes are numpy float32 arrays (flat
+``W*H*D`` like the reference's ``Pixel*``, or shaped (D, H, W)); results are
+returned instead of ``new Pixel[]``; a failing HIP call raises AmbError
+instead of calling exit() (reference source/modules/cuda/cuda_helper.in:11-22).
+tau = sigma = 1/sqrt(20) and theta = 1 as the reference hard-codes
+(tgv_deshade/tgv2_l1_deshade.cu:148-150), overridable by keyword.
+"""
+from __future__ import annotations
+
+import threading
+
+import numpy as np
+
+from . import native
+
+_tls = threading.local()
+
+
+def _ctx() -> native.Context:
+    c = getattr(_tls, "ctx", None)
+    if c is None:
+        c = native.Context(0)
+        _tls.ctx = c
+    return c
+
+
+def _vol(a, width, height, depth):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    return a.reshape(depth, height, width)
+
+
+def _params(lambda_, iteration_count, paint_iteration_interval, alpha0, alpha1, data_term, **kw):
+    p = native.default_params(lam=lambda_, alpha0=alpha0, alpha1=alpha1,
+                              iterations=iteration_count, data_term=data_term,
+                              callback_interval=paint_iteration_interval or 0)
+    for k, v in kw.items():
+        setattr(p, "lambda_" if k == "lam" else k, v)
+    return p
+
+
+def tgv2_l1_deshade_launch(f_host, width, height, depth, lambda_, iteration_count,
+                           paint_iteration_interval, cuda_block_dimension,
+                           iteration_finished_callback, alpha0, alpha1, **kw):
+    """Reference tgv_deshade/tgv2_l1_deshade.cu:33-41 -> (u, v_x, v_y, v_z).
+    cuda_block_dimension is accepted and ignored (launch shapes are fixed per
+    kernel on gfx950).  The callback gets (iteration_index, iteration_count,
+    u, v_x, v_y, v_z) and returns True to stop.  ``data_term=native.DATA_L2``
+    gives the L2 deshade loop (configurations that the reference composes
+    from tgv2_l2.cu's u-update)."""
+    del cuda_block_dimension
+    data_term = kw.pop("data_term", native.DATA_L1)
+    f = _vol(f_host, width, height, depth)
+    p = _params(lambda_, iteration_count, paint_iteration_interval, alpha0, alpha1, data_term, **kw)
+    return _ctx().tgv2_deshade(f, p, iteration_finished_callback)
+
+
+def _denoise(data_term, f_host, width, height, depth, lambda_, iteration_count,
+             paint_iteration_interval, iteration_finished_callback, alpha0, alpha1, **kw):
+    f = _vol(f_host, width, height, depth)
+    p = _params(lambda_, iteration_count, paint_iteration_interval, alpha0, alpha1, data_term, **kw)
+    return _ctx().tgv2_denoise(f, p, iteration_finished_callback)
+
+
+def tgv2_l1_launch(f_host, width, height, depth, lambda_, iteration_count,
+                   paint_iteration_interval, iteration_finished_callback, alpha0, alpha1, **kw):
+    """Reference tgv_cuda/tgv2_l1.cu:64-71 -> u."""
+    return _denoise(native.DATA_L1, f_host, width, height, depth, lambda_, iteration_count,
+                    paint_iteration_interval, iteration_finished_callback, alpha0, alpha1, **kw)
+
+
+def tgv2_l2_launch(f_host, width, height, depth, lambda_, iteration_count,
+                   paint_iteration_interval, iteration_finished_callback, alpha0, alpha1, **kw):
+    """Reference tgv_cuda/tgv2_l2.cu:60-67 -> u."""
+    return _denoise(native.DATA_L2, f_host, width, height, depth, lambda_, iteration_count,
+                    paint_iteration_interval, iteration_finished_callback, alpha0, alpha1, **kw)
+
+
+def divergence_kernel_launch(dx, dy, dz, width, height, depth, is_host_data=False):
+    """Reference cuda_image_operations/divergence.cu:59-141 (host arrays either way)."""
+    del is_host_data
+    return _ctx().divergence(_vol(dx, width, height, depth), _vol(dy, width, height, depth),
+                             _vol(dz, width, height, depth))
+
+
+def cosine_transform_kernel_launch(image_host, width, height, depth,
+                                   precision=native.DCT_REFERENCE):
+    """Reference cuda_image_operations/cosine_transform.cu:226-293."""
+    return _ctx().cosine_transform(_vol(image_host, width, height, depth), precision)
+
+
+def inverse_cosine_transform_kernel_launch(image_host, width, height, depth,
+                                           precision=native.DCT_REFERENCE):
+    """Reference cuda_image_operations/inverse_cosine_transform.cu:167-227."""
+    return _ctx().inverse_cosine_transform(_vol(image_host, width, height, depth), precision)
+
+
+def solve_poisson_in_cosine_domain_kernel_launch(image_host, width, height, depth,
+                                                 index_quirk=False):
+    """Reference cuda_image_operations/solve_poisson_in_cosine_domain.cu:86-130."""
+    return _ctx().solve_poisson(_vol(image_host, width, height, depth), index_quirk)
+
+
+class TGVDeshadeProcessor:
+    """Reference tgv_deshade/TGVDeshadeProcessor.{h,cpp} (the methods on the hot path)."""
+
+    @staticmethod
+    def integrate_image_gradients_poisson_cosine_transform(gradient_x, gradient_y, gradient_z,
+                                                           width, height, depth,
+                                                           is_host_data=False,
+                                                           precision=native.DCT_REFERENCE):
+        """TGVDeshadeProcessor.cpp:386-408."""
+        del is_host_data
+        return _ctx().integrate_gradients_poisson(_vol(gradient_x, width, height, depth),
+                                                  _vol(gradient_y, width, height, depth),
+                                                  _vol(gradient_z, width, height, depth),
+                                                  precision)
+
+    @staticmethod
+    def processTGV2L1GPUCuda(input_image, lambda_, alpha0, alpha1, iteration_count,
+                             cuda_block_dimension=-1, mask_image=None,
+                             set_negative_values_to_zero=True, **kw):
+        """TGVDeshadeProcessor.cpp:122-166 (CLI overload): returns
+        (denoised_image, shading_image, deshaded_image) as (D, H, W) arrays.
+        The mask multiplies the deshaded image only, as in the reference."""
+        del cuda_block_dimension
+        f = np.ascontiguousarray(input_image, dtype=np.float32)
+        flags = native.FLAG_SET_NEGATIVE_ZERO if set_negative_values_to_zero else 0
+        p = _params(lambda_, iteration_count, 0, alpha0, alpha1, kw.pop("data_term", native.DATA_L1),
+                    flags=flags, **kw)
+        return _ctx().deshade_pipeline(f, p, mask_image)
+
+    @staticmethod
+    def processTGV2L1GPUCuda_callback(input_image, lambda_, alpha0, alpha1, iteration_count,
+                                      paint_iteration_interval, iteration_finished_callback,
+                                      mask=None, set_negative_values_to_zero=True,
+                                      add_background_back=False, **kw):
+        """TGVDeshadeProcessor.cpp:285-314 -> processTVGPUCuda :224-282 (mask also
+        multiplies the shading; optional add-back of u*(1-mask)).  The callback
+        receives (iteration_index, iteration_count, u, v_x, v_y, v_z)."""
+        f = np.ascontiguousarray(input_image, dtype=np.float32)
+        flags = native.FLAG_MASK_SHADING
+        if set_negative_values_to_zero:
+            flags |= native.FLAG_SET_NEGATIVE_ZERO
+        if add_background_back:
+            flags |= native.FLAG_ADD_BACKGROUND_BACK
+        p = _params(lambda_, iteration_count, paint_iteration_interval, alpha0, alpha1,
+                    kw.pop("data_term", native.DATA_L1), flags=flags, **kw)
+        return _ctx().deshade_pipeline(f, p, mask, iteration_finished_callback)
+
+
+class TGVProcessor:
+    """Reference tgv/TGVProcessor.cpp:140-180 (TGV2 denoise entry points)."""
+
+    @staticmethod
+    def processTGV2L2GPUCuda(input_image, lambda_, alpha0, alpha1, iteration_count,
+                             paint_iteration_interval=0, iteration_finished_callback=None, **kw):
+        f = np.ascontiguousarray(input_image, dtype=np.float32)
+        d, h, w = f.shape
+        return tgv2_l2_launch(f, w, h, d, lambda_, iteration_count, paint_iteration_interval,
+                              iteration_finished_callback, alpha0, alpha1, **kw)
+
+    @staticmethod
+    def processTGV2L1GPUCuda(input_image, lambda_, alpha0, alpha1, iteration_count,
+                             paint_iteration_interval=0, iteration_finished_callback=None, **kw):
+        f = np.ascontiguousarray(input_image, dtype=np.float32)
+        d, h, w = f.shape
+        return tgv2_l1_launch(f, w, h, d, lambda_, iteration_count, paint_iteration_interval,
+                              iteration_finished_callback, alpha0, alpha1, **kw)

@@ -1,0 +1,222 @@
+/*
+ * ambient_amd.h - C ABI of the MI355X (gfx950) TGV-DCT deshade hot path.
+ *
+ * Drop-in replacement for the CUDA launchers of josefkoller/ambient_image_processor
+ * (reference paths relative to its repository root).  Conventions that differ
+ * from the reference, and why:
+ *   - every entry point returns an amb_status instead of calling exit() from
+ *     cudaCheckError (reference source/modules/cuda/cuda_helper.in:11-22);
+ *     amb_last_error() gives the message of the calling thread's last failure;
+ *   - the caller owns every buffer: outputs are written into caller-provided
+ *     arrays instead of being returned as `new Pixel[N]`
+ *     (reference tgv_deshade/TGVDeshadeProcessor.cpp:146-157);
+ *   - buffers may live on the host or in device memory (amb_memory); the
+ *     reference's launchers take host arrays only;
+ *   - tau, sigma and theta are parameters (the reference hard-codes
+ *     tau = sigma = 1/sqrt(20), theta = 1, tgv_deshade/tgv2_l1_deshade.cu:148-150;
+ *     amb_tgv_params_default() sets exactly those values).
+ * Device state (the iteration fields, DCT scratch) lives in an amb_context,
+ * which owns one HIP stream; a context must not be used by two threads at once.
+ * Layout of every volume: float32, x fastest, idx = z*W*H + y*W + x
+ * (reference source/modules/itk/ITKImage.cpp:426-429).
+ * include/ambient_amd_reference_compat.hpp re-exposes the reference's own
+ * C++ launcher templates on top of this ABI.
+ */
+#ifndef AMBIENT_AMD_H
+#define AMBIENT_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define AMB_API __attribute__((visibility("default")))
+
+typedef enum {
+    AMB_OK = 0,
+    AMB_ERR_INVALID_ARGUMENT = 1,
+    AMB_ERR_HIP = 2,
+    AMB_ERR_OUT_OF_MEMORY = 3,
+    AMB_ERR_NOT_SUPPORTED = 4
+} amb_status;
+
+/* Where the caller's buffers live. */
+typedef enum { AMB_MEM_HOST = 0, AMB_MEM_DEVICE = 1 } amb_memory;
+
+/* Data term of the u-update: L1 = tgv_kernel_part4_tgv2_l1 (reference
+ * tgv_cuda/tgv2_l1.cu:27-60), L2 = tgv_kernel_part4_tgv2_l2 (tgv_cuda/tgv2_l2.cu:27-56). */
+typedef enum { AMB_DATA_L1 = 0, AMB_DATA_L2 = 1 } amb_data_term;
+
+/* Storage type of the iteration fields u, u_bar, v, v_bar, p, q.  Arithmetic
+ * is FP32 in both cases; f is always stored FP32. */
+typedef enum { AMB_STORAGE_F32 = 0, AMB_STORAGE_BF16 = 1 } amb_storage;
+
+/* Arithmetic of the separable DCT-II/III passes.
+ *   REFERENCE: the reference's numerics - FP64 cosines, FP64 products, FP32
+ *              accumulator rounded after every term, reference term order
+ *              (cuda_image_operations/cosine_transform.cu:142-223);
+ *   F32:       per-axis dense products on FP32 MFMA, FP32 accumulate;
+ *   BF16X3:    per-axis dense products on BF16 MFMA with hi/lo splits of both
+ *              operands (hi*hi + hi*lo + lo*hi), FP32 accumulate. */
+typedef enum { AMB_DCT_REFERENCE = 0, AMB_DCT_F32 = 1, AMB_DCT_BF16X3 = 2 } amb_dct_precision;
+
+/* amb_tgv_params.flags */
+#define AMB_FLAG_SET_NEGATIVE_ZERO    0x1u  /* r < 0 -> 0 (clamp_negative_values.cu:27) */
+#define AMB_FLAG_MASK_SHADING         0x2u  /* mask also multiplies l (TGVDeshadeProcessor.cpp:83-87) */
+#define AMB_FLAG_ADD_BACKGROUND_BACK  0x4u  /* r += u*(1-mask) (TGVDeshadeProcessor.cpp:275-279) */
+#define AMB_FLAG_POISSON_INDEX_QUIRK  0x8u  /* recover voxel coordinates through float arithmetic as
+                                               solve_poisson_in_cosine_domain.cu:36-40 does (differs
+                                               from exact coordinates only for linear index >= 2^24) */
+
+typedef struct {
+    float lambda;                /* data-term weight */
+    float alpha0;                /* second-order weight (q projection, tgv2_common.cu:262) */
+    float alpha1;                /* first-order weight (p projection, tgv2_common.cu:124) */
+    float tau, sigma, theta;     /* primal/dual step sizes, over-relaxation */
+    uint32_t iterations;
+    uint32_t data_term;          /* amb_data_term */
+    uint32_t storage;            /* amb_storage */
+    uint32_t dct_precision;      /* amb_dct_precision */
+    uint32_t callback_interval;  /* paint_iteration_interval; 0 = never */
+    uint32_t flags;              /* AMB_FLAG_* */
+} amb_tgv_params;
+
+/* Called on the calling thread when iteration_index > 0 and
+ * iteration_index % callback_interval == 0, with host copies of u and v that
+ * are valid only during the call (reference tgv_cuda/tgv2_common.cu:374-408).
+ * vx/vy/vz are NULL for the denoise entry points.  Return nonzero to stop. */
+typedef int (*amb_iteration_callback)(uint32_t iteration_index, uint32_t iteration_count,
+                                      const float* u, const float* vx, const float* vy,
+                                      const float* vz, void* user);
+
+typedef struct amb_context amb_context;
+
+/* ---- context, memory ---------------------------------------------------- */
+AMB_API amb_status amb_context_create(int device, amb_context** ctx);
+AMB_API amb_status amb_context_destroy(amb_context* ctx);
+AMB_API const char* amb_last_error(void);
+AMB_API const char* amb_version(void);
+/* Reference defaults: tau = sigma = 1/sqrt(20), theta = 1, lambda = 1,
+ * alpha0 = 2, alpha1 = 1 (TGVDeshadeWidget.ui:44,94,119), L1, FP32, REFERENCE DCT,
+ * 10000 iterations, no callback, AMB_FLAG_SET_NEGATIVE_ZERO. */
+AMB_API void amb_tgv_params_default(amb_tgv_params* params);
+AMB_API amb_status amb_device_alloc(amb_context* ctx, size_t bytes, void** ptr);
+AMB_API amb_status amb_device_free(amb_context* ctx, void* ptr);
+/* kind: 0 host->device, 1 device->host, 2 device->device (stream-ordered, then synchronised) */
+AMB_API amb_status amb_memcpy(amb_context* ctx, void* dst, const void* src, size_t bytes, int kind);
+AMB_API amb_status amb_synchronize(amb_context* ctx);
+/* The context's HIP stream (a hipStream_t), for callers that order their own work after ours. */
+AMB_API void* amb_context_stream(amb_context* ctx);
+
+/* ---- A13/A19: TGV2 deshade iteration ------------------------------------
+ * Replaces tgv2_l1_deshade_launch<float>(f_host, width, height, depth, lambda,
+ * iteration_count, paint_iteration_interval, cuda_block_dimension, callback,
+ * alpha0, alpha1, &v_x, &v_y, &v_z)  (reference tgv_deshade/tgv2_l1_deshade.cu:33-303,
+ * declared TGVDeshadeProcessor.cpp:29-39).  data_term = AMB_DATA_L2 gives the
+ * same loop with the L2 u-update (the reference has no L2 deshade launcher).
+ * Writes u and v (x, y, z) into the caller's buffers. */
+AMB_API amb_status amb_tgv2_deshade(amb_context* ctx, amb_memory mem, const float* f,
+                                    uint32_t width, uint32_t height, uint32_t depth,
+                                    const amb_tgv_params* params, float* u, float* v_x,
+                                    float* v_y, float* v_z, amb_iteration_callback callback,
+                                    void* user);
+
+/* ---- A20: TGV2 denoise ----------------------------------------------------
+ * Replaces tgv2_l1_launch<float> / tgv2_l2_launch<float>(f_host, width, height,
+ * depth, lambda, iteration_count, paint_iteration_interval, callback, alpha0,
+ * alpha1)  (reference tgv_cuda/tgv2_l1.cu:64-321, tgv_cuda/tgv2_l2.cu:60-316,
+ * declared tgv/TGVProcessor.cpp:52-69).  The callback receives u only. */
+AMB_API amb_status amb_tgv2_denoise(amb_context* ctx, amb_memory mem, const float* f,
+                                    uint32_t width, uint32_t height, uint32_t depth,
+                                    const amb_tgv_params* params, float* u,
+                                    amb_iteration_callback callback, void* user);
+
+/* ---- A14-A18: DCT Poisson bias solve ------------------------------------- */
+/* Replaces divergence_kernel_launch<float>(dx, dy, dz, w, h, d, is_host_data)
+ * (reference cuda_image_operations/divergence.cu:59-141): out = B_x dx + B_y dy + B_z dz. */
+AMB_API amb_status amb_divergence(amb_context* ctx, amb_memory mem, const float* dx,
+                                  const float* dy, const float* dz, uint32_t width,
+                                  uint32_t height, uint32_t depth, float* out);
+/* Replaces cosine_transform_kernel_launch<float>(image, w, h, d)
+ * (reference cuda_image_operations/cosine_transform.cu:226-293). */
+AMB_API amb_status amb_cosine_transform(amb_context* ctx, amb_memory mem, const float* image,
+                                        uint32_t width, uint32_t height, uint32_t depth,
+                                        uint32_t dct_precision, float* out);
+/* Replaces inverse_cosine_transform_kernel_launch<float>(image, w, h, d)
+ * (reference cuda_image_operations/inverse_cosine_transform.cu:167-227). */
+AMB_API amb_status amb_inverse_cosine_transform(amb_context* ctx, amb_memory mem,
+                                                const float* image, uint32_t width,
+                                                uint32_t height, uint32_t depth,
+                                                uint32_t dct_precision, float* out);
+/* Replaces solve_poisson_in_cosine_domain_kernel_launch<float>(image, w, h, d)
+ * (reference cuda_image_operations/solve_poisson_in_cosine_domain.cu:86-130).
+ * flags: AMB_FLAG_POISSON_INDEX_QUIRK. */
+AMB_API amb_status amb_solve_poisson_in_cosine_domain(amb_context* ctx, amb_memory mem,
+                                                      const float* image, uint32_t width,
+                                                      uint32_t height, uint32_t depth,
+                                                      uint32_t flags, float* out);
+/* Replaces TGVDeshadeProcessor::integrate_image_gradients_poisson_cosine_transform
+ * (reference tgv_deshade/TGVDeshadeProcessor.cpp:386-408): l = iDCT(P(DCT(B-div v))). */
+AMB_API amb_status amb_integrate_gradients_poisson(amb_context* ctx, amb_memory mem,
+                                                   const float* v_x, const float* v_y,
+                                                   const float* v_z, uint32_t width,
+                                                   uint32_t height, uint32_t depth,
+                                                   uint32_t dct_precision, uint32_t flags,
+                                                   float* shading);
+
+/* ---- A19: full deshade ----------------------------------------------------
+ * Replaces TGVDeshadeProcessor::processTGV2L1GPUCuda (reference
+ * TGVDeshadeProcessor.cpp:122-166 with flags = SET_NEGATIVE_ZERO and without
+ * MASK_SHADING; the callback overload :285-314 -> :224-282 with MASK_SHADING and
+ * optionally ADD_BACKGROUND_BACK): iteration, Poisson integration of v, then
+ * r = u - l, mask, clamp, add-back.  mask may be NULL.  The callback (if any)
+ * receives u and v as in amb_tgv2_deshade.  Any of u, shading, deshaded may be
+ * NULL if not wanted. */
+AMB_API amb_status amb_tgv2_deshade_pipeline(amb_context* ctx, amb_memory mem, const float* f,
+                                             uint32_t width, uint32_t height, uint32_t depth,
+                                             const amb_tgv_params* params, const float* mask,
+                                             float* u, float* shading, float* deshaded,
+                                             amb_iteration_callback callback, void* user);
+
+/* ---- A21 (no counterpart in the reference): FP64 energy and gap ----------
+ * Evaluates, for the iteration state left in ctx by the last amb_tgv2_* call
+ * (or set by amb_tgv2_set_state), with FP64 accumulation on the GPU:
+ *   out[0] E = G(u) + alpha1 sum|D+u - v|_2 + alpha0 sum|eps_B v|_w
+ *   out[1] G(u)   (lambda|u-f|_1 or lambda/2 |u-f|^2)
+ *   out[2] alpha1 term, out[3] alpha0 term
+ *   out[4] gap = out[5] + out[6] + out[7]  (Fenchel-Young brackets, >= 0)
+ *   out[5] data bracket with s = -(D+)^T p (true transpose)
+ *   out[6] alpha1 sum|D+u - v| - <D+u - v, p> ; out[7] alpha0 sum|eps v|_w - <eps v, q>_w
+ *   out[8] L1 dual infeasibility sum max(0, |s| - lambda) (0 for L2)
+ *   out[9] ||eps_B^T q - p||_2  (v stationarity residual) */
+AMB_API amb_status amb_tgv2_energy(amb_context* ctx, double out[10]);
+
+/* ---- stepwise access to the iteration (tests, checkpointing) -------------
+ * fields order: u, u_bar, v_x, v_y, v_z, vbar_x, vbar_y, vbar_z, p_x, p_y, p_z,
+ *               q_xx, q_yy, q_zz, q_xy, q_xz, q_yz  (17 arrays of W*H*D floats). */
+AMB_API amb_status amb_tgv2_begin(amb_context* ctx, amb_memory mem, const float* f,
+                                  uint32_t width, uint32_t height, uint32_t depth,
+                                  const amb_tgv_params* params);
+AMB_API amb_status amb_tgv2_set_state(amb_context* ctx, amb_memory mem, const float* const* fields);
+AMB_API amb_status amb_tgv2_get_state(amb_context* ctx, amb_memory mem, float* const* fields);
+AMB_API amb_status amb_tgv2_iterate(amb_context* ctx, uint32_t iterations);
+/* One half-iteration only: which = 0 dual sweep (p, q), 1 primal sweep (u, v). */
+AMB_API amb_status amb_tgv2_sweep(amb_context* ctx, int which);
+
+/* ---- timing ----------------------------------------------------------------
+ * With profiling on, every kernel launch of the context is bracketed by HIP
+ * events on the context's stream; amb_profile_read returns the summed device
+ * time (ms) and launch count per kernel class:
+ *   0 dual sweep, 1 primal sweep, 2 DCT passes, 3 Poisson divide,
+ *   4 divergence, 5 epilogue, 6 init, 7 energy. */
+AMB_API amb_status amb_profile_enable(amb_context* ctx, int enable);
+AMB_API amb_status amb_profile_read(amb_context* ctx, int kernel_class, double* total_ms,
+                                    uint64_t* launches);
+AMB_API amb_status amb_profile_reset(amb_context* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* AMBIENT_AMD_H */

@@ -1,0 +1,229 @@
+"""ctypes binding of the CPU oracle (oracle/libtgv_oracle.so).
+
+TEST INFRASTRUCTURE ONLY: used by tests/, by __graft_entry__.smoke() as the
+checker, and by bench.py's cpu_baseline leg.  The product package never
+imports this module.
+
+Every function takes and returns numpy arrays shaped (D, H, W) (x fastest).
+``dtype`` selects the mode: float32 = the reference's FP32 operation order
+("mirror"), float64 = FP64 throughout.  See tgv_oracle.cpp for the reference
+file:line each routine follows.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "libtgv_oracle.so")
+_lib = None
+
+N_FIELDS = 17   # u, ubar, v[3], vbar[3], p[3], q[6]  (q: xx, yy, zz, xy, xz, yz)
+FIELD_NAMES = ["u", "ubar", "vx", "vy", "vz", "vbx", "vby", "vbz", "px", "py", "pz",
+               "qxx", "qyy", "qzz", "qxy", "qxz", "qyz"]
+
+
+def build(force: bool = False) -> str:
+    """Compile the oracle with its Makefile (g++, OpenMP)."""
+    if force or not os.path.exists(_LIB_PATH):
+        subprocess.run(["make", "-C", _HERE, "-B" if force else "-s"], check=True,
+                       stdout=subprocess.DEVNULL)
+    return _LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        _lib = ctypes.CDLL(_LIB_PATH)
+        _lib.orc_cospi.restype = ctypes.c_double
+        _lib.orc_cospi.argtypes = [ctypes.c_double]
+        _lib.orc_num_threads.restype = ctypes.c_int
+    return _lib
+
+
+def _suf(dtype):
+    dt = np.dtype(dtype)
+    if dt == np.float32:
+        return "f32"
+    if dt == np.float64:
+        return "f64"
+    raise TypeError(dtype)
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _c(a, dtype):
+    return np.ascontiguousarray(a, dtype=dtype)
+
+
+def _dims(a):
+    d, h, w = a.shape
+    return ctypes.c_longlong(w), ctypes.c_longlong(h), ctypes.c_longlong(d)
+
+
+def set_num_threads(n: int):
+    lib().orc_set_num_threads(int(n))
+
+
+def num_threads() -> int:
+    return lib().orc_num_threads()
+
+
+def cospi(t: float) -> float:
+    return lib().orc_cospi(float(t))
+
+
+def forward_difference(a, axis, dtype=np.float64):
+    a = _c(a, dtype)
+    out = np.empty_like(a)
+    getattr(lib(), "orc_forward_difference_" + _suf(dtype))(_p(a), _p(out), *_dims(a), int(axis))
+    return out
+
+
+def backward_difference(a, axis, dtype=np.float64):
+    a = _c(a, dtype)
+    out = np.empty_like(a)
+    getattr(lib(), "orc_backward_difference_" + _suf(dtype))(_p(a), _p(out), *_dims(a), int(axis))
+    return out
+
+
+class TGVState:
+    """The 17 iteration fields as one contiguous (17, D, H, W) array."""
+
+    def __init__(self, f, dtype=np.float64):
+        self.dtype = np.dtype(dtype)
+        self.f = _c(f, dtype)
+        self.fields = np.zeros((N_FIELDS,) + self.f.shape, dtype=dtype)
+        self._ptrs()
+        getattr(lib(), "orc_tgv2_init_" + _suf(dtype))(self._pp, _p(self.f), *_dims(self.f))
+
+    def _ptrs(self):
+        n = self.fields[0].size * self.dtype.itemsize
+        base = self.fields.ctypes.data
+        self._pp = (ctypes.c_void_p * N_FIELDS)(*[base + i * n for i in range(N_FIELDS)])
+
+    def __getitem__(self, name):
+        return self.fields[FIELD_NAMES.index(name)]
+
+    @property
+    def u(self):
+        return self.fields[0]
+
+    @property
+    def v(self):
+        return self.fields[2:5]
+
+    @property
+    def p(self):
+        return self.fields[8:11]
+
+    @property
+    def q(self):
+        return self.fields[11:17]
+
+    def copy(self):
+        c = TGVState.__new__(TGVState)
+        c.dtype = self.dtype
+        c.f = self.f.copy()
+        c.fields = self.fields.copy()
+        c._ptrs()
+        return c
+
+    def iterate(self, iterations, lam, alpha0, alpha1, tau, sigma, theta=1.0, data_term="l2"):
+        prm = (ctypes.c_double * 6)(lam, alpha0, alpha1, tau, sigma, theta)
+        dt = 0 if data_term == "l1" else 1
+        getattr(lib(), "orc_tgv2_iterate_" + _suf(self.dtype))(
+            self._pp, _p(self.f), *_dims(self.f), prm, dt, int(iterations))
+        return self
+
+    def energy(self, lam, alpha0, alpha1, data_term="l2"):
+        """FP64 energy/gap: [E, E_data, E_first, E_second, gap, b1, b2, b3,
+        infeasibility, v_residual]."""
+        prm = (ctypes.c_double * 6)(lam, alpha0, alpha1, 0.0, 0.0, 0.0)
+        out = (ctypes.c_double * 10)()
+        getattr(lib(), "orc_tgv2_energy_" + _suf(self.dtype))(
+            _p(self.f), self._pp, *_dims(self.f), prm, 0 if data_term == "l1" else 1, out)
+        return np.array(out[:])
+
+
+def tgv2(f, iterations, lam, alpha0, alpha1, tau, sigma, theta=1.0, data_term="l2",
+         dtype=np.float64):
+    """Run the reference loop from the reference initialisation; returns the state."""
+    st = TGVState(f, dtype)
+    st.iterate(iterations, lam, alpha0, alpha1, tau, sigma, theta, data_term)
+    return st
+
+
+def divergence(vx, vy, vz, dtype=np.float64):
+    vx, vy, vz = (_c(a, dtype) for a in (vx, vy, vz))
+    out = np.empty_like(vx)
+    getattr(lib(), "orc_divergence_" + _suf(dtype))(_p(vx), _p(vy), _p(vz), _p(out), *_dims(vx))
+    return out
+
+
+def cosine_transform(a, dtype=np.float64):
+    a = _c(a, dtype)
+    out = np.empty_like(a)
+    getattr(lib(), "orc_cosine_transform_" + _suf(dtype))(_p(a), _p(out), *_dims(a))
+    return out
+
+
+def inverse_cosine_transform(a, dtype=np.float64):
+    a = _c(a, dtype)
+    out = np.empty_like(a)
+    getattr(lib(), "orc_inverse_cosine_transform_" + _suf(dtype))(_p(a), _p(out), *_dims(a))
+    return out
+
+
+def dct_pass(a, axis, inverse=False, last=False, dtype=np.float64):
+    a = _c(a, dtype)
+    out = np.empty_like(a)
+    getattr(lib(), "orc_dct_pass_" + _suf(dtype))(_p(a), _p(out), *_dims(a), int(axis),
+                                                  int(inverse), int(last))
+    return out
+
+
+def solve_poisson(a, index_quirk=False, dtype=np.float64):
+    a = _c(a, dtype)
+    out = np.empty_like(a)
+    getattr(lib(), "orc_solve_poisson_" + _suf(dtype))(_p(a), _p(out), *_dims(a), int(index_quirk))
+    return out
+
+
+def integrate_poisson(vx, vy, vz, index_quirk=False, dtype=np.float64):
+    vx, vy, vz = (_c(a, dtype) for a in (vx, vy, vz))
+    out = np.empty_like(vx)
+    getattr(lib(), "orc_integrate_poisson_" + _suf(dtype))(_p(vx), _p(vy), _p(vz), _p(out),
+                                                          *_dims(vx), int(index_quirk))
+    return out
+
+
+def deshade_epilogue(u, l, mask=None, mask_shading=False, set_negative_zero=True,
+                     add_background_back=False):
+    """Returns (r, l_out) in FP32 (reference epilogue op order)."""
+    u = _c(u, np.float32)
+    l = _c(l, np.float32).copy()
+    r = np.empty_like(u)
+    m = _c(mask, np.float32) if mask is not None else None
+    lib().orc_deshade_epilogue_f32(_p(u), _p(l), _p(m) if m is not None else None, _p(r),
+                                   ctypes.c_longlong(u.size), int(mask_shading),
+                                   int(set_negative_zero), int(add_background_back))
+    return r, l
+
+
+def deshade(f, iterations, lam, alpha0, alpha1, tau, sigma, theta=1.0, data_term="l1",
+            mask=None, set_negative_zero=True, index_quirk=False, dtype=np.float32):
+    """Full reference pipeline (CLI overload, TGVDeshadeProcessor.cpp:122-166):
+    TGV2 iteration -> Poisson integration of v -> r = u - l (mask, clamp).
+    Returns (u, l, r, state)."""
+    st = tgv2(f, iterations, lam, alpha0, alpha1, tau, sigma, theta, data_term, dtype)
+    l = integrate_poisson(st.v[0], st.v[1], st.v[2], index_quirk, dtype)
+    r, l2 = deshade_epilogue(st.u, l, mask, False, set_negative_zero, False)
+    return st.u.copy(), l2, r, st
