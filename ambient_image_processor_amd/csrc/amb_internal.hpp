@@ -12,16 +12,39 @@ namespace amb {
 // Field storage.  Arithmetic is always FP32; StoreBF16 keeps the iteration
 // fields as bfloat16 (round to nearest even on store).
 // ---------------------------------------------------------------------------
+// Element address base + i.  With a 32-bit index the byte offset is formed in
+// 32-bit arithmetic, so the load/store can use a scalar base register plus a
+// 32-bit vector offset instead of a 64-bit vector address per field (callers
+// pick uint32_t only when every byte offset fits in 32 bits).
+template <class T, class Idx>
+__device__ __forceinline__ T* elem(T* base, Idx i) {
+    if constexpr (sizeof(Idx) == 4)
+        return reinterpret_cast<T*>(reinterpret_cast<char*>(base) + uint32_t(i * uint32_t(sizeof(T))));
+    else
+        return base + i;
+}
+template <class T, class Idx>
+__device__ __forceinline__ const T* elem(const T* base, Idx i) {
+    if constexpr (sizeof(Idx) == 4)
+        return reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) +
+                                          uint32_t(i * uint32_t(sizeof(T))));
+    else
+        return base + i;
+}
+
 struct StoreF32 {
     typedef float T;
-    static __device__ __forceinline__ float ld(const float* p, size_t i) { return p[i]; }
-    static __device__ __forceinline__ void st(float* p, size_t i, float v) { p[i] = v; }
+    template <class Idx>
+    static __device__ __forceinline__ float ld(const float* p, Idx i) { return *elem(p, i); }
+    template <class Idx>
+    static __device__ __forceinline__ void st(float* p, Idx i, float v) { *elem(p, i) = v; }
 };
 
 struct StoreBF16 {
     typedef uint16_t T;
-    static __device__ __forceinline__ float ld(const uint16_t* p, size_t i) {
-        return __uint_as_float(uint32_t(p[i]) << 16);
+    template <class Idx>
+    static __device__ __forceinline__ float ld(const uint16_t* p, Idx i) {
+        return __uint_as_float(uint32_t(*elem(p, i)) << 16);
     }
     static __device__ __forceinline__ uint16_t rne(float v) {
         uint32_t u = __float_as_uint(v);
@@ -29,7 +52,8 @@ struct StoreBF16 {
         u += 0x7fffu + ((u >> 16) & 1u);
         return uint16_t(u >> 16);
     }
-    static __device__ __forceinline__ void st(uint16_t* p, size_t i, float v) { p[i] = rne(v); }
+    template <class Idx>
+    static __device__ __forceinline__ void st(uint16_t* p, Idx i, float v) { *elem(p, i) = rne(v); }
 };
 
 // Field indices inside the 17-field state block (C ABI order).
@@ -48,7 +72,27 @@ struct SweepParams {
     float lambda, alpha0, alpha1, tau, sigma, theta;
     int data_term;   // 0 L1, 1 L2
     int zchunk;      // planes per block along z
+    int tile;        // tile shape id (tile_shape())
 };
+
+// Column-tile shapes of the sweep kernels: TX x TY threads, each thread owning
+// VX consecutive x voxels (ids 0-3: one voxel per thread; ids 4-7: four voxels
+// per thread with vector loads, used when the width is a multiple of 4).
+inline void tile_shape(int id, int& tx, int& ty, int& vx) {
+    vx = id >= 4 ? 4 : 1;
+    switch (id) {
+        case 1: tx = 64; ty = 8; break;
+        case 2: tx = 32; ty = 8; break;
+        case 3: tx = 64; ty = 2; break;
+        case 4: tx = 64; ty = 4; break;
+        case 5: tx = 64; ty = 2; break;
+        case 6: tx = 64; ty = 8; break;
+        case 7: tx = 32; ty = 8; break;
+        default: tx = 64; ty = 4; break;
+    }
+}
+constexpr int TILE_SCALAR_DEFAULT = 0;
+constexpr int TILE_VECTOR_DEFAULT = 4;
 
 // Launchers (stream-ordered, asynchronous).  `fields` points at 17 arrays of
 // the storage type (void* so both storages share the signature).
@@ -72,6 +116,7 @@ hipError_t launch_dct_mfma_pass(hipStream_t s, const float* in, float* out, cons
                                 int axis, int inverse, int last, int precision,
                                 const float* eig_x, const float* eig_y, const float* eig_z,
                                 int fuse_poisson, int quirk);
+int dct_mfma_max_extent();
 hipError_t launch_poisson_divide(hipStream_t s, const float* in, float* out, const float* cx,
                                  const float* cy, const float* cz, Geometry g, int quirk);
 hipError_t launch_epilogue(hipStream_t s, const float* u, float* l, const float* mask, float* r,

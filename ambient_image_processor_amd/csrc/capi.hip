@@ -14,6 +14,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <algorithm>
 #include <cstdlib>
 #include <string>
 #include <vector>
@@ -58,7 +59,7 @@ struct AxisTables {
     int n = 0;
     double* fwd64 = nullptr;     // [m][k] cospi((m+0.5)k/n)
     double* inv64 = nullptr;     // [m][k] cospi((k+0.5)m/n)
-    float* fwd32 = nullptr;      // FP32 copies for the MFMA passes, [k][m] layout
+    float* fwd32 = nullptr;      // FP32 copies for the MFMA passes, [m][k] layout
     float* inv32 = nullptr;
     uint16_t* fwd_hi = nullptr;  // bf16 splits, [k][m]
     uint16_t* fwd_lo = nullptr;
@@ -258,15 +259,16 @@ amb_status ensure_axis(amb_context* c, int axis, int n) {
             f64[size_t(m) * n + k] = host_cospi((double(m) + 0.5) * double(k) / size);
             i64v[size_t(m) * n + k] = host_cospi((double(k) + 0.5) * double(m) / size);
         }
-    // MFMA tables, [k][m] (row = output index, contiguous over the summed index)
+    // MFMA tables: FP32 [m][k] (contiguous over the output index), bf16 splits
+    // [k][m] (contiguous over the summed index)
     for (int k = 0; k < n; ++k)
         for (int m = 0; m < n; ++m) {
             const double cf = f64[size_t(m) * n + k];
             // inverse: out[k] = in[0] + 2 sum_{m>=1} in[m] cos  -> fold the factor 2 and the
             // m = 0 term into the matrix (weights 1 and 2)
             const double ci = (m == 0 ? 1.0 : 2.0) * i64v[size_t(m) * n + k];
-            f32[size_t(k) * n + m] = float(cf);
-            i32[size_t(k) * n + m] = float(ci);
+            f32[size_t(m) * n + k] = float(cf);
+            i32[size_t(m) * n + k] = float(ci);
             const uint16_t h1 = host_bf16_rne(float(cf));
             fh[size_t(k) * n + m] = h1;
             fl[size_t(k) * n + m] = host_bf16_rne(float(cf - double(host_bf16_to_float(h1))));
@@ -312,13 +314,31 @@ hipMemcpyKind kind_out(amb_memory mem) {
     return mem == AMB_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
 }
 
-int choose_zchunk(const Geometry& g) {
+// Sweep tile shape: 64x4 threads unless AMB_TILE selects another shape id
+// (tile_shape() in amb_internal.hpp), for tuning.
+int choose_tile(const Geometry& g) {
+    int t = g.w % 4 == 0 ? TILE_VECTOR_DEFAULT : TILE_SCALAR_DEFAULT;
+    const char* env = std::getenv("AMB_TILE");
+    if (env) {
+        const int v = std::atoi(env);
+        if (v >= 0 && v <= 7) t = v;
+    }
+    if (t >= 4 && g.w % 4 != 0) t = TILE_SCALAR_DEFAULT;   // vector tiles need W % 4 == 0
+    return t;
+}
+
+// Planes per block: 4 for FP32 vector tiles and 16 for BF16 vector tiles
+// (measured on MI355X at 256^3), else >= 4096 blocks with chunks of 8..64.
+int choose_zchunk(const Geometry& g, int tile, int storage) {
     const char* env = std::getenv("AMB_ZCHUNK");
     if (env) {
         const int v = std::atoi(env);
         if (v > 0) return v;
     }
-    const long nxy = long((g.w + 63) / 64) * long((g.h + 3) / 4);
+    if (tile >= 4) return std::min(storage == 1 ? 16 : 4, g.d);
+    int tx, ty, vx;
+    tile_shape(tile, tx, ty, vx);
+    const long nxy = long((g.w + tx * vx - 1) / (tx * vx)) * long((g.h + ty - 1) / ty);
     // aim for >= 4096 blocks (16 per CU) while keeping chunks >= 8 planes
     long zc = (long(g.d) * nxy) / 4096;
     if (zc < 8) zc = 8;
@@ -382,7 +402,8 @@ amb_status begin(amb_context* c, amb_memory mem, const float* f, uint32_t w, uin
     c->sp.lambda = p->lambda; c->sp.alpha0 = p->alpha0; c->sp.alpha1 = p->alpha1;
     c->sp.tau = p->tau; c->sp.sigma = p->sigma; c->sp.theta = p->theta;
     c->sp.data_term = int(p->data_term);
-    c->sp.zchunk = choose_zchunk(g);
+    c->sp.tile = choose_tile(g);
+    c->sp.zchunk = choose_zchunk(g, c->sp.tile, int(p->storage));
     c->have_state = true;
     return AMB_OK;
 }
@@ -424,10 +445,22 @@ amb_status run(amb_context* c, uint32_t iterations, uint32_t interval, amb_itera
     return AMB_OK;
 }
 
-// One separable 3-pass transform from src into dst (src is overwritten as scratch).
+amb_status check_precision(const Geometry& g, int precision) {
+    if (precision == AMB_DCT_REFERENCE) return AMB_OK;
+    const int mx = dct_mfma_max_extent();
+    if (g.w > mx || g.h > mx || g.d > mx)
+        return fail(AMB_ERR_NOT_SUPPORTED,
+                    "MFMA DCT passes support extents up to 512; use AMB_DCT_REFERENCE");
+    return AMB_OK;
+}
+
+// One separable 3-pass transform from src into dst (src is overwritten as
+// scratch).  poisson_flags >= 0 fuses the eigenvalue divide into the last
+// forward pass (MFMA precisions only).
 amb_status transform3(amb_context* c, float* src, float* tmp, float* dst, int inverse,
-                      int precision) {
+                      int precision, int poisson_flags = -1) {
     const Geometry g = c->g;
+    AMB_TRY(check_precision(g, precision));
     AMB_TRY(ensure_tables(c, g));
     float* bufs[4] = {src, tmp, src, dst};   // x: src->tmp, y: tmp->src, z: src->dst
     for (int axis = 0; axis < 3; ++axis) {
@@ -438,10 +471,13 @@ amb_status transform3(amb_context* c, float* src, float* tmp, float* dst, int in
             AMB_HIP(launch_dct_reference_pass(c->stream, bufs[axis], bufs[axis + 1],
                                               inverse ? t.inv64 : t.fwd64, g, axis, inverse, last));
         } else {
+            const int fuse = (last && !inverse && poisson_flags >= 0) ? 1 : 0;
+            const int quirk = (poisson_flags > 0 && (poisson_flags & AMB_FLAG_POISSON_INDEX_QUIRK)) ? 1 : 0;
             AMB_HIP(launch_dct_mfma_pass(c->stream, bufs[axis], bufs[axis + 1],
                                          inverse ? t.inv32 : t.fwd32, inverse ? t.inv_hi : t.fwd_hi,
                                          inverse ? t.inv_lo : t.fwd_lo, g, axis, inverse, last,
-                                         precision, nullptr, nullptr, nullptr, 0, 0));
+                                         precision, c->axes[0].eig, c->axes[1].eig, c->axes[2].eig,
+                                         fuse, quirk));
         }
     }
     return AMB_OK;
@@ -465,9 +501,15 @@ amb_status integrate(amb_context* c, const float* vx, const float* vy, const flo
         ProfScope ps(c, P_DIV);
         AMB_HIP(launch_divergence(c->stream, vx, vy, vz, s0, c->g));
     }
-    AMB_TRY(transform3(c, s0, s1, s2, 0, precision));     // DCT -> s2
-    AMB_TRY(poisson(c, s2, s0, flags));                    // -> s0
-    AMB_TRY(transform3(c, s0, s2, s1, 1, precision));     // iDCT -> s1
+    if (precision == AMB_DCT_REFERENCE) {
+        AMB_TRY(transform3(c, s0, s1, s2, 0, precision));  // DCT -> s2
+        AMB_TRY(poisson(c, s2, s0, flags));                 // -> s0
+        AMB_TRY(transform3(c, s0, s2, s1, 1, precision));  // iDCT -> s1
+    } else {
+        // eigenvalue divide fused into the forward z pass
+        AMB_TRY(transform3(c, s0, s1, s2, 0, precision, int(flags & AMB_FLAG_POISSON_INDEX_QUIRK)));
+        AMB_TRY(transform3(c, s2, s0, s1, 1, precision));  // iDCT -> s1
+    }
     *result = s1;
     return AMB_OK;
 }

@@ -188,10 +188,4 @@ hipError_t launch_epilogue(hipStream_t s, const float* u, float* l, const float*
     return hipGetLastError();
 }
 
-hipError_t launch_dct_mfma_pass(hipStream_t, const float*, float*, const float*, const uint16_t*,
-                                const uint16_t*, Geometry, int, int, int, int, const float*,
-                                const float*, const float*, int, int) {
-    return hipErrorNotSupported;
-}
-
 }  // namespace amb

@@ -30,12 +30,10 @@
 // -ffp-contract=off).
 #include "amb_internal.hpp"
 
+#include <cstdlib>
+
 namespace amb {
 namespace {
-
-constexpr int TX = 64;
-constexpr int TY = 4;
-constexpr int NT = TX * TY;
 
 // Block id -> tile id such that each XCD (blocks are dealt round-robin to the
 // 8 XCDs) receives a contiguous range of tiles.
@@ -44,21 +42,19 @@ __device__ __forceinline__ int xcd_tile(int bid, int nblocks) {
     return (bid & 7) * (nblocks >> 3) + (bid >> 3);
 }
 
+template <int TX, int TY>
 struct TileCoord {
     int x0, y0, z0, z1, tx, ty, x, y;
+    __device__ __forceinline__ TileCoord(const Geometry& g, int zchunk) {
+        const int nbx = (g.w + TX - 1) / TX, nby = (g.h + TY - 1) / TY;
+        const int t = xcd_tile(blockIdx.x, gridDim.x);
+        const int bx = t % nbx, r = t / nbx, by = r % nby, bz = r / nby;
+        x0 = bx * TX; y0 = by * TY;
+        z0 = bz * zchunk; z1 = min(z0 + zchunk, g.d);
+        tx = threadIdx.x % TX; ty = threadIdx.x / TX;
+        x = x0 + tx; y = y0 + ty;
+    }
 };
-
-__device__ __forceinline__ TileCoord tile_coord(const Geometry& g, int zchunk) {
-    const int nbx = (g.w + TX - 1) / TX, nby = (g.h + TY - 1) / TY;
-    const int t = xcd_tile(blockIdx.x, gridDim.x);
-    TileCoord c;
-    const int bx = t % nbx, r = t / nbx, by = r % nby, bz = r / nby;
-    c.x0 = bx * TX; c.y0 = by * TY;
-    c.z0 = bz * zchunk; c.z1 = min(c.z0 + zchunk, g.d);
-    c.tx = threadIdx.x % TX; c.ty = threadIdx.x / TX;
-    c.x = c.x0 + c.tx; c.y = c.y0 + c.ty;
-    return c;
-}
 
 // Reference "backward difference" (tgv_cuda/tgv_common.cu:133-197): -a[i] + a[i-1], -a[0] at 0.
 __device__ __forceinline__ float bwd(float center, float prev, bool has_prev) {
@@ -72,8 +68,8 @@ __device__ __forceinline__ float fwd(float center, float next, bool has_next) {
 // ---------------------------------------------------------------------------
 // Dual sweep
 // ---------------------------------------------------------------------------
-template <class S>
-__global__ __launch_bounds__(NT) void dual_sweep_kernel(
+template <class S, class I, int TX, int TY>
+__global__ __launch_bounds__(TX * TY) void dual_sweep_kernel(
     typename S::T* __restrict__ ubar, typename S::T* __restrict__ vbx,
     typename S::T* __restrict__ vby, typename S::T* __restrict__ vbz,
     typename S::T* __restrict__ px_, typename S::T* __restrict__ py_,
@@ -82,38 +78,40 @@ __global__ __launch_bounds__(NT) void dual_sweep_kernel(
     typename S::T* __restrict__ qxy_, typename S::T* __restrict__ qxz_,
     typename S::T* __restrict__ qyz_, Geometry g, float sigma, float alpha0, float alpha1,
     int zchunk) {
-    // ubar plane with +x/+y halo, v_bar planes with -x/-y halo; centre at [ty+1][tx+1].
+    // u_bar plane with +x/+y halo, v_bar planes with -x/-y halo; centre at [ty+1][tx+1].
     __shared__ float sU[TY + 2][TX + 2];
     __shared__ float sV[3][TY + 2][TX + 2];
 
-    const TileCoord c = tile_coord(g, zchunk);
+    const TileCoord<TX, TY> c(g, zchunk);
     const int W = g.w, H = g.h, D = g.d;
-    const size_t plane = size_t(W) * size_t(H);
+    const I plane = I(W) * I(H);
     const bool in = (c.x < W) && (c.y < H);
-    const size_t col = in ? size_t(c.y) * W + c.x : 0;
+    const I col = in ? I(c.y) * I(W) + I(c.x) : I(0);
     const bool hx = (c.tx == 0) && (c.y < H);    // loads the x halo of its row
     const bool hy = (c.ty == 0) && (c.x < W);    // loads the y halo of its column
     const bool has_xp = c.x0 + TX < W, has_xm = c.x0 > 0;
     const bool has_yp = c.y0 + TY < H, has_ym = c.y0 > 0;
-    const size_t hx_p = size_t(c.y) * W + (c.x0 + TX), hx_m = size_t(c.y) * W + (c.x0 - 1);
-    const size_t hy_p = size_t(c.y0 + TY) * W + c.x, hy_m = size_t(c.y0 - 1) * W + c.x;
+    const I hx_p = has_xp && hx ? I(c.y) * I(W) + I(c.x0 + TX) : I(0);
+    const I hx_m = has_xm && hx ? I(c.y) * I(W) + I(c.x0 - 1) : I(0);
+    const I hy_p = has_yp && hy ? I(c.y0 + TY) * I(W) + I(c.x) : I(0);
+    const I hy_m = has_ym && hy ? I(c.y0 - 1) * I(W) + I(c.x) : I(0);
 
     typename S::T* const vb[3] = {vbx, vby, vbz};
+    typename S::T* const pf[3] = {px_, py_, pz_};
+    typename S::T* const qf[6] = {qxx_, qyy_, qzz_, qxy_, qxz_, qyz_};
 
     // z-pipeline registers: u_bar (z, z+1), v_bar (z-1, z), p and q (z)
     float ub_c = 0.f, ub_n = 0.f, vb_p[3] = {0.f, 0.f, 0.f}, vb_c[3] = {0.f, 0.f, 0.f};
     float pc[3] = {0.f, 0.f, 0.f}, qc[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-    typename S::T* const pf[3] = {px_, py_, pz_};
-    typename S::T* const qf[6] = {qxx_, qyy_, qzz_, qxy_, qxz_, qyz_};
     int z = c.z0;
     if (in) {
-        const size_t i0 = size_t(z) * plane + col;
+        const I i0 = I(z) * plane + col;
         ub_c = S::ld(ubar, i0);
-        if (z + 1 < D) ub_n = S::ld(ubar, i0 + plane);
+        if (z + 1 < D) ub_n = S::ld(ubar, I(i0 + plane));
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
             vb_c[k] = S::ld(vb[k], i0);
-            if (z > 0) vb_p[k] = S::ld(vb[k], i0 - plane);
+            if (z > 0) vb_p[k] = S::ld(vb[k], I(i0 - plane));
             pc[k] = S::ld(pf[k], i0);
         }
 #pragma unroll
@@ -121,21 +119,21 @@ __global__ __launch_bounds__(NT) void dual_sweep_kernel(
     }
 
     for (; z < c.z1; ++z) {
-        const size_t zo = size_t(z) * plane;
-        const size_t i = zo + col;
+        const I zo = I(z) * plane;
+        const I i = zo + col;
         // stage this plane in LDS
         sU[c.ty + 1][c.tx + 1] = ub_c;
 #pragma unroll
         for (int k = 0; k < 3; ++k) sV[k][c.ty + 1][c.tx + 1] = vb_c[k];
         if (hx) {
-            sU[c.ty + 1][TX + 1] = has_xp ? S::ld(ubar, zo + hx_p) : 0.f;
+            sU[c.ty + 1][TX + 1] = has_xp ? S::ld(ubar, I(zo + hx_p)) : 0.f;
 #pragma unroll
-            for (int k = 0; k < 3; ++k) sV[k][c.ty + 1][0] = has_xm ? S::ld(vb[k], zo + hx_m) : 0.f;
+            for (int k = 0; k < 3; ++k) sV[k][c.ty + 1][0] = has_xm ? S::ld(vb[k], I(zo + hx_m)) : 0.f;
         }
         if (hy) {
-            sU[TY + 1][c.tx + 1] = has_yp ? S::ld(ubar, zo + hy_p) : 0.f;
+            sU[TY + 1][c.tx + 1] = has_yp ? S::ld(ubar, I(zo + hy_p)) : 0.f;
 #pragma unroll
-            for (int k = 0; k < 3; ++k) sV[k][0][c.tx + 1] = has_ym ? S::ld(vb[k], zo + hy_m) : 0.f;
+            for (int k = 0; k < 3; ++k) sV[k][0][c.tx + 1] = has_ym ? S::ld(vb[k], I(zo + hy_m)) : 0.f;
         }
         __syncthreads();
 
@@ -143,17 +141,16 @@ __global__ __launch_bounds__(NT) void dual_sweep_kernel(
         float ub_nn = 0.f, vb_n[3] = {0.f, 0.f, 0.f}, pn[3] = {0.f, 0.f, 0.f};
         float qn[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
         if (in && z + 1 < c.z1) {
-            if (z + 2 < D) ub_nn = S::ld(ubar, i + 2 * plane);
+            const I j = i + plane;
+            if (z + 2 < D) ub_nn = S::ld(ubar, I(j + plane));
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
-                vb_n[k] = S::ld(vb[k], i + plane);
-                pn[k] = S::ld(pf[k], i + plane);
+                vb_n[k] = S::ld(vb[k], j);
+                pn[k] = S::ld(pf[k], j);
             }
 #pragma unroll
-            for (int k = 0; k < 6; ++k) qn[k] = S::ld(qf[k], i + plane);
+            for (int k = 0; k < 6; ++k) qn[k] = S::ld(qf[k], j);
         }
-        const float p0 = pc[0], p1 = pc[1], p2 = pc[2];
-        const float q0 = qc[0], q1 = qc[1], q2 = qc[2], q3 = qc[3], q4 = qc[4], q5 = qc[5];
 
         if (in) {
             // D+ u_bar
@@ -161,16 +158,18 @@ __global__ __launch_bounds__(NT) void dual_sweep_kernel(
             const float dy = fwd(ub_c, sU[c.ty + 2][c.tx + 1], c.y < H - 1);
             const float dz = fwd(ub_c, ub_n, z < D - 1);
             // p update + projection onto |p| <= alpha1 (tgv2_common.cu:118-128)
-            float ax = fmaf(sigma, dx - vb_c[0], p0);
-            float ay = fmaf(sigma, dy - vb_c[1], p1);
-            float az = fmaf(sigma, dz - vb_c[2], p2);
+            const float ax = fmaf(sigma, dx - vb_c[0], pc[0]);
+            const float ay = fmaf(sigma, dy - vb_c[1], pc[1]);
+            const float az = fmaf(sigma, dz - vb_c[2], pc[2]);
             float n2 = ax * ax;
             n2 = fmaf(ay, ay, n2);
             n2 = fmaf(az, az, n2);
-            const float np = fmaxf(1.0f, sqrtf(n2) / alpha1);
-            S::st(px_, i, ax / np);
-            S::st(py_, i, ay / np);
-            S::st(pz_, i, az / np);
+            // p / max(1, |p|/alpha1): the division is skipped when the max is 1 (x / 1 == x)
+            const float tp = sqrtf(n2) / alpha1;
+            const bool sp = tp > 1.0f;
+            S::st(px_, i, sp ? ax / tp : ax);
+            S::st(py_, i, sp ? ay / tp : ay);
+            S::st(pz_, i, sp ? az / tp : az);
 
             // eps_B(v_bar) (tgv2_common.cu:174-219): B_a applied to v_bar_c
             float bxv[3], byv[3], bzv[3];
@@ -185,25 +184,26 @@ __global__ __launch_bounds__(NT) void dual_sweep_kernel(
             const float exz = (bxv[2] + bzv[0]) * 0.5f;
             const float eyz = (byv[2] + bzv[1]) * 0.5f;
             // q update + projection onto |q|_w <= alpha0 (tgv2_common.cu:246-270)
-            const float bxx = fmaf(sigma, exx, q0);
-            const float byy = fmaf(sigma, eyy, q1);
-            const float bxy = fmaf(sigma, exy, q3);
-            const float bzz = fmaf(sigma, ezz, q2);
-            const float bxz = fmaf(sigma, exz, q4);
-            const float byz = fmaf(sigma, eyz, q5);
+            const float bxx = fmaf(sigma, exx, qc[0]);
+            const float byy = fmaf(sigma, eyy, qc[1]);
+            const float bxy = fmaf(sigma, exy, qc[3]);
+            const float bzz = fmaf(sigma, ezz, qc[2]);
+            const float bxz = fmaf(sigma, exz, qc[4]);
+            const float byz = fmaf(sigma, eyz, qc[5]);
             float m2 = bxx * bxx;
             m2 = fmaf(byy, byy, m2);
             m2 = fmaf(2.0f * bxy, bxy, m2);
             m2 = fmaf(bzz, bzz, m2);
             m2 = fmaf(2.0f * bxz, bxz, m2);
             m2 = fmaf(2.0f * byz, byz, m2);
-            const float nq = fmaxf(1.0f, sqrtf(m2) / alpha0);
-            S::st(qxx_, i, bxx / nq);
-            S::st(qyy_, i, byy / nq);
-            S::st(qxy_, i, bxy / nq);
-            S::st(qzz_, i, bzz / nq);
-            S::st(qxz_, i, bxz / nq);
-            S::st(qyz_, i, byz / nq);
+            const float tq = sqrtf(m2) / alpha0;
+            const bool sq = tq > 1.0f;
+            S::st(qxx_, i, sq ? bxx / tq : bxx);
+            S::st(qyy_, i, sq ? byy / tq : byy);
+            S::st(qxy_, i, sq ? bxy / tq : bxy);
+            S::st(qzz_, i, sq ? bzz / tq : bzz);
+            S::st(qxz_, i, sq ? bxz / tq : bxz);
+            S::st(qyz_, i, sq ? byz / tq : byz);
         }
         __syncthreads();
         ub_c = ub_n; ub_n = ub_nn;
@@ -221,8 +221,8 @@ __global__ __launch_bounds__(NT) void dual_sweep_kernel(
 // +x: q_xx, q_xy, q_xz.  +y: q_xy, q_yy, q_yz.
 enum { L_PX = 0, L_PY = 1, L_QXX = 2, L_QXY = 3, L_QXZ = 4, L_QYY = 5, L_QYZ = 6, L_N = 7 };
 
-template <class S, int L2>
-__global__ __launch_bounds__(NT) void primal_sweep_kernel(
+template <class S, class I, int TX, int TY, int L2>
+__global__ __launch_bounds__(TX * TY) void primal_sweep_kernel(
     typename S::T* __restrict__ u_, typename S::T* __restrict__ ubar_,
     typename S::T* __restrict__ vx_, typename S::T* __restrict__ vy_,
     typename S::T* __restrict__ vz_, typename S::T* __restrict__ vbx_,
@@ -235,17 +235,19 @@ __global__ __launch_bounds__(NT) void primal_sweep_kernel(
     float tau, float lambda, float theta, int zchunk) {
     __shared__ float sL[L_N][TY + 2][TX + 2];
 
-    const TileCoord c = tile_coord(g, zchunk);
+    const TileCoord<TX, TY> c(g, zchunk);
     const int W = g.w, H = g.h, D = g.d;
-    const size_t plane = size_t(W) * size_t(H);
+    const I plane = I(W) * I(H);
     const bool in = (c.x < W) && (c.y < H);
-    const size_t col = in ? size_t(c.y) * W + c.x : 0;
+    const I col = in ? I(c.y) * I(W) + I(c.x) : I(0);
     const bool hx = (c.tx == 0) && (c.y < H);
     const bool hy = (c.ty == 0) && (c.x < W);
     const bool has_xp = c.x0 + TX < W, has_xm = c.x0 > 0;
     const bool has_yp = c.y0 + TY < H, has_ym = c.y0 > 0;
-    const size_t hx_p = size_t(c.y) * W + (c.x0 + TX), hx_m = size_t(c.y) * W + (c.x0 - 1);
-    const size_t hy_p = size_t(c.y0 + TY) * W + c.x, hy_m = size_t(c.y0 - 1) * W + c.x;
+    const I hx_p = has_xp && hx ? I(c.y) * I(W) + I(c.x0 + TX) : I(0);
+    const I hx_m = has_xm && hx ? I(c.y) * I(W) + I(c.x0 - 1) : I(0);
+    const I hy_p = has_yp && hy ? I(c.y0 + TY) * I(W) + I(c.x) : I(0);
+    const I hy_m = has_ym && hy ? I(c.y0 - 1) * I(W) + I(c.x) : I(0);
     // depth == 1 drops B_z p_z in the L2 update (tgv2_l2.cu:39-42); L1 always keeps it.
     const bool use_bz = !L2 || D > 1;
     const float tl = tau * lambda;
@@ -257,23 +259,23 @@ __global__ __launch_bounds__(NT) void primal_sweep_kernel(
     float u_c = 0.f, f_c = 0.f, v_c[3] = {0.f, 0.f, 0.f};
     int z = c.z0;
     if (in) {
-        const size_t i0 = size_t(z) * plane + col;
-        if (z > 0) pz_p = S::ld(pz_, i0 - plane);
+        const I i0 = I(z) * plane + col;
+        if (z > 0) pz_p = S::ld(pz_, I(i0 - plane));
         pz_c = S::ld(pz_, i0);
         qzz_c = S::ld(qzz_, i0); qxz_c = S::ld(qxz_, i0); qyz_c = S::ld(qyz_, i0);
         if (z + 1 < D) {
-            qzz_n = S::ld(qzz_, i0 + plane); qxz_n = S::ld(qxz_, i0 + plane);
-            qyz_n = S::ld(qyz_, i0 + plane);
+            const I j = i0 + plane;
+            qzz_n = S::ld(qzz_, j); qxz_n = S::ld(qxz_, j); qyz_n = S::ld(qyz_, j);
         }
         px_c = S::ld(px_, i0); py_c = S::ld(py_, i0);
         qxx_c = S::ld(qxx_, i0); qxy_c = S::ld(qxy_, i0); qyy_c = S::ld(qyy_, i0);
-        u_c = S::ld(u_, i0); f_c = f_[i0];
+        u_c = S::ld(u_, i0); f_c = *elem(f_, i0);
         v_c[0] = S::ld(vx_, i0); v_c[1] = S::ld(vy_, i0); v_c[2] = S::ld(vz_, i0);
     }
 
     for (; z < c.z1; ++z) {
-        const size_t zo = size_t(z) * plane;
-        const size_t i = zo + col;
+        const I zo = I(z) * plane;
+        const I i = zo + col;
         const int ly = c.ty + 1, lx = c.tx + 1;
         sL[L_PX][ly][lx] = px_c;
         sL[L_PY][ly][lx] = py_c;
@@ -283,16 +285,16 @@ __global__ __launch_bounds__(NT) void primal_sweep_kernel(
         sL[L_QYY][ly][lx] = qyy_c;
         sL[L_QYZ][ly][lx] = qyz_c;
         if (hx) {
-            sL[L_PX][ly][0] = has_xm ? S::ld(px_, zo + hx_m) : 0.f;
-            sL[L_QXX][ly][TX + 1] = has_xp ? S::ld(qxx_, zo + hx_p) : 0.f;
-            sL[L_QXY][ly][TX + 1] = has_xp ? S::ld(qxy_, zo + hx_p) : 0.f;
-            sL[L_QXZ][ly][TX + 1] = has_xp ? S::ld(qxz_, zo + hx_p) : 0.f;
+            sL[L_PX][ly][0] = has_xm ? S::ld(px_, I(zo + hx_m)) : 0.f;
+            sL[L_QXX][ly][TX + 1] = has_xp ? S::ld(qxx_, I(zo + hx_p)) : 0.f;
+            sL[L_QXY][ly][TX + 1] = has_xp ? S::ld(qxy_, I(zo + hx_p)) : 0.f;
+            sL[L_QXZ][ly][TX + 1] = has_xp ? S::ld(qxz_, I(zo + hx_p)) : 0.f;
         }
         if (hy) {
-            sL[L_PY][0][lx] = has_ym ? S::ld(py_, zo + hy_m) : 0.f;
-            sL[L_QXY][TY + 1][lx] = has_yp ? S::ld(qxy_, zo + hy_p) : 0.f;
-            sL[L_QYY][TY + 1][lx] = has_yp ? S::ld(qyy_, zo + hy_p) : 0.f;
-            sL[L_QYZ][TY + 1][lx] = has_yp ? S::ld(qyz_, zo + hy_p) : 0.f;
+            sL[L_PY][0][lx] = has_ym ? S::ld(py_, I(zo + hy_m)) : 0.f;
+            sL[L_QXY][TY + 1][lx] = has_yp ? S::ld(qxy_, I(zo + hy_p)) : 0.f;
+            sL[L_QYY][TY + 1][lx] = has_yp ? S::ld(qyy_, I(zo + hy_p)) : 0.f;
+            sL[L_QYZ][TY + 1][lx] = has_yp ? S::ld(qyz_, I(zo + hy_p)) : 0.f;
         }
         __syncthreads();
 
@@ -301,15 +303,15 @@ __global__ __launch_bounds__(NT) void primal_sweep_kernel(
               n_qxx = 0.f, n_qxy = 0.f, n_qyy = 0.f, n_u = 0.f, n_f = 0.f;
         float n_v[3] = {0.f, 0.f, 0.f};
         if (in && z + 1 < c.z1) {
-            const size_t j = i + plane;
+            const I j = i + plane;
             n_pz = S::ld(pz_, j);
             if (z + 2 < D) {
-                n_qzz = S::ld(qzz_, j + plane); n_qxz = S::ld(qxz_, j + plane);
-                n_qyz = S::ld(qyz_, j + plane);
+                const I j2 = j + plane;
+                n_qzz = S::ld(qzz_, j2); n_qxz = S::ld(qxz_, j2); n_qyz = S::ld(qyz_, j2);
             }
             n_px = S::ld(px_, j); n_py = S::ld(py_, j);
             n_qxx = S::ld(qxx_, j); n_qxy = S::ld(qxy_, j); n_qyy = S::ld(qyy_, j);
-            n_u = S::ld(u_, j); n_f = f_[j];
+            n_u = S::ld(u_, j); n_f = *elem(f_, j);
             n_v[0] = S::ld(vx_, j); n_v[1] = S::ld(vy_, j); n_v[2] = S::ld(vz_, j);
         }
 
@@ -346,16 +348,15 @@ __global__ __launch_bounds__(NT) void primal_sweep_kernel(
             const float w1 = (dyy + dxxy) + dzyz;
             const float w2 = (dzz + dxxz) + dyyz;
             // v update + over-relaxation (tgv2_common.cu:338-347)
-            const float pc[3] = {px_c, py_c, pz_c};
-            const float wv[3] = {w0, w1, w2};
-            typename S::T* const vo[3] = {vx_, vy_, vz_};
-            typename S::T* const vbo[3] = {vbx_, vby_, vbz_};
-#pragma unroll
-            for (int k = 0; k < 3; ++k) {
-                const float vn = fmaf(-tau, wv[k] - pc[k], v_c[k]);
-                S::st(vo[k], i, vn);
-                S::st(vbo[k], i, fmaf(theta, vn - v_c[k], vn));
-            }
+            const float vn0 = fmaf(-tau, w0 - px_c, v_c[0]);
+            const float vn1 = fmaf(-tau, w1 - py_c, v_c[1]);
+            const float vn2 = fmaf(-tau, w2 - pz_c, v_c[2]);
+            S::st(vx_, i, vn0);
+            S::st(vy_, i, vn1);
+            S::st(vz_, i, vn2);
+            S::st(vbx_, i, fmaf(theta, vn0 - v_c[0], vn0));
+            S::st(vby_, i, fmaf(theta, vn1 - v_c[1], vn1));
+            S::st(vbz_, i, fmaf(theta, vn2 - v_c[2], vn2));
         }
         __syncthreads();
         pz_p = pz_c; pz_c = n_pz;
@@ -368,11 +369,425 @@ __global__ __launch_bounds__(NT) void primal_sweep_kernel(
     }
 }
 
+// ---------------------------------------------------------------------------
+// Vector sweeps: each thread owns VX consecutive x voxels (W % VX == 0), so
+// every field moves as one VX-wide load/store per thread (16 B for FP32, 8 B
+// for BF16 at VX = 4).  y neighbours come from full LDS rows (+/- one halo row
+// loaded by the ty == 0 threads); x neighbours inside the vector are in
+// registers, across threads one edge element per field goes through LDS.
+// PF = 1 issues the next plane's loads before the current plane's arithmetic.
+// ---------------------------------------------------------------------------
+template <class S, int VX>
+struct VecIO;
+
+template <>
+struct VecIO<StoreF32, 4> {
+    template <class I>
+    static __device__ __forceinline__ void ld(const float* p, I i, float (&o)[4]) {
+        const float4 t = *reinterpret_cast<const float4*>(elem(p, i));
+        o[0] = t.x; o[1] = t.y; o[2] = t.z; o[3] = t.w;
+    }
+    template <class I>
+    static __device__ __forceinline__ void st(float* p, I i, const float (&v)[4]) {
+        *reinterpret_cast<float4*>(elem(p, i)) = make_float4(v[0], v[1], v[2], v[3]);
+    }
+};
+
+template <>
+struct VecIO<StoreBF16, 4> {
+    template <class I>
+    static __device__ __forceinline__ void ld(const uint16_t* p, I i, float (&o)[4]) {
+        const uint2 t = *reinterpret_cast<const uint2*>(elem(p, i));
+        o[0] = __uint_as_float(t.x << 16); o[1] = __uint_as_float(t.x & 0xffff0000u);
+        o[2] = __uint_as_float(t.y << 16); o[3] = __uint_as_float(t.y & 0xffff0000u);
+    }
+    template <class I>
+    static __device__ __forceinline__ void st(uint16_t* p, I i, const float (&v)[4]) {
+        uint2 t;
+        t.x = uint32_t(StoreBF16::rne(v[0])) | (uint32_t(StoreBF16::rne(v[1])) << 16);
+        t.y = uint32_t(StoreBF16::rne(v[2])) | (uint32_t(StoreBF16::rne(v[3])) << 16);
+        *reinterpret_cast<uint2*>(elem(p, i)) = t;
+    }
+};
+
+template <int VX>
+__device__ __forceinline__ void zero(float (&a)[VX]) {
+#pragma unroll
+    for (int e = 0; e < VX; ++e) a[e] = 0.f;
+}
+template <int VX>
+__device__ __forceinline__ void copyv(float (&d)[VX], const float (&s)[VX]) {
+#pragma unroll
+    for (int e = 0; e < VX; ++e) d[e] = s[e];
+}
+template <int VX>
+__device__ __forceinline__ void lds_put(float* row, const float (&v)[VX]) {
+#pragma unroll
+    for (int e = 0; e < VX; ++e) row[e] = v[e];
+}
+template <int VX>
+__device__ __forceinline__ void lds_get(const float* row, float (&v)[VX]) {
+#pragma unroll
+    for (int e = 0; e < VX; ++e) v[e] = row[e];
+}
+
+template <int TX, int TY, int VX>
+struct TileCoordV {
+    int x0, y0, z0, z1, tx, ty, xb, y;
+    __device__ __forceinline__ TileCoordV(const Geometry& g, int zchunk) {
+        constexpr int RW = TX * VX;
+        const int nbx = (g.w + RW - 1) / RW, nby = (g.h + TY - 1) / TY;
+        const int t = xcd_tile(blockIdx.x, gridDim.x);
+        const int bx = t % nbx, r = t / nbx, by = r % nby, bz = r / nby;
+        x0 = bx * RW; y0 = by * TY;
+        z0 = bz * zchunk; z1 = min(z0 + zchunk, g.d);
+        tx = threadIdx.x % TX; ty = threadIdx.x / TX;
+        xb = x0 + tx * VX; y = y0 + ty;
+    }
+};
+
+template <int VX>
+struct DualPlane {
+    float ubn[VX];        // u_bar at z + 1
+    float vb[3][VX];      // v_bar at z
+    float p[3][VX];
+    float q[6][VX];       // xx, yy, zz, xy, xz, yz
+};
+
+template <class S, class I, int TX, int TY, int VX, int PF>
+__global__ __launch_bounds__(TX * TY) void dual_sweep_vec_kernel(
+    typename S::T* __restrict__ ubar, typename S::T* __restrict__ vbx,
+    typename S::T* __restrict__ vby, typename S::T* __restrict__ vbz,
+    typename S::T* __restrict__ px_, typename S::T* __restrict__ py_,
+    typename S::T* __restrict__ pz_, typename S::T* __restrict__ qxx_,
+    typename S::T* __restrict__ qyy_, typename S::T* __restrict__ qzz_,
+    typename S::T* __restrict__ qxy_, typename S::T* __restrict__ qxz_,
+    typename S::T* __restrict__ qyz_, Geometry g, float sigma, float alpha0, float alpha1,
+    int zchunk) {
+    typedef VecIO<S, VX> V;
+    constexpr int RW = TX * VX;
+    __shared__ __align__(16) float sY[4][TY + 2][RW];   // 0: u_bar (+y halo row TY+1); 1-3: v_bar (-y halo row 0)
+    __shared__ float sXp[TY][TX + 1];                   // u_bar element 0 per thread; [ty][TX] = +x halo
+    __shared__ float sXm[3][TY][TX + 1];                // v_bar element VX-1 at [ty][tx+1]; [ty][0] = -x halo
+
+    const TileCoordV<TX, TY, VX> c(g, zchunk);
+    const int W = g.w, H = g.h, D = g.d;
+    const I plane = I(W) * I(H);
+    const bool in = (c.xb < W) && (c.y < H);
+    const I col = in ? I(c.y) * I(W) + I(c.xb) : I(0);
+    const bool hx = (c.tx == 0) && (c.y < H);
+    const bool hy = (c.ty == 0) && (c.xb < W);
+    const bool has_xp = c.x0 + RW < W, has_xm = c.x0 > 0;
+    const bool has_yp = c.y0 + TY < H, has_ym = c.y0 > 0;
+    const I hx_p = has_xp && hx ? I(c.y) * I(W) + I(c.x0 + RW) : I(0);
+    const I hx_m = has_xm && hx ? I(c.y) * I(W) + I(c.x0 - 1) : I(0);
+    const I hy_p = has_yp && hy ? I(c.y0 + TY) * I(W) + I(c.xb) : I(0);
+    const I hy_m = has_ym && hy ? I(c.y0 - 1) * I(W) + I(c.xb) : I(0);
+    typename S::T* const vb[3] = {vbx, vby, vbz};
+    typename S::T* const pf[3] = {px_, py_, pz_};
+    typename S::T* const qf[6] = {qxx_, qyy_, qzz_, qxy_, qxz_, qyz_};
+    const int lx = c.tx * VX;
+
+    auto load_plane = [&](int zz, DualPlane<VX>& P) {
+        const I i = I(zz) * plane + col;
+        if (zz + 1 < D) V::ld(ubar, I(i + plane), P.ubn); else zero(P.ubn);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) { V::ld(vb[k], i, P.vb[k]); V::ld(pf[k], i, P.p[k]); }
+#pragma unroll
+        for (int k = 0; k < 6; ++k) V::ld(qf[k], i, P.q[k]);
+    };
+
+    float ub_c[VX], vb_p[3][VX];
+    DualPlane<VX> P;
+    zero(ub_c);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) zero(vb_p[k]);
+    if (in) {
+        const I i0 = I(c.z0) * plane + col;
+        V::ld(ubar, i0, ub_c);
+        if (c.z0 > 0)
+#pragma unroll
+            for (int k = 0; k < 3; ++k) V::ld(vb[k], I(i0 - plane), vb_p[k]);
+        if (PF) load_plane(c.z0, P);
+    }
+
+    for (int z = c.z0; z < c.z1; ++z) {
+        const I zo = I(z) * plane;
+        const I i = zo + col;
+        if (!PF && in) load_plane(z, P);
+        // stage this plane
+        lds_put<VX>(&sY[0][c.ty + 1][lx], ub_c);
+        sXp[c.ty][c.tx] = ub_c[0];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            lds_put<VX>(&sY[1 + k][c.ty + 1][lx], P.vb[k]);
+            sXm[k][c.ty][c.tx + 1] = P.vb[k][VX - 1];
+        }
+        if (hx) {
+            sXp[c.ty][TX] = has_xp ? S::ld(ubar, I(zo + hx_p)) : 0.f;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) sXm[k][c.ty][0] = has_xm ? S::ld(vb[k], I(zo + hx_m)) : 0.f;
+        }
+        if (hy) {
+            float h[VX];
+            if (has_yp) V::ld(ubar, I(zo + hy_p), h); else zero(h);
+            lds_put<VX>(&sY[0][TY + 1][lx], h);
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                if (has_ym) V::ld(vb[k], I(zo + hy_m), h); else zero(h);
+                lds_put<VX>(&sY[1 + k][0][lx], h);
+            }
+        }
+        __syncthreads();
+
+        DualPlane<VX> Pn;
+        if (PF && in && z + 1 < c.z1) load_plane(z + 1, Pn);
+
+        if (in) {
+            float uyn[VX], vym[3][VX], vxm[3];
+            lds_get<VX>(&sY[0][c.ty + 2][lx], uyn);
+            const float uxn = sXp[c.ty][c.tx + 1];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                lds_get<VX>(&sY[1 + k][c.ty][lx], vym[k]);
+                vxm[k] = sXm[k][c.ty][c.tx];
+            }
+            float po[3][VX], qo[6][VX];
+#pragma unroll
+            for (int e = 0; e < VX; ++e) {
+                const int x = c.xb + e;
+                const float ub = ub_c[e];
+                const float dx = fwd(ub, e < VX - 1 ? ub_c[e + 1 < VX ? e + 1 : e] : uxn, x < W - 1);
+                const float dy = fwd(ub, uyn[e], c.y < H - 1);
+                const float dz = fwd(ub, P.ubn[e], z < D - 1);
+                const float ax = fmaf(sigma, dx - P.vb[0][e], P.p[0][e]);
+                const float ay = fmaf(sigma, dy - P.vb[1][e], P.p[1][e]);
+                const float az = fmaf(sigma, dz - P.vb[2][e], P.p[2][e]);
+                float n2 = ax * ax;
+                n2 = fmaf(ay, ay, n2);
+                n2 = fmaf(az, az, n2);
+                // p / max(1, |p|/alpha1): no division when the max is 1 (x / 1 == x)
+                const float tp = sqrtf(n2) / alpha1;
+                if (tp > 1.0f) {
+                    po[0][e] = ax / tp; po[1][e] = ay / tp; po[2][e] = az / tp;
+                } else {
+                    po[0][e] = ax; po[1][e] = ay; po[2][e] = az;
+                }
+
+                float bxv[3], byv[3], bzv[3];
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    const float vc = P.vb[k][e];
+                    bxv[k] = bwd(vc, e > 0 ? P.vb[k][e > 0 ? e - 1 : 0] : vxm[k], x > 0);
+                    byv[k] = bwd(vc, vym[k][e], c.y > 0);
+                    bzv[k] = bwd(vc, vb_p[k][e], z > 0);
+                }
+                const float exy = (bxv[1] + byv[0]) * 0.5f;
+                const float exz = (bxv[2] + bzv[0]) * 0.5f;
+                const float eyz = (byv[2] + bzv[1]) * 0.5f;
+                const float bxx = fmaf(sigma, bxv[0], P.q[0][e]);
+                const float byy = fmaf(sigma, byv[1], P.q[1][e]);
+                const float bxy = fmaf(sigma, exy, P.q[3][e]);
+                const float bzz = fmaf(sigma, bzv[2], P.q[2][e]);
+                const float bxz = fmaf(sigma, exz, P.q[4][e]);
+                const float byz = fmaf(sigma, eyz, P.q[5][e]);
+                float m2 = bxx * bxx;
+                m2 = fmaf(byy, byy, m2);
+                m2 = fmaf(2.0f * bxy, bxy, m2);
+                m2 = fmaf(bzz, bzz, m2);
+                m2 = fmaf(2.0f * bxz, bxz, m2);
+                m2 = fmaf(2.0f * byz, byz, m2);
+                const float tq = sqrtf(m2) / alpha0;
+                if (tq > 1.0f) {
+                    qo[0][e] = bxx / tq; qo[1][e] = byy / tq; qo[2][e] = bzz / tq;
+                    qo[3][e] = bxy / tq; qo[4][e] = bxz / tq; qo[5][e] = byz / tq;
+                } else {
+                    qo[0][e] = bxx; qo[1][e] = byy; qo[2][e] = bzz;
+                    qo[3][e] = bxy; qo[4][e] = bxz; qo[5][e] = byz;
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < 3; ++k) V::st(pf[k], i, po[k]);
+#pragma unroll
+            for (int k = 0; k < 6; ++k) V::st(qf[k], i, qo[k]);
+        }
+        __syncthreads();
+        copyv(ub_c, P.ubn);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) copyv(vb_p[k], P.vb[k]);
+        if (PF) P = Pn;
+    }
+}
+
+template <int VX>
+struct PrimalPlane {
+    float px[VX], py[VX], pz[VX], qxx[VX], qxy[VX], qyy[VX], u[VX], f[VX], v[3][VX];
+    float qzzn[VX], qxzn[VX], qyzn[VX];     // at z + 1
+};
+
+template <class S, class I, int TX, int TY, int VX, int PF, int L2>
+__global__ __launch_bounds__(TX * TY) void primal_sweep_vec_kernel(
+    typename S::T* __restrict__ u_, typename S::T* __restrict__ ubar_,
+    typename S::T* __restrict__ vx_, typename S::T* __restrict__ vy_,
+    typename S::T* __restrict__ vz_, typename S::T* __restrict__ vbx_,
+    typename S::T* __restrict__ vby_, typename S::T* __restrict__ vbz_,
+    const typename S::T* __restrict__ px_, const typename S::T* __restrict__ py_,
+    const typename S::T* __restrict__ pz_, const typename S::T* __restrict__ qxx_,
+    const typename S::T* __restrict__ qyy_, const typename S::T* __restrict__ qzz_,
+    const typename S::T* __restrict__ qxy_, const typename S::T* __restrict__ qxz_,
+    const typename S::T* __restrict__ qyz_, const float* __restrict__ f_, Geometry g,
+    float tau, float lambda, float theta, int zchunk) {
+    typedef VecIO<S, VX> V;
+    typedef VecIO<StoreF32, VX> VF;
+    constexpr int RW = TX * VX;
+    // y rows: 0 p_y (-y halo row 0), 1 q_xy, 2 q_yy, 3 q_yz (+y halo row TY+1)
+    __shared__ __align__(16) float sY[4][TY + 2][RW];
+    __shared__ float sXp[3][TY][TX + 1];    // q_xx, q_xy, q_xz element 0; [ty][TX] = +x halo
+    __shared__ float sXm[TY][TX + 1];       // p_x element VX-1 at [ty][tx+1]; [ty][0] = -x halo
+
+    const TileCoordV<TX, TY, VX> c(g, zchunk);
+    const int W = g.w, H = g.h, D = g.d;
+    const I plane = I(W) * I(H);
+    const bool in = (c.xb < W) && (c.y < H);
+    const I col = in ? I(c.y) * I(W) + I(c.xb) : I(0);
+    const bool hx = (c.tx == 0) && (c.y < H);
+    const bool hy = (c.ty == 0) && (c.xb < W);
+    const bool has_xp = c.x0 + RW < W, has_xm = c.x0 > 0;
+    const bool has_yp = c.y0 + TY < H, has_ym = c.y0 > 0;
+    const I hx_p = has_xp && hx ? I(c.y) * I(W) + I(c.x0 + RW) : I(0);
+    const I hx_m = has_xm && hx ? I(c.y) * I(W) + I(c.x0 - 1) : I(0);
+    const I hy_p = has_yp && hy ? I(c.y0 + TY) * I(W) + I(c.xb) : I(0);
+    const I hy_m = has_ym && hy ? I(c.y0 - 1) * I(W) + I(c.xb) : I(0);
+    const bool use_bz = !L2 || D > 1;
+    const float tl = tau * lambda;
+    const int lx = c.tx * VX;
+
+    auto load_plane = [&](int zz, PrimalPlane<VX>& P) {
+        const I i = I(zz) * plane + col;
+        V::ld(px_, i, P.px); V::ld(py_, i, P.py); V::ld(pz_, i, P.pz);
+        V::ld(qxx_, i, P.qxx); V::ld(qxy_, i, P.qxy); V::ld(qyy_, i, P.qyy);
+        V::ld(u_, i, P.u); VF::ld(f_, i, P.f);
+        V::ld(vx_, i, P.v[0]); V::ld(vy_, i, P.v[1]); V::ld(vz_, i, P.v[2]);
+        if (zz + 1 < D) {
+            const I j = i + plane;
+            V::ld(qzz_, j, P.qzzn); V::ld(qxz_, j, P.qxzn); V::ld(qyz_, j, P.qyzn);
+        } else {
+            zero(P.qzzn); zero(P.qxzn); zero(P.qyzn);
+        }
+    };
+
+    float pz_p[VX], qzz_c[VX], qxz_c[VX], qyz_c[VX];
+    PrimalPlane<VX> P;
+    zero(pz_p); zero(qzz_c); zero(qxz_c); zero(qyz_c);
+    if (in) {
+        const I i0 = I(c.z0) * plane + col;
+        if (c.z0 > 0) V::ld(pz_, I(i0 - plane), pz_p);
+        V::ld(qzz_, i0, qzz_c); V::ld(qxz_, i0, qxz_c); V::ld(qyz_, i0, qyz_c);
+        if (PF) load_plane(c.z0, P);
+    }
+
+    for (int z = c.z0; z < c.z1; ++z) {
+        const I zo = I(z) * plane;
+        const I i = zo + col;
+        if (!PF && in) load_plane(z, P);
+        lds_put<VX>(&sY[0][c.ty + 1][lx], P.py);
+        lds_put<VX>(&sY[1][c.ty + 1][lx], P.qxy);
+        lds_put<VX>(&sY[2][c.ty + 1][lx], P.qyy);
+        lds_put<VX>(&sY[3][c.ty + 1][lx], qyz_c);
+        sXp[0][c.ty][c.tx] = P.qxx[0];
+        sXp[1][c.ty][c.tx] = P.qxy[0];
+        sXp[2][c.ty][c.tx] = qxz_c[0];
+        sXm[c.ty][c.tx + 1] = P.px[VX - 1];
+        if (hx) {
+            sXm[c.ty][0] = has_xm ? S::ld(px_, I(zo + hx_m)) : 0.f;
+            sXp[0][c.ty][TX] = has_xp ? S::ld(qxx_, I(zo + hx_p)) : 0.f;
+            sXp[1][c.ty][TX] = has_xp ? S::ld(qxy_, I(zo + hx_p)) : 0.f;
+            sXp[2][c.ty][TX] = has_xp ? S::ld(qxz_, I(zo + hx_p)) : 0.f;
+        }
+        if (hy) {
+            float h[VX];
+            if (has_ym) V::ld(py_, I(zo + hy_m), h); else zero(h);
+            lds_put<VX>(&sY[0][0][lx], h);
+            if (has_yp) V::ld(qxy_, I(zo + hy_p), h); else zero(h);
+            lds_put<VX>(&sY[1][TY + 1][lx], h);
+            if (has_yp) V::ld(qyy_, I(zo + hy_p), h); else zero(h);
+            lds_put<VX>(&sY[2][TY + 1][lx], h);
+            if (has_yp) V::ld(qyz_, I(zo + hy_p), h); else zero(h);
+            lds_put<VX>(&sY[3][TY + 1][lx], h);
+        }
+        __syncthreads();
+
+        PrimalPlane<VX> Pn;
+        if (PF && in && z + 1 < c.z1) load_plane(z + 1, Pn);
+
+        if (in) {
+            float pym[VX], qxyn_y[VX], qyyn_y[VX], qyzn_y[VX];
+            lds_get<VX>(&sY[0][c.ty][lx], pym);
+            lds_get<VX>(&sY[1][c.ty + 2][lx], qxyn_y);
+            lds_get<VX>(&sY[2][c.ty + 2][lx], qyyn_y);
+            lds_get<VX>(&sY[3][c.ty + 2][lx], qyzn_y);
+            const float pxm = sXm[c.ty][c.tx];
+            const float qxxn_x = sXp[0][c.ty][c.tx + 1];
+            const float qxyn_x = sXp[1][c.ty][c.tx + 1];
+            const float qxzn_x = sXp[2][c.ty][c.tx + 1];
+            float uo[VX], ubo[VX], vo[3][VX], vbo[3][VX];
+#pragma unroll
+            for (int e = 0; e < VX; ++e) {
+                const int x = c.xb + e;
+                const int en = e + 1 < VX ? e + 1 : e, ep = e > 0 ? e - 1 : 0;
+                const bool xl = e < VX - 1;
+                const float bx = bwd(P.px[e], e > 0 ? P.px[ep] : pxm, x > 0);
+                const float by = bwd(P.py[e], pym[e], c.y > 0);
+                const float bz = bwd(P.pz[e], pz_p[e], z > 0);
+                const float div = use_bz ? (bx + by) + bz : bx + by;
+                const float uc = P.u[e], fc = P.f[e];
+                float un = fmaf(-tau, div, uc);
+                if (L2) {
+                    un = fmaf(tl, fc, un) / (1.0f + tl);
+                } else {
+                    const float umf = un - fc;
+                    if (umf > tl) un = un - tl;
+                    else if (umf < -tl) un = un + tl;
+                    else un = fc;
+                }
+                uo[e] = un;
+                ubo[e] = fmaf(theta, un - uc, un);
+                const bool xn = x < W - 1, yn = c.y < H - 1, zn = z < D - 1;
+                const float dxx = fwd(P.qxx[e], xl ? P.qxx[en] : qxxn_x, xn);
+                const float dyxy = fwd(P.qxy[e], qxyn_y[e], yn);
+                const float dzxz = fwd(qxz_c[e], P.qxzn[e], zn);
+                const float dyy = fwd(P.qyy[e], qyyn_y[e], yn);
+                const float dxxy = fwd(P.qxy[e], xl ? P.qxy[en] : qxyn_x, xn);
+                const float dzyz = fwd(qyz_c[e], P.qyzn[e], zn);
+                const float dzz = fwd(qzz_c[e], P.qzzn[e], zn);
+                const float dxxz = fwd(qxz_c[e], xl ? qxz_c[en] : qxzn_x, xn);
+                const float dyyz = fwd(qyz_c[e], qyzn_y[e], yn);
+                const float w0 = (dxx + dyxy) + dzxz;
+                const float w1 = (dyy + dxxy) + dzyz;
+                const float w2 = (dzz + dxxz) + dyyz;
+                const float vn0 = fmaf(-tau, w0 - P.px[e], P.v[0][e]);
+                const float vn1 = fmaf(-tau, w1 - P.py[e], P.v[1][e]);
+                const float vn2 = fmaf(-tau, w2 - P.pz[e], P.v[2][e]);
+                vo[0][e] = vn0; vo[1][e] = vn1; vo[2][e] = vn2;
+                vbo[0][e] = fmaf(theta, vn0 - P.v[0][e], vn0);
+                vbo[1][e] = fmaf(theta, vn1 - P.v[1][e], vn1);
+                vbo[2][e] = fmaf(theta, vn2 - P.v[2][e], vn2);
+            }
+            V::st(u_, i, uo);
+            V::st(ubar_, i, ubo);
+            V::st(vx_, i, vo[0]); V::st(vy_, i, vo[1]); V::st(vz_, i, vo[2]);
+            V::st(vbx_, i, vbo[0]); V::st(vby_, i, vbo[1]); V::st(vbz_, i, vbo[2]);
+        }
+        __syncthreads();
+        copyv(pz_p, P.pz);
+        copyv(qzz_c, P.qzzn); copyv(qxz_c, P.qxzn); copyv(qyz_c, P.qyzn);
+        if (PF) P = Pn;
+    }
+}
+
 template <class S>
-__global__ void init_kernel(typename S::T* const* fields_unused, typename S::T* u,
-                            typename S::T* ubar, typename S::T* zero_base, size_t zero_count,
-                            const float* f, size_t n) {
-    (void)fields_unused;
+__global__ void init_kernel(typename S::T* u, typename S::T* ubar, typename S::T* zero_base,
+                            size_t zero_count, const float* f, size_t n) {
     const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
     const size_t stride = size_t(gridDim.x) * blockDim.x;
     for (size_t k = i; k < n; k += stride) {
@@ -390,53 +805,150 @@ __global__ void convert_kernel(const typename SI::T* src, typename SO::T* dst, s
         SO::st(dst, k, SI::ld(src, k));
 }
 
-inline int grid_blocks(const Geometry& g, int zchunk) {
-    const int nbx = (g.w + TX - 1) / TX, nby = (g.h + TY - 1) / TY;
+template <int TX, int TY, int VX = 1>
+int grid_blocks(const Geometry& g, int zchunk) {
+    const int nbx = (g.w + TX * VX - 1) / (TX * VX), nby = (g.h + TY - 1) / TY;
     const int nbz = (g.d + zchunk - 1) / zchunk;
     return nbx * nby * nbz;
 }
 
+// Next-plane prefetch of the vector sweeps: on for FP32 fields, off for BF16
+// (measured on MI355X at 256^3); AMB_SWEEP_PF=0/1 overrides for tuning.
+// Both variants compute identical results.
 template <class S>
-hipError_t dual_impl(hipStream_t s, void* const* fv, Geometry g, const SweepParams& p) {
+inline int sweep_prefetch() {
+    const char* e = std::getenv("AMB_SWEEP_PF");
+    if (e && (e[0] == '0' || e[0] == '1')) return e[0] == '1';
+    return sizeof(typename S::T) == 4 ? 1 : 0;
+}
+
+template <class S, class I, int TX, int TY, int PF>
+hipError_t dual_vec_launch(hipStream_t s, void* const* fv, Geometry g, const SweepParams& p) {
     typedef typename S::T T;
     T* const* F = reinterpret_cast<T* const*>(fv);
-    hipLaunchKernelGGL(dual_sweep_kernel<S>, dim3(grid_blocks(g, p.zchunk)), dim3(NT), 0, s,
-                       F[F_UBAR], F[F_VBX], F[F_VBY], F[F_VBZ], F[F_PX], F[F_PY], F[F_PZ],
-                       F[F_QXX], F[F_QYY], F[F_QZZ], F[F_QXY], F[F_QXZ], F[F_QYZ], g, p.sigma,
-                       p.alpha0, p.alpha1, p.zchunk);
+    hipLaunchKernelGGL((dual_sweep_vec_kernel<S, I, TX, TY, 4, PF>),
+                       dim3(grid_blocks<TX, TY, 4>(g, p.zchunk)), dim3(TX * TY), 0, s, F[F_UBAR],
+                       F[F_VBX], F[F_VBY], F[F_VBZ], F[F_PX], F[F_PY], F[F_PZ], F[F_QXX],
+                       F[F_QYY], F[F_QZZ], F[F_QXY], F[F_QXZ], F[F_QYZ], g, p.sigma, p.alpha0,
+                       p.alpha1, p.zchunk);
     return hipGetLastError();
 }
 
-template <class S>
-hipError_t primal_impl(hipStream_t s, void* const* fv, const float* f, Geometry g,
-                       const SweepParams& p) {
+template <class S, class I, int TX, int TY, int PF>
+hipError_t primal_vec_launch(hipStream_t s, void* const* fv, const float* f, Geometry g,
+                             const SweepParams& p) {
     typedef typename S::T T;
     T* const* F = reinterpret_cast<T* const*>(fv);
-    const dim3 grid(grid_blocks(g, p.zchunk)), block(NT);
+    const dim3 grid(grid_blocks<TX, TY, 4>(g, p.zchunk)), block(TX * TY);
     if (p.data_term == 1)
-        hipLaunchKernelGGL((primal_sweep_kernel<S, 1>), grid, block, 0, s, F[F_U], F[F_UBAR],
-                           F[F_VX], F[F_VY], F[F_VZ], F[F_VBX], F[F_VBY], F[F_VBZ], F[F_PX],
-                           F[F_PY], F[F_PZ], F[F_QXX], F[F_QYY], F[F_QZZ], F[F_QXY], F[F_QXZ],
-                           F[F_QYZ], f, g, p.tau, p.lambda, p.theta, p.zchunk);
+        hipLaunchKernelGGL((primal_sweep_vec_kernel<S, I, TX, TY, 4, PF, 1>), grid, block, 0, s,
+                           F[F_U], F[F_UBAR], F[F_VX], F[F_VY], F[F_VZ], F[F_VBX], F[F_VBY],
+                           F[F_VBZ], F[F_PX], F[F_PY], F[F_PZ], F[F_QXX], F[F_QYY], F[F_QZZ],
+                           F[F_QXY], F[F_QXZ], F[F_QYZ], f, g, p.tau, p.lambda, p.theta,
+                           p.zchunk);
     else
-        hipLaunchKernelGGL((primal_sweep_kernel<S, 0>), grid, block, 0, s, F[F_U], F[F_UBAR],
-                           F[F_VX], F[F_VY], F[F_VZ], F[F_VBX], F[F_VBY], F[F_VBZ], F[F_PX],
-                           F[F_PY], F[F_PZ], F[F_QXX], F[F_QYY], F[F_QZZ], F[F_QXY], F[F_QXZ],
-                           F[F_QYZ], f, g, p.tau, p.lambda, p.theta, p.zchunk);
+        hipLaunchKernelGGL((primal_sweep_vec_kernel<S, I, TX, TY, 4, PF, 0>), grid, block, 0, s,
+                           F[F_U], F[F_UBAR], F[F_VX], F[F_VY], F[F_VZ], F[F_VBX], F[F_VBY],
+                           F[F_VBZ], F[F_PX], F[F_PY], F[F_PZ], F[F_QXX], F[F_QYY], F[F_QZZ],
+                           F[F_QXY], F[F_QXZ], F[F_QYZ], f, g, p.tau, p.lambda, p.theta,
+                           p.zchunk);
     return hipGetLastError();
+}
+
+template <class S, class I, int TX, int TY>
+hipError_t dual_vec_pf(hipStream_t s, void* const* fv, Geometry g, const SweepParams& p) {
+    return sweep_prefetch<S>() ? dual_vec_launch<S, I, TX, TY, 1>(s, fv, g, p)
+                            : dual_vec_launch<S, I, TX, TY, 0>(s, fv, g, p);
+}
+template <class S, class I, int TX, int TY>
+hipError_t primal_vec_pf(hipStream_t s, void* const* fv, const float* f, Geometry g,
+                         const SweepParams& p) {
+    return sweep_prefetch<S>() ? primal_vec_launch<S, I, TX, TY, 1>(s, fv, f, g, p)
+                            : primal_vec_launch<S, I, TX, TY, 0>(s, fv, f, g, p);
+}
+
+// 32-bit offsets when every byte offset of a 4-byte field fits in 32 bits.
+inline bool small_index(const Geometry& g) { return g.n() * 4 < (size_t(1) << 32); }
+
+template <class S, class I, int TX, int TY>
+hipError_t dual_launch(hipStream_t s, void* const* fv, Geometry g, const SweepParams& p) {
+    typedef typename S::T T;
+    T* const* F = reinterpret_cast<T* const*>(fv);
+    hipLaunchKernelGGL((dual_sweep_kernel<S, I, TX, TY>), dim3(grid_blocks<TX, TY>(g, p.zchunk)),
+                       dim3(TX * TY), 0, s, F[F_UBAR], F[F_VBX], F[F_VBY], F[F_VBZ], F[F_PX],
+                       F[F_PY], F[F_PZ], F[F_QXX], F[F_QYY], F[F_QZZ], F[F_QXY], F[F_QXZ],
+                       F[F_QYZ], g, p.sigma, p.alpha0, p.alpha1, p.zchunk);
+    return hipGetLastError();
+}
+
+template <class S, class I, int TX, int TY>
+hipError_t primal_launch(hipStream_t s, void* const* fv, const float* f, Geometry g,
+                         const SweepParams& p) {
+    typedef typename S::T T;
+    T* const* F = reinterpret_cast<T* const*>(fv);
+    const dim3 grid(grid_blocks<TX, TY>(g, p.zchunk)), block(TX * TY);
+    if (p.data_term == 1)
+        hipLaunchKernelGGL((primal_sweep_kernel<S, I, TX, TY, 1>), grid, block, 0, s, F[F_U],
+                           F[F_UBAR], F[F_VX], F[F_VY], F[F_VZ], F[F_VBX], F[F_VBY], F[F_VBZ],
+                           F[F_PX], F[F_PY], F[F_PZ], F[F_QXX], F[F_QYY], F[F_QZZ], F[F_QXY],
+                           F[F_QXZ], F[F_QYZ], f, g, p.tau, p.lambda, p.theta, p.zchunk);
+    else
+        hipLaunchKernelGGL((primal_sweep_kernel<S, I, TX, TY, 0>), grid, block, 0, s, F[F_U],
+                           F[F_UBAR], F[F_VX], F[F_VY], F[F_VZ], F[F_VBX], F[F_VBY], F[F_VBZ],
+                           F[F_PX], F[F_PY], F[F_PZ], F[F_QXX], F[F_QYY], F[F_QZZ], F[F_QXY],
+                           F[F_QXZ], F[F_QYZ], f, g, p.tau, p.lambda, p.theta, p.zchunk);
+    return hipGetLastError();
+}
+
+template <class S, class I>
+hipError_t dual_tiles(hipStream_t s, void* const* fv, Geometry g, const SweepParams& p) {
+    switch (p.tile) {
+        case 4: return dual_vec_pf<S, I, 64, 4>(s, fv, g, p);
+        case 5: return dual_vec_pf<S, I, 64, 2>(s, fv, g, p);
+        case 6: return dual_vec_pf<S, I, 64, 8>(s, fv, g, p);
+        case 7: return dual_vec_pf<S, I, 32, 8>(s, fv, g, p);
+        case 1: return dual_launch<S, I, 64, 8>(s, fv, g, p);
+        case 2: return dual_launch<S, I, 32, 8>(s, fv, g, p);
+        case 3: return dual_launch<S, I, 64, 2>(s, fv, g, p);
+        default: return dual_launch<S, I, 64, 4>(s, fv, g, p);
+    }
+}
+
+template <class S, class I>
+hipError_t primal_tiles(hipStream_t s, void* const* fv, const float* f, Geometry g,
+                        const SweepParams& p) {
+    switch (p.tile) {
+        case 4: return primal_vec_pf<S, I, 64, 4>(s, fv, f, g, p);
+        case 5: return primal_vec_pf<S, I, 64, 2>(s, fv, f, g, p);
+        case 6: return primal_vec_pf<S, I, 64, 8>(s, fv, f, g, p);
+        case 7: return primal_vec_pf<S, I, 32, 8>(s, fv, f, g, p);
+        case 1: return primal_launch<S, I, 64, 8>(s, fv, f, g, p);
+        case 2: return primal_launch<S, I, 32, 8>(s, fv, f, g, p);
+        case 3: return primal_launch<S, I, 64, 2>(s, fv, f, g, p);
+        default: return primal_launch<S, I, 64, 4>(s, fv, f, g, p);
+    }
 }
 
 }  // namespace
 
 hipError_t launch_dual_sweep(hipStream_t s, int storage, void* const* fields, Geometry g,
                              const SweepParams& p) {
-    return storage == 1 ? dual_impl<StoreBF16>(s, fields, g, p) : dual_impl<StoreF32>(s, fields, g, p);
+    const bool small = small_index(g);
+    if (storage == 1)
+        return small ? dual_tiles<StoreBF16, uint32_t>(s, fields, g, p)
+                     : dual_tiles<StoreBF16, size_t>(s, fields, g, p);
+    return small ? dual_tiles<StoreF32, uint32_t>(s, fields, g, p)
+                 : dual_tiles<StoreF32, size_t>(s, fields, g, p);
 }
 
 hipError_t launch_primal_sweep(hipStream_t s, int storage, void* const* fields, const float* f,
                                Geometry g, const SweepParams& p) {
-    return storage == 1 ? primal_impl<StoreBF16>(s, fields, f, g, p)
-                        : primal_impl<StoreF32>(s, fields, f, g, p);
+    const bool small = small_index(g);
+    if (storage == 1)
+        return small ? primal_tiles<StoreBF16, uint32_t>(s, fields, f, g, p)
+                     : primal_tiles<StoreBF16, size_t>(s, fields, f, g, p);
+    return small ? primal_tiles<StoreF32, uint32_t>(s, fields, f, g, p)
+                 : primal_tiles<StoreF32, size_t>(s, fields, f, g, p);
 }
 
 // Fields are allocated as one block: field k starts at fields[0] + k * stride.
@@ -446,15 +958,15 @@ hipError_t launch_init(hipStream_t s, int storage, void* const* fields, const fl
     const size_t n = g.n();
     const char* base = static_cast<const char*>(fields[F_VX]);
     const char* last = static_cast<const char*>(fields[F_QYZ]);
-    const size_t elem = storage == 1 ? 2 : 4;
-    const size_t zero_count = size_t(last - base) / elem + n;
+    const size_t elem_bytes = storage == 1 ? 2 : 4;
+    const size_t zero_count = size_t(last - base) / elem_bytes + n;
     const int blocks = 2048;
     if (storage == 1)
-        hipLaunchKernelGGL(init_kernel<StoreBF16>, dim3(blocks), dim3(256), 0, s, nullptr,
+        hipLaunchKernelGGL(init_kernel<StoreBF16>, dim3(blocks), dim3(256), 0, s,
                            static_cast<uint16_t*>(fields[F_U]), static_cast<uint16_t*>(fields[F_UBAR]),
                            static_cast<uint16_t*>(fields[F_VX]), zero_count, f, n);
     else
-        hipLaunchKernelGGL(init_kernel<StoreF32>, dim3(blocks), dim3(256), 0, s, nullptr,
+        hipLaunchKernelGGL(init_kernel<StoreF32>, dim3(blocks), dim3(256), 0, s,
                            static_cast<float*>(fields[F_U]), static_cast<float*>(fields[F_UBAR]),
                            static_cast<float*>(fields[F_VX]), zero_count, f, n);
     return hipGetLastError();

@@ -70,15 +70,18 @@ def cosine_basis(n: int, k: int) -> np.ndarray:
 
 
 def separable_modes(shape, coeffs: dict) -> np.ndarray:
-    """sum_{(i,j,k)} a_ijk c_i(x) c_j(y) c_k(z) as float64 array (D, H, W)."""
+    """sum_{(i,j,k)} a_ijk c_i(x) c_j(y) c_k(z) as float64 array (D, H, W),
+    evaluated as one (D*H, M) x (M, W) matrix product over the M modes."""
     d, h, w = shape
-    out = np.zeros((d, h, w), dtype=np.float64)
-    for (i, j, k), a in coeffs.items():
-        cx = cosine_basis(w, i)
-        cy = cosine_basis(h, j)
-        cz = cosine_basis(d, k)
-        out += a * cz[:, None, None] * (cy[:, None] * cx[None, :])[None, :, :]
-    return out
+    if not coeffs:
+        return np.zeros((d, h, w), dtype=np.float64)
+    keys = list(coeffs.keys())
+    a = np.array([coeffs[k] for k in keys], dtype=np.float64)
+    cx = np.stack([cosine_basis(w, i) for (i, _, _) in keys], axis=1)        # (W, M)
+    cy = np.stack([cosine_basis(h, j) for (_, j, _) in keys], axis=1)        # (H, M)
+    cz = np.stack([cosine_basis(d, k) for (_, _, k) in keys], axis=1)        # (D, M)
+    t = (cz[:, None, :] * cy[None, :, :] * a[None, None, :]).reshape(d * h, len(keys))
+    return (t @ cx.T).reshape(d, h, w)
 
 
 def _boxes(rng: SplitMix64, shape, count, lo_max, size_range, inten, background):

@@ -39,20 +39,33 @@ def _random_state(shape, seed):
     return f, st
 
 
-SHAPES = [(11, 19, 37), (1, 20, 70), (9, 8, 64), (6, 5, 130)]
+# (D, H, W): odd widths use the one-voxel-per-thread tiles, widths that are a
+# multiple of 4 the vector tiles (partial tiles in x and y included).
+SHAPES = [(11, 19, 37), (1, 20, 70), (9, 8, 64), (6, 5, 130), (7, 9, 132), (5, 13, 260),
+          (3, 6, 8)]
+# tile ids (tile_shape in csrc/amb_internal.hpp) x next-plane prefetch
+VARIANTS = [("auto", "1"), ("auto", "0"), ("0", "1"), ("1", "1"), ("5", "1"), ("6", "0"),
+            ("7", "1")]
 
 
 @pytest.mark.parametrize("shape", SHAPES)
 @pytest.mark.parametrize("data_term", ["l1", "l2"])
 @pytest.mark.parametrize("zchunk", ["auto", "3"])
-def test_one_iteration_from_random_state(ctx, orc, shape, data_term, zchunk, monkeypatch):
+@pytest.mark.parametrize("variant", VARIANTS)
+def test_one_iteration_from_random_state(ctx, orc, shape, data_term, zchunk, variant,
+                                         monkeypatch):
     """Both sweeps on an arbitrary state (odd sizes, tile and z-chunk edges)."""
     from ambient_image_processor_amd import native
-    _say("case", shape, data_term, zchunk)
+    _say("case", shape, data_term, zchunk, variant)
     if zchunk != "auto":
         monkeypatch.setenv("AMB_ZCHUNK", zchunk)
     else:
         monkeypatch.delenv("AMB_ZCHUNK", raising=False)
+    if variant[0] != "auto":
+        monkeypatch.setenv("AMB_TILE", variant[0])
+    else:
+        monkeypatch.delenv("AMB_TILE", raising=False)
+    monkeypatch.setenv("AMB_SWEEP_PF", variant[1])
     f, st = _random_state(shape, seed=sum(shape))
     p = _params(native, PRM12, 1, data_term)
     ctx.begin(f, p)
@@ -270,3 +283,45 @@ def test_reference_named_launchers(orc):
     assert rel_err(u2, ref2.u) < 1e-6
     X = processor.cosine_transform_kernel_launch(f, w, h, d)
     assert rel_err(X, orc.cosine_transform(f, np.float32)) < 1e-6
+
+
+# ---------------------------------------------------------------------------
+# MFMA DCT passes (F32 and BF16x3 split precision)
+# ---------------------------------------------------------------------------
+MFMA_SHAPES = [(16, 16, 16), (5, 6, 8), (20, 24, 32), (7, 1, 9), (33, 17, 70), (64, 64, 64),
+               (40, 130, 96)]
+
+
+@pytest.mark.parametrize("shape", MFMA_SHAPES)
+@pytest.mark.parametrize("precision,tol", [(1, 3e-6), (2, 4e-5)])
+def test_dct_mfma_against_fp64(ctx, orc, shape, precision, tol):
+    rng = np.random.default_rng(11)
+    x = rng.standard_normal(shape).astype(np.float32)
+    X = ctx.cosine_transform(x, precision)
+    Y = ctx.inverse_cosine_transform(x, precision)
+    X64 = orc.cosine_transform(x.astype(np.float64))
+    Y64 = orc.inverse_cosine_transform(x.astype(np.float64))
+    ex, ey = rel_err(X, X64), rel_err(Y, Y64)
+    _say(f"MFMA DCT precision {precision} {shape}: fwd {ex:.2e} inv {ey:.2e}")
+    assert ex < tol and ey < tol
+
+
+@pytest.mark.parametrize("precision,tol", [(1, 1e-5), (2, 1e-4)])
+def test_poisson_solve_mfma_known_answer(ctx, orc, precision, tol):
+    """RHS = Neumann Laplacian of 64 cosine modes: the solve returns the field
+    (fused eigenvalue divide in the forward z pass)."""
+    from ambient_image_processor_amd import phantoms
+    rhs, phi = phantoms.poisson_field((48, 80, 96), seed=3, modes=64, kmax=15)
+    # integrate_gradients with v = (D+ phi) reproduces rhs via the B-divergence
+    v = [orc.forward_difference(phi, a).astype(np.float32) for a in range(3)]
+    l = ctx.integrate_gradients_poisson(*v, precision=precision)
+    e = rel_err(l, phi - phi.mean())
+    _say(f"MFMA Poisson precision {precision}: rel err vs closed form {e:.2e}")
+    assert e < tol
+
+
+def test_mfma_extent_limit_is_an_error(ctx):
+    from ambient_image_processor_amd import native
+    x = np.zeros((2, 2, 520), dtype=np.float32)
+    with pytest.raises(native.AmbError):
+        ctx.cosine_transform(x, native.DCT_BF16X3)
