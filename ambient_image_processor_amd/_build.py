@@ -56,9 +56,26 @@ def build(verbose: bool = False) -> str:
     if r.returncode != 0:
         raise RuntimeError("link failed:\n" + r.stdout + r.stderr)
     os.replace(tmp, LIB_PATH)
+    _build_compat_demo()
     if verbose:
         print("built", LIB_PATH)
     return LIB_PATH
+
+
+COMPAT_DEMO = os.path.join(BUILD_DIR, "processor_demo")
+
+
+def _build_compat_demo() -> str:
+    """A C++ caller that uses only the reference's launcher templates (through
+    include/ambient_amd_reference_compat.hpp), linked against the library."""
+    src = os.path.join(ROOT, "tests", "compat", "processor_demo.cpp")
+    cmd = ["g++", "-O2", "-std=c++17", "-I" + os.path.join(ROOT, "include"), src, "-o",
+           COMPAT_DEMO, "-L" + PKG, "-lambient_amd", "-Wl,-rpath," + PKG,
+           "-Wl,-rpath,$ORIGIN/../ambient_image_processor_amd", "-Wl,-rpath,/opt/rocm/lib"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("compat demo build failed:\n" + r.stdout + r.stderr)
+    return COMPAT_DEMO
 
 
 if __name__ == "__main__":

@@ -325,3 +325,35 @@ def test_mfma_extent_limit_is_an_error(ctx):
     x = np.zeros((2, 2, 520), dtype=np.float32)
     with pytest.raises(native.AmbError):
         ctx.cosine_transform(x, native.DCT_BF16X3)
+
+
+def test_reference_launcher_caller_links_unchanged(ctx, orc, tmp_path):
+    """tests/compat/processor_demo.cpp composes the reference's processor from the
+    launcher templates of include/ambient_amd_reference_compat.hpp; its outputs
+    equal the oracle's and the device-resident pipeline's."""
+    import subprocess
+    from ambient_image_processor_amd import _build, native, phantoms
+    assert os.path.exists(_build.COMPAT_DEMO), "run __graft_entry__.build() first"
+    f = phantoms.deshade_phantom((12, 20, 24), seed=8)[0]
+    d, h, w = f.shape
+    f.tofile(tmp_path / "f.raw")
+    out = [str(tmp_path / n) for n in ("u.raw", "l.raw", "r.raw")]
+    _say("running", _build.COMPAT_DEMO)
+    r = subprocess.run([_build.COMPAT_DEMO, str(tmp_path / "f.raw"), str(w), str(h), str(d),
+                        "0.5", "2.0", "1.0", "60"] + out, capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    u, l, rr = (np.fromfile(p, dtype=np.float32).reshape(f.shape) for p in out)
+    tau = float(np.float32(1 / np.sqrt(20)))
+    st = orc.tgv2(f, 60, lam=0.5, alpha0=2.0, alpha1=1.0, tau=tau, sigma=tau,
+                  data_term="l1", dtype=np.float32)
+    lo = orc.integrate_poisson(*st.v, dtype=np.float32)
+    ro, _ = orc.deshade_epilogue(st.u, lo, None, False, True, False)
+    np.testing.assert_array_equal(u, st.u)
+    np.testing.assert_array_equal(l, lo)
+    np.testing.assert_array_equal(rr, ro)
+    p = native.default_params(lam=0.5, iterations=60)
+    u2, l2, r2 = ctx.deshade_pipeline(f, p)
+    np.testing.assert_array_equal(u2, u)
+    np.testing.assert_array_equal(l2, l)
+    np.testing.assert_array_equal(r2, rr)
