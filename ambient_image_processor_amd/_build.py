@@ -20,7 +20,7 @@ CSRC = os.path.join(PKG, "csrc")
 BUILD_DIR = os.path.join(ROOT, "_build")
 LIB_NAME = "libambient_amd.so"
 LIB_PATH = os.path.join(PKG, LIB_NAME)
-SOURCES = ["tgv_sweeps.hip", "dct.hip", "dct_mfma.hip", "energy.hip", "capi.hip"]
+SOURCES = ["tgv_sweeps.hip", "dct.hip", "dct_mfma.hip", "energy.hip", "adjoint.hip", "capi.hip"]
 ARCH = "gfx950"
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
@@ -57,6 +57,7 @@ def build(verbose: bool = False) -> str:
         raise RuntimeError("link failed:\n" + r.stdout + r.stderr)
     os.replace(tmp, LIB_PATH)
     _build_compat_demo()
+    _build_cli()
     if verbose:
         print("built", LIB_PATH)
     return LIB_PATH
@@ -76,6 +77,27 @@ def _build_compat_demo() -> str:
     if r.returncode != 0:
         raise RuntimeError("compat demo build failed:\n" + r.stdout + r.stderr)
     return COMPAT_DEMO
+
+
+CLI = os.path.join(BUILD_DIR, "tgv2_deshade_application")
+METAIMAGE_CONVERT = os.path.join(BUILD_DIR, "metaimage_convert")
+
+
+def _build_cli() -> None:
+    """The tgv2_deshade_application CLI (reference argument contract, MetaImage
+    files) and the MetaImage converter, both plain C++ (g++, zlib)."""
+    cli = os.path.join(PKG, "cli")
+    common = ["g++", "-O2", "-std=c++17", "-I" + os.path.join(ROOT, "include"), "-I" + cli]
+    jobs = [
+        common + [os.path.join(cli, "tgv2_deshade_main.cpp"), "-o", CLI, "-L" + PKG,
+                  "-lambient_amd", "-lz", "-Wl,-rpath," + PKG,
+                  "-Wl,-rpath,$ORIGIN/../ambient_image_processor_amd", "-Wl,-rpath,/opt/rocm/lib"],
+        common + [os.path.join(cli, "metaimage_convert.cpp"), "-o", METAIMAGE_CONVERT, "-lz"],
+    ]
+    for cmd in jobs:
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError("CLI build failed:\n" + r.stdout + r.stderr)
 
 
 if __name__ == "__main__":

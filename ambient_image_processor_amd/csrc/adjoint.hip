@@ -1,0 +1,292 @@
+// Hand-written adjoint of the unrolled TGV2-L2 iteration (no counterpart in the
+// reference).  The forward map is the loop of tgv_deshade/tgv2_l1_deshade.cu:169-264
+// with the L2 u-update of tgv_cuda/tgv2_l2.cu:27-56, exactly as the sweeps in
+// tgv_sweeps.hip compute it; the reverse map uses the true transposes of the
+// operators as implemented (D+, the reference's B, eps_B, div2).
+//
+// One reverse iteration k, given x_k (S), x_{k+1} (T) and the adjoint A of
+// x_{k+1} (17 fields, the state layout), in three launches:
+//   phase A (pointwise): u/v over-relaxation and L2 prox adjoints; writes the new
+//            u and v adjoints in place, A* = U'/(1 + tau lambda) and
+//            W* = -tau V' into scratch, adds tau V' into the p adjoint;
+//            FP64 partial of dL/dlambda = sum U' tau (f - u') / (1 + tau lambda).
+//   phase B (stencil): P' = A.p - tau B^T A*, Q' = A.q + div2^T W*; projection
+//            Jacobians at p~ = p + sigma (D+ u_bar - v_bar), q~ = q + sigma eps_B(v_bar)
+//            recomputed from S with the forward's FP32 arithmetic (same active
+//            set); writes the p/q adjoints; FP64 partials of dL/dalpha1, dL/dalpha0.
+//   phase C (stencil): u_bar adjoint = sigma (D+)^T P~, v_bar adjoint =
+//            sigma eps_B^T Q~ - sigma P~.
+// Scalar partials are reduced per block (wave64 shuffles) and summed in a fixed
+// order by a single-block kernel (deterministic).
+#include "amb_internal.hpp"
+
+namespace amb {
+namespace {
+
+constexpr int AB = 256;
+constexpr int MAXB = 2048;
+
+struct Fields17 {
+    float* f[N_FIELDS];
+};
+
+__device__ __forceinline__ double wsum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    return v;
+}
+
+template <int NQ>
+__device__ __forceinline__ void block_partials(double (&acc)[NQ], double* partials) {
+    __shared__ double red[AB / 64][NQ];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < NQ; ++k) {
+        const double s = wsum(acc[k]);
+        if (lane == 0) red[w][k] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x < NQ) {
+        double s = 0.0;
+        for (int i = 0; i < AB / 64; ++i) s += red[i][threadIdx.x];
+        partials[size_t(blockIdx.x) * 4 + threadIdx.x] = s;
+    }
+}
+
+struct Coord {
+    int x, y, z;
+};
+__device__ __forceinline__ Coord coord_of(size_t i, const Geometry& g) {
+    Coord c;
+    c.x = int(i % size_t(g.w));
+    c.y = int((i / size_t(g.w)) % size_t(g.h));
+    c.z = int(i / (size_t(g.w) * g.h));
+    return c;
+}
+
+// (D+)^T b [j] = b[j-1] [j >= 1] - b[j] [j < n-1]
+__device__ __forceinline__ float dplus_t(const float* b, size_t i, size_t st, int c, int n) {
+    float r = 0.0f;
+    if (c >= 1) r += b[i - st];
+    if (c < n - 1) r -= b[i];
+    return r;
+}
+// B^T b [j] = b[j+1] [j < n-1] - b[j]
+__device__ __forceinline__ float bm_t(const float* b, size_t i, size_t st, int c, int n) {
+    float r = -b[i];
+    if (c < n - 1) r += b[i + st];
+    return r;
+}
+
+__global__ __launch_bounds__(AB) void adjoint_phase_a(Fields17 A, const float* __restrict__ f,
+                                                      const float* __restrict__ u_next,
+                                                      float* __restrict__ ast,
+                                                      float* __restrict__ wst,
+                                                      float* __restrict__ grad_f, Geometry g,
+                                                      float tau, float lambda, float theta,
+                                                      double* __restrict__ partials) {
+    const size_t n = g.n();
+    const float tl = tau * lambda;
+    double acc[1] = {0.0};
+    for (size_t i = size_t(blockIdx.x) * AB + threadIdx.x; i < n; i += size_t(gridDim.x) * AB) {
+        const float ub = A.f[F_UBAR][i];
+        const float un = A.f[F_U][i] + (1.0f + theta) * ub;
+        const float as = un / (1.0f + tl);
+        A.f[F_U][i] = -theta * ub + as;
+        ast[i] = as;
+        grad_f[i] += tl / (1.0f + tl) * un;
+        acc[0] += double(un) * double(tau) * (double(f[i]) - double(u_next[i])) / (1.0 + double(tl));
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const float vb = A.f[F_VBX + c][i];
+            const float vn = A.f[F_VX + c][i] + (1.0f + theta) * vb;
+            A.f[F_VX + c][i] = -theta * vb + vn;
+            wst[size_t(c) * n + i] = -tau * vn;
+            A.f[F_PX + c][i] += tau * vn;
+        }
+    }
+    block_partials<1>(acc, partials);
+}
+
+__global__ __launch_bounds__(AB) void adjoint_phase_b(Fields17 A, Fields17 S,
+                                                      const float* __restrict__ ast,
+                                                      const float* __restrict__ wst, Geometry g,
+                                                      float tau, float sigma, float alpha0,
+                                                      float alpha1, double* __restrict__ partials) {
+    const size_t n = g.n();
+    const size_t st[3] = {1, size_t(g.w), size_t(g.w) * g.h};
+    const int ext[3] = {g.w, g.h, g.d};
+    double acc[2] = {0.0, 0.0};   // alpha0, alpha1
+    for (size_t i = size_t(blockIdx.x) * AB + threadIdx.x; i < n; i += size_t(gridDim.x) * AB) {
+        const Coord cc = coord_of(i, g);
+        const int c3[3] = {cc.x, cc.y, cc.z};
+        // P' = A.p - tau B^T A*   (component a uses B_a)
+        float P[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) P[a] = A.f[F_PX + a][i] - tau * bm_t(ast, i, st[a], c3[a], ext[a]);
+        // Q' = A.q + div2^T W*
+        const float* W0 = wst;
+        const float* W1 = wst + n;
+        const float* W2 = wst + 2 * n;
+        float Q[6];
+        Q[0] = A.f[F_QXX][i] + dplus_t(W0, i, st[0], c3[0], ext[0]);
+        Q[1] = A.f[F_QYY][i] + dplus_t(W1, i, st[1], c3[1], ext[1]);
+        Q[2] = A.f[F_QZZ][i] + dplus_t(W2, i, st[2], c3[2], ext[2]);
+        Q[3] = A.f[F_QXY][i] + (dplus_t(W0, i, st[1], c3[1], ext[1]) + dplus_t(W1, i, st[0], c3[0], ext[0]));
+        Q[4] = A.f[F_QXZ][i] + (dplus_t(W0, i, st[2], c3[2], ext[2]) + dplus_t(W2, i, st[0], c3[0], ext[0]));
+        Q[5] = A.f[F_QYZ][i] + (dplus_t(W1, i, st[2], c3[2], ext[2]) + dplus_t(W2, i, st[1], c3[1], ext[1]));
+
+        // p~ from x_k, forward arithmetic (tgv_sweeps.hip dual sweep)
+        const float ubc = S.f[F_UBAR][i];
+        float pt[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const float d = (c3[a] < ext[a] - 1) ? S.f[F_UBAR][i + st[a]] - ubc : 0.0f;
+            pt[a] = fmaf(sigma, d - S.f[F_VBX + a][i], S.f[F_PX + a][i]);
+        }
+        float n2 = pt[0] * pt[0];
+        n2 = fmaf(pt[1], pt[1], n2);
+        n2 = fmaf(pt[2], pt[2], n2);
+        const float npn = sqrtf(n2);
+        if (npn / alpha1 > 1.0f) {
+            const float dot = P[0] * pt[0] + P[1] * pt[1] + P[2] * pt[2];
+            acc[1] += double(dot) / double(npn);
+            const float k1 = alpha1 / npn, k2 = dot / (npn * npn);
+#pragma unroll
+            for (int a = 0; a < 3; ++a) P[a] = k1 * (P[a] - k2 * pt[a]);
+        }
+        // q~ from x_k
+        float bxv[3], byv[3], bzv[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const float* vb = S.f[F_VBX + c];
+            const float vc = vb[i];
+            bxv[c] = c3[0] > 0 ? -vc + vb[i - st[0]] : -vc;
+            byv[c] = c3[1] > 0 ? -vc + vb[i - st[1]] : -vc;
+            bzv[c] = c3[2] > 0 ? -vc + vb[i - st[2]] : -vc;
+        }
+        float qt[6];
+        qt[0] = fmaf(sigma, bxv[0], S.f[F_QXX][i]);
+        qt[1] = fmaf(sigma, byv[1], S.f[F_QYY][i]);
+        qt[2] = fmaf(sigma, bzv[2], S.f[F_QZZ][i]);
+        qt[3] = fmaf(sigma, (bxv[1] + byv[0]) * 0.5f, S.f[F_QXY][i]);
+        qt[4] = fmaf(sigma, (bxv[2] + bzv[0]) * 0.5f, S.f[F_QXZ][i]);
+        qt[5] = fmaf(sigma, (byv[2] + bzv[1]) * 0.5f, S.f[F_QYZ][i]);
+        float m2 = qt[0] * qt[0];
+        m2 = fmaf(qt[1], qt[1], m2);
+        m2 = fmaf(2.0f * qt[3], qt[3], m2);
+        m2 = fmaf(qt[2], qt[2], m2);
+        m2 = fmaf(2.0f * qt[4], qt[4], m2);
+        m2 = fmaf(2.0f * qt[5], qt[5], m2);
+        const float nq = sqrtf(m2);
+        if (nq / alpha0 > 1.0f) {
+            float dot = 0.0f;
+#pragma unroll
+            for (int k = 0; k < 6; ++k) dot += Q[k] * qt[k];
+            acc[0] += double(dot) / double(nq);
+            const float k1 = alpha0 / nq, k2 = dot / (nq * nq);
+#pragma unroll
+            for (int k = 0; k < 6; ++k) Q[k] = k1 * (Q[k] - k2 * (k < 3 ? 1.0f : 2.0f) * qt[k]);
+        }
+#pragma unroll
+        for (int a = 0; a < 3; ++a) A.f[F_PX + a][i] = P[a];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) A.f[F_QXX + k][i] = Q[k];
+    }
+    block_partials<2>(acc, partials);
+}
+
+__global__ __launch_bounds__(AB) void adjoint_phase_c(Fields17 A, Geometry g, float sigma) {
+    const size_t n = g.n();
+    const size_t st[3] = {1, size_t(g.w), size_t(g.w) * g.h};
+    const int ext[3] = {g.w, g.h, g.d};
+    for (size_t i = size_t(blockIdx.x) * AB + threadIdx.x; i < n; i += size_t(gridDim.x) * AB) {
+        const Coord cc = coord_of(i, g);
+        const int c3[3] = {cc.x, cc.y, cc.z};
+        // u_bar adjoint = sigma (D+)^T P~
+        float ub = 0.0f;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) ub += dplus_t(A.f[F_PX + a], i, st[a], c3[a], ext[a]);
+        A.f[F_UBAR][i] = sigma * ub;
+        // v_bar adjoint = sigma eps_B^T Q~ - sigma P~
+        //   x: B_x^T Q_xx + 1/2 B_y^T Q_xy + 1/2 B_z^T Q_xz, etc.
+        const float* Qxx = A.f[F_QXX]; const float* Qyy = A.f[F_QYY]; const float* Qzz = A.f[F_QZZ];
+        const float* Qxy = A.f[F_QXY]; const float* Qxz = A.f[F_QXZ]; const float* Qyz = A.f[F_QYZ];
+        const float ex = bm_t(Qxx, i, st[0], c3[0], ext[0]) +
+                         0.5f * bm_t(Qxy, i, st[1], c3[1], ext[1]) +
+                         0.5f * bm_t(Qxz, i, st[2], c3[2], ext[2]);
+        const float ey = bm_t(Qyy, i, st[1], c3[1], ext[1]) +
+                         0.5f * bm_t(Qxy, i, st[0], c3[0], ext[0]) +
+                         0.5f * bm_t(Qyz, i, st[2], c3[2], ext[2]);
+        const float ez = bm_t(Qzz, i, st[2], c3[2], ext[2]) +
+                         0.5f * bm_t(Qxz, i, st[0], c3[0], ext[0]) +
+                         0.5f * bm_t(Qyz, i, st[1], c3[1], ext[1]);
+        A.f[F_VBX][i] = sigma * ex - sigma * A.f[F_PX][i];
+        A.f[F_VBY][i] = sigma * ey - sigma * A.f[F_PY][i];
+        A.f[F_VBZ][i] = sigma * ez - sigma * A.f[F_PZ][i];
+    }
+}
+
+// out[k] += sum over blocks of partials[b][k], fixed order, k < nq
+__global__ void adjoint_accumulate(const double* __restrict__ partials, int nblocks, int nq,
+                                   int offset, double* __restrict__ acc) {
+    const int k = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (k >= nq) return;
+    double s = 0.0;
+    for (int b = lane; b < nblocks; b += 64) s += partials[size_t(b) * 4 + k];
+    s = wsum(s);
+    if (lane == 0) acc[offset + k] += s;
+}
+
+__global__ void adjoint_finish_f(const float* __restrict__ au, const float* __restrict__ aub,
+                                 float* __restrict__ grad_f, size_t n) {
+    for (size_t i = size_t(blockIdx.x) * AB + threadIdx.x; i < n; i += size_t(gridDim.x) * AB)
+        grad_f[i] += au[i] + aub[i];
+}
+
+inline int blocks_for(size_t n) {
+    size_t b = (n + AB - 1) / AB;
+    if (b > MAXB) b = MAXB;
+    return int(b ? b : 1);
+}
+
+Fields17 to_fields(void* const* p) {
+    Fields17 r;
+    for (int k = 0; k < N_FIELDS; ++k) r.f[k] = static_cast<float*>(p[k]);
+    return r;
+}
+
+}  // namespace
+
+size_t adjoint_partials_size() { return size_t(MAXB) * 4; }
+
+// acc: device double[3] = (alpha0, alpha1, lambda) gradients, accumulated.
+hipError_t launch_adjoint_step(hipStream_t s, void* const* adj, void* const* xk, void* const* xk1,
+                               const float* f, float* ast, float* wst, float* grad_f, Geometry g,
+                               float tau, float sigma, float theta, float lambda, float alpha0,
+                               float alpha1, double* partials, double* acc) {
+    const int nb = blocks_for(g.n());
+    const Fields17 A = to_fields(adj), S = to_fields(xk), T = to_fields(xk1);
+    hipLaunchKernelGGL(adjoint_phase_a, dim3(nb), dim3(AB), 0, s, A, f, T.f[F_U], ast, wst,
+                       grad_f, g, tau, lambda, theta, partials);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(adjoint_accumulate, dim3(1), dim3(64), 0, s, partials, nb, 1, 2, acc);
+    hipLaunchKernelGGL(adjoint_phase_b, dim3(nb), dim3(AB), 0, s, A, S, ast, wst, g, tau, sigma,
+                       alpha0, alpha1, partials);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    // partial slots 0: alpha0, 1: alpha1 -> acc[0], acc[1]
+    hipLaunchKernelGGL(adjoint_accumulate, dim3(1), dim3(128), 0, s, partials, nb, 2, 0, acc);
+    hipLaunchKernelGGL(adjoint_phase_c, dim3(nb), dim3(AB), 0, s, A, g, sigma);
+    return hipGetLastError();
+}
+
+hipError_t launch_adjoint_finish(hipStream_t s, void* const* adj, float* grad_f, Geometry g) {
+    hipLaunchKernelGGL(adjoint_finish_f, dim3(blocks_for(g.n())), dim3(AB), 0, s,
+                       static_cast<const float*>(adj[F_U]), static_cast<const float*>(adj[F_UBAR]),
+                       grad_f, g.n());
+    return hipGetLastError();
+}
+
+}  // namespace amb

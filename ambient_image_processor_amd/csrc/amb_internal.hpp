@@ -129,6 +129,16 @@ hipError_t launch_energy(hipStream_t s, int storage, void* const* fields, const 
                          Geometry g, double lambda, double alpha0, double alpha1, int data_term,
                          double* partials, double* out10);
 
+// Adjoint of the unrolled L2 iteration (adjoint.hip).  adj, xk, xk1: 17-field
+// tables (adjoint of x_{k+1} updated in place to the adjoint of x_k); acc: device
+// double[3] (alpha0, alpha1, lambda) accumulated; partials: adjoint_partials_size().
+size_t adjoint_partials_size();
+hipError_t launch_adjoint_step(hipStream_t s, void* const* adj, void* const* xk, void* const* xk1,
+                               const float* f, float* ast, float* wst, float* grad_f, Geometry g,
+                               float tau, float sigma, float theta, float lambda, float alpha0,
+                               float alpha1, double* partials, double* acc);
+hipError_t launch_adjoint_finish(hipStream_t s, void* const* adj, float* grad_f, Geometry g);
+
 // Host-side cos(pi t) from the mathematics (exact reduction modulo 2, octant
 // folding, libm cos/sin on |pi r| <= pi/4).
 double host_cospi(double t);

@@ -51,9 +51,9 @@ amb_status fail_hip(hipError_t e, const char* what, int line) {
     } while (0)
 
 constexpr int N_SCRATCH = 5;
-constexpr int N_PROF = 8;
+constexpr int N_PROF = 9;
 enum { P_DUAL = 0, P_PRIMAL = 1, P_DCT = 2, P_POISSON = 3, P_DIV = 4, P_EPI = 5, P_INIT = 6,
-       P_ENERGY = 7 };
+       P_ENERGY = 7, P_ADJOINT = 8 };
 
 struct AxisTables {
     int n = 0;
@@ -100,6 +100,10 @@ struct amb_context {
     uint64_t prof_n[N_PROF] = {};
     // host staging for callbacks
     std::vector<float> cb_u, cb_v[3];
+    // adjoint workspace (grow-only): checkpoints, recompute slots, adjoint state,
+    // A*, W*, dL/df, partials, scalar accumulators
+    char* adj_block = nullptr;
+    size_t adj_cap = 0;
 };
 
 namespace {
@@ -608,6 +612,7 @@ AMB_API amb_status amb_context_destroy(amb_context* c) {
     if (c->fields_dev) (void)hipFree(c->fields_dev);
     if (c->partials) (void)hipFree(c->partials);
     if (c->energy_out) (void)hipFree(c->energy_out);
+    if (c->adj_block) (void)hipFree(c->adj_block);
     (void)hipStreamDestroy(c->stream);
     delete c;
     return AMB_OK;
@@ -862,6 +867,94 @@ AMB_API amb_status amb_tgv2_energy(amb_context* c, double out[10]) {
                               c->partials, c->energy_out));
     }
     AMB_HIP(hipMemcpyAsync(out, c->energy_out, 10 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    AMB_HIP(hipStreamSynchronize(c->stream));
+    return AMB_OK;
+}
+
+AMB_API amb_status amb_tgv2_denoise_adjoint(amb_context* c, amb_memory mem, const float* f,
+                                            uint32_t w, uint32_t h, uint32_t d,
+                                            const amb_tgv_params* p, const float* dL_du,
+                                            float* u_out, float* dL_df, double dL_dparams[3],
+                                            uint32_t checkpoint_every) {
+    if (!c || !dL_du || !dL_df || !dL_dparams) return fail(AMB_ERR_INVALID_ARGUMENT, "NULL argument");
+    AMB_TRY(check_params(p));
+    if (p->data_term != AMB_DATA_L2)
+        return fail(AMB_ERR_NOT_SUPPORTED, "the adjoint is implemented for the L2 data term");
+    if (p->storage != AMB_STORAGE_F32)
+        return fail(AMB_ERR_NOT_SUPPORTED, "the adjoint runs on FP32 fields");
+    const uint32_t K = checkpoint_every ? checkpoint_every : 20;
+    const uint32_t N = p->iterations;
+    AMB_TRY(begin(c, mem, f, w, h, d, p));
+    const Geometry g = c->g;
+    const size_t n = g.n();
+    const size_t fstride = align_up(n * 4, 256);
+    const size_t state_bytes = fstride * N_FIELDS;
+    const uint32_t nseg = N ? (N + K - 1) / K : 0;
+    const size_t part_bytes = align_up(adjoint_partials_size() * sizeof(double), 256);
+    const size_t need = state_bytes * (size_t(nseg) + (K + 1) + 1) + fstride * 5 + part_bytes + 256;
+    if (need > c->adj_cap) {
+        if (c->adj_block) AMB_HIP(hipFree(c->adj_block));
+        c->adj_block = nullptr;
+        c->adj_cap = 0;
+        AMB_HIP(hipMalloc(reinterpret_cast<void**>(&c->adj_block), need));
+        c->adj_cap = need;
+    }
+    char* base = c->adj_block;
+    auto state_at = [&](char* blk, void** tab) {
+        for (int k = 0; k < N_FIELDS; ++k) tab[k] = blk + size_t(k) * fstride;
+    };
+    char* ckpt = base;
+    char* slots = ckpt + state_bytes * nseg;
+    char* adjs = slots + state_bytes * (K + 1);
+    float* ast = reinterpret_cast<float*>(adjs + state_bytes);
+    float* wst = reinterpret_cast<float*>(adjs + state_bytes + fstride);   // 3 fields
+    float* gradf = reinterpret_cast<float*>(adjs + state_bytes + fstride * 4);
+    double* partials = reinterpret_cast<double*>(adjs + state_bytes + fstride * 5);
+    double* acc = reinterpret_cast<double*>(adjs + state_bytes + fstride * 5 + part_bytes);
+    char* work = static_cast<char*>(c->fields[0]);   // the working state block
+
+    ProfScope ps(c, P_ADJOINT);
+    // forward with a checkpoint of x_k at every k % K == 0
+    for (uint32_t it = 0; it < N; ++it) {
+        if (it % K == 0)
+            AMB_HIP(hipMemcpyAsync(ckpt + state_bytes * (it / K), work, state_bytes,
+                                   hipMemcpyDeviceToDevice, c->stream));
+        AMB_TRY(sweep(c, 0));
+        AMB_TRY(sweep(c, 1));
+    }
+    if (u_out) AMB_TRY(copy_field_out(c, mem, u_out, F_U));
+    // reverse: adjoint of x_N = (dL/du, 0, ...)
+    void* adj[N_FIELDS];
+    state_at(adjs, adj);
+    AMB_HIP(hipMemsetAsync(adjs, 0, state_bytes, c->stream));
+    AMB_HIP(hipMemcpyAsync(adj[F_U], dL_du, n * 4, kind_in(mem), c->stream));
+    AMB_HIP(hipMemsetAsync(gradf, 0, n * 4, c->stream));
+    AMB_HIP(hipMemsetAsync(acc, 0, 3 * sizeof(double), c->stream));
+    for (int seg = int(nseg) - 1; seg >= 0; --seg) {
+        const uint32_t k0 = uint32_t(seg) * K, k1 = std::min(k0 + K, N), L = k1 - k0;
+        void* tabs[2][N_FIELDS];
+        // slots[0] = x_{k0}; slots[j] = x_{k0 + j} recomputed with the same sweeps
+        AMB_HIP(hipMemcpyAsync(slots, ckpt + state_bytes * size_t(seg), state_bytes,
+                               hipMemcpyDeviceToDevice, c->stream));
+        for (uint32_t j = 1; j <= L; ++j) {
+            char* dst = slots + state_bytes * j;
+            AMB_HIP(hipMemcpyAsync(dst, dst - state_bytes, state_bytes, hipMemcpyDeviceToDevice,
+                                   c->stream));
+            state_at(dst, tabs[0]);
+            AMB_HIP(launch_dual_sweep(c->stream, 0, tabs[0], g, c->sp));
+            AMB_HIP(launch_primal_sweep(c->stream, 0, tabs[0], c->f, g, c->sp));
+        }
+        for (int j = int(L) - 1; j >= 0; --j) {
+            state_at(slots + state_bytes * size_t(j), tabs[0]);
+            state_at(slots + state_bytes * size_t(j + 1), tabs[1]);
+            AMB_HIP(launch_adjoint_step(c->stream, adj, tabs[0], tabs[1], c->f, ast, wst, gradf, g,
+                                        c->sp.tau, c->sp.sigma, c->sp.theta, c->sp.lambda,
+                                        c->sp.alpha0, c->sp.alpha1, partials, acc));
+        }
+    }
+    AMB_HIP(launch_adjoint_finish(c->stream, adj, gradf, g));
+    AMB_HIP(hipMemcpyAsync(dL_df, gradf, n * 4, kind_out(mem), c->stream));
+    AMB_HIP(hipMemcpyAsync(dL_dparams, acc, 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     AMB_HIP(hipStreamSynchronize(c->stream));
     return AMB_OK;
 }

@@ -27,7 +27,7 @@ N_FIELDS = 17
 FIELD_NAMES = ["u", "ubar", "vx", "vy", "vz", "vbx", "vby", "vbz", "px", "py", "pz",
                "qxx", "qyy", "qzz", "qxy", "qxz", "qyz"]
 PROFILE_CLASSES = ["dual_sweep", "primal_sweep", "dct", "poisson", "divergence", "epilogue",
-                   "init", "energy"]
+                   "init", "energy", "adjoint"]
 
 # Every function include/ambient_amd.h declares (checked by the CPU test suite).
 EXPORTS = [
@@ -38,7 +38,7 @@ EXPORTS = [
     "amb_solve_poisson_in_cosine_domain", "amb_integrate_gradients_poisson",
     "amb_tgv2_deshade_pipeline", "amb_tgv2_energy", "amb_tgv2_begin", "amb_tgv2_set_state",
     "amb_tgv2_get_state", "amb_tgv2_iterate", "amb_tgv2_sweep", "amb_profile_enable",
-    "amb_profile_read", "amb_profile_reset",
+    "amb_profile_read", "amb_profile_reset", "amb_tgv2_denoise_adjoint",
 ]
 
 
@@ -101,6 +101,8 @@ def lib():
         "amb_profile_enable": [vp, c_int],
         "amb_profile_read": [vp, c_int, POINTER(c_double), POINTER(c_uint64)],
         "amb_profile_reset": [vp],
+        "amb_tgv2_denoise_adjoint": [vp, mem, fp, u32, u32, u32, POINTER(Params), fp, fp, fp,
+                                     POINTER(c_double), u32],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
@@ -318,6 +320,26 @@ class Context:
         _check(lib().amb_integrate_gradients_poisson(self._h, MEM_DEVICE, c_void_p(vx),
                                                      c_void_p(vy), c_void_p(vz), *_dims(shape),
                                                      precision, flags, c_void_p(out)))
+
+    # ---- adjoint ------------------------------------------------------------
+    def tgv2_denoise_adjoint(self, f, params: Params, dL_du, checkpoint_every=20):
+        """amb_tgv2_denoise_adjoint on host arrays -> (u, dL/df, (dL/dalpha0, dL/dalpha1,
+        dL/dlambda))."""
+        f, g = _f32(f), _f32(dL_du)
+        u, gf = np.empty_like(f), np.empty_like(f)
+        gp = (c_double * 3)()
+        _check(lib().amb_tgv2_denoise_adjoint(self._h, MEM_HOST, _ptr(f), *_dims(f.shape),
+                                              ctypes.byref(params), _ptr(g), _ptr(u), _ptr(gf),
+                                              gp, checkpoint_every))
+        return u, gf, np.array(gp[:])
+
+    def tgv2_denoise_adjoint_device(self, f_dev, shape, params: Params, g_dev, u_dev, gf_dev,
+                                    checkpoint_every=20):
+        gp = (c_double * 3)()
+        _check(lib().amb_tgv2_denoise_adjoint(self._h, MEM_DEVICE, c_void_p(f_dev), *_dims(shape),
+                                              ctypes.byref(params), c_void_p(g_dev),
+                                              _ptr(u_dev), c_void_p(gf_dev), gp, checkpoint_every))
+        return np.array(gp[:])
 
     # ---- profiling ----------------------------------------------------------
     def profile(self, enable=True):

@@ -193,6 +193,20 @@ AMB_API amb_status amb_tgv2_deshade_pipeline(amb_context* ctx, amb_memory mem, c
  *   out[9] ||eps_B^T q - p||_2  (v stationarity residual) */
 AMB_API amb_status amb_tgv2_energy(amb_context* ctx, double out[10]);
 
+/* ---- adjoint of the unrolled TGV2-L2 denoise (no counterpart in the reference)
+ * Runs the amb_tgv2_denoise iteration (data_term must be AMB_DATA_L2, storage
+ * AMB_STORAGE_F32) with a checkpoint of the state every `checkpoint_every`
+ * iterations (0 = 20), then the hand-written reverse sweeps, recomputing each
+ * segment from its checkpoint.  Given dL/du for the final u, returns dL/df and
+ * dL_dparams = (dL/dalpha0, dL/dalpha1, dL/dlambda) (FP64 reductions).  u_out
+ * (may be NULL) receives the forward result.  Workspace: (iterations /
+ * checkpoint_every + checkpoint_every + 2) * 17 + 5 volumes of FP32. */
+AMB_API amb_status amb_tgv2_denoise_adjoint(amb_context* ctx, amb_memory mem, const float* f,
+                                            uint32_t width, uint32_t height, uint32_t depth,
+                                            const amb_tgv_params* params, const float* dL_du,
+                                            float* u_out, float* dL_df, double dL_dparams[3],
+                                            uint32_t checkpoint_every);
+
 /* ---- stepwise access to the iteration (tests, checkpointing) -------------
  * fields order: u, u_bar, v_x, v_y, v_z, vbar_x, vbar_y, vbar_z, p_x, p_y, p_z,
  *               q_xx, q_yy, q_zz, q_xy, q_xz, q_yz  (17 arrays of W*H*D floats). */
@@ -210,7 +224,7 @@ AMB_API amb_status amb_tgv2_sweep(amb_context* ctx, int which);
  * events on the context's stream; amb_profile_read returns the summed device
  * time (ms) and launch count per kernel class:
  *   0 dual sweep, 1 primal sweep, 2 DCT passes, 3 Poisson divide,
- *   4 divergence, 5 epilogue, 6 init, 7 energy. */
+ *   4 divergence, 5 epilogue, 6 init, 7 energy, 8 adjoint (whole call). */
 AMB_API amb_status amb_profile_enable(amb_context* ctx, int enable);
 AMB_API amb_status amb_profile_read(amb_context* ctx, int kernel_class, double* total_ms,
                                     uint64_t* launches);

@@ -4,6 +4,10 @@ import sys
 import numpy as np
 import pytest
 
+# torch first: its HIP runtime (same SONAME as the system's) is then the one
+# the library binds to in every test process, whatever order tests import in.
+import torch  # noqa: F401
+
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 if ROOT not in sys.path:
     sys.path.insert(0, ROOT)

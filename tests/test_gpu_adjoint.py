@@ -1,0 +1,132 @@
+"""GPU tests of the hand-written adjoint of the unrolled TGV2-L2 denoise
+(amb_tgv2_denoise_adjoint) against the FP64 CPU adjoint (oracle/adjoint.py)
+and against central finite differences (eps = 1e-3) of the FP64 oracle."""
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import rel_err
+
+pytestmark = pytest.mark.gpu
+
+TAU = float(np.float32(1 / np.sqrt(20)))
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _vol(shape, seed):
+    from ambient_image_processor_amd import phantoms
+    rng = phantoms.SplitMix64(seed)
+    clean = phantoms._boxes(rng, shape, 6, max(shape[2] - 6, 1), (3, max(4, shape[2] // 2)),
+                            (0.1, 1.0), 0.0)
+    f = clean + rng.normal(int(np.prod(shape)), 0.05).reshape(shape)
+    return f.astype(np.float32), clean.astype(np.float32)
+
+
+def _params(native, iters, alpha0, alpha1, lam):
+    return native.default_params(lam=lam, alpha0=alpha0, alpha1=alpha1, tau=TAU, sigma=TAU,
+                                 theta=1.0, iterations=iters, data_term=native.DATA_L2)
+
+
+@pytest.mark.parametrize("alphas", [(2.0, 1.0), (0.02, 0.01)])
+def test_adjoint_matches_fp64_cpu_adjoint(ctx, alphas):
+    from ambient_image_processor_amd import native
+    from oracle import adjoint as adj
+    f, clean = _vol((10, 14, 18), 5)
+    a0, a1 = alphas
+    p = _params(native, 40, a0, a1, 1.0)
+    n = f.size
+    u_ref = adj.forward(f, 40, a0, a1, 1.0, TAU, TAU)["u"]
+    g = (2.0 / n) * (u_ref - clean)
+    u, gf, gp = ctx.tgv2_denoise_adjoint(f, p, g.astype(np.float32), checkpoint_every=7)
+    _, ref = adj.adjoint(f.astype(np.float64), 40, a0, a1, 1.0, TAU, TAU, 1.0, g)
+    _say("dL/df rel", rel_err(gf, ref["f"]), "params", gp,
+         [ref["alpha0"], ref["alpha1"], ref["lam"]])
+    assert rel_err(u, u_ref) < 1e-5
+    assert rel_err(gf, ref["f"]) < 2e-3
+    for k, name in enumerate(("alpha0", "alpha1", "lam")):
+        assert abs(gp[k] - ref[name]) <= 2e-3 * max(abs(ref[name]), 1e-9 + 1e-3 * np.abs(ref["f"]).sum()), name
+
+
+def test_adjoint_independent_of_checkpoint_interval(ctx):
+    from ambient_image_processor_amd import native
+    f, clean = _vol((8, 12, 20), 6)
+    p = _params(native, 23, 0.05, 0.02, 1.0)
+    g = np.random.default_rng(1).standard_normal(f.shape).astype(np.float32)
+    outs = [ctx.tgv2_denoise_adjoint(f, p, g, checkpoint_every=k) for k in (1, 5, 23, 40)]
+    for u, gf, gp in outs[1:]:
+        np.testing.assert_array_equal(u, outs[0][0])
+        np.testing.assert_array_equal(gf, outs[0][1])
+        np.testing.assert_array_equal(gp, outs[0][2])
+
+
+@pytest.mark.parametrize("alphas", [(2.0, 1.0), (0.02, 0.01)])
+def test_adjoint_against_central_differences(ctx, orc, alphas):
+    """32^3 volume, 200 iterations, checkpoints every 20, MSE loss to the clean
+    phantom; central differences of the FP64 oracle forward.
+
+    * Directional derivative along a seeded unit direction: FD at eps = 1e-3
+      agrees to 1e-3.
+    * Scalars: the loss is piecewise smooth (projection kinks).  Once a p or q
+      has been projected it sits on the ball |.| = alpha and later iterates
+      hover there, so FD steps of 1e-6..1e-3 cross kinks; FD converges to the
+      adjoint only as eps -> 1e-8 (FP64).  Checked: converged FD (eps = 1e-8)
+      to 1e-3, and FD at eps = 1e-3 to 1e-2 (the kink-crossing spread).
+      With alpha0 = 2 the q projection never activates and dL/dalpha0 is 0;
+      in the second case many q's hover at the ball, where the FP32 forward's
+      active set can differ from the FP64 one, so dL/dalpha0 is held to 2e-2."""
+    from ambient_image_processor_amd import native
+    f, clean = _vol((32, 32, 32), 7)
+    a0, a1 = alphas
+    lam, iters = 1.0, 200
+    n = f.size
+    f64 = f.astype(np.float64)
+
+    def loss(ff, a0_, a1_, lam_):
+        st = orc.tgv2(ff, iters, lam=lam_, alpha0=a0_, alpha1=a1_, tau=TAU, sigma=TAU,
+                      data_term="l2", dtype=np.float64)
+        return float(np.mean((st.u - clean) ** 2)), st.u
+
+    L0, u64 = loss(f64, a0, a1, lam)
+    p = _params(native, iters, a0, a1, lam)
+    u, gf, gp = ctx.tgv2_denoise_adjoint(f, p, ((2.0 / n) * (u64 - clean)).astype(np.float32), 20)
+
+    def fd_scalar(k, eps):
+        plus, minus = [a0, a1, lam], [a0, a1, lam]
+        plus[k] += eps
+        minus[k] -= eps
+        return (loss(f64, *plus)[0] - loss(f64, *minus)[0]) / (2 * eps)
+
+    d = np.random.default_rng(3).standard_normal(f.shape)
+    d /= np.linalg.norm(d)
+    eps = 1e-3
+    fdf = (loss(f64 + eps * d, a0, a1, lam)[0] - loss(f64 - eps * d, a0, a1, lam)[0]) / (2 * eps)
+    gd = float(np.sum(gf.astype(np.float64) * d))
+    _say(f"alphas {alphas}: directional adjoint {gd:.6e} fd(1e-3) {fdf:.6e}")
+    assert abs(gd - fdf) <= 1e-3 * abs(fdf)
+    scale = max(abs(x) for x in gp)
+    for k, name in enumerate(("alpha0", "alpha1", "lambda")):
+        fd3, fd8 = fd_scalar(k, 1e-3), fd_scalar(k, 1e-8)
+        _say(f"  d/d{name}: adjoint {gp[k]:.6e} fd(1e-3) {fd3:.6e} fd(1e-8) {fd8:.6e}")
+        tol = 2e-2 if (k == 0 and a0 < 1) else 1e-3
+        assert abs(gp[k] - fd8) <= tol * abs(fd8) + 1e-6 * scale, name
+        assert abs(gp[k] - fd3) <= max(tol, 1e-2) * abs(fd3) + 1e-6 * scale, name
+    if a0 < 1:
+        assert abs(gp[0]) > 1e-6 * scale      # the q projection is active
+
+
+def test_adjoint_rejects_l1_and_bf16(ctx):
+    from ambient_image_processor_amd import native
+    f = np.zeros((4, 4, 4), np.float32)
+    p = _params(native, 3, 2.0, 1.0, 1.0)
+    p.data_term = native.DATA_L1
+    with pytest.raises(native.AmbError):
+        ctx.tgv2_denoise_adjoint(f, p, f)
+    p = _params(native, 3, 2.0, 1.0, 1.0)
+    p.storage = native.STORAGE_BF16
+    with pytest.raises(native.AmbError):
+        ctx.tgv2_denoise_adjoint(f, p, f)

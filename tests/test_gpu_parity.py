@@ -357,3 +357,28 @@ def test_reference_launcher_caller_links_unchanged(ctx, orc, tmp_path):
     np.testing.assert_array_equal(u2, u)
     np.testing.assert_array_equal(l2, l)
     np.testing.assert_array_equal(r2, rr)
+
+
+def test_cli_deshade_metaimage(orc, tmp_path):
+    """tgv2_deshade_application on a MetaImage input: the output trio equals the
+    oracle's TGV2-L1 deshade (tau = sigma = 1/sqrt(20)) bit for bit."""
+    import subprocess
+    from ambient_image_processor_amd import _build, metaimage, phantoms
+    f = phantoms.deshade_phantom((10, 16, 20), seed=4)[0]
+    src = str(tmp_path / "in.mha")
+    metaimage.write(src, f, spacing=[0.9, 0.9, 2.0])
+    outs = [str(tmp_path / n) for n in ("u.mha", "l.mhd", "r.mha")]
+    _say("running", _build.CLI)
+    r = subprocess.run([_build.CLI, src, "0.5", "2", "1", "40", "-1"] + outs,
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    (u, mu), (l, _), (rr, _) = (metaimage.read(p) for p in outs)
+    assert mu["spacing"] == [0.9, 0.9, 2.0]
+    tau = float(np.float32(1 / np.sqrt(20)))
+    st = orc.tgv2(f, 40, lam=0.5, alpha0=2.0, alpha1=1.0, tau=tau, sigma=tau,
+                  data_term="l1", dtype=np.float32)
+    lo = orc.integrate_poisson(*st.v, dtype=np.float32)
+    ro, _ = orc.deshade_epilogue(st.u, lo, None, False, True, False)
+    np.testing.assert_array_equal(u, st.u)
+    np.testing.assert_array_equal(l, lo)
+    np.testing.assert_array_equal(rr, ro)
