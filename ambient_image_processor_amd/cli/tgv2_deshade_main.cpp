@@ -10,9 +10,12 @@
 // integration of v and r = u - l with negative values set to zero
 // (TGVDeshadeProcessor.cpp:122-166).  Images are MetaImage files (.mha/.mhd)
 // instead of any ITK-readable format; outputs keep the input's spacing and
-// origin.  With a mask the reference switches to its masked deshade
-// (tgv_deshade_masked); that variant is not part of this build, so a mask
-// argument is rejected with exit code 4.
+// origin.  With a mask, as the reference CLI (:69-84), the masked deshade
+// TGVDeshadeMaskedProcessor::processTGV2L1GPUCuda with negative values set to
+// zero and the background added back: the iteration runs on the unmasked
+// input and updates only voxels with mask >= 1, l and r are multiplied by
+// the mask, r clamped, and u*(1 - mask) added to r
+// (TGVDeshadeMaskedProcessor.cpp:76-210).
 #include "ambient_amd.h"
 #include "metaimage.hpp"
 
@@ -50,15 +53,18 @@ int main(int argc, char* argv[]) {
                   << std::endl;
         return 2;
     }
+    metaimage::Volume mask;
     if (mask_given) {
         try {
-            (void)metaimage::read(mask_path);
+            mask = metaimage::read(mask_path);
         } catch (const std::exception& e) {
             std::cerr << "mask image file does not exist: " << mask_path << std::endl;
             return 3;
         }
-        std::cerr << "masked TGV2 deshade is not available in this build" << std::endl;
-        return 4;
+        if (mask.w != in.w || mask.h != in.h || mask.d != in.d) {
+            std::cerr << "mask image size differs from the input image size" << std::endl;
+            return 4;
+        }
     }
     if (iteration_count < 0) {
         std::cerr << "iteration_count must be >= 0" << std::endl;
@@ -77,10 +83,13 @@ int main(int argc, char* argv[]) {
     p.alpha1 = alpha1;
     p.iterations = uint32_t(iteration_count);
     p.flags = AMB_FLAG_SET_NEGATIVE_ZERO;
+    if (mask_given)
+        p.flags |= AMB_FLAG_MASKED_ITERATION | AMB_FLAG_MASK_SHADING | AMB_FLAG_ADD_BACKGROUND_BACK;
     metaimage::Volume u = in, l = in, r = in;
-    const amb_status st = amb_tgv2_deshade_pipeline(ctx, AMB_MEM_HOST, in.data.data(), in.w, in.h,
-                                                    in.d, &p, nullptr, u.data.data(),
-                                                    l.data.data(), r.data.data(), nullptr, nullptr);
+    const amb_status st = amb_tgv2_deshade_pipeline(
+        ctx, AMB_MEM_HOST, in.data.data(), in.w, in.h, in.d, &p,
+        mask_given ? mask.data.data() : nullptr, u.data.data(), l.data.data(), r.data.data(),
+        nullptr, nullptr);
     if (st != AMB_OK) {
         std::printf("ambient_amd error: %s\n", amb_last_error());
         amb_context_destroy(ctx);

@@ -73,6 +73,8 @@ struct SweepParams {
     int data_term;   // 0 L1, 1 L2
     int zchunk;      // planes per block along z
     int tile;        // tile shape id (tile_shape())
+    // masked deshade: updates only where active[i] != 0 (nullptr: every voxel)
+    const uint8_t* active;
 };
 
 // Column-tile shapes of the sweep kernels: TX x TY threads, each thread owning
@@ -121,7 +123,8 @@ hipError_t launch_poisson_divide(hipStream_t s, const float* in, float* out, con
                                  const float* cy, const float* cz, Geometry g, int quirk);
 hipError_t launch_epilogue(hipStream_t s, const float* u, float* l, const float* mask, float* r,
                            size_t n, int mask_shading, int set_negative_zero,
-                           int add_background_back);
+                           int add_background_back, int mask_again);
+hipError_t launch_active_from_mask(hipStream_t s, const float* mask, uint8_t* act, size_t n);
 
 // FP64 energy / gap (energy.hip).  partials: at least energy_partials_size(g) doubles.
 size_t energy_partials_size(Geometry g);

@@ -104,6 +104,9 @@ struct amb_context {
     // A*, W*, dL/df, partials, scalar accumulators
     char* adj_block = nullptr;
     size_t adj_cap = 0;
+    // masked iteration: one byte per voxel, nonzero = update (grow-only)
+    uint8_t* act = nullptr;
+    size_t act_cap = 0;
 };
 
 namespace {
@@ -389,6 +392,9 @@ amb_status copy_field_out(amb_context* c, amb_memory mem, float* dst, int field)
     return AMB_OK;
 }
 
+amb_status run(amb_context* c, uint32_t iterations, uint32_t interval, amb_iteration_callback cb,
+               void* user, bool with_v);
+
 amb_status begin(amb_context* c, amb_memory mem, const float* f, uint32_t w, uint32_t h,
                  uint32_t d, const amb_tgv_params* p) {
     AMB_TRY(check_geometry(w, h, d));
@@ -408,8 +414,35 @@ amb_status begin(amb_context* c, amb_memory mem, const float* f, uint32_t w, uin
     c->sp.data_term = int(p->data_term);
     c->sp.tile = choose_tile(g);
     c->sp.zchunk = choose_zchunk(g, c->sp.tile, int(p->storage));
+    c->sp.active = nullptr;
     c->have_state = true;
     return AMB_OK;
+}
+
+// Active-set buffer for the masked iteration (n bytes, 256-B aligned so the
+// vector sweeps can read four bytes at a time).
+amb_status ensure_active(amb_context* c) {
+    const size_t need = align_up(c->g.n(), 256);
+    if (need > c->act_cap) {
+        if (c->act) AMB_HIP(hipFree(c->act));
+        c->act = nullptr;
+        c->act_cap = 0;
+        AMB_HIP(hipMalloc(reinterpret_cast<void**>(&c->act), need));
+        c->act_cap = need;
+    }
+    return AMB_OK;
+}
+
+// Runs the iteration with updates restricted to c->act; the restriction is
+// dropped again on every exit path.
+amb_status run_masked(amb_context* c, uint32_t iterations, uint32_t interval,
+                      amb_iteration_callback cb, void* user) {
+    struct Reset {
+        amb_context* c;
+        ~Reset() { c->sp.active = nullptr; }
+    } reset{c};
+    c->sp.active = c->act;
+    return run(c, iterations, interval, cb, user, true);
 }
 
 amb_status sweep(amb_context* c, int which) {
@@ -613,6 +646,7 @@ AMB_API amb_status amb_context_destroy(amb_context* c) {
     if (c->partials) (void)hipFree(c->partials);
     if (c->energy_out) (void)hipFree(c->energy_out);
     if (c->adj_block) (void)hipFree(c->adj_block);
+    if (c->act) (void)hipFree(c->act);
     (void)hipStreamDestroy(c->stream);
     delete c;
     return AMB_OK;
@@ -728,6 +762,25 @@ AMB_API amb_status amb_tgv2_deshade(amb_context* c, amb_memory mem, const float*
     return AMB_OK;
 }
 
+AMB_API amb_status amb_tgv2_deshade_masked(amb_context* c, amb_memory mem, const float* f,
+                                           uint32_t w, uint32_t h, uint32_t d,
+                                           const amb_tgv_params* p, const uint8_t* active, float* u,
+                                           float* vx, float* vy, float* vz,
+                                           amb_iteration_callback cb, void* user) {
+    if (!c) return fail(AMB_ERR_INVALID_ARGUMENT, "ctx is NULL");
+    if (!active) return fail(AMB_ERR_INVALID_ARGUMENT, "active is NULL");
+    AMB_TRY(begin(c, mem, f, w, h, d, p));
+    AMB_TRY(ensure_active(c));
+    AMB_HIP(hipMemcpyAsync(c->act, active, c->g.n(), kind_in(mem), c->stream));
+    AMB_TRY(run_masked(c, p->iterations, p->callback_interval, cb, user));
+    AMB_TRY(copy_field_out(c, mem, u, F_U));
+    AMB_TRY(copy_field_out(c, mem, vx, F_VX));
+    AMB_TRY(copy_field_out(c, mem, vy, F_VY));
+    AMB_TRY(copy_field_out(c, mem, vz, F_VZ));
+    AMB_HIP(hipStreamSynchronize(c->stream));
+    return AMB_OK;
+}
+
 AMB_API amb_status amb_tgv2_denoise(amb_context* c, amb_memory mem, const float* f, uint32_t w,
                                     uint32_t h, uint32_t d, const amb_tgv_params* p, float* u,
                                     amb_iteration_callback cb, void* user) {
@@ -807,6 +860,31 @@ AMB_API amb_status amb_integrate_gradients_poisson(amb_context* c, amb_memory me
     return AMB_OK;
 }
 
+AMB_API amb_status amb_deshade_epilogue(amb_context* c, amb_memory mem, const float* u,
+                                        const float* shading, const float* mask, uint32_t w,
+                                        uint32_t h, uint32_t d, uint32_t flags, float* shading_out,
+                                        float* deshaded) {
+    if (!c || !u || !shading || !deshaded) return fail(AMB_ERR_INVALID_ARGUMENT, "NULL argument");
+    AMB_TRY(geometry_only(c, w, h, d));
+    const size_t n = c->g.n();
+    // u -> scratch[2], l -> scratch[1] (modified in place), mask -> scratch[3], r -> scratch[4]
+    AMB_HIP(hipMemcpyAsync(c->scratch[2], u, n * 4, kind_in(mem), c->stream));
+    AMB_HIP(hipMemcpyAsync(c->scratch[1], shading, n * 4, kind_in(mem), c->stream));
+    if (mask) AMB_HIP(hipMemcpyAsync(c->scratch[3], mask, n * 4, kind_in(mem), c->stream));
+    const bool masked = mask && (flags & AMB_FLAG_MASKED_ITERATION);
+    {
+        ProfScope ps(c, P_EPI);
+        AMB_HIP(launch_epilogue(c->stream, c->scratch[2], c->scratch[1], mask ? c->scratch[3] : nullptr,
+                                c->scratch[4], n, (masked || (flags & AMB_FLAG_MASK_SHADING)) ? 1 : 0,
+                                (flags & AMB_FLAG_SET_NEGATIVE_ZERO) ? 1 : 0,
+                                (flags & AMB_FLAG_ADD_BACKGROUND_BACK) ? 1 : 0, masked ? 1 : 0));
+    }
+    AMB_TRY(copy_out(c, mem, shading_out, c->scratch[1]));
+    AMB_TRY(copy_out(c, mem, deshaded, c->scratch[4]));
+    AMB_HIP(hipStreamSynchronize(c->stream));
+    return AMB_OK;
+}
+
 AMB_API amb_status amb_tgv2_deshade_pipeline(amb_context* c, amb_memory mem, const float* f,
                                              uint32_t w, uint32_t h, uint32_t d,
                                              const amb_tgv_params* p, const float* mask, float* u,
@@ -814,7 +892,21 @@ AMB_API amb_status amb_tgv2_deshade_pipeline(amb_context* c, amb_memory mem, con
                                              amb_iteration_callback cb, void* user) {
     if (!c) return fail(AMB_ERR_INVALID_ARGUMENT, "ctx is NULL");
     AMB_TRY(begin(c, mem, f, w, h, d, p));
-    AMB_TRY(run(c, p->iterations, p->callback_interval, cb, user, true));
+    const bool masked = mask && (p->flags & AMB_FLAG_MASKED_ITERATION);
+    if (masked) {
+        // active set = mask >= 1 (TGVDeshadeMaskedProcessor.cpp:348-353); f stays unmasked
+        // (:99 clones it before the mask multiply at :109)
+        AMB_TRY(ensure_active(c));
+        const float* md = mask;
+        if (mem != AMB_MEM_DEVICE) {
+            AMB_HIP(hipMemcpyAsync(c->scratch[3], mask, c->g.n() * 4, hipMemcpyHostToDevice, c->stream));
+            md = c->scratch[3];
+        }
+        AMB_HIP(launch_active_from_mask(c->stream, md, c->act, c->g.n()));
+        AMB_TRY(run_masked(c, p->iterations, p->callback_interval, cb, user));
+    } else {
+        AMB_TRY(run(c, p->iterations, p->callback_interval, cb, user, true));
+    }
     // v as FP32 device views (BF16 storage: converted into scratch[2..4])
     const float* v[3];
     for (int k = 0; k < 3; ++k) {
@@ -845,9 +937,10 @@ AMB_API amb_status amb_tgv2_deshade_pipeline(amb_context* c, amb_memory mem, con
     {
         ProfScope ps(c, P_EPI);
         AMB_HIP(launch_epilogue(c->stream, uf, l, m, c->scratch[4], c->g.n(),
-                                (p->flags & AMB_FLAG_MASK_SHADING) ? 1 : 0,
+                                (masked || (p->flags & AMB_FLAG_MASK_SHADING)) ? 1 : 0,
                                 (p->flags & AMB_FLAG_SET_NEGATIVE_ZERO) ? 1 : 0,
-                                (p->flags & AMB_FLAG_ADD_BACKGROUND_BACK) ? 1 : 0));
+                                (p->flags & AMB_FLAG_ADD_BACKGROUND_BACK) ? 1 : 0,
+                                masked ? 1 : 0));
     }
     AMB_TRY(copy_out(c, mem, u, uf));
     AMB_TRY(copy_out(c, mem, shading, l));

@@ -125,9 +125,12 @@ __global__ void poisson_kernel(const float* __restrict__ in, float* __restrict__
 }
 
 // A19 epilogue: r = u - l ; mask ; clamp ; add-back (see ambient_amd.h flags).
+// mask_again: the masked processor's final r * mask when no add-back
+// (tgv_deshade_masked/TGVDeshadeMaskedProcessor.cpp:201-209).
 __global__ void epilogue_kernel(const float* __restrict__ u, float* __restrict__ l,
                                 const float* __restrict__ mask, float* __restrict__ r, size_t n,
-                                int mask_shading, int set_negative_zero, int add_background_back) {
+                                int mask_shading, int set_negative_zero, int add_background_back,
+                                int mask_again) {
     const size_t stride = size_t(gridDim.x) * blockDim.x;
     for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
         const float uv = u[i];
@@ -141,9 +144,19 @@ __global__ void epilogue_kernel(const float* __restrict__ u, float* __restrict__
         if (add_background_back && mask) {
             const float bg = float(1.0 - double(mask[i]));
             ri = ri + uv * bg;
+        } else if (mask_again && mask) {
+            ri = ri * mask[i];
         }
         r[i] = ri;
     }
+}
+
+// Active set of the masked iteration: mask >= 1
+// (TGVDeshadeMaskedProcessor.cpp:348-353).
+__global__ void active_kernel(const float* __restrict__ mask, uint8_t* __restrict__ act, size_t n) {
+    const size_t stride = size_t(gridDim.x) * blockDim.x;
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride)
+        act[i] = mask[i] >= 1.0f ? uint8_t(1) : uint8_t(0);
 }
 
 inline int elem_blocks(size_t n) {
@@ -182,9 +195,14 @@ hipError_t launch_poisson_divide(hipStream_t s, const float* in, float* out, con
 
 hipError_t launch_epilogue(hipStream_t s, const float* u, float* l, const float* mask, float* r,
                            size_t n, int mask_shading, int set_negative_zero,
-                           int add_background_back) {
+                           int add_background_back, int mask_again) {
     hipLaunchKernelGGL(epilogue_kernel, dim3(elem_blocks(n)), dim3(256), 0, s, u, l, mask, r, n,
-                       mask_shading, set_negative_zero, add_background_back);
+                       mask_shading, set_negative_zero, add_background_back, mask_again);
+    return hipGetLastError();
+}
+
+hipError_t launch_active_from_mask(hipStream_t s, const float* mask, uint8_t* act, size_t n) {
+    hipLaunchKernelGGL(active_kernel, dim3(elem_blocks(n)), dim3(256), 0, s, mask, act, n);
     return hipGetLastError();
 }
 

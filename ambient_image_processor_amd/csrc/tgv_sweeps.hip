@@ -68,9 +68,9 @@ __device__ __forceinline__ float fwd(float center, float next, bool has_next) {
 // ---------------------------------------------------------------------------
 // Dual sweep
 // ---------------------------------------------------------------------------
-template <class S, class I, int TX, int TY>
+template <class S, class I, int TX, int TY, int MK>
 __global__ __launch_bounds__(TX * TY) void dual_sweep_kernel(
-    typename S::T* __restrict__ ubar, typename S::T* __restrict__ vbx,
+    const uint8_t* __restrict__ act, typename S::T* __restrict__ ubar, typename S::T* __restrict__ vbx,
     typename S::T* __restrict__ vby, typename S::T* __restrict__ vbz,
     typename S::T* __restrict__ px_, typename S::T* __restrict__ py_,
     typename S::T* __restrict__ pz_, typename S::T* __restrict__ qxx_,
@@ -152,7 +152,7 @@ __global__ __launch_bounds__(TX * TY) void dual_sweep_kernel(
             for (int k = 0; k < 6; ++k) qn[k] = S::ld(qf[k], j);
         }
 
-        if (in) {
+        if (in && (!MK || *elem(act, i))) {
             // D+ u_bar
             const float dx = fwd(ub_c, sU[c.ty + 1][c.tx + 2], c.x < W - 1);
             const float dy = fwd(ub_c, sU[c.ty + 2][c.tx + 1], c.y < H - 1);
@@ -221,9 +221,9 @@ __global__ __launch_bounds__(TX * TY) void dual_sweep_kernel(
 // +x: q_xx, q_xy, q_xz.  +y: q_xy, q_yy, q_yz.
 enum { L_PX = 0, L_PY = 1, L_QXX = 2, L_QXY = 3, L_QXZ = 4, L_QYY = 5, L_QYZ = 6, L_N = 7 };
 
-template <class S, class I, int TX, int TY, int L2>
+template <class S, class I, int TX, int TY, int L2, int MK>
 __global__ __launch_bounds__(TX * TY) void primal_sweep_kernel(
-    typename S::T* __restrict__ u_, typename S::T* __restrict__ ubar_,
+    const uint8_t* __restrict__ act, typename S::T* __restrict__ u_, typename S::T* __restrict__ ubar_,
     typename S::T* __restrict__ vx_, typename S::T* __restrict__ vy_,
     typename S::T* __restrict__ vz_, typename S::T* __restrict__ vbx_,
     typename S::T* __restrict__ vby_, typename S::T* __restrict__ vbz_,
@@ -315,7 +315,7 @@ __global__ __launch_bounds__(TX * TY) void primal_sweep_kernel(
             n_v[0] = S::ld(vx_, j); n_v[1] = S::ld(vy_, j); n_v[2] = S::ld(vz_, j);
         }
 
-        if (in) {
+        if (in && (!MK || *elem(act, i))) {
             // B p  (tgv2_l1_deshade.cu:186-193)
             const float bx = bwd(px_c, sL[L_PX][ly][lx - 1], c.x > 0);
             const float by = bwd(py_c, sL[L_PY][ly - 1][lx], c.y > 0);
@@ -431,6 +431,19 @@ __device__ __forceinline__ void lds_get(const float* row, float (&v)[VX]) {
     for (int e = 0; e < VX; ++e) v[e] = row[e];
 }
 
+// Vector store; with MK only the elements whose active byte is set are written.
+template <class S, int VX, int MK, class I>
+__device__ __forceinline__ void vstore(typename S::T* p, I i, const float (&v)[VX],
+                                       const uint32_t actv) {
+    if (!MK) {
+        VecIO<S, VX>::st(p, i, v);
+    } else {
+#pragma unroll
+        for (int e = 0; e < VX; ++e)
+            if ((actv >> (8 * e)) & 0xffu) S::st(p, I(i + e), v[e]);
+    }
+}
+
 template <int TX, int TY, int VX>
 struct TileCoordV {
     int x0, y0, z0, z1, tx, ty, xb, y;
@@ -454,9 +467,9 @@ struct DualPlane {
     float q[6][VX];       // xx, yy, zz, xy, xz, yz
 };
 
-template <class S, class I, int TX, int TY, int VX, int PF>
+template <class S, class I, int TX, int TY, int VX, int PF, int MK>
 __global__ __launch_bounds__(TX * TY) void dual_sweep_vec_kernel(
-    typename S::T* __restrict__ ubar, typename S::T* __restrict__ vbx,
+    const uint8_t* __restrict__ act, typename S::T* __restrict__ ubar, typename S::T* __restrict__ vbx,
     typename S::T* __restrict__ vby, typename S::T* __restrict__ vbz,
     typename S::T* __restrict__ px_, typename S::T* __restrict__ py_,
     typename S::T* __restrict__ pz_, typename S::T* __restrict__ qxx_,
@@ -543,7 +556,8 @@ __global__ __launch_bounds__(TX * TY) void dual_sweep_vec_kernel(
         DualPlane<VX> Pn;
         if (PF && in && z + 1 < c.z1) load_plane(z + 1, Pn);
 
-        if (in) {
+        const uint32_t actv = (MK && in) ? *reinterpret_cast<const uint32_t*>(elem(act, i)) : 0xffffffffu;
+        if (in && (!MK || actv)) {
             float uyn[VX], vym[3][VX], vxm[3];
             lds_get<VX>(&sY[0][c.ty + 2][lx], uyn);
             const float uxn = sXp[c.ty][c.tx + 1];
@@ -607,9 +621,9 @@ __global__ __launch_bounds__(TX * TY) void dual_sweep_vec_kernel(
                 }
             }
 #pragma unroll
-            for (int k = 0; k < 3; ++k) V::st(pf[k], i, po[k]);
+            for (int k = 0; k < 3; ++k) vstore<S, VX, MK>(pf[k], i, po[k], actv);
 #pragma unroll
-            for (int k = 0; k < 6; ++k) V::st(qf[k], i, qo[k]);
+            for (int k = 0; k < 6; ++k) vstore<S, VX, MK>(qf[k], i, qo[k], actv);
         }
         __syncthreads();
         copyv(ub_c, P.ubn);
@@ -625,9 +639,9 @@ struct PrimalPlane {
     float qzzn[VX], qxzn[VX], qyzn[VX];     // at z + 1
 };
 
-template <class S, class I, int TX, int TY, int VX, int PF, int L2>
+template <class S, class I, int TX, int TY, int VX, int PF, int L2, int MK>
 __global__ __launch_bounds__(TX * TY) void primal_sweep_vec_kernel(
-    typename S::T* __restrict__ u_, typename S::T* __restrict__ ubar_,
+    const uint8_t* __restrict__ act, typename S::T* __restrict__ u_, typename S::T* __restrict__ ubar_,
     typename S::T* __restrict__ vx_, typename S::T* __restrict__ vy_,
     typename S::T* __restrict__ vz_, typename S::T* __restrict__ vbx_,
     typename S::T* __restrict__ vby_, typename S::T* __restrict__ vbz_,
@@ -720,7 +734,8 @@ __global__ __launch_bounds__(TX * TY) void primal_sweep_vec_kernel(
         PrimalPlane<VX> Pn;
         if (PF && in && z + 1 < c.z1) load_plane(z + 1, Pn);
 
-        if (in) {
+        const uint32_t actv = (MK && in) ? *reinterpret_cast<const uint32_t*>(elem(act, i)) : 0xffffffffu;
+        if (in && (!MK || actv)) {
             float pym[VX], qxyn_y[VX], qyyn_y[VX], qyzn_y[VX];
             lds_get<VX>(&sY[0][c.ty][lx], pym);
             lds_get<VX>(&sY[1][c.ty + 2][lx], qxyn_y);
@@ -773,10 +788,14 @@ __global__ __launch_bounds__(TX * TY) void primal_sweep_vec_kernel(
                 vbo[1][e] = fmaf(theta, vn1 - P.v[1][e], vn1);
                 vbo[2][e] = fmaf(theta, vn2 - P.v[2][e], vn2);
             }
-            V::st(u_, i, uo);
-            V::st(ubar_, i, ubo);
-            V::st(vx_, i, vo[0]); V::st(vy_, i, vo[1]); V::st(vz_, i, vo[2]);
-            V::st(vbx_, i, vbo[0]); V::st(vby_, i, vbo[1]); V::st(vbz_, i, vbo[2]);
+            vstore<S, VX, MK>(u_, i, uo, actv);
+            vstore<S, VX, MK>(ubar_, i, ubo, actv);
+            vstore<S, VX, MK>(vx_, i, vo[0], actv);
+            vstore<S, VX, MK>(vy_, i, vo[1], actv);
+            vstore<S, VX, MK>(vz_, i, vo[2], actv);
+            vstore<S, VX, MK>(vbx_, i, vbo[0], actv);
+            vstore<S, VX, MK>(vby_, i, vbo[1], actv);
+            vstore<S, VX, MK>(vbz_, i, vbo[2], actv);
         }
         __syncthreads();
         copyv(pz_p, P.pz);
@@ -822,33 +841,33 @@ inline int sweep_prefetch() {
     return sizeof(typename S::T) == 4 ? 1 : 0;
 }
 
-template <class S, class I, int TX, int TY, int PF>
+template <class S, class I, int TX, int TY, int PF, int MK = 0>
 hipError_t dual_vec_launch(hipStream_t s, void* const* fv, Geometry g, const SweepParams& p) {
     typedef typename S::T T;
     T* const* F = reinterpret_cast<T* const*>(fv);
-    hipLaunchKernelGGL((dual_sweep_vec_kernel<S, I, TX, TY, 4, PF>),
-                       dim3(grid_blocks<TX, TY, 4>(g, p.zchunk)), dim3(TX * TY), 0, s, F[F_UBAR],
+    hipLaunchKernelGGL((dual_sweep_vec_kernel<S, I, TX, TY, 4, PF, MK>),
+                       dim3(grid_blocks<TX, TY, 4>(g, p.zchunk)), dim3(TX * TY), 0, s, p.active, F[F_UBAR],
                        F[F_VBX], F[F_VBY], F[F_VBZ], F[F_PX], F[F_PY], F[F_PZ], F[F_QXX],
                        F[F_QYY], F[F_QZZ], F[F_QXY], F[F_QXZ], F[F_QYZ], g, p.sigma, p.alpha0,
                        p.alpha1, p.zchunk);
     return hipGetLastError();
 }
 
-template <class S, class I, int TX, int TY, int PF>
+template <class S, class I, int TX, int TY, int PF, int MK = 0>
 hipError_t primal_vec_launch(hipStream_t s, void* const* fv, const float* f, Geometry g,
                              const SweepParams& p) {
     typedef typename S::T T;
     T* const* F = reinterpret_cast<T* const*>(fv);
     const dim3 grid(grid_blocks<TX, TY, 4>(g, p.zchunk)), block(TX * TY);
     if (p.data_term == 1)
-        hipLaunchKernelGGL((primal_sweep_vec_kernel<S, I, TX, TY, 4, PF, 1>), grid, block, 0, s,
-                           F[F_U], F[F_UBAR], F[F_VX], F[F_VY], F[F_VZ], F[F_VBX], F[F_VBY],
+        hipLaunchKernelGGL((primal_sweep_vec_kernel<S, I, TX, TY, 4, PF, 1, MK>), grid, block, 0, s,
+                           p.active, F[F_U], F[F_UBAR], F[F_VX], F[F_VY], F[F_VZ], F[F_VBX], F[F_VBY],
                            F[F_VBZ], F[F_PX], F[F_PY], F[F_PZ], F[F_QXX], F[F_QYY], F[F_QZZ],
                            F[F_QXY], F[F_QXZ], F[F_QYZ], f, g, p.tau, p.lambda, p.theta,
                            p.zchunk);
     else
-        hipLaunchKernelGGL((primal_sweep_vec_kernel<S, I, TX, TY, 4, PF, 0>), grid, block, 0, s,
-                           F[F_U], F[F_UBAR], F[F_VX], F[F_VY], F[F_VZ], F[F_VBX], F[F_VBY],
+        hipLaunchKernelGGL((primal_sweep_vec_kernel<S, I, TX, TY, 4, PF, 0, MK>), grid, block, 0, s,
+                           p.active, F[F_U], F[F_UBAR], F[F_VX], F[F_VY], F[F_VZ], F[F_VBX], F[F_VBY],
                            F[F_VBZ], F[F_PX], F[F_PY], F[F_PZ], F[F_QXX], F[F_QYY], F[F_QZZ],
                            F[F_QXY], F[F_QXZ], F[F_QYZ], f, g, p.tau, p.lambda, p.theta,
                            p.zchunk);
@@ -870,38 +889,46 @@ hipError_t primal_vec_pf(hipStream_t s, void* const* fv, const float* f, Geometr
 // 32-bit offsets when every byte offset of a 4-byte field fits in 32 bits.
 inline bool small_index(const Geometry& g) { return g.n() * 4 < (size_t(1) << 32); }
 
-template <class S, class I, int TX, int TY>
+template <class S, class I, int TX, int TY, int MK = 0>
 hipError_t dual_launch(hipStream_t s, void* const* fv, Geometry g, const SweepParams& p) {
     typedef typename S::T T;
     T* const* F = reinterpret_cast<T* const*>(fv);
-    hipLaunchKernelGGL((dual_sweep_kernel<S, I, TX, TY>), dim3(grid_blocks<TX, TY>(g, p.zchunk)),
-                       dim3(TX * TY), 0, s, F[F_UBAR], F[F_VBX], F[F_VBY], F[F_VBZ], F[F_PX],
+    hipLaunchKernelGGL((dual_sweep_kernel<S, I, TX, TY, MK>), dim3(grid_blocks<TX, TY>(g, p.zchunk)),
+                       dim3(TX * TY), 0, s, p.active, F[F_UBAR], F[F_VBX], F[F_VBY], F[F_VBZ], F[F_PX],
                        F[F_PY], F[F_PZ], F[F_QXX], F[F_QYY], F[F_QZZ], F[F_QXY], F[F_QXZ],
                        F[F_QYZ], g, p.sigma, p.alpha0, p.alpha1, p.zchunk);
     return hipGetLastError();
 }
 
-template <class S, class I, int TX, int TY>
+template <class S, class I, int TX, int TY, int MK = 0>
 hipError_t primal_launch(hipStream_t s, void* const* fv, const float* f, Geometry g,
                          const SweepParams& p) {
     typedef typename S::T T;
     T* const* F = reinterpret_cast<T* const*>(fv);
     const dim3 grid(grid_blocks<TX, TY>(g, p.zchunk)), block(TX * TY);
     if (p.data_term == 1)
-        hipLaunchKernelGGL((primal_sweep_kernel<S, I, TX, TY, 1>), grid, block, 0, s, F[F_U],
+        hipLaunchKernelGGL((primal_sweep_kernel<S, I, TX, TY, 1, MK>), grid, block, 0, s, p.active, F[F_U],
                            F[F_UBAR], F[F_VX], F[F_VY], F[F_VZ], F[F_VBX], F[F_VBY], F[F_VBZ],
                            F[F_PX], F[F_PY], F[F_PZ], F[F_QXX], F[F_QYY], F[F_QZZ], F[F_QXY],
                            F[F_QXZ], F[F_QYZ], f, g, p.tau, p.lambda, p.theta, p.zchunk);
     else
-        hipLaunchKernelGGL((primal_sweep_kernel<S, I, TX, TY, 0>), grid, block, 0, s, F[F_U],
+        hipLaunchKernelGGL((primal_sweep_kernel<S, I, TX, TY, 0, MK>), grid, block, 0, s, p.active, F[F_U],
                            F[F_UBAR], F[F_VX], F[F_VY], F[F_VZ], F[F_VBX], F[F_VBY], F[F_VBZ],
                            F[F_PX], F[F_PY], F[F_PZ], F[F_QXX], F[F_QYY], F[F_QZZ], F[F_QXY],
                            F[F_QXZ], F[F_QYZ], f, g, p.tau, p.lambda, p.theta, p.zchunk);
     return hipGetLastError();
 }
 
+// Masked sweeps use the default shapes only (64x4 threads; vector tiles when
+// W % 4 == 0, with the storage's default prefetch).
 template <class S, class I>
 hipError_t dual_tiles(hipStream_t s, void* const* fv, Geometry g, const SweepParams& p) {
+    if (p.active) {
+        if (p.tile >= 4)
+            return sizeof(typename S::T) == 4 ? dual_vec_launch<S, I, 64, 4, 1, 1>(s, fv, g, p)
+                                              : dual_vec_launch<S, I, 64, 4, 0, 1>(s, fv, g, p);
+        return dual_launch<S, I, 64, 4, 1>(s, fv, g, p);
+    }
     switch (p.tile) {
         case 4: return dual_vec_pf<S, I, 64, 4>(s, fv, g, p);
         case 5: return dual_vec_pf<S, I, 64, 2>(s, fv, g, p);
@@ -917,6 +944,12 @@ hipError_t dual_tiles(hipStream_t s, void* const* fv, Geometry g, const SweepPar
 template <class S, class I>
 hipError_t primal_tiles(hipStream_t s, void* const* fv, const float* f, Geometry g,
                         const SweepParams& p) {
+    if (p.active) {
+        if (p.tile >= 4)
+            return sizeof(typename S::T) == 4 ? primal_vec_launch<S, I, 64, 4, 1, 1>(s, fv, f, g, p)
+                                              : primal_vec_launch<S, I, 64, 4, 0, 1>(s, fv, f, g, p);
+        return primal_launch<S, I, 64, 4, 1>(s, fv, f, g, p);
+    }
     switch (p.tile) {
         case 4: return primal_vec_pf<S, I, 64, 4>(s, fv, f, g, p);
         case 5: return primal_vec_pf<S, I, 64, 2>(s, fv, f, g, p);

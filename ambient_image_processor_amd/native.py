@@ -23,6 +23,7 @@ FLAG_SET_NEGATIVE_ZERO = 0x1
 FLAG_MASK_SHADING = 0x2
 FLAG_ADD_BACKGROUND_BACK = 0x4
 FLAG_POISSON_INDEX_QUIRK = 0x8
+FLAG_MASKED_ITERATION = 0x10
 N_FIELDS = 17
 FIELD_NAMES = ["u", "ubar", "vx", "vy", "vz", "vbx", "vby", "vbz", "px", "py", "pz",
                "qxx", "qyy", "qzz", "qxy", "qxz", "qyz"]
@@ -39,6 +40,7 @@ EXPORTS = [
     "amb_tgv2_deshade_pipeline", "amb_tgv2_energy", "amb_tgv2_begin", "amb_tgv2_set_state",
     "amb_tgv2_get_state", "amb_tgv2_iterate", "amb_tgv2_sweep", "amb_profile_enable",
     "amb_profile_read", "amb_profile_reset", "amb_tgv2_denoise_adjoint",
+    "amb_tgv2_deshade_masked", "amb_deshade_epilogue",
 ]
 
 
@@ -85,6 +87,9 @@ def lib():
         "amb_tgv2_deshade": [vp, mem, fp, u32, u32, u32, POINTER(Params), fp, fp, fp, fp,
                              CALLBACK, vp],
         "amb_tgv2_denoise": [vp, mem, fp, u32, u32, u32, POINTER(Params), fp, CALLBACK, vp],
+        "amb_deshade_epilogue": [vp, mem, fp, fp, fp, u32, u32, u32, u32, fp, fp],
+        "amb_tgv2_deshade_masked": [vp, mem, fp, u32, u32, u32, POINTER(Params), vp, fp, fp, fp,
+                                    fp, CALLBACK, vp],
         "amb_divergence": [vp, mem, fp, fp, fp, u32, u32, u32, fp],
         "amb_cosine_transform": [vp, mem, fp, u32, u32, u32, u32, fp],
         "amb_inverse_cosine_transform": [vp, mem, fp, u32, u32, u32, u32, fp],
@@ -210,10 +215,27 @@ class Context:
     def tgv2_deshade(self, f, params: Params, callback=None):
         """amb_tgv2_deshade on host arrays -> (u, vx, vy, vz)."""
         f = _f32(f)
+        self.shape = f.shape
         outs = [np.empty_like(f) for _ in range(4)]
         cb, keep = self._callback(callback, f.shape, True)
         _check(lib().amb_tgv2_deshade(self._h, MEM_HOST, _ptr(f), *_dims(f.shape),
                                       ctypes.byref(params), *[_ptr(o) for o in outs], cb, None))
+        del keep
+        return tuple(outs)
+
+    def tgv2_deshade_masked(self, f, params: Params, active, callback=None):
+        """amb_tgv2_deshade_masked on host arrays; active: nonzero = update
+        -> (u, vx, vy, vz)."""
+        f = _f32(f)
+        act = np.ascontiguousarray(np.asarray(active) != 0, dtype=np.uint8)
+        if act.shape != f.shape:
+            raise ValueError("active must have the shape of f")
+        self.shape = f.shape
+        outs = [np.empty_like(f) for _ in range(4)]
+        cb, keep = self._callback(callback, f.shape, True)
+        _check(lib().amb_tgv2_deshade_masked(self._h, MEM_HOST, _ptr(f), *_dims(f.shape),
+                                             ctypes.byref(params), _ptr(act),
+                                             *[_ptr(o) for o in outs], cb, None))
         del keep
         return tuple(outs)
 
@@ -229,6 +251,7 @@ class Context:
     def deshade_pipeline(self, f, params: Params, mask=None, callback=None):
         """amb_tgv2_deshade_pipeline on host arrays -> (u, shading, deshaded)."""
         f = _f32(f)
+        self.shape = f.shape
         m = _f32(mask) if mask is not None else None
         u, l, r = (np.empty_like(f) for _ in range(3))
         cb, keep = self._callback(callback, f.shape, True)
@@ -237,6 +260,15 @@ class Context:
                                                _ptr(r), cb, None))
         del keep
         return u, l, r
+
+    def deshade_epilogue(self, u, l, mask=None, flags=FLAG_SET_NEGATIVE_ZERO):
+        """amb_deshade_epilogue on host arrays -> (r, l_out)."""
+        u, l = _f32(u), _f32(l)
+        m = _f32(mask) if mask is not None else None
+        r, lo = np.empty_like(u), np.empty_like(u)
+        _check(lib().amb_deshade_epilogue(self._h, MEM_HOST, _ptr(u), _ptr(l), _ptr(m),
+                                          *_dims(u.shape), int(flags), _ptr(lo), _ptr(r)))
+        return r, lo
 
     def deshade_pipeline_device(self, f_dev: int, shape, params: Params, u_dev=None, l_dev=None,
                                 r_dev=None, mask_dev=None):

@@ -58,6 +58,28 @@ def tgv2_l1_deshade_launch(f_host, width, height, depth, lambda_, iteration_coun
     return _ctx().tgv2_deshade(f, p, iteration_finished_callback)
 
 
+def tgv2_l1_deshade_masked_launch(f_host, width, height, depth, lambda_, iteration_count,
+                                  paint_iteration_interval, cuda_block_dimension,
+                                  iteration_finished_callback, alpha0, alpha1,
+                                  masked_pixel_indices, *edge_pixel_indices, **kw):
+    """Reference tgv_deshade_masked/tgv2_l1_deshade_masked.cu:27-305 -> (u, v_x, v_y, v_z).
+    Updates run only at the linear indices in masked_pixel_indices.  The twelve
+    edge lists that follow (left, not_left, right, ..., back, not_back) are
+    accepted for signature parity; the reference builds them from the volume
+    faces (TGVDeshadeMaskedProcessor.cpp:355-383), which the sweeps handle
+    themselves, so they carry no extra information."""
+    del cuda_block_dimension, edge_pixel_indices
+    f = _vol(f_host, width, height, depth)
+    idx = np.asarray(masked_pixel_indices, dtype=np.int64).ravel()
+    if idx.size and (idx.min() < 0 or idx.max() >= f.size):
+        raise IndexError("masked pixel index outside the volume")
+    active = np.zeros(f.size, np.uint8)
+    active[idx] = 1
+    p = _params(lambda_, iteration_count, paint_iteration_interval, alpha0, alpha1,
+                native.DATA_L1, **kw)
+    return _ctx().tgv2_deshade_masked(f, p, active.reshape(f.shape), iteration_finished_callback)
+
+
 def _denoise(data_term, f_host, width, height, depth, lambda_, iteration_count,
              paint_iteration_interval, iteration_finished_callback, alpha0, alpha1, **kw):
     f = _vol(f_host, width, height, depth)
@@ -170,3 +192,71 @@ class TGVProcessor:
         d, h, w = f.shape
         return tgv2_l1_launch(f, w, h, d, lambda_, iteration_count, paint_iteration_interval,
                               iteration_finished_callback, alpha0, alpha1, **kw)
+
+
+def _aux_ctx() -> native.Context:
+    """Second context for work issued from inside an iteration callback."""
+    c = getattr(_tls, "aux", None)
+    if c is None:
+        c = native.Context(0)
+        _tls.aux = c
+    return c
+
+
+class TGVDeshadeMaskedProcessor:
+    """Reference tgv_deshade_masked/TGVDeshadeMaskedProcessor.{h,cpp} (3-D path)."""
+
+    @staticmethod
+    def buildMaskIndices(mask):
+        """TGVDeshadeMaskedProcessor.cpp:337-385: linear indices of the voxels
+        with mask >= 1, and of those on / not on each volume face.  Returns
+        (pixel_indices_inside_mask, left, not_left, right, not_right, top,
+        not_top, bottom, not_bottom, front, not_front, back, not_back)."""
+        m = np.asarray(mask, dtype=np.float32)
+        d, h, w = m.shape
+        inside = np.flatnonzero(m.ravel() >= 1).astype(np.uint32)
+        z, rem = np.divmod(inside, w * h)
+        y, x = np.divmod(rem, w)
+        out = [inside]
+        for coord, last in ((x, w - 1), (y, h - 1), (z, d - 1)):
+            for edge in (coord == 0, coord == last):
+                out += [inside[edge], inside[~edge]]
+        return tuple(out)
+
+    @staticmethod
+    def processTGV2L1GPUCuda(input_image, lambda_, alpha0, alpha1, iteration_count,
+                             cuda_block_dimension=-1, paint_iteration_interval=0,
+                             iteration_finished_callback=None, mask=None,
+                             set_negative_values_to_zero=True, add_background_back=False,
+                             calculate_div_v=False, **kw):
+        """TGVDeshadeMaskedProcessor.cpp:76-210 -> (deshaded_image, denoised_image,
+        shading_image, div_v_image or None).  f is the unmasked input; the
+        iteration updates only voxels with mask >= 1; l and r are multiplied by
+        the mask, r clamped, then either u*(1 - mask) is added back or r is
+        multiplied by the mask again.  Without a mask the mask is all ones.
+        The callback receives (iteration_index, iteration_count, u, l, r)
+        computed with the same composition, as in the reference."""
+        del cuda_block_dimension
+        f = np.ascontiguousarray(input_image, dtype=np.float32)
+        m = (np.ones_like(f) if mask is None
+             else np.ascontiguousarray(mask, dtype=np.float32).reshape(f.shape))
+        flags = native.FLAG_MASKED_ITERATION | native.FLAG_MASK_SHADING
+        if set_negative_values_to_zero:
+            flags |= native.FLAG_SET_NEGATIVE_ZERO
+        if add_background_back:
+            flags |= native.FLAG_ADD_BACKGROUND_BACK
+        p = _params(lambda_, iteration_count, paint_iteration_interval, alpha0, alpha1,
+                    native.DATA_L1, flags=flags, **kw)
+        cb = None
+        if iteration_finished_callback is not None:
+            def cb(it, count, u, vx, vy, vz):
+                aux = _aux_ctx()
+                l = aux.integrate_gradients_poisson(vx, vy, vz, p.dct_precision)
+                r, l = aux.deshade_epilogue(u, l, m, flags)
+                return iteration_finished_callback(it, count, u, l, r)
+        u, l, r = _ctx().deshade_pipeline(f, p, m, cb)
+        div_v = None
+        if calculate_div_v:
+            st = _ctx().get_state()
+            div_v = _ctx().divergence(st[2], st[3], st[4])
+        return r, u, l, div_v

@@ -69,6 +69,11 @@ typedef enum { AMB_DCT_REFERENCE = 0, AMB_DCT_F32 = 1, AMB_DCT_BF16X3 = 2 } amb_
 #define AMB_FLAG_POISSON_INDEX_QUIRK  0x8u  /* recover voxel coordinates through float arithmetic as
                                                solve_poisson_in_cosine_domain.cu:36-40 does (differs
                                                from exact coordinates only for linear index >= 2^24) */
+#define AMB_FLAG_MASKED_ITERATION     0x10u /* amb_tgv2_deshade_pipeline with a mask: the masked
+                                               deshade composition of
+                                               TGVDeshadeMaskedProcessor.cpp:76-210 - f unmasked,
+                                               updates only where mask >= 1, l and r times the
+                                               mask, clamp, then add-back or r times the mask */
 
 typedef struct {
     float lambda;                /* data-term weight */
@@ -166,14 +171,45 @@ AMB_API amb_status amb_integrate_gradients_poisson(amb_context* ctx, amb_memory 
                                                    uint32_t dct_precision, uint32_t flags,
                                                    float* shading);
 
+/* ---- A19 epilogue alone ----------------------------------------------------
+ * Replaces the image-operation chain after the Poisson integration
+ * (reference TGVDeshadeProcessor.cpp:83-93 subtract / multiply /
+ * clamp_negative_values, :251-255 add-back; masked composition
+ * TGVDeshadeMaskedProcessor.cpp:119-135): r = u - l; with a mask, r times the
+ * mask (and l times the mask with MASK_SHADING or MASKED_ITERATION); clamp
+ * with SET_NEGATIVE_ZERO; then add-back of u*(1 - mask) with
+ * ADD_BACKGROUND_BACK, else with MASKED_ITERATION r times the mask again.
+ * shading_out (optional) receives l after masking; mask may be NULL. */
+AMB_API amb_status amb_deshade_epilogue(amb_context* ctx, amb_memory mem, const float* u,
+                                        const float* shading, const float* mask, uint32_t width,
+                                        uint32_t height, uint32_t depth, uint32_t flags,
+                                        float* shading_out, float* deshaded);
+
+/* ---- masked TGV2-L1 deshade ----------------------------------------------
+ * Replaces tgv2_l1_deshade_masked_launch (reference
+ * tgv_deshade_masked/tgv2_l1_deshade_masked.cu:27-305): the amb_tgv2_deshade
+ * iteration, but the dual and primal updates run only at voxels with
+ * active[i] != 0 (the reference's masked_pixel_indices,
+ * TGVDeshadeMaskedProcessor.cpp:337-385); the stencils are the unmasked
+ * ones, so voxels outside the set keep their initial values (u = u_bar = f,
+ * v = p = q = 0) and are read as neighbours.  active: width*height*depth bytes
+ * in `mem` space.  Same outputs and callback as amb_tgv2_deshade. */
+AMB_API amb_status amb_tgv2_deshade_masked(amb_context* ctx, amb_memory mem, const float* f,
+                                           uint32_t width, uint32_t height, uint32_t depth,
+                                           const amb_tgv_params* params, const uint8_t* active,
+                                           float* u, float* v_x, float* v_y, float* v_z,
+                                           amb_iteration_callback callback, void* user);
+
 /* ---- A19: full deshade ----------------------------------------------------
  * Replaces TGVDeshadeProcessor::processTGV2L1GPUCuda (reference
  * TGVDeshadeProcessor.cpp:122-166 with flags = SET_NEGATIVE_ZERO and without
  * MASK_SHADING; the callback overload :285-314 -> :224-282 with MASK_SHADING and
  * optionally ADD_BACKGROUND_BACK): iteration, Poisson integration of v, then
- * r = u - l, mask, clamp, add-back.  mask may be NULL.  The callback (if any)
- * receives u and v as in amb_tgv2_deshade.  Any of u, shading, deshaded may be
- * NULL if not wanted. */
+ * r = u - l, mask, clamp, add-back.  mask may be NULL.  With a mask and
+ * AMB_FLAG_MASKED_ITERATION: TGVDeshadeMaskedProcessor::processTGV2L1GPUCuda
+ * (reference TGVDeshadeMaskedProcessor.cpp:76-210; MASK_SHADING implied).
+ * The callback (if any) receives u and v as in amb_tgv2_deshade.  Any of u,
+ * shading, deshaded may be NULL if not wanted. */
 AMB_API amb_status amb_tgv2_deshade_pipeline(amb_context* ctx, amb_memory mem, const float* f,
                                              uint32_t width, uint32_t height, uint32_t depth,
                                              const amb_tgv_params* params, const float* mask,

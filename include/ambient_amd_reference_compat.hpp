@@ -24,6 +24,7 @@
 #include <cstdlib>
 #include <functional>
 #include <type_traits>
+#include <vector>
 
 namespace ambient_amd_compat {
 
@@ -114,6 +115,61 @@ Pixel* tgv2_l1_deshade_launch(Pixel* f_host, ambient_amd_compat::uint width,
     check(amb_tgv2_deshade(context(), AMB_MEM_HOST, f_host, width, height, depth, &p, u,
                            *v_x_host, *v_y_host, *v_z_host, cb, &iteration_finished_callback),
           "tgv2_l1_deshade_launch");
+    return u;
+}
+
+// ---- reference tgv_deshade_masked/tgv2_l1_deshade_masked.cu:27-44 ---------
+// IndexVector is const std::vector<uint>& (tgv_masked_common.cu:41).  The
+// iteration updates only the voxels in masked_pixel_indices; the twelve edge
+// lists are built by the reference from the volume faces
+// (TGVDeshadeMaskedProcessor.cpp:355-383), which the sweeps know themselves,
+// so they are accepted and not read.
+template <typename Pixel>
+Pixel* tgv2_l1_deshade_masked_launch(
+    Pixel* f_host, ambient_amd_compat::uint width, ambient_amd_compat::uint height,
+    ambient_amd_compat::uint depth, Pixel lambda, ambient_amd_compat::uint iteration_count,
+    ambient_amd_compat::uint paint_iteration_interval, const int cuda_block_dimension,
+    std::function<bool(ambient_amd_compat::uint, ambient_amd_compat::uint, Pixel*, Pixel*, Pixel*,
+                       Pixel*)>
+        iteration_finished_callback,
+    Pixel alpha0, Pixel alpha1, Pixel** v_x_host, Pixel** v_y_host, Pixel** v_z_host,
+    const std::vector<ambient_amd_compat::uint>& masked_pixel_indices,
+    const std::vector<ambient_amd_compat::uint>& /*left_edge_pixel_indices*/,
+    const std::vector<ambient_amd_compat::uint>& /*not_left_edge_pixel_indices*/,
+    const std::vector<ambient_amd_compat::uint>& /*right_edge_pixel_indices*/,
+    const std::vector<ambient_amd_compat::uint>& /*not_right_edge_pixel_indices*/,
+    const std::vector<ambient_amd_compat::uint>& /*top_edge_pixel_indices*/,
+    const std::vector<ambient_amd_compat::uint>& /*not_top_edge_pixel_indices*/,
+    const std::vector<ambient_amd_compat::uint>& /*bottom_edge_pixel_indices*/,
+    const std::vector<ambient_amd_compat::uint>& /*not_bottom_edge_pixel_indices*/,
+    const std::vector<ambient_amd_compat::uint>& /*front_edge_pixel_indices*/,
+    const std::vector<ambient_amd_compat::uint>& /*not_front_edge_pixel_indices*/,
+    const std::vector<ambient_amd_compat::uint>& /*back_edge_pixel_indices*/,
+    const std::vector<ambient_amd_compat::uint>& /*not_back_edge_pixel_indices*/) {
+    using namespace ambient_amd_compat;
+    static_assert(std::is_same<Pixel, float>::value, "only the float instantiation exists");
+    (void)cuda_block_dimension;
+    typedef std::function<bool(uint, uint, Pixel*, Pixel*, Pixel*, Pixel*)> Fn;
+    const size_t n = size_t(width) * height * depth;
+    std::vector<uint8_t> active(n, 0);
+    for (uint i : masked_pixel_indices) {
+        if (size_t(i) >= n) {
+            std::printf("ambient_amd error: masked pixel index %u outside the volume\n", i);
+            std::exit(int(AMB_ERR_INVALID_ARGUMENT));
+        }
+        active[i] = 1;
+    }
+    amb_tgv_params p = reference_params(lambda, iteration_count, paint_iteration_interval,
+                                        alpha0, alpha1, AMB_DATA_L1);
+    Pixel* u = new Pixel[n];
+    *v_x_host = new Pixel[n];
+    *v_y_host = new Pixel[n];
+    *v_z_host = new Pixel[n];
+    amb_iteration_callback cb = iteration_finished_callback ? &Trampoline<Fn>::deshade : nullptr;
+    check(amb_tgv2_deshade_masked(context(), AMB_MEM_HOST, f_host, width, height, depth, &p,
+                                  active.data(), u, *v_x_host, *v_y_host, *v_z_host, cb,
+                                  &iteration_finished_callback),
+          "tgv2_l1_deshade_masked_launch");
     return u;
 }
 

@@ -136,11 +136,17 @@ class TGVState:
         c._ptrs()
         return c
 
-    def iterate(self, iterations, lam, alpha0, alpha1, tau, sigma, theta=1.0, data_term="l2"):
+    def iterate(self, iterations, lam, alpha0, alpha1, tau, sigma, theta=1.0, data_term="l2",
+                mask=None):
+        """mask (optional): updates only where mask >= 1 (the reference's masked deshade)."""
         prm = (ctypes.c_double * 6)(lam, alpha0, alpha1, tau, sigma, theta)
         dt = 0 if data_term == "l1" else 1
+        act = None
+        if mask is not None:
+            act = np.ascontiguousarray(np.asarray(mask) >= 1, dtype=np.uint8)
         getattr(lib(), "orc_tgv2_iterate_" + _suf(self.dtype))(
-            self._pp, _p(self.f), *_dims(self.f), prm, dt, int(iterations))
+            self._pp, _p(self.f), *_dims(self.f), prm, dt, int(iterations),
+            _p(act) if act is not None else None)
         return self
 
     def energy(self, lam, alpha0, alpha1, data_term="l2"):
@@ -154,10 +160,11 @@ class TGVState:
 
 
 def tgv2(f, iterations, lam, alpha0, alpha1, tau, sigma, theta=1.0, data_term="l2",
-         dtype=np.float64):
-    """Run the reference loop from the reference initialisation; returns the state."""
+         dtype=np.float64, mask=None):
+    """Run the reference loop from the reference initialisation; returns the state.
+    With a mask: the masked loop (updates only where mask >= 1)."""
     st = TGVState(f, dtype)
-    st.iterate(iterations, lam, alpha0, alpha1, tau, sigma, theta, data_term)
+    st.iterate(iterations, lam, alpha0, alpha1, tau, sigma, theta, data_term, mask)
     return st
 
 
@@ -206,7 +213,7 @@ def integrate_poisson(vx, vy, vz, index_quirk=False, dtype=np.float64):
 
 
 def deshade_epilogue(u, l, mask=None, mask_shading=False, set_negative_zero=True,
-                     add_background_back=False):
+                     add_background_back=False, mask_again=False):
     """Returns (r, l_out) in FP32 (reference epilogue op order)."""
     u = _c(u, np.float32)
     l = _c(l, np.float32).copy()
@@ -214,7 +221,8 @@ def deshade_epilogue(u, l, mask=None, mask_shading=False, set_negative_zero=True
     m = _c(mask, np.float32) if mask is not None else None
     lib().orc_deshade_epilogue_f32(_p(u), _p(l), _p(m) if m is not None else None, _p(r),
                                    ctypes.c_longlong(u.size), int(mask_shading),
-                                   int(set_negative_zero), int(add_background_back))
+                                   int(set_negative_zero), int(add_background_back),
+                                   int(mask_again))
     return r, l
 
 
@@ -226,4 +234,19 @@ def deshade(f, iterations, lam, alpha0, alpha1, tau, sigma, theta=1.0, data_term
     st = tgv2(f, iterations, lam, alpha0, alpha1, tau, sigma, theta, data_term, dtype)
     l = integrate_poisson(st.v[0], st.v[1], st.v[2], index_quirk, dtype)
     r, l2 = deshade_epilogue(st.u, l, mask, False, set_negative_zero, False)
+    return st.u.copy(), l2, r, st
+
+
+def deshade_masked(f, iterations, lam, alpha0, alpha1, tau, sigma, mask, theta=1.0,
+                   set_negative_zero=True, add_background_back=False, index_quirk=False,
+                   dtype=np.float32):
+    """Masked deshade composition (TGVDeshadeMaskedProcessor.cpp:76-210): the
+    masked TGV2-L1 loop on the unmasked f (updates only where mask >= 1), the
+    Poisson integration of v, then l and r times the mask, clamp, and either
+    the add-back of u*(1 - mask) or r times the mask once more.
+    Returns (u, l, r, state)."""
+    st = tgv2(f, iterations, lam, alpha0, alpha1, tau, sigma, theta, "l1", dtype, mask=mask)
+    l = integrate_poisson(st.v[0], st.v[1], st.v[2], index_quirk, dtype)
+    r, l2 = deshade_epilogue(st.u, l, mask, True, set_negative_zero, add_background_back,
+                             mask_again=True)
     return st.u.copy(), l2, r, st

@@ -125,9 +125,14 @@ void tgv2_init(State<T>& st, const T* f, const Dims& s) {
 // One iteration of the reference loop, tgv_deshade/tgv2_l1_deshade.cu:169-264
 // (and tgv_cuda/tgv2_l2.cu:194-… with the L2 u-update), kernel by kernel,
 // with the reference's temporaries.
+// active (may be null): masked deshade, reference tgv_deshade_masked/tgv2_l1_deshade_masked.cu:161-272
+// - the update kernels run only over in-mask voxels (mask >= 1,
+// TGVDeshadeMaskedProcessor.cpp:337-372); the stencils are the unmasked ones
+// (the edge lists are the volume faces), so out-of-mask voxels keep their
+// initial values and are read as neighbours.
 template <typename T>
 void tgv2_iteration(State<T>& st, const T* f, const Dims& s, const Params& prm,
-                    std::vector<T>* tmp /* 9 arrays of n */) {
+                    std::vector<T>* tmp /* 9 arrays of n */, const unsigned char* active) {
     const i64 n = s.n();
     const T tau = T(prm.tau), sigma = T(prm.sigma), theta = T(prm.theta);
     const T lambda = T(prm.lambda), alpha0 = T(prm.alpha0), alpha1 = T(prm.alpha1);
@@ -144,6 +149,7 @@ void tgv2_iteration(State<T>& st, const T* f, const Dims& s, const Params& prm,
     // A5 tgv_kernel_part22, tgv_cuda/tgv2_common.cu:96-131
 #pragma omp parallel for schedule(static)
     for (i64 i = 0; i < n; ++i) {
+        if (active && !active[i]) continue;
         T px = fma_t(sigma, gx[i] - st.vbar[0][i], st.p[0][i]);
         T py = fma_t(sigma, gy[i] - st.vbar[1][i], st.p[1][i]);
         T pz = fma_t(sigma, gz[i] - st.vbar[2][i], st.p[2][i]);
@@ -166,6 +172,7 @@ void tgv2_iteration(State<T>& st, const T* f, const Dims& s, const Params& prm,
     const T tl = tau * lambda;
 #pragma omp parallel for schedule(static)
     for (i64 i = 0; i < n; ++i) {
+        if (active && !active[i]) continue;
         T div = (s.d > 1 || prm.data_term == 0) ? (gx[i] + gy[i]) + gz[i] : gx[i] + gy[i];
         T u = fma_t(-tau, div, st.u[i]);
         if (prm.data_term == 0) {
@@ -206,6 +213,7 @@ void tgv2_iteration(State<T>& st, const T* f, const Dims& s, const Params& prm,
     // (caller swaps the argument names, tgv2_l1_deshade.cu:224-232)
 #pragma omp parallel for schedule(static)
     for (i64 i = 0; i < n; ++i) {
+        if (active && !active[i]) continue;
         T qxx = fma_t(sigma, e[0][i], st.q[0][i]);
         T qyy = fma_t(sigma, e[1][i], st.q[1][i]);
         T qxy = fma_t(sigma, e[3][i], st.q[3][i]);
@@ -254,6 +262,7 @@ void tgv2_iteration(State<T>& st, const T* f, const Dims& s, const Params& prm,
     // A12 tgv_kernel_part6, tgv_cuda/tgv2_common.cu:324-354
 #pragma omp parallel for schedule(static)
     for (i64 i = 0; i < n; ++i) {
+        if (active && !active[i]) continue;
         for (int c = 0; c < 3; ++c) {
             const T vn = fma_t(-tau, w[c][i] - st.p[c][i], st.v[c][i]);
             st.v[c][i] = vn;
@@ -496,13 +505,13 @@ ORC_EXPORT void orc_backward_difference_##SUF(const T* a, T* out, i64 w, i64 h, 
 ORC_EXPORT void orc_tgv2_init_##SUF(T** fields, const T* f, i64 w, i64 h, i64 d) {           \
     State<T> st; fill_state_pointers(st, fields); tgv2_init(st, f, Dims{w, h, d}); }           \
 ORC_EXPORT int orc_tgv2_iterate_##SUF(T** fields, const T* f, i64 w, i64 h, i64 d,            \
-        const double* prm6, int data_term, int iterations) {                                    \
+        const double* prm6, int data_term, int iterations, const unsigned char* active) {       \
     State<T> st; fill_state_pointers(st, fields);                                               \
     Params prm{prm6[0], prm6[1], prm6[2], prm6[3], prm6[4], prm6[5], data_term};                \
     const Dims s{w, h, d};                                                                      \
     std::vector<T> tmp[9];                                                                      \
     for (auto& t : tmp) t.assign(size_t(s.n()), T(0));                                          \
-    for (int it = 0; it < iterations; ++it) tgv2_iteration(st, f, s, prm, tmp);                 \
+    for (int it = 0; it < iterations; ++it) tgv2_iteration(st, f, s, prm, tmp, active);         \
     return iterations; }                                                                        \
 ORC_EXPORT void orc_divergence_##SUF(const T* vx, const T* vy, const T* vz, T* out,           \
         i64 w, i64 h, i64 d) { divergence<T>(vx, vy, vz, out, Dims{w, h, d}); }               \
@@ -534,9 +543,12 @@ ORC_EXPORT void orc_set_num_threads(int n) { omp_set_num_threads(n); }
 // l and r; optional add-back of u*(1-mask)).
 //   r = u - l ; if mask: [l *= m if mask_shading], r *= m ; if clamp: r<0 -> 0 ;
 //   if add_background_back and mask: r = r + u*(1 - m)
+//   else if mask_again and mask:     r = r * m
+// mask_again is the masked processor's last step without add-back
+// (tgv_deshade_masked/TGVDeshadeMaskedProcessor.cpp:201-209).
 ORC_EXPORT void orc_deshade_epilogue_f32(const float* u, float* l, const float* mask, float* r,
                                          i64 n, int mask_shading, int set_negative_zero,
-                                         int add_background_back) {
+                                         int add_background_back, int mask_again) {
 #pragma omp parallel for schedule(static)
     for (i64 i = 0; i < n; ++i) {
         float ri = u[i] - l[i];
@@ -548,6 +560,8 @@ ORC_EXPORT void orc_deshade_epilogue_f32(const float* u, float* l, const float* 
         if (add_background_back && mask) {
             const float bg = float(1.0 - double(mask[i]));
             ri = ri + u[i] * bg;
+        } else if (mask_again && mask) {
+            ri = ri * mask[i];
         }
         r[i] = ri;
     }

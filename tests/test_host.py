@@ -86,3 +86,35 @@ def test_context_without_gpu_raises():
     from ambient_image_processor_amd import native
     with pytest.raises(native.AmbError):
         native.Context(0)
+
+
+def test_build_mask_indices_matches_brute_force():
+    """TGVDeshadeMaskedProcessor.buildMaskIndices mirror (reference
+    TGVDeshadeMaskedProcessor.cpp:337-385) against a per-voxel loop."""
+    from ambient_image_processor_amd import processor
+    rng = np.random.default_rng(0)
+    d, h, w = 4, 5, 6
+    m = (rng.random((d, h, w)) * 1.6).astype(np.float32)
+    got = processor.TGVDeshadeMaskedProcessor.buildMaskIndices(m)
+    want = [[] for _ in range(13)]
+    for z in range(d):
+        for y in range(h):
+            for x in range(w):
+                if m[z, y, x] < 1:
+                    continue
+                i = (z * h + y) * w + x
+                want[0].append(i)
+                for k, edge in enumerate((x == 0, x == w - 1, y == 0, y == h - 1, z == 0,
+                                          z == d - 1)):
+                    want[1 + 2 * k + (0 if edge else 1)].append(i)
+    assert len(got) == 13
+    for g, ww in zip(got, want):
+        assert g.dtype == np.uint32
+        assert g.tolist() == ww
+
+
+def test_masked_launch_rejects_out_of_range_index():
+    from ambient_image_processor_amd import processor
+    f = np.zeros(2 * 3 * 4, np.float32)
+    with pytest.raises(IndexError):
+        processor.tgv2_l1_deshade_masked_launch(f, 4, 3, 2, 1.0, 1, 0, -1, None, 2.0, 1.0, [24])

@@ -315,3 +315,87 @@ def test_energy_transpose_terms_by_adjoint_identity(orc):
                    0.5 * (B(vv[2], 0) + B(vv[0], 2)), 0.5 * (B(vv[2], 1) + B(vv[1], 2))]
             tq[c, j] = sum(w[k] * np.sum(eps[k] * q[k]) for k in range(6))
     assert abs(e[9] - np.sqrt((tq ** 2).sum())) < 1e-10 * np.sqrt((tq ** 2).sum())
+
+
+# ---------------------------------------------------------------------------
+# masked deshade (next row: reference tgv_deshade_masked)
+# ---------------------------------------------------------------------------
+def _masked_numpy_iteration(f, mask, iters, lam, alpha0, alpha1, tau, sigma, theta):
+    """Vectorised restatement of the masked loop: the unmasked update, applied
+    only where mask >= 1; everything else keeps its initial value."""
+    import oracle.adjoint as A
+    act = mask >= 1
+    X, Y, Z = 2, 1, 0
+    u = f.copy(); ub = f.copy()
+    v = np.zeros((3,) + f.shape); vb = v.copy(); p = v.copy(); q = np.zeros((6,) + f.shape)
+    for _ in range(iters):
+        u_old, v_old = u.copy(), v.copy()
+        pn = p + sigma * (np.stack([A.dplus(ub, X), A.dplus(ub, Y), A.dplus(ub, Z)]) - vb)
+        pn = pn / np.maximum(1, np.sqrt((pn ** 2).sum(0)) / alpha1)
+        qn = q + sigma * A.eps(vb)
+        nq = np.sqrt(qn[0] ** 2 + qn[1] ** 2 + qn[2] ** 2 + 2 * (qn[3] ** 2 + qn[4] ** 2 + qn[5] ** 2))
+        qn = qn / np.maximum(1, nq / alpha0)
+        p = np.where(act, pn, p)
+        q = np.where(act, qn, q)
+        un = u - tau * (A.bm(p[0], X) + A.bm(p[1], Y) + A.bm(p[2], Z))
+        d = un - f
+        un = np.where(d > tau * lam, un - tau * lam, np.where(d < -tau * lam, un + tau * lam, f))
+        u = np.where(act, un, u)
+        ub = np.where(act, u + theta * (u - u_old), ub)
+        vn = v - tau * (A.div2(q) - p)
+        v = np.where(act, vn, v)
+        vb = np.where(act, v + theta * (v - v_old), vb)
+    return u, v, p, q
+
+
+def test_masked_iteration(orc):
+    f = _phantom((6, 7, 9))
+    rng = np.random.default_rng(2)
+    mask = (rng.random(f.shape) > 0.35).astype(np.float64) * 1.0
+    mask[0, 0, 0] = 0.7          # below 1: outside
+    prm = dict(PRM, alpha0=0.05, alpha1=0.02)
+    st = orc.tgv2(f, 20, data_term="l1", mask=mask, **prm)
+    u, v, p, q = _masked_numpy_iteration(f, mask, 20, **prm)
+    assert rel_err(st.u, u) < 1e-13 and rel_err(st.v, v) < 1e-12
+    assert rel_err(st.p, p) < 1e-12 and rel_err(st.q, q) < 1e-12
+    out = mask < 1
+    np.testing.assert_array_equal(st.u[out], f[out])
+    assert not st.v[:, out].any() and not st.p[:, out].any() and not st.q[:, out].any()
+    ones = orc.tgv2(f, 20, data_term="l1", mask=np.ones_like(f), dtype=np.float32, **prm)
+    plain = orc.tgv2(f, 20, data_term="l1", dtype=np.float32, **prm)
+    np.testing.assert_array_equal(ones.fields, plain.fields)
+
+
+def test_deshade_epilogue_special_cases(orc):
+    """The epilogue (TGVDeshadeProcessor.cpp:83-93, TGVDeshadeMaskedProcessor.cpp:
+    119-135, 201-209) on masks whose result is fixed without the formula:
+    no mask -> u - l (clamped: max(u - l, 0)); all-ones mask -> as no mask;
+    zero mask -> 0, or u with the add-back; a binary mask makes the masked
+    processor's second multiply a no-op; a fractional one scales r twice."""
+    rng = np.random.default_rng(4)
+    u = rng.standard_normal((3, 4, 5)).astype(np.float32)
+    l = rng.standard_normal((3, 4, 5)).astype(np.float32)
+    d = u - l
+    r, lo = orc.deshade_epilogue(u, l, None, set_negative_zero=False)
+    np.testing.assert_array_equal(r, d)
+    np.testing.assert_array_equal(lo, l)
+    r, _ = orc.deshade_epilogue(u, l, None, set_negative_zero=True)
+    np.testing.assert_array_equal(r, np.maximum(d, 0))
+    ones, zeros = np.ones_like(u), np.zeros_like(u)
+    for kw in (dict(), dict(mask_shading=True, mask_again=True),
+               dict(mask_shading=True, add_background_back=True)):
+        r, lo = orc.deshade_epilogue(u, l, ones, set_negative_zero=True, **kw)
+        np.testing.assert_array_equal(r, np.maximum(d, 0))
+        np.testing.assert_array_equal(lo, l)
+    r, lo = orc.deshade_epilogue(u, l, zeros, True, True, add_background_back=False,
+                                 mask_again=True)
+    assert np.all(r == 0) and np.all(lo == 0)
+    r, _ = orc.deshade_epilogue(u, l, zeros, True, True, add_background_back=True)
+    np.testing.assert_array_equal(r, u)
+    binary = (rng.random(u.shape) > 0.5).astype(np.float32)
+    a, _ = orc.deshade_epilogue(u, l, binary, True, True, mask_again=False)
+    b, _ = orc.deshade_epilogue(u, l, binary, True, True, mask_again=True)
+    np.testing.assert_array_equal(a, b)
+    half = np.full_like(u, 0.5)
+    r, _ = orc.deshade_epilogue(u, l, half, True, False, mask_again=True)
+    np.testing.assert_allclose(r, d * 0.25, rtol=1e-6, atol=1e-7)
