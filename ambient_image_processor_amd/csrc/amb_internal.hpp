@@ -38,6 +38,8 @@ struct StoreF32 {
     static __device__ __forceinline__ float ld(const float* p, Idx i) { return *elem(p, i); }
     template <class Idx>
     static __device__ __forceinline__ void st(float* p, Idx i, float v) { *elem(p, i) = v; }
+    // value after a store/load round trip
+    static __device__ __forceinline__ float rt(float v) { return v; }
 };
 
 struct StoreBF16 {
@@ -54,6 +56,7 @@ struct StoreBF16 {
     }
     template <class Idx>
     static __device__ __forceinline__ void st(uint16_t* p, Idx i, float v) { *elem(p, i) = rne(v); }
+    static __device__ __forceinline__ float rt(float v) { return __uint_as_float(uint32_t(rne(v)) << 16); }
 };
 
 // Field indices inside the 17-field state block (C ABI order).
@@ -99,6 +102,23 @@ constexpr int TILE_VECTOR_DEFAULT = 4;
 // Launchers (stream-ordered, asynchronous).  `fields` points at 17 arrays of
 // the storage type (void* so both storages share the signature).
 hipError_t launch_init(hipStream_t s, int storage, void* const* fields, const float* f, Geometry g);
+// Fused sweep (primal half of iteration k + dual half of iteration k + 1):
+// reads u, v, p, q through the 17-field table `src` and writes them to the 13
+// fields `dst13` (order u, vx, vy, vz, px, py, pz, qxx, qyy, qzz, qxy, qxz,
+// qyz); u_bar and v_bar are neither read nor written.  Needs W % 4 == 0.
+struct FusedPlan {
+    int ty;       // owned y rows per block
+    int zchunk;   // planes per block
+    int pf;       // 1: next-plane loads issued one step ahead
+};
+bool fused_plan(const Geometry& g, int device, FusedPlan* plan);
+size_t fused_lds_bytes(const Geometry& g, const FusedPlan& plan);
+hipError_t launch_fused_sweep(hipStream_t s, int storage, void* const* src, void* const* dst13,
+                              const float* f, Geometry g, const SweepParams& p,
+                              const FusedPlan& plan);
+constexpr int N_FUSED_FIELDS = 13;
+extern const int FUSED_FIELD_MAP[N_FUSED_FIELDS];
+
 hipError_t launch_dual_sweep(hipStream_t s, int storage, void* const* fields, Geometry g,
                              const SweepParams& p);
 hipError_t launch_primal_sweep(hipStream_t s, int storage, void* const* fields, const float* f,

@@ -17,6 +17,7 @@
 #include <algorithm>
 #include <cstdlib>
 #include <string>
+#include <utility>
 #include <vector>
 
 using namespace amb;
@@ -51,9 +52,9 @@ amb_status fail_hip(hipError_t e, const char* what, int line) {
     } while (0)
 
 constexpr int N_SCRATCH = 5;
-constexpr int N_PROF = 9;
+constexpr int N_PROF = 10;
 enum { P_DUAL = 0, P_PRIMAL = 1, P_DCT = 2, P_POISSON = 3, P_DIV = 4, P_EPI = 5, P_INIT = 6,
-       P_ENERGY = 7, P_ADJOINT = 8 };
+       P_ENERGY = 7, P_ADJOINT = 8, P_FUSED = 9 };
 
 struct AxisTables {
     int n = 0;
@@ -107,6 +108,12 @@ struct amb_context {
     // masked iteration: one byte per voxel, nonzero = update (grow-only)
     uint8_t* act = nullptr;
     size_t act_cap = 0;
+    // fused sweep: second copy of u, v, p, q (grow-only) and the launch plan
+    char* alt_block = nullptr;
+    size_t alt_cap = 0;
+    void* alt[N_FUSED_FIELDS] = {};
+    bool fused_ok = false;
+    FusedPlan fplan{0, 0};
 };
 
 namespace {
@@ -415,7 +422,45 @@ amb_status begin(amb_context* c, amb_memory mem, const float* f, uint32_t w, uin
     c->sp.tile = choose_tile(g);
     c->sp.zchunk = choose_zchunk(g, c->sp.tile, int(p->storage));
     c->sp.active = nullptr;
+    c->fused_ok = fused_plan(g, c->device, &c->fplan);
     c->have_state = true;
+    return AMB_OK;
+}
+
+// Second copy of the 13 ping-ponged fields for the fused sweep.  Field k of
+// the state lives either in the state block or in the second block; c->alt[k]
+// is set to the other one.
+amb_status ensure_alt(amb_context* c) {
+    const size_t elem = c->storage == 1 ? 2 : 4;
+    const size_t fstride = align_up(c->g.n() * elem, 256);
+    const size_t need = fstride * N_FUSED_FIELDS;
+    auto main_at = [&](int k) { return static_cast<void*>(c->block + size_t(FUSED_FIELD_MAP[k]) * fstride); };
+    if (need > c->alt_cap) {
+        for (int k = 0; k < N_FUSED_FIELDS; ++k)   // state must be in the state block
+            if (c->fields[FUSED_FIELD_MAP[k]] != main_at(k))
+                return fail(AMB_ERR_NOT_SUPPORTED, "fused-sweep buffers out of step with the state");
+        if (c->alt_block) AMB_HIP(hipFree(c->alt_block));
+        c->alt_block = nullptr;
+        c->alt_cap = 0;
+        AMB_HIP(hipMalloc(reinterpret_cast<void**>(&c->alt_block), need));
+        c->alt_cap = need;
+    }
+    for (int k = 0; k < N_FUSED_FIELDS; ++k) {
+        void* a = c->alt_block + size_t(k) * fstride;
+        c->alt[k] = c->fields[FUSED_FIELD_MAP[k]] == main_at(k) ? a : main_at(k);
+    }
+    return AMB_OK;
+}
+
+// Primal half of iteration k + dual half of iteration k + 1 into the second
+// copy, then swap the copies (the state table points at the new values).
+amb_status fused_step(amb_context* c) {
+    {
+        ProfScope ps(c, P_FUSED);
+        AMB_HIP(launch_fused_sweep(c->stream, c->storage, c->fields, c->alt, c->f, c->g, c->sp,
+                                   c->fplan));
+    }
+    for (int k = 0; k < N_FUSED_FIELDS; ++k) std::swap(c->fields[FUSED_FIELD_MAP[k]], c->alt[k]);
     return AMB_OK;
 }
 
@@ -458,12 +503,29 @@ amb_status sweep(amb_context* c, int which) {
 
 // The reference loop with its callback placement (tgv2_l1_deshade.cu:169-272,
 // callback rule tgv2_common.cu:381-383).
+//
+// With the fused sweep the loop runs dual_0, fused(primal_k + dual_k+1) for
+// k = 0 .. N - 2, primal_N-1: the same updates in the same order.  An iteration
+// whose end is observed by the callback ends with the primal sweep alone (the
+// next one then starts with the dual sweep), so the callback and an early stop
+// see exactly the state of the two-sweep loop.  u_bar and v_bar are stored
+// only by the primal sweep; the last iteration is always a primal sweep.
 amb_status run(amb_context* c, uint32_t iterations, uint32_t interval, amb_iteration_callback cb,
                void* user, bool with_v) {
+    const bool fused = c->fused_ok && iterations > 1;
+    if (fused) AMB_TRY(ensure_alt(c));
+    bool need_dual = true;
     for (uint32_t it = 0; it < iterations; ++it) {
-        AMB_TRY(sweep(c, 0));
-        AMB_TRY(sweep(c, 1));
-        if (cb && interval > 0 && it > 0 && it % interval == 0) {
+        const bool observed = cb && interval > 0 && it > 0 && it % interval == 0;
+        if (need_dual) AMB_TRY(sweep(c, 0));
+        if (fused && !observed && it + 1 < iterations) {
+            AMB_TRY(fused_step(c));
+            need_dual = false;
+        } else {
+            AMB_TRY(sweep(c, 1));
+            need_dual = true;
+        }
+        if (observed) {
             const size_t n = c->g.n();
             c->cb_u.resize(n);
             AMB_TRY(copy_field_out(c, AMB_MEM_HOST, c->cb_u.data(), F_U));
@@ -647,6 +709,7 @@ AMB_API amb_status amb_context_destroy(amb_context* c) {
     if (c->energy_out) (void)hipFree(c->energy_out);
     if (c->adj_block) (void)hipFree(c->adj_block);
     if (c->act) (void)hipFree(c->act);
+    if (c->alt_block) (void)hipFree(c->alt_block);
     (void)hipStreamDestroy(c->stream);
     delete c;
     return AMB_OK;
@@ -953,6 +1016,10 @@ AMB_API amb_status amb_tgv2_energy(amb_context* c, double out[10]) {
     if (!c || !out) return fail(AMB_ERR_INVALID_ARGUMENT, "NULL argument");
     if (!c->have_state) return fail(AMB_ERR_INVALID_ARGUMENT, "no iteration state");
     AMB_TRY(set_device(c));
+    // the field table moves with the fused sweep's buffer swaps: refresh the
+    // device copy here (c->fields does not change before the sync below)
+    AMB_HIP(hipMemcpyAsync(c->fields_dev, c->fields, sizeof(void*) * N_FIELDS,
+                           hipMemcpyHostToDevice, c->stream));
     {
         ProfScope ps(c, P_ENERGY);
         AMB_HIP(launch_energy(c->stream, c->storage, c->fields_dev, c->f, c->g, c->params.lambda,

@@ -30,6 +30,7 @@
 // -ffp-contract=off).
 #include "amb_internal.hpp"
 
+#include <algorithm>
 #include <cstdlib>
 
 namespace amb {
@@ -804,6 +805,354 @@ __global__ __launch_bounds__(TX * TY) void primal_sweep_vec_kernel(
     }
 }
 
+// ---------------------------------------------------------------------------
+// Fused sweep: the primal half of iteration k and the dual half of iteration
+// k + 1 in one z-march (same per-voxel arithmetic as the two sweeps above).
+//
+// The dual update at a voxel reads u_bar / v_bar of its x, y, z neighbours,
+// i.e. primal results of neighbouring voxels.  A block therefore owns whole
+// x rows and a band of `ty` y rows, computes the primal half on one extra row
+// above and below the band (and one extra plane before and after its z
+// chunk), and runs the dual half one plane behind the primal half.  u_bar and
+// v_bar never leave the CU.  Neighbouring blocks recompute the halo rows from
+// the same inputs, so u, v, p, q are read from one copy of the state (src)
+// and written to another (dst).  Per voxel-iteration: reads u, f, v x3, p x3,
+// q x6 = 14, writes u, v x3, p x3, q x6 = 13: 27 element accesses (108 B FP32).
+//
+// Threads: blockDim.x = L * (ty + 2) with L = W / VX lanes per row; thread row
+// r holds y = y0 - 1 + r.  Rows 1..ty are owned; row 0 only feeds v_bar
+// (its u half is not needed), row ty + 1 only u_bar.  Step s computes the
+// primal half at plane s and the dual half at plane s - 1 for s = z0 - 1 ..
+// z1 (plane z0 - 1 needs only v_bar, plane z1 only u_bar).
+//
+// LDS (dynamic, floats): P region for the primal half's x/y neighbours of
+// p and q at plane s, D region for the dual half's x/y neighbours of u_bar and
+// v_bar at plane s - 1; rows of W floats plus one edge float per lane.
+// ---------------------------------------------------------------------------
+enum FusedField { G_U = 0, G_VX, G_VY, G_VZ, G_PX, G_PY, G_PZ, G_QXX, G_QYY, G_QZZ, G_QXY,
+                  G_QXZ, G_QYZ, N_GFIELDS };
+
+struct FusedFields {
+    const void* src[N_GFIELDS];
+    void* dst[N_GFIELDS];
+};
+
+constexpr int FUSED_MAX_THREADS = 768;
+
+template <int VX>
+inline size_t fused_lds_floats(int W, int ty) {
+    const int L = W / VX, RT = ty + 2;
+    return size_t(8) * RT * W + size_t(8) * RT * (L + 1);
+}
+
+template <int VX>
+struct FusedPlane {
+    float px[VX], py[VX], pz[VX], qxx[VX], qxy[VX], qyy[VX], u[VX], f[VX], v[3][VX];
+    float qnn[3][VX];     // q_zz, q_xz, q_yz at the next plane
+    uint32_t act;
+};
+
+// PF = 1 loads plane s + 1 right after the first barrier of step s, so the
+// loads are in flight during both halves of step s.
+template <class S, class I, int VX, int L2, int MK, int MAXT, int PF>
+__global__ __launch_bounds__(MAXT) void fused_sweep_kernel(
+    FusedFields ff, const float* __restrict__ f_, const uint8_t* __restrict__ act, Geometry g,
+    float tau, float sigma, float lambda, float alpha0, float alpha1, float theta, int ty,
+    int zchunk) {
+    typedef typename S::T T;
+    typedef VecIO<S, VX> V;
+    typedef VecIO<StoreF32, VX> VF;
+    extern __shared__ __align__(16) float lds[];
+
+    const int W = g.w, H = g.h, D = g.d;
+    const int L = W / VX, RT = ty + 2;
+    const int lane = int(threadIdx.x) % L, r = int(threadIdx.x) / L;
+    const int nb = (H + ty - 1) / ty;
+    const int t = xcd_tile(blockIdx.x, gridDim.x);
+    const int y0 = (t % nb) * ty;
+    const int z0 = (t / nb) * zchunk, z1 = min(z0 + zchunk, D);
+    const int y = y0 - 1 + r;
+    const int xb = lane * VX;
+    const bool rowin = y >= 0 && y < H;
+    const bool own = rowin && r >= 1 && r <= ty;
+    const I plane = I(W) * I(H);
+    const I col = rowin ? I(y) * I(W) + I(xb) : I(0);
+    const bool use_bz = !L2 || D > 1;
+    const float tl = tau * lambda;
+
+    // LDS carve-up
+    const int RW = RT * W, RE = RT * (L + 1);
+    float* const sPY = lds;               // P region: p_y, q_xy, q_yy, q_yz rows
+    float* const sQXY = sPY + RW;
+    float* const sQYY = sQXY + RW;
+    float* const sQYZ = sQYY + RW;
+    float* const sPXm = sQYZ + RW;        // p_x element VX-1 of lane at [r][lane+1]
+    float* const sQXXp = sPXm + RE;       // q_xx element 0 of lane at [r][lane]
+    float* const sQXYp = sQXXp + RE;
+    float* const sQXZp = sQXYp + RE;
+    float* const sUB = sQXZp + RE;        // D region: u_bar, v_bar x3 rows
+    float* const sVB = sUB + RW;          // [3][RT][W]
+    float* const sUBp = sVB + 3 * RW;     // u_bar element 0 at [r][lane]
+    float* const sVBm = sUBp + RE;        // [3][RT][L+1]: v_bar element VX-1 at [r][lane+1]
+
+    // zero everything once: edge slots at x = -1 / x = W and rows outside the
+    // volume are never written afterwards and read only under false flags
+    for (int k = int(threadIdx.x); k < 8 * RW + 8 * RE; k += int(blockDim.x)) lds[k] = 0.f;
+    __syncthreads();
+
+    const T* const* src = reinterpret_cast<const T* const*>(ff.src);
+    T* const* dst = reinterpret_cast<T* const*>(ff.dst);
+    const int rw = r * W + xb;            // this thread's offset in a row array
+    const int re = r * (L + 1) + lane;    // this thread's edge slot
+
+    float p_c[3][VX], q_c[6][VX];         // p, q at plane s - 1 (dual base; p_z for B_z)
+    float qn[3][VX];                      // q_zz, q_xz, q_yz at plane s
+    float vb_m2[3][VX];                   // v_bar at plane s - 2
+    uint32_t act_c = 0xffffffffu;         // active bytes at plane s - 1
+#pragma unroll
+    for (int k = 0; k < 3; ++k) { zero(p_c[k]); zero(qn[k]); zero(vb_m2[k]); }
+#pragma unroll
+    for (int k = 0; k < 6; ++k) zero(q_c[k]);
+
+    const int zs = z0 > 0 ? z0 - 1 : 0;
+    if (rowin && zs < D) {
+        const I i = I(zs) * plane + col;
+        V::ld(src[G_QZZ], i, qn[0]); V::ld(src[G_QXZ], i, qn[1]); V::ld(src[G_QYZ], i, qn[2]);
+    }
+
+    auto load_plane = [&](int zz, FusedPlane<VX>& P) {
+        P.act = 0xffffffffu;
+        if (rowin && zz < D) {
+            const I i = I(zz) * plane + col;
+            V::ld(src[G_PX], i, P.px); V::ld(src[G_PY], i, P.py); V::ld(src[G_PZ], i, P.pz);
+            V::ld(src[G_QXX], i, P.qxx); V::ld(src[G_QXY], i, P.qxy); V::ld(src[G_QYY], i, P.qyy);
+            V::ld(src[G_U], i, P.u); VF::ld(f_, i, P.f);
+            V::ld(src[G_VX], i, P.v[0]); V::ld(src[G_VY], i, P.v[1]); V::ld(src[G_VZ], i, P.v[2]);
+            if (zz + 1 < D) {
+                const I j = i + plane;
+                V::ld(src[G_QZZ], j, P.qnn[0]); V::ld(src[G_QXZ], j, P.qnn[1]);
+                V::ld(src[G_QYZ], j, P.qnn[2]);
+            } else {
+                zero(P.qnn[0]); zero(P.qnn[1]); zero(P.qnn[2]);
+            }
+            if (MK) P.act = *reinterpret_cast<const uint32_t*>(elem(act, i));
+        } else {
+            zero(P.px); zero(P.py); zero(P.pz); zero(P.qxx); zero(P.qxy); zero(P.qyy);
+            zero(P.u); zero(P.f);
+#pragma unroll
+            for (int k = 0; k < 3; ++k) { zero(P.v[k]); zero(P.qnn[k]); }
+        }
+    };
+
+    FusedPlane<VX> P;
+    if (PF) load_plane(zs, P);
+
+    for (int s = zs; s <= z1; ++s) {
+        const bool pin = s < D;                       // plane s exists
+        const I zo = I(s) * plane;
+        const I i = zo + col;
+        if (!PF) load_plane(s, P);
+        float (&px)[VX] = P.px;
+        float (&py)[VX] = P.py;
+        float (&pz)[VX] = P.pz;
+        float (&qxx)[VX] = P.qxx;
+        float (&qxy)[VX] = P.qxy;
+        float (&qyy)[VX] = P.qyy;
+        float (&u)[VX] = P.u;
+        float (&fv)[VX] = P.f;
+        float (&v)[3][VX] = P.v;
+        float (&qnn)[3][VX] = P.qnn;
+        const uint32_t actv = P.act;
+
+        // P region: plane s
+        lds_put<VX>(&sPY[rw], py);
+        lds_put<VX>(&sQXY[rw], qxy);
+        lds_put<VX>(&sQYY[rw], qyy);
+        lds_put<VX>(&sQYZ[rw], qn[2]);
+        sPXm[re + 1] = px[VX - 1];
+        sQXXp[re] = qxx[0];
+        sQXYp[re] = qxy[0];
+        sQXZp[re] = qn[1][0];
+        __syncthreads();
+
+        FusedPlane<VX> Pn;
+        if (PF && s < z1) load_plane(s + 1, Pn);
+
+        // ---- primal half at plane s
+        const bool need_u = rowin && pin && r > 0 && s >= z0;
+        const bool need_v = rowin && pin && r < RT - 1 && s < z1;
+        float ubn[VX], vbn[3][VX];
+        zero(ubn);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) zero(vbn[k]);
+        float uo[VX], vo[3][VX];
+        if (need_u) {
+            float pym[VX];
+            lds_get<VX>(&sPY[rw - W], pym);
+            const float pxm = sPXm[re];
+#pragma unroll
+            for (int e = 0; e < VX; ++e) {
+                const int x = xb + e;
+                const float bx = bwd(px[e], e > 0 ? px[e > 0 ? e - 1 : 0] : pxm, x > 0);
+                const float by = bwd(py[e], pym[e], y > 0);
+                const float bz = bwd(pz[e], p_c[2][e], s > 0);
+                const float div = use_bz ? (bx + by) + bz : bx + by;
+                const float uc = u[e], fc = fv[e];
+                float un = fmaf(-tau, div, uc);
+                if (L2) {
+                    un = fmaf(tl, fc, un) / (1.0f + tl);
+                } else {
+                    const float umf = un - fc;
+                    if (umf > tl) un = un - tl;
+                    else if (umf < -tl) un = un + tl;
+                    else un = fc;
+                }
+                if (MK && !((actv >> (8 * e)) & 0xffu)) un = uc;
+                uo[e] = un;
+                ubn[e] = S::rt(fmaf(theta, un - uc, un));
+            }
+        }
+        if (need_v) {
+            float qxyn_y[VX], qyyn_y[VX], qyzn_y[VX];
+            lds_get<VX>(&sQXY[rw + W], qxyn_y);
+            lds_get<VX>(&sQYY[rw + W], qyyn_y);
+            lds_get<VX>(&sQYZ[rw + W], qyzn_y);
+            const float qxxn_x = sQXXp[re + 1];
+            const float qxyn_x = sQXYp[re + 1];
+            const float qxzn_x = sQXZp[re + 1];
+#pragma unroll
+            for (int e = 0; e < VX; ++e) {
+                const int x = xb + e;
+                const int en = e + 1 < VX ? e + 1 : e;
+                const bool xl = e < VX - 1;
+                const bool xn = x < W - 1, yn = y < H - 1, zn = s < D - 1;
+                const float dxx = fwd(qxx[e], xl ? qxx[en] : qxxn_x, xn);
+                const float dyxy = fwd(qxy[e], qxyn_y[e], yn);
+                const float dzxz = fwd(qn[1][e], qnn[1][e], zn);
+                const float dyy = fwd(qyy[e], qyyn_y[e], yn);
+                const float dxxy = fwd(qxy[e], xl ? qxy[en] : qxyn_x, xn);
+                const float dzyz = fwd(qn[2][e], qnn[2][e], zn);
+                const float dzz = fwd(qn[0][e], qnn[0][e], zn);
+                const float dxxz = fwd(qn[1][e], xl ? qn[1][en] : qxzn_x, xn);
+                const float dyyz = fwd(qn[2][e], qyzn_y[e], yn);
+                const float w0 = (dxx + dyxy) + dzxz;
+                const float w1 = (dyy + dxxy) + dzyz;
+                const float w2 = (dzz + dxxz) + dyyz;
+                float vn0 = fmaf(-tau, w0 - px[e], v[0][e]);
+                float vn1 = fmaf(-tau, w1 - py[e], v[1][e]);
+                float vn2 = fmaf(-tau, w2 - pz[e], v[2][e]);
+                if (MK && !((actv >> (8 * e)) & 0xffu)) { vn0 = v[0][e]; vn1 = v[1][e]; vn2 = v[2][e]; }
+                vo[0][e] = vn0; vo[1][e] = vn1; vo[2][e] = vn2;
+                vbn[0][e] = S::rt(fmaf(theta, vn0 - v[0][e], vn0));
+                vbn[1][e] = S::rt(fmaf(theta, vn1 - v[1][e], vn1));
+                vbn[2][e] = S::rt(fmaf(theta, vn2 - v[2][e], vn2));
+            }
+        }
+        if (own && pin && s >= z0 && s < z1) {
+            V::st(dst[G_U], i, uo);
+            V::st(dst[G_VX], i, vo[0]); V::st(dst[G_VY], i, vo[1]); V::st(dst[G_VZ], i, vo[2]);
+        }
+
+        // ---- dual half at plane zd = s - 1 (D region holds u_bar, v_bar at zd)
+        const int zd = s - 1;
+        float vbc[3][VX];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) lds_get<VX>(&sVB[k * RW + rw], vbc[k]);
+        if (own && zd >= z0) {
+            float ubc[VX], uyn[VX], vym[3][VX], vxm[3];
+            lds_get<VX>(&sUB[rw], ubc);
+            lds_get<VX>(&sUB[rw + W], uyn);
+            const float uxn = sUBp[re + 1];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                lds_get<VX>(&sVB[k * RW + rw - W], vym[k]);
+                vxm[k] = sVBm[k * RE + re];
+            }
+            const I id = I(zd) * plane + col;
+            float po[3][VX], qo[6][VX];
+#pragma unroll
+            for (int e = 0; e < VX; ++e) {
+                const int x = xb + e;
+                const float ub = ubc[e];
+                const float dx = fwd(ub, e < VX - 1 ? ubc[e + 1 < VX ? e + 1 : e] : uxn, x < W - 1);
+                const float dy = fwd(ub, uyn[e], y < H - 1);
+                const float dz = fwd(ub, ubn[e], zd < D - 1);
+                const float ax = fmaf(sigma, dx - vbc[0][e], p_c[0][e]);
+                const float ay = fmaf(sigma, dy - vbc[1][e], p_c[1][e]);
+                const float az = fmaf(sigma, dz - vbc[2][e], p_c[2][e]);
+                float n2 = ax * ax;
+                n2 = fmaf(ay, ay, n2);
+                n2 = fmaf(az, az, n2);
+                const float tp = sqrtf(n2) / alpha1;
+                if (tp > 1.0f) {
+                    po[0][e] = ax / tp; po[1][e] = ay / tp; po[2][e] = az / tp;
+                } else {
+                    po[0][e] = ax; po[1][e] = ay; po[2][e] = az;
+                }
+                float bxv[3], byv[3], bzv[3];
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    const float vc = vbc[k][e];
+                    bxv[k] = bwd(vc, e > 0 ? vbc[k][e > 0 ? e - 1 : 0] : vxm[k], x > 0);
+                    byv[k] = bwd(vc, vym[k][e], y > 0);
+                    bzv[k] = bwd(vc, vb_m2[k][e], zd > 0);
+                }
+                const float exy = (bxv[1] + byv[0]) * 0.5f;
+                const float exz = (bxv[2] + bzv[0]) * 0.5f;
+                const float eyz = (byv[2] + bzv[1]) * 0.5f;
+                const float bxx = fmaf(sigma, bxv[0], q_c[0][e]);
+                const float byy = fmaf(sigma, byv[1], q_c[1][e]);
+                const float bxy = fmaf(sigma, exy, q_c[3][e]);
+                const float bzz = fmaf(sigma, bzv[2], q_c[2][e]);
+                const float bxz = fmaf(sigma, exz, q_c[4][e]);
+                const float byz = fmaf(sigma, eyz, q_c[5][e]);
+                float m2 = bxx * bxx;
+                m2 = fmaf(byy, byy, m2);
+                m2 = fmaf(2.0f * bxy, bxy, m2);
+                m2 = fmaf(bzz, bzz, m2);
+                m2 = fmaf(2.0f * bxz, bxz, m2);
+                m2 = fmaf(2.0f * byz, byz, m2);
+                const float tq = sqrtf(m2) / alpha0;
+                if (tq > 1.0f) {
+                    qo[0][e] = bxx / tq; qo[1][e] = byy / tq; qo[2][e] = bzz / tq;
+                    qo[3][e] = bxy / tq; qo[4][e] = bxz / tq; qo[5][e] = byz / tq;
+                } else {
+                    qo[0][e] = bxx; qo[1][e] = byy; qo[2][e] = bzz;
+                    qo[3][e] = bxy; qo[4][e] = bxz; qo[5][e] = byz;
+                }
+                if (MK && !((act_c >> (8 * e)) & 0xffu)) {
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) po[k][e] = p_c[k][e];
+#pragma unroll
+                    for (int k = 0; k < 6; ++k) qo[k][e] = q_c[k][e];
+                }
+            }
+            V::st(dst[G_PX], id, po[0]); V::st(dst[G_PY], id, po[1]); V::st(dst[G_PZ], id, po[2]);
+            V::st(dst[G_QXX], id, qo[0]); V::st(dst[G_QYY], id, qo[1]);
+            V::st(dst[G_QZZ], id, qo[2]); V::st(dst[G_QXY], id, qo[3]);
+            V::st(dst[G_QXZ], id, qo[4]); V::st(dst[G_QYZ], id, qo[5]);
+        }
+        __syncthreads();
+
+        // D region <- u_bar, v_bar at plane s; carry plane s to the next step
+        lds_put<VX>(&sUB[rw], ubn);
+        sUBp[re] = ubn[0];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            lds_put<VX>(&sVB[k * RW + rw], vbn[k]);
+            sVBm[k * RE + re + 1] = vbn[k][VX - 1];
+            copyv(vb_m2[k], vbc[k]);
+        }
+        copyv(p_c[0], px); copyv(p_c[1], py); copyv(p_c[2], pz);
+        copyv(q_c[0], qxx); copyv(q_c[1], qyy); copyv(q_c[2], qn[0]);
+        copyv(q_c[3], qxy); copyv(q_c[4], qn[1]); copyv(q_c[5], qn[2]);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) copyv(qn[k], qnn[k]);
+        act_c = actv;
+        if (PF && s < z1) P = Pn;
+    }
+}
+
 template <class S>
 __global__ void init_kernel(typename S::T* u, typename S::T* ubar, typename S::T* zero_base,
                             size_t zero_count, const float* f, size_t n) {
@@ -962,7 +1311,135 @@ hipError_t primal_tiles(hipStream_t s, void* const* fv, const float* f, Geometry
     }
 }
 
+template <class S, class I, int L2, int MK, int MAXT, int PF>
+hipError_t fused_launch(hipStream_t s, const FusedFields& ff, const float* f, Geometry g,
+                        const SweepParams& p, const FusedPlan& plan) {
+    const int L = g.w / 4, RT = plan.ty + 2;
+    const int nb = (g.h + plan.ty - 1) / plan.ty, nc = (g.d + plan.zchunk - 1) / plan.zchunk;
+    const size_t lds = fused_lds_floats<4>(g.w, plan.ty) * sizeof(float);
+    auto kern = fused_sweep_kernel<S, I, 4, L2, MK, MAXT, PF>;
+    static size_t lds_set = 0;   // per instantiation: raise the dynamic LDS limit once
+    if (lds > lds_set) {
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
+        if (e != hipSuccess) return e;
+        lds_set = lds;
+    }
+    hipLaunchKernelGGL(kern, dim3(nb * nc), dim3(L * RT), lds, s, ff, f, p.active, g, p.tau,
+                       p.sigma, p.lambda, p.alpha0, p.alpha1, p.theta, plan.ty, plan.zchunk);
+    return hipGetLastError();
+}
+
+// Blocks of up to 512 threads get the 256-VGPR budget of two waves per SIMD;
+// larger blocks (up to 768) three waves per SIMD and 168 VGPRs.
+template <class S, class I, int MK, int PF>
+hipError_t fused_bound(hipStream_t s, const FusedFields& ff, const float* f, Geometry g,
+                       const SweepParams& p, const FusedPlan& plan) {
+    const bool big = (g.w / 4) * (plan.ty + 2) > 512;
+    if (p.data_term == 1)
+        return big ? fused_launch<S, I, 1, MK, 768, PF>(s, ff, f, g, p, plan)
+                   : fused_launch<S, I, 1, MK, 512, PF>(s, ff, f, g, p, plan);
+    return big ? fused_launch<S, I, 0, MK, 768, PF>(s, ff, f, g, p, plan)
+               : fused_launch<S, I, 0, MK, 512, PF>(s, ff, f, g, p, plan);
+}
+
+template <class S, class I, int MK>
+hipError_t fused_pf(hipStream_t s, const FusedFields& ff, const float* f, Geometry g,
+                    const SweepParams& p, const FusedPlan& plan) {
+    return plan.pf ? fused_bound<S, I, MK, 1>(s, ff, f, g, p, plan)
+                   : fused_bound<S, I, MK, 0>(s, ff, f, g, p, plan);
+}
+
+template <class S, class I>
+hipError_t fused_dispatch(hipStream_t s, const FusedFields& ff, const float* f, Geometry g,
+                          const SweepParams& p, const FusedPlan& plan) {
+    return p.active ? fused_pf<S, I, 1>(s, ff, f, g, p, plan)
+                    : fused_pf<S, I, 0>(s, ff, f, g, p, plan);
+}
+
 }  // namespace
+
+// Plan of the fused sweep for a geometry: band height ty and z chunk.  The
+// cost model counts rounds of resident blocks times the rows and planes each
+// block marches (one halo row / plane counted per side) divided by the blocks
+// resident per CU.  Residency: the kernels use ~200-250 VGPRs at the
+// 512-thread bound (two waves per SIMD) and 168 at the 768-thread bound (three
+// waves per SIMD, with spills); LDS 160 KB per CU.  AMB_FUSED=0 disables the
+// fused sweep; AMB_FUSED_TY / AMB_FUSED_ZCHUNK / AMB_FUSED_PF override.
+bool fused_plan(const Geometry& g, int device, FusedPlan* plan) {
+    const char* en = std::getenv("AMB_FUSED");
+    if (en && en[0] == '0') return false;
+    if (g.w % 4 != 0) return false;
+    const int L = g.w / 4;
+    int max_lds = 0, cus = 0;
+    if (hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) !=
+            hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess)
+        return false;
+    const char* ety = std::getenv("AMB_FUSED_TY");
+    const bool forced = ety && std::atoi(ety) > 0;
+    // default: blocks of at most 512 threads (no register spills)
+    const int rt_max = (forced ? FUSED_MAX_THREADS : 512) / L;
+    double best = 1e300;
+    FusedPlan bp{0, 0, 1};
+    for (int rt = 3; rt <= rt_max && rt - 2 <= g.h + 1; ++rt) {
+        const size_t lds = fused_lds_floats<4>(g.w, rt - 2) * sizeof(float);
+        if (lds > size_t(max_lds)) break;
+        const int ty = rt - 2;
+        const int threads = L * rt;
+        const int waves_simd = ((threads + 63) / 64 + 3) / 4;   // per SIMD, one block
+        const int per_cu = std::max(1, std::min(int((160 * 1024) / lds), 2 / waves_simd));
+        const long conc = long(cus) * per_cu;
+        const long nb = (g.h + ty - 1) / ty;
+        for (int nc = 1; nc <= g.d; ++nc) {
+            const int zc = (g.d + nc - 1) / nc;
+            const long ncz = (g.d + zc - 1) / zc;
+            const long blocks = nb * ncz;
+            const long rounds = (blocks + conc - 1) / conc;
+            const double cost = double(rounds) * (ty + 1) * (zc + 1) / per_cu;
+            if (cost < best) { best = cost; bp = FusedPlan{ty, zc, 1}; }
+        }
+    }
+    if (bp.ty == 0 && !forced) return false;
+    const char* ezc = std::getenv("AMB_FUSED_ZCHUNK");
+    if (forced) {
+        const int ty = std::atoi(ety);
+        if (L * (ty + 2) > FUSED_MAX_THREADS ||
+            fused_lds_floats<4>(g.w, ty) * sizeof(float) > size_t(max_lds))
+            return false;
+        bp.ty = ty;
+        if (bp.zchunk == 0) bp.zchunk = g.d;
+    }
+    if (ezc && std::atoi(ezc) > 0) bp.zchunk = std::atoi(ezc);
+    const char* epf = std::getenv("AMB_FUSED_PF");
+    if (epf && (epf[0] == '0' || epf[0] == '1')) bp.pf = epf[0] == '1';
+    *plan = bp;
+    return true;
+}
+
+size_t fused_lds_bytes(const Geometry& g, const FusedPlan& plan) {
+    return fused_lds_floats<4>(g.w, plan.ty) * sizeof(float);
+}
+
+const int FUSED_FIELD_MAP[N_FUSED_FIELDS] = {F_U, F_VX, F_VY, F_VZ, F_PX, F_PY, F_PZ,
+                                             F_QXX, F_QYY, F_QZZ, F_QXY, F_QXZ, F_QYZ};
+
+hipError_t launch_fused_sweep(hipStream_t s, int storage, void* const* src, void* const* dst13,
+                              const float* f, Geometry g, const SweepParams& p,
+                              const FusedPlan& plan) {
+    static_assert(int(N_GFIELDS) == N_FUSED_FIELDS, "fused field count");
+    FusedFields ff;
+    for (int k = 0; k < N_GFIELDS; ++k) {
+        ff.src[k] = src[FUSED_FIELD_MAP[k]];
+        ff.dst[k] = dst13[k];
+    }
+    const bool small = small_index(g);
+    if (storage == 1)
+        return small ? fused_dispatch<StoreBF16, uint32_t>(s, ff, f, g, p, plan)
+                     : fused_dispatch<StoreBF16, size_t>(s, ff, f, g, p, plan);
+    return small ? fused_dispatch<StoreF32, uint32_t>(s, ff, f, g, p, plan)
+                 : fused_dispatch<StoreF32, size_t>(s, ff, f, g, p, plan);
+}
 
 hipError_t launch_dual_sweep(hipStream_t s, int storage, void* const* fields, Geometry g,
                              const SweepParams& p) {

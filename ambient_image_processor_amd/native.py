@@ -28,7 +28,7 @@ N_FIELDS = 17
 FIELD_NAMES = ["u", "ubar", "vx", "vy", "vz", "vbx", "vby", "vbz", "px", "py", "pz",
                "qxx", "qyy", "qzz", "qxy", "qxz", "qyz"]
 PROFILE_CLASSES = ["dual_sweep", "primal_sweep", "dct", "poisson", "divergence", "epilogue",
-                   "init", "energy", "adjoint"]
+                   "init", "energy", "adjoint", "fused_sweep"]
 
 # Every function include/ambient_amd.h declares (checked by the CPU test suite).
 EXPORTS = [

@@ -10,6 +10,13 @@ the same workload on the host cores.
 
     python bench.py [--steps K] [--warmup W] [--storage f32|bf16]
                     [--dct reference|f32|bf16x3] [--cpu-iters N] [--no-cpu-baseline]
+                    [--no-split]
+    python bench.py --copy-calibration     # plain 1 GiB device copies (PMC scale)
+
+Unless --no-split, the same workload also runs in the split form (BF16 field
+storage + BF16x3 MFMA DCT) and the line carries its throughput, its speed-up
+over the plain form and the norm-wise relative error of its u and shading
+against the plain form's.
 """
 import argparse
 import json
@@ -23,16 +30,60 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-# element accesses per voxel for one launch of each sweep
-DUAL_ACCESSES = 22      # read u_bar, v_bar x3, p x3, q x6 ; write p x3, q x6
-PRIMAL_ACCESSES = 22    # read p x3, q x6, u, f, v x3 ; write u, u_bar, v x3, v_bar x3
+# element accesses per voxel for one launch of each sweep (f is always FP32)
+#   dual    read u_bar, v_bar x3, p x3, q x6 ; write p x3, q x6              22
+#   primal  read p x3, q x6, u, f, v x3 ; write u, u_bar, v x3, v_bar x3      22
+#   fused   read u, f, v x3, p x3, q x6 ; write u, v x3, p x3, q x6          27
+SWEEPS = ("dual_sweep", "primal_sweep", "fused_sweep")
 
 
 def sweep_bytes(kind, storage_bytes, n):
     if kind == "dual_sweep":
-        return DUAL_ACCESSES * storage_bytes * n
-    # f stays FP32
-    return ((PRIMAL_ACCESSES - 1) * storage_bytes + 4) * n
+        return 22 * storage_bytes * n
+    if kind == "primal_sweep":
+        return (21 * storage_bytes + 4) * n
+    return (26 * storage_bytes + 4) * n
+
+
+def copy_calibration(nbytes=1 << 30, reps=3):
+    """Plain device-to-device copies of a known size: under rocprofv3 --pmc the
+    FETCH_SIZE / WRITE_SIZE of these dispatches against nbytes give the
+    counter's scale for the sweep kernels' traffic figures."""
+    import torch
+    src = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": nbytes, "reps": reps}))
+
+
+def run_pipeline(ctx, native, f_dev, shape, p, steps, warmup, out):
+    """Warm-up, then `steps` timed deshades; returns (seconds, profile)."""
+    u_dev, l_dev, r_dev = out
+
+    def step():
+        ctx.deshade_pipeline_device(f_dev, shape, p, u_dev, l_dev, r_dev)
+
+    for _ in range(warmup):
+        step()
+        print("warmup step done", file=sys.stderr, flush=True)
+    ctx.synchronize()
+    ctx.profile(True)
+    ctx.profile_reset()
+    t0 = time.perf_counter()
+    for k in range(steps):
+        step()
+        print(f"timed step {k + 1}/{steps} done", file=sys.stderr, flush=True)
+    ctx.synchronize()
+    elapsed = time.perf_counter() - t0
+    prof = ctx.profile_read()
+    ctx.profile(False)
+    return elapsed, prof
+
+
+def rel_err(a, b):
+    return float(np.max(np.abs(a.astype(np.float64) - b)) / max(np.max(np.abs(b)), 1e-30))
 
 
 def main():
@@ -46,12 +97,18 @@ def main():
     ap.add_argument("--dct", choices=["reference", "f32", "bf16x3"], default="reference")
     ap.add_argument("--cpu-iters", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-split", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build measures one GPU only (the TGV sweeps would shard into "
-              "z-slabs with one-plane halos; see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+              "z-slabs exchanging one-plane halos, which is not built). Run with --gpus 1.",
+              file=sys.stderr)
         sys.exit(2)
+    if args.copy_calibration:
+        copy_calibration()
+        return
 
     from ambient_image_processor_amd import native, phantoms
 
@@ -71,26 +128,11 @@ def main():
 
     ctx = native.Context(0)
     f_dev = ctx.alloc(n * 4)
-    u_dev, l_dev, r_dev = ctx.alloc(n * 4), ctx.alloc(n * 4), ctx.alloc(n * 4)
+    out = (ctx.alloc(n * 4), ctx.alloc(n * 4), ctx.alloc(n * 4))
+    u_dev, l_dev, r_dev = out
     ctx.upload(f_dev, f)
 
-    def step():
-        ctx.deshade_pipeline_device(f_dev, shape, p, u_dev, l_dev, r_dev)
-
-    for _ in range(args.warmup):
-        step()
-        print(f"warmup step done", file=sys.stderr, flush=True)
-    ctx.synchronize()
-    ctx.profile(True)
-    ctx.profile_reset()
-    t0 = time.perf_counter()
-    for k in range(args.steps):
-        step()
-        print(f"timed step {k + 1}/{args.steps} done", file=sys.stderr, flush=True)
-    ctx.synchronize()
-    elapsed = time.perf_counter() - t0
-    prof = ctx.profile_read()
-    ctx.profile(False)
+    elapsed, prof = run_pipeline(ctx, native, f_dev, shape, p, args.steps, args.warmup, out)
 
     voxel_iters = float(n) * args.iterations * args.steps
     value = voxel_iters / elapsed
@@ -98,16 +140,36 @@ def main():
 
     sb = 2 if storage == native.STORAGE_BF16 else 4
     kernels = {}
-    for name in ("dual_sweep", "primal_sweep"):
+    for name in SWEEPS:
         ms, cnt = prof[name]
-        avg_ms = ms / max(cnt, 1)
+        if cnt == 0:
+            continue
+        avg_ms = ms / cnt
         b = sweep_bytes(name, sb, n)
         kernels[name] = {"avg_us": 1000.0 * avg_ms, "launches": cnt,
                          "bytes_per_launch": b, "gbs": b / (avg_ms * 1e-3) / 1e9}
     dom = max(kernels, key=lambda k: kernels[k]["avg_us"] * kernels[k]["launches"])
     achieved = kernels[dom]["gbs"]
-    it_ms = (prof["dual_sweep"][0] + prof["primal_sweep"][0]) / max(prof["dual_sweep"][1], 1)
-    iter_bytes = sweep_bytes("dual_sweep", sb, n) + sweep_bytes("primal_sweep", sb, n)
+    sweep_ms = sum(prof[k][0] for k in SWEEPS)
+    sweep_b = sum(sweep_bytes(k, sb, n) * prof[k][1] for k in SWEEPS)
+    it_ms = sweep_ms / (args.iterations * args.steps)
+    iter_bytes = sweep_b / (args.iterations * args.steps)
+
+    split = None
+    if not args.no_split and storage == native.STORAGE_F32 and dct == native.DCT_REFERENCE:
+        u_plain, l_plain = ctx.download(u_dev, shape), ctx.download(l_dev, shape)
+        ps = native.default_params(lam=0.5, alpha0=2.0, alpha1=1.0, tau=tau, sigma=tau,
+                                   theta=1.0, iterations=args.iterations,
+                                   data_term=native.DATA_L1, storage=native.STORAGE_BF16,
+                                   dct_precision=native.DCT_BF16X3,
+                                   flags=native.FLAG_SET_NEGATIVE_ZERO)
+        s_el, _ = run_pipeline(ctx, native, f_dev, shape, ps, args.steps, args.warmup, out)
+        s_val = voxel_iters / s_el
+        split = {"form": "bf16 field storage + bf16x3 MFMA DCT", "value": s_val,
+                 "ms_per_step": 1000.0 * s_el / args.steps, "speedup_vs_plain": s_val / value,
+                 "u_rel_err_vs_plain": rel_err(ctx.download(u_dev, shape), u_plain),
+                 "shading_rel_err_vs_plain": rel_err(ctx.download(l_dev, shape), l_plain),
+                 "err_norm": "max|a-b| / max|b|"}
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -125,7 +187,7 @@ def main():
                          f"(FP32 reference op order, OpenMP), {cpu_s:.1f} s"}
 
     line = {
-        "metric": "voxel-iters/s TGV-DCT deshade 256^3 (TGV2-L1, 1000 it + DCT Poisson + epilogue)",
+        "metric": "voxel-iters/s TGV-DCT deshade 256\u00b3 + %HBM roofline; split vs plain at 1e-4 err",
         "value": value,
         "unit": "voxel-iters/s",
         "n_gpus": 1,
@@ -151,6 +213,7 @@ def main():
                         for kk, vv in v.items()} for k, v in kernels.items()},
         "other_kernels_ms": {k: round(prof[k][0], 3) for k in ("dct", "poisson", "divergence",
                                                                 "epilogue", "init")},
+        "split": split,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))

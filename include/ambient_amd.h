@@ -260,7 +260,8 @@ AMB_API amb_status amb_tgv2_sweep(amb_context* ctx, int which);
  * events on the context's stream; amb_profile_read returns the summed device
  * time (ms) and launch count per kernel class:
  *   0 dual sweep, 1 primal sweep, 2 DCT passes, 3 Poisson divide,
- *   4 divergence, 5 epilogue, 6 init, 7 energy, 8 adjoint (whole call). */
+ *   4 divergence, 5 epilogue, 6 init, 7 energy, 8 adjoint (whole call),
+ *   9 fused sweep (primal half of one iteration + dual half of the next). */
 AMB_API amb_status amb_profile_enable(amb_context* ctx, int enable);
 AMB_API amb_status amb_profile_read(amb_context* ctx, int kernel_class, double* total_ms,
                                     uint64_t* launches);

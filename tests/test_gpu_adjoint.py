@@ -75,7 +75,11 @@ def test_adjoint_against_central_differences(ctx, orc, alphas):
       has been projected it sits on the ball |.| = alpha and later iterates
       hover there, so FD steps of 1e-6..1e-3 cross kinks; FD converges to the
       adjoint only as eps -> 1e-8 (FP64).  Checked: converged FD (eps = 1e-8)
-      to 1e-3, and FD at eps = 1e-3 to 1e-2 (the kink-crossing spread).
+      to 1e-3 in both cases; FD at eps = 1e-3 to 1e-2 (the kink-crossing
+      spread) with alpha0 = 2, alpha1 = 1, where eps is 1e-3 of the scalar.
+      With alpha0 = 0.02 and alpha1 = 0.01 a step of 1e-3 changes the scalar by
+      5-10 % and crosses many kinks (FD(1e-3) of dL/dalpha0 is then off by
+      ~75 %), so only the converged FD is compared there.
       With alpha0 = 2 the q projection never activates and dL/dalpha0 is 0;
       in the second case many q's hover at the ball, where the FP32 forward's
       active set can differ from the FP64 one, so dL/dalpha0 is held to 2e-2."""
@@ -114,7 +118,8 @@ def test_adjoint_against_central_differences(ctx, orc, alphas):
         _say(f"  d/d{name}: adjoint {gp[k]:.6e} fd(1e-3) {fd3:.6e} fd(1e-8) {fd8:.6e}")
         tol = 2e-2 if (k == 0 and a0 < 1) else 1e-3
         assert abs(gp[k] - fd8) <= tol * abs(fd8) + 1e-6 * scale, name
-        assert abs(gp[k] - fd3) <= max(tol, 1e-2) * abs(fd3) + 1e-6 * scale, name
+        if a0 >= 1:
+            assert abs(gp[k] - fd3) <= 1e-2 * abs(fd3) + 1e-6 * scale, name
     if a0 < 1:
         assert abs(gp[0]) > 1e-6 * scale      # the q projection is active
 

@@ -1,0 +1,165 @@
+"""GPU tests of the fused sweep (primal half of iteration k + dual half of
+iteration k + 1 in one launch, u/v/p/q ping-ponged between two copies).
+
+The fused schedule performs the same per-voxel operations in the same order
+as the two-sweep loop, so every result is compared bit for bit: with the two
+sweeps (AMB_FUSED=0) and with the FP32 oracle.  Cases cover band and z-chunk
+edges (band heights and chunk lengths forced through AMB_FUSED_TY /
+AMB_FUSED_ZCHUNK, including the 768-thread block shape), partial bands,
+depth 1, L1/L2, BF16 storage, the masked loop, callbacks (which split the
+fused schedule) and continuing from a state the fused loop left in the
+second buffer copy."""
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+TAU = float(np.float32(1 / np.sqrt(20)))
+PRM = dict(lam=0.5, alpha0=2.0, alpha1=1.0, tau=TAU, sigma=TAU, theta=1.0)
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _params(native, iterations, data_term="l1", **kw):
+    return native.default_params(lam=PRM["lam"], alpha0=PRM["alpha0"], alpha1=PRM["alpha1"],
+                                 tau=TAU, sigma=TAU, theta=1.0, iterations=iterations,
+                                 data_term=native.DATA_L1 if data_term == "l1" else native.DATA_L2,
+                                 **kw)
+
+
+def _f(shape, seed):
+    from ambient_image_processor_amd import phantoms
+    return phantoms.deshade_phantom(shape, seed=seed)[0]
+
+
+def _profile_counts(ctx, fn):
+    ctx.profile(True)
+    ctx.profile_reset()
+    out = fn()
+    prof = ctx.profile_read()
+    ctx.profile(False)
+    return out, prof
+
+
+SHAPES = [(9, 13, 32), (17, 20, 64), (6, 7, 128), (5, 11, 256), (3, 4, 8), (1, 12, 36),
+          (20, 3, 16)]
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+@pytest.mark.parametrize("data_term", ["l1", "l2"])
+def test_fused_equals_two_sweeps_and_oracle(ctx, orc, shape, data_term, monkeypatch):
+    from ambient_image_processor_amd import native
+    _say("fused", shape, data_term)
+    f = _f(shape, 3)
+    p = _params(native, 23, data_term)
+    monkeypatch.setenv("AMB_FUSED", "0")
+    ref = ctx.tgv2_deshade(f, p)
+    monkeypatch.delenv("AMB_FUSED")
+    out, prof = _profile_counts(ctx, lambda: ctx.tgv2_deshade(f, p))
+    assert prof["fused_sweep"][1] == 22 and prof["dual_sweep"][1] == 1
+    assert prof["primal_sweep"][1] == 1
+    for a, b in zip(out, ref):
+        np.testing.assert_array_equal(a, b)
+    st = orc.tgv2(f, 23, data_term=data_term, dtype=np.float32, **PRM)
+    np.testing.assert_array_equal(out[0], st.u)
+    np.testing.assert_array_equal(out[3], st.v[2])
+
+
+@pytest.mark.parametrize("ty,zchunk,shape", [
+    ("1", "1", (7, 9, 32)), ("3", "2", (11, 10, 64)), ("2", "5", (13, 17, 40)),
+    ("10", "4", (9, 23, 256)),        # 64 x 12 = 768 threads per block
+    ("6", "3", (8, 5, 256)), ("4", "7", (1, 9, 64))])
+def test_fused_forced_plans(ctx, monkeypatch, ty, zchunk, shape):
+    from ambient_image_processor_amd import native
+    _say("fused plan", ty, zchunk, shape)
+    f = _f(shape, 5)
+    p = _params(native, 9)
+    monkeypatch.setenv("AMB_FUSED", "0")
+    ref = ctx.tgv2_deshade(f, p)
+    monkeypatch.delenv("AMB_FUSED")
+    monkeypatch.setenv("AMB_FUSED_TY", ty)
+    monkeypatch.setenv("AMB_FUSED_ZCHUNK", zchunk)
+    out, prof = _profile_counts(ctx, lambda: ctx.tgv2_deshade(f, p))
+    assert prof["fused_sweep"][1] == 8
+    for a, b in zip(out, ref):
+        np.testing.assert_array_equal(a, b)
+
+
+def test_fused_bf16_storage(ctx, monkeypatch):
+    from ambient_image_processor_amd import native
+    f = _f((12, 18, 64), 6)
+    p = _params(native, 30, storage=native.STORAGE_BF16)
+    monkeypatch.setenv("AMB_FUSED", "0")
+    ref = ctx.tgv2_deshade(f, p)
+    monkeypatch.delenv("AMB_FUSED")
+    out = ctx.tgv2_deshade(f, p)
+    for a, b in zip(out, ref):
+        np.testing.assert_array_equal(a, b)
+
+
+def test_fused_masked(ctx, orc, monkeypatch):
+    from ambient_image_processor_amd import native
+    shape = (10, 14, 32)
+    f = _f(shape, 7)
+    m = (np.random.default_rng(8).random(shape) > 0.3).astype(np.float32)
+    p = _params(native, 25)
+    monkeypatch.setenv("AMB_FUSED", "0")
+    ref = ctx.tgv2_deshade_masked(f, p, m >= 1)
+    monkeypatch.delenv("AMB_FUSED")
+    out, prof = _profile_counts(ctx, lambda: ctx.tgv2_deshade_masked(f, p, m >= 1))
+    assert prof["fused_sweep"][1] == 24
+    for a, b in zip(out, ref):
+        np.testing.assert_array_equal(a, b)
+    st = orc.tgv2(f, 25, data_term="l1", dtype=np.float32, mask=m, **PRM)
+    np.testing.assert_array_equal(out[0], st.u)
+
+
+def test_fused_with_callback(ctx, monkeypatch):
+    from ambient_image_processor_amd import native
+    f = _f((8, 12, 32), 9)
+    p = _params(native, 40, callback_interval=7)
+    seen = {}
+
+    def make(store):
+        def cb(it, count, u, vx, vy, vz):
+            store[it] = (u.copy(), vz.copy())
+            return it >= 28
+        return cb
+
+    monkeypatch.setenv("AMB_FUSED", "0")
+    ref_seen = {}
+    ref = ctx.tgv2_deshade(f, p, make(ref_seen))
+    monkeypatch.delenv("AMB_FUSED")
+    out = ctx.tgv2_deshade(f, p, make(seen))
+    assert sorted(seen) == sorted(ref_seen) == [7, 14, 21, 28]
+    for it in seen:
+        np.testing.assert_array_equal(seen[it][0], ref_seen[it][0])
+        np.testing.assert_array_equal(seen[it][1], ref_seen[it][1])
+    for a, b in zip(out, ref):
+        np.testing.assert_array_equal(a, b)
+
+
+def test_fused_state_continues_and_energy(ctx, monkeypatch):
+    """begin + iterate(7) + iterate(6) (the second call starts from the copy the
+    first left the state in) equals 13 two-sweep iterations in every field,
+    and the FP64 energy of the final state agrees."""
+    from ambient_image_processor_amd import native
+    f = _f((9, 10, 64), 10)
+    p = _params(native, 0)
+    monkeypatch.setenv("AMB_FUSED", "0")
+    ctx.begin(f, p)
+    ctx.iterate(13)
+    ref_state = ctx.get_state()
+    ref_e = ctx.energy()
+    monkeypatch.delenv("AMB_FUSED")
+    ctx.begin(f, p)
+    ctx.iterate(7)
+    ctx.iterate(6)
+    state = ctx.get_state()
+    np.testing.assert_array_equal(state, ref_state)
+    np.testing.assert_array_equal(ctx.energy(), ref_e)
