@@ -110,8 +110,9 @@ struct FusedPlan {
     int ty;       // owned y rows per block
     int zchunk;   // planes per block
     int pf;       // 1: next-plane loads issued one step ahead
+    int vx;       // voxels per thread along x (2 or 4)
 };
-bool fused_plan(const Geometry& g, int device, FusedPlan* plan);
+bool fused_plan(const Geometry& g, int storage, int device, FusedPlan* plan);
 size_t fused_lds_bytes(const Geometry& g, const FusedPlan& plan);
 hipError_t launch_fused_sweep(hipStream_t s, int storage, void* const* src, void* const* dst13,
                               const float* f, Geometry g, const SweepParams& p,

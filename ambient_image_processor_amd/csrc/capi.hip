@@ -422,7 +422,7 @@ amb_status begin(amb_context* c, amb_memory mem, const float* f, uint32_t w, uin
     c->sp.tile = choose_tile(g);
     c->sp.zchunk = choose_zchunk(g, c->sp.tile, int(p->storage));
     c->sp.active = nullptr;
-    c->fused_ok = fused_plan(g, c->device, &c->fplan);
+    c->fused_ok = fused_plan(g, c->storage, c->device, &c->fplan);
     c->have_state = true;
     return AMB_OK;
 }

@@ -411,6 +411,40 @@ struct VecIO<StoreBF16, 4> {
     }
 };
 
+template <>
+struct VecIO<StoreF32, 2> {
+    template <class I>
+    static __device__ __forceinline__ void ld(const float* p, I i, float (&o)[2]) {
+        const float2 t = *reinterpret_cast<const float2*>(elem(p, i));
+        o[0] = t.x; o[1] = t.y;
+    }
+    template <class I>
+    static __device__ __forceinline__ void st(float* p, I i, const float (&v)[2]) {
+        *reinterpret_cast<float2*>(elem(p, i)) = make_float2(v[0], v[1]);
+    }
+};
+
+template <>
+struct VecIO<StoreBF16, 2> {
+    template <class I>
+    static __device__ __forceinline__ void ld(const uint16_t* p, I i, float (&o)[2]) {
+        const uint32_t t = *reinterpret_cast<const uint32_t*>(elem(p, i));
+        o[0] = __uint_as_float(t << 16); o[1] = __uint_as_float(t & 0xffff0000u);
+    }
+    template <class I>
+    static __device__ __forceinline__ void st(uint16_t* p, I i, const float (&v)[2]) {
+        *reinterpret_cast<uint32_t*>(elem(p, i)) =
+            uint32_t(StoreBF16::rne(v[0])) | (uint32_t(StoreBF16::rne(v[1])) << 16);
+    }
+};
+
+// Active bytes of VX consecutive voxels (byte e of the result = voxel e).
+template <int VX, class I>
+__device__ __forceinline__ uint32_t load_active(const uint8_t* act, I i) {
+    if constexpr (VX == 4) return *reinterpret_cast<const uint32_t*>(elem(act, i));
+    else return uint32_t(*reinterpret_cast<const uint16_t*>(elem(act, i)));
+}
+
 template <int VX>
 __device__ __forceinline__ void zero(float (&a)[VX]) {
 #pragma unroll
@@ -935,7 +969,7 @@ __global__ __launch_bounds__(MAXT) void fused_sweep_kernel(
             } else {
                 zero(P.qnn[0]); zero(P.qnn[1]); zero(P.qnn[2]);
             }
-            if (MK) P.act = *reinterpret_cast<const uint32_t*>(elem(act, i));
+            if (MK) P.act = load_active<VX>(act, i);
         } else {
             zero(P.px); zero(P.py); zero(P.pz); zero(P.qxx); zero(P.qxy); zero(P.qyy);
             zero(P.u); zero(P.f);
@@ -1311,13 +1345,13 @@ hipError_t primal_tiles(hipStream_t s, void* const* fv, const float* f, Geometry
     }
 }
 
-template <class S, class I, int L2, int MK, int MAXT, int PF>
+template <class S, class I, int VX, int L2, int MK, int MAXT, int PF>
 hipError_t fused_launch(hipStream_t s, const FusedFields& ff, const float* f, Geometry g,
                         const SweepParams& p, const FusedPlan& plan) {
-    const int L = g.w / 4, RT = plan.ty + 2;
+    const int L = g.w / VX, RT = plan.ty + 2;
     const int nb = (g.h + plan.ty - 1) / plan.ty, nc = (g.d + plan.zchunk - 1) / plan.zchunk;
-    const size_t lds = fused_lds_floats<4>(g.w, plan.ty) * sizeof(float);
-    auto kern = fused_sweep_kernel<S, I, 4, L2, MK, MAXT, PF>;
+    const size_t lds = fused_lds_floats<VX>(g.w, plan.ty) * sizeof(float);
+    auto kern = fused_sweep_kernel<S, I, VX, L2, MK, MAXT, PF>;
     static size_t lds_set = 0;   // per instantiation: raise the dynamic LDS limit once
     if (lds > lds_set) {
         hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
@@ -1330,24 +1364,31 @@ hipError_t fused_launch(hipStream_t s, const FusedFields& ff, const float* f, Ge
     return hipGetLastError();
 }
 
-// Blocks of up to 512 threads get the 256-VGPR budget of two waves per SIMD;
-// larger blocks (up to 768) three waves per SIMD and 168 VGPRs.
+// VX = 4: blocks of up to 512 threads (two waves per SIMD, ~200-250 VGPRs).
+// VX = 2: blocks of up to 1024 threads (four waves per SIMD, <= 128 VGPRs),
+// no prefetch (it would spill).  64-bit offsets (volumes over 2^30 voxels):
+// VX = 4 without prefetch.
 template <class S, class I, int MK, int PF>
-hipError_t fused_bound(hipStream_t s, const FusedFields& ff, const float* f, Geometry g,
+hipError_t fused_shape(hipStream_t s, const FusedFields& ff, const float* f, Geometry g,
                        const SweepParams& p, const FusedPlan& plan) {
-    const bool big = (g.w / 4) * (plan.ty + 2) > 512;
-    if (p.data_term == 1)
-        return big ? fused_launch<S, I, 1, MK, 768, PF>(s, ff, f, g, p, plan)
-                   : fused_launch<S, I, 1, MK, 512, PF>(s, ff, f, g, p, plan);
-    return big ? fused_launch<S, I, 0, MK, 768, PF>(s, ff, f, g, p, plan)
-               : fused_launch<S, I, 0, MK, 512, PF>(s, ff, f, g, p, plan);
+    if constexpr (sizeof(I) == 4 && PF == 0) {
+        if (plan.vx == 2)
+            return p.data_term == 1 ? fused_launch<S, I, 2, 1, MK, 1024, 0>(s, ff, f, g, p, plan)
+                                    : fused_launch<S, I, 2, 0, MK, 1024, 0>(s, ff, f, g, p, plan);
+    }
+    return p.data_term == 1 ? fused_launch<S, I, 4, 1, MK, 512, PF>(s, ff, f, g, p, plan)
+                             : fused_launch<S, I, 4, 0, MK, 512, PF>(s, ff, f, g, p, plan);
 }
 
 template <class S, class I, int MK>
 hipError_t fused_pf(hipStream_t s, const FusedFields& ff, const float* f, Geometry g,
                     const SweepParams& p, const FusedPlan& plan) {
-    return plan.pf ? fused_bound<S, I, MK, 1>(s, ff, f, g, p, plan)
-                   : fused_bound<S, I, MK, 0>(s, ff, f, g, p, plan);
+    if constexpr (sizeof(I) == 8) {
+        return fused_shape<S, I, MK, 0>(s, ff, f, g, p, plan);
+    } else {
+        return plan.pf && plan.vx == 4 ? fused_shape<S, I, MK, 1>(s, ff, f, g, p, plan)
+                                       : fused_shape<S, I, MK, 0>(s, ff, f, g, p, plan);
+    }
 }
 
 template <class S, class I>
@@ -1359,58 +1400,66 @@ hipError_t fused_dispatch(hipStream_t s, const FusedFields& ff, const float* f, 
 
 }  // namespace
 
-// Plan of the fused sweep for a geometry: band height ty and z chunk.  The
-// cost model counts rounds of resident blocks times the rows and planes each
-// block marches (one halo row / plane counted per side) divided by the blocks
-// resident per CU.  Residency: the kernels use ~200-250 VGPRs at the
-// 512-thread bound (two waves per SIMD) and 168 at the 768-thread bound (three
-// waves per SIMD, with spills); LDS 160 KB per CU.  AMB_FUSED=0 disables the
-// fused sweep; AMB_FUSED_TY / AMB_FUSED_ZCHUNK / AMB_FUSED_PF override.
-bool fused_plan(const Geometry& g, int device, FusedPlan* plan) {
+// Plan of the fused sweep for a geometry: voxels per thread (vx), band height
+// ty and z chunk.  Cost model: rounds of resident blocks x blocks per CU x the
+// rows and planes each block marches (the halo row / plane on each side does
+// half the work of an owned one).  Blocks resident per CU follow from LDS
+// (160 KB) and registers: vx = 4 kernels use ~200-250 VGPRs (two waves per
+// SIMD), vx = 2 kernels at most 128 (four waves per SIMD).  With BF16 fields
+// the sweep is latency-bound rather than bandwidth-bound (measured on MI355X
+// at 256^3), so the cost is scaled by 16 / (waves per CU, at most 16) and the
+// halo rows count half; with FP32 fields the halo traffic dominates.  Ties go
+// to vx = 2 and to the lower band.
+// AMB_FUSED=0 disables the fused sweep; AMB_FUSED_VX / _TY / _ZCHUNK / _PF
+// override.
+bool fused_plan(const Geometry& g, int storage, int device, FusedPlan* plan) {
     const char* en = std::getenv("AMB_FUSED");
     if (en && en[0] == '0') return false;
-    if (g.w % 4 != 0) return false;
-    const int L = g.w / 4;
     int max_lds = 0, cus = 0;
     if (hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) !=
             hipSuccess ||
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess)
         return false;
-    const char* ety = std::getenv("AMB_FUSED_TY");
-    const bool forced = ety && std::atoi(ety) > 0;
-    // default: blocks of at most 512 threads (no register spills)
-    const int rt_max = (forced ? FUSED_MAX_THREADS : 512) / L;
+    auto env_int = [](const char* name) {
+        const char* e = std::getenv(name);
+        return e ? std::atoi(e) : 0;
+    };
+    const int want_vx = env_int("AMB_FUSED_VX"), want_ty = env_int("AMB_FUSED_TY");
+    const int want_zc = env_int("AMB_FUSED_ZCHUNK");
+    const bool big_index = !small_index(g);
     double best = 1e300;
-    FusedPlan bp{0, 0, 1};
-    for (int rt = 3; rt <= rt_max && rt - 2 <= g.h + 1; ++rt) {
-        const size_t lds = fused_lds_floats<4>(g.w, rt - 2) * sizeof(float);
-        if (lds > size_t(max_lds)) break;
-        const int ty = rt - 2;
-        const int threads = L * rt;
-        const int waves_simd = ((threads + 63) / 64 + 3) / 4;   // per SIMD, one block
-        const int per_cu = std::max(1, std::min(int((160 * 1024) / lds), 2 / waves_simd));
-        const long conc = long(cus) * per_cu;
-        const long nb = (g.h + ty - 1) / ty;
-        for (int nc = 1; nc <= g.d; ++nc) {
-            const int zc = (g.d + nc - 1) / nc;
-            const long ncz = (g.d + zc - 1) / zc;
-            const long blocks = nb * ncz;
-            const long rounds = (blocks + conc - 1) / conc;
-            const double cost = double(rounds) * (ty + 1) * (zc + 1) / per_cu;
-            if (cost < best) { best = cost; bp = FusedPlan{ty, zc, 1}; }
+    FusedPlan bp{0, 0, 1, 0};
+    for (int vx : {2, 4}) {
+        if (want_vx && vx != want_vx) continue;
+        if (g.w % vx != 0 || (vx == 2 && big_index)) continue;
+        const int L = g.w / vx;
+        const int max_threads = vx == 4 ? 512 : 1024;
+        const int waves_simd_cap = vx == 4 ? 2 : 4;
+        for (int rt = 3; rt * L <= max_threads && rt - 2 <= g.h + 1; ++rt) {
+            const int ty = rt - 2;
+            if (want_ty && ty != want_ty) continue;
+            const size_t lds = (vx == 4 ? fused_lds_floats<4>(g.w, ty)
+                                        : fused_lds_floats<2>(g.w, ty)) * sizeof(float);
+            if (lds > size_t(max_lds)) break;
+            const int waves_simd = ((rt * L + 63) / 64 + 3) / 4;
+            const int per_cu = std::max(
+                1, std::min(int((160 * 1024) / lds), waves_simd_cap / waves_simd));
+            const long conc = long(cus) * per_cu;
+            const long nb = (g.h + ty - 1) / ty;
+            for (int nc = 1; nc <= g.d; ++nc) {
+                const int zc = want_zc ? want_zc : (g.d + nc - 1) / nc;
+                const long ncz = (g.d + zc - 1) / zc;
+                const long blocks = nb * ncz;
+                const long rounds = (blocks + conc - 1) / conc;
+                const int waves_cu = std::min(16, per_cu * ((rt * L + 63) / 64));
+                const double occ = storage == 1 ? 16.0 / waves_cu : 1.0;
+                const double halo = storage == 1 ? 0.5 : 1.0;
+                const double cost = double(rounds) * per_cu * (ty + halo) * (zc + 1) * occ;
+                if (cost < best) { best = cost; bp = FusedPlan{ty, zc, vx == 4 ? 1 : 0, vx}; }
+            }
         }
     }
-    if (bp.ty == 0 && !forced) return false;
-    const char* ezc = std::getenv("AMB_FUSED_ZCHUNK");
-    if (forced) {
-        const int ty = std::atoi(ety);
-        if (L * (ty + 2) > FUSED_MAX_THREADS ||
-            fused_lds_floats<4>(g.w, ty) * sizeof(float) > size_t(max_lds))
-            return false;
-        bp.ty = ty;
-        if (bp.zchunk == 0) bp.zchunk = g.d;
-    }
-    if (ezc && std::atoi(ezc) > 0) bp.zchunk = std::atoi(ezc);
+    if (bp.ty == 0) return false;
     const char* epf = std::getenv("AMB_FUSED_PF");
     if (epf && (epf[0] == '0' || epf[0] == '1')) bp.pf = epf[0] == '1';
     *plan = bp;
@@ -1418,7 +1467,8 @@ bool fused_plan(const Geometry& g, int device, FusedPlan* plan) {
 }
 
 size_t fused_lds_bytes(const Geometry& g, const FusedPlan& plan) {
-    return fused_lds_floats<4>(g.w, plan.ty) * sizeof(float);
+    return (plan.vx == 2 ? fused_lds_floats<2>(g.w, plan.ty) : fused_lds_floats<4>(g.w, plan.ty)) *
+           sizeof(float);
 }
 
 const int FUSED_FIELD_MAP[N_FUSED_FIELDS] = {F_U, F_VX, F_VY, F_VZ, F_PX, F_PY, F_PZ,

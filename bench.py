@@ -10,13 +10,18 @@ the same workload on the host cores.
 
     python bench.py [--steps K] [--warmup W] [--storage f32|bf16]
                     [--dct reference|f32|bf16x3] [--cpu-iters N] [--no-cpu-baseline]
-                    [--no-split]
-    python bench.py --copy-calibration     # plain 1 GiB device copies (PMC scale)
+                    [--no-split] [--no-host]
+    python bench.py --workload poisson_512      # DCT Poisson solve 512x512x256
+    python bench.py --workload bf16_384         # BF16 field storage, L2, 384^3, 800 it
+    python bench.py --workload adjoint_4x128    # unrolled TGV2-L2 + adjoint, 4 x 128^3
+    python bench.py --copy-calibration          # plain 1 GiB device copies (PMC scale)
 
-Unless --no-split, the same workload also runs in the split form (BF16 field
-storage + BF16x3 MFMA DCT) and the line carries its throughput, its speed-up
-over the plain form and the norm-wise relative error of its u and shading
-against the plain form's.
+Unless --no-split, the default workload also runs in the split form (BF16
+field storage + BF16x3 MFMA DCT) and the line carries its throughput, its
+speed-up over the plain form and the max- and RMS-norm relative errors of
+its u and shading against the plain form's.  Unless --no-host, one more
+deshade runs from host buffers (PCIe transfers included) and is reported as
+`host_buffers`.
 """
 import argparse
 import json
@@ -30,19 +35,60 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-# element accesses per voxel for one launch of each sweep (f is always FP32)
-#   dual    read u_bar, v_bar x3, p x3, q x6 ; write p x3, q x6              22
-#   primal  read p x3, q x6, u, f, v x3 ; write u, u_bar, v x3, v_bar x3      22
-#   fused   read u, f, v x3, p x3, q x6 ; write u, v x3, p x3, q x6          27
+FP32_PEAK_TFLOPS = 157.3       # dense FP32 (vector / matrix)
+BF16_PEAK_TFLOPS = 2500.0      # dense BF16 MFMA
+# Algorithmic bytes (the per-unit figure of the reference algorithm, f FP32):
+# one voxel-iteration is 44 element accesses (176 B FP32, 90 B BF16 fields):
+#   dual half    read u_bar, v_bar x3, p x3, q x6 ; write p x3, q x6          22
+#   primal half  read p x3, q x6, u, f, v x3 ; write u, u_bar, v x3, v_bar x3  22
+# A dual or primal sweep launch processes half a voxel-iteration per voxel; a
+# fused launch (primal half of k + dual half of k + 1) one.  The fused kernel
+# itself needs only 27 accesses per voxel (u, f, v x3, p x3, q x6 read; u, v x3,
+# p x3, q x6 written): u_bar and v_bar never reach HBM.
 SWEEPS = ("dual_sweep", "primal_sweep", "fused_sweep")
+# HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on
+# the default workload (256^3, FP32 fields, default plan), FETCH_SIZE scaled by
+# 2.0 and WRITE_SIZE by 1.0 (what the counters read for a plain 1 GiB copy);
+# summaries in profiles/.  Keyed by (kernel, storage bytes, edge length).
+MEASURED_TRAFFIC = {}
 
 
 def sweep_bytes(kind, storage_bytes, n):
+    """Algorithmic bytes per launch."""
     if kind == "dual_sweep":
         return 22 * storage_bytes * n
     if kind == "primal_sweep":
         return (21 * storage_bytes + 4) * n
-    return (26 * storage_bytes + 4) * n
+    return (43 * storage_bytes + 4) * n
+
+
+def kernel_min_bytes(kind, storage_bytes, n):
+    """Bytes each field must move once for the kernel's own formulation."""
+    if kind == "fused_sweep":
+        return (26 * storage_bytes + 4) * n
+    return sweep_bytes(kind, storage_bytes, n)
+
+
+def rel_err(a, b):
+    """max |a - b| / max |b|"""
+    return float(np.max(np.abs(a.astype(np.float64) - b)) / max(np.max(np.abs(b)), 1e-30))
+
+
+def rms_err(a, b):
+    """||a - b||_2 / ||b||_2"""
+    b = np.asarray(b, dtype=np.float64)
+    return float(np.linalg.norm(a.astype(np.float64) - b) / max(np.linalg.norm(b), 1e-30))
+
+
+def forward_difference(a, axis):
+    """D+ along axis 0 = x (fastest), 1 = y, 2 = z: a[i+1] - a[i], 0 at the last index."""
+    ax = 2 - axis
+    out = np.zeros_like(a)
+    lo = [slice(None)] * 3
+    hi = [slice(None)] * 3
+    lo[ax], hi[ax] = slice(0, -1), slice(1, None)
+    out[tuple(lo)] = a[tuple(hi)] - a[tuple(lo)]
+    return out
 
 
 def copy_calibration(nbytes=1 << 30, reps=3):
@@ -58,13 +104,8 @@ def copy_calibration(nbytes=1 << 30, reps=3):
     print(json.dumps({"copy_bytes": nbytes, "reps": reps}))
 
 
-def run_pipeline(ctx, native, f_dev, shape, p, steps, warmup, out):
-    """Warm-up, then `steps` timed deshades; returns (seconds, profile)."""
-    u_dev, l_dev, r_dev = out
-
-    def step():
-        ctx.deshade_pipeline_device(f_dev, shape, p, u_dev, l_dev, r_dev)
-
+def timed(ctx, step, steps, warmup):
+    """Warm-up, then `steps` timed calls of step(); returns (seconds, profile)."""
     for _ in range(warmup):
         step()
         print("warmup step done", file=sys.stderr, flush=True)
@@ -82,34 +123,27 @@ def run_pipeline(ctx, native, f_dev, shape, p, steps, warmup, out):
     return elapsed, prof
 
 
-def rel_err(a, b):
-    return float(np.max(np.abs(a.astype(np.float64) - b)) / max(np.max(np.abs(b)), 1e-30))
+def sweep_kernels(prof, sb, n):
+    kernels = {}
+    for name in SWEEPS:
+        ms, cnt = prof[name]
+        if cnt == 0:
+            continue
+        avg_ms = ms / cnt
+        b = sweep_bytes(name, sb, n)
+        kmin = kernel_min_bytes(name, sb, n)
+        kernels[name] = {"avg_us": 1000.0 * avg_ms, "launches": cnt,
+                         "bytes_per_launch": b, "gbs": b / (avg_ms * 1e-3) / 1e9,
+                         "kernel_min_bytes": kmin, "kernel_min_gbs": kmin / (avg_ms * 1e-3) / 1e9}
+    return kernels
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--size", type=int, default=256)
-    ap.add_argument("--iterations", type=int, default=1000)
-    ap.add_argument("--storage", choices=["f32", "bf16"], default="f32")
-    ap.add_argument("--dct", choices=["reference", "f32", "bf16x3"], default="reference")
-    ap.add_argument("--cpu-iters", type=int, default=20)
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-split", action="store_true")
-    ap.add_argument("--copy-calibration", action="store_true")
-    args = ap.parse_args()
+def rounded(d):
+    return {k: {kk: (round(vv, 3) if isinstance(vv, float) else vv) for kk, vv in v.items()}
+            for k, v in d.items()}
 
-    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py: this build measures one GPU only (the TGV sweeps would shard into "
-              "z-slabs exchanging one-plane halos, which is not built). Run with --gpus 1.",
-              file=sys.stderr)
-        sys.exit(2)
-    if args.copy_calibration:
-        copy_calibration()
-        return
 
+def deshade_256(args):
     from ambient_image_processor_amd import native, phantoms
 
     n_side = args.size
@@ -121,55 +155,59 @@ def main():
     storage = native.STORAGE_BF16 if args.storage == "bf16" else native.STORAGE_F32
     dct = {"reference": native.DCT_REFERENCE, "f32": native.DCT_F32,
            "bf16x3": native.DCT_BF16X3}[args.dct]
-    p = native.default_params(lam=0.5, alpha0=2.0, alpha1=1.0, tau=tau, sigma=tau, theta=1.0,
-                              iterations=args.iterations, data_term=native.DATA_L1,
-                              storage=storage, dct_precision=dct,
-                              flags=native.FLAG_SET_NEGATIVE_ZERO)
 
+    def params(storage_, dct_):
+        return native.default_params(lam=0.5, alpha0=2.0, alpha1=1.0, tau=tau, sigma=tau,
+                                     theta=1.0, iterations=args.iterations,
+                                     data_term=native.DATA_L1, storage=storage_,
+                                     dct_precision=dct_, flags=native.FLAG_SET_NEGATIVE_ZERO)
+
+    p = params(storage, dct)
     ctx = native.Context(0)
     f_dev = ctx.alloc(n * 4)
-    out = (ctx.alloc(n * 4), ctx.alloc(n * 4), ctx.alloc(n * 4))
-    u_dev, l_dev, r_dev = out
+    u_dev, l_dev, r_dev = ctx.alloc(n * 4), ctx.alloc(n * 4), ctx.alloc(n * 4)
     ctx.upload(f_dev, f)
 
-    elapsed, prof = run_pipeline(ctx, native, f_dev, shape, p, args.steps, args.warmup, out)
+    def step_with(pp):
+        return lambda: ctx.deshade_pipeline_device(f_dev, shape, pp, u_dev, l_dev, r_dev)
 
+    elapsed, prof = timed(ctx, step_with(p), args.steps, args.warmup)
     voxel_iters = float(n) * args.iterations * args.steps
     value = voxel_iters / elapsed
     ms_per_step = 1000.0 * elapsed / args.steps
 
     sb = 2 if storage == native.STORAGE_BF16 else 4
-    kernels = {}
-    for name in SWEEPS:
-        ms, cnt = prof[name]
-        if cnt == 0:
-            continue
-        avg_ms = ms / cnt
-        b = sweep_bytes(name, sb, n)
-        kernels[name] = {"avg_us": 1000.0 * avg_ms, "launches": cnt,
-                         "bytes_per_launch": b, "gbs": b / (avg_ms * 1e-3) / 1e9}
+    kernels = sweep_kernels(prof, sb, n)
     dom = max(kernels, key=lambda k: kernels[k]["avg_us"] * kernels[k]["launches"])
     achieved = kernels[dom]["gbs"]
     sweep_ms = sum(prof[k][0] for k in SWEEPS)
     sweep_b = sum(sweep_bytes(k, sb, n) * prof[k][1] for k in SWEEPS)
     it_ms = sweep_ms / (args.iterations * args.steps)
     iter_bytes = sweep_b / (args.iterations * args.steps)
+    it_rate = n / (it_ms * 1e-3)
 
     split = None
     if not args.no_split and storage == native.STORAGE_F32 and dct == native.DCT_REFERENCE:
         u_plain, l_plain = ctx.download(u_dev, shape), ctx.download(l_dev, shape)
-        ps = native.default_params(lam=0.5, alpha0=2.0, alpha1=1.0, tau=tau, sigma=tau,
-                                   theta=1.0, iterations=args.iterations,
-                                   data_term=native.DATA_L1, storage=native.STORAGE_BF16,
-                                   dct_precision=native.DCT_BF16X3,
-                                   flags=native.FLAG_SET_NEGATIVE_ZERO)
-        s_el, _ = run_pipeline(ctx, native, f_dev, shape, ps, args.steps, args.warmup, out)
+        s_el, _ = timed(ctx, step_with(params(native.STORAGE_BF16, native.DCT_BF16X3)),
+                        args.steps, args.warmup)
         s_val = voxel_iters / s_el
+        u_s, l_s = ctx.download(u_dev, shape), ctx.download(l_dev, shape)
         split = {"form": "bf16 field storage + bf16x3 MFMA DCT", "value": s_val,
                  "ms_per_step": 1000.0 * s_el / args.steps, "speedup_vs_plain": s_val / value,
-                 "u_rel_err_vs_plain": rel_err(ctx.download(u_dev, shape), u_plain),
-                 "shading_rel_err_vs_plain": rel_err(ctx.download(l_dev, shape), l_plain),
-                 "err_norm": "max|a-b| / max|b|"}
+                 "u_rel_err_vs_plain": rel_err(u_s, u_plain),
+                 "u_rms_err_vs_plain": rms_err(u_s, u_plain),
+                 "shading_rel_err_vs_plain": rel_err(l_s, l_plain),
+                 "shading_rms_err_vs_plain": rms_err(l_s, l_plain),
+                 "err_norms": "rel: max|a-b|/max|b|; rms: ||a-b||_2/||b||_2"}
+
+    host = None
+    if not args.no_host:
+        t0 = time.perf_counter()
+        ctx.deshade_pipeline(f, p)
+        h_s = time.perf_counter() - t0
+        host = {"value": float(n) * args.iterations / h_s, "ms_per_step": 1000.0 * h_s,
+                "note": "one deshade from host buffers: H2D of f, D2H of u, shading, deshaded"}
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -186,8 +224,10 @@ def main():
                "sample": f"{args.cpu_iters} TGV2-L1 iterations of the same {n_side}^3 volume "
                          f"(FP32 reference op order, OpenMP), {cpu_s:.1f} s"}
 
+    traffic = MEASURED_TRAFFIC.get((dom, sb, n_side))
+    avg_s = kernels[dom]["avg_us"] * 1e-6
     line = {
-        "metric": "voxel-iters/s TGV-DCT deshade 256\u00b3 + %HBM roofline; split vs plain at 1e-4 err",
+        "metric": "voxel-iters/s TGV-DCT deshade 256³ + %HBM roofline; split vs plain at 1e-4 err",
         "value": value,
         "unit": "voxel-iters/s",
         "n_gpus": 1,
@@ -204,21 +244,192 @@ def main():
                    "tau_sigma": "1/sqrt(20)", "storage": args.storage, "dct": args.dct,
                    "parallelism": "single-gpu"},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "bytes_per_launch": kernels[dom]["bytes_per_launch"]},
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "bytes_per_launch": kernels[dom]["bytes_per_launch"],
+                     "kernel_min_bytes_per_launch": kernels[dom]["kernel_min_bytes"],
+                     "kernel_min_frac": kernels[dom]["kernel_min_gbs"] / HBM_PEAK_GBS,
+                     "measured_hbm_gbs": (traffic / avg_s / 1e9) if traffic else None},
         "iteration": {"ms": it_ms, "bytes": iter_bytes,
-                      "gbs": iter_bytes / (it_ms * 1e-3) / 1e9,
-                      "voxel_iters_per_s": n / (it_ms * 1e-3)},
-        "kernels": {k: {kk: (round(vv, 3) if isinstance(vv, float) else vv)
-                        for kk, vv in v.items()} for k, v in kernels.items()},
+                      "gbs": iter_bytes / (it_ms * 1e-3) / 1e9, "voxel_iters_per_s": it_rate},
+        "kernels": rounded(kernels),
         "other_kernels_ms": {k: round(prof[k][0], 3) for k in ("dct", "poisson", "divergence",
                                                                 "epilogue", "init")},
         "split": split,
+        "host_buffers": host,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))
-    ctx.free(f_dev); ctx.free(u_dev); ctx.free(l_dev); ctx.free(r_dev)
+    for ptr in (f_dev, u_dev, l_dev, r_dev):
+        ctx.free(ptr)
     ctx.close()
+
+
+def poisson_512(args):
+    """DCT Poisson integration of v = D+ phi at 512 x 512 x 256, phi a sum of
+    64 cosine modes: B-divergence, forward DCT with the eigenvalue divide, and
+    inverse DCT; the exact answer is phi - mean(phi) = phi."""
+    from ambient_image_processor_amd import native, phantoms
+    shape = (256, 512, 512)
+    n = int(np.prod(shape))
+    _, phi = phantoms.poisson_field(shape, seed=3)
+    v = [forward_difference(phi, a).astype(np.float32) for a in range(3)]
+    ctx = native.Context(0)
+    vd = [ctx.alloc(n * 4) for _ in range(3)]
+    for d, a in zip(vd, v):
+        ctx.upload(d, a)
+    out = ctx.alloc(n * 4)
+    flops = 4.0 * n * sum(shape)
+    res = {}
+    for name, prec in (("reference", native.DCT_REFERENCE), ("f32", native.DCT_F32),
+                       ("bf16x3", native.DCT_BF16X3)):
+        step = lambda: ctx.integrate_gradients_poisson_device(*vd, shape, out, prec)
+        el, prof = timed(ctx, step, args.steps, args.warmup)
+        l = ctx.download(out, shape)
+        ms, cnt = prof["dct"]
+        pass_ms = ms / max(cnt, 1)
+        dct_ms = ms / args.steps
+        mf_peak = BF16_PEAK_TFLOPS if prec == native.DCT_BF16X3 else FP32_PEAK_TFLOPS
+        issued = 3 * flops if prec == native.DCT_BF16X3 else flops
+        res[name] = {"ms_per_solve": 1000.0 * el / args.steps, "dct_ms_per_solve": dct_ms,
+                     "dct_pass_us": 1000.0 * pass_ms,
+                     "mfma_frac": (issued / (dct_ms * 1e-3) / 1e12 / mf_peak
+                                   if prec != native.DCT_REFERENCE else None),
+                     "hbm_frac": 6 * 2 * n * 4 / (dct_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                     "max_rel_err_vs_closed_form": rel_err(l, phi),
+                     "rms_rel_err_vs_closed_form": rms_err(l, phi)}
+        print(f"{name} done", file=sys.stderr, flush=True)
+    best = res["bf16x3"]
+    print(json.dumps({
+        "metric": "DCT Poisson solves/s 512x512x256 (B-div + DCT-II + eigen divide + DCT-III)",
+        "value": 1000.0 / best["ms_per_solve"], "unit": "solves/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": best["ms_per_solve"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "bf16x3/f32-acc", "data": "synthetic (64 seeded cosine modes, closed form)",
+        "config": {"workload": "poisson_512x512x256", "shape": list(shape),
+                   "flops_per_solve_fp32_equiv": flops},
+        "variants": {k: {kk: (round(vv, 6) if isinstance(vv, float) else vv)
+                         for kk, vv in v.items()} for k, v in res.items()}}))
+    for d in vd + [out]:
+        ctx.free(d)
+    ctx.close()
+
+
+def bf16_384(args):
+    """TGV2-L2 deshade at 384^3, 800 iterations, BF16 field storage against the
+    FP32 path (errors of u and of the shading l)."""
+    from ambient_image_processor_amd import native, phantoms
+    w = phantoms.workload("deshade_384")
+    f = w["f"]
+    shape = f.shape
+    n = f.size
+    iters = args.iterations if args.iterations != 1000 else w["iterations"]
+    ctx = native.Context(0)
+    f_dev = ctx.alloc(n * 4)
+    u_dev, l_dev, r_dev = ctx.alloc(n * 4), ctx.alloc(n * 4), ctx.alloc(n * 4)
+    ctx.upload(f_dev, f)
+    out = {}
+    for name, storage in (("f32", native.STORAGE_F32), ("bf16", native.STORAGE_BF16)):
+        p = native.default_params(lam=w["lam"], alpha0=w["alpha0"], alpha1=w["alpha1"],
+                                  tau=w["tau"], sigma=w["tau"], theta=1.0, iterations=iters,
+                                  data_term=native.DATA_L2, storage=storage,
+                                  dct_precision=native.DCT_F32, flags=0)
+        el, prof = timed(ctx, lambda: ctx.deshade_pipeline_device(f_dev, shape, p, u_dev, l_dev,
+                                                                 r_dev), 1, 0)
+        sb = 2 if storage == native.STORAGE_BF16 else 4
+        out[name] = {"s": el, "voxel_iters_per_s": n * iters / el,
+                     "kernels": rounded(sweep_kernels(prof, sb, n)),
+                     "u": ctx.download(u_dev, shape), "l": ctx.download(l_dev, shape)}
+    res = {k: {kk: vv for kk, vv in v.items() if kk not in ("u", "l")} for k, v in out.items()}
+    b, r = out["bf16"], out["f32"]
+    print(json.dumps({
+        "metric": "voxel-iters/s TGV2-L2 deshade 384^3 800 it, BF16 field storage",
+        "value": b["voxel_iters_per_s"], "unit": "voxel-iters/s", "n_gpus": 1, "steps": 1,
+        "warmup": 0, "ms_per_step": 1000.0 * b["s"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "bf16-storage/f32-arith",
+        "data": "synthetic (spheres, bias max 0.2, noise 0.02)",
+        "config": {"workload": "tgv2_l2_deshade_384^3_bf16", "iterations": iters},
+        "speedup_vs_f32": b["voxel_iters_per_s"] / r["voxel_iters_per_s"],
+        "u_rel_err_vs_f32": rel_err(b["u"], r["u"]), "u_rms_err_vs_f32": rms_err(b["u"], r["u"]),
+        "shading_rel_err_vs_f32": rel_err(b["l"], r["l"]),
+        "shading_rms_err_vs_f32": rms_err(b["l"], r["l"]),
+        "runs": res}))
+    for ptr in (f_dev, u_dev, l_dev, r_dev):
+        ctx.free(ptr)
+    ctx.close()
+
+
+def adjoint_4x128(args):
+    """Unrolled TGV2-L2 denoise (200 iterations) and its hand-written adjoint
+    (checkpoints every 20) on 4 volumes of 128^3; MSE loss to the clean volumes."""
+    from ambient_image_processor_amd import native, phantoms
+    w = phantoms.workload("denoise_128x4")
+    vols, cleans = w["f"], w["clean"]
+    shape = vols.shape[1:]
+    n = int(np.prod(shape))
+    iters = w["iterations"]
+    p = native.default_params(lam=w["lam"], alpha0=w["alpha0"], alpha1=w["alpha1"],
+                              tau=w["tau"], sigma=w["tau"], theta=1.0, iterations=iters,
+                              data_term=native.DATA_L2)
+    ctx = native.Context(0)
+    f_dev, g_dev, u_dev, gf_dev = (ctx.alloc(n * 4) for _ in range(4))
+    t_fwd = t_adj = 0.0
+    for b in range(vols.shape[0]):
+        ctx.upload(f_dev, vols[b])
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        u = ctx.tgv2_denoise(vols[b], p)
+        t_fwd += time.perf_counter() - t0
+        ctx.upload(g_dev, ((2.0 / (vols.shape[0] * n)) * (u - cleans[b])).astype(np.float32))
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        ctx.tgv2_denoise_adjoint_device(f_dev, shape, p, g_dev, u_dev, gf_dev,
+                                        w["checkpoint_every"])
+        t_adj += time.perf_counter() - t0
+        print(f"volume {b} done", file=sys.stderr, flush=True)
+    vi = float(n) * iters * vols.shape[0]
+    print(json.dumps({
+        "metric": "voxel-iters/s unrolled TGV2-L2 denoise forward + adjoint, 4 x 128^3, 200 it",
+        "value": vi / t_adj, "unit": "voxel-iters/s (forward + reverse)", "n_gpus": 1,
+        "steps": vols.shape[0], "warmup": 0, "ms_per_step": 1000.0 * t_adj / vols.shape[0],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (12 seeded boxes + N(0,0.05^2) per volume)",
+        "config": {"workload": "tgv2_l2_adjoint_4x128^3", "iterations": iters,
+                   "checkpoint_every": w["checkpoint_every"]},
+        "forward_only_voxel_iters_per_s": vi / t_fwd,
+        "adjoint_over_forward_time": t_adj / t_fwd}))
+    for ptr in (f_dev, g_dev, u_dev, gf_dev):
+        ctx.free(ptr)
+    ctx.close()
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--size", type=int, default=256)
+    ap.add_argument("--iterations", type=int, default=1000)
+    ap.add_argument("--storage", choices=["f32", "bf16"], default="f32")
+    ap.add_argument("--dct", choices=["reference", "f32", "bf16x3"], default="reference")
+    ap.add_argument("--cpu-iters", type=int, default=20)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-split", action="store_true")
+    ap.add_argument("--no-host", action="store_true")
+    ap.add_argument("--workload", choices=["deshade_256", "poisson_512", "bf16_384",
+                                           "adjoint_4x128"], default="deshade_256")
+    ap.add_argument("--copy-calibration", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this build measures one GPU only (the TGV sweeps would shard into "
+              "z-slabs exchanging one-plane halos, which is not built). Run with --gpus 1.",
+              file=sys.stderr)
+        sys.exit(2)
+    if args.copy_calibration:
+        copy_calibration()
+        return
+    {"deshade_256": deshade_256, "poisson_512": poisson_512, "bf16_384": bf16_384,
+     "adjoint_4x128": adjoint_4x128}[args.workload](args)
 
 
 if __name__ == "__main__":

@@ -4,8 +4,9 @@ iteration k + 1 in one launch, u/v/p/q ping-ponged between two copies).
 The fused schedule performs the same per-voxel operations in the same order
 as the two-sweep loop, so every result is compared bit for bit: with the two
 sweeps (AMB_FUSED=0) and with the FP32 oracle.  Cases cover band and z-chunk
-edges (band heights and chunk lengths forced through AMB_FUSED_TY /
-AMB_FUSED_ZCHUNK, including the 768-thread block shape), partial bands,
+edges (voxels per thread, band heights, chunk lengths and prefetch forced
+through AMB_FUSED_VX / _TY / _ZCHUNK / _PF, including 1024-thread blocks),
+partial bands,
 depth 1, L1/L2, BF16 storage, the masked loop, callbacks (which split the
 fused schedule) and continuing from a state the fused loop left in the
 second buffer copy."""
@@ -70,24 +71,46 @@ def test_fused_equals_two_sweeps_and_oracle(ctx, orc, shape, data_term, monkeypa
     np.testing.assert_array_equal(out[3], st.v[2])
 
 
-@pytest.mark.parametrize("ty,zchunk,shape", [
-    ("1", "1", (7, 9, 32)), ("3", "2", (11, 10, 64)), ("2", "5", (13, 17, 40)),
-    ("10", "4", (9, 23, 256)),        # 64 x 12 = 768 threads per block
-    ("6", "3", (8, 5, 256)), ("4", "7", (1, 9, 64))])
-def test_fused_forced_plans(ctx, monkeypatch, ty, zchunk, shape):
+@pytest.mark.parametrize("vx,ty,zchunk,pf,shape", [
+    ("4", "1", "1", "1", (7, 9, 32)), ("4", "3", "2", "0", (11, 10, 64)),
+    ("4", "2", "5", "1", (13, 17, 40)), ("4", "6", "3", "1", (8, 5, 256)),
+    ("4", "4", "7", "0", (1, 9, 64)), ("4", "4", "64", "1", (70, 13, 256)),
+    ("2", "6", "4", "0", (9, 23, 256)),      # 128 x 8 = 1024 threads per block
+    ("2", "1", "1", "0", (7, 9, 30)), ("2", "3", "5", "0", (6, 13, 66)),
+    ("2", "2", "3", "0", (5, 4, 8))])
+def test_fused_forced_plans(ctx, monkeypatch, vx, ty, zchunk, pf, shape):
     from ambient_image_processor_amd import native
-    _say("fused plan", ty, zchunk, shape)
+    _say("fused plan vx", vx, "ty", ty, "zchunk", zchunk, "pf", pf, shape)
     f = _f(shape, 5)
     p = _params(native, 9)
     monkeypatch.setenv("AMB_FUSED", "0")
     ref = ctx.tgv2_deshade(f, p)
     monkeypatch.delenv("AMB_FUSED")
-    monkeypatch.setenv("AMB_FUSED_TY", ty)
-    monkeypatch.setenv("AMB_FUSED_ZCHUNK", zchunk)
+    for k, v in (("VX", vx), ("TY", ty), ("ZCHUNK", zchunk), ("PF", pf)):
+        monkeypatch.setenv("AMB_FUSED_" + k, v)
     out, prof = _profile_counts(ctx, lambda: ctx.tgv2_deshade(f, p))
     assert prof["fused_sweep"][1] == 8
     for a, b in zip(out, ref):
         np.testing.assert_array_equal(a, b)
+
+
+@pytest.mark.parametrize("vx", ["2", "4"])
+def test_fused_masked_and_bf16_per_width(ctx, monkeypatch, vx):
+    from ambient_image_processor_amd import native
+    shape = (9, 11, 48)
+    f = _f(shape, 12)
+    m = (np.random.default_rng(13).random(shape) > 0.4).astype(np.float32)
+    for storage in (native.STORAGE_F32, native.STORAGE_BF16):
+        p = _params(native, 14, storage=storage)
+        monkeypatch.setenv("AMB_FUSED", "0")
+        ref = ctx.tgv2_deshade_masked(f, p, m >= 1)
+        monkeypatch.delenv("AMB_FUSED")
+        monkeypatch.setenv("AMB_FUSED_VX", vx)
+        out, prof = _profile_counts(ctx, lambda: ctx.tgv2_deshade_masked(f, p, m >= 1))
+        monkeypatch.delenv("AMB_FUSED_VX")
+        assert prof["fused_sweep"][1] == 13
+        for a, b in zip(out, ref):
+            np.testing.assert_array_equal(a, b)
 
 
 def test_fused_bf16_storage(ctx, monkeypatch):
