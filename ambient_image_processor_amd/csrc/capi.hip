@@ -60,7 +60,7 @@ struct AxisTables {
     int n = 0;
     double* fwd64 = nullptr;     // [m][k] cospi((m+0.5)k/n)
     double* inv64 = nullptr;     // [m][k] cospi((k+0.5)m/n)
-    float* fwd32 = nullptr;      // FP32 copies for the MFMA passes, [m][k] layout
+    float* fwd32 = nullptr;      // FP32 copies for the MFMA passes, [k][m] layout
     float* inv32 = nullptr;
     uint16_t* fwd_hi = nullptr;  // bf16 splits, [k][m]
     uint16_t* fwd_lo = nullptr;
@@ -273,16 +273,15 @@ amb_status ensure_axis(amb_context* c, int axis, int n) {
             f64[size_t(m) * n + k] = host_cospi((double(m) + 0.5) * double(k) / size);
             i64v[size_t(m) * n + k] = host_cospi((double(k) + 0.5) * double(m) / size);
         }
-    // MFMA tables: FP32 [m][k] (contiguous over the output index), bf16 splits
-    // [k][m] (contiguous over the summed index)
+    // MFMA tables, FP32 and bf16 splits: [k][m] (contiguous over the summed index)
     for (int k = 0; k < n; ++k)
         for (int m = 0; m < n; ++m) {
             const double cf = f64[size_t(m) * n + k];
             // inverse: out[k] = in[0] + 2 sum_{m>=1} in[m] cos  -> fold the factor 2 and the
             // m = 0 term into the matrix (weights 1 and 2)
             const double ci = (m == 0 ? 1.0 : 2.0) * i64v[size_t(m) * n + k];
-            f32[size_t(m) * n + k] = float(cf);
-            i32[size_t(m) * n + k] = float(ci);
+            f32[size_t(k) * n + m] = float(cf);
+            i32[size_t(k) * n + m] = float(ci);
             const uint16_t h1 = host_bf16_rne(float(cf));
             fh[size_t(k) * n + m] = h1;
             fl[size_t(k) * n + m] = host_bf16_rne(float(cf - double(host_bf16_to_float(h1))));
@@ -549,7 +548,7 @@ amb_status check_precision(const Geometry& g, int precision) {
     const int mx = dct_mfma_max_extent();
     if (g.w > mx || g.h > mx || g.d > mx)
         return fail(AMB_ERR_NOT_SUPPORTED,
-                    "MFMA DCT passes support extents up to 512; use AMB_DCT_REFERENCE");
+                    "MFMA DCT passes support extents up to 4096; use AMB_DCT_REFERENCE");
     return AMB_OK;
 }
 

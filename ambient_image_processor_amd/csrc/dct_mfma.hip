@@ -18,12 +18,16 @@
 //           C_hi*X_hi + C_hi*X_lo + C_lo*X_hi, FP32 accumulate (the lo*lo
 //           term is below the split's own rounding)
 //
-// Two kernels: the column pass (y, z: the summed index is strided, output
-// columns are contiguous) keeps an n x 32 output tile per block and stages
-// 32 x 32 input tiles in LDS; the row pass (x: the summed index is
-// contiguous) keeps a 32-row x n tile.  4 waves per block split the n
-// outputs; operand k-slots are assigned identically for A and B, so the
-// products are independent of the hardware's k ordering within a slot group.
+// One kernel for every axis: a tiled product D[i][j] = sum_m X[i][m] Y[m][j]
+// with 128 x 128 output tiles per block (2 x 2 waves of 64 x 64), 32-deep
+// K chunks staged in LDS with K contiguous for both operands (one 16-byte LDS
+// read per lane and split for the BF16 operands).
+//   column pass (y, z): X = C[k][m] (table), Y = in[o][m][s], D = out[o][k][s]
+//   row pass (x):       X = in[r][m],       Y[m][k] = C[k][m], D = out[r][k]
+// The data tile is read once per K chunk with 16-byte loads; blocks that share
+// a data panel are adjacent in the grid (same XCD L2).  Operand k-slots are
+// assigned identically for X and Y, so the products do not depend on the
+// hardware's k order within a slot group.
 #include "amb_internal.hpp"
 
 namespace amb {
@@ -33,10 +37,6 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef short i16x8 __attribute__((ext_vector_type(8)));
 
-constexpr int KC = 32;     // summed-index chunk
-constexpr int NS = 32;     // output columns (column pass) / rows (row pass) per block
-constexpr int NT = 256;    // threads per block (4 waves)
-
 __device__ __forceinline__ uint16_t bf16_rne(float v) {
     uint32_t u = __float_as_uint(v);
     u += 0x7fffu + ((u >> 16) & 1u);
@@ -44,291 +44,365 @@ __device__ __forceinline__ uint16_t bf16_rne(float v) {
 }
 __device__ __forceinline__ float bf16_f(uint16_t h) { return __uint_as_float(uint32_t(h) << 16); }
 
-// hi/lo bf16 split of 8 floats into MFMA operands
-__device__ __forceinline__ void split8(const float (&x)[8], bf16x8& hi, bf16x8& lo) {
-    i16x8 h, l;
-#pragma unroll
-    for (int e = 0; e < 8; ++e) {
-        const uint16_t a = bf16_rne(x[e]);
-        const uint16_t b = bf16_rne(x[e] - bf16_f(a));
-        h[e] = short(a);
-        l[e] = short(b);
-    }
-    hi = __builtin_bit_cast(bf16x8, h);
-    lo = __builtin_bit_cast(bf16x8, l);
-}
-
-// 8 consecutive bf16 matrix entries (zero beyond n_valid); one 16-byte load
-// when all 8 are valid and aligned (extent a multiple of 8).
-__device__ __forceinline__ bf16x8 load_bf16x8(const uint16_t* p, int n_valid, bool aligned) {
-    i16x8 v;
-    if (n_valid == 8 && aligned) {
-        v = *reinterpret_cast<const i16x8*>(p);
+// Poisson eigenvalue divide on an output element at linear index idx (the
+// same arithmetic as poisson_kernel in dct.hip, solve_poisson_in_cosine_domain.cu:42-53).
+__device__ __forceinline__ float poisson_apply(float v, size_t idx, const Geometry& g, int quirk,
+                                               const float* cx, const float* cy,
+                                               const float* cz) {
+    const unsigned wh = unsigned(g.w) * unsigned(g.h);
+    int x, y, z;
+    if (!quirk) {
+        x = int(idx % g.w); y = int((idx / g.w) % g.h); z = int(idx / wh);
     } else {
-#pragma unroll
-        for (int e = 0; e < 8; ++e) v[e] = e < n_valid ? short(p[e]) : short(0);
+        const float zf = floorf(float(unsigned(idx) / wh));
+        int rest = int(float(int(idx)) - zf * float(wh));
+        const float yf = floorf(float(unsigned(rest) / unsigned(g.w)));
+        rest = int(float(rest) - yf * float(unsigned(g.w)));
+        x = rest; y = int(yf); z = int(zf);
     }
-    return __builtin_bit_cast(bf16x8, v);
+    if (x == 0 && y == 0 && z == 0) return 0.0f;
+    const double den = 6.0 - 2.0 * double(cx[x]) - 2.0 * double(cy[y]) - 2.0 * double(cz[z]);
+    return float(double(v) / den);
 }
 
-struct PassArgs {
-    const float* in;
+constexpr int BM = 128, BN = 128, BK = 32;
+constexpr int PF32 = BK + 4;   // LDS row pitch (floats) of an FP32 tile
+constexpr int PB16 = BK + 8;   // LDS row pitch (bf16) of a split tile
+
+struct GemmArgs {
+    const float* data;      // FP32 volume (X in the row pass, Y in the column pass)
+    const float* t32;       // FP32 table [k][m]
+    const uint16_t* thi;    // BF16 split tables [k][m]
+    const uint16_t* tlo;
     float* out;
-    const float* ct;        // FP32 matrix, [m][k] layout (transposed)
-    const uint16_t* chi;    // BF16 hi split, [k][m] layout
-    const uint16_t* clo;    // BF16 lo split, [k][m] layout
-    int n;                  // extent of the transformed axis
-    int S;                  // column pass: stride of the axis (= inner extent); row pass: unused
-    int O;                  // column pass: outer count; row pass: number of rows
-    float scale;            // multiplies the result (last pass)
-    // fused Poisson divide (forward z pass)
-    int poisson;
-    int quirk;
+    int M, N, K;            // output rows, output columns, summed extent
+    int O;                  // outer count (column pass: planes above the axis)
+    int rowpass;            // 1: X = data [M][K], Y = table [N][K]; 0: X = table, Y = data [K][N]
+    float scale;
+    int poisson, quirk;
     Geometry g;
     const float* cx;
     const float* cy;
     const float* cz;
 };
 
-// Poisson eigenvalue divide on an output element at linear index idx
-// (the same arithmetic as poisson_kernel in dct.hip).
-__device__ __forceinline__ float poisson_apply(float v, size_t idx, const PassArgs& a) {
-    const unsigned wh = unsigned(a.g.w) * unsigned(a.g.h);
-    int x, y, z;
-    if (!a.quirk) {
-        x = int(idx % a.g.w); y = int((idx / a.g.w) % a.g.h); z = int(idx / wh);
-    } else {
-        const float zf = floorf(float(unsigned(idx) / wh));
-        int rest = int(float(int(idx)) - zf * float(wh));
-        const float yf = floorf(float(unsigned(rest) / unsigned(a.g.w)));
-        rest = int(float(rest) - yf * float(unsigned(a.g.w)));
-        x = rest; y = int(yf); z = int(zf);
-    }
-    if (x == 0 && y == 0 && z == 0) return 0.0f;
-    const double den = 6.0 - 2.0 * double(a.cx[x]) - 2.0 * double(a.cy[y]) - 2.0 * double(a.cz[z]);
-    return float(double(v) / den);
+__device__ __forceinline__ float poisson_at(float v, size_t idx, const GemmArgs& a) {
+    return poisson_apply(v, idx, a.g, a.quirk, a.cx, a.cy, a.cz);
 }
 
-// ---------------------------------------------------------------------------
-// Column pass: out[o][k][s] = sum_m C[k][m] in[o][m][s]
-// MT = 16-row output tiles per wave (n padded to 64 * MT).
-// ---------------------------------------------------------------------------
-template <int PREC, int MT>
-__global__ __launch_bounds__(NT) void dct_col_kernel(PassArgs a) {
-    __shared__ float sB[KC][NS + 4];
-    const int n = a.n, S = a.S;
-    const int nst = (S + NS - 1) / NS;
-    const int o = blockIdx.x / nst;
-    const int s0 = (blockIdx.x % nst) * NS;
+// LDS tiles: X [BM][pitch], Y [BN][pitch]; FP32 (PREC 1) or bf16 hi/lo (PREC 2)
+template <int PREC>
+struct GemmLds;
+template <>
+struct GemmLds<1> {
+    float x[BM][PF32];
+    float y[BN][PF32];
+};
+template <>
+struct GemmLds<2> {
+    uint16_t xh[BM][PB16], xl[BM][PB16];
+    uint16_t yh[BN][PB16], yl[BN][PB16];
+};
+
+// Staging.  Each thread moves one 16-float piece of the data tile and 16
+// table entries per K chunk; the global loads of chunk c + 1 are issued into
+// registers (DataRegs / TableRegs) before the MFMAs of chunk c and written to
+// LDS after them.
+struct DataRegs {
+    float v[4][4];    // [piece row][4 consecutive elements]
+};
+struct TableRegs {
+    float f[4][4];    // FP32 table: 4 x float4
+    i16x8 h[2], l[2]; // BF16 table: 2 x 8 hi / lo
+};
+
+// Data operand, K contiguous in memory (row pass X = in[r][m]): thread e
+// loads rows r0 + (e / 8) + 32 q (q < 4), columns m0 + 4 (e % 8) .. + 4.
+__device__ __forceinline__ void load_data_kc(DataRegs& R, const float* src, int rows, int r0,
+                                             int m0, int K) {
+    const int e = threadIdx.x, c4 = (e % 8) * 4;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int rg = r0 + e / 8 + 32 * q, m = m0 + c4;
+        if (rg < rows && m + 3 < K && (K & 3) == 0) {
+            const float4 t = *reinterpret_cast<const float4*>(src + size_t(rg) * K + m);
+            R.v[q][0] = t.x; R.v[q][1] = t.y; R.v[q][2] = t.z; R.v[q][3] = t.w;
+        } else {
+#pragma unroll
+            for (int c = 0; c < 4; ++c)
+                R.v[q][c] = (rg < rows && m + c < K) ? src[size_t(rg) * K + m + c] : 0.0f;
+        }
+    }
+}
+
+// Data operand, N contiguous in memory (column pass Y = in[m][s]): thread e
+// loads rows m0 + 4 (e / 32) .. + 4 at columns j0 + 4 (e % 32) .. + 4.
+__device__ __forceinline__ void load_data_nc(DataRegs& R, const float* src, int N, int K, int j0,
+                                             int m0) {
+    const int e = threadIdx.x, k4 = (e / 32) * 4, j = j0 + (e % 32) * 4;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int m = m0 + k4 + q;
+        if (m < K && j + 3 < N && (N & 3) == 0) {
+            const float4 t = *reinterpret_cast<const float4*>(src + size_t(m) * N + j);
+            R.v[q][0] = t.x; R.v[q][1] = t.y; R.v[q][2] = t.z; R.v[q][3] = t.w;
+        } else {
+#pragma unroll
+            for (int c = 0; c < 4; ++c) R.v[q][c] = (m < K && j + c < N) ? src[size_t(m) * N + j + c] : 0.0f;
+        }
+    }
+}
+
+// Table [k][m] (K contiguous): FP32 thread e loads rows t0 + e / 8 + 32 q,
+// columns m0 + 4 (e % 8); BF16 thread e loads rows t0 + e / 4 + 64 q,
+// columns m0 + 8 (e % 4).
+template <int PREC>
+__device__ __forceinline__ void load_table(TableRegs& T, const GemmArgs& a, int t0, int nrows,
+                                           int m0) {
+    const int K = a.K, e = threadIdx.x;
+    if constexpr (PREC == 1) {
+        const int c4 = (e % 8) * 4;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int tr = t0 + e / 8 + 32 * q, m = m0 + c4;
+            if (tr < nrows && m + 3 < K && (K & 3) == 0) {
+                const float4 t = *reinterpret_cast<const float4*>(a.t32 + size_t(tr) * K + m);
+                T.f[q][0] = t.x; T.f[q][1] = t.y; T.f[q][2] = t.z; T.f[q][3] = t.w;
+            } else {
+#pragma unroll
+                for (int c = 0; c < 4; ++c)
+                    T.f[q][c] = (tr < nrows && m + c < K) ? a.t32[size_t(tr) * K + m + c] : 0.0f;
+            }
+        }
+    } else {
+        const int c8 = (e % 4) * 8;
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const int tr = t0 + e / 4 + 64 * q, m = m0 + c8;
+            if (tr < nrows && m + 7 < K && (K & 7) == 0) {
+                T.h[q] = *reinterpret_cast<const i16x8*>(a.thi + size_t(tr) * K + m);
+                T.l[q] = *reinterpret_cast<const i16x8*>(a.tlo + size_t(tr) * K + m);
+            } else {
+#pragma unroll
+                for (int c = 0; c < 8; ++c) {
+                    const bool ok = tr < nrows && m + c < K;
+                    T.h[q][c] = ok ? short(a.thi[size_t(tr) * K + m + c]) : short(0);
+                    T.l[q][c] = ok ? short(a.tlo[size_t(tr) * K + m + c]) : short(0);
+                }
+            }
+        }
+    }
+}
+
+// 4 floats -> packed bf16 hi and lo (2 x 64 bits)
+__device__ __forceinline__ void split4(const float (&x)[4], uint2& hi, uint2& lo) {
+    uint16_t h[4], l[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        h[c] = bf16_rne(x[c]);
+        l[c] = bf16_rne(x[c] - bf16_f(h[c]));
+    }
+    hi = make_uint2(uint32_t(h[0]) | (uint32_t(h[1]) << 16), uint32_t(h[2]) | (uint32_t(h[3]) << 16));
+    lo = make_uint2(uint32_t(l[0]) | (uint32_t(l[1]) << 16), uint32_t(l[2]) | (uint32_t(l[3]) << 16));
+}
+
+// 4 K-consecutive values of row r of operand tile X (isx) or Y, at column k
+template <int PREC>
+__device__ __forceinline__ void put4(GemmLds<PREC>& L, bool isx, int r, int k, const float (&x)[4]) {
+    if constexpr (PREC == 1) {
+        float* d = isx ? &L.x[r][k] : &L.y[r][k];
+        *reinterpret_cast<float4*>(d) = make_float4(x[0], x[1], x[2], x[3]);
+    } else {
+        uint2 h, l;
+        split4(x, h, l);
+        *reinterpret_cast<uint2*>(isx ? &L.xh[r][k] : &L.yh[r][k]) = h;
+        *reinterpret_cast<uint2*>(isx ? &L.xl[r][k] : &L.yl[r][k]) = l;
+    }
+}
+
+template <int PREC>
+__device__ __forceinline__ void store_data_kc(GemmLds<PREC>& L, const DataRegs& R) {
+    const int e = threadIdx.x, c4 = (e % 8) * 4;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) put4<PREC>(L, true, e / 8 + 32 * q, c4, R.v[q]);
+}
+
+// column pass: transpose the 4 x 4 piece so that Y[j][k..k+3] is contiguous
+template <int PREC>
+__device__ __forceinline__ void store_data_nc(GemmLds<PREC>& L, const DataRegs& R) {
+    const int e = threadIdx.x, k4 = (e / 32) * 4, j = (e % 32) * 4;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        const float x[4] = {R.v[0][c], R.v[1][c], R.v[2][c], R.v[3][c]};
+        put4<PREC>(L, false, j + c, k4, x);
+    }
+}
+
+template <int PREC>
+__device__ __forceinline__ void store_table(GemmLds<PREC>& L, bool isx, const TableRegs& T) {
+    const int e = threadIdx.x;
+    if constexpr (PREC == 1) {
+        const int c4 = (e % 8) * 4;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            float* d = isx ? &L.x[e / 8 + 32 * q][c4] : &L.y[e / 8 + 32 * q][c4];
+            *reinterpret_cast<float4*>(d) = make_float4(T.f[q][0], T.f[q][1], T.f[q][2], T.f[q][3]);
+        }
+    } else {
+        const int c8 = (e % 4) * 8;
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const int r = e / 4 + 64 * q;
+            *reinterpret_cast<i16x8*>(isx ? &L.xh[r][c8] : &L.yh[r][c8]) = T.h[q];
+            *reinterpret_cast<i16x8*>(isx ? &L.xl[r][c8] : &L.yl[r][c8]) = T.l[q];
+        }
+    }
+}
+
+template <int PREC>
+__global__ __launch_bounds__(256) void dct_gemm_kernel(GemmArgs a) {
+    __shared__ __align__(16) GemmLds<PREC> L;
+    const int M = a.M, N = a.N, K = a.K;
+    const int tm = (M + BM - 1) / BM, tn = (N + BN - 1) / BN;
+    // grid order: the tiles sharing a data panel are adjacent
+    int bid = blockIdx.x;
+    int it, jt, o;
+    if (a.rowpass) {            // data panel = rows it; j tiles fastest
+        jt = bid % tn; bid /= tn; it = bid % tm; o = bid / tm;
+    } else {                    // data panel = columns jt; i tiles fastest
+        it = bid % tm; bid /= tm; jt = bid % tn; o = bid / tn;
+    }
+    const int i0 = it * BM, j0 = jt * BN;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int li = lane & 15, lg = lane >> 4;
-    const int kbase = wave * (MT * 16);
-    const size_t obase = size_t(o) * size_t(n) * size_t(S);
+    const int wi = (wave >> 1) * 64, wj = (wave & 1) * 64;
+    const size_t obase = size_t(o) * size_t(M) * size_t(N);
 
-    f32x4 acc[MT][2];
+    f32x4 acc[4][4];
 #pragma unroll
-    for (int t = 0; t < MT; ++t)
+    for (int t = 0; t < 4; ++t)
 #pragma unroll
-        for (int u = 0; u < 2; ++u) acc[t][u] = f32x4{0.f, 0.f, 0.f, 0.f};
+        for (int u = 0; u < 4; ++u) acc[t][u] = f32x4{0.f, 0.f, 0.f, 0.f};
 
-    for (int m0 = 0; m0 < n; m0 += KC) {
-        // stage in[o][m0 .. m0+KC)[s0 .. s0+NS) (zero-padded)
-        for (int e = threadIdx.x; e < KC * NS; e += NT) {
-            const int r = e / NS, cidx = e % NS;
-            const int m = m0 + r, s = s0 + cidx;
-            sB[r][cidx] = (m < n && s < S) ? a.in[obase + size_t(m) * S + s] : 0.0f;
+    const float* dsrc = a.rowpass ? a.data : a.data + size_t(o) * size_t(K) * size_t(N);
+    DataRegs R;
+    TableRegs T;
+    auto load = [&](int m0) {
+        if (a.rowpass) {
+            load_data_kc(R, dsrc, M, i0, m0, K);
+            load_table<PREC>(T, a, j0, N, m0);
+        } else {
+            load_table<PREC>(T, a, i0, M, m0);
+            load_data_nc(R, dsrc, N, K, j0, m0);
+        }
+    };
+    load(0);
+    for (int m0 = 0; m0 < K; m0 += BK) {
+        if (a.rowpass) {
+            store_data_kc<PREC>(L, R);
+            store_table<PREC>(L, false, T);
+        } else {
+            store_table<PREC>(L, true, T);
+            store_data_nc<PREC>(L, R);
         }
         __syncthreads();
-        if (PREC == 1) {
+        if (m0 + BK < K) load(m0 + BK);
+        if constexpr (PREC == 1) {
 #pragma unroll
-            for (int kk = 0; kk < KC / 4; ++kk) {
-                const int m = m0 + 4 * kk + lg;
-                float bfrag[2];
+            for (int kk = 0; kk < BK / 4; ++kk) {
+                const int kc = 4 * kk + lg;
+                float xf[4], yf[4];
 #pragma unroll
-                for (int u = 0; u < 2; ++u) bfrag[u] = sB[4 * kk + lg][u * 16 + li];
+                for (int t = 0; t < 4; ++t) xf[t] = L.x[wi + t * 16 + li][kc];
 #pragma unroll
-                for (int t = 0; t < MT; ++t) {
-                    const int k = kbase + t * 16 + li;
-                    const float afrag = (m < n && k < n) ? a.ct[size_t(m) * n + k] : 0.0f;
+                for (int u = 0; u < 4; ++u) yf[u] = L.y[wj + u * 16 + li][kc];
 #pragma unroll
-                    for (int u = 0; u < 2; ++u)
-                        acc[t][u] = __builtin_amdgcn_mfma_f32_16x16x4f32(afrag, bfrag[u], acc[t][u], 0, 0, 0);
-                }
+                for (int t = 0; t < 4; ++t)
+#pragma unroll
+                    for (int u = 0; u < 4; ++u)
+                        acc[t][u] = __builtin_amdgcn_mfma_f32_16x16x4f32(xf[t], yf[u], acc[t][u], 0, 0, 0);
             }
         } else {
-            bf16x8 bhi[2], blo[2];
+            bf16x8 xh[4], xl[4], yh[4], yl[4];
 #pragma unroll
-            for (int u = 0; u < 2; ++u) {
-                float x[8];
-#pragma unroll
-                for (int e = 0; e < 8; ++e) x[e] = sB[8 * lg + e][u * 16 + li];
-                split8(x, bhi[u], blo[u]);
+            for (int t = 0; t < 4; ++t) {
+                xh[t] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const i16x8*>(&L.xh[wi + t * 16 + li][8 * lg]));
+                xl[t] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const i16x8*>(&L.xl[wi + t * 16 + li][8 * lg]));
             }
-            const int mb = m0 + 8 * lg;
-            const int nv = n - mb < 8 ? (n - mb > 0 ? n - mb : 0) : 8;
 #pragma unroll
-            for (int t = 0; t < MT; ++t) {
-                const int k = kbase + t * 16 + li;
-                const int kv = k < n ? nv : 0;
-                const bool al = (n & 7) == 0;
-                const bf16x8 ahi = load_bf16x8(a.chi + size_t(k < n ? k : 0) * n + (kv ? mb : 0), kv, al);
-                const bf16x8 alo = load_bf16x8(a.clo + size_t(k < n ? k : 0) * n + (kv ? mb : 0), kv, al);
-#pragma unroll
-                for (int u = 0; u < 2; ++u) {
-                    acc[t][u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi, bhi[u], acc[t][u], 0, 0, 0);
-                    acc[t][u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi, blo[u], acc[t][u], 0, 0, 0);
-                    acc[t][u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alo, bhi[u], acc[t][u], 0, 0, 0);
-                }
+            for (int u = 0; u < 4; ++u) {
+                yh[u] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const i16x8*>(&L.yh[wj + u * 16 + li][8 * lg]));
+                yl[u] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const i16x8*>(&L.yl[wj + u * 16 + li][8 * lg]));
             }
+            // each product over all 16 tiles before the next: consecutive MFMAs
+            // are independent (same per-accumulator order hi*hi, hi*lo, lo*hi)
+#pragma unroll
+            for (int t = 0; t < 4; ++t)
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+                    acc[t][u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xh[t], yh[u], acc[t][u], 0, 0, 0);
+#pragma unroll
+            for (int t = 0; t < 4; ++t)
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+                    acc[t][u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xh[t], yl[u], acc[t][u], 0, 0, 0);
+#pragma unroll
+            for (int t = 0; t < 4; ++t)
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+                    acc[t][u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xl[t], yh[u], acc[t][u], 0, 0, 0);
         }
         __syncthreads();
     }
-    // epilogue: lane holds rows 4*lg + r of each 16x16 tile, column li
+    // lane holds D rows 4*lg + r, column li of each 16 x 16 tile
 #pragma unroll
-    for (int t = 0; t < MT; ++t)
+    for (int t = 0; t < 4; ++t)
 #pragma unroll
-        for (int u = 0; u < 2; ++u)
+        for (int u = 0; u < 4; ++u)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int k = kbase + t * 16 + 4 * lg + r;
-                const int s = s0 + u * 16 + li;
-                if (k < n && s < S) {
-                    const size_t idx = obase + size_t(k) * S + s;
+                const int i = i0 + wi + t * 16 + 4 * lg + r;
+                const int j = j0 + wj + u * 16 + li;
+                if (i < M && j < N) {
+                    const size_t idx = obase + size_t(i) * N + j;
                     float v = acc[t][u][r] * a.scale;
-                    if (a.poisson) v = poisson_apply(v, idx, a);
+                    if (a.poisson) v = poisson_at(v, idx, a);
                     a.out[idx] = v;
                 }
             }
 }
 
-// ---------------------------------------------------------------------------
-// Row pass: out[r][k] = sum_m in[r][m] C[k][m]   (rows contiguous, x axis)
-// ---------------------------------------------------------------------------
-template <int PREC, int MT>
-__global__ __launch_bounds__(NT) void dct_row_kernel(PassArgs a) {
-    __shared__ float sA[NS][KC + 1];
-    const int n = a.n, R = a.O;
-    const int r0 = blockIdx.x * NS;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int li = lane & 15, lg = lane >> 4;
-    const int kbase = wave * (MT * 16);
-
-    f32x4 acc[2][MT];
-#pragma unroll
-    for (int u = 0; u < 2; ++u)
-#pragma unroll
-        for (int t = 0; t < MT; ++t) acc[u][t] = f32x4{0.f, 0.f, 0.f, 0.f};
-
-    for (int m0 = 0; m0 < n; m0 += KC) {
-        for (int e = threadIdx.x; e < NS * KC; e += NT) {
-            const int rr = e / KC, cidx = e % KC;
-            const int r = r0 + rr, m = m0 + cidx;
-            sA[rr][cidx] = (r < R && m < n) ? a.in[size_t(r) * n + m] : 0.0f;
-        }
-        __syncthreads();
-        if (PREC == 1) {
-#pragma unroll
-            for (int kk = 0; kk < KC / 4; ++kk) {
-                const int m = m0 + 4 * kk + lg;
-                float afrag[2];
-#pragma unroll
-                for (int u = 0; u < 2; ++u) afrag[u] = sA[u * 16 + li][4 * kk + lg];
-#pragma unroll
-                for (int t = 0; t < MT; ++t) {
-                    const int k = kbase + t * 16 + li;
-                    const float bfrag = (m < n && k < n) ? a.ct[size_t(m) * n + k] : 0.0f;
-#pragma unroll
-                    for (int u = 0; u < 2; ++u)
-                        acc[u][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(afrag[u], bfrag, acc[u][t], 0, 0, 0);
-                }
-            }
-        } else {
-            bf16x8 ahi[2], alo[2];
-#pragma unroll
-            for (int u = 0; u < 2; ++u) {
-                float x[8];
-#pragma unroll
-                for (int e = 0; e < 8; ++e) x[e] = sA[u * 16 + li][8 * lg + e];
-                split8(x, ahi[u], alo[u]);
-            }
-            const int mb = m0 + 8 * lg;
-            const int nv = n - mb < 8 ? (n - mb > 0 ? n - mb : 0) : 8;
-#pragma unroll
-            for (int t = 0; t < MT; ++t) {
-                const int k = kbase + t * 16 + li;
-                const int kv = k < n ? nv : 0;
-                const bool al = (n & 7) == 0;
-                const bf16x8 bhi = load_bf16x8(a.chi + size_t(k < n ? k : 0) * n + (kv ? mb : 0), kv, al);
-                const bf16x8 blo = load_bf16x8(a.clo + size_t(k < n ? k : 0) * n + (kv ? mb : 0), kv, al);
-#pragma unroll
-                for (int u = 0; u < 2; ++u) {
-                    acc[u][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi[u], bhi, acc[u][t], 0, 0, 0);
-                    acc[u][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alo[u], bhi, acc[u][t], 0, 0, 0);
-                    acc[u][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi[u], blo, acc[u][t], 0, 0, 0);
-                }
-            }
-        }
-        __syncthreads();
-    }
-    // lane holds rows 4*lg + rr (data rows) of each tile, column li (output k)
-#pragma unroll
-    for (int u = 0; u < 2; ++u)
-#pragma unroll
-        for (int t = 0; t < MT; ++t)
-#pragma unroll
-            for (int rr = 0; rr < 4; ++rr) {
-                const int r = r0 + u * 16 + 4 * lg + rr;
-                const int k = kbase + t * 16 + li;
-                if (r < R && k < n) a.out[size_t(r) * n + k] = acc[u][t][rr] * a.scale;
-            }
-}
-
-template <int PREC, int MT>
-hipError_t launch_pass(hipStream_t s, const PassArgs& a, int axis) {
-    if (axis == 0) {
-        const int blocks = (a.O + NS - 1) / NS;
-        hipLaunchKernelGGL((dct_row_kernel<PREC, MT>), dim3(blocks), dim3(NT), 0, s, a);
-    } else {
-        const int blocks = a.O * ((a.S + NS - 1) / NS);
-        hipLaunchKernelGGL((dct_col_kernel<PREC, MT>), dim3(blocks), dim3(NT), 0, s, a);
-    }
-    return hipGetLastError();
-}
-
-template <int PREC>
-hipError_t launch_prec(hipStream_t s, const PassArgs& a, int axis) {
-    const int mt = (a.n + 63) / 64;   // 16-row tiles per wave
-    if (mt <= 1) return launch_pass<PREC, 1>(s, a, axis);
-    if (mt <= 2) return launch_pass<PREC, 2>(s, a, axis);
-    if (mt <= 4) return launch_pass<PREC, 4>(s, a, axis);
-    if (mt <= 8) return launch_pass<PREC, 8>(s, a, axis);
-    return hipErrorInvalidValue;   // extents above 512 use the reference-order passes
-}
-
 }  // namespace
 
-int dct_mfma_max_extent() { return 512; }
+int dct_mfma_max_extent() { return 4096; }
 
 hipError_t launch_dct_mfma_pass(hipStream_t s, const float* in, float* out, const float* table,
                                 const uint16_t* table_hi, const uint16_t* table_lo, Geometry g,
                                 int axis, int inverse, int last, int precision,
                                 const float* eig_x, const float* eig_y, const float* eig_z,
                                 int fuse_poisson, int quirk) {
-    PassArgs a;
-    a.in = in; a.out = out; a.ct = table; a.chi = table_hi; a.clo = table_lo;
+    GemmArgs a;
+    a.data = in; a.out = out; a.t32 = table; a.thi = table_hi; a.tlo = table_lo;
     a.g = g; a.cx = eig_x; a.cy = eig_y; a.cz = eig_z;
     a.quirk = quirk;
     a.poisson = (last && !inverse && fuse_poisson) ? 1 : 0;
-    if (axis == 0) {
-        a.n = g.w; a.S = 1; a.O = g.h * g.d;
-    } else if (axis == 1) {
-        a.n = g.h; a.S = g.w; a.O = g.d;
-    } else {
-        a.n = g.d; a.S = g.w * g.h; a.O = 1;
+    if (axis == 0) {           // rows r of length W
+        a.rowpass = 1; a.M = g.h * g.d; a.N = g.w; a.K = g.w; a.O = 1;
+    } else if (axis == 1) {    // per plane: out[z][k][x] = sum_m C[k][m] in[z][m][x]
+        a.rowpass = 0; a.M = g.h; a.N = g.w; a.K = g.h; a.O = g.d;
+    } else {                   // out[k][s] = sum_m C[k][m] in[m][s], s over the plane
+        a.rowpass = 0; a.M = g.d; a.N = g.w * g.h; a.K = g.d; a.O = 1;
     }
     a.scale = 1.0f;
     if (last) a.scale = inverse ? float(1.0 / (double(g.n()) * 8.0)) : 8.0f;
-    return precision == 2 ? launch_prec<2>(s, a, axis) : launch_prec<1>(s, a, axis);
+    const long tm = (a.M + BM - 1) / BM, tn = (a.N + BN - 1) / BN;
+    const long blocks = tm * tn * a.O;
+    if (blocks > 0x7fffffffL) return hipErrorInvalidValue;
+    if (precision == 2)
+        hipLaunchKernelGGL(dct_gemm_kernel<2>, dim3(unsigned(blocks)), dim3(256), 0, s, a);
+    else
+        hipLaunchKernelGGL(dct_gemm_kernel<1>, dim3(unsigned(blocks)), dim3(256), 0, s, a);
+    return hipGetLastError();
 }
 
 }  // namespace amb

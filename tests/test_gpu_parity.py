@@ -289,7 +289,7 @@ def test_reference_named_launchers(orc):
 # MFMA DCT passes (F32 and BF16x3 split precision)
 # ---------------------------------------------------------------------------
 MFMA_SHAPES = [(16, 16, 16), (5, 6, 8), (20, 24, 32), (7, 1, 9), (33, 17, 70), (64, 64, 64),
-               (40, 130, 96)]
+               (40, 130, 96), (300, 7, 12), (3, 520, 24), (9, 10, 600), (2, 131, 129)]
 
 
 @pytest.mark.parametrize("shape", MFMA_SHAPES)
@@ -322,7 +322,7 @@ def test_poisson_solve_mfma_known_answer(ctx, orc, precision, tol):
 
 def test_mfma_extent_limit_is_an_error(ctx):
     from ambient_image_processor_amd import native
-    x = np.zeros((2, 2, 520), dtype=np.float32)
+    x = np.zeros((2, 2, 4100), dtype=np.float32)
     with pytest.raises(native.AmbError):
         ctx.cosine_transform(x, native.DCT_BF16X3)
 
