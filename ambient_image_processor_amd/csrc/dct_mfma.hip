@@ -44,16 +44,20 @@ __device__ __forceinline__ uint16_t bf16_rne(float v) {
 }
 __device__ __forceinline__ float bf16_f(uint16_t h) { return __uint_as_float(uint32_t(h) << 16); }
 
-// Poisson eigenvalue divide on an output element at linear index idx (the
-// same arithmetic as poisson_kernel in dct.hip, solve_poisson_in_cosine_domain.cu:42-53).
-__device__ __forceinline__ float poisson_apply(float v, size_t idx, const Geometry& g, int quirk,
+// Poisson eigenvalue divide on the output element (x, y, z) of the forward z
+// pass, i = z and j = y W + x (the same arithmetic as poisson_kernel in
+// dct.hip, solve_poisson_in_cosine_domain.cu:42-53; with the index quirk the
+// coordinates are recovered from the linear index through float arithmetic as
+// the reference does).
+__device__ __forceinline__ float poisson_apply(float v, int i, int j, const Geometry& g, int quirk,
                                                const float* cx, const float* cy,
                                                const float* cz) {
-    const unsigned wh = unsigned(g.w) * unsigned(g.h);
     int x, y, z;
     if (!quirk) {
-        x = int(idx % g.w); y = int((idx / g.w) % g.h); z = int(idx / wh);
+        z = i; y = j / g.w; x = j - y * g.w;
     } else {
+        const unsigned wh = unsigned(g.w) * unsigned(g.h);
+        const unsigned idx = unsigned(i) * wh + unsigned(j);
         const float zf = floorf(float(unsigned(idx) / wh));
         int rest = int(float(int(idx)) - zf * float(wh));
         const float yf = floorf(float(unsigned(rest) / unsigned(g.w)));
@@ -86,8 +90,8 @@ struct GemmArgs {
     const float* cz;
 };
 
-__device__ __forceinline__ float poisson_at(float v, size_t idx, const GemmArgs& a) {
-    return poisson_apply(v, idx, a.g, a.quirk, a.cx, a.cy, a.cz);
+__device__ __forceinline__ float poisson_at(float v, int i, int j, const GemmArgs& a) {
+    return poisson_apply(v, i, j, a.g, a.quirk, a.cx, a.cy, a.cz);
 }
 
 // LDS tiles: X [BM][pitch], Y [BN][pitch]; FP32 (PREC 1) or bf16 hi/lo (PREC 2)
@@ -103,6 +107,16 @@ struct GemmLds<2> {
     uint16_t xh[BM][PB16], xl[BM][PB16];
     uint16_t yh[BN][PB16], yl[BN][PB16];
 };
+
+// Column pass, data operand Y.  FP32: kept as loaded, [k][j] with a pitch of
+// BN + 16 floats (the 16 lanes of a fragment read consecutive j; the four
+// k rows of a read are 16 banks apart).  BF16: transposed to [j][k] with the
+// 4-element k groups XOR-swizzled by bits 4-5 of j, so that the transposing
+// 8-byte writes of 16 lanes (rows 4 apart) hit distinct banks while a
+// fragment's 8 k values stay contiguous and in order.
+constexpr int PYK = BN + 16;
+static_assert(BK * PYK == BN * PF32, "column-pass FP32 Y tile reuses the Y array");
+__device__ __forceinline__ int yswz(int j) { return ((j >> 4) & 3) << 3; }
 
 // Staging.  Each thread moves one 16-float piece of the data tile and 16
 // table entries per K chunk; the global loads of chunk c + 1 are issued into
@@ -227,14 +241,22 @@ __device__ __forceinline__ void store_data_kc(GemmLds<PREC>& L, const DataRegs& 
     for (int q = 0; q < 4; ++q) put4<PREC>(L, true, e / 8 + 32 * q, c4, R.v[q]);
 }
 
-// column pass: transpose the 4 x 4 piece so that Y[j][k..k+3] is contiguous
+// column pass: FP32 rows k as loaded; BF16 transposed (4 x 4 piece) and swizzled
 template <int PREC>
 __device__ __forceinline__ void store_data_nc(GemmLds<PREC>& L, const DataRegs& R) {
     const int e = threadIdx.x, k4 = (e / 32) * 4, j = (e % 32) * 4;
+    if constexpr (PREC == 1) {
+        float* y = &L.y[0][0];
 #pragma unroll
-    for (int c = 0; c < 4; ++c) {
-        const float x[4] = {R.v[0][c], R.v[1][c], R.v[2][c], R.v[3][c]};
-        put4<PREC>(L, false, j + c, k4, x);
+        for (int q = 0; q < 4; ++q)
+            *reinterpret_cast<float4*>(y + (k4 + q) * PYK + j) =
+                make_float4(R.v[q][0], R.v[q][1], R.v[q][2], R.v[q][3]);
+    } else {
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const float x[4] = {R.v[0][c], R.v[1][c], R.v[2][c], R.v[3][c]};
+            put4<PREC>(L, false, j + c, k4 ^ yswz(j + c), x);
+        }
     }
 }
 
@@ -259,7 +281,7 @@ __device__ __forceinline__ void store_table(GemmLds<PREC>& L, bool isx, const Ta
     }
 }
 
-template <int PREC>
+template <int PREC, int ROW>
 __global__ __launch_bounds__(256) void dct_gemm_kernel(GemmArgs a) {
     __shared__ __align__(16) GemmLds<PREC> L;
     const int M = a.M, N = a.N, K = a.K;
@@ -267,7 +289,7 @@ __global__ __launch_bounds__(256) void dct_gemm_kernel(GemmArgs a) {
     // grid order: the tiles sharing a data panel are adjacent
     int bid = blockIdx.x;
     int it, jt, o;
-    if (a.rowpass) {            // data panel = rows it; j tiles fastest
+    if (ROW) {                  // data panel = rows it; j tiles fastest
         jt = bid % tn; bid /= tn; it = bid % tm; o = bid / tm;
     } else {                    // data panel = columns jt; i tiles fastest
         it = bid % tm; bid /= tm; jt = bid % tn; o = bid / tn;
@@ -284,11 +306,15 @@ __global__ __launch_bounds__(256) void dct_gemm_kernel(GemmArgs a) {
 #pragma unroll
         for (int u = 0; u < 4; ++u) acc[t][u] = f32x4{0.f, 0.f, 0.f, 0.f};
 
-    const float* dsrc = a.rowpass ? a.data : a.data + size_t(o) * size_t(K) * size_t(N);
-    DataRegs R;
-    TableRegs T;
-    auto load = [&](int m0) {
-        if (a.rowpass) {
+    const float* dsrc = ROW ? a.data : a.data + size_t(o) * size_t(K) * size_t(N);
+    // FP32: two register sets, chunk c + 2 is loaded while chunk c is
+    // multiplied; BF16x3: one set (chunk c + 1), which keeps two waves per
+    // SIMD (measured faster on MI355X at 512 x 512 x 256)
+    constexpr int DEPTH = PREC == 1 ? 2 : 1;
+    DataRegs R0, R1;
+    TableRegs T0, T1;
+    auto load = [&](DataRegs& R, TableRegs& T, int m0) {
+        if (ROW) {
             load_data_kc(R, dsrc, M, i0, m0, K);
             load_table<PREC>(T, a, j0, N, m0);
         } else {
@@ -296,9 +322,8 @@ __global__ __launch_bounds__(256) void dct_gemm_kernel(GemmArgs a) {
             load_data_nc(R, dsrc, N, K, j0, m0);
         }
     };
-    load(0);
-    for (int m0 = 0; m0 < K; m0 += BK) {
-        if (a.rowpass) {
+    auto chunk = [&](DataRegs& R, TableRegs& T, int m0) {
+        if (ROW) {
             store_data_kc<PREC>(L, R);
             store_table<PREC>(L, false, T);
         } else {
@@ -306,7 +331,7 @@ __global__ __launch_bounds__(256) void dct_gemm_kernel(GemmArgs a) {
             store_data_nc<PREC>(L, R);
         }
         __syncthreads();
-        if (m0 + BK < K) load(m0 + BK);
+        if (m0 + DEPTH * BK < K) load(R, T, m0 + DEPTH * BK);
         if constexpr (PREC == 1) {
 #pragma unroll
             for (int kk = 0; kk < BK / 4; ++kk) {
@@ -315,7 +340,8 @@ __global__ __launch_bounds__(256) void dct_gemm_kernel(GemmArgs a) {
 #pragma unroll
                 for (int t = 0; t < 4; ++t) xf[t] = L.x[wi + t * 16 + li][kc];
 #pragma unroll
-                for (int u = 0; u < 4; ++u) yf[u] = L.y[wj + u * 16 + li][kc];
+                for (int u = 0; u < 4; ++u)
+                    yf[u] = ROW ? L.y[wj + u * 16 + li][kc] : (&L.y[0][0])[kc * PYK + wj + u * 16 + li];
 #pragma unroll
                 for (int t = 0; t < 4; ++t)
 #pragma unroll
@@ -331,8 +357,10 @@ __global__ __launch_bounds__(256) void dct_gemm_kernel(GemmArgs a) {
             }
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
-                yh[u] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const i16x8*>(&L.yh[wj + u * 16 + li][8 * lg]));
-                yl[u] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const i16x8*>(&L.yl[wj + u * 16 + li][8 * lg]));
+                const int r = wj + u * 16 + li;
+                const int kq = ROW ? 8 * lg : (8 * lg) ^ yswz(r);
+                yh[u] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const i16x8*>(&L.yh[r][kq]));
+                yl[u] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const i16x8*>(&L.yl[r][kq]));
             }
             // each product over all 16 tiles before the next: consecutive MFMAs
             // are independent (same per-accumulator order hi*hi, hi*lo, lo*hi)
@@ -353,6 +381,16 @@ __global__ __launch_bounds__(256) void dct_gemm_kernel(GemmArgs a) {
                     acc[t][u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xl[t], yh[u], acc[t][u], 0, 0, 0);
         }
         __syncthreads();
+    };
+    load(R0, T0, 0);
+    if constexpr (DEPTH == 2) {
+        if (BK < K) load(R1, T1, BK);
+        for (int m0 = 0; m0 < K; m0 += 2 * BK) {
+            chunk(R0, T0, m0);
+            if (m0 + BK < K) chunk(R1, T1, m0 + BK);
+        }
+    } else {
+        for (int m0 = 0; m0 < K; m0 += BK) chunk(R0, T0, m0);
     }
     // lane holds D rows 4*lg + r, column li of each 16 x 16 tile
 #pragma unroll
@@ -366,7 +404,7 @@ __global__ __launch_bounds__(256) void dct_gemm_kernel(GemmArgs a) {
                 if (i < M && j < N) {
                     const size_t idx = obase + size_t(i) * N + j;
                     float v = acc[t][u][r] * a.scale;
-                    if (a.poisson) v = poisson_at(v, idx, a);
+                    if (a.poisson) v = poisson_at(v, i, j, a);
                     a.out[idx] = v;
                 }
             }
@@ -398,10 +436,14 @@ hipError_t launch_dct_mfma_pass(hipStream_t s, const float* in, float* out, cons
     const long tm = (a.M + BM - 1) / BM, tn = (a.N + BN - 1) / BN;
     const long blocks = tm * tn * a.O;
     if (blocks > 0x7fffffffL) return hipErrorInvalidValue;
-    if (precision == 2)
-        hipLaunchKernelGGL(dct_gemm_kernel<2>, dim3(unsigned(blocks)), dim3(256), 0, s, a);
-    else
-        hipLaunchKernelGGL(dct_gemm_kernel<1>, dim3(unsigned(blocks)), dim3(256), 0, s, a);
+    const dim3 grid{static_cast<unsigned>(blocks), 1, 1}, block{256, 1, 1};
+    if (precision == 2) {
+        if (a.rowpass) hipLaunchKernelGGL((dct_gemm_kernel<2, 1>), grid, block, 0, s, a);
+        else hipLaunchKernelGGL((dct_gemm_kernel<2, 0>), grid, block, 0, s, a);
+    } else {
+        if (a.rowpass) hipLaunchKernelGGL((dct_gemm_kernel<1, 1>), grid, block, 0, s, a);
+        else hipLaunchKernelGGL((dct_gemm_kernel<1, 0>), grid, block, 0, s, a);
+    }
     return hipGetLastError();
 }
 
