@@ -78,6 +78,9 @@ struct SweepParams {
     int tile;        // tile shape id (tile_shape())
     // masked deshade: updates only where active[i] != 0 (nullptr: every voxel)
     const uint8_t* active;
+    // depth 1 only: the reference's 2-D algebra (no z backward differences in
+    // eps_B(v_bar); the z components of v, p, q stay 0)
+    int planar;
 };
 
 // Column-tile shapes of the sweep kernels: TX x TY threads, each thread owning
@@ -151,7 +154,7 @@ hipError_t launch_active_from_mask(hipStream_t s, const float* mask, uint8_t* ac
 size_t energy_partials_size(Geometry g);
 hipError_t launch_energy(hipStream_t s, int storage, void* const* fields, const float* f,
                          Geometry g, double lambda, double alpha0, double alpha1, int data_term,
-                         double* partials, double* out10);
+                         int planar, double* partials, double* out10);
 
 // Adjoint of the unrolled L2 iteration (adjoint.hip).  adj, xk, xk1: 17-field
 // tables (adjoint of x_{k+1} updated in place to the adjoint of x_k); acc: device

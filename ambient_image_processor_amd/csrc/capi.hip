@@ -406,6 +406,8 @@ amb_status begin(amb_context* c, amb_memory mem, const float* f, uint32_t w, uin
     AMB_TRY(check_geometry(w, h, d));
     AMB_TRY(check_params(p));
     if (!f) return fail(AMB_ERR_INVALID_ARGUMENT, "f is NULL");
+    if ((p->flags & AMB_FLAG_PLANAR) && d != 1)
+        return fail(AMB_ERR_INVALID_ARGUMENT, "AMB_FLAG_PLANAR needs depth 1");
     AMB_TRY(set_device(c));
     const Geometry g{int(w), int(h), int(d)};
     AMB_TRY(ensure_buffers(c, g, int(p->storage)));
@@ -421,6 +423,7 @@ amb_status begin(amb_context* c, amb_memory mem, const float* f, uint32_t w, uin
     c->sp.tile = choose_tile(g);
     c->sp.zchunk = choose_zchunk(g, c->sp.tile, int(p->storage));
     c->sp.active = nullptr;
+    c->sp.planar = (p->flags & AMB_FLAG_PLANAR) ? 1 : 0;
     c->fused_ok = fused_plan(g, c->storage, c->device, &c->fplan);
     c->have_state = true;
     return AMB_OK;
@@ -955,6 +958,9 @@ AMB_API amb_status amb_tgv2_deshade_pipeline(amb_context* c, amb_memory mem, con
     if (!c) return fail(AMB_ERR_INVALID_ARGUMENT, "ctx is NULL");
     AMB_TRY(begin(c, mem, f, w, h, d, p));
     const bool masked = mask && (p->flags & AMB_FLAG_MASKED_ITERATION);
+    // the masked processor takes the 2-D loop for depth-1 images
+    // (TGVDeshadeMaskedProcessor.cpp:93-98)
+    if (masked && d == 1) c->sp.planar = 1;
     if (masked) {
         // active set = mask >= 1 (TGVDeshadeMaskedProcessor.cpp:348-353); f stays unmasked
         // (:99 clones it before the mask multiply at :109)
@@ -1023,7 +1029,7 @@ AMB_API amb_status amb_tgv2_energy(amb_context* c, double out[10]) {
         ProfScope ps(c, P_ENERGY);
         AMB_HIP(launch_energy(c->stream, c->storage, c->fields_dev, c->f, c->g, c->params.lambda,
                               c->params.alpha0, c->params.alpha1, int(c->params.data_term),
-                              c->partials, c->energy_out));
+                              c->sp.planar, c->partials, c->energy_out));
     }
     AMB_HIP(hipMemcpyAsync(out, c->energy_out, 10 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     AMB_HIP(hipStreamSynchronize(c->stream));
@@ -1041,6 +1047,8 @@ AMB_API amb_status amb_tgv2_denoise_adjoint(amb_context* c, amb_memory mem, cons
         return fail(AMB_ERR_NOT_SUPPORTED, "the adjoint is implemented for the L2 data term");
     if (p->storage != AMB_STORAGE_F32)
         return fail(AMB_ERR_NOT_SUPPORTED, "the adjoint runs on FP32 fields");
+    if (p->flags & AMB_FLAG_PLANAR)
+        return fail(AMB_ERR_NOT_SUPPORTED, "the adjoint is implemented for the 3-D loop");
     const uint32_t K = checkpoint_every ? checkpoint_every : 20;
     const uint32_t N = p->iterations;
     AMB_TRY(begin(c, mem, f, w, h, d, p));

@@ -110,6 +110,8 @@ __device__ __forceinline__ void poisson_coords(size_t i, const Geometry& g, int 
 
 // A16: X / (6.0 - 2.0*cx[x] - 2.0*cy[y] - 2.0*cz[z]), (0,0,0) -> 0; cx etc. are the
 // reference's FP32 cospif(x / W) values, the denominator and quotient are double.
+// Depth 1: the reference's 2-D kernel, 4.0 - 2.0*cx[x] - 2.0*cy[y]
+// (solve_poisson_in_cosine_domain.cu:57-84, chosen at :107).
 __global__ void poisson_kernel(const float* __restrict__ in, float* __restrict__ out,
                                const float* __restrict__ cx, const float* __restrict__ cy,
                                const float* __restrict__ cz, Geometry g, int quirk) {
@@ -119,7 +121,9 @@ __global__ void poisson_kernel(const float* __restrict__ in, float* __restrict__
         int x, y, z;
         poisson_coords(i, g, quirk, x, y, z);
         if (x == 0 && y == 0 && z == 0) { out[i] = 0.0f; continue; }
-        const double den = 6.0 - 2.0 * double(cx[x]) - 2.0 * double(cy[y]) - 2.0 * double(cz[z]);
+        const double den =
+            g.d == 1 ? 4.0 - 2.0 * double(cx[x]) - 2.0 * double(cy[y])
+                     : 6.0 - 2.0 * double(cx[x]) - 2.0 * double(cy[y]) - 2.0 * double(cz[z]);
         out[i] = float(double(in[i]) / den);
     }
 }

@@ -65,7 +65,9 @@ __device__ __forceinline__ float poisson_apply(float v, int i, int j, const Geom
         x = rest; y = int(yf); z = int(zf);
     }
     if (x == 0 && y == 0 && z == 0) return 0.0f;
-    const double den = 6.0 - 2.0 * double(cx[x]) - 2.0 * double(cy[y]) - 2.0 * double(cz[z]);
+    const double den =
+        g.d == 1 ? 4.0 - 2.0 * double(cx[x]) - 2.0 * double(cy[y])   // the 2-D kernel
+                 : 6.0 - 2.0 * double(cx[x]) - 2.0 * double(cy[y]) - 2.0 * double(cz[z]);
     return float(double(v) / den);
 }
 

@@ -30,11 +30,13 @@ __device__ __forceinline__ double wave_sum(double v) {
 template <class S>
 __global__ __launch_bounds__(EB) void energy_kernel(
     const typename S::T* const* __restrict__ F, const float* __restrict__ f, Geometry g,
-    double lam, double a0, double a1, int l2, double* __restrict__ partials) {
+    double lam, double a0, double a1, int l2, int planar, double* __restrict__ partials) {
     const size_t n = g.n();
     const size_t plane = size_t(g.w) * g.h;
     const size_t st[3] = {1, size_t(g.w), plane};
     const int ext[3] = {g.w, g.h, g.d};
+    // planar (depth 1): the 2-D second-order term has no z backward differences
+    const int na = planar ? 2 : 3;
     double acc[NQ];
 #pragma unroll
     for (int k = 0; k < NQ; ++k) acc[k] = 0.0;
@@ -56,7 +58,7 @@ __global__ __launch_bounds__(EB) void energy_kernel(
 #pragma unroll
             for (int cc = 0; cc < 3; ++cc) {
                 const double vc = S::ld(v[cc], i);
-                bv[a][cc] = (xc[a] > 0) ? double(S::ld(v[cc], i - st[a])) - vc : -vc;
+                bv[a][cc] = a >= na ? 0.0 : (xc[a] > 0) ? double(S::ld(v[cc], i - st[a])) - vc : -vc;
             }
         }
         double sdiv = 0.0;
@@ -99,11 +101,9 @@ __global__ __launch_bounds__(EB) void energy_kernel(
         en = sqrt(en);
         acc[3] += a0 * en;
         acc[7] += a0 * en - eq;
-#pragma unroll
-        for (int cc = 0; cc < 3; ++cc) {
+        for (int cc = 0; cc < na; ++cc) {
             double t = 0.0;
-#pragma unroll
-            for (int a = 0; a < 3; ++a) {
+            for (int a = 0; a < na; ++a) {
                 const typename S::T* qq = q[qi[a][cc]];
                 double bt = -double(S::ld(qq, i));
                 if (xc[a] < ext[a] - 1) bt += double(S::ld(qq, i + st[a]));
@@ -150,11 +150,12 @@ inline int energy_blocks(const Geometry& g) {
 
 template <class S>
 hipError_t energy_impl(hipStream_t s, void* const* fields_dev_table, const float* f, Geometry g,
-                       double lam, double a0, double a1, int l2, double* partials, double* out) {
+                       double lam, double a0, double a1, int l2, int planar, double* partials,
+                       double* out) {
     const int nb = energy_blocks(g);
     hipLaunchKernelGGL(energy_kernel<S>, dim3(nb), dim3(EB), 0, s,
                        reinterpret_cast<const typename S::T* const*>(fields_dev_table), f, g, lam,
-                       a0, a1, l2, partials);
+                       a0, a1, l2, planar, partials);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(energy_finish_kernel, dim3(1), dim3(64 * NQ), 0, s, partials, nb, out);
@@ -168,12 +169,12 @@ size_t energy_partials_size(Geometry g) { return size_t(energy_blocks(g)) * NQ +
 // `fields` here is a DEVICE array of the 17 field pointers.
 hipError_t launch_energy(hipStream_t s, int storage, void* const* fields, const float* f,
                          Geometry g, double lambda, double alpha0, double alpha1, int data_term,
-                         double* partials, double* out10) {
+                         int planar, double* partials, double* out10) {
     if (storage == 1)
         return energy_impl<StoreBF16>(s, fields, f, g, lambda, alpha0, alpha1, data_term == 1,
-                                      partials, out10);
+                                      planar, partials, out10);
     return energy_impl<StoreF32>(s, fields, f, g, lambda, alpha0, alpha1, data_term == 1,
-                                 partials, out10);
+                                 planar, partials, out10);
 }
 
 }  // namespace amb

@@ -78,7 +78,7 @@ __global__ __launch_bounds__(TX * TY) void dual_sweep_kernel(
     typename S::T* __restrict__ qyy_, typename S::T* __restrict__ qzz_,
     typename S::T* __restrict__ qxy_, typename S::T* __restrict__ qxz_,
     typename S::T* __restrict__ qyz_, Geometry g, float sigma, float alpha0, float alpha1,
-    int zchunk) {
+    int zchunk, int planar) {
     // u_bar plane with +x/+y halo, v_bar planes with -x/-y halo; centre at [ty+1][tx+1].
     __shared__ float sU[TY + 2][TX + 2];
     __shared__ float sV[3][TY + 2][TX + 2];
@@ -178,7 +178,7 @@ __global__ __launch_bounds__(TX * TY) void dual_sweep_kernel(
             for (int k = 0; k < 3; ++k) {
                 bxv[k] = bwd(vb_c[k], sV[k][c.ty + 1][c.tx], c.x > 0);
                 byv[k] = bwd(vb_c[k], sV[k][c.ty][c.tx + 1], c.y > 0);
-                bzv[k] = bwd(vb_c[k], vb_p[k], z > 0);
+                bzv[k] = planar ? 0.0f : bwd(vb_c[k], vb_p[k], z > 0);
             }
             const float exx = bxv[0], eyy = byv[1], ezz = bzv[2];
             const float exy = (bxv[1] + byv[0]) * 0.5f;
@@ -511,7 +511,7 @@ __global__ __launch_bounds__(TX * TY) void dual_sweep_vec_kernel(
     typename S::T* __restrict__ qyy_, typename S::T* __restrict__ qzz_,
     typename S::T* __restrict__ qxy_, typename S::T* __restrict__ qxz_,
     typename S::T* __restrict__ qyz_, Geometry g, float sigma, float alpha0, float alpha1,
-    int zchunk) {
+    int zchunk, int planar) {
     typedef VecIO<S, VX> V;
     constexpr int RW = TX * VX;
     __shared__ __align__(16) float sY[4][TY + 2][RW];   // 0: u_bar (+y halo row TY+1); 1-3: v_bar (-y halo row 0)
@@ -629,7 +629,7 @@ __global__ __launch_bounds__(TX * TY) void dual_sweep_vec_kernel(
                     const float vc = P.vb[k][e];
                     bxv[k] = bwd(vc, e > 0 ? P.vb[k][e > 0 ? e - 1 : 0] : vxm[k], x > 0);
                     byv[k] = bwd(vc, vym[k][e], c.y > 0);
-                    bzv[k] = bwd(vc, vb_p[k][e], z > 0);
+                    bzv[k] = planar ? 0.0f : bwd(vc, vb_p[k][e], z > 0);
                 }
                 const float exy = (bxv[1] + byv[0]) * 0.5f;
                 const float exz = (bxv[2] + bzv[0]) * 0.5f;
@@ -892,7 +892,7 @@ template <class S, class I, int VX, int L2, int MK, int MAXT, int PF>
 __global__ __launch_bounds__(MAXT) void fused_sweep_kernel(
     FusedFields ff, const float* __restrict__ f_, const uint8_t* __restrict__ act, Geometry g,
     float tau, float sigma, float lambda, float alpha0, float alpha1, float theta, int ty,
-    int zchunk) {
+    int zchunk, int planar) {
     typedef typename S::T T;
     typedef VecIO<S, VX> V;
     typedef VecIO<StoreF32, VX> VF;
@@ -1129,7 +1129,7 @@ __global__ __launch_bounds__(MAXT) void fused_sweep_kernel(
                     const float vc = vbc[k][e];
                     bxv[k] = bwd(vc, e > 0 ? vbc[k][e > 0 ? e - 1 : 0] : vxm[k], x > 0);
                     byv[k] = bwd(vc, vym[k][e], y > 0);
-                    bzv[k] = bwd(vc, vb_m2[k][e], zd > 0);
+                    bzv[k] = planar ? 0.0f : bwd(vc, vb_m2[k][e], zd > 0);
                 }
                 const float exy = (bxv[1] + byv[0]) * 0.5f;
                 const float exz = (bxv[2] + bzv[0]) * 0.5f;
@@ -1232,7 +1232,7 @@ hipError_t dual_vec_launch(hipStream_t s, void* const* fv, Geometry g, const Swe
                        dim3(grid_blocks<TX, TY, 4>(g, p.zchunk)), dim3(TX * TY), 0, s, p.active, F[F_UBAR],
                        F[F_VBX], F[F_VBY], F[F_VBZ], F[F_PX], F[F_PY], F[F_PZ], F[F_QXX],
                        F[F_QYY], F[F_QZZ], F[F_QXY], F[F_QXZ], F[F_QYZ], g, p.sigma, p.alpha0,
-                       p.alpha1, p.zchunk);
+                       p.alpha1, p.zchunk, p.planar);
     return hipGetLastError();
 }
 
@@ -1279,7 +1279,7 @@ hipError_t dual_launch(hipStream_t s, void* const* fv, Geometry g, const SweepPa
     hipLaunchKernelGGL((dual_sweep_kernel<S, I, TX, TY, MK>), dim3(grid_blocks<TX, TY>(g, p.zchunk)),
                        dim3(TX * TY), 0, s, p.active, F[F_UBAR], F[F_VBX], F[F_VBY], F[F_VBZ], F[F_PX],
                        F[F_PY], F[F_PZ], F[F_QXX], F[F_QYY], F[F_QZZ], F[F_QXY], F[F_QXZ],
-                       F[F_QYZ], g, p.sigma, p.alpha0, p.alpha1, p.zchunk);
+                       F[F_QYZ], g, p.sigma, p.alpha0, p.alpha1, p.zchunk, p.planar);
     return hipGetLastError();
 }
 
@@ -1360,7 +1360,8 @@ hipError_t fused_launch(hipStream_t s, const FusedFields& ff, const float* f, Ge
         lds_set = lds;
     }
     hipLaunchKernelGGL(kern, dim3(nb * nc), dim3(L * RT), lds, s, ff, f, p.active, g, p.tau,
-                       p.sigma, p.lambda, p.alpha0, p.alpha1, p.theta, plan.ty, plan.zchunk);
+                       p.sigma, p.lambda, p.alpha0, p.alpha1, p.theta, plan.ty, plan.zchunk,
+                       p.planar);
     return hipGetLastError();
 }
 

@@ -24,6 +24,7 @@ FLAG_MASK_SHADING = 0x2
 FLAG_ADD_BACKGROUND_BACK = 0x4
 FLAG_POISSON_INDEX_QUIRK = 0x8
 FLAG_MASKED_ITERATION = 0x10
+FLAG_PLANAR = 0x20
 N_FIELDS = 17
 FIELD_NAMES = ["u", "ubar", "vx", "vy", "vz", "vbx", "vby", "vbz", "px", "py", "pz",
                "qxx", "qyy", "qzz", "qxy", "qxz", "qyz"]

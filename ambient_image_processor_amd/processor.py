@@ -80,6 +80,54 @@ def tgv2_l1_deshade_masked_launch(f_host, width, height, depth, lambda_, iterati
     return _ctx().tgv2_deshade_masked(f, p, active.reshape(f.shape), iteration_finished_callback)
 
 
+def _planar_params(p):
+    p.flags |= native.FLAG_PLANAR
+    return p
+
+
+def _drop_vz(cb):
+    """The 2-D callback signature (iteration_index, iteration_count, u, v_x, v_y)."""
+    if cb is None:
+        return None
+    return lambda it, count, u, vx, vy, vz: cb(it, count, u, vx, vy)
+
+
+def tgv2_l1_deshade_launch_2d(f_host, width, height, lambda_, iteration_count,
+                              paint_iteration_interval, iteration_finished_callback, alpha0,
+                              alpha1, **kw):
+    """Reference tgv_deshade/tgv2_l1_deshade_2d.cu:27-215 -> (u, v_x, v_y), the
+    2-D loop (no z backward differences in eps_B(v_bar)).  The callback gets
+    (iteration_index, iteration_count, u, v_x, v_y)."""
+    data_term = kw.pop("data_term", native.DATA_L1)
+    f = _vol(f_host, width, height, 1)
+    p = _planar_params(_params(lambda_, iteration_count, paint_iteration_interval, alpha0, alpha1,
+                               data_term, **kw))
+    u, vx, vy, _ = _ctx().tgv2_deshade(f, p, _drop_vz(iteration_finished_callback))
+    return u, vx, vy
+
+
+def tgv2_l1_deshade_masked_2d_launch(f_host, width, height, lambda_, iteration_count,
+                                     paint_iteration_interval, cuda_block_dimension,
+                                     iteration_finished_callback, alpha0, alpha1,
+                                     masked_pixel_indices, *edge_pixel_indices, **kw):
+    """Reference tgv_deshade_masked/tgv2_l1_deshade_masked_2d.cu (declared
+    TGVDeshadeMaskedProcessor.cpp:53-68) -> (u, v_x, v_y): the masked 2-D
+    loop.  The eight edge lists are accepted for signature parity, as in
+    the 3-D launcher."""
+    del cuda_block_dimension, edge_pixel_indices
+    f = _vol(f_host, width, height, 1)
+    idx = np.asarray(masked_pixel_indices, dtype=np.int64).ravel()
+    if idx.size and (idx.min() < 0 or idx.max() >= f.size):
+        raise IndexError("masked pixel index outside the volume")
+    active = np.zeros(f.size, np.uint8)
+    active[idx] = 1
+    p = _planar_params(_params(lambda_, iteration_count, paint_iteration_interval, alpha0, alpha1,
+                               native.DATA_L1, **kw))
+    u, vx, vy, _ = _ctx().tgv2_deshade_masked(f, p, active.reshape(f.shape),
+                                              _drop_vz(iteration_finished_callback))
+    return u, vx, vy
+
+
 def _denoise(data_term, f_host, width, height, depth, lambda_, iteration_count,
              paint_iteration_interval, iteration_finished_callback, alpha0, alpha1, **kw):
     f = _vol(f_host, width, height, depth)
@@ -174,6 +222,41 @@ class TGVDeshadeProcessor:
         return _ctx().deshade_pipeline(f, p, mask, iteration_finished_callback)
 
 
+    @staticmethod
+    def processTGV2L1GPUCuda2D(input_image, lambda_, alpha0, alpha1, iteration_count,
+                               paint_iteration_interval=0, iteration_finished_callback=None,
+                               mask=None, set_negative_values_to_zero=True,
+                               add_background_back=False, **kw):
+        """TGVDeshadeProcessor.cpp:316-378, the widget's path for depth-1 images
+        (TGVDeshadeWidget.cpp:100-127): the 2-D loop, then l from the 2-D
+        divergence of v, r = u - l, mask multiplying l and r, clamp, optional
+        add-back of u*(1-mask).  Returns (deshaded_image, denoised_image,
+        shading_image) as (1, H, W) arrays; the callback receives
+        (iteration_index, iteration_count, u, l, r) with the same composition."""
+        f = np.ascontiguousarray(input_image, dtype=np.float32)
+        if f.ndim == 2:
+            f = f[None]
+        if f.shape[0] != 1:
+            raise ValueError("processTGV2L1GPUCuda2D needs a depth-1 image")
+        m = None if mask is None else np.ascontiguousarray(mask, dtype=np.float32).reshape(f.shape)
+        flags = native.FLAG_MASK_SHADING | native.FLAG_PLANAR
+        if set_negative_values_to_zero:
+            flags |= native.FLAG_SET_NEGATIVE_ZERO
+        if add_background_back:
+            flags |= native.FLAG_ADD_BACKGROUND_BACK
+        p = _params(lambda_, iteration_count, paint_iteration_interval, alpha0, alpha1,
+                    kw.pop("data_term", native.DATA_L1), flags=flags, **kw)
+        cb = None
+        if iteration_finished_callback is not None:
+            def cb(it, count, u, vx, vy, vz):
+                aux = _aux_ctx()
+                l = aux.integrate_gradients_poisson(vx, vy, vz, p.dct_precision)
+                r, l = aux.deshade_epilogue(u, l, m, flags)
+                return iteration_finished_callback(it, count, u, l, r)
+        u, l, r = _ctx().deshade_pipeline(f, p, m, cb)
+        return r, u, l
+
+
 class TGVProcessor:
     """Reference tgv/TGVProcessor.cpp:140-180 (TGV2 denoise entry points)."""
 
@@ -204,7 +287,7 @@ def _aux_ctx() -> native.Context:
 
 
 class TGVDeshadeMaskedProcessor:
-    """Reference tgv_deshade_masked/TGVDeshadeMaskedProcessor.{h,cpp} (3-D path)."""
+    """Reference tgv_deshade_masked/TGVDeshadeMaskedProcessor.{h,cpp}."""
 
     @staticmethod
     def buildMaskIndices(mask):
@@ -224,6 +307,18 @@ class TGVDeshadeMaskedProcessor:
         return tuple(out)
 
     @staticmethod
+    def buildMaskIndices2D(mask):
+        """TGVDeshadeMaskedProcessor.cpp:388-421: linear indices of the pixels
+        with mask >= 1, and of those on / not on each image edge.  Returns
+        (pixel_indices_inside_mask, left, not_left, right, not_right, top,
+        not_top, bottom, not_bottom)."""
+        m = np.asarray(mask, dtype=np.float32)
+        m = m.reshape((-1,) + m.shape[-2:])
+        if m.shape[0] != 1:
+            raise ValueError("buildMaskIndices2D needs a depth-1 mask")
+        return TGVDeshadeMaskedProcessor.buildMaskIndices(m)[:9]
+
+    @staticmethod
     def processTGV2L1GPUCuda(input_image, lambda_, alpha0, alpha1, iteration_count,
                              cuda_block_dimension=-1, paint_iteration_interval=0,
                              iteration_finished_callback=None, mask=None,
@@ -235,12 +330,16 @@ class TGVDeshadeMaskedProcessor:
         the mask, r clamped, then either u*(1 - mask) is added back or r is
         multiplied by the mask again.  Without a mask the mask is all ones.
         The callback receives (iteration_index, iteration_count, u, l, r)
-        computed with the same composition, as in the reference."""
+        computed with the same composition, as in the reference.  A depth-1
+        image takes the 2-D loop (processTGV2L1GPUCuda2D, :93-98 and :215-335;
+        div_v from the 2-D divergence)."""
         del cuda_block_dimension
         f = np.ascontiguousarray(input_image, dtype=np.float32)
         m = (np.ones_like(f) if mask is None
              else np.ascontiguousarray(mask, dtype=np.float32).reshape(f.shape))
         flags = native.FLAG_MASKED_ITERATION | native.FLAG_MASK_SHADING
+        if f.shape[0] == 1:
+            flags |= native.FLAG_PLANAR
         if set_negative_values_to_zero:
             flags |= native.FLAG_SET_NEGATIVE_ZERO
         if add_background_back:

@@ -73,7 +73,17 @@ typedef enum { AMB_DCT_REFERENCE = 0, AMB_DCT_F32 = 1, AMB_DCT_BF16X3 = 2 } amb_
                                                deshade composition of
                                                TGVDeshadeMaskedProcessor.cpp:76-210 - f unmasked,
                                                updates only where mask >= 1, l and r times the
-                                               mask, clamp, then add-back or r times the mask */
+                                               mask, clamp, then add-back or r times the mask;
+                                               at depth 1 the 2-D loop (:93-98), as with
+                                               AMB_FLAG_PLANAR */
+#define AMB_FLAG_PLANAR               0x20u /* depth 1 only: the reference's 2-D loop
+                                               (tgv_deshade/tgv2_l1_deshade_2d.cu:33-215,
+                                               tgv_deshade_masked/tgv2_l1_deshade_masked_2d.cu)
+                                               - eps_B(v_bar) without z backward differences, so
+                                               v_z, p_z and the q z-components stay 0; the
+                                               widget (TGVDeshadeWidget.cpp:100-127) and the
+                                               masked processor (TGVDeshadeMaskedProcessor.cpp:93)
+                                               take this loop for every depth-1 image */
 
 typedef struct {
     float lambda;                /* data-term weight */

@@ -68,6 +68,12 @@ struct Trampoline {
         return fn(it, count, const_cast<float*>(u), const_cast<float*>(vx),
                   const_cast<float*>(vy), const_cast<float*>(vz)) ? 1 : 0;
     }
+    static int deshade_2d(uint32_t it, uint32_t count, const float* u, const float* vx,
+                          const float* vy, const float*, void* user) {
+        Fn& fn = *static_cast<Fn*>(user);
+        return fn(it, count, const_cast<float*>(u), const_cast<float*>(vx),
+                  const_cast<float*>(vy)) ? 1 : 0;
+    }
     static int denoise(uint32_t it, uint32_t count, const float* u, const float*, const float*,
                        const float*, void* user) {
         Fn& fn = *static_cast<Fn*>(user);
@@ -173,6 +179,83 @@ Pixel* tgv2_l1_deshade_masked_launch(
     return u;
 }
 
+// ---- reference tgv_deshade/tgv2_l1_deshade_2d.cu:27-36 (declared
+// TGVDeshadeProcessor.cpp:41-50) -------------------------------------------
+// The 2-D loop: AMB_FLAG_PLANAR at depth 1; v_z is computed as 0 and dropped.
+template <typename Pixel>
+Pixel* tgv2_l1_deshade_launch_2d(Pixel* f_host, ambient_amd_compat::uint width,
+                                 ambient_amd_compat::uint height, Pixel lambda,
+                                 ambient_amd_compat::uint iteration_count,
+                                 ambient_amd_compat::uint paint_iteration_interval,
+                                 std::function<bool(ambient_amd_compat::uint,
+                                                    ambient_amd_compat::uint, Pixel*, Pixel*,
+                                                    Pixel*)>
+                                     iteration_finished_callback,
+                                 Pixel alpha0, Pixel alpha1, Pixel** v_x_host, Pixel** v_y_host) {
+    using namespace ambient_amd_compat;
+    static_assert(std::is_same<Pixel, float>::value, "only the float instantiation exists");
+    typedef std::function<bool(uint, uint, Pixel*, Pixel*, Pixel*)> Fn;
+    const size_t n = size_t(width) * height;
+    amb_tgv_params p = reference_params(lambda, iteration_count, paint_iteration_interval,
+                                        alpha0, alpha1, AMB_DATA_L1);
+    p.flags |= AMB_FLAG_PLANAR;
+    Pixel* u = new Pixel[n];
+    *v_x_host = new Pixel[n];
+    *v_y_host = new Pixel[n];
+    amb_iteration_callback cb = iteration_finished_callback ? &Trampoline<Fn>::deshade_2d : nullptr;
+    check(amb_tgv2_deshade(context(), AMB_MEM_HOST, f_host, width, height, 1, &p, u, *v_x_host,
+                           *v_y_host, nullptr, cb, &iteration_finished_callback),
+          "tgv2_l1_deshade_launch_2d");
+    return u;
+}
+
+// ---- reference tgv_deshade_masked/tgv2_l1_deshade_masked_2d.cu (declared
+// TGVDeshadeMaskedProcessor.cpp:53-68) ---------------------------------------
+// The masked 2-D loop; the eight edge lists are accepted and not read.
+template <typename Pixel>
+Pixel* tgv2_l1_deshade_masked_2d_launch(
+    Pixel* f_host, ambient_amd_compat::uint width, ambient_amd_compat::uint height, Pixel lambda,
+    ambient_amd_compat::uint iteration_count, ambient_amd_compat::uint paint_iteration_interval,
+    const int cuda_block_dimension,
+    std::function<bool(ambient_amd_compat::uint, ambient_amd_compat::uint, Pixel*, Pixel*, Pixel*)>
+        iteration_finished_callback,
+    Pixel alpha0, Pixel alpha1, Pixel** v_x_host, Pixel** v_y_host,
+    const std::vector<ambient_amd_compat::uint>& masked_pixel_indices,
+    const std::vector<ambient_amd_compat::uint>& /*left_edge_pixel_indices*/,
+    const std::vector<ambient_amd_compat::uint>& /*not_left_edge_pixel_indices*/,
+    const std::vector<ambient_amd_compat::uint>& /*right_edge_pixel_indices*/,
+    const std::vector<ambient_amd_compat::uint>& /*not_right_edge_pixel_indices*/,
+    const std::vector<ambient_amd_compat::uint>& /*top_edge_pixel_indices*/,
+    const std::vector<ambient_amd_compat::uint>& /*not_top_edge_pixel_indices*/,
+    const std::vector<ambient_amd_compat::uint>& /*bottom_edge_pixel_indices*/,
+    const std::vector<ambient_amd_compat::uint>& /*not_bottom_edge_pixel_indices*/) {
+    using namespace ambient_amd_compat;
+    static_assert(std::is_same<Pixel, float>::value, "only the float instantiation exists");
+    (void)cuda_block_dimension;
+    typedef std::function<bool(uint, uint, Pixel*, Pixel*, Pixel*)> Fn;
+    const size_t n = size_t(width) * height;
+    std::vector<uint8_t> active(n, 0);
+    for (uint i : masked_pixel_indices) {
+        if (size_t(i) >= n) {
+            std::printf("ambient_amd error: masked pixel index %u outside the image\n", i);
+            std::exit(int(AMB_ERR_INVALID_ARGUMENT));
+        }
+        active[i] = 1;
+    }
+    amb_tgv_params p = reference_params(lambda, iteration_count, paint_iteration_interval,
+                                        alpha0, alpha1, AMB_DATA_L1);
+    p.flags |= AMB_FLAG_PLANAR;
+    Pixel* u = new Pixel[n];
+    *v_x_host = new Pixel[n];
+    *v_y_host = new Pixel[n];
+    amb_iteration_callback cb = iteration_finished_callback ? &Trampoline<Fn>::deshade_2d : nullptr;
+    check(amb_tgv2_deshade_masked(context(), AMB_MEM_HOST, f_host, width, height, 1, &p,
+                                  active.data(), u, *v_x_host, *v_y_host, nullptr, cb,
+                                  &iteration_finished_callback),
+          "tgv2_l1_deshade_masked_2d_launch");
+    return u;
+}
+
 // ---- reference tgv_cuda/tgv2_l1.cu:64-71 and tgv_cuda/tgv2_l2.cu:60-67 -------
 template <typename Pixel>
 Pixel* tgv2_denoise_launch_impl(uint32_t data_term, Pixel* f_host, ambient_amd_compat::uint width,
@@ -233,6 +316,22 @@ Pixel* divergence_kernel_launch(Pixel* dx, Pixel* dy, Pixel* dz, const ambient_a
     Pixel* out = new Pixel[size_t(width) * height * depth];
     check(amb_divergence(context(), AMB_MEM_HOST, dx, dy, dz, width, height, depth, out),
           "divergence_kernel_launch");
+    return out;
+}
+
+// ---- reference cuda_image_operations/divergence_2d.cu:59-117 ---------------
+// B_x dx + B_y dy: the 3-D divergence at depth 1 with dz = 0 (B_z of zeros adds 0).
+template <typename Pixel>
+Pixel* divergence_2d_kernel_launch(Pixel* dx, Pixel* dy, const ambient_amd_compat::uint width,
+                                   const ambient_amd_compat::uint height, bool is_host_data = false) {
+    using namespace ambient_amd_compat;
+    static_assert(std::is_same<Pixel, float>::value, "only the float instantiation exists");
+    (void)is_host_data;
+    const size_t n = size_t(width) * height;
+    std::vector<float> zeros(n, 0.0f);
+    Pixel* out = new Pixel[n];
+    check(amb_divergence(context(), AMB_MEM_HOST, dx, dy, zeros.data(), width, height, 1, out),
+          "divergence_2d_kernel_launch");
     return out;
 }
 

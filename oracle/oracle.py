@@ -136,9 +136,16 @@ class TGVState:
         c._ptrs()
         return c
 
+    def _planar(self, planar):
+        if planar and self.f.shape[0] != 1:
+            raise ValueError("the planar (2-D) loop needs depth 1")
+        return int(bool(planar))
+
     def iterate(self, iterations, lam, alpha0, alpha1, tau, sigma, theta=1.0, data_term="l2",
-                mask=None):
-        """mask (optional): updates only where mask >= 1 (the reference's masked deshade)."""
+                mask=None, planar=False):
+        """mask (optional): updates only where mask >= 1 (the reference's masked deshade).
+        planar: the reference's 2-D loop at depth 1 (tgv_deshade/tgv2_l1_deshade_2d.cu)."""
+        pl = self._planar(planar)
         prm = (ctypes.c_double * 6)(lam, alpha0, alpha1, tau, sigma, theta)
         dt = 0 if data_term == "l1" else 1
         act = None
@@ -146,25 +153,27 @@ class TGVState:
             act = np.ascontiguousarray(np.asarray(mask) >= 1, dtype=np.uint8)
         getattr(lib(), "orc_tgv2_iterate_" + _suf(self.dtype))(
             self._pp, _p(self.f), *_dims(self.f), prm, dt, int(iterations),
-            _p(act) if act is not None else None)
+            _p(act) if act is not None else None, pl)
         return self
 
-    def energy(self, lam, alpha0, alpha1, data_term="l2"):
+    def energy(self, lam, alpha0, alpha1, data_term="l2", planar=False):
         """FP64 energy/gap: [E, E_data, E_first, E_second, gap, b1, b2, b3,
         infeasibility, v_residual]."""
         prm = (ctypes.c_double * 6)(lam, alpha0, alpha1, 0.0, 0.0, 0.0)
         out = (ctypes.c_double * 10)()
         getattr(lib(), "orc_tgv2_energy_" + _suf(self.dtype))(
-            _p(self.f), self._pp, *_dims(self.f), prm, 0 if data_term == "l1" else 1, out)
+            _p(self.f), self._pp, *_dims(self.f), prm, 0 if data_term == "l1" else 1,
+            self._planar(planar), out)
         return np.array(out[:])
 
 
 def tgv2(f, iterations, lam, alpha0, alpha1, tau, sigma, theta=1.0, data_term="l2",
-         dtype=np.float64, mask=None):
+         dtype=np.float64, mask=None, planar=False):
     """Run the reference loop from the reference initialisation; returns the state.
-    With a mask: the masked loop (updates only where mask >= 1)."""
+    With a mask: the masked loop (updates only where mask >= 1).  planar: the
+    2-D loop at depth 1."""
     st = TGVState(f, dtype)
-    st.iterate(iterations, lam, alpha0, alpha1, tau, sigma, theta, data_term, mask)
+    st.iterate(iterations, lam, alpha0, alpha1, tau, sigma, theta, data_term, mask, planar)
     return st
 
 
@@ -227,11 +236,13 @@ def deshade_epilogue(u, l, mask=None, mask_shading=False, set_negative_zero=True
 
 
 def deshade(f, iterations, lam, alpha0, alpha1, tau, sigma, theta=1.0, data_term="l1",
-            mask=None, set_negative_zero=True, index_quirk=False, dtype=np.float32):
+            mask=None, set_negative_zero=True, index_quirk=False, dtype=np.float32,
+            planar=False):
     """Full reference pipeline (CLI overload, TGVDeshadeProcessor.cpp:122-166):
     TGV2 iteration -> Poisson integration of v -> r = u - l (mask, clamp).
-    Returns (u, l, r, state)."""
-    st = tgv2(f, iterations, lam, alpha0, alpha1, tau, sigma, theta, data_term, dtype)
+    planar: the 2-D loop at depth 1.  Returns (u, l, r, state)."""
+    st = tgv2(f, iterations, lam, alpha0, alpha1, tau, sigma, theta, data_term, dtype,
+              planar=planar)
     l = integrate_poisson(st.v[0], st.v[1], st.v[2], index_quirk, dtype)
     r, l2 = deshade_epilogue(st.u, l, mask, False, set_negative_zero, False)
     return st.u.copy(), l2, r, st
@@ -243,9 +254,11 @@ def deshade_masked(f, iterations, lam, alpha0, alpha1, tau, sigma, mask, theta=1
     """Masked deshade composition (TGVDeshadeMaskedProcessor.cpp:76-210): the
     masked TGV2-L1 loop on the unmasked f (updates only where mask >= 1), the
     Poisson integration of v, then l and r times the mask, clamp, and either
-    the add-back of u*(1 - mask) or r times the mask once more.
+    the add-back of u*(1 - mask) or r times the mask once more.  A depth-1
+    image takes the 2-D loop (TGVDeshadeMaskedProcessor.cpp:93-98).
     Returns (u, l, r, state)."""
-    st = tgv2(f, iterations, lam, alpha0, alpha1, tau, sigma, theta, "l1", dtype, mask=mask)
+    st = tgv2(f, iterations, lam, alpha0, alpha1, tau, sigma, theta, "l1", dtype, mask=mask,
+              planar=np.shape(f)[0] == 1)
     l = integrate_poisson(st.v[0], st.v[1], st.v[2], index_quirk, dtype)
     r, l2 = deshade_epilogue(st.u, l, mask, True, set_negative_zero, add_background_back,
                              mask_again=True)

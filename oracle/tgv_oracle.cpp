@@ -100,6 +100,7 @@ struct State {
 struct Params {
     double lambda, alpha0, alpha1, tau, sigma, theta;
     int data_term;  // 0 = L1, 1 = L2
+    int planar;     // depth 1: the 2-D loop (no z backward differences in eps_B(v_bar))
 };
 
 template <typename T>
@@ -199,13 +200,21 @@ void tgv2_iteration(State<T>& st, const T* f, const Dims& s, const Params& prm,
     backward_difference(st.vbar[0], t2, s, 1);                     // B_y v_x
 #pragma omp parallel for schedule(static)
     for (i64 i = 0; i < n; ++i) e[3][i] = (t1[i] + t2[i]) * T(0.5);     // xy
-    backward_difference(st.vbar[2], e[2].data(), s, 2);            // zz
+    // The 2-D loop (tgv_cuda/tgv2_common_2d.cu:151-172, launched by
+    // tgv_deshade/tgv2_l1_deshade_2d.cu:112-252) has no z backward differences:
+    // planar replaces every B_z term by 0, so e_zz, e_xz, e_yz and with them
+    // q_z*, v_z, p_z stay 0 and the 3-D kernels reduce to the 2-D ones.
+    auto bz = [&](const T* a, T* out) {
+        if (prm.planar) std::fill(out, out + n, T(0));
+        else backward_difference(a, out, s, 2);
+    };
+    bz(st.vbar[2], e[2].data());                                   // zz
     backward_difference(st.vbar[2], t1, s, 0);                     // B_x v_z
-    backward_difference(st.vbar[0], t2, s, 2);                     // B_z v_x
+    bz(st.vbar[0], t2);                                            // B_z v_x
 #pragma omp parallel for schedule(static)
     for (i64 i = 0; i < n; ++i) e[4][i] = (t1[i] + t2[i]) * T(0.5);     // xz
     backward_difference(st.vbar[2], t1, s, 1);                     // B_y v_z
-    backward_difference(st.vbar[1], t2, s, 2);                     // B_z v_y
+    bz(st.vbar[1], t2);                                            // B_z v_y
 #pragma omp parallel for schedule(static)
     for (i64 i = 0; i < n; ++i) e[5][i] = (t1[i] + t2[i]) * T(0.5);     // yz
 
@@ -341,6 +350,8 @@ void cosine_transform(const T* in, T* out, const Dims& s, bool inverse) {
 //   the denominator and the quotient are double, the store is float.
 //   index_quirk = 1 reproduces the reference's coordinate recovery through
 //   float arithmetic (exact only for linear indices below 2^24).
+//   Depth 1: the 2-D kernel, X / (4.0 - 2.0*cospif(x/W) - 2.0*cospif(y/H))
+//   (solve_poisson_in_cosine_domain.cu:57-84, chosen at :107).
 // ---------------------------------------------------------------------------
 void poisson_coords(const Dims& s, i64 index, int quirk, double& xd, double& yd, double& zd) {
     if (!quirk) {
@@ -368,10 +379,12 @@ void solve_poisson(const T* in, T* out, const Dims& s, int quirk) {
             const float cx = float(cospi_d(double(float(x) / float(s.w))));
             const float cy = float(cospi_d(double(float(y) / float(s.h))));
             const float cz = float(cospi_d(double(float(z) / float(s.d))));
-            den = 6.0 - 2.0 * double(cx) - 2.0 * double(cy) - 2.0 * double(cz);
+            den = s.d == 1 ? 4.0 - 2.0 * double(cx) - 2.0 * double(cy)
+                           : 6.0 - 2.0 * double(cx) - 2.0 * double(cy) - 2.0 * double(cz);
         } else {
-            den = 6.0 - 2.0 * cospi_d(x / double(s.w)) - 2.0 * cospi_d(y / double(s.h))
-                  - 2.0 * cospi_d(z / double(s.d));
+            den = s.d == 1 ? 4.0 - 2.0 * cospi_d(x / double(s.w)) - 2.0 * cospi_d(y / double(s.h))
+                           : 6.0 - 2.0 * cospi_d(x / double(s.w)) - 2.0 * cospi_d(y / double(s.h))
+                                 - 2.0 * cospi_d(z / double(s.d));
         }
         out[i] = T(double(in[i]) / den);
     }
@@ -423,6 +436,7 @@ void tgv2_energy(const T* f, const T* u, T* const* v, T* const* p, T* const* q,
     const double lam = prm.lambda, a0 = prm.alpha0, a1 = prm.alpha1;
     auto at = [&](const T* a, i64 i) -> double { return double(a[i]); };
     double Ed = 0, E1 = 0, E2 = 0, b1 = 0, b2 = 0, b3 = 0, inf = 0, vr = 0;
+    const int na = prm.planar ? 2 : 3;   // planar: no z terms in eps_B and its adjoint
 #pragma omp parallel for schedule(static) reduction(+:Ed,E1,E2,b1,b2,b3,inf,vr)
     for (i64 i = 0; i < n; ++i) {
         const i64 xc[3] = {coord(s, i, 0), coord(s, i, 1), coord(s, i, 2)};
@@ -431,7 +445,7 @@ void tgv2_energy(const T* f, const T* u, T* const* v, T* const* p, T* const* q,
             const i64 st = s.stride(a), ext = s.extent(a);
             du[a] = (xc[a] < ext - 1) ? at(u, i + st) - at(u, i) : 0.0;
             for (int c = 0; c < 3; ++c)   // B_a v_c
-                bv[a][c] = (xc[a] > 0) ? at(v[c], i - st) - at(v[c], i) : -at(v[c], i);
+                bv[a][c] = a >= na ? 0.0 : (xc[a] > 0) ? at(v[c], i - st) - at(v[c], i) : -at(v[c], i);
         }
         // s = -(D+)^T p : ((D+)^T b)[j] = b[j-1]*[j>=1] - b[j]*[j<n-1]
         double sdiv = 0;
@@ -472,9 +486,9 @@ void tgv2_energy(const T* f, const T* u, T* const* v, T* const* p, T* const* q,
         // B^T = D+ - E_last :  (B^T b)[j] = b[j+1]*[j<n-1] - b[j]
         // <eps v, q>_w = sum_c v_c * sum_a (B_a^T qq(a,c)), qq(a,a)=q_aa, qq(a,c)=q_ac (a!=c)
         static const int qi[3][3] = {{0, 3, 4}, {3, 1, 5}, {4, 5, 2}};
-        for (int c = 0; c < 3; ++c) {
+        for (int c = 0; c < na; ++c) {
             double t = 0;
-            for (int a = 0; a < 3; ++a) {
+            for (int a = 0; a < na; ++a) {
                 const i64 st = s.stride(a), ext = s.extent(a);
                 const T* qq = q[qi[a][c]];
                 double bt = -at(qq, i);
@@ -505,9 +519,10 @@ ORC_EXPORT void orc_backward_difference_##SUF(const T* a, T* out, i64 w, i64 h, 
 ORC_EXPORT void orc_tgv2_init_##SUF(T** fields, const T* f, i64 w, i64 h, i64 d) {           \
     State<T> st; fill_state_pointers(st, fields); tgv2_init(st, f, Dims{w, h, d}); }           \
 ORC_EXPORT int orc_tgv2_iterate_##SUF(T** fields, const T* f, i64 w, i64 h, i64 d,            \
-        const double* prm6, int data_term, int iterations, const unsigned char* active) {       \
+        const double* prm6, int data_term, int iterations, const unsigned char* active,         \
+        int planar) {                                                                           \
     State<T> st; fill_state_pointers(st, fields);                                               \
-    Params prm{prm6[0], prm6[1], prm6[2], prm6[3], prm6[4], prm6[5], data_term};                \
+    Params prm{prm6[0], prm6[1], prm6[2], prm6[3], prm6[4], prm6[5], data_term, planar};        \
     const Dims s{w, h, d};                                                                      \
     std::vector<T> tmp[9];                                                                      \
     for (auto& t : tmp) t.assign(size_t(s.n()), T(0));                                          \
@@ -526,9 +541,9 @@ ORC_EXPORT void orc_solve_poisson_##SUF(const T* in, T* out, i64 w, i64 h, i64 d
 ORC_EXPORT void orc_integrate_poisson_##SUF(const T* vx, const T* vy, const T* vz, T* out,    \
         i64 w, i64 h, i64 d, int quirk) { integrate_poisson<T>(vx, vy, vz, out, Dims{w, h, d}, quirk); } \
 ORC_EXPORT void orc_tgv2_energy_##SUF(const T* f, T** fields, i64 w, i64 h, i64 d,            \
-        const double* prm6, int data_term, double* out) {                                       \
+        const double* prm6, int data_term, int planar, double* out) {                           \
     State<T> st; fill_state_pointers(st, fields);                                               \
-    Params prm{prm6[0], prm6[1], prm6[2], prm6[3], prm6[4], prm6[5], data_term};                \
+    Params prm{prm6[0], prm6[1], prm6[2], prm6[3], prm6[4], prm6[5], data_term, planar};        \
     tgv2_energy<T>(f, st.u, st.v, st.p, st.q, Dims{w, h, d}, prm, out); }
 
 ORC_DEFINE_TYPED(float, f32)

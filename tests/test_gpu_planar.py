@@ -1,0 +1,256 @@
+"""GPU tests of the 2-D loop at depth 1 (AMB_FLAG_PLANAR): the reference's
+tgv_deshade/tgv2_l1_deshade_2d.cu and tgv_deshade_masked/tgv2_l1_deshade_masked_2d.cu,
+which the widget (TGVDeshadeWidget.cpp:100-127) and the masked processor
+(TGVDeshadeMaskedProcessor.cpp:93-98) take for every depth-1 image.
+
+The planar sweeps perform the oracle's planar operations in the same order,
+so iteration results are compared bit for bit with the FP32 oracle, on the
+scalar, vector and fused sweep paths; the 2-D Poisson solve (denominator
+4 - 2cx - 2cy) is checked against a closed form on every DCT precision."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import rel_err
+
+pytestmark = pytest.mark.gpu
+
+TAU = float(np.float32(1 / np.sqrt(20)))
+PRM = dict(lam=0.5, alpha0=2.0, alpha1=1.0, tau=TAU, sigma=TAU, theta=1.0)
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _params(native, iterations, data_term="l1", flags=0, **kw):
+    return native.default_params(lam=PRM["lam"], alpha0=PRM["alpha0"], alpha1=PRM["alpha1"],
+                                 tau=TAU, sigma=TAU, theta=1.0, iterations=iterations,
+                                 data_term=native.DATA_L1 if data_term == "l1" else native.DATA_L2,
+                                 flags=flags | native.FLAG_PLANAR, **kw)
+
+
+def _f(shape, seed):
+    from ambient_image_processor_amd import phantoms
+    return phantoms.deshade_phantom(shape, seed=seed)[0]
+
+
+def _mask(shape, seed):
+    rng = np.random.default_rng(seed)
+    d, h, w = shape
+    yy, xx = np.mgrid[0:h, 0:w]
+    m = ((yy - h / 2) ** 2 / (0.4 * h) ** 2 + (xx - w / 2) ** 2 / (0.45 * w) ** 2) < 1
+    m = m.astype(np.float32)[None]
+    return np.where(m > 0, 1.0, rng.random(shape) * 0.999).astype(np.float32)
+
+
+# odd widths: one pixel per thread; widths % 4 == 0: the vector sweeps
+SHAPES = [(1, 13, 21), (1, 12, 32), (1, 9, 256), (1, 40, 68), (1, 3, 8)]
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+@pytest.mark.parametrize("data_term", ["l1", "l2"])
+@pytest.mark.parametrize("fused", ["0", "1"])
+def test_planar_iteration_bitwise(ctx, orc, shape, data_term, fused, monkeypatch):
+    from ambient_image_processor_amd import native
+    _say("planar", shape, data_term, "fused", fused)
+    f = _f(shape, 3)
+    monkeypatch.setenv("AMB_FUSED", fused)
+    u, vx, vy, vz = ctx.tgv2_deshade(f, _params(native, 31, data_term))
+    st = orc.tgv2(f, 31, data_term=data_term, dtype=np.float32, planar=True, **PRM)
+    np.testing.assert_array_equal(u, st.u)
+    np.testing.assert_array_equal(vx, st.v[0])
+    np.testing.assert_array_equal(vy, st.v[1])
+    assert not np.any(vz)
+    state = ctx.get_state()
+    for k in (4, 7, 10, 13, 15, 16):     # v_z, v_bar_z, p_z, q_zz, q_xz, q_yz
+        assert not np.any(state[k]), k
+
+
+@pytest.mark.parametrize("vx,ty,zchunk,pf", [("4", "3", "1", "1"), ("2", "5", "1", "0")])
+def test_planar_forced_fused_plans(ctx, orc, monkeypatch, vx, ty, zchunk, pf):
+    from ambient_image_processor_amd import native
+    shape = (1, 23, 64)
+    _say("planar fused plan", vx, ty, zchunk, pf, shape)
+    f = _f(shape, 4)
+    for k, v in (("VX", vx), ("TY", ty), ("ZCHUNK", zchunk), ("PF", pf)):
+        monkeypatch.setenv("AMB_FUSED_" + k, v)
+    u, vxo, vyo, _ = ctx.tgv2_deshade(f, _params(native, 17))
+    st = orc.tgv2(f, 17, data_term="l1", dtype=np.float32, planar=True, **PRM)
+    np.testing.assert_array_equal(u, st.u)
+    np.testing.assert_array_equal(vyo, st.v[1])
+
+
+def test_planar_differs_from_3d_at_depth1(ctx):
+    """Without the flag the 3-D loop runs at depth 1 (the CLI overload does
+    that): v_z grows there and stays 0 in the 2-D loop."""
+    from ambient_image_processor_amd import native
+    f = _f((1, 16, 24), 5)
+    p = _params(native, 40)
+    p.flags &= ~native.FLAG_PLANAR
+    _, _, _, vz3 = ctx.tgv2_deshade(f, p)
+    assert np.abs(vz3).max() > 1e-4
+
+
+def test_planar_needs_depth_1(ctx):
+    from ambient_image_processor_amd import native
+    with pytest.raises(native.AmbError):
+        ctx.tgv2_deshade(_f((2, 8, 8), 1), _params(native, 3))
+
+
+@pytest.mark.parametrize("fused", ["0", "1"])
+def test_planar_masked_bitwise(ctx, orc, fused, monkeypatch):
+    from ambient_image_processor_amd import native
+    shape = (1, 30, 44)
+    _say("planar masked fused", fused)
+    f = _f(shape, 6)
+    m = _mask(shape, 7)
+    monkeypatch.setenv("AMB_FUSED", fused)
+    u, vx, vy, vz = ctx.tgv2_deshade_masked(f, _params(native, 25), m >= 1)
+    st = orc.tgv2(f, 25, data_term="l1", dtype=np.float32, mask=m, planar=True, **PRM)
+    np.testing.assert_array_equal(u, st.u)
+    np.testing.assert_array_equal(vx, st.v[0])
+    np.testing.assert_array_equal(vy, st.v[1])
+    assert not np.any(vz)
+
+
+@pytest.mark.parametrize("data_term", ["l1", "l2"])
+def test_planar_energy_matches_fp64_oracle(ctx, orc, data_term):
+    from ambient_image_processor_amd import native
+    f = _f((1, 40, 52), 8)
+    ctx.begin(f, _params(native, 0, data_term))
+    ctx.iterate(30)
+    e_gpu = ctx.energy()
+    ref = orc.TGVState(f, np.float64)
+    ref.fields[:] = ctx.get_state()
+    e_ref = ref.energy(PRM["lam"], PRM["alpha0"], PRM["alpha1"], data_term, planar=True)
+    for k in range(10):
+        scale = max(abs(e_ref[k]), 1e-12 * abs(e_ref[0]))
+        assert abs(e_gpu[k] - e_ref[k]) <= 1e-9 * scale + 1e-12, (k, e_gpu[k], e_ref[k])
+
+
+# FP32 arithmetic, amplified by the small eigenvalue of mode (1, 0) (2 - 2cos(pi/80)
+# = 1.5e-3): the FP32 oracle itself ends 1.37e-5 from the closed form
+@pytest.mark.parametrize("precision,tol", [(0, 3e-5), (1, 3e-5), (2, 1e-4)])
+def test_depth1_poisson_known_answer(ctx, orc, precision, tol):
+    """v = D+ phi of a zero-mean field of cosine modes at depth 1: the
+    integration returns phi (2-D Neumann Laplacian, denominator 4 - 2cx - 2cy)."""
+    from ambient_image_processor_amd import phantoms
+    phi = phantoms.separable_modes((1, 48, 80), {(3, 1, 0): 0.7, (0, 2, 0): -0.4,
+                                                 (5, 4, 0): 0.25, (1, 0, 0): 1.1,
+                                                 (11, 7, 0): 0.05})
+    v = [orc.forward_difference(phi, a).astype(np.float32) for a in range(3)]
+    assert not np.any(v[2])
+    l = ctx.integrate_gradients_poisson(*v, precision=precision)
+    e = rel_err(l, phi - phi.mean())
+    _say(f"depth-1 Poisson precision {precision}: rel err {e:.2e}")
+    assert e < tol
+    if precision == 0:
+        ref = orc.integrate_poisson(*v, dtype=np.float32)
+        assert rel_err(l, ref) < 1e-5
+
+
+def test_masked_processor_takes_2d_loop_at_depth1(orc):
+    """TGVDeshadeMaskedProcessor.processTGV2L1GPUCuda on a depth-1 image: the
+    masked 2-D loop and the masked composition (add-back off and on)."""
+    from ambient_image_processor_amd.processor import TGVDeshadeMaskedProcessor
+    shape = (1, 36, 40)
+    f = _f(shape, 9)
+    m = _mask(shape, 10)
+    for add_back in (False, True):
+        r, u, l, div_v = TGVDeshadeMaskedProcessor.processTGV2L1GPUCuda(
+            f, PRM["lam"], PRM["alpha0"], PRM["alpha1"], 35, mask=m,
+            add_background_back=add_back, calculate_div_v=True)
+        uo, lo, ro, st = orc.deshade_masked(f, 35, mask=m, add_background_back=add_back, **PRM)
+        np.testing.assert_array_equal(u, uo)
+        assert rel_err(l, lo) < 1e-5 and rel_err(r, ro) < 1e-5
+        np.testing.assert_array_equal(
+            div_v, orc.divergence(st.v[0], st.v[1], np.zeros_like(st.v[2]), dtype=np.float32))
+
+
+def test_reference_2d_launchers(orc):
+    from ambient_image_processor_amd import processor
+    shape = (1, 20, 28)
+    f = _f(shape, 11)
+    d, h, w = shape
+    seen = []
+    u, vx, vy = processor.tgv2_l1_deshade_launch_2d(
+        f.ravel(), w, h, PRM["lam"], 30, 10, lambda it, n, uu, a, b: seen.append(it) or False,
+        PRM["alpha0"], PRM["alpha1"])
+    st = orc.tgv2(f, 30, data_term="l1", dtype=np.float32, planar=True, **PRM)
+    assert seen == [10, 20]
+    np.testing.assert_array_equal(u, st.u)
+    np.testing.assert_array_equal(vx, st.v[0])
+    m = _mask(shape, 12)
+    lists = processor.TGVDeshadeMaskedProcessor.buildMaskIndices2D(m)
+    assert len(lists) == 9
+    u2, vx2, vy2 = processor.tgv2_l1_deshade_masked_2d_launch(
+        f.ravel(), w, h, PRM["lam"], 30, 0, -1, None, PRM["alpha0"], PRM["alpha1"], *lists)
+    st2 = orc.tgv2(f, 30, data_term="l1", dtype=np.float32, planar=True, mask=m, **PRM)
+    np.testing.assert_array_equal(u2, st2.u)
+    np.testing.assert_array_equal(vy2, st2.v[1])
+
+
+def test_widget_2d_processor(orc):
+    """TGVDeshadeProcessor.processTGV2L1GPUCuda2D with a mask and add-back,
+    and its callback's (u, l, r)."""
+    from ambient_image_processor_amd.processor import TGVDeshadeProcessor
+    shape = (1, 24, 36)
+    f = _f(shape, 13)
+    m = (np.random.default_rng(14).random(shape) > 0.3).astype(np.float32)
+    got = {}
+
+    def cb(it, n, u, l, r):
+        got[it] = (u.copy(), l.copy(), r.copy())
+        return False
+
+    r, u, l = TGVDeshadeProcessor.processTGV2L1GPUCuda2D(
+        f[0], PRM["lam"], PRM["alpha0"], PRM["alpha1"], 40, 20, cb, mask=m,
+        add_background_back=True)
+    st = orc.tgv2(f, 40, data_term="l1", dtype=np.float32, planar=True, **PRM)
+    lo = orc.integrate_poisson(*st.v, dtype=np.float32)
+    ro, lo2 = orc.deshade_epilogue(st.u, lo, m, mask_shading=True, set_negative_zero=True,
+                                   add_background_back=True)
+    np.testing.assert_array_equal(u, st.u)
+    assert rel_err(l, lo2) < 1e-5 and rel_err(r, ro) < 1e-5
+    # one call, at iteration_index 20, after 21 iterations
+    assert sorted(got) == [20]
+    st21 = orc.tgv2(f, 21, data_term="l1", dtype=np.float32, planar=True, **PRM)
+    np.testing.assert_array_equal(got[20][0], st21.u)
+    lo = orc.integrate_poisson(*st21.v, dtype=np.float32)
+    ro, lo2 = orc.deshade_epilogue(st21.u, lo, m, mask_shading=True, set_negative_zero=True,
+                                   add_background_back=True)
+    assert rel_err(got[20][1], lo2) < 1e-5 and rel_err(got[20][2], ro) < 1e-5
+
+
+@pytest.mark.parametrize("masked", [False, True])
+def test_reference_launcher_caller_2d(orc, tmp_path, masked):
+    """tests/compat/processor_demo.cpp --2d: the 2-D launchers of the
+    compatibility header, then divergence_2d and the Poisson solve."""
+    import subprocess
+    from ambient_image_processor_amd import _build
+    assert os.path.exists(_build.COMPAT_DEMO), "run __graft_entry__.build() first"
+    shape = (1, 26, 34)
+    f = _f(shape, 15)
+    m = _mask(shape, 16)
+    d, h, w = shape
+    f.tofile(tmp_path / "f.raw")
+    m.tofile(tmp_path / "m.raw")
+    out = [str(tmp_path / n) for n in ("u.raw", "l.raw", "r.raw")]
+    _say("running", _build.COMPAT_DEMO, "--2d masked" if masked else "--2d")
+    r = subprocess.run([_build.COMPAT_DEMO, "--2d", str(tmp_path / "f.raw"), str(w), str(h),
+                        str(d), "0.5", "2.0", "1.0", "30"] + out
+                       + ([str(tmp_path / "m.raw")] if masked else []),
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    u, l, rr = (np.fromfile(p, dtype=np.float32).reshape(shape) for p in out)
+    if masked:
+        uo, lo, ro, _ = orc.deshade_masked(f, 30, mask=m, add_background_back=False, **PRM)
+    else:
+        uo, lo, ro, _ = orc.deshade(f, 30, planar=True, **PRM)
+    np.testing.assert_array_equal(u, uo)
+    np.testing.assert_array_equal(l, lo)
+    np.testing.assert_array_equal(rr, ro)

@@ -399,3 +399,115 @@ def test_deshade_epilogue_special_cases(orc):
     half = np.full_like(u, 0.5)
     r, _ = orc.deshade_epilogue(u, l, half, True, False, mask_again=True)
     np.testing.assert_allclose(r, d * 0.25, rtol=1e-6, atol=1e-7)
+
+
+def test_poisson_known_answer_depth1(orc):
+    """Depth 1 (the reference's 2-D Poisson kernel, denominator 4 - 2cx - 2cy):
+    DCT -> divide -> iDCT inverts the 2-D Neumann Laplacian of a zero-mean field
+    built from cosine modes."""
+    from ambient_image_processor_amd import phantoms
+    phi = phantoms.separable_modes((1, 12, 17), {(3, 1, 0): 0.7, (0, 2, 0): -0.4,
+                                                 (5, 4, 0): 0.25, (1, 0, 0): 1.1})
+    rhs = phantoms.neumann_laplacian(phi)   # sum over axes of (D+)^T D+ phi
+    sol = orc.inverse_cosine_transform(orc.solve_poisson(orc.cosine_transform(rhs)))
+    assert rel_err(sol, phi) < 1e-12
+    sol32 = orc.inverse_cosine_transform(
+        orc.solve_poisson(orc.cosine_transform(rhs, np.float32), dtype=np.float32), np.float32)
+    assert rel_err(sol32, phi) < 1e-5
+
+
+# ---------------------------------------------------------------------------
+# The 2-D loop at depth 1 (planar)
+# ---------------------------------------------------------------------------
+def _numpy_iteration_2d(f2, iters, lam, alpha0, alpha1, tau, sigma, theta, data_term):
+    """The reference's 2-D loop on (H, W) arrays, kernel by kernel from
+    tgv_deshade/tgv2_l1_deshade_2d.cu:95-240 with tgv_cuda/tgv2_common_2d.cu
+    (part22 :79-121, gradient2 :151-172, part5 :174-219, divergence2 :221-245,
+    part6 :248-271) and tgv_cuda/tgv2_l1_2d.cu:27-60; numpy slicing, FP64."""
+    def dplus(a, ax):
+        out = np.zeros_like(a)
+        if ax == 1:
+            out[:, :-1] = a[:, 1:] - a[:, :-1]
+        else:
+            out[:-1, :] = a[1:, :] - a[:-1, :]
+        return out
+
+    def bminus(a, ax):
+        out = -a.copy()
+        if ax == 1:
+            out[:, 1:] += a[:, :-1]
+        else:
+            out[1:, :] += a[:-1, :]
+        return out
+
+    X, Y = 1, 0
+    u = f2.copy(); ub = f2.copy()
+    vx = np.zeros_like(f2); vy = vx.copy(); vbx = vx.copy(); vby = vx.copy()
+    px = vx.copy(); py = vx.copy(); qx = vx.copy(); qy = vx.copy(); qxy = vx.copy()
+    for _ in range(iters):
+        px = px + sigma * (dplus(ub, X) - vbx)
+        py = py + sigma * (dplus(ub, Y) - vby)
+        nrm = np.maximum(1, np.sqrt(px * px + py * py) / alpha1)
+        px, py = px / nrm, py / nrm
+        u_old = u
+        u = u - tau * (bminus(px, X) + bminus(py, Y))
+        if data_term == "l1":
+            d = u - f2
+            u = np.where(d > tau * lam, u - tau * lam, np.where(d < -tau * lam, u + tau * lam, f2))
+        else:
+            u = (u + tau * lam * f2) / (1 + tau * lam)
+        ub = u + theta * (u - u_old)
+        ex, ey = bminus(vbx, X), bminus(vby, Y)
+        exy = (bminus(vby, X) + bminus(vbx, Y)) * 0.5
+        qx, qy, qxy = qx + sigma * ex, qy + sigma * ey, qxy + sigma * exy
+        nrm = np.maximum(1, np.sqrt(qx * qx + qy * qy + 2 * qxy * qxy) / alpha0)
+        qx, qy, qxy = qx / nrm, qy / nrm, qxy / nrm
+        vx_old, vy_old = vx, vy
+        vx = vx - tau * (dplus(qx, X) + dplus(qxy, Y) - px)
+        vy = vy - tau * (dplus(qy, Y) + dplus(qxy, X) - py)
+        vbx = vx + theta * (vx - vx_old)
+        vby = vy + theta * (vy - vy_old)
+    return u, (vx, vy), (px, py), (qx, qy, qxy)
+
+
+@pytest.mark.parametrize("data_term", ["l1", "l2"])
+def test_planar_iteration_matches_2d_restatement(orc, data_term):
+    """The planar oracle equals an independent 2-D restatement; its z
+    components stay exactly 0, unlike those of the 3-D loop at depth 1 (whose
+    z backward differences read -v at index 0)."""
+    from ambient_image_processor_amd import phantoms
+    f = phantoms.deshade_phantom((1, 13, 17), seed=4)[0].astype(np.float64)
+    st = orc.tgv2(f, 20, data_term=data_term, planar=True, **PRM)
+    u, v, p, q = _numpy_iteration_2d(f[0], 20, data_term=data_term, **PRM)
+    assert rel_err(st.u[0], u) < 1e-13
+    assert rel_err(st.v[:2, 0], np.stack(v)) < 1e-12
+    assert rel_err(st.p[:2, 0], np.stack(p)) < 1e-12
+    assert rel_err(np.stack([st.q[0, 0], st.q[1, 0], st.q[3, 0]]), np.stack(q)) < 1e-12
+    for a in (st.v[2], st.p[2], st.q[2], st.q[4], st.q[5]):
+        assert not np.any(a)
+    # the 3-D loop at depth 1 grows v_z (through q_xz, q_yz); x and y differ
+    # from the planar loop only where the q projection is active
+    st3 = orc.tgv2(f, 20, data_term=data_term, **PRM)
+    assert np.abs(st3.v[2]).max() > 1e-4
+    with pytest.raises(ValueError):
+        orc.tgv2(np.zeros((2, 3, 4)), 1, planar=True, **PRM)
+
+
+def test_planar_energy_is_the_2d_energy(orc):
+    """Planar energy: the second-order term is alpha0 * sum |eps_B v|_w with the
+    2-D eps_B (no z terms); the other terms are those of the 3-D energy."""
+    from ambient_image_processor_amd import phantoms
+    f = phantoms.deshade_phantom((1, 11, 14), seed=6)[0].astype(np.float64)
+    st = orc.tgv2(f, 30, data_term="l1", planar=True, **PRM)
+    e = st.energy(PRM["lam"], PRM["alpha0"], PRM["alpha1"], "l1", planar=True)
+    e3 = st.energy(PRM["lam"], PRM["alpha0"], PRM["alpha1"], "l1")
+    vx, vy = st.v[0, 0], st.v[1, 0]
+    bx = lambda a: np.concatenate([-a[:, :1], a[:, :-1] - a[:, 1:]], axis=1)   # noqa: E731
+    by = lambda a: np.concatenate([-a[:1, :], a[:-1, :] - a[1:, :]], axis=0)   # noqa: E731
+    exy = 0.5 * (bx(vy) + by(vx))
+    e2 = PRM["alpha0"] * np.sqrt(bx(vx) ** 2 + by(vy) ** 2 + 2 * exy ** 2).sum()
+    assert abs(e[3] - e2) < 1e-12 * e2
+    # the OpenMP reduction combines per-thread sums in no fixed order
+    assert abs(e[1] - e3[1]) <= 1e-13 * e3[1] and abs(e[2] - e3[2]) <= 1e-13 * e3[2]
+    assert e[3] < e3[3]           # the 3-D term adds |B_z v| = |v| at depth 1
+    assert e[6] >= -1e-12 and e[7] >= -1e-12
