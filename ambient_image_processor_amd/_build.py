@@ -20,7 +20,8 @@ CSRC = os.path.join(PKG, "csrc")
 BUILD_DIR = os.path.join(ROOT, "_build")
 LIB_NAME = "libambient_amd.so"
 LIB_PATH = os.path.join(PKG, LIB_NAME)
-SOURCES = ["tgv_sweeps.hip", "dct.hip", "dct_mfma.hip", "energy.hip", "adjoint.hip", "capi.hip"]
+SOURCES = ["tgv_sweeps.hip", "tgvk.hip", "dct.hip", "dct_mfma.hip", "energy.hip", "adjoint.hip",
+           "capi.hip"]
 ARCH = "gfx950"
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 

@@ -150,6 +150,16 @@ hipError_t launch_epilogue(hipStream_t s, const float* u, float* l, const float*
                            int add_background_back, int mask_again);
 hipError_t launch_active_from_mask(hipStream_t s, const float* mask, uint8_t* act, size_t n);
 
+// TGV-k (k >= 3) sweeps (tgvk.hip).  fields_dev: device table of the 17 state
+// fields; levels_dev: device table of 18 pointers per level (w[6], w_bar[6],
+// r[6]); alpha_dev: the k weights.  FP32 storage.
+hipError_t launch_tgvk_dual(hipStream_t s, float* const* fields_dev, float* const* levels_dev,
+                            const float* alpha_dev, Geometry g, int levels, float sigma,
+                            int planar);
+hipError_t launch_tgvk_primal(hipStream_t s, float* const* fields_dev, float* const* levels_dev,
+                              const float* f, Geometry g, int levels, float tau, float lambda,
+                              float theta);
+
 // FP64 energy / gap (energy.hip).  partials: at least energy_partials_size(g) doubles.
 size_t energy_partials_size(Geometry g);
 hipError_t launch_energy(hipStream_t s, int storage, void* const* fields, const float* f,

@@ -52,9 +52,9 @@ amb_status fail_hip(hipError_t e, const char* what, int line) {
     } while (0)
 
 constexpr int N_SCRATCH = 5;
-constexpr int N_PROF = 10;
+constexpr int N_PROF = 12;
 enum { P_DUAL = 0, P_PRIMAL = 1, P_DCT = 2, P_POISSON = 3, P_DIV = 4, P_EPI = 5, P_INIT = 6,
-       P_ENERGY = 7, P_ADJOINT = 8, P_FUSED = 9 };
+       P_ENERGY = 7, P_ADJOINT = 8, P_FUSED = 9, P_TGVK_DUAL = 10, P_TGVK_PRIMAL = 11 };
 
 struct AxisTables {
     int n = 0;
@@ -114,6 +114,17 @@ struct amb_context {
     void* alt[N_FUSED_FIELDS] = {};
     bool fused_ok = false;
     FusedPlan fplan{0, 0};
+    // TGV-k: 18 fields per level (w[6], w_bar[6], r[6]; grow-only), their
+    // device pointer table and the k weights
+    char* kblock = nullptr;
+    size_t kcap = 0;
+    float** klv_dev = nullptr;
+    size_t klv_cap = 0;
+    float* kalpha_dev = nullptr;
+    size_t kalpha_cap = 0;
+    std::vector<float*> klv_host;
+    std::vector<float> kalpha_host;
+    int klevels = 0;
 };
 
 namespace {
@@ -546,6 +557,99 @@ amb_status run(amb_context* c, uint32_t iterations, uint32_t interval, amb_itera
     return AMB_OK;
 }
 
+// TGV-k (k >= 3): the TGV2 state from begin() plus k - 2 levels of w, w_bar, r,
+// all zero at the start (tgvk_deshade/tgvk_l1_deshade.cu:116-151).
+amb_status tgvk_begin(amb_context* c, amb_memory mem, const float* f, uint32_t w, uint32_t h,
+                      uint32_t d, const amb_tgv_params* p, uint32_t order, const float* alpha) {
+    if (!p) return fail(AMB_ERR_INVALID_ARGUMENT, "params is NULL");
+    if (order < 3 || order > 1000) return fail(AMB_ERR_INVALID_ARGUMENT, "order must be 3 .. 1000");
+    if (!alpha) return fail(AMB_ERR_INVALID_ARGUMENT, "alpha is NULL");
+    for (uint32_t k = 0; k < order; ++k)
+        if (!(alpha[k] > 0.f)) return fail(AMB_ERR_INVALID_ARGUMENT, "alpha weights must be positive");
+    if (p->data_term != AMB_DATA_L1)
+        return fail(AMB_ERR_NOT_SUPPORTED, "the TGV-k deshade has the L1 data term only");
+    if (p->storage != AMB_STORAGE_F32)
+        return fail(AMB_ERR_NOT_SUPPORTED, "the TGV-k deshade runs on FP32 fields");
+    if (h > 4u * 65535u) return fail(AMB_ERR_NOT_SUPPORTED, "height above 262140");
+    amb_tgv_params pk = *p;   // the TGV2 part's weights come from alpha
+    pk.alpha0 = alpha[1];
+    pk.alpha1 = alpha[0];
+    AMB_TRY(begin(c, mem, f, w, h, d, &pk));
+    const int L = int(order) - 2;
+    const size_t fstride = align_up(c->g.n() * 4, 256);
+    const size_t need = fstride * 18 * size_t(L);
+    if (need > c->kcap) {
+        if (c->kblock) AMB_HIP(hipFree(c->kblock));
+        c->kblock = nullptr;
+        c->kcap = 0;
+        AMB_HIP(hipMalloc(reinterpret_cast<void**>(&c->kblock), need));
+        c->kcap = need;
+    }
+    AMB_HIP(hipMemsetAsync(c->kblock, 0, need, c->stream));
+    const size_t nt = 18 * size_t(L);
+    if (nt > c->klv_cap) {
+        if (c->klv_dev) AMB_HIP(hipFree(c->klv_dev));
+        c->klv_dev = nullptr;
+        c->klv_cap = 0;
+        AMB_HIP(hipMalloc(reinterpret_cast<void**>(&c->klv_dev), nt * sizeof(float*)));
+        c->klv_cap = nt;
+    }
+    if (order > c->kalpha_cap) {
+        if (c->kalpha_dev) AMB_HIP(hipFree(c->kalpha_dev));
+        c->kalpha_dev = nullptr;
+        c->kalpha_cap = 0;
+        AMB_HIP(hipMalloc(reinterpret_cast<void**>(&c->kalpha_dev), order * sizeof(float)));
+        c->kalpha_cap = order;
+    }
+    // the tables stay in the context until the copies have run (next sync)
+    AMB_HIP(hipStreamSynchronize(c->stream));
+    c->klv_host.resize(nt);
+    for (size_t j = 0; j < nt; ++j)
+        c->klv_host[j] = reinterpret_cast<float*>(c->kblock + j * fstride);
+    c->kalpha_host.assign(alpha, alpha + order);
+    AMB_HIP(hipMemcpyAsync(c->klv_dev, c->klv_host.data(), nt * sizeof(float*),
+                           hipMemcpyHostToDevice, c->stream));
+    AMB_HIP(hipMemcpyAsync(c->kalpha_dev, c->kalpha_host.data(), order * sizeof(float),
+                           hipMemcpyHostToDevice, c->stream));
+    AMB_HIP(hipMemcpyAsync(c->fields_dev, c->fields, sizeof(void*) * N_FIELDS,
+                           hipMemcpyHostToDevice, c->stream));
+    c->klevels = L;
+    return AMB_OK;
+}
+
+// The TGV-k loop: per iteration one dual launch (p, q, every r_i) and one
+// primal launch (u, v, every w_i); callback rule as in run().
+amb_status tgvk_run(amb_context* c, uint32_t iterations, uint32_t interval,
+                    amb_iteration_callback cb, void* user) {
+    float* const* F = reinterpret_cast<float* const*>(c->fields_dev);
+    for (uint32_t it = 0; it < iterations; ++it) {
+        {
+            ProfScope ps(c, P_TGVK_DUAL);
+            AMB_HIP(launch_tgvk_dual(c->stream, F, c->klv_dev, c->kalpha_dev, c->g, c->klevels,
+                                     c->sp.sigma, c->sp.planar));
+        }
+        {
+            ProfScope ps(c, P_TGVK_PRIMAL);
+            AMB_HIP(launch_tgvk_primal(c->stream, F, c->klv_dev, c->f, c->g, c->klevels,
+                                       c->sp.tau, c->sp.lambda, c->sp.theta));
+        }
+        if (cb && interval > 0 && it > 0 && it % interval == 0) {
+            const size_t n = c->g.n();
+            c->cb_u.resize(n);
+            AMB_TRY(copy_field_out(c, AMB_MEM_HOST, c->cb_u.data(), F_U));
+            for (int k = 0; k < 3; ++k) {
+                c->cb_v[k].resize(n);
+                AMB_TRY(copy_field_out(c, AMB_MEM_HOST, c->cb_v[k].data(), F_VX + k));
+            }
+            AMB_HIP(hipStreamSynchronize(c->stream));
+            if (cb(it, iterations, c->cb_u.data(), c->cb_v[0].data(), c->cb_v[1].data(),
+                   c->cb_v[2].data(), user))
+                break;
+        }
+    }
+    return AMB_OK;
+}
+
 amb_status check_precision(const Geometry& g, int precision) {
     if (precision == AMB_DCT_REFERENCE) return AMB_OK;
     const int mx = dct_mfma_max_extent();
@@ -712,6 +816,9 @@ AMB_API amb_status amb_context_destroy(amb_context* c) {
     if (c->adj_block) (void)hipFree(c->adj_block);
     if (c->act) (void)hipFree(c->act);
     if (c->alt_block) (void)hipFree(c->alt_block);
+    if (c->kblock) (void)hipFree(c->kblock);
+    if (c->klv_dev) (void)hipFree(c->klv_dev);
+    if (c->kalpha_dev) (void)hipFree(c->kalpha_dev);
     (void)hipStreamDestroy(c->stream);
     delete c;
     return AMB_OK;
@@ -950,6 +1057,53 @@ AMB_API amb_status amb_deshade_epilogue(amb_context* c, amb_memory mem, const fl
     return AMB_OK;
 }
 
+// Poisson integration of the state's v, then the epilogue (A18, A19): shared by
+// the TGV2 and TGV-k pipelines.
+static amb_status deshade_tail(amb_context* c, amb_memory mem, const float* mask, uint32_t flags,
+                               uint32_t precision, bool masked, float* u, float* shading,
+                               float* deshaded) {
+    // v as FP32 device views (BF16 storage: converted into scratch[2..4])
+    const float* v[3];
+    for (int k = 0; k < 3; ++k) {
+        if (c->storage == 0) {
+            v[k] = static_cast<const float*>(c->fields[F_VX + k]);
+        } else {
+            AMB_HIP(launch_convert(c->stream, 1, c->fields[F_VX + k], c->scratch[2 + k], c->g.n(), 0));
+            v[k] = c->scratch[2 + k];
+        }
+    }
+    float* l = nullptr;
+    AMB_TRY(integrate(c, v[0], v[1], v[2], int(precision), flags, &l));   // l in scratch[1]
+    // u as FP32 (scratch[2] for BF16), mask into scratch[3], r into scratch[4]
+    const float* uf = static_cast<const float*>(c->fields[F_U]);
+    if (c->storage == 1) {
+        AMB_HIP(launch_convert(c->stream, 1, c->fields[F_U], c->scratch[2], c->g.n(), 0));
+        uf = c->scratch[2];
+    }
+    const float* m = nullptr;
+    if (mask) {
+        if (mem == AMB_MEM_DEVICE) {
+            m = mask;
+        } else {
+            AMB_HIP(hipMemcpyAsync(c->scratch[3], mask, c->g.n() * 4, hipMemcpyHostToDevice, c->stream));
+            m = c->scratch[3];
+        }
+    }
+    {
+        ProfScope ps(c, P_EPI);
+        AMB_HIP(launch_epilogue(c->stream, uf, l, m, c->scratch[4], c->g.n(),
+                                (masked || (flags & AMB_FLAG_MASK_SHADING)) ? 1 : 0,
+                                (flags & AMB_FLAG_SET_NEGATIVE_ZERO) ? 1 : 0,
+                                (flags & AMB_FLAG_ADD_BACKGROUND_BACK) ? 1 : 0,
+                                masked ? 1 : 0));
+    }
+    AMB_TRY(copy_out(c, mem, u, uf));
+    AMB_TRY(copy_out(c, mem, shading, l));
+    AMB_TRY(copy_out(c, mem, deshaded, c->scratch[4]));
+    AMB_HIP(hipStreamSynchronize(c->stream));
+    return AMB_OK;
+}
+
 AMB_API amb_status amb_tgv2_deshade_pipeline(amb_context* c, amb_memory mem, const float* f,
                                              uint32_t w, uint32_t h, uint32_t d,
                                              const amb_tgv_params* p, const float* mask, float* u,
@@ -975,46 +1129,48 @@ AMB_API amb_status amb_tgv2_deshade_pipeline(amb_context* c, amb_memory mem, con
     } else {
         AMB_TRY(run(c, p->iterations, p->callback_interval, cb, user, true));
     }
-    // v as FP32 device views (BF16 storage: converted into scratch[2..4])
-    const float* v[3];
-    for (int k = 0; k < 3; ++k) {
-        if (c->storage == 0) {
-            v[k] = static_cast<const float*>(c->fields[F_VX + k]);
-        } else {
-            AMB_HIP(launch_convert(c->stream, 1, c->fields[F_VX + k], c->scratch[2 + k], c->g.n(), 0));
-            v[k] = c->scratch[2 + k];
-        }
-    }
-    float* l = nullptr;
-    AMB_TRY(integrate(c, v[0], v[1], v[2], int(p->dct_precision), p->flags, &l));   // l in scratch[1]
-    // u as FP32 (scratch[2] for BF16), mask into scratch[3], r into scratch[4]
-    const float* uf = static_cast<const float*>(c->fields[F_U]);
-    if (c->storage == 1) {
-        AMB_HIP(launch_convert(c->stream, 1, c->fields[F_U], c->scratch[2], c->g.n(), 0));
-        uf = c->scratch[2];
-    }
-    const float* m = nullptr;
-    if (mask) {
-        if (mem == AMB_MEM_DEVICE) {
-            m = mask;
-        } else {
-            AMB_HIP(hipMemcpyAsync(c->scratch[3], mask, c->g.n() * 4, hipMemcpyHostToDevice, c->stream));
-            m = c->scratch[3];
-        }
-    }
-    {
-        ProfScope ps(c, P_EPI);
-        AMB_HIP(launch_epilogue(c->stream, uf, l, m, c->scratch[4], c->g.n(),
-                                (masked || (p->flags & AMB_FLAG_MASK_SHADING)) ? 1 : 0,
-                                (p->flags & AMB_FLAG_SET_NEGATIVE_ZERO) ? 1 : 0,
-                                (p->flags & AMB_FLAG_ADD_BACKGROUND_BACK) ? 1 : 0,
-                                masked ? 1 : 0));
-    }
-    AMB_TRY(copy_out(c, mem, u, uf));
-    AMB_TRY(copy_out(c, mem, shading, l));
-    AMB_TRY(copy_out(c, mem, deshaded, c->scratch[4]));
+    return deshade_tail(c, mem, mask, p->flags, p->dct_precision, masked, u, shading, deshaded);
+}
+
+AMB_API amb_status amb_tgvk_deshade(amb_context* c, amb_memory mem, const float* f, uint32_t w,
+                                    uint32_t h, uint32_t d, const amb_tgv_params* p,
+                                    uint32_t order, const float* alpha, float* u, float* vx,
+                                    float* vy, float* vz, amb_iteration_callback cb, void* user) {
+    if (!c) return fail(AMB_ERR_INVALID_ARGUMENT, "ctx is NULL");
+    AMB_TRY(tgvk_begin(c, mem, f, w, h, d, p, order, alpha));
+    AMB_TRY(tgvk_run(c, p->iterations, p->callback_interval, cb, user));
+    AMB_TRY(copy_field_out(c, mem, u, F_U));
+    AMB_TRY(copy_field_out(c, mem, vx, F_VX));
+    AMB_TRY(copy_field_out(c, mem, vy, F_VY));
+    AMB_TRY(copy_field_out(c, mem, vz, F_VZ));
     AMB_HIP(hipStreamSynchronize(c->stream));
     return AMB_OK;
+}
+
+AMB_API amb_status amb_tgvk_deshade_pipeline(amb_context* c, amb_memory mem, const float* f,
+                                             uint32_t w, uint32_t h, uint32_t d,
+                                             const amb_tgv_params* p, uint32_t order,
+                                             const float* alpha, const float* mask, float* u,
+                                             float* shading, float* deshaded,
+                                             amb_iteration_callback cb, void* user) {
+    if (!c) return fail(AMB_ERR_INVALID_ARGUMENT, "ctx is NULL");
+    if (!p) return fail(AMB_ERR_INVALID_ARGUMENT, "params is NULL");
+    if (!alpha) return fail(AMB_ERR_INVALID_ARGUMENT, "alpha is NULL");
+    if (order < 2) return fail(AMB_ERR_INVALID_ARGUMENT, "order must be at least 2");
+    // TGVKDeshadeProcessor.cpp:93-100: depth 1 takes the 2-D loops; the mask
+    // multiplies l and r (deshade_poisson_cosine_transform, :155-158)
+    amb_tgv_params pp = *p;
+    pp.flags = (pp.flags & ~AMB_FLAG_MASKED_ITERATION) | AMB_FLAG_MASK_SHADING;
+    if (d == 1) pp.flags |= AMB_FLAG_PLANAR;
+    if (order == 2) {   // :132-143, the TGV2 launcher with alpha0 = alpha[1], alpha1 = alpha[0]
+        pp.alpha0 = alpha[1];
+        pp.alpha1 = alpha[0];
+        return amb_tgv2_deshade_pipeline(c, mem, f, w, h, d, &pp, mask, u, shading, deshaded, cb,
+                                         user);
+    }
+    AMB_TRY(tgvk_begin(c, mem, f, w, h, d, &pp, order, alpha));
+    AMB_TRY(tgvk_run(c, pp.iterations, pp.callback_interval, cb, user));
+    return deshade_tail(c, mem, mask, pp.flags, pp.dct_precision, false, u, shading, deshaded);
 }
 
 AMB_API amb_status amb_tgv2_energy(amb_context* c, double out[10]) {

@@ -29,7 +29,7 @@ N_FIELDS = 17
 FIELD_NAMES = ["u", "ubar", "vx", "vy", "vz", "vbx", "vby", "vbz", "px", "py", "pz",
                "qxx", "qyy", "qzz", "qxy", "qxz", "qyz"]
 PROFILE_CLASSES = ["dual_sweep", "primal_sweep", "dct", "poisson", "divergence", "epilogue",
-                   "init", "energy", "adjoint", "fused_sweep"]
+                   "init", "energy", "adjoint", "fused_sweep", "tgvk_dual", "tgvk_primal"]
 
 # Every function include/ambient_amd.h declares (checked by the CPU test suite).
 EXPORTS = [
@@ -41,7 +41,8 @@ EXPORTS = [
     "amb_tgv2_deshade_pipeline", "amb_tgv2_energy", "amb_tgv2_begin", "amb_tgv2_set_state",
     "amb_tgv2_get_state", "amb_tgv2_iterate", "amb_tgv2_sweep", "amb_profile_enable",
     "amb_profile_read", "amb_profile_reset", "amb_tgv2_denoise_adjoint",
-    "amb_tgv2_deshade_masked", "amb_deshade_epilogue",
+    "amb_tgv2_deshade_masked", "amb_deshade_epilogue", "amb_tgvk_deshade",
+    "amb_tgvk_deshade_pipeline",
 ]
 
 
@@ -98,6 +99,10 @@ def lib():
         "amb_integrate_gradients_poisson": [vp, mem, fp, fp, fp, u32, u32, u32, u32, u32, fp],
         "amb_tgv2_deshade_pipeline": [vp, mem, fp, u32, u32, u32, POINTER(Params), fp, fp, fp,
                                       fp, CALLBACK, vp],
+        "amb_tgvk_deshade": [vp, mem, fp, u32, u32, u32, POINTER(Params), u32, fp, fp, fp, fp,
+                             fp, CALLBACK, vp],
+        "amb_tgvk_deshade_pipeline": [vp, mem, fp, u32, u32, u32, POINTER(Params), u32, fp, fp,
+                                      fp, fp, fp, CALLBACK, vp],
         "amb_tgv2_energy": [vp, POINTER(c_double)],
         "amb_tgv2_begin": [vp, mem, fp, u32, u32, u32, POINTER(Params)],
         "amb_tgv2_set_state": [vp, mem, POINTER(c_void_p)],
@@ -240,6 +245,34 @@ class Context:
         del keep
         return tuple(outs)
 
+    def tgvk_deshade(self, f, params: Params, alpha, callback=None):
+        """amb_tgvk_deshade (order = len(alpha) >= 3) on host arrays -> (u, vx, vy, vz)."""
+        f = _f32(f)
+        al = np.ascontiguousarray(alpha, dtype=np.float32)
+        self.shape = f.shape
+        outs = [np.empty_like(f) for _ in range(4)]
+        cb, keep = self._callback(callback, f.shape, True)
+        _check(lib().amb_tgvk_deshade(self._h, MEM_HOST, _ptr(f), *_dims(f.shape),
+                                      ctypes.byref(params), al.size, _ptr(al),
+                                      *[_ptr(o) for o in outs], cb, None))
+        del keep
+        return tuple(outs)
+
+    def tgvk_deshade_pipeline(self, f, params: Params, alpha, mask=None, callback=None):
+        """amb_tgvk_deshade_pipeline (order = len(alpha) >= 2) on host arrays
+        -> (u, shading, deshaded)."""
+        f = _f32(f)
+        al = np.ascontiguousarray(alpha, dtype=np.float32)
+        self.shape = f.shape
+        m = _f32(mask) if mask is not None else None
+        u, l, r = (np.empty_like(f) for _ in range(3))
+        cb, keep = self._callback(callback, f.shape, True)
+        _check(lib().amb_tgvk_deshade_pipeline(self._h, MEM_HOST, _ptr(f), *_dims(f.shape),
+                                               ctypes.byref(params), al.size, _ptr(al), _ptr(m),
+                                               _ptr(u), _ptr(l), _ptr(r), cb, None))
+        del keep
+        return u, l, r
+
     def tgv2_denoise(self, f, params: Params, callback=None):
         f = _f32(f)
         u = np.empty_like(f)
@@ -276,6 +309,14 @@ class Context:
         _check(lib().amb_tgv2_deshade_pipeline(self._h, MEM_DEVICE, c_void_p(f_dev), *_dims(shape),
                                                ctypes.byref(params), _ptr(mask_dev), _ptr(u_dev),
                                                _ptr(l_dev), _ptr(r_dev), CALLBACK(), None))
+
+    def tgvk_deshade_pipeline_device(self, f_dev: int, shape, params: Params, alpha, u_dev=None,
+                                     l_dev=None, r_dev=None, mask_dev=None):
+        al = np.ascontiguousarray(alpha, dtype=np.float32)
+        _check(lib().amb_tgvk_deshade_pipeline(self._h, MEM_DEVICE, c_void_p(f_dev), *_dims(shape),
+                                               ctypes.byref(params), al.size, _ptr(al),
+                                               _ptr(mask_dev), _ptr(u_dev), _ptr(l_dev),
+                                               _ptr(r_dev), CALLBACK(), None))
 
     def begin(self, f, params: Params):
         f = _f32(f)

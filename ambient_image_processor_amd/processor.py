@@ -128,6 +128,36 @@ def tgv2_l1_deshade_masked_2d_launch(f_host, width, height, lambda_, iteration_c
     return u, vx, vy
 
 
+def tgvk_l1_deshade_launch(f_host, width, height, depth, lambda_, iteration_count,
+                           paint_iteration_interval, iteration_finished_callback, order, alpha,
+                           **kw):
+    """Reference tgvk_deshade/tgvk_l1_deshade.cu:35-45 (order >= 3) -> (u, v_x,
+    v_y, v_z).  alpha: order weights (alpha[0] for p, alpha[1] for q, alpha[i+2]
+    for the dual of level i).  The callback gets (iteration_index,
+    iteration_count, u, v_x, v_y, v_z)."""
+    if len(alpha) != order:
+        raise ValueError("alpha needs one weight per order")
+    f = _vol(f_host, width, height, depth)
+    p = _params(lambda_, iteration_count, paint_iteration_interval, 1.0, 1.0, native.DATA_L1,
+                **kw)
+    return _ctx().tgvk_deshade(f, p, alpha, iteration_finished_callback)
+
+
+def tgvk_l1_deshade_launch_2d(f_host, width, height, lambda_, iteration_count,
+                              paint_iteration_interval, iteration_finished_callback, order, alpha,
+                              **kw):
+    """Reference tgvk_deshade/tgvk_l1_deshade_2d.cu (declared
+    TGVKDeshadeProcessor.cpp:49-58) -> (u, v_x, v_y); the callback gets
+    (iteration_index, iteration_count, u, v_x, v_y)."""
+    if len(alpha) != order:
+        raise ValueError("alpha needs one weight per order")
+    f = _vol(f_host, width, height, 1)
+    p = _planar_params(_params(lambda_, iteration_count, paint_iteration_interval, 1.0, 1.0,
+                               native.DATA_L1, **kw))
+    u, vx, vy, _ = _ctx().tgvk_deshade(f, p, alpha, _drop_vz(iteration_finished_callback))
+    return u, vx, vy
+
+
 def _denoise(data_term, f_host, width, height, depth, lambda_, iteration_count,
              paint_iteration_interval, iteration_finished_callback, alpha0, alpha1, **kw):
     f = _vol(f_host, width, height, depth)
@@ -275,6 +305,54 @@ class TGVProcessor:
         d, h, w = f.shape
         return tgv2_l1_launch(f, w, h, d, lambda_, iteration_count, paint_iteration_interval,
                               iteration_finished_callback, alpha0, alpha1, **kw)
+
+
+class TGVKDeshadeProcessor:
+    """Reference tgvk_deshade/TGVKDeshadeProcessor.{h,cpp}."""
+
+    @staticmethod
+    def processTGVKL1Cuda(input_image, lambda_, order, alpha, iteration_count, mask=None,
+                          set_negative_values_to_zero=True, add_background_back=False,
+                          paint_iteration_interval=0, iteration_finished_callback=None,
+                          calculate_div_v=False, **kw):
+        """TGVKDeshadeProcessor.cpp:76-174 (and :177-270 at depth 1) ->
+        (denoised_image, shading_image, deshaded_image, div_v_image or None).
+        Order 2 runs the TGV2 loop with alpha0 = alpha[1], alpha1 = alpha[0];
+        order >= 3 the TGV-k loop; depth 1 the 2-D loops.  The mask multiplies
+        l and r, r is clamped, u*(1 - mask) optionally added back.  The callback
+        receives (iteration_index, iteration_count, u, l, r) computed with the
+        same composition."""
+        if len(alpha) != order:
+            raise ValueError("alpha needs one weight per order")
+        f = np.ascontiguousarray(input_image, dtype=np.float32)
+        m = None if mask is None else np.ascontiguousarray(mask, dtype=np.float32).reshape(f.shape)
+        flags = native.FLAG_MASK_SHADING
+        if set_negative_values_to_zero:
+            flags |= native.FLAG_SET_NEGATIVE_ZERO
+        if add_background_back:
+            flags |= native.FLAG_ADD_BACKGROUND_BACK
+        p = _params(lambda_, iteration_count, paint_iteration_interval, 1.0, 1.0, native.DATA_L1,
+                    flags=flags, **kw)
+        cb = None
+        if iteration_finished_callback is not None:
+            def cb(it, count, u, vx, vy, vz):
+                aux = _aux_ctx()
+                l = aux.integrate_gradients_poisson(vx, vy, vz, p.dct_precision)
+                r, l = aux.deshade_epilogue(u, l, m, flags)
+                return iteration_finished_callback(it, count, u, l, r)
+        u, l, r = _ctx().tgvk_deshade_pipeline(f, p, alpha, m, cb)
+        div_v = None
+        if calculate_div_v:
+            st = _ctx().get_state()
+            div_v = _ctx().divergence(st[2], st[3], st[4])
+        return u, l, r, div_v
+
+    @staticmethod
+    def updateAlpha(order, updateElement):
+        """TGVKDeshadeProcessor.cpp:272-281: the widget's default weights
+        1, 4, 7, ... for alpha[0 .. order-1]."""
+        for i in range(order):
+            updateElement(1 + 3 * i)
 
 
 def _aux_ctx() -> native.Context:

@@ -14,6 +14,7 @@ the same workload on the host cores.
     python bench.py --workload poisson_512      # DCT Poisson solve 512x512x256
     python bench.py --workload bf16_384         # BF16 field storage, L2, 384^3, 800 it
     python bench.py --workload adjoint_4x128    # unrolled TGV2-L2 + adjoint, 4 x 128^3
+    python bench.py --workload tgvk3_256        # TGV-k (k = 3) deshade 256^3
     python bench.py --copy-calibration          # plain 1 GiB device copies (PMC scale)
 
 Unless --no-split, the default workload also runs in two split forms -- FP32
@@ -54,6 +55,16 @@ SWEEPS = ("dual_sweep", "primal_sweep", "fused_sweep")
 MEASURED_TRAFFIC = {("fused_sweep", 4, 256): 1882845639,
                     ("dual_sweep", 4, 256): 1557605469,
                     ("primal_sweep", 4, 256): 1552974764}
+
+
+def tgvk_bytes(kind, levels, n):
+    """Algorithmic bytes of one TGV-k launch (FP32 fields), L = k - 2 levels:
+      dual    read u_bar, v_bar x3, p x3, q x6, per level w_bar x6, r x6;
+              write p x3, q x6, per level r x6                    (22 + 18 L) x 4
+      primal  read u, f, p x3, q x6, v x3, per level r x6, w x6;
+              write u, u_bar, v x3, v_bar x3, per level w x6, w_bar x6  (22 + 24 L) x 4"""
+    per = 22 + 18 * levels if kind == "tgvk_dual" else 22 + 24 * levels
+    return 4 * per * n
 
 
 def sweep_bytes(kind, storage_bytes, n):
@@ -407,6 +418,70 @@ def adjoint_4x128(args):
     ctx.close()
 
 
+def tgvk3_256(args):
+    """TGV-k deshade (k = 3, the widget's default weights 1, 4, 7;
+    TGVKDeshadeWidget.ui / TGVKDeshadeProcessor.cpp:272-281) of the 256^3
+    volume: the TGV-k loop, the Poisson solve and the epilogue."""
+    from ambient_image_processor_amd import native, phantoms
+    n_side = args.size
+    shape = (n_side, n_side, n_side)
+    f, _, _ = phantoms.deshade_phantom(shape, seed=2, kind="spheres", noise=0.03, bias_order=3,
+                                       bias_sigma=0.05)
+    n = f.size
+    alpha = (1.0, 4.0, 7.0)
+    levels = len(alpha) - 2
+    tau = float(1.0 / np.sqrt(20.0))
+    p = native.default_params(lam=0.5, tau=tau, sigma=tau, theta=1.0, iterations=args.iterations,
+                              data_term=native.DATA_L1, flags=native.FLAG_SET_NEGATIVE_ZERO)
+    ctx = native.Context(0)
+    f_dev = ctx.alloc(n * 4)
+    u_dev, l_dev, r_dev = ctx.alloc(n * 4), ctx.alloc(n * 4), ctx.alloc(n * 4)
+    ctx.upload(f_dev, f)
+    elapsed, prof = timed(ctx, lambda: ctx.tgvk_deshade_pipeline_device(
+        f_dev, shape, p, alpha, u_dev, l_dev, r_dev), args.steps, args.warmup)
+    value = float(n) * args.iterations * args.steps / elapsed
+    kernels = {}
+    for name in ("tgvk_dual", "tgvk_primal"):
+        ms, cnt = prof[name]
+        b = tgvk_bytes(name, levels, n)
+        kernels[name] = {"avg_us": 1000.0 * ms / cnt, "launches": cnt, "bytes_per_launch": b,
+                         "gbs": b / (ms / cnt * 1e-3) / 1e9}
+    dom = max(kernels, key=lambda k: kernels[k]["avg_us"])
+    achieved = kernels[dom]["gbs"]
+    cpu = None
+    if not args.no_cpu_baseline:
+        from oracle import oracle
+        oracle.build()
+        st = oracle.TGVKState(f, len(alpha), np.float32)
+        st.iterate(1, lam=0.5, alpha=alpha, tau=tau, sigma=tau)
+        t1 = time.perf_counter()
+        st.iterate(args.cpu_iters // 2, lam=0.5, alpha=alpha, tau=tau, sigma=tau)
+        cpu_s = time.perf_counter() - t1
+        cpu = {"value": float(n) * (args.cpu_iters // 2) / cpu_s, "unit": "voxel-iters/s",
+               "cores": oracle.num_threads(), "kind": "port",
+               "sample": f"{args.cpu_iters // 2} TGV-k (k=3) iterations of the same {n_side}^3 "
+                         f"volume (FP32 reference op order, OpenMP), {cpu_s:.1f} s"}
+    print(json.dumps({
+        "metric": "voxel-iters/s TGV-k (k=3) deshade 256^3 + %HBM roofline",
+        "value": value, "unit": "voxel-iters/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": 1000.0 * elapsed / args.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (32 seeded spheres + DCT bias + N(0,0.03^2) noise)",
+        "config": {"workload": f"tgvk3_l1_deshade_{n_side}^3", "shape": list(shape),
+                   "iterations": args.iterations, "lambda": 0.5, "alpha": list(alpha),
+                   "parallelism": "single-gpu"},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "bytes_per_launch": kernels[dom]["bytes_per_launch"]},
+        "kernels": rounded(kernels),
+        "other_kernels_ms": {k: round(prof[k][0], 3) for k in ("dct", "poisson", "divergence",
+                                                                "epilogue", "init")},
+        "cpu_baseline": cpu}))
+    for ptr in (f_dev, u_dev, l_dev, r_dev):
+        ctx.free(ptr)
+    ctx.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -421,7 +496,7 @@ def main():
     ap.add_argument("--no-split", action="store_true")
     ap.add_argument("--no-host", action="store_true")
     ap.add_argument("--workload", choices=["deshade_256", "poisson_512", "bf16_384",
-                                           "adjoint_4x128"], default="deshade_256")
+                                           "adjoint_4x128", "tgvk3_256"], default="deshade_256")
     ap.add_argument("--copy-calibration", action="store_true")
     args = ap.parse_args()
 
@@ -434,7 +509,7 @@ def main():
         copy_calibration()
         return
     {"deshade_256": deshade_256, "poisson_512": poisson_512, "bf16_384": bf16_384,
-     "adjoint_4x128": adjoint_4x128}[args.workload](args)
+     "adjoint_4x128": adjoint_4x128, "tgvk3_256": tgvk3_256}[args.workload](args)
 
 
 if __name__ == "__main__":

@@ -265,13 +265,43 @@ AMB_API amb_status amb_tgv2_iterate(amb_context* ctx, uint32_t iterations);
 /* One half-iteration only: which = 0 dual sweep (p, q), 1 primal sweep (u, v). */
 AMB_API amb_status amb_tgv2_sweep(amb_context* ctx, int which);
 
+/* ---- TGV-k deshade (k >= 3) ----------------------------------------------
+ * Replaces tgvk_l1_deshade_launch<float>(f_host, width, height, depth, lambda,
+ * iteration_count, paint_iteration_interval, callback, order, alpha, &v_x, &v_y,
+ * &v_z) (reference tgvk_deshade/tgvk_l1_deshade.cu:35-45, declared
+ * TGVKDeshadeProcessor.cpp:27-36).  order = k in 3 .. 1000; alpha: k positive
+ * host weights, alpha[0] bounding p, alpha[1] q and alpha[i + 2] the dual of
+ * level i.  From params: lambda, tau, sigma, theta, iterations,
+ * callback_interval and AMB_FLAG_PLANAR (depth 1: tgvk_l1_deshade_launch_2d,
+ * tgvk_deshade/tgvk_l1_deshade_2d.cu); data_term must be L1 and storage F32;
+ * params->alpha0/alpha1 are not read.  Writes u and v (x, y, z). */
+AMB_API amb_status amb_tgvk_deshade(amb_context* ctx, amb_memory mem, const float* f,
+                                    uint32_t width, uint32_t height, uint32_t depth,
+                                    const amb_tgv_params* params, uint32_t order,
+                                    const float* alpha, float* u, float* v_x, float* v_y,
+                                    float* v_z, amb_iteration_callback callback, void* user);
+
+/* TGVKDeshadeProcessor::processTGVKL1Cuda (TGVKDeshadeProcessor.cpp:76-174):
+ * order 2 runs the TGV2 loop with alpha0 = alpha[1], alpha1 = alpha[0], order >= 3
+ * the TGV-k loop, depth 1 the 2-D loops; then l from the Poisson integration of
+ * v, r = u - l, the mask (may be NULL) multiplying l and r, the clamp
+ * (AMB_FLAG_SET_NEGATIVE_ZERO) and the add-back of u*(1 - mask)
+ * (AMB_FLAG_ADD_BACKGROUND_BACK).  params->dct_precision selects the DCT. */
+AMB_API amb_status amb_tgvk_deshade_pipeline(amb_context* ctx, amb_memory mem, const float* f,
+                                             uint32_t width, uint32_t height, uint32_t depth,
+                                             const amb_tgv_params* params, uint32_t order,
+                                             const float* alpha, const float* mask, float* u,
+                                             float* shading, float* deshaded,
+                                             amb_iteration_callback callback, void* user);
+
 /* ---- timing ----------------------------------------------------------------
  * With profiling on, every kernel launch of the context is bracketed by HIP
  * events on the context's stream; amb_profile_read returns the summed device
  * time (ms) and launch count per kernel class:
  *   0 dual sweep, 1 primal sweep, 2 DCT passes, 3 Poisson divide,
  *   4 divergence, 5 epilogue, 6 init, 7 energy, 8 adjoint (whole call),
- *   9 fused sweep (primal half of one iteration + dual half of the next). */
+ *   9 fused sweep (primal half of one iteration + dual half of the next),
+ *   10 TGV-k dual sweep, 11 TGV-k primal sweep. */
 AMB_API amb_status amb_profile_enable(amb_context* ctx, int enable);
 AMB_API amb_status amb_profile_read(amb_context* ctx, int kernel_class, double* total_ms,
                                     uint64_t* launches);

@@ -256,6 +256,67 @@ Pixel* tgv2_l1_deshade_masked_2d_launch(
     return u;
 }
 
+// ---- reference tgvk_deshade/tgvk_l1_deshade.cu:35-45 (declared
+// TGVKDeshadeProcessor.cpp:27-36) -------------------------------------------
+// order >= 3; alpha holds `order` weights.
+template <typename Pixel>
+Pixel* tgvk_l1_deshade_launch(Pixel* f_host, ambient_amd_compat::uint width,
+                              ambient_amd_compat::uint height, ambient_amd_compat::uint depth,
+                              Pixel lambda, ambient_amd_compat::uint iteration_count,
+                              ambient_amd_compat::uint paint_iteration_interval,
+                              std::function<bool(ambient_amd_compat::uint, ambient_amd_compat::uint,
+                                                 Pixel*, Pixel*, Pixel*, Pixel*)>
+                                  iteration_finished_callback,
+                              const ambient_amd_compat::uint order, const Pixel* alpha,
+                              Pixel** v_x_host, Pixel** v_y_host, Pixel** v_z_host) {
+    using namespace ambient_amd_compat;
+    static_assert(std::is_same<Pixel, float>::value, "only the float instantiation exists");
+    typedef std::function<bool(uint, uint, Pixel*, Pixel*, Pixel*, Pixel*)> Fn;
+    const size_t n = size_t(width) * height * depth;
+    amb_tgv_params p = reference_params(lambda, iteration_count, paint_iteration_interval,
+                                        alpha[1], alpha[0], AMB_DATA_L1);
+    Pixel* u = new Pixel[n];
+    *v_x_host = new Pixel[n];
+    *v_y_host = new Pixel[n];
+    *v_z_host = new Pixel[n];
+    amb_iteration_callback cb = iteration_finished_callback ? &Trampoline<Fn>::deshade : nullptr;
+    check(amb_tgvk_deshade(context(), AMB_MEM_HOST, f_host, width, height, depth, &p, order,
+                           alpha, u, *v_x_host, *v_y_host, *v_z_host, cb,
+                           &iteration_finished_callback),
+          "tgvk_l1_deshade_launch");
+    return u;
+}
+
+// ---- reference tgvk_deshade/tgvk_l1_deshade_2d.cu (declared
+// TGVKDeshadeProcessor.cpp:49-58) -------------------------------------------
+template <typename Pixel>
+Pixel* tgvk_l1_deshade_launch_2d(Pixel* f_host, ambient_amd_compat::uint width,
+                                 ambient_amd_compat::uint height, Pixel lambda,
+                                 ambient_amd_compat::uint iteration_count,
+                                 ambient_amd_compat::uint paint_iteration_interval,
+                                 std::function<bool(ambient_amd_compat::uint,
+                                                    ambient_amd_compat::uint, Pixel*, Pixel*,
+                                                    Pixel*)>
+                                     iteration_finished_callback,
+                                 const ambient_amd_compat::uint order, const Pixel* alpha,
+                                 Pixel** v_x_host, Pixel** v_y_host) {
+    using namespace ambient_amd_compat;
+    static_assert(std::is_same<Pixel, float>::value, "only the float instantiation exists");
+    typedef std::function<bool(uint, uint, Pixel*, Pixel*, Pixel*)> Fn;
+    const size_t n = size_t(width) * height;
+    amb_tgv_params p = reference_params(lambda, iteration_count, paint_iteration_interval,
+                                        alpha[1], alpha[0], AMB_DATA_L1);
+    p.flags |= AMB_FLAG_PLANAR;
+    Pixel* u = new Pixel[n];
+    *v_x_host = new Pixel[n];
+    *v_y_host = new Pixel[n];
+    amb_iteration_callback cb = iteration_finished_callback ? &Trampoline<Fn>::deshade_2d : nullptr;
+    check(amb_tgvk_deshade(context(), AMB_MEM_HOST, f_host, width, height, 1, &p, order, alpha,
+                           u, *v_x_host, *v_y_host, nullptr, cb, &iteration_finished_callback),
+          "tgvk_l1_deshade_launch_2d");
+    return u;
+}
+
 // ---- reference tgv_cuda/tgv2_l1.cu:64-71 and tgv_cuda/tgv2_l2.cu:60-67 -------
 template <typename Pixel>
 Pixel* tgv2_denoise_launch_impl(uint32_t data_term, Pixel* f_host, ambient_amd_compat::uint width,

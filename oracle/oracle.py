@@ -177,6 +177,97 @@ def tgv2(f, iterations, lam, alpha0, alpha1, tau, sigma, theta=1.0, data_term="l
     return st
 
 
+class TGVKState:
+    """TGV-k (k >= 3) iteration state: the 17 TGV2 fields, then per level
+    i < k - 2 the tensors w_i, w_bar_i, r_i (components x, y, z, xy, xz, yz),
+    as one contiguous (17 + 18 (k-2), D, H, W) array."""
+
+    def __init__(self, f, order, dtype=np.float64):
+        if order < 3:
+            raise ValueError("TGV-k needs order >= 3")
+        self.order = int(order)
+        self.dtype = np.dtype(dtype)
+        self.f = _c(f, dtype)
+        nf = N_FIELDS + 18 * (self.order - 2)
+        self.fields = np.zeros((nf,) + self.f.shape, dtype=dtype)
+        n = self.fields[0].size * self.dtype.itemsize
+        base = self.fields.ctypes.data
+        self._pp = (ctypes.c_void_p * nf)(*[base + i * n for i in range(nf)])
+        getattr(lib(), "orc_tgv2_init_" + _suf(dtype))(self._pp, _p(self.f), *_dims(self.f))
+
+    @property
+    def u(self):
+        return self.fields[0]
+
+    @property
+    def v(self):
+        return self.fields[2:5]
+
+    @property
+    def p(self):
+        return self.fields[8:11]
+
+    @property
+    def q(self):
+        return self.fields[11:17]
+
+    def level(self, i):
+        """(w, w_bar, r) of level i, each (6, D, H, W)."""
+        b = N_FIELDS + 18 * i
+        return self.fields[b:b + 6], self.fields[b + 6:b + 12], self.fields[b + 12:b + 18]
+
+    def iterate(self, iterations, lam, alpha, tau, sigma, theta=1.0, planar=False):
+        """alpha: order weights; alpha[0] bounds p, alpha[1] q, alpha[i + 2] r_i."""
+        alpha = np.ascontiguousarray(alpha, dtype=np.float64)
+        if alpha.size != self.order:
+            raise ValueError("alpha needs one weight per order")
+        if planar and self.f.shape[0] != 1:
+            raise ValueError("the planar (2-D) loop needs depth 1")
+        prm = (ctypes.c_double * 4)(lam, tau, sigma, theta)
+        rc = getattr(lib(), "orc_tgvk_iterate_" + _suf(self.dtype))(
+            self._pp, self.order, _p(self.f), *_dims(self.f), prm, _p(alpha), int(iterations),
+            int(bool(planar)))
+        assert rc == iterations
+        return self
+
+
+def tensor_op(t, forward, halve):
+    """The TGV-k level operators on a (6, D, H, W) symmetric tensor (x, y, z, xy,
+    xz, yz): halve=True is G (w_bar -> r2), halve=False is H (r -> r2); forward
+    selects D+ instead of the reference's backward difference."""
+    t = np.ascontiguousarray(t, dtype=np.float64)
+    out = np.empty_like(t)
+    pin = (ctypes.c_void_p * 6)(*[t[c].ctypes.data for c in range(6)])
+    pout = (ctypes.c_void_p * 6)(*[out[c].ctypes.data for c in range(6)])
+    lib().orc_tensor_op_f64(pin, pout, *_dims(t[0]), int(forward), int(halve))
+    return out
+
+
+def tgvk(f, iterations, lam, alpha, tau, sigma, theta=1.0, dtype=np.float64, planar=False):
+    """TGV-k (k = len(alpha) >= 3) deshade loop from the reference initialisation."""
+    st = TGVKState(f, len(alpha), dtype)
+    st.iterate(iterations, lam, alpha, tau, sigma, theta, planar)
+    return st
+
+
+def deshade_k(f, iterations, lam, alpha, tau, sigma, theta=1.0, mask=None,
+              set_negative_zero=True, add_background_back=False, dtype=np.float32):
+    """TGVKDeshadeProcessor::processTGVKL1Cuda (TGVKDeshadeProcessor.cpp:76-174):
+    order 2 runs the TGV2 loop with alpha0 = alpha[1], alpha1 = alpha[0]; order
+    >= 3 the TGV-k loop; depth 1 the 2-D loops.  Then the Poisson integration
+    of v, l and r times the mask, clamp, optional add-back of u*(1 - mask).
+    Returns (u, l, r, state)."""
+    planar = np.shape(f)[0] == 1
+    if len(alpha) == 2:
+        st = tgv2(f, iterations, lam, alpha[1], alpha[0], tau, sigma, theta, "l1", dtype,
+                  planar=planar)
+    else:
+        st = tgvk(f, iterations, lam, alpha, tau, sigma, theta, dtype, planar=planar)
+    l = integrate_poisson(st.v[0], st.v[1], st.v[2], False, dtype)
+    r, l2 = deshade_epilogue(st.u, l, mask, True, set_negative_zero, add_background_back)
+    return st.u.copy(), l2, r, st
+
+
 def divergence(vx, vy, vz, dtype=np.float64):
     vx, vy, vz = (_c(a, dtype) for a in (vx, vy, vz))
     out = np.empty_like(vx)

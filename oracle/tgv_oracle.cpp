@@ -131,9 +131,12 @@ void tgv2_init(State<T>& st, const T* f, const Dims& s) {
 // TGVDeshadeMaskedProcessor.cpp:337-372); the stencils are the unmasked ones
 // (the edge lists are the volume faces), so out-of-mask voxels keep their
 // initial values and are read as neighbours.
+// wbar0 (may be null): TGV-k's q-update q + sigma*(eps_B(v_bar) - w_bar_0)
+// (tgv3/tgv3_common.cu:28-86, launched tgvk_deshade/tgvk_l1_deshade.cu:200-214).
 template <typename T>
 void tgv2_iteration(State<T>& st, const T* f, const Dims& s, const Params& prm,
-                    std::vector<T>* tmp /* 9 arrays of n */, const unsigned char* active) {
+                    std::vector<T>* tmp /* 9 arrays of n */, const unsigned char* active,
+                    T* const* wbar0 = nullptr) {
     const i64 n = s.n();
     const T tau = T(prm.tau), sigma = T(prm.sigma), theta = T(prm.theta);
     const T lambda = T(prm.lambda), alpha0 = T(prm.alpha0), alpha1 = T(prm.alpha1);
@@ -223,12 +226,14 @@ void tgv2_iteration(State<T>& st, const T* f, const Dims& s, const Params& prm,
 #pragma omp parallel for schedule(static)
     for (i64 i = 0; i < n; ++i) {
         if (active && !active[i]) continue;
-        T qxx = fma_t(sigma, e[0][i], st.q[0][i]);
-        T qyy = fma_t(sigma, e[1][i], st.q[1][i]);
-        T qxy = fma_t(sigma, e[3][i], st.q[3][i]);
-        T qzz = fma_t(sigma, e[2][i], st.q[2][i]);
-        T qxz = fma_t(sigma, e[4][i], st.q[4][i]);
-        T qyz = fma_t(sigma, e[5][i], st.q[5][i]);
+        T ec[6];
+        for (int c = 0; c < 6; ++c) ec[c] = wbar0 ? e[c][i] - wbar0[c][i] : e[c][i];
+        T qxx = fma_t(sigma, ec[0], st.q[0][i]);
+        T qyy = fma_t(sigma, ec[1], st.q[1][i]);
+        T qxy = fma_t(sigma, ec[3], st.q[3][i]);
+        T qzz = fma_t(sigma, ec[2], st.q[2][i]);
+        T qxz = fma_t(sigma, ec[4], st.q[4][i]);
+        T qyz = fma_t(sigma, ec[5], st.q[5][i]);
         T n2 = qxx * qxx;
         n2 = fma_t(qyy, qyy, n2);
         n2 = fma_t(T(2) * qxy, qxy, n2);
@@ -276,6 +281,92 @@ void tgv2_iteration(State<T>& st, const T* f, const Dims& s, const Params& prm,
             const T vn = fma_t(-tau, w[c][i] - st.p[c][i], st.v[c][i]);
             st.v[c][i] = vn;
             st.vbar[c][i] = fma_t(theta, vn - vprev[c][i], vn);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// TGV-k (k >= 3) deshade, reference tgvk_deshade/tgvk_l1_deshade.cu:152-351.
+// Level i (0 <= i < k-2) holds the symmetric tensors w_i, w_bar_i and the dual
+// r_i, components (x, y, z, xy, xz, yz).  One iteration: the TGV2 iteration with
+// alpha1 = alpha[0] (p), alpha0 = alpha[1] (q) and q shifted by -w_bar_0, then
+// per level i:
+//   r2  = G_i(w_bar_i)                 (i odd: D+, tgv3/tgv3_common.cu:214-262;
+//                                       i even: B, tgvk/tgvk_common.cu:89-135)
+//   r_i = proj_{alpha[i+2]}(r_i + sigma*(r2 - w_bar_{i+1}))   (last level: no
+//         w_bar_{i+1}; tgv3_common.cu:311-360, tgvk_common.cu:28-86)
+//   r2  = H_i(r_i)                     (i odd: B, tgv3_common.cu:265-309;
+//                                       i even: D+, tgvk_common.cu:138-182)
+//   w_i = w_i - tau*(r2 - q~), q~ = q (i = 0) or r_{i-1} (updated this iteration);
+//   w_bar_i = w_i + theta*(w_i - w_i_old)            (tgv3_common.cu:362-405)
+// G: diagonal component a differenced along a, off-diagonal (a, b) as
+// (d_a + d_b) * 0.5; H: diagonal d_a, off-diagonal d_a + d_b.
+// ---------------------------------------------------------------------------
+template <typename T>
+struct Level { T* w[6]; T* wbar[6]; T* r[6]; };
+
+constexpr int TENSOR_AXES[6][2] = {{0, -1}, {1, -1}, {2, -1}, {0, 1}, {0, 2}, {1, 2}};
+
+template <typename T>
+void tensor_op(T* const* in, T* const* out, const Dims& s, bool forward, bool halve, T* tmp) {
+    const i64 n = s.n();
+    for (int c = 0; c < 6; ++c) {
+        const int a = TENSOR_AXES[c][0], b = TENSOR_AXES[c][1];
+        if (forward) forward_difference(in[c], out[c], s, a);
+        else backward_difference(in[c], out[c], s, a);
+        if (b < 0) continue;
+        if (forward) forward_difference(in[c], tmp, s, b);
+        else backward_difference(in[c], tmp, s, b);
+#pragma omp parallel for schedule(static)
+        for (i64 i = 0; i < n; ++i)
+            out[c][i] = halve ? (out[c][i] + tmp[i]) * T(0.5) : out[c][i] + tmp[i];
+    }
+}
+
+template <typename T>
+void tgvk_iteration(State<T>& st, std::vector<Level<T>>& lv, const T* f, const Dims& s,
+                    const Params& prm, const std::vector<double>& alpha,
+                    std::vector<T>* tmp /* 9 */, std::vector<T>* r2buf /* 6 */) {
+    const i64 n = s.n();
+    const int L = int(lv.size());
+    const T tau = T(prm.tau), sigma = T(prm.sigma), theta = T(prm.theta);
+    tgv2_iteration(st, f, s, prm, tmp, nullptr, lv[0].wbar);
+    T* r2[6];
+    for (int c = 0; c < 6; ++c) r2[c] = r2buf[c].data();
+    T* t = tmp[7].data();
+    std::vector<T> wprev[6];
+    for (int c = 0; c < 6; ++c) wprev[c].assign(size_t(n), T(0));
+    for (int li = 0; li < L; ++li) {
+        Level<T>& l = lv[li];
+        const bool odd = (li % 2) == 1;
+        const bool last = li == L - 1;
+        const T a = T(alpha[size_t(li) + 2]);
+        tensor_op(l.wbar, r2, s, /*forward=*/odd, /*halve=*/true, t);
+        T* const* wnext = last ? nullptr : lv[li + 1].wbar;
+#pragma omp parallel for schedule(static)
+        for (i64 i = 0; i < n; ++i) {
+            T rc[6];
+            for (int c = 0; c < 6; ++c)
+                rc[c] = wnext ? fma_t(sigma, r2[c][i] - wnext[c][i], l.r[c][i])
+                              : fma_t(sigma, r2[c][i], l.r[c][i]);
+            T n2 = rc[0] * rc[0];
+            n2 = fma_t(rc[1], rc[1], n2);
+            n2 = fma_t(T(2) * rc[3], rc[3], n2);
+            n2 = fma_t(rc[2], rc[2], n2);
+            n2 = fma_t(T(2) * rc[4], rc[4], n2);
+            n2 = fma_t(T(2) * rc[5], rc[5], n2);
+            const T nrm = std::max(T(1), sqrt_t(n2) / a);
+            for (int c = 0; c < 6; ++c) { l.r[c][i] = rc[c] / nrm; wprev[c][i] = l.w[c][i]; }
+        }
+        tensor_op(l.r, r2, s, /*forward=*/!odd, /*halve=*/false, t);
+        T* const* qt = li == 0 ? st.q : lv[li - 1].r;
+#pragma omp parallel for schedule(static)
+        for (i64 i = 0; i < n; ++i) {
+            for (int c = 0; c < 6; ++c) {
+                const T wn = fma_t(-tau, r2[c][i] - qt[c][i], l.w[c][i]);
+                l.w[c][i] = wn;
+                l.wbar[c][i] = fma_t(theta, wn - wprev[c][i], wn);
+            }
         }
     }
 }
@@ -548,6 +639,41 @@ ORC_EXPORT void orc_tgv2_energy_##SUF(const T* f, T** fields, i64 w, i64 h, i64 
 
 ORC_DEFINE_TYPED(float, f32)
 ORC_DEFINE_TYPED(double, f64)
+
+// TGV-k: fields = the 17 TGV2 fields, then per level w[6], w_bar[6], r[6].
+// prm4 = (lambda, tau, sigma, theta); alpha[order].
+#define ORC_DEFINE_TGVK(T, SUF)                                                               \
+ORC_EXPORT int orc_tgvk_iterate_##SUF(T** fields, int order, const T* f, i64 w, i64 h, i64 d, \
+        const double* prm4, const double* alpha, int iterations, int planar) {                  \
+    if (order < 3) return -1;                                                                   \
+    State<T> st; fill_state_pointers(st, fields);                                               \
+    const int L = order - 2;                                                                    \
+    std::vector<Level<T>> lv(static_cast<size_t>(L));                                           \
+    for (int li = 0; li < L; ++li)                                                              \
+        for (int c = 0; c < 6; ++c) {                                                           \
+            lv[li].w[c] = fields[17 + 18 * li + c];                                             \
+            lv[li].wbar[c] = fields[17 + 18 * li + 6 + c];                                      \
+            lv[li].r[c] = fields[17 + 18 * li + 12 + c];                                        \
+        }                                                                                       \
+    Params prm{prm4[0], alpha[1], alpha[0], prm4[1], prm4[2], prm4[3], 0, planar};              \
+    std::vector<double> al(alpha, alpha + order);                                               \
+    const Dims s{w, h, d};                                                                      \
+    std::vector<T> tmp[9], r2[6];                                                               \
+    for (auto& t : tmp) t.assign(size_t(s.n()), T(0));                                          \
+    for (auto& t : r2) t.assign(size_t(s.n()), T(0));                                           \
+    for (int it = 0; it < iterations; ++it) tgvk_iteration(st, lv, f, s, prm, al, tmp, r2);         \
+    return iterations; }
+
+ORC_DEFINE_TGVK(float, f32)
+ORC_DEFINE_TGVK(double, f64)
+
+// The level operators G (halve = 1) and H (halve = 0) on six FP64 arrays.
+ORC_EXPORT void orc_tensor_op_f64(double** in, double** out, i64 w, i64 h, i64 d, int forward,
+                                  int halve) {
+    const Dims s{w, h, d};
+    std::vector<double> t(size_t(s.n()));
+    tensor_op<double>(in, out, s, forward != 0, halve != 0, t.data());
+}
 
 ORC_EXPORT double orc_cospi(double t) { return cospi_d(t); }
 ORC_EXPORT int orc_num_threads(void) { return omp_get_max_threads(); }

@@ -14,8 +14,12 @@
 // mask, buildMaskIndices2D-style lists into tgv2_l1_deshade_masked_2d_launch
 // (TGVDeshadeMaskedProcessor.cpp:215-335), and divergence_2d_kernel_launch.
 //
-//   processor_demo [--2d] in.raw W H D lambda alpha0 alpha1 iterations out_u.raw out_l.raw
-//                  out_r.raw [mask.raw]
+// --tgvk a0,a1,...,a(k-1) (k >= 3, no mask): TGVKDeshadeProcessor::processTGVKL1Cuda
+// (TGVKDeshadeProcessor.cpp:76-174) from tgvk_l1_deshade_launch (with --2d:
+// tgvk_l1_deshade_launch_2d); alpha0/alpha1 are then ignored.
+//
+//   processor_demo [--2d] [--tgvk a0,a1,...] in.raw W H D lambda alpha0 alpha1 iterations
+//                  out_u.raw out_l.raw out_r.raw [mask.raw]
 #include "ambient_amd_reference_compat.hpp"
 
 #include <cstdio>
@@ -44,8 +48,26 @@ static void write_raw(const char* path, const float* p, size_t n) {
 }
 
 int main(int argc, char** argv) {
-    const bool two_d = argc > 1 && std::string(argv[1]) == "--2d";
-    if (two_d) { ++argv; --argc; }
+    bool two_d = false;
+    std::vector<float> kalpha;
+    for (;;) {
+        if (argc > 1 && std::string(argv[1]) == "--2d") {
+            two_d = true;
+            ++argv; --argc;
+        } else if (argc > 2 && std::string(argv[1]) == "--tgvk") {
+            const std::string list = argv[2];
+            size_t pos = 0;
+            while (pos <= list.size()) {
+                const size_t e = list.find(',', pos);
+                kalpha.push_back(float(std::atof(list.substr(pos, e - pos).c_str())));
+                if (e == std::string::npos) break;
+                pos = e + 1;
+            }
+            argv += 2; argc -= 2;
+        } else {
+            break;
+        }
+    }
     if (argc != 12 && argc != 13) {
         std::fprintf(stderr, "usage: %s in.raw W H D lambda alpha0 alpha1 iterations u l r [mask]\n",
                      argv[0]);
@@ -64,10 +86,21 @@ int main(int argc, char** argv) {
     std::vector<float> f = read_raw(argv[1], n);
 
     const bool masked = argc == 13;
+    if (!kalpha.empty() && (masked || kalpha.size() < 3)) {
+        std::fprintf(stderr, "--tgvk needs at least three weights and no mask\n");
+        return 1;
+    }
     std::vector<float> mask = masked ? read_raw(argv[12], n) : std::vector<float>();
     float *v_x, *v_y, *v_z = nullptr;
     float* u = nullptr;
-    if (two_d && !masked) {
+    const unsigned order = unsigned(kalpha.size());
+    if (order && two_d) {
+        u = tgvk_l1_deshade_launch_2d<float>(f.data(), W, H, lambda, iterations, iterations + 1,
+                                             nullptr, order, kalpha.data(), &v_x, &v_y);
+    } else if (order) {
+        u = tgvk_l1_deshade_launch<float>(f.data(), W, H, D, lambda, iterations, iterations + 1,
+                                          nullptr, order, kalpha.data(), &v_x, &v_y, &v_z);
+    } else if (two_d && !masked) {
         u = tgv2_l1_deshade_launch_2d<float>(f.data(), W, H, lambda, iterations, iterations + 1,
                                              nullptr, alpha0, alpha1, &v_x, &v_y);
     } else if (two_d) {

@@ -1,0 +1,232 @@
+"""GPU tests of the TGV-k (k >= 3) deshade (reference
+tgvk_deshade/tgvk_l1_deshade.cu, its 2-D form tgvk_l1_deshade_2d.cu and the
+processor TGVKDeshadeProcessor.cpp:76-174).
+
+The two TGV-k sweeps perform the oracle's per-voxel operations in the same
+order, so u and v are compared bit for bit with the FP32 oracle after enough
+iterations for every level to feed back into them (odd and even levels, the
+last-level rule and q~ = r_{i-1} all reach v through w_0)."""
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import rel_err
+
+pytestmark = pytest.mark.gpu
+
+TAU = float(np.float32(1 / np.sqrt(20)))
+KPRM = dict(lam=0.7, tau=TAU, sigma=TAU, theta=1.0)
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _params(native, iterations, **kw):
+    return native.default_params(lam=KPRM["lam"], tau=TAU, sigma=TAU, theta=1.0,
+                                 iterations=iterations, data_term=native.DATA_L1, **kw)
+
+
+def _f(shape, seed):
+    from ambient_image_processor_amd import phantoms
+    return phantoms.deshade_phantom(shape, seed=seed)[0]
+
+
+SHAPES = [(9, 13, 21), (8, 12, 64), (5, 70, 33), (17, 4, 130), (3, 3, 3), (1, 20, 28)]
+ALPHAS = [(1.0, 2.0, 0.6), (1.0, 2.0, 0.8, 0.5), (0.8, 1.5, 0.7, 0.4, 0.3)]
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+@pytest.mark.parametrize("alpha", ALPHAS)
+def test_tgvk_bitwise_against_oracle(ctx, orc, shape, alpha):
+    from ambient_image_processor_amd import native
+    _say("tgvk", shape, "order", len(alpha))
+    f = _f(shape, 3)
+    u, vx, vy, vz = ctx.tgvk_deshade(f, _params(native, 25), alpha)
+    st = orc.tgvk(f, 25, alpha=alpha, dtype=np.float32, **KPRM)
+    np.testing.assert_array_equal(u, st.u)
+    np.testing.assert_array_equal(vx, st.v[0])
+    np.testing.assert_array_equal(vy, st.v[1])
+    np.testing.assert_array_equal(vz, st.v[2])
+
+
+def test_tgvk_many_levels(ctx, orc):
+    from ambient_image_processor_amd import native
+    alpha = (1.0, 2.0, 0.9, 0.8, 0.7, 0.6, 0.5, 0.4)
+    f = _f((6, 10, 40), 4)
+    u, vx, vy, vz = ctx.tgvk_deshade(f, _params(native, 30), alpha)
+    st = orc.tgvk(f, 30, alpha=alpha, dtype=np.float32, **KPRM)
+    np.testing.assert_array_equal(u, st.u)
+    np.testing.assert_array_equal(vz, st.v[2])
+
+
+@pytest.mark.parametrize("alpha", ALPHAS[:2])
+def test_tgvk_planar_bitwise(ctx, orc, alpha):
+    from ambient_image_processor_amd import native
+    f = _f((1, 24, 36), 5)
+    u, vx, vy, vz = ctx.tgvk_deshade(f, _params(native, 30, flags=native.FLAG_PLANAR), alpha)
+    st = orc.tgvk(f, 30, alpha=alpha, dtype=np.float32, planar=True, **KPRM)
+    np.testing.assert_array_equal(u, st.u)
+    np.testing.assert_array_equal(vx, st.v[0])
+    np.testing.assert_array_equal(vy, st.v[1])
+    assert not np.any(vz)
+
+
+def test_tgvk_callback_and_stop(ctx, orc):
+    from ambient_image_processor_amd import native
+    f = _f((7, 9, 20), 6)
+    alpha = (1.0, 2.0, 0.6, 0.4)
+    seen = []
+
+    def cb(it, count, u, vx, vy, vz):
+        seen.append(it)
+        return it >= 20
+
+    u, vx, vy, vz = ctx.tgvk_deshade(f, _params(native, 100, callback_interval=10), alpha, cb)
+    assert seen == [10, 20]
+    st = orc.tgvk(f, 21, alpha=alpha, dtype=np.float32, **KPRM)
+    np.testing.assert_array_equal(u, st.u)
+    np.testing.assert_array_equal(vy, st.v[1])
+
+
+def test_tgvk_launch_counts(ctx):
+    from ambient_image_processor_amd import native
+    f = _f((5, 8, 16), 7)
+    ctx.profile(True)
+    ctx.profile_reset()
+    ctx.tgvk_deshade(f, _params(native, 13), (1.0, 2.0, 0.5, 0.5, 0.5))
+    prof = ctx.profile_read()
+    ctx.profile(False)
+    assert prof["tgvk_dual"][1] == 13 and prof["tgvk_primal"][1] == 13
+    assert prof["dual_sweep"][1] == 0 and prof["fused_sweep"][1] == 0
+
+
+@pytest.mark.parametrize("with_mask", [False, True])
+def test_tgvk_pipeline_against_oracle(ctx, orc, with_mask):
+    """processTGVKL1Cuda: the TGV-k loop, the Poisson integration, the mask on
+    l and r, the clamp and the add-back."""
+    from ambient_image_processor_amd import native
+    f = _f((10, 14, 22), 8)
+    alpha = (1.0, 2.0, 0.6)
+    m = None
+    flags = native.FLAG_SET_NEGATIVE_ZERO
+    if with_mask:
+        m = (np.random.default_rng(9).random(f.shape) > 0.3).astype(np.float32)
+        flags |= native.FLAG_ADD_BACKGROUND_BACK
+    u, l, r = ctx.tgvk_deshade_pipeline(f, _params(native, 40, flags=flags), alpha, m)
+    uo, lo, ro, _ = orc.deshade_k(f, 40, alpha=alpha, mask=m, set_negative_zero=True,
+                                  add_background_back=with_mask, **KPRM)
+    np.testing.assert_array_equal(u, uo)
+    assert rel_err(l, lo) < 1e-5 and rel_err(r, ro) < 1e-5
+
+
+def test_tgvk_pipeline_order_2_and_depth_1(ctx, orc):
+    from ambient_image_processor_amd import native
+    f = _f((6, 10, 16), 10)
+    u, l, r = ctx.tgvk_deshade_pipeline(f, _params(native, 30), (1.0, 2.0))
+    uo, lo, ro, _ = orc.deshade_k(f, 30, alpha=(1.0, 2.0), **KPRM)
+    np.testing.assert_array_equal(u, uo)
+    assert rel_err(l, lo) < 1e-5
+    f1 = _f((1, 18, 26), 11)
+    for alpha in ((1.0, 2.0), (1.0, 2.0, 0.5)):
+        u, l, r = ctx.tgvk_deshade_pipeline(f1, _params(native, 30), alpha)
+        uo, lo, ro, _ = orc.deshade_k(f1, 30, alpha=alpha, **KPRM)
+        np.testing.assert_array_equal(u, uo)
+        assert rel_err(l, lo) < 3e-5 and rel_err(r, ro) < 3e-5
+
+
+def test_tgvk_argument_errors(ctx):
+    from ambient_image_processor_amd import native
+    f = _f((3, 4, 8), 1)
+    with pytest.raises(native.AmbError):
+        ctx.tgvk_deshade(f, _params(native, 2), (1.0, 2.0))
+    with pytest.raises(native.AmbError):
+        ctx.tgvk_deshade(f, _params(native, 2), (1.0, 2.0, 0.0))
+    with pytest.raises(native.AmbError):
+        p = _params(native, 2)
+        p.data_term = native.DATA_L2
+        ctx.tgvk_deshade(f, p, (1.0, 2.0, 0.5))
+    with pytest.raises(native.AmbError):
+        ctx.tgvk_deshade(f, _params(native, 2, storage=native.STORAGE_BF16), (1.0, 2.0, 0.5))
+    with pytest.raises(native.AmbError):
+        ctx.tgvk_deshade(f, _params(native, 2, flags=native.FLAG_PLANAR), (1.0, 2.0, 0.5))
+    # the context still works afterwards
+    u, *_ = ctx.tgvk_deshade(f, _params(native, 0), (1.0, 2.0, 0.5))
+    np.testing.assert_array_equal(u, f)
+
+
+def test_processor_mirror_with_callback_and_div_v(orc):
+    """TGVKDeshadeProcessor.processTGVKL1Cuda: outputs, the callback's
+    (u, l, r) and div v."""
+    from ambient_image_processor_amd.processor import TGVKDeshadeProcessor
+    f = _f((8, 12, 20), 12)
+    m = (np.random.default_rng(13).random(f.shape) > 0.25).astype(np.float32)
+    alpha = (1.0, 2.0, 0.6)
+    got = {}
+
+    def cb(it, count, u, l, r):
+        got[it] = (u.copy(), l.copy(), r.copy())
+        return False
+
+    u, l, r, div_v = TGVKDeshadeProcessor.processTGVKL1Cuda(
+        f, KPRM["lam"], 3, alpha, 30, mask=m, add_background_back=True,
+        paint_iteration_interval=15, iteration_finished_callback=cb, calculate_div_v=True)
+    uo, lo, ro, st = orc.deshade_k(f, 30, alpha=alpha, mask=m, add_background_back=True, **KPRM)
+    np.testing.assert_array_equal(u, uo)
+    assert rel_err(l, lo) < 1e-5 and rel_err(r, ro) < 1e-5
+    np.testing.assert_array_equal(div_v, orc.divergence(*st.v, dtype=np.float32))
+    assert sorted(got) == [15]
+    u16, l16, r16, _ = orc.deshade_k(f, 16, alpha=alpha, mask=m, add_background_back=True,
+                                     **KPRM)
+    np.testing.assert_array_equal(got[15][0], u16)
+    assert rel_err(got[15][1], l16) < 1e-5 and rel_err(got[15][2], r16) < 1e-5
+    seen = []
+    TGVKDeshadeProcessor.updateAlpha(4, seen.append)
+    assert seen == [1, 4, 7, 10]
+
+
+def test_reference_tgvk_launchers(orc):
+    from ambient_image_processor_amd import processor
+    f = _f((5, 9, 24), 14)
+    d, h, w = f.shape
+    alpha = (1.0, 2.0, 0.7, 0.4)
+    u, vx, vy, vz = processor.tgvk_l1_deshade_launch(f.ravel(), w, h, d, KPRM["lam"], 20, 0,
+                                                     None, 4, alpha)
+    st = orc.tgvk(f, 20, alpha=alpha, dtype=np.float32, **KPRM)
+    np.testing.assert_array_equal(u, st.u)
+    np.testing.assert_array_equal(vz, st.v[2])
+    f1 = _f((1, 16, 22), 15)
+    u, vx, vy = processor.tgvk_l1_deshade_launch_2d(f1.ravel(), 22, 16, KPRM["lam"], 20, 0, None,
+                                                    4, alpha)
+    st = orc.tgvk(f1, 20, alpha=alpha, dtype=np.float32, planar=True, **KPRM)
+    np.testing.assert_array_equal(u, st.u)
+    np.testing.assert_array_equal(vy, st.v[1])
+
+
+@pytest.mark.parametrize("two_d", [False, True])
+def test_reference_tgvk_launcher_caller(orc, tmp_path, two_d):
+    """tests/compat/processor_demo.cpp --tgvk: tgvk_l1_deshade_launch(_2d) of the
+    compatibility header, then the Poisson integration and r = u - l, clamp."""
+    import os
+    import subprocess
+    from ambient_image_processor_amd import _build
+    assert os.path.exists(_build.COMPAT_DEMO), "run __graft_entry__.build() first"
+    shape = (1, 18, 30) if two_d else (6, 10, 24)
+    f = _f(shape, 16)
+    d, h, w = shape
+    f.tofile(tmp_path / "f.raw")
+    out = [str(tmp_path / n) for n in ("u.raw", "l.raw", "r.raw")]
+    alpha = (1.0, 2.0, 0.6, 0.4)
+    args = [_build.COMPAT_DEMO] + (["--2d"] if two_d else []) + [
+        "--tgvk", ",".join(str(a) for a in alpha), str(tmp_path / "f.raw"), str(w), str(h),
+        str(d), str(KPRM["lam"]), "0", "0", "25"] + out
+    _say("running", " ".join(args[:4]))
+    r = subprocess.run(args, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    u, l, rr = (np.fromfile(p, dtype=np.float32).reshape(shape) for p in out)
+    uo, lo, ro, _ = orc.deshade_k(f, 25, alpha=alpha, **KPRM)
+    np.testing.assert_array_equal(u, uo)
+    np.testing.assert_array_equal(l, lo)
+    np.testing.assert_array_equal(rr, ro)
