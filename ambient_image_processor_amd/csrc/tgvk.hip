@@ -12,41 +12,69 @@
 // launches whatever k is:
 //   dual   - p, q and every r_i, in place (each voxel reads only its own p, q, r);
 //   primal - u, v and every w_i with their over-relaxations, in place.
-// Each thread owns one (x, y) column and marches along z over `zchunk`
-// planes; neighbours are read from global memory (the x/y neighbours were just
-// read by adjacent lanes and waves, the z neighbour by the same thread one
-// plane earlier, so they come from L1/L2).  Level fields are reached through a
-// device table of 18 pointers per level (w[6], w_bar[6], r[6], components
-// x, y, z, xy, xz, yz), so the order is a run-time value.
+// Each thread owns one (x, y) column of a 64 x 4 tile and marches along z over
+// `zchunk` planes; tiles are dealt so that each XCD holds a contiguous range
+// (whole z-slabs), so halo rows are read through its own L2.  Level fields are
+// reached through a device table of 18 pointers per level (w[6], w_bar[6],
+// r[6], components x, y, z, xy, xz, yz).  For 1 to 3 levels the kernels are
+// instantiated per level count: every load of a plane is issued before any of
+// its stores, and the -z neighbours (previous plane's centres) stay in
+// registers.  Higher orders run the generic kernels, which loop over the
+// levels at run time.
 //
 // Per-voxel arithmetic keeps the reference's FP32 operation order (fmaf where
 // the reference writes fmaf or nvcc contracts a multiply-add; correctly
 // rounded division and sqrt; built with -ffp-contract=off).
 #include "amb_internal.hpp"
 
+#include <algorithm>
+#include <cstdlib>
+
 namespace amb {
 namespace {
 
-constexpr int KTX = 64, KTY = 4;
+constexpr int KTX = 64, KTY = 4, KT = KTX * KTY;
 
 // tensor component c -> its first and second axis (-1: diagonal component)
 __device__ __forceinline__ int comp_axis_a(int c) { return c < 3 ? c : (c == 5 ? 1 : 0); }
 __device__ __forceinline__ int comp_axis_b(int c) { return c < 3 ? -1 : (c == 3 ? 1 : 2); }
+
+// Block id -> tile id such that each XCD (blocks are dealt round-robin to the
+// 8 XCDs) receives a contiguous range of tiles.
+__device__ __forceinline__ int kxcd_tile(int bid, int nblocks) {
+    if ((nblocks & 7) != 0) return bid;
+    return (bid & 7) * (nblocks >> 3) + (bid >> 3);
+}
 
 struct KCoord {
     int x, y, z0, z1;
     size_t plane, col;
     bool in;
     __device__ __forceinline__ KCoord(const Geometry& g, int zchunk) {
-        x = int(blockIdx.x) * KTX + int(threadIdx.x);
-        y = int(blockIdx.y) * KTY + int(threadIdx.y);
-        z0 = int(blockIdx.z) * zchunk;
+        const int nbx = (g.w + KTX - 1) / KTX, nby = (g.h + KTY - 1) / KTY;
+        const int t = kxcd_tile(int(blockIdx.x), int(gridDim.x));
+        const int bx = t % nbx, r = t / nbx, by = r % nby, bz = r / nby;
+        x = bx * KTX + int(threadIdx.x) % KTX;
+        y = by * KTY + int(threadIdx.x) / KTX;
+        z0 = bz * zchunk;
         z1 = min(z0 + zchunk, g.d);
         in = x < g.w && y < g.h;
         plane = size_t(g.w) * size_t(g.h);
         col = size_t(y) * size_t(g.w) + size_t(x);
     }
 };
+
+// Loads and stores at a 32-bit byte offset from a uniform base (one offset
+// register serves every field; used where 4 * voxels < 2^31).
+__device__ __forceinline__ float LDO(const float* p, uint32_t off) {
+    return *reinterpret_cast<const float*>(reinterpret_cast<const char*>(p) + off);
+}
+__device__ __forceinline__ void STO(float* p, uint32_t off, float v) {
+    *reinterpret_cast<float*>(reinterpret_cast<char*>(p) + off) = v;
+}
+
+// slot of a z-differenced tensor component (2: z, 4: xz, 5: yz) in the carries
+__device__ __forceinline__ int zslot(int c) { return c == 2 ? 0 : (c == 4 ? 1 : 2); }
 
 // Reference "backward difference" (tgv_cuda/tgv_common.cu:133-197): -a[i] + a[i-1], -a[0] at 0.
 __device__ __forceinline__ float kbwd(const float* a, float center, size_t i, size_t st, bool has) {
@@ -57,7 +85,7 @@ __device__ __forceinline__ float kfwd(const float* a, float center, size_t i, si
     return has ? a[i + st] - center : 0.0f;
 }
 
-__global__ __launch_bounds__(KTX * KTY) void tgvk_dual_kernel(
+__global__ __launch_bounds__(KT) void tgvk_dual_kernel(
     float* const* __restrict__ F, float* const* __restrict__ LV,
     const float* __restrict__ alpha, Geometry g, int levels, float sigma, int planar, int zchunk) {
     const KCoord c(g, zchunk);
@@ -175,7 +203,7 @@ __global__ __launch_bounds__(KTX * KTY) void tgvk_dual_kernel(
     }
 }
 
-__global__ __launch_bounds__(KTX * KTY) void tgvk_primal_kernel(
+__global__ __launch_bounds__(KT) void tgvk_primal_kernel(
     float* const* __restrict__ F, float* const* __restrict__ LV, const float* __restrict__ f,
     Geometry g, int levels, float tau, float lambda, float theta, int zchunk) {
     const KCoord c(g, zchunk);
@@ -265,15 +293,355 @@ __global__ __launch_bounds__(KTX * KTY) void tgvk_primal_kernel(
     }
 }
 
-dim3 tgvk_grid(const Geometry& g, int* zchunk) {
+// The same updates for a compile-time level count, loads first.
+template <int L>
+__global__ __launch_bounds__(KT) void tgvk_dual_fixed(
+    float* const* __restrict__ F, float* const* __restrict__ LV,
+    const float* __restrict__ alpha, Geometry g, float sigma, int planar, int zchunk) {
+    const KCoord c(g, zchunk);
+    if (!c.in) return;
+    const int W = g.w, H = g.h, D = g.d;
+    const uint32_t st[3] = {4u, 4u * uint32_t(W), 4u * uint32_t(c.plane)};
+    const float a_p = alpha[0], a_q = alpha[1];
+    const float* ub = F[F_UBAR];
+    const float* vb[3] = {F[F_VBX], F[F_VBY], F[F_VBZ]};
+    float* pp[3] = {F[F_PX], F[F_PY], F[F_PZ]};
+    float* qq[6] = {F[F_QXX], F[F_QYY], F[F_QZZ], F[F_QXY], F[F_QXZ], F[F_QYZ]};
+    const float* wb[L][6];
+    float* rr[L][6];
+    float ar[L];
+#pragma unroll
+    for (int l = 0; l < L; ++l) {
+        ar[l] = alpha[l + 2];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            wb[l][k] = LV[18 * l + 6 + k];
+            rr[l][k] = LV[18 * l + 12 + k];
+        }
+    }
+    const bool prv0[2] = {c.x > 0, c.y > 0};
+    const bool nxt0[2] = {c.x < W - 1, c.y < H - 1};
+
+    // previous plane's v_bar and (even levels, backward G) w_bar z-components
+    float vb_prev[3] = {0.f, 0.f, 0.f};
+    float wb_prev[L][3];
+#pragma unroll
+    for (int l = 0; l < L; ++l) wb_prev[l][0] = wb_prev[l][1] = wb_prev[l][2] = 0.f;
+    if (c.z0 > 0) {
+        const uint32_t j = 4u * (uint32_t(c.z0 - 1) * uint32_t(c.plane) + uint32_t(c.col));
+#pragma unroll
+        for (int k = 0; k < 3; ++k) vb_prev[k] = LDO(vb[k], j);
+#pragma unroll
+        for (int l = 0; l < L; l += 2) {
+            wb_prev[l][0] = LDO(wb[l][2], j);
+            wb_prev[l][1] = LDO(wb[l][4], j);
+            wb_prev[l][2] = LDO(wb[l][5], j);
+        }
+    }
+
+    for (int z = c.z0; z < c.z1; ++z) {
+        const uint32_t i = 4u * (uint32_t(z) * uint32_t(c.plane) + uint32_t(c.col));
+        const bool prv[3] = {prv0[0], prv0[1], z > 0};
+        const bool nxt[3] = {nxt0[0], nxt0[1], z < D - 1};
+        // ---- loads
+        const float u0 = LDO(ub, i);
+        float un[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) un[a] = nxt[a] ? LDO(ub, i + st[a]) : 0.f;
+        float v0[3], vxm[3], vym[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            v0[k] = LDO(vb[k], i);
+            vxm[k] = prv[0] ? LDO(vb[k], i - st[0]) : 0.f;
+            vym[k] = prv[1] ? LDO(vb[k], i - st[1]) : 0.f;
+        }
+        float p0[3], q0[6];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) p0[k] = LDO(pp[k], i);
+#pragma unroll
+        for (int k = 0; k < 6; ++k) q0[k] = LDO(qq[k], i);
+        float w0[L][6], wa[L][6], wbn[L][6], r0[L][6];
+#pragma unroll
+        for (int l = 0; l < L; ++l) {
+            const bool fw = (l & 1) != 0;
+#pragma unroll
+            for (int k = 0; k < 6; ++k) {
+                w0[l][k] = LDO(wb[l][k], i);
+                r0[l][k] = LDO(rr[l][k], i);
+                const int a = comp_axis_a(k), b = comp_axis_b(k);
+                if (fw) wa[l][k] = nxt[a] ? LDO(wb[l][k], i + st[a]) : 0.f;
+                else wa[l][k] = (a == 2) ? wb_prev[l][0] : (prv[a] ? LDO(wb[l][k], i - st[a]) : 0.f);
+                wbn[l][k] = 0.f;
+                if (b == 2) wbn[l][k] = fw ? (nxt[2] ? LDO(wb[l][k], i + st[2]) : 0.f) : wb_prev[l][zslot(k)];
+                else if (b >= 0) wbn[l][k] = fw ? (nxt[b] ? LDO(wb[l][k], i + st[b]) : 0.f)
+                                                : (prv[b] ? LDO(wb[l][k], i - st[b]) : 0.f);
+            }
+        }
+
+        // ---- p update + projection (tgv2_common.cu:96-131)
+        float a3[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const float d = nxt[a] ? un[a] - u0 : 0.0f;
+            a3[a] = fmaf(sigma, d - v0[a], p0[a]);
+        }
+        float n2 = a3[0] * a3[0];
+        n2 = fmaf(a3[1], a3[1], n2);
+        n2 = fmaf(a3[2], a3[2], n2);
+        const float tp = sqrtf(n2) / a_p;
+        const bool sp = tp > 1.0f;
+
+        // ---- q update with eps_B(v_bar) - w_bar_0 (tgv2_common.cu:174-219, tgv3_common.cu:28-86)
+        float bxv[3], byv[3], bzv[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            bxv[k] = prv[0] ? -v0[k] + vxm[k] : -v0[k];
+            byv[k] = prv[1] ? -v0[k] + vym[k] : -v0[k];
+            bzv[k] = planar ? 0.0f : (prv[2] ? -v0[k] + vb_prev[k] : -v0[k]);
+        }
+        float e[6];
+        e[0] = bxv[0];
+        e[1] = byv[1];
+        e[2] = bzv[2];
+        e[3] = (bxv[1] + byv[0]) * 0.5f;
+        e[4] = (bxv[2] + bzv[0]) * 0.5f;
+        e[5] = (byv[2] + bzv[1]) * 0.5f;
+        float b6[6];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) b6[k] = fmaf(sigma, e[k] - w0[0][k], q0[k]);
+        float m2 = b6[0] * b6[0];
+        m2 = fmaf(b6[1], b6[1], m2);
+        m2 = fmaf(2.0f * b6[3], b6[3], m2);
+        m2 = fmaf(b6[2], b6[2], m2);
+        m2 = fmaf(2.0f * b6[4], b6[4], m2);
+        m2 = fmaf(2.0f * b6[5], b6[5], m2);
+        const float tq = sqrtf(m2) / a_q;
+        const bool sq = tq > 1.0f;
+
+        // ---- levels: r_l update + projection with alpha[l + 2]
+        float rn[L][6];
+#pragma unroll
+        for (int l = 0; l < L; ++l) {
+            const bool fw = (l & 1) != 0;
+            float rc[6];
+#pragma unroll
+            for (int k = 0; k < 6; ++k) {
+                const int a = comp_axis_a(k), b = comp_axis_b(k);
+                float d = fw ? (nxt[a] ? wa[l][k] - w0[l][k] : 0.0f)
+                             : (prv[a] ? -w0[l][k] + wa[l][k] : -w0[l][k]);
+                if (b >= 0) {
+                    const float d2 = fw ? (nxt[b] ? wbn[l][k] - w0[l][k] : 0.0f)
+                                        : (prv[b] ? -w0[l][k] + wbn[l][k] : -w0[l][k]);
+                    d = (d + d2) * 0.5f;
+                }
+                rc[k] = (l + 1 < L) ? fmaf(sigma, d - w0[l + 1][k], r0[l][k])
+                                    : fmaf(sigma, d, r0[l][k]);
+            }
+            float r2 = rc[0] * rc[0];
+            r2 = fmaf(rc[1], rc[1], r2);
+            r2 = fmaf(2.0f * rc[3], rc[3], r2);
+            r2 = fmaf(rc[2], rc[2], r2);
+            r2 = fmaf(2.0f * rc[4], rc[4], r2);
+            r2 = fmaf(2.0f * rc[5], rc[5], r2);
+            const float tr = sqrtf(r2) / ar[l];
+            const bool sr = tr > 1.0f;
+#pragma unroll
+            for (int k = 0; k < 6; ++k) rn[l][k] = sr ? rc[k] / tr : rc[k];
+        }
+
+        // ---- stores
+#pragma unroll
+        for (int k = 0; k < 3; ++k) STO(pp[k], i, sp ? a3[k] / tp : a3[k]);
+#pragma unroll
+        for (int k = 0; k < 6; ++k) STO(qq[k], i, sq ? b6[k] / tq : b6[k]);
+#pragma unroll
+        for (int l = 0; l < L; ++l)
+#pragma unroll
+            for (int k = 0; k < 6; ++k) STO(rr[l][k], i, rn[l][k]);
+
+        // ---- carries
+#pragma unroll
+        for (int k = 0; k < 3; ++k) vb_prev[k] = v0[k];
+#pragma unroll
+        for (int l = 0; l < L; l += 2) {
+            wb_prev[l][0] = w0[l][2];
+            wb_prev[l][1] = w0[l][4];
+            wb_prev[l][2] = w0[l][5];
+        }
+    }
+}
+
+template <int L>
+__global__ __launch_bounds__(KT) void tgvk_primal_fixed(
+    float* const* __restrict__ F, float* const* __restrict__ LV, const float* __restrict__ f,
+    Geometry g, float tau, float lambda, float theta, int zchunk) {
+    const KCoord c(g, zchunk);
+    if (!c.in) return;
+    const int W = g.w, H = g.h, D = g.d;
+    const uint32_t st[3] = {4u, 4u * uint32_t(W), 4u * uint32_t(c.plane)};
+    const float tl = tau * lambda;
+    float* uu = F[F_U];
+    float* ub = F[F_UBAR];
+    float* vv[3] = {F[F_VX], F[F_VY], F[F_VZ]};
+    float* vb[3] = {F[F_VBX], F[F_VBY], F[F_VBZ]};
+    const float* pp[3] = {F[F_PX], F[F_PY], F[F_PZ]};
+    const float* qq[6] = {F[F_QXX], F[F_QYY], F[F_QZZ], F[F_QXY], F[F_QXZ], F[F_QYZ]};
+    float* ww[L][6];
+    float* wbb[L][6];
+    const float* rr[L][6];
+#pragma unroll
+    for (int l = 0; l < L; ++l)
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            ww[l][k] = LV[18 * l + k];
+            wbb[l][k] = LV[18 * l + 6 + k];
+            rr[l][k] = LV[18 * l + 12 + k];
+        }
+    const bool prv0[2] = {c.x > 0, c.y > 0};
+    const bool nxt0[2] = {c.x < W - 1, c.y < H - 1};
+
+    // previous plane's p_z and (odd levels, backward H) r z-components
+    float pz_prev = 0.f;
+    float r_prev[L][3];
+#pragma unroll
+    for (int l = 0; l < L; ++l) r_prev[l][0] = r_prev[l][1] = r_prev[l][2] = 0.f;
+    if (c.z0 > 0) {
+        const uint32_t j = 4u * (uint32_t(c.z0 - 1) * uint32_t(c.plane) + uint32_t(c.col));
+        pz_prev = LDO(pp[2], j);
+#pragma unroll
+        for (int l = 1; l < L; l += 2) {
+            r_prev[l][0] = LDO(rr[l][2], j);
+            r_prev[l][1] = LDO(rr[l][4], j);
+            r_prev[l][2] = LDO(rr[l][5], j);
+        }
+    }
+
+    for (int z = c.z0; z < c.z1; ++z) {
+        const uint32_t i = 4u * (uint32_t(z) * uint32_t(c.plane) + uint32_t(c.col));
+        const bool prv[3] = {prv0[0], prv0[1], z > 0};
+        const bool nxt[3] = {nxt0[0], nxt0[1], z < D - 1};
+        // ---- loads
+        float p0[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) p0[k] = LDO(pp[k], i);
+        const float pxm = prv[0] ? LDO(pp[0], i - st[0]) : 0.f;
+        const float pym = prv[1] ? LDO(pp[1], i - st[1]) : 0.f;
+        const float uo = LDO(uu, i), fc = LDO(f, i);
+        float q0[6], qa[6], qbn[6];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            q0[k] = LDO(qq[k], i);
+            const int a = comp_axis_a(k), b = comp_axis_b(k);
+            qa[k] = nxt[a] ? LDO(qq[k], i + st[a]) : 0.f;
+            qbn[k] = (b >= 0 && nxt[b]) ? LDO(qq[k], i + st[b]) : 0.f;
+        }
+        float v0[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) v0[k] = LDO(vv[k], i);
+        float r0[L][6], ra[L][6], rbn[L][6], w0[L][6];
+#pragma unroll
+        for (int l = 0; l < L; ++l) {
+            const bool fw = (l & 1) == 0;
+#pragma unroll
+            for (int k = 0; k < 6; ++k) {
+                r0[l][k] = LDO(rr[l][k], i);
+                w0[l][k] = LDO(ww[l][k], i);
+                const int a = comp_axis_a(k), b = comp_axis_b(k);
+                if (fw) ra[l][k] = nxt[a] ? LDO(rr[l][k], i + st[a]) : 0.f;
+                else ra[l][k] = (a == 2) ? r_prev[l][0] : (prv[a] ? LDO(rr[l][k], i - st[a]) : 0.f);
+                rbn[l][k] = 0.f;
+                if (b == 2) rbn[l][k] = fw ? (nxt[2] ? LDO(rr[l][k], i + st[2]) : 0.f) : r_prev[l][zslot(k)];
+                else if (b >= 0) rbn[l][k] = fw ? (nxt[b] ? LDO(rr[l][k], i + st[b]) : 0.f)
+                                                : (prv[b] ? LDO(rr[l][k], i - st[b]) : 0.f);
+            }
+        }
+
+        // ---- u update, L1 prox, over-relaxation (tgv2_l1.cu:27-60)
+        const float bx = prv[0] ? -p0[0] + pxm : -p0[0];
+        const float by = prv[1] ? -p0[1] + pym : -p0[1];
+        const float bz = prv[2] ? -p0[2] + pz_prev : -p0[2];
+        const float div = (bx + by) + bz;
+        float un = fmaf(-tau, div, uo);
+        const float umf = un - fc;
+        if (umf > tl) un = un - tl;
+        else if (umf < -tl) un = un + tl;
+        else un = fc;
+
+        // ---- div2(q), v update (tgv2_common.cu:279-354)
+        const float dxx = nxt[0] ? qa[0] - q0[0] : 0.0f;
+        const float dyxy = nxt[1] ? qbn[3] - q0[3] : 0.0f;
+        const float dzxz = nxt[2] ? qbn[4] - q0[4] : 0.0f;
+        const float dyy = nxt[1] ? qa[1] - q0[1] : 0.0f;
+        const float dxxy = nxt[0] ? qa[3] - q0[3] : 0.0f;
+        const float dzyz = nxt[2] ? qbn[5] - q0[5] : 0.0f;
+        const float dzz = nxt[2] ? qa[2] - q0[2] : 0.0f;
+        const float dxxz = nxt[0] ? qa[4] - q0[4] : 0.0f;
+        const float dyyz = nxt[1] ? qa[5] - q0[5] : 0.0f;
+        const float w3[3] = {(dxx + dyxy) + dzxz, (dyy + dxxy) + dzyz, (dzz + dxxz) + dyyz};
+        float vn[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) vn[k] = fmaf(-tau, w3[k] - p0[k], v0[k]);
+
+        // ---- levels: w_l from H(r_l) - q~ (tgv3_common.cu:362-405)
+        float wn[L][6];
+#pragma unroll
+        for (int l = 0; l < L; ++l) {
+            const bool fw = (l & 1) == 0;
+#pragma unroll
+            for (int k = 0; k < 6; ++k) {
+                const int a = comp_axis_a(k), b = comp_axis_b(k);
+                float h = fw ? (nxt[a] ? ra[l][k] - r0[l][k] : 0.0f)
+                             : (prv[a] ? -r0[l][k] + ra[l][k] : -r0[l][k]);
+                if (b >= 0)
+                    h = h + (fw ? (nxt[b] ? rbn[l][k] - r0[l][k] : 0.0f)
+                                : (prv[b] ? -r0[l][k] + rbn[l][k] : -r0[l][k]));
+                const float qt = l == 0 ? q0[k] : r0[l > 0 ? l - 1 : 0][k];
+                wn[l][k] = fmaf(-tau, h - qt, w0[l][k]);
+            }
+        }
+
+        // ---- stores
+        STO(uu, i, un);
+        STO(ub, i, fmaf(theta, un - uo, un));
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            STO(vv[k], i, vn[k]);
+            STO(vb[k], i, fmaf(theta, vn[k] - v0[k], vn[k]));
+        }
+#pragma unroll
+        for (int l = 0; l < L; ++l)
+#pragma unroll
+            for (int k = 0; k < 6; ++k) {
+                STO(ww[l][k], i, wn[l][k]);
+                STO(wbb[l][k], i, fmaf(theta, wn[l][k] - w0[l][k], wn[l][k]));
+            }
+
+        // ---- carries
+        pz_prev = p0[2];
+#pragma unroll
+        for (int l = 1; l < L; l += 2) {
+            r_prev[l][0] = r0[l][2];
+            r_prev[l][1] = r0[l][4];
+            r_prev[l][2] = r0[l][5];
+        }
+    }
+}
+
+// Planes per block: enough z-chunks for about 4096 blocks (AMB_TGVK_ZCHUNK
+// overrides, for tests of the chunk edges).
+int tgvk_blocks(const Geometry& g, int* zchunk) {
     const int nbx = (g.w + KTX - 1) / KTX, nby = (g.h + KTY - 1) / KTY;
     const long cols = long(nbx) * nby;
     long nz = 4096 / (cols > 0 ? cols : 1);
     if (nz < 1) nz = 1;
     if (nz > g.d) nz = g.d;
-    const int zc = int((g.d + nz - 1) / nz);
+    int zc = int((g.d + nz - 1) / nz);
+    if (const char* e = std::getenv("AMB_TGVK_ZCHUNK")) {
+        const int v = std::atoi(e);
+        if (v > 0) zc = std::min(v, g.d);
+    }
     *zchunk = zc;
-    return dim3(unsigned(nbx), unsigned(nby), unsigned((g.d + zc - 1) / zc));
+    return int(cols * ((g.d + zc - 1) / zc));
 }
 
 }  // namespace
@@ -282,9 +650,19 @@ hipError_t launch_tgvk_dual(hipStream_t s, float* const* fields_dev, float* cons
                             const float* alpha_dev, Geometry g, int levels, float sigma,
                             int planar) {
     int zc = 1;
-    const dim3 grid = tgvk_grid(g, &zc);
-    hipLaunchKernelGGL(tgvk_dual_kernel, grid, dim3(KTX, KTY, 1), 0, s, fields_dev, levels_dev,
-                       alpha_dev, g, levels, sigma, planar, zc);
+    const int nb = tgvk_blocks(g, &zc);
+    // fixed-level kernels address with 32-bit byte offsets
+    switch (g.n() <= (size_t(1) << 29) ? levels : 0) {
+    case 1: hipLaunchKernelGGL(tgvk_dual_fixed<1>, dim3(nb), dim3(KT), 0, s, fields_dev,
+                               levels_dev, alpha_dev, g, sigma, planar, zc); break;
+    case 2: hipLaunchKernelGGL(tgvk_dual_fixed<2>, dim3(nb), dim3(KT), 0, s, fields_dev,
+                               levels_dev, alpha_dev, g, sigma, planar, zc); break;
+    case 3: hipLaunchKernelGGL(tgvk_dual_fixed<3>, dim3(nb), dim3(KT), 0, s, fields_dev,
+                               levels_dev, alpha_dev, g, sigma, planar, zc); break;
+    default:
+        hipLaunchKernelGGL(tgvk_dual_kernel, dim3(nb), dim3(KT), 0, s, fields_dev, levels_dev,
+                           alpha_dev, g, levels, sigma, planar, zc);
+    }
     return hipGetLastError();
 }
 
@@ -292,9 +670,19 @@ hipError_t launch_tgvk_primal(hipStream_t s, float* const* fields_dev, float* co
                               const float* f, Geometry g, int levels, float tau, float lambda,
                               float theta) {
     int zc = 1;
-    const dim3 grid = tgvk_grid(g, &zc);
-    hipLaunchKernelGGL(tgvk_primal_kernel, grid, dim3(KTX, KTY, 1), 0, s, fields_dev, levels_dev,
-                       f, g, levels, tau, lambda, theta, zc);
+    const int nb = tgvk_blocks(g, &zc);
+    // fixed-level kernels address with 32-bit byte offsets
+    switch (g.n() <= (size_t(1) << 29) ? levels : 0) {
+    case 1: hipLaunchKernelGGL(tgvk_primal_fixed<1>, dim3(nb), dim3(KT), 0, s, fields_dev,
+                               levels_dev, f, g, tau, lambda, theta, zc); break;
+    case 2: hipLaunchKernelGGL(tgvk_primal_fixed<2>, dim3(nb), dim3(KT), 0, s, fields_dev,
+                               levels_dev, f, g, tau, lambda, theta, zc); break;
+    case 3: hipLaunchKernelGGL(tgvk_primal_fixed<3>, dim3(nb), dim3(KT), 0, s, fields_dev,
+                               levels_dev, f, g, tau, lambda, theta, zc); break;
+    default:
+        hipLaunchKernelGGL(tgvk_primal_kernel, dim3(nb), dim3(KT), 0, s, fields_dev, levels_dev, f,
+                           g, levels, tau, lambda, theta, zc);
+    }
     return hipGetLastError();
 }
 

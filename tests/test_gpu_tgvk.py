@@ -52,6 +52,21 @@ def test_tgvk_bitwise_against_oracle(ctx, orc, shape, alpha):
     np.testing.assert_array_equal(vz, st.v[2])
 
 
+@pytest.mark.parametrize("zchunk", ["1", "3", "7"])
+@pytest.mark.parametrize("alpha", ALPHAS + [(1.0, 2.0, 0.9, 0.8, 0.7, 0.6)])
+def test_tgvk_z_chunks(ctx, orc, monkeypatch, zchunk, alpha):
+    """Chunk edges and the -z register carries of the fixed-level kernels (1 to
+    3 levels) and the generic kernels (4 levels)."""
+    from ambient_image_processor_amd import native
+    _say("tgvk zchunk", zchunk, "order", len(alpha))
+    monkeypatch.setenv("AMB_TGVK_ZCHUNK", zchunk)
+    f = _f((17, 9, 70), 17)
+    u, vx, vy, vz = ctx.tgvk_deshade(f, _params(native, 15), alpha)
+    st = orc.tgvk(f, 15, alpha=alpha, dtype=np.float32, **KPRM)
+    np.testing.assert_array_equal(u, st.u)
+    np.testing.assert_array_equal(vz, st.v[2])
+
+
 def test_tgvk_many_levels(ctx, orc):
     from ambient_image_processor_amd import native
     alpha = (1.0, 2.0, 0.9, 0.8, 0.7, 0.6, 0.5, 0.4)
