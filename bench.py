@@ -51,10 +51,13 @@ SWEEPS = ("dual_sweep", "primal_sweep", "fused_sweep")
 # HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on
 # the default workload (256^3, FP32 fields, default plan), FETCH_SIZE scaled by
 # 2.0 and WRITE_SIZE by 1.0 (what the counters read for a plain 1 GiB copy);
-# summaries in profiles/.  Keyed by (kernel, storage bytes, edge length).
+# summaries in profiles/.  Keyed by (kernel, storage bytes, edge length); the
+# TGV-k entries by (kernel, levels, edge length) (k = 3, FP32 fields).
 MEASURED_TRAFFIC = {("fused_sweep", 4, 256): 1882845639,
                     ("dual_sweep", 4, 256): 1557605469,
-                    ("primal_sweep", 4, 256): 1552974764}
+                    ("primal_sweep", 4, 256): 1552974764,
+                    ("tgvk_dual", 1, 256): 2908351910,
+                    ("tgvk_primal", 1, 256): 3564456435}
 
 
 def tgvk_bytes(kind, levels, n):
@@ -448,6 +451,7 @@ def tgvk3_256(args):
                          "gbs": b / (ms / cnt * 1e-3) / 1e9}
     dom = max(kernels, key=lambda k: kernels[k]["avg_us"])
     achieved = kernels[dom]["gbs"]
+    traffic = MEASURED_TRAFFIC.get((dom, levels, n_side))
     cpu = None
     if not args.no_cpu_baseline:
         from oracle import oracle
@@ -471,8 +475,10 @@ def tgvk3_256(args):
                    "iterations": args.iterations, "lambda": 0.5, "alpha": list(alpha),
                    "parallelism": "single-gpu"},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "bytes_per_launch": kernels[dom]["bytes_per_launch"]},
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "bytes_per_launch": kernels[dom]["bytes_per_launch"],
+                     "measured_hbm_gbs": (traffic / (kernels[dom]["avg_us"] * 1e-6) / 1e9)
+                     if traffic else None},
         "kernels": rounded(kernels),
         "other_kernels_ms": {k: round(prof[k][0], 3) for k in ("dct", "poisson", "divergence",
                                                                 "epilogue", "init")},
