@@ -152,13 +152,14 @@ hipError_t launch_active_from_mask(hipStream_t s, const float* mask, uint8_t* ac
 
 // TGV-k (k >= 3) sweeps (tgvk.hip).  fields_dev: device table of the 17 state
 // fields; levels_dev: device table of 18 pointers per level (w[6], w_bar[6],
-// r[6]); alpha_dev: the k weights.  FP32 storage.
+// r[6]); alpha_dev: the k weights; active (may be null): one byte per voxel,
+// updates only where nonzero.  FP32 storage.
 hipError_t launch_tgvk_dual(hipStream_t s, float* const* fields_dev, float* const* levels_dev,
-                            const float* alpha_dev, Geometry g, int levels, float sigma,
-                            int planar);
+                            const float* alpha_dev, const uint8_t* active, Geometry g, int levels,
+                            float sigma, int planar);
 hipError_t launch_tgvk_primal(hipStream_t s, float* const* fields_dev, float* const* levels_dev,
-                              const float* f, Geometry g, int levels, float tau, float lambda,
-                              float theta);
+                              const float* f, const uint8_t* active, Geometry g, int levels,
+                              float tau, float lambda, float theta);
 
 // FP64 energy / gap (energy.hip).  partials: at least energy_partials_size(g) doubles.
 size_t energy_partials_size(Geometry g);

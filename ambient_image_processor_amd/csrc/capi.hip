@@ -625,13 +625,13 @@ amb_status tgvk_run(amb_context* c, uint32_t iterations, uint32_t interval,
     for (uint32_t it = 0; it < iterations; ++it) {
         {
             ProfScope ps(c, P_TGVK_DUAL);
-            AMB_HIP(launch_tgvk_dual(c->stream, F, c->klv_dev, c->kalpha_dev, c->g, c->klevels,
-                                     c->sp.sigma, c->sp.planar));
+            AMB_HIP(launch_tgvk_dual(c->stream, F, c->klv_dev, c->kalpha_dev, c->sp.active, c->g,
+                                     c->klevels, c->sp.sigma, c->sp.planar));
         }
         {
             ProfScope ps(c, P_TGVK_PRIMAL);
-            AMB_HIP(launch_tgvk_primal(c->stream, F, c->klv_dev, c->f, c->g, c->klevels,
-                                       c->sp.tau, c->sp.lambda, c->sp.theta));
+            AMB_HIP(launch_tgvk_primal(c->stream, F, c->klv_dev, c->f, c->sp.active, c->g,
+                                       c->klevels, c->sp.tau, c->sp.lambda, c->sp.theta));
         }
         if (cb && interval > 0 && it > 0 && it % interval == 0) {
             const size_t n = c->g.n();
@@ -648,6 +648,17 @@ amb_status tgvk_run(amb_context* c, uint32_t iterations, uint32_t interval,
         }
     }
     return AMB_OK;
+}
+
+// The TGV-k loop with updates restricted to c->act (dropped again on exit).
+amb_status tgvk_run_masked(amb_context* c, uint32_t iterations, uint32_t interval,
+                           amb_iteration_callback cb, void* user) {
+    struct Reset {
+        amb_context* c;
+        ~Reset() { c->sp.active = nullptr; }
+    } reset{c};
+    c->sp.active = c->act;
+    return tgvk_run(c, iterations, interval, cb, user);
 }
 
 amb_status check_precision(const Geometry& g, int precision) {
@@ -1147,6 +1158,26 @@ AMB_API amb_status amb_tgvk_deshade(amb_context* c, amb_memory mem, const float*
     return AMB_OK;
 }
 
+AMB_API amb_status amb_tgvk_deshade_masked(amb_context* c, amb_memory mem, const float* f,
+                                           uint32_t w, uint32_t h, uint32_t d,
+                                           const amb_tgv_params* p, uint32_t order,
+                                           const float* alpha, const uint8_t* active, float* u,
+                                           float* vx, float* vy, float* vz,
+                                           amb_iteration_callback cb, void* user) {
+    if (!c) return fail(AMB_ERR_INVALID_ARGUMENT, "ctx is NULL");
+    if (!active) return fail(AMB_ERR_INVALID_ARGUMENT, "active is NULL");
+    AMB_TRY(tgvk_begin(c, mem, f, w, h, d, p, order, alpha));
+    AMB_TRY(ensure_active(c));
+    AMB_HIP(hipMemcpyAsync(c->act, active, c->g.n(), kind_in(mem), c->stream));
+    AMB_TRY(tgvk_run_masked(c, p->iterations, p->callback_interval, cb, user));
+    AMB_TRY(copy_field_out(c, mem, u, F_U));
+    AMB_TRY(copy_field_out(c, mem, vx, F_VX));
+    AMB_TRY(copy_field_out(c, mem, vy, F_VY));
+    AMB_TRY(copy_field_out(c, mem, vz, F_VZ));
+    AMB_HIP(hipStreamSynchronize(c->stream));
+    return AMB_OK;
+}
+
 AMB_API amb_status amb_tgvk_deshade_pipeline(amb_context* c, amb_memory mem, const float* f,
                                              uint32_t w, uint32_t h, uint32_t d,
                                              const amb_tgv_params* p, uint32_t order,
@@ -1157,10 +1188,13 @@ AMB_API amb_status amb_tgvk_deshade_pipeline(amb_context* c, amb_memory mem, con
     if (!p) return fail(AMB_ERR_INVALID_ARGUMENT, "params is NULL");
     if (!alpha) return fail(AMB_ERR_INVALID_ARGUMENT, "alpha is NULL");
     if (order < 2) return fail(AMB_ERR_INVALID_ARGUMENT, "order must be at least 2");
-    // TGVKDeshadeProcessor.cpp:93-100: depth 1 takes the 2-D loops; the mask
-    // multiplies l and r (deshade_poisson_cosine_transform, :155-158)
+    // TGVKDeshadeProcessor.cpp:93-100 (and TGVKDeshadeMaskedProcessor.cpp:95-108):
+    // depth 1 takes the 2-D loops; the mask multiplies l and r
+    // (deshade_poisson_cosine_transform, :155-158)
+    const bool masked = mask && (p->flags & AMB_FLAG_MASKED_ITERATION);
     amb_tgv_params pp = *p;
-    pp.flags = (pp.flags & ~AMB_FLAG_MASKED_ITERATION) | AMB_FLAG_MASK_SHADING;
+    pp.flags |= AMB_FLAG_MASK_SHADING;
+    if (!masked) pp.flags &= ~AMB_FLAG_MASKED_ITERATION;
     if (d == 1) pp.flags |= AMB_FLAG_PLANAR;
     if (order == 2) {   // :132-143, the TGV2 launcher with alpha0 = alpha[1], alpha1 = alpha[0]
         pp.alpha0 = alpha[1];
@@ -1169,8 +1203,21 @@ AMB_API amb_status amb_tgvk_deshade_pipeline(amb_context* c, amb_memory mem, con
                                          user);
     }
     AMB_TRY(tgvk_begin(c, mem, f, w, h, d, &pp, order, alpha));
-    AMB_TRY(tgvk_run(c, pp.iterations, pp.callback_interval, cb, user));
-    return deshade_tail(c, mem, mask, pp.flags, pp.dct_precision, false, u, shading, deshaded);
+    if (masked) {
+        // active set = mask >= 1 (TGVDeshadeMaskedProcessor.cpp:348-353); f stays unmasked
+        // (TGVKDeshadeMaskedProcessor.cpp:110 clones it before the mask multiply at :118)
+        AMB_TRY(ensure_active(c));
+        const float* md = mask;
+        if (mem != AMB_MEM_DEVICE) {
+            AMB_HIP(hipMemcpyAsync(c->scratch[3], mask, c->g.n() * 4, hipMemcpyHostToDevice, c->stream));
+            md = c->scratch[3];
+        }
+        AMB_HIP(launch_active_from_mask(c->stream, md, c->act, c->g.n()));
+        AMB_TRY(tgvk_run_masked(c, pp.iterations, pp.callback_interval, cb, user));
+    } else {
+        AMB_TRY(tgvk_run(c, pp.iterations, pp.callback_interval, cb, user));
+    }
+    return deshade_tail(c, mem, mask, pp.flags, pp.dct_precision, masked, u, shading, deshaded);
 }
 
 AMB_API amb_status amb_tgv2_energy(amb_context* c, double out[10]) {

@@ -87,7 +87,8 @@ __device__ __forceinline__ float kfwd(const float* a, float center, size_t i, si
 
 __global__ __launch_bounds__(KT) void tgvk_dual_kernel(
     float* const* __restrict__ F, float* const* __restrict__ LV,
-    const float* __restrict__ alpha, Geometry g, int levels, float sigma, int planar, int zchunk) {
+    const float* __restrict__ alpha, const uint8_t* __restrict__ act, Geometry g, int levels,
+    float sigma, int planar, int zchunk) {
     const KCoord c(g, zchunk);
     if (!c.in) return;
     const int W = g.w, H = g.h, D = g.d;
@@ -100,6 +101,7 @@ __global__ __launch_bounds__(KT) void tgvk_dual_kernel(
 
     for (int z = c.z0; z < c.z1; ++z) {
         const size_t i = size_t(z) * c.plane + c.col;
+        if (act && !act[i]) continue;   // masked loop: out-of-mask voxels keep their values
         const bool nxt[3] = {c.x < W - 1, c.y < H - 1, z < D - 1};
         const bool prv[3] = {c.x > 0, c.y > 0, z > 0};
 
@@ -205,7 +207,8 @@ __global__ __launch_bounds__(KT) void tgvk_dual_kernel(
 
 __global__ __launch_bounds__(KT) void tgvk_primal_kernel(
     float* const* __restrict__ F, float* const* __restrict__ LV, const float* __restrict__ f,
-    Geometry g, int levels, float tau, float lambda, float theta, int zchunk) {
+    const uint8_t* __restrict__ act, Geometry g, int levels, float tau, float lambda, float theta,
+    int zchunk) {
     const KCoord c(g, zchunk);
     if (!c.in) return;
     const int W = g.w, H = g.h, D = g.d;
@@ -220,6 +223,7 @@ __global__ __launch_bounds__(KT) void tgvk_primal_kernel(
 
     for (int z = c.z0; z < c.z1; ++z) {
         const size_t i = size_t(z) * c.plane + c.col;
+        if (act && !act[i]) continue;
         const bool nxt[3] = {c.x < W - 1, c.y < H - 1, z < D - 1};
         const bool prv[3] = {c.x > 0, c.y > 0, z > 0};
 
@@ -297,7 +301,8 @@ __global__ __launch_bounds__(KT) void tgvk_primal_kernel(
 template <int L>
 __global__ __launch_bounds__(KT) void tgvk_dual_fixed(
     float* const* __restrict__ F, float* const* __restrict__ LV,
-    const float* __restrict__ alpha, Geometry g, float sigma, int planar, int zchunk) {
+    const float* __restrict__ alpha, const uint8_t* __restrict__ act, Geometry g, float sigma,
+    int planar, int zchunk) {
     const KCoord c(g, zchunk);
     if (!c.in) return;
     const int W = g.w, H = g.h, D = g.d;
@@ -449,15 +454,17 @@ __global__ __launch_bounds__(KT) void tgvk_dual_fixed(
             for (int k = 0; k < 6; ++k) rn[l][k] = sr ? rc[k] / tr : rc[k];
         }
 
-        // ---- stores
+        // ---- stores (masked loop: only in-mask voxels)
+        if (!act || act[i >> 2]) {
 #pragma unroll
-        for (int k = 0; k < 3; ++k) STO(pp[k], i, sp ? a3[k] / tp : a3[k]);
+            for (int k = 0; k < 3; ++k) STO(pp[k], i, sp ? a3[k] / tp : a3[k]);
 #pragma unroll
-        for (int k = 0; k < 6; ++k) STO(qq[k], i, sq ? b6[k] / tq : b6[k]);
+            for (int k = 0; k < 6; ++k) STO(qq[k], i, sq ? b6[k] / tq : b6[k]);
 #pragma unroll
-        for (int l = 0; l < L; ++l)
+            for (int l = 0; l < L; ++l)
 #pragma unroll
-            for (int k = 0; k < 6; ++k) STO(rr[l][k], i, rn[l][k]);
+                for (int k = 0; k < 6; ++k) STO(rr[l][k], i, rn[l][k]);
+        }
 
         // ---- carries
 #pragma unroll
@@ -474,7 +481,8 @@ __global__ __launch_bounds__(KT) void tgvk_dual_fixed(
 template <int L>
 __global__ __launch_bounds__(KT) void tgvk_primal_fixed(
     float* const* __restrict__ F, float* const* __restrict__ LV, const float* __restrict__ f,
-    Geometry g, float tau, float lambda, float theta, int zchunk) {
+    const uint8_t* __restrict__ act, Geometry g, float tau, float lambda, float theta,
+    int zchunk) {
     const KCoord c(g, zchunk);
     if (!c.in) return;
     const int W = g.w, H = g.h, D = g.d;
@@ -600,21 +608,23 @@ __global__ __launch_bounds__(KT) void tgvk_primal_fixed(
             }
         }
 
-        // ---- stores
-        STO(uu, i, un);
-        STO(ub, i, fmaf(theta, un - uo, un));
+        // ---- stores (masked loop: only in-mask voxels)
+        if (!act || act[i >> 2]) {
+            STO(uu, i, un);
+            STO(ub, i, fmaf(theta, un - uo, un));
 #pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            STO(vv[k], i, vn[k]);
-            STO(vb[k], i, fmaf(theta, vn[k] - v0[k], vn[k]));
-        }
-#pragma unroll
-        for (int l = 0; l < L; ++l)
-#pragma unroll
-            for (int k = 0; k < 6; ++k) {
-                STO(ww[l][k], i, wn[l][k]);
-                STO(wbb[l][k], i, fmaf(theta, wn[l][k] - w0[l][k], wn[l][k]));
+            for (int k = 0; k < 3; ++k) {
+                STO(vv[k], i, vn[k]);
+                STO(vb[k], i, fmaf(theta, vn[k] - v0[k], vn[k]));
             }
+#pragma unroll
+            for (int l = 0; l < L; ++l)
+#pragma unroll
+                for (int k = 0; k < 6; ++k) {
+                    STO(ww[l][k], i, wn[l][k]);
+                    STO(wbb[l][k], i, fmaf(theta, wn[l][k] - w0[l][k], wn[l][k]));
+                }
+        }
 
         // ---- carries
         pz_prev = p0[2];
@@ -647,41 +657,41 @@ int tgvk_blocks(const Geometry& g, int* zchunk) {
 }  // namespace
 
 hipError_t launch_tgvk_dual(hipStream_t s, float* const* fields_dev, float* const* levels_dev,
-                            const float* alpha_dev, Geometry g, int levels, float sigma,
-                            int planar) {
+                            const float* alpha_dev, const uint8_t* active, Geometry g, int levels,
+                            float sigma, int planar) {
     int zc = 1;
     const int nb = tgvk_blocks(g, &zc);
     // fixed-level kernels address with 32-bit byte offsets
     switch (g.n() <= (size_t(1) << 29) ? levels : 0) {
     case 1: hipLaunchKernelGGL(tgvk_dual_fixed<1>, dim3(nb), dim3(KT), 0, s, fields_dev,
-                               levels_dev, alpha_dev, g, sigma, planar, zc); break;
+                               levels_dev, alpha_dev, active, g, sigma, planar, zc); break;
     case 2: hipLaunchKernelGGL(tgvk_dual_fixed<2>, dim3(nb), dim3(KT), 0, s, fields_dev,
-                               levels_dev, alpha_dev, g, sigma, planar, zc); break;
+                               levels_dev, alpha_dev, active, g, sigma, planar, zc); break;
     case 3: hipLaunchKernelGGL(tgvk_dual_fixed<3>, dim3(nb), dim3(KT), 0, s, fields_dev,
-                               levels_dev, alpha_dev, g, sigma, planar, zc); break;
+                               levels_dev, alpha_dev, active, g, sigma, planar, zc); break;
     default:
         hipLaunchKernelGGL(tgvk_dual_kernel, dim3(nb), dim3(KT), 0, s, fields_dev, levels_dev,
-                           alpha_dev, g, levels, sigma, planar, zc);
+                           alpha_dev, active, g, levels, sigma, planar, zc);
     }
     return hipGetLastError();
 }
 
 hipError_t launch_tgvk_primal(hipStream_t s, float* const* fields_dev, float* const* levels_dev,
-                              const float* f, Geometry g, int levels, float tau, float lambda,
-                              float theta) {
+                              const float* f, const uint8_t* active, Geometry g, int levels,
+                              float tau, float lambda, float theta) {
     int zc = 1;
     const int nb = tgvk_blocks(g, &zc);
     // fixed-level kernels address with 32-bit byte offsets
     switch (g.n() <= (size_t(1) << 29) ? levels : 0) {
     case 1: hipLaunchKernelGGL(tgvk_primal_fixed<1>, dim3(nb), dim3(KT), 0, s, fields_dev,
-                               levels_dev, f, g, tau, lambda, theta, zc); break;
+                               levels_dev, f, active, g, tau, lambda, theta, zc); break;
     case 2: hipLaunchKernelGGL(tgvk_primal_fixed<2>, dim3(nb), dim3(KT), 0, s, fields_dev,
-                               levels_dev, f, g, tau, lambda, theta, zc); break;
+                               levels_dev, f, active, g, tau, lambda, theta, zc); break;
     case 3: hipLaunchKernelGGL(tgvk_primal_fixed<3>, dim3(nb), dim3(KT), 0, s, fields_dev,
-                               levels_dev, f, g, tau, lambda, theta, zc); break;
+                               levels_dev, f, active, g, tau, lambda, theta, zc); break;
     default:
         hipLaunchKernelGGL(tgvk_primal_kernel, dim3(nb), dim3(KT), 0, s, fields_dev, levels_dev, f,
-                           g, levels, tau, lambda, theta, zc);
+                           active, g, levels, tau, lambda, theta, zc);
     }
     return hipGetLastError();
 }

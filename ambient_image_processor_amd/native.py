@@ -42,7 +42,7 @@ EXPORTS = [
     "amb_tgv2_get_state", "amb_tgv2_iterate", "amb_tgv2_sweep", "amb_profile_enable",
     "amb_profile_read", "amb_profile_reset", "amb_tgv2_denoise_adjoint",
     "amb_tgv2_deshade_masked", "amb_deshade_epilogue", "amb_tgvk_deshade",
-    "amb_tgvk_deshade_pipeline",
+    "amb_tgvk_deshade_pipeline", "amb_tgvk_deshade_masked",
 ]
 
 
@@ -103,6 +103,8 @@ def lib():
                              fp, CALLBACK, vp],
         "amb_tgvk_deshade_pipeline": [vp, mem, fp, u32, u32, u32, POINTER(Params), u32, fp, fp,
                                       fp, fp, fp, CALLBACK, vp],
+        "amb_tgvk_deshade_masked": [vp, mem, fp, u32, u32, u32, POINTER(Params), u32, fp, vp, fp,
+                                    fp, fp, fp, CALLBACK, vp],
         "amb_tgv2_energy": [vp, POINTER(c_double)],
         "amb_tgv2_begin": [vp, mem, fp, u32, u32, u32, POINTER(Params)],
         "amb_tgv2_set_state": [vp, mem, POINTER(c_void_p)],
@@ -255,6 +257,23 @@ class Context:
         _check(lib().amb_tgvk_deshade(self._h, MEM_HOST, _ptr(f), *_dims(f.shape),
                                       ctypes.byref(params), al.size, _ptr(al),
                                       *[_ptr(o) for o in outs], cb, None))
+        del keep
+        return tuple(outs)
+
+    def tgvk_deshade_masked(self, f, params: Params, alpha, active, callback=None):
+        """amb_tgvk_deshade_masked on host arrays; active: nonzero = update
+        -> (u, vx, vy, vz)."""
+        f = _f32(f)
+        al = np.ascontiguousarray(alpha, dtype=np.float32)
+        act = np.ascontiguousarray(np.asarray(active) != 0, dtype=np.uint8)
+        if act.shape != f.shape:
+            raise ValueError("active must have the shape of f")
+        self.shape = f.shape
+        outs = [np.empty_like(f) for _ in range(4)]
+        cb, keep = self._callback(callback, f.shape, True)
+        _check(lib().amb_tgvk_deshade_masked(self._h, MEM_HOST, _ptr(f), *_dims(f.shape),
+                                             ctypes.byref(params), al.size, _ptr(al), _ptr(act),
+                                             *[_ptr(o) for o in outs], cb, None))
         del keep
         return tuple(outs)
 

@@ -158,6 +158,52 @@ def tgvk_l1_deshade_launch_2d(f_host, width, height, lambda_, iteration_count,
     return u, vx, vy
 
 
+def _active_from_indices(masked_pixel_indices, n):
+    idx = np.asarray(masked_pixel_indices, dtype=np.int64).ravel()
+    if idx.size and (idx.min() < 0 or idx.max() >= n):
+        raise IndexError("masked pixel index outside the volume")
+    active = np.zeros(n, np.uint8)
+    active[idx] = 1
+    return active
+
+
+def tgvk_l1_deshade_masked_launch(f_host, width, height, depth, lambda_, iteration_count,
+                                  paint_iteration_interval, cuda_block_dimension,
+                                  iteration_finished_callback, order, alpha,
+                                  masked_pixel_indices, *edge_pixel_indices, **kw):
+    """Reference tgvk_deshade_masked/tgvk_l1_deshade_masked.cu:29-… (declared
+    TGVKDeshadeMaskedProcessor.cpp:30-50) -> (u, v_x, v_y, v_z): the TGV-k loop
+    updating only the voxels in masked_pixel_indices; the twelve edge lists are
+    accepted for signature parity."""
+    del cuda_block_dimension, edge_pixel_indices
+    if len(alpha) != order:
+        raise ValueError("alpha needs one weight per order")
+    f = _vol(f_host, width, height, depth)
+    active = _active_from_indices(masked_pixel_indices, f.size)
+    p = _params(lambda_, iteration_count, paint_iteration_interval, 1.0, 1.0, native.DATA_L1,
+                **kw)
+    return _ctx().tgvk_deshade_masked(f, p, alpha, active.reshape(f.shape),
+                                      iteration_finished_callback)
+
+
+def tgvk_l1_deshade_masked_2d_launch(f_host, width, height, lambda_, iteration_count,
+                                     paint_iteration_interval, cuda_block_dimension,
+                                     iteration_finished_callback, order, alpha,
+                                     masked_pixel_indices, *edge_pixel_indices, **kw):
+    """Reference tgvk_deshade_masked/tgvk_l1_deshade_masked_2d.cu (declared
+    TGVKDeshadeMaskedProcessor.cpp:53-68) -> (u, v_x, v_y)."""
+    del cuda_block_dimension, edge_pixel_indices
+    if len(alpha) != order:
+        raise ValueError("alpha needs one weight per order")
+    f = _vol(f_host, width, height, 1)
+    active = _active_from_indices(masked_pixel_indices, f.size)
+    p = _planar_params(_params(lambda_, iteration_count, paint_iteration_interval, 1.0, 1.0,
+                               native.DATA_L1, **kw))
+    u, vx, vy, _ = _ctx().tgvk_deshade_masked(f, p, alpha, active.reshape(f.shape),
+                                              _drop_vz(iteration_finished_callback))
+    return u, vx, vy
+
+
 def _denoise(data_term, f_host, width, height, depth, lambda_, iteration_count,
              paint_iteration_interval, iteration_finished_callback, alpha0, alpha1, **kw):
     f = _vol(f_host, width, height, depth)
@@ -353,6 +399,49 @@ class TGVKDeshadeProcessor:
         1, 4, 7, ... for alpha[0 .. order-1]."""
         for i in range(order):
             updateElement(1 + 3 * i)
+
+
+class TGVKDeshadeMaskedProcessor:
+    """Reference tgvk_deshade_masked/TGVKDeshadeMaskedProcessor.{h,cpp}."""
+
+    @staticmethod
+    def processTGVKL1Cuda(input_image, lambda_, order, alpha, iteration_count,
+                          cuda_block_dimension=-1, mask=None, set_negative_values_to_zero=True,
+                          add_background_back=False, paint_iteration_interval=0,
+                          iteration_finished_callback=None, calculate_div_v=False, **kw):
+        """TGVKDeshadeMaskedProcessor.cpp:76-226 -> (denoised_image, shading_image,
+        deshaded_image, div_v_image or None).  Order 2 is the masked TGV2
+        processor (alpha0 = alpha[1], alpha1 = alpha[0]); order >= 3 the masked
+        TGV-k loop on the unmasked input (2-D at depth 1), updates where
+        mask >= 1; l and r times the mask, clamp, then the add-back of
+        u*(1 - mask) or r times the mask again.  No mask: all ones.  The
+        callback receives (iteration_index, iteration_count, u, l, r)."""
+        del cuda_block_dimension
+        if len(alpha) != order:
+            raise ValueError("alpha needs one weight per order")
+        f = np.ascontiguousarray(input_image, dtype=np.float32)
+        m = (np.ones_like(f) if mask is None
+             else np.ascontiguousarray(mask, dtype=np.float32).reshape(f.shape))
+        flags = native.FLAG_MASKED_ITERATION | native.FLAG_MASK_SHADING
+        if set_negative_values_to_zero:
+            flags |= native.FLAG_SET_NEGATIVE_ZERO
+        if add_background_back:
+            flags |= native.FLAG_ADD_BACKGROUND_BACK
+        p = _params(lambda_, iteration_count, paint_iteration_interval, 1.0, 1.0, native.DATA_L1,
+                    flags=flags, **kw)
+        cb = None
+        if iteration_finished_callback is not None:
+            def cb(it, count, u, vx, vy, vz):
+                aux = _aux_ctx()
+                l = aux.integrate_gradients_poisson(vx, vy, vz, p.dct_precision)
+                r, l = aux.deshade_epilogue(u, l, m, flags)
+                return iteration_finished_callback(it, count, u, l, r)
+        u, l, r = _ctx().tgvk_deshade_pipeline(f, p, alpha, m, cb)
+        div_v = None
+        if calculate_div_v:
+            st = _ctx().get_state()
+            div_v = _ctx().divergence(st[2], st[3], st[4])
+        return u, l, r, div_v
 
 
 def _aux_ctx() -> native.Context:

@@ -281,12 +281,27 @@ AMB_API amb_status amb_tgvk_deshade(amb_context* ctx, amb_memory mem, const floa
                                     const float* alpha, float* u, float* v_x, float* v_y,
                                     float* v_z, amb_iteration_callback callback, void* user);
 
+/* Replaces tgvk_l1_deshade_masked_launch<float>(..., masked_pixel_indices, twelve edge
+ * lists) (reference tgvk_deshade_masked/tgvk_l1_deshade_masked.cu:29-…, declared
+ * TGVKDeshadeMaskedProcessor.cpp:27-50): the TGV-k loop with every update only where
+ * active[i] != 0 (one byte per voxel); the stencils are the unmasked ones. */
+AMB_API amb_status amb_tgvk_deshade_masked(amb_context* ctx, amb_memory mem, const float* f,
+                                           uint32_t width, uint32_t height, uint32_t depth,
+                                           const amb_tgv_params* params, uint32_t order,
+                                           const float* alpha, const uint8_t* active, float* u,
+                                           float* v_x, float* v_y, float* v_z,
+                                           amb_iteration_callback callback, void* user);
+
 /* TGVKDeshadeProcessor::processTGVKL1Cuda (TGVKDeshadeProcessor.cpp:76-174):
  * order 2 runs the TGV2 loop with alpha0 = alpha[1], alpha1 = alpha[0], order >= 3
  * the TGV-k loop, depth 1 the 2-D loops; then l from the Poisson integration of
  * v, r = u - l, the mask (may be NULL) multiplying l and r, the clamp
  * (AMB_FLAG_SET_NEGATIVE_ZERO) and the add-back of u*(1 - mask)
- * (AMB_FLAG_ADD_BACKGROUND_BACK).  params->dct_precision selects the DCT. */
+ * (AMB_FLAG_ADD_BACKGROUND_BACK).  params->dct_precision selects the DCT.
+ * With a mask and AMB_FLAG_MASKED_ITERATION: TGVKDeshadeMaskedProcessor::processTGVKL1Cuda
+ * (TGVKDeshadeMaskedProcessor.cpp:76-226) - order 2 the masked TGV2 composition, else the
+ * masked TGV-k loop on the unmasked f (updates where mask >= 1), then l and r times the mask,
+ * clamp, and the add-back or r times the mask again. */
 AMB_API amb_status amb_tgvk_deshade_pipeline(amb_context* ctx, amb_memory mem, const float* f,
                                              uint32_t width, uint32_t height, uint32_t depth,
                                              const amb_tgv_params* params, uint32_t order,

@@ -317,6 +317,106 @@ Pixel* tgvk_l1_deshade_launch_2d(Pixel* f_host, ambient_amd_compat::uint width,
     return u;
 }
 
+namespace ambient_amd_compat {
+// active set from the reference's index list of in-mask voxels
+inline std::vector<uint8_t> active_from_indices(const std::vector<uint>& indices, size_t n) {
+    std::vector<uint8_t> active(n, 0);
+    for (uint i : indices) {
+        if (size_t(i) >= n) {
+            std::printf("ambient_amd error: masked pixel index %u outside the volume\n", i);
+            std::exit(int(AMB_ERR_INVALID_ARGUMENT));
+        }
+        active[i] = 1;
+    }
+    return active;
+}
+}  // namespace ambient_amd_compat
+
+// ---- reference tgvk_deshade_masked/tgvk_l1_deshade_masked.cu:29-… (declared
+// TGVKDeshadeMaskedProcessor.cpp:30-50) --------------------------------------
+// The twelve edge lists are accepted and not read (the sweeps know the faces).
+template <typename Pixel>
+Pixel* tgvk_l1_deshade_masked_launch(
+    Pixel* f_host, const ambient_amd_compat::uint width, const ambient_amd_compat::uint height,
+    const ambient_amd_compat::uint depth, const Pixel lambda,
+    const ambient_amd_compat::uint iteration_count,
+    const ambient_amd_compat::uint paint_iteration_interval, const int cuda_block_dimension,
+    std::function<bool(ambient_amd_compat::uint, ambient_amd_compat::uint, Pixel*, Pixel*, Pixel*,
+                       Pixel*)>
+        iteration_finished_callback,
+    const ambient_amd_compat::uint order, const Pixel* alpha, Pixel** v_x_host, Pixel** v_y_host,
+    Pixel** v_z_host, const std::vector<ambient_amd_compat::uint>& masked_pixel_indices,
+    const std::vector<ambient_amd_compat::uint>& /*left*/,
+    const std::vector<ambient_amd_compat::uint>& /*not_left*/,
+    const std::vector<ambient_amd_compat::uint>& /*right*/,
+    const std::vector<ambient_amd_compat::uint>& /*not_right*/,
+    const std::vector<ambient_amd_compat::uint>& /*top*/,
+    const std::vector<ambient_amd_compat::uint>& /*not_top*/,
+    const std::vector<ambient_amd_compat::uint>& /*bottom*/,
+    const std::vector<ambient_amd_compat::uint>& /*not_bottom*/,
+    const std::vector<ambient_amd_compat::uint>& /*front*/,
+    const std::vector<ambient_amd_compat::uint>& /*not_front*/,
+    const std::vector<ambient_amd_compat::uint>& /*back*/,
+    const std::vector<ambient_amd_compat::uint>& /*not_back*/) {
+    using namespace ambient_amd_compat;
+    static_assert(std::is_same<Pixel, float>::value, "only the float instantiation exists");
+    (void)cuda_block_dimension;
+    typedef std::function<bool(uint, uint, Pixel*, Pixel*, Pixel*, Pixel*)> Fn;
+    const size_t n = size_t(width) * height * depth;
+    const std::vector<uint8_t> active = active_from_indices(masked_pixel_indices, n);
+    amb_tgv_params p = reference_params(lambda, iteration_count, paint_iteration_interval,
+                                        alpha[1], alpha[0], AMB_DATA_L1);
+    Pixel* u = new Pixel[n];
+    *v_x_host = new Pixel[n];
+    *v_y_host = new Pixel[n];
+    *v_z_host = new Pixel[n];
+    amb_iteration_callback cb = iteration_finished_callback ? &Trampoline<Fn>::deshade : nullptr;
+    check(amb_tgvk_deshade_masked(context(), AMB_MEM_HOST, f_host, width, height, depth, &p,
+                                  order, alpha, active.data(), u, *v_x_host, *v_y_host, *v_z_host,
+                                  cb, &iteration_finished_callback),
+          "tgvk_l1_deshade_masked_launch");
+    return u;
+}
+
+// ---- reference tgvk_deshade_masked/tgvk_l1_deshade_masked_2d.cu (declared
+// TGVKDeshadeMaskedProcessor.cpp:53-68) --------------------------------------
+template <typename Pixel>
+Pixel* tgvk_l1_deshade_masked_2d_launch(
+    Pixel* f_host, const ambient_amd_compat::uint width, const ambient_amd_compat::uint height,
+    const Pixel lambda, const ambient_amd_compat::uint iteration_count,
+    const ambient_amd_compat::uint paint_iteration_interval, const int cuda_block_dimension,
+    std::function<bool(ambient_amd_compat::uint, ambient_amd_compat::uint, Pixel*, Pixel*, Pixel*)>
+        iteration_finished_callback,
+    const ambient_amd_compat::uint order, const Pixel* alpha, Pixel** v_x_host, Pixel** v_y_host,
+    const std::vector<ambient_amd_compat::uint>& masked_pixel_indices,
+    const std::vector<ambient_amd_compat::uint>& /*left*/,
+    const std::vector<ambient_amd_compat::uint>& /*not_left*/,
+    const std::vector<ambient_amd_compat::uint>& /*right*/,
+    const std::vector<ambient_amd_compat::uint>& /*not_right*/,
+    const std::vector<ambient_amd_compat::uint>& /*top*/,
+    const std::vector<ambient_amd_compat::uint>& /*not_top*/,
+    const std::vector<ambient_amd_compat::uint>& /*bottom*/,
+    const std::vector<ambient_amd_compat::uint>& /*not_bottom*/) {
+    using namespace ambient_amd_compat;
+    static_assert(std::is_same<Pixel, float>::value, "only the float instantiation exists");
+    (void)cuda_block_dimension;
+    typedef std::function<bool(uint, uint, Pixel*, Pixel*, Pixel*)> Fn;
+    const size_t n = size_t(width) * height;
+    const std::vector<uint8_t> active = active_from_indices(masked_pixel_indices, n);
+    amb_tgv_params p = reference_params(lambda, iteration_count, paint_iteration_interval,
+                                        alpha[1], alpha[0], AMB_DATA_L1);
+    p.flags |= AMB_FLAG_PLANAR;
+    Pixel* u = new Pixel[n];
+    *v_x_host = new Pixel[n];
+    *v_y_host = new Pixel[n];
+    amb_iteration_callback cb = iteration_finished_callback ? &Trampoline<Fn>::deshade_2d : nullptr;
+    check(amb_tgvk_deshade_masked(context(), AMB_MEM_HOST, f_host, width, height, 1, &p, order,
+                                  alpha, active.data(), u, *v_x_host, *v_y_host, nullptr, cb,
+                                  &iteration_finished_callback),
+          "tgvk_l1_deshade_masked_2d_launch");
+    return u;
+}
+
 // ---- reference tgv_cuda/tgv2_l1.cu:64-71 and tgv_cuda/tgv2_l2.cu:60-67 -------
 template <typename Pixel>
 Pixel* tgv2_denoise_launch_impl(uint32_t data_term, Pixel* f_host, ambient_amd_compat::uint width,

@@ -216,17 +216,21 @@ class TGVKState:
         b = N_FIELDS + 18 * i
         return self.fields[b:b + 6], self.fields[b + 6:b + 12], self.fields[b + 12:b + 18]
 
-    def iterate(self, iterations, lam, alpha, tau, sigma, theta=1.0, planar=False):
-        """alpha: order weights; alpha[0] bounds p, alpha[1] q, alpha[i + 2] r_i."""
+    def iterate(self, iterations, lam, alpha, tau, sigma, theta=1.0, planar=False, mask=None):
+        """alpha: order weights; alpha[0] bounds p, alpha[1] q, alpha[i + 2] r_i.
+        mask (optional): updates only where mask >= 1 (the masked TGV-k loop)."""
         alpha = np.ascontiguousarray(alpha, dtype=np.float64)
         if alpha.size != self.order:
             raise ValueError("alpha needs one weight per order")
         if planar and self.f.shape[0] != 1:
             raise ValueError("the planar (2-D) loop needs depth 1")
         prm = (ctypes.c_double * 4)(lam, tau, sigma, theta)
+        act = None
+        if mask is not None:
+            act = np.ascontiguousarray(np.asarray(mask) >= 1, dtype=np.uint8)
         rc = getattr(lib(), "orc_tgvk_iterate_" + _suf(self.dtype))(
             self._pp, self.order, _p(self.f), *_dims(self.f), prm, _p(alpha), int(iterations),
-            int(bool(planar)))
+            int(bool(planar)), _p(act) if act is not None else None)
         assert rc == iterations
         return self
 
@@ -243,10 +247,12 @@ def tensor_op(t, forward, halve):
     return out
 
 
-def tgvk(f, iterations, lam, alpha, tau, sigma, theta=1.0, dtype=np.float64, planar=False):
-    """TGV-k (k = len(alpha) >= 3) deshade loop from the reference initialisation."""
+def tgvk(f, iterations, lam, alpha, tau, sigma, theta=1.0, dtype=np.float64, planar=False,
+         mask=None):
+    """TGV-k (k = len(alpha) >= 3) deshade loop from the reference initialisation;
+    with a mask the masked loop (updates only where mask >= 1)."""
     st = TGVKState(f, len(alpha), dtype)
-    st.iterate(iterations, lam, alpha, tau, sigma, theta, planar)
+    st.iterate(iterations, lam, alpha, tau, sigma, theta, planar, mask)
     return st
 
 
@@ -265,6 +271,27 @@ def deshade_k(f, iterations, lam, alpha, tau, sigma, theta=1.0, mask=None,
         st = tgvk(f, iterations, lam, alpha, tau, sigma, theta, dtype, planar=planar)
     l = integrate_poisson(st.v[0], st.v[1], st.v[2], False, dtype)
     r, l2 = deshade_epilogue(st.u, l, mask, True, set_negative_zero, add_background_back)
+    return st.u.copy(), l2, r, st
+
+
+def deshade_k_masked(f, iterations, lam, alpha, tau, sigma, mask=None, theta=1.0,
+                     set_negative_zero=True, add_background_back=False, dtype=np.float32):
+    """TGVKDeshadeMaskedProcessor::processTGVKL1Cuda (TGVKDeshadeMaskedProcessor.cpp:76-226):
+    order 2 is the masked TGV2 composition (deshade_masked with alpha0 = alpha[1],
+    alpha1 = alpha[0]); order >= 3 the masked TGV-k loop on the unmasked f (2-D at
+    depth 1), the Poisson integration, l and r times the mask, clamp, then the
+    add-back of u*(1 - mask) or r times the mask again.  No mask: all ones.
+    Returns (u, l, r, state)."""
+    f = np.asarray(f)
+    m = np.ones(f.shape, np.float32) if mask is None else np.asarray(mask, np.float32)
+    if len(alpha) == 2:
+        return deshade_masked(f, iterations, lam, alpha[1], alpha[0], tau, sigma, m, theta,
+                              set_negative_zero, add_background_back, dtype=dtype)
+    st = tgvk(f, iterations, lam, alpha, tau, sigma, theta, dtype, planar=f.shape[0] == 1,
+              mask=m)
+    l = integrate_poisson(st.v[0], st.v[1], st.v[2], False, dtype)
+    r, l2 = deshade_epilogue(st.u, l, m, True, set_negative_zero, add_background_back,
+                             mask_again=True)
     return st.u.copy(), l2, r, st
 
 

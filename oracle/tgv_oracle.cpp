@@ -299,6 +299,8 @@ void tgv2_iteration(State<T>& st, const T* f, const Dims& s, const Params& prm,
 //                                       i even: D+, tgvk_common.cu:138-182)
 //   w_i = w_i - tau*(r2 - q~), q~ = q (i = 0) or r_{i-1} (updated this iteration);
 //   w_bar_i = w_i + theta*(w_i - w_i_old)            (tgv3_common.cu:362-405)
+// active (may be null): the masked loop, tgvk_deshade_masked/tgvk_l1_deshade_masked.cu:204-520 -
+// every update only at voxels with active != 0, the stencils unmasked (as in tgv2_iteration).
 // G: diagonal component a differenced along a, off-diagonal (a, b) as
 // (d_a + d_b) * 0.5; H: diagonal d_a, off-diagonal d_a + d_b.
 // ---------------------------------------------------------------------------
@@ -326,11 +328,12 @@ void tensor_op(T* const* in, T* const* out, const Dims& s, bool forward, bool ha
 template <typename T>
 void tgvk_iteration(State<T>& st, std::vector<Level<T>>& lv, const T* f, const Dims& s,
                     const Params& prm, const std::vector<double>& alpha,
-                    std::vector<T>* tmp /* 9 */, std::vector<T>* r2buf /* 6 */) {
+                    std::vector<T>* tmp /* 9 */, std::vector<T>* r2buf /* 6 */,
+                    const unsigned char* active) {
     const i64 n = s.n();
     const int L = int(lv.size());
     const T tau = T(prm.tau), sigma = T(prm.sigma), theta = T(prm.theta);
-    tgv2_iteration(st, f, s, prm, tmp, nullptr, lv[0].wbar);
+    tgv2_iteration(st, f, s, prm, tmp, active, lv[0].wbar);
     T* r2[6];
     for (int c = 0; c < 6; ++c) r2[c] = r2buf[c].data();
     T* t = tmp[7].data();
@@ -345,6 +348,7 @@ void tgvk_iteration(State<T>& st, std::vector<Level<T>>& lv, const T* f, const D
         T* const* wnext = last ? nullptr : lv[li + 1].wbar;
 #pragma omp parallel for schedule(static)
         for (i64 i = 0; i < n; ++i) {
+            if (active && !active[i]) continue;
             T rc[6];
             for (int c = 0; c < 6; ++c)
                 rc[c] = wnext ? fma_t(sigma, r2[c][i] - wnext[c][i], l.r[c][i])
@@ -362,6 +366,7 @@ void tgvk_iteration(State<T>& st, std::vector<Level<T>>& lv, const T* f, const D
         T* const* qt = li == 0 ? st.q : lv[li - 1].r;
 #pragma omp parallel for schedule(static)
         for (i64 i = 0; i < n; ++i) {
+            if (active && !active[i]) continue;
             for (int c = 0; c < 6; ++c) {
                 const T wn = fma_t(-tau, r2[c][i] - qt[c][i], l.w[c][i]);
                 l.w[c][i] = wn;
@@ -644,7 +649,8 @@ ORC_DEFINE_TYPED(double, f64)
 // prm4 = (lambda, tau, sigma, theta); alpha[order].
 #define ORC_DEFINE_TGVK(T, SUF)                                                               \
 ORC_EXPORT int orc_tgvk_iterate_##SUF(T** fields, int order, const T* f, i64 w, i64 h, i64 d, \
-        const double* prm4, const double* alpha, int iterations, int planar) {                  \
+        const double* prm4, const double* alpha, int iterations, int planar,                    \
+        const unsigned char* active) {                                                          \
     if (order < 3) return -1;                                                                   \
     State<T> st; fill_state_pointers(st, fields);                                               \
     const int L = order - 2;                                                                    \
@@ -661,7 +667,7 @@ ORC_EXPORT int orc_tgvk_iterate_##SUF(T** fields, int order, const T* f, i64 w, 
     std::vector<T> tmp[9], r2[6];                                                               \
     for (auto& t : tmp) t.assign(size_t(s.n()), T(0));                                          \
     for (auto& t : r2) t.assign(size_t(s.n()), T(0));                                           \
-    for (int it = 0; it < iterations; ++it) tgvk_iteration(st, lv, f, s, prm, al, tmp, r2);         \
+    for (int it = 0; it < iterations; ++it) tgvk_iteration(st, lv, f, s, prm, al, tmp, r2, active); \
     return iterations; }
 
 ORC_DEFINE_TGVK(float, f32)

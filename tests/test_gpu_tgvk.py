@@ -245,3 +245,78 @@ def test_reference_tgvk_launcher_caller(orc, tmp_path, two_d):
     np.testing.assert_array_equal(u, uo)
     np.testing.assert_array_equal(l, lo)
     np.testing.assert_array_equal(rr, ro)
+
+
+# ---------------------------------------------------------------------------
+# masked TGV-k (tgvk_deshade_masked/)
+# ---------------------------------------------------------------------------
+def _mask(shape, seed):
+    rng = np.random.default_rng(seed)
+    return np.where(rng.random(shape) > 0.35, 1.0, rng.random(shape) * 0.999).astype(np.float32)
+
+
+@pytest.mark.parametrize("alpha", ALPHAS + [(1.0, 2.0, 0.9, 0.8, 0.7, 0.6)])
+@pytest.mark.parametrize("zchunk", ["0", "4"])
+def test_masked_tgvk_bitwise(ctx, orc, monkeypatch, alpha, zchunk):
+    from ambient_image_processor_amd import native
+    _say("masked tgvk order", len(alpha), "zchunk", zchunk)
+    if zchunk != "0":
+        monkeypatch.setenv("AMB_TGVK_ZCHUNK", zchunk)
+    shape = (11, 13, 68)
+    f = _f(shape, 18)
+    m = _mask(shape, 19)
+    u, vx, vy, vz = ctx.tgvk_deshade_masked(f, _params(native, 20), alpha, m >= 1)
+    st = orc.tgvk(f, 20, alpha=alpha, dtype=np.float32, mask=m, **KPRM)
+    np.testing.assert_array_equal(u, st.u)
+    np.testing.assert_array_equal(vx, st.v[0])
+    np.testing.assert_array_equal(vz, st.v[2])
+    out = m < 1
+    np.testing.assert_array_equal(u[out], f[out])
+
+
+@pytest.mark.parametrize("order,add_back", [(3, False), (3, True), (4, True), (2, False)])
+def test_masked_tgvk_processor(orc, order, add_back):
+    """TGVKDeshadeMaskedProcessor.processTGVKL1Cuda against the oracle's
+    composition (order 2: the masked TGV2 processor with swapped weights)."""
+    from ambient_image_processor_amd.processor import TGVKDeshadeMaskedProcessor
+    shape = (9, 14, 24)
+    f = _f(shape, 20)
+    m = _mask(shape, 21)
+    alpha = (1.0, 2.0, 0.6, 0.4)[:order]
+    u, l, r, div_v = TGVKDeshadeMaskedProcessor.processTGVKL1Cuda(
+        f, KPRM["lam"], order, alpha, 25, mask=m, add_background_back=add_back,
+        calculate_div_v=True)
+    uo, lo, ro, st = orc.deshade_k_masked(f, 25, alpha=alpha, mask=m,
+                                          add_background_back=add_back, **KPRM)
+    np.testing.assert_array_equal(u, uo)
+    assert rel_err(l, lo) < 1e-5 and rel_err(r, ro) < 1e-5
+    np.testing.assert_array_equal(div_v, orc.divergence(*st.v, dtype=np.float32))
+
+
+def test_masked_tgvk_depth1_and_launchers(orc):
+    from ambient_image_processor_amd import processor
+    from ambient_image_processor_amd.processor import (TGVDeshadeMaskedProcessor,
+                                                       TGVKDeshadeMaskedProcessor)
+    shape = (1, 26, 30)
+    f = _f(shape, 22)
+    m = _mask(shape, 23)
+    alpha = (1.0, 2.0, 0.6)
+    u, l, r, _ = TGVKDeshadeMaskedProcessor.processTGVKL1Cuda(f, KPRM["lam"], 3, alpha, 25,
+                                                              mask=m)
+    uo, lo, ro, _ = orc.deshade_k_masked(f, 25, alpha=alpha, mask=m, **KPRM)
+    np.testing.assert_array_equal(u, uo)
+    assert rel_err(l, lo) < 3e-5 and rel_err(r, ro) < 3e-5
+    lists = TGVDeshadeMaskedProcessor.buildMaskIndices2D(m)
+    u2, vx2, vy2 = processor.tgvk_l1_deshade_masked_2d_launch(
+        f.ravel(), 30, 26, KPRM["lam"], 25, 0, -1, None, 3, alpha, *lists)
+    st = orc.tgvk(f, 25, alpha=alpha, dtype=np.float32, planar=True, mask=m, **KPRM)
+    np.testing.assert_array_equal(u2, st.u)
+    np.testing.assert_array_equal(vy2, st.v[1])
+    f3 = _f((5, 8, 20), 24)
+    m3 = _mask(f3.shape, 25)
+    lists3 = TGVDeshadeMaskedProcessor.buildMaskIndices(m3)
+    u3, vx3, vy3, vz3 = processor.tgvk_l1_deshade_masked_launch(
+        f3.ravel(), 20, 8, 5, KPRM["lam"], 25, 0, -1, None, 3, alpha, *lists3)
+    st3 = orc.tgvk(f3, 25, alpha=alpha, dtype=np.float32, mask=m3, **KPRM)
+    np.testing.assert_array_equal(u3, st3.u)
+    np.testing.assert_array_equal(vz3, st3.v[2])

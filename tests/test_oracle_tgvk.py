@@ -67,12 +67,19 @@ def test_level_operators_match_dense(orc, forward, halve):
 # ---------------------------------------------------------------------------
 # the loop against an independent restatement
 # ---------------------------------------------------------------------------
-def _numpy_tgvk(f, iters, lam, alpha, tau, sigma, theta):
+def _numpy_tgvk(f, iters, lam, alpha, tau, sigma, theta, mask=None):
     """tgvk_l1_deshade.cu:152-351 written with numpy slicing: the TGV2 part
     (p with alpha[0], u L1 prox, q with alpha[1] shifted by w_bar_0, v), then
     per level i: r_i from G(w_bar_i) (B for even i, D+ for odd i) minus
     w_bar_{i+1} (not on the last level), projection with alpha[i+2], w_i from
-    H(r_i) (D+ for even i, B for odd i) minus q~ (q, or r_{i-1})."""
+    H(r_i) (D+ for even i, B for odd i) minus q~ (q, or r_{i-1}).  With a mask
+    (tgvk_l1_deshade_masked.cu) every update keeps its old value where the
+    mask is below 1."""
+    act = np.ones(f.shape, bool) if mask is None else np.asarray(mask) >= 1
+
+    def upd(new, old):
+        return np.where(act, new, old)
+
     X, Y, Z = 2, 1, 0
     L = len(alpha) - 2
     u = f.copy(); ub = f.copy()
@@ -97,35 +104,36 @@ def _numpy_tgvk(f, iters, lam, alpha, tau, sigma, theta):
 
     for _ in range(iters):
         u_old, v_old = u.copy(), v.copy()
-        p = p + sigma * (np.stack([_dplus(ub, X), _dplus(ub, Y), _dplus(ub, Z)]) - vb)
-        p = p / np.maximum(1, np.sqrt((p ** 2).sum(0)) / alpha[0])
-        u = u - tau * (_bminus(p[0], X) + _bminus(p[1], Y) + _bminus(p[2], Z))
-        d = u - f
-        u = np.where(d > tau * lam, u - tau * lam, np.where(d < -tau * lam, u + tau * lam, f))
-        ub = u + theta * (u - u_old)
+        pn = p + sigma * (np.stack([_dplus(ub, X), _dplus(ub, Y), _dplus(ub, Z)]) - vb)
+        p = upd(pn / np.maximum(1, np.sqrt((pn ** 2).sum(0)) / alpha[0]), p)
+        un = u - tau * (_bminus(p[0], X) + _bminus(p[1], Y) + _bminus(p[2], Z))
+        d = un - f
+        un = np.where(d > tau * lam, un - tau * lam, np.where(d < -tau * lam, un + tau * lam, f))
+        u = upd(un, u)
+        ub = upd(u + theta * (u - u_old), ub)
         e = np.stack([_bminus(vb[0], X), _bminus(vb[1], Y), _bminus(vb[2], Z),
                       0.5 * (_bminus(vb[1], X) + _bminus(vb[0], Y)),
                       0.5 * (_bminus(vb[2], X) + _bminus(vb[0], Z)),
                       0.5 * (_bminus(vb[2], Y) + _bminus(vb[1], Z))])
-        q = q + sigma * (e - wb[0])
-        q = q / np.maximum(1, wnorm(q) / alpha[1])
+        qn = q + sigma * (e - wb[0])
+        q = upd(qn / np.maximum(1, wnorm(qn) / alpha[1]), q)
         div = np.stack([_dplus(q[0], X) + _dplus(q[3], Y) + _dplus(q[4], Z),
                         _dplus(q[1], Y) + _dplus(q[3], X) + _dplus(q[5], Z),
                         _dplus(q[2], Z) + _dplus(q[4], X) + _dplus(q[5], Y)])
-        v = v - tau * (div - p)
-        vb = v + theta * (v - v_old)
+        v = upd(v - tau * (div - p), v)
+        vb = upd(v + theta * (v - v_old), vb)
         wb_old = wb.copy()
         for i in range(L):
             odd = i % 2 == 1
             g = G(wb_old[i], fwd=odd)
             if i < L - 1:
                 g = g - wb_old[i + 1]
-            r[i] = r[i] + sigma * g
-            r[i] = r[i] / np.maximum(1, wnorm(r[i]) / alpha[i + 2])
+            rn = r[i] + sigma * g
+            r[i] = upd(rn / np.maximum(1, wnorm(rn) / alpha[i + 2]), r[i])
             qt = q if i == 0 else r[i - 1]
             w_old = w[i].copy()
-            w[i] = w[i] - tau * (H(r[i], fwd=not odd) - qt)
-            wb[i] = w[i] + theta * (w[i] - w_old)
+            w[i] = upd(w[i] - tau * (H(r[i], fwd=not odd) - qt), w[i])
+            wb[i] = upd(w[i] + theta * (w[i] - w_old), wb[i])
     return u, v, p, q, w, wb, r
 
 
@@ -146,6 +154,28 @@ def test_tgvk_matches_vectorised_restatement(orc, alpha):
         assert rel_err(sr, r[i]) < 1e-12, i
     # the levels move: the check is not vacuous
     assert np.abs(st.level(len(alpha) - 3)[2]).max() > 1e-3
+
+
+@pytest.mark.parametrize("alpha", [(1.0, 2.0, 0.6), (0.8, 1.5, 0.7, 0.4, 0.3)])
+def test_masked_tgvk_matches_vectorised_restatement(orc, alpha):
+    """The masked loop: updates only where mask >= 1 (fractional values below 1
+    count as outside), stencils read the unmasked neighbours' initial values."""
+    f = _phantom((6, 7, 9), 8)
+    rng = np.random.default_rng(9)
+    m = np.where(rng.random(f.shape) > 0.35, 1.0, rng.random(f.shape) * 0.999)
+    st = orc.tgvk(f, 12, alpha=alpha, mask=m, **KPRM)
+    u, v, p, q, w, wb, r = _numpy_tgvk(f, 12, alpha=alpha, mask=m, **KPRM)
+    assert rel_err(st.u, u) < 1e-13
+    assert rel_err(st.v, v) < 1e-12
+    assert rel_err(st.q, q) < 1e-12
+    for i in range(len(alpha) - 2):
+        assert rel_err(st.level(i)[0], w[i]) < 1e-12, i
+        assert rel_err(st.level(i)[2], r[i]) < 1e-12, i
+    out = m < 1
+    np.testing.assert_array_equal(st.u[out], f[out])
+    assert not np.any(st.v[:, out]) and not np.any(st.level(0)[2][:, out])
+    unmasked = orc.tgvk(f, 12, alpha=alpha, **KPRM)
+    assert rel_err(st.u, unmasked.u) > 1e-4
 
 
 def test_tgvk_planar_matches_3d_restatement_without_z(orc):
