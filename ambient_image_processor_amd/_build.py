@@ -21,7 +21,7 @@ BUILD_DIR = os.path.join(ROOT, "_build")
 LIB_NAME = "libambient_amd.so"
 LIB_PATH = os.path.join(PKG, LIB_NAME)
 SOURCES = ["tgv_sweeps.hip", "tgvk.hip", "dct.hip", "dct_mfma.hip", "energy.hip", "adjoint.hip",
-           "capi.hip"]
+           "resize.hip", "capi.hip"]
 ARCH = "gfx950"
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
@@ -81,17 +81,22 @@ def _build_compat_demo() -> str:
 
 
 CLI = os.path.join(BUILD_DIR, "tgv2_deshade_application")
+CLI_DOWNSAMPLED = os.path.join(BUILD_DIR, "tgvk_deshade_downsampled_application")
 METAIMAGE_CONVERT = os.path.join(BUILD_DIR, "metaimage_convert")
 
 
 def _build_cli() -> None:
-    """The tgv2_deshade_application CLI (reference argument contract, MetaImage
-    files) and the MetaImage converter, both plain C++ (g++, zlib)."""
+    """The tgv2_deshade_application and tgvk_deshade_downsampled_application CLIs
+    (reference argument contracts, MetaImage files) and the MetaImage converter,
+    plain C++ (g++, zlib)."""
     cli = os.path.join(PKG, "cli")
     common = ["g++", "-O2", "-std=c++17", "-I" + os.path.join(ROOT, "include"), "-I" + cli]
     jobs = [
         common + [os.path.join(cli, "tgv2_deshade_main.cpp"), "-o", CLI, "-L" + PKG,
                   "-lambient_amd", "-lz", "-Wl,-rpath," + PKG,
+                  "-Wl,-rpath,$ORIGIN/../ambient_image_processor_amd", "-Wl,-rpath,/opt/rocm/lib"],
+        common + [os.path.join(cli, "tgvk_deshade_downsampled_main.cpp"), "-o", CLI_DOWNSAMPLED,
+                  "-L" + PKG, "-lambient_amd", "-lz", "-Wl,-rpath," + PKG,
                   "-Wl,-rpath,$ORIGIN/../ambient_image_processor_amd", "-Wl,-rpath,/opt/rocm/lib"],
         common + [os.path.join(cli, "metaimage_convert.cpp"), "-o", METAIMAGE_CONVERT, "-lz"],
     ]

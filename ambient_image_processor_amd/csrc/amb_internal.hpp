@@ -161,6 +161,16 @@ hipError_t launch_tgvk_primal(hipStream_t s, float* const* fields_dev, float* co
                               const float* f, const uint8_t* active, Geometry g, int levels,
                               float tau, float lambda, float theta);
 
+// Cubic B-spline resampling (resize.hip): in (gi) to out (go, the kept voxels);
+// coef: gi.n() doubles of scratch.
+hipError_t launch_resize_spline3(hipStream_t s, const float* in, Geometry gi, double* coef,
+                                 float* out, Geometry go);
+// Elementwise tail of the downsampled deshade (resize.hip); mask may be null.
+hipError_t launch_downsampled_tail(hipStream_t s, const float* f, const float* mask, float* l,
+                                   float* r, size_t n, int masked_variant, int add_background_back,
+                                   int set_negative_zero);
+hipError_t launch_multiply(hipStream_t s, const float* a, const float* b, float* out, size_t n);
+
 // FP64 energy / gap (energy.hip).  partials: at least energy_partials_size(g) doubles.
 size_t energy_partials_size(Geometry g);
 hipError_t launch_energy(hipStream_t s, int storage, void* const* fields, const float* f,

@@ -52,9 +52,10 @@ amb_status fail_hip(hipError_t e, const char* what, int line) {
     } while (0)
 
 constexpr int N_SCRATCH = 5;
-constexpr int N_PROF = 12;
+constexpr int N_PROF = 13;
 enum { P_DUAL = 0, P_PRIMAL = 1, P_DCT = 2, P_POISSON = 3, P_DIV = 4, P_EPI = 5, P_INIT = 6,
-       P_ENERGY = 7, P_ADJOINT = 8, P_FUSED = 9, P_TGVK_DUAL = 10, P_TGVK_PRIMAL = 11 };
+       P_ENERGY = 7, P_ADJOINT = 8, P_FUSED = 9, P_TGVK_DUAL = 10, P_TGVK_PRIMAL = 11,
+       P_RESIZE = 12 };
 
 struct AxisTables {
     int n = 0;
@@ -125,6 +126,9 @@ struct amb_context {
     std::vector<float*> klv_host;
     std::vector<float> kalpha_host;
     int klevels = 0;
+    // resampling and the downsampled deshade: one grow-only workspace
+    char* rblock = nullptr;
+    size_t rcap = 0;
 };
 
 namespace {
@@ -830,6 +834,7 @@ AMB_API amb_status amb_context_destroy(amb_context* c) {
     if (c->kblock) (void)hipFree(c->kblock);
     if (c->klv_dev) (void)hipFree(c->klv_dev);
     if (c->kalpha_dev) (void)hipFree(c->kalpha_dev);
+    if (c->rblock) (void)hipFree(c->rblock);
     (void)hipStreamDestroy(c->stream);
     delete c;
     return AMB_OK;
@@ -1218,6 +1223,149 @@ AMB_API amb_status amb_tgvk_deshade_pipeline(amb_context* c, amb_memory mem, con
         AMB_TRY(tgvk_run(c, pp.iterations, pp.callback_interval, cb, user));
     }
     return deshade_tail(c, mem, mask, pp.flags, pp.dct_precision, masked, u, shading, deshaded);
+}
+
+// ---- resampling -------------------------------------------------------------
+namespace {
+
+// Carves `count` regions of the given byte sizes out of the context's resize
+// workspace (256-B aligned), growing it when needed.
+amb_status resize_workspace(amb_context* c, const size_t* bytes, int count, char** out) {
+    size_t need = 0;
+    for (int k = 0; k < count; ++k) need += align_up(bytes[k], 256);
+    if (need > c->rcap) {
+        if (c->rblock) AMB_HIP(hipFree(c->rblock));
+        c->rblock = nullptr;
+        c->rcap = 0;
+        AMB_HIP(hipMalloc(reinterpret_cast<void**>(&c->rblock), need));
+        c->rcap = need;
+    }
+    size_t off = 0;
+    for (int k = 0; k < count; ++k) {
+        out[k] = c->rblock + off;
+        off += align_up(bytes[k], 256);
+    }
+    return AMB_OK;
+}
+
+uint32_t resize_extent(uint32_t n, float factor) {
+    const float v = std::ceil(float(n) * factor);
+    return v < 1.0f ? 1u : uint32_t(v);
+}
+
+}  // namespace
+
+AMB_API void amb_resize_extents(uint32_t w, uint32_t h, uint32_t d, float factor,
+                                uint32_t out[3]) {
+    out[0] = resize_extent(w, factor);
+    out[1] = resize_extent(h, factor);
+    out[2] = resize_extent(d, factor);
+}
+
+AMB_API amb_status amb_resize_bspline3(amb_context* c, amb_memory mem, const float* in, uint32_t w,
+                                       uint32_t h, uint32_t d, float* out, uint32_t ow,
+                                       uint32_t oh, uint32_t od) {
+    if (!c || !in || !out) return fail(AMB_ERR_INVALID_ARGUMENT, "NULL argument");
+    AMB_TRY(check_geometry(w, h, d));
+    AMB_TRY(check_geometry(ow, oh, od));
+    AMB_TRY(set_device(c));
+    const Geometry gi{int(w), int(h), int(d)}, go{int(ow), int(oh), int(od)};
+    const size_t sz[3] = {gi.n() * sizeof(double), gi.n() * 4, go.n() * 4};
+    char* ws[3];
+    AMB_TRY(resize_workspace(c, sz, 3, ws));
+    const float* src = in;
+    float* dst = out;
+    if (mem != AMB_MEM_DEVICE) {
+        AMB_HIP(hipMemcpyAsync(ws[1], in, gi.n() * 4, hipMemcpyHostToDevice, c->stream));
+        src = reinterpret_cast<float*>(ws[1]);
+        dst = reinterpret_cast<float*>(ws[2]);
+    }
+    {
+        ProfScope ps(c, P_RESIZE);
+        AMB_HIP(launch_resize_spline3(c->stream, src, gi, reinterpret_cast<double*>(ws[0]), dst, go));
+    }
+    if (mem != AMB_MEM_DEVICE)
+        AMB_HIP(hipMemcpyAsync(out, dst, go.n() * 4, hipMemcpyDeviceToHost, c->stream));
+    AMB_HIP(hipStreamSynchronize(c->stream));
+    return AMB_OK;
+}
+
+AMB_API amb_status amb_tgvk_deshade_downsampled(amb_context* c, amb_memory mem, const float* f,
+                                                uint32_t w, uint32_t h, uint32_t d,
+                                                const amb_tgv_params* p, float factor,
+                                                uint32_t order, const float* alpha,
+                                                const float* mask, float* u, float* shading,
+                                                float* deshaded, amb_iteration_callback cb,
+                                                void* user) {
+    if (!c || !f || !p || !alpha) return fail(AMB_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (!(factor > 0.f)) return fail(AMB_ERR_INVALID_ARGUMENT, "downsampling factor must be positive");
+    AMB_TRY(check_geometry(w, h, d));
+    AMB_TRY(set_device(c));
+    const bool masked_variant = (p->flags & AMB_FLAG_MASKED_ITERATION) != 0;
+    const Geometry g{int(w), int(h), int(d)};
+    uint32_t se[3] = {w, h, d};
+    if (factor != 1.0f) amb_resize_extents(w, h, d, factor, se);
+    const Geometry gs{int(se[0]), int(se[1]), int(se[2])};
+    const size_t n = g.n(), ns = gs.n();
+    // full: f, mask, f*mask, u, l, r; small: f, mask, u, l, r; coefficients
+    const size_t sz[12] = {n * 4, n * 4, n * 4, n * 4, n * 4, n * 4,
+                           ns * 4, ns * 4, ns * 4, ns * 4, ns * 4,
+                           std::max(n, ns) * sizeof(double)};
+    char* ws[12];
+    AMB_TRY(resize_workspace(c, sz, 12, ws));
+    float* F = reinterpret_cast<float*>(ws[0]);
+    float* M = mask ? reinterpret_cast<float*>(ws[1]) : nullptr;
+    float* FM = reinterpret_cast<float*>(ws[2]);
+    float* U = reinterpret_cast<float*>(ws[3]);
+    float* L = reinterpret_cast<float*>(ws[4]);
+    float* R = reinterpret_cast<float*>(ws[5]);
+    float* Fs = reinterpret_cast<float*>(ws[6]);
+    float* Ms = reinterpret_cast<float*>(ws[7]);
+    float* Us = reinterpret_cast<float*>(ws[8]);
+    float* Ls = reinterpret_cast<float*>(ws[9]);
+    float* Rs = reinterpret_cast<float*>(ws[10]);
+    double* coef = reinterpret_cast<double*>(ws[11]);
+    const hipMemcpyKind kin = kind_in(mem);
+    AMB_HIP(hipMemcpyAsync(F, f, n * 4, kin, c->stream));
+    if (mask) AMB_HIP(hipMemcpyAsync(M, mask, n * 4, kin, c->stream));
+    // TGVKDeshadeDownsampledProcessor.cpp:46-53: the masked input goes down
+    const float* src = F;
+    if (masked_variant && M) {
+        AMB_HIP(launch_multiply(c->stream, F, M, FM, n));
+        src = FM;
+    }
+    const bool resample = factor != 1.0f;
+    const float* fs = src;
+    const float* ms = M;
+    if (resample) {
+        ProfScope ps(c, P_RESIZE);
+        AMB_HIP(launch_resize_spline3(c->stream, src, g, coef, Fs, gs));
+        fs = Fs;
+        if (M) {
+            AMB_HIP(launch_resize_spline3(c->stream, M, g, coef, Ms, gs));
+            ms = Ms;
+        }
+    }
+    // the (masked) TGV-k processor on the resampled volume, device to device
+    AMB_TRY(amb_tgvk_deshade_pipeline(c, AMB_MEM_DEVICE, fs, gs.w, gs.h, gs.d, p, order, alpha, ms,
+                                      resample ? Us : U, resample ? Ls : L, Rs, cb, user));
+    if (resample) {
+        ProfScope ps(c, P_RESIZE);
+        AMB_HIP(launch_resize_spline3(c->stream, Us, gs, coef, U, g));
+        AMB_HIP(launch_resize_spline3(c->stream, Ls, gs, coef, L, g));
+    }
+    {
+        ProfScope ps(c, P_EPI);
+        AMB_HIP(launch_downsampled_tail(c->stream, F, M, L, R, n, masked_variant ? 1 : 0,
+                                        (p->flags & AMB_FLAG_ADD_BACKGROUND_BACK) ? 1 : 0,
+                                        (p->flags & AMB_FLAG_SET_NEGATIVE_ZERO) ? 1 : 0));
+    }
+    const hipMemcpyKind kout = kind_out(mem);
+    if (u) AMB_HIP(hipMemcpyAsync(u, U, n * 4, kout, c->stream));
+    if (shading) AMB_HIP(hipMemcpyAsync(shading, L, n * 4, kout, c->stream));
+    if (deshaded) AMB_HIP(hipMemcpyAsync(deshaded, R, n * 4, kout, c->stream));
+    AMB_HIP(hipStreamSynchronize(c->stream));
+    return AMB_OK;
 }
 
 AMB_API amb_status amb_tgv2_energy(amb_context* c, double out[10]) {

@@ -29,7 +29,8 @@ N_FIELDS = 17
 FIELD_NAMES = ["u", "ubar", "vx", "vy", "vz", "vbx", "vby", "vbz", "px", "py", "pz",
                "qxx", "qyy", "qzz", "qxy", "qxz", "qyz"]
 PROFILE_CLASSES = ["dual_sweep", "primal_sweep", "dct", "poisson", "divergence", "epilogue",
-                   "init", "energy", "adjoint", "fused_sweep", "tgvk_dual", "tgvk_primal"]
+                   "init", "energy", "adjoint", "fused_sweep", "tgvk_dual", "tgvk_primal",
+                   "resize"]
 
 # Every function include/ambient_amd.h declares (checked by the CPU test suite).
 EXPORTS = [
@@ -42,7 +43,8 @@ EXPORTS = [
     "amb_tgv2_get_state", "amb_tgv2_iterate", "amb_tgv2_sweep", "amb_profile_enable",
     "amb_profile_read", "amb_profile_reset", "amb_tgv2_denoise_adjoint",
     "amb_tgv2_deshade_masked", "amb_deshade_epilogue", "amb_tgvk_deshade",
-    "amb_tgvk_deshade_pipeline", "amb_tgvk_deshade_masked",
+    "amb_tgvk_deshade_pipeline", "amb_tgvk_deshade_masked", "amb_resize_bspline3",
+    "amb_resize_extents", "amb_tgvk_deshade_downsampled",
 ]
 
 
@@ -105,6 +107,10 @@ def lib():
                                       fp, fp, fp, CALLBACK, vp],
         "amb_tgvk_deshade_masked": [vp, mem, fp, u32, u32, u32, POINTER(Params), u32, fp, vp, fp,
                                     fp, fp, fp, CALLBACK, vp],
+        "amb_resize_bspline3": [vp, mem, fp, u32, u32, u32, fp, u32, u32, u32],
+        "amb_resize_extents": [u32, u32, u32, c_float, POINTER(c_uint32)],
+        "amb_tgvk_deshade_downsampled": [vp, mem, fp, u32, u32, u32, POINTER(Params), c_float,
+                                         u32, fp, fp, fp, fp, fp, CALLBACK, vp],
         "amb_tgv2_energy": [vp, POINTER(c_double)],
         "amb_tgv2_begin": [vp, mem, fp, u32, u32, u32, POINTER(Params)],
         "amb_tgv2_set_state": [vp, mem, POINTER(c_void_p)],
@@ -121,7 +127,7 @@ def lib():
         fn = getattr(L, name)
         fn.argtypes = args
         fn.restype = c_void_p if name == "amb_context_stream" else (
-            None if name == "amb_tgv_params_default" else c_int)
+            None if name in ("amb_tgv_params_default", "amb_resize_extents") else c_int)
     L.amb_last_error.restype = ctypes.c_char_p
     L.amb_version.restype = ctypes.c_char_p
     _lib = L
@@ -139,6 +145,14 @@ def default_params(**kw) -> Params:
     for k, v in kw.items():
         setattr(p, "lambda_" if k in ("lam", "lambda") else k, v)
     return p
+
+
+def resize_extents(shape, factor):
+    """amb_resize_extents for a (D, H, W) shape -> (D', H', W')."""
+    d, h, w = (int(v) for v in shape)
+    out = (c_uint32 * 3)()
+    lib().amb_resize_extents(w, h, d, float(factor), out)
+    return int(out[2]), int(out[1]), int(out[0])
 
 
 def _f32(a):
@@ -276,6 +290,32 @@ class Context:
                                              *[_ptr(o) for o in outs], cb, None))
         del keep
         return tuple(outs)
+
+    def resize_bspline3(self, a, out_shape):
+        """amb_resize_bspline3 on host arrays: (D, H, W) -> out_shape (D', H', W')."""
+        a = _f32(a)
+        d, h, w = (int(v) for v in out_shape)
+        out = np.empty((d, h, w), np.float32)
+        _check(lib().amb_resize_bspline3(self._h, MEM_HOST, _ptr(a), *_dims(a.shape), _ptr(out),
+                                         w, h, d))
+        return out
+
+    def tgvk_deshade_downsampled(self, f, params: Params, factor, alpha, mask=None,
+                                 callback=None):
+        """amb_tgvk_deshade_downsampled on host arrays -> (u, shading, deshaded)."""
+        f = _f32(f)
+        al = np.ascontiguousarray(alpha, dtype=np.float32)
+        m = _f32(mask) if mask is not None else None
+        u, l, r = (np.empty_like(f) for _ in range(3))
+        small = resize_extents(f.shape, factor) if factor != 1 else f.shape
+        self.shape = small   # the iteration state lives on the resized volume
+        cb, keep = self._callback(callback, small, True)
+        _check(lib().amb_tgvk_deshade_downsampled(self._h, MEM_HOST, _ptr(f), *_dims(f.shape),
+                                                  ctypes.byref(params), float(factor), al.size,
+                                                  _ptr(al), _ptr(m), _ptr(u), _ptr(l), _ptr(r),
+                                                  cb, None))
+        del keep
+        return u, l, r
 
     def tgvk_deshade_pipeline(self, f, params: Params, alpha, mask=None, callback=None):
         """amb_tgvk_deshade_pipeline (order = len(alpha) >= 2) on host arrays

@@ -444,6 +444,105 @@ class TGVKDeshadeMaskedProcessor:
         return u, l, r, div_v
 
 
+class ResizeProcessor:
+    """Reference resize/ResizeProcessor.{h,cpp}: the cubic B-spline resampling
+    (InterpolationMethod::BSpline3) the downsampled deshade uses."""
+
+    @staticmethod
+    def process(image, *args):
+        """process(image, size_factor) or process(image, width, height, depth)
+        (ResizeProcessor.cpp:41-152, BSpline3) -> the resized (D', H', W') array."""
+        a = np.ascontiguousarray(image, dtype=np.float32)
+        if len(args) == 1:
+            out_shape = native.resize_extents(a.shape, args[0])
+        elif len(args) == 3:
+            w, h, d = args
+            out_shape = (int(d), int(h), int(w))
+        else:
+            raise TypeError("process(image, size_factor) or process(image, width, height, depth)")
+        return _ctx().resize_bspline3(a, out_shape)
+
+
+def _downsampled(masked_variant, input_image, downsampling_factor, lambda_, order, alpha,
+                 iteration_count, mask, set_negative_values_to_zero, add_background_back,
+                 paint_iteration_interval, iteration_finished_callback, calculate_div_v, kw):
+    if len(alpha) != order:
+        raise ValueError("alpha needs one weight per order")
+    f = np.ascontiguousarray(input_image, dtype=np.float32)
+    m = None if mask is None else np.ascontiguousarray(mask, dtype=np.float32).reshape(f.shape)
+    flags = native.FLAG_MASKED_ITERATION if masked_variant else 0
+    if set_negative_values_to_zero:
+        flags |= native.FLAG_SET_NEGATIVE_ZERO
+    if add_background_back:
+        flags |= native.FLAG_ADD_BACKGROUND_BACK
+    p = _params(lambda_, iteration_count, paint_iteration_interval, 1.0, 1.0, native.DATA_L1,
+                flags=flags, **kw)
+    cb = None
+    if iteration_finished_callback is not None:
+        small = (f.shape if downsampling_factor == 1
+                 else native.resize_extents(f.shape, downsampling_factor))
+        ms = None
+        if m is not None:
+            ms = m if downsampling_factor == 1 else _aux_ctx().resize_bspline3(m, small)
+        cflags = native.FLAG_MASK_SHADING | (flags & native.FLAG_SET_NEGATIVE_ZERO)
+        if masked_variant:
+            cflags |= native.FLAG_MASKED_ITERATION | (flags & native.FLAG_ADD_BACKGROUND_BACK)
+            if ms is None:
+                ms = np.ones(small, np.float32)
+        elif ms is not None:
+            cflags |= flags & native.FLAG_ADD_BACKGROUND_BACK
+
+        def cb(it, count, u, vx, vy, vz):
+            aux = _aux_ctx()
+            l = aux.integrate_gradients_poisson(vx, vy, vz, p.dct_precision)
+            r, l = aux.deshade_epilogue(u, l, ms, cflags)
+            return iteration_finished_callback(it, count, u, l, r)
+    u, l, r = _ctx().tgvk_deshade_downsampled(f, p, downsampling_factor, alpha, m, cb)
+    div_v = None
+    if calculate_div_v:
+        st = _ctx().get_state()
+        div_v = _ctx().divergence(st[2], st[3], st[4])
+        if downsampling_factor != 1:
+            div_v = _ctx().resize_bspline3(div_v, f.shape)
+    return u, l, r, div_v
+
+
+class TGVKDeshadeDownsampledProcessor:
+    """Reference tgvk_deshade_downsampled/TGVKDeshadeDownsampledProcessor.{h,cpp}."""
+
+    @staticmethod
+    def processTGVKL1Cuda(input_image, downsampling_factor, lambda_, order, alpha,
+                          iteration_count, mask=None, set_negative_values_to_zero=True,
+                          add_background_back=False, paint_iteration_interval=0,
+                          iteration_finished_callback=None, calculate_div_v=False, **kw):
+        """TGVKDeshadeDownsampledProcessor.cpp:29-129 -> (denoised_image,
+        shading_image, deshaded_image, div_v_image or None): the masked input
+        and the mask resized by the factor (cubic B-splines), the masked TGV-k
+        processor there, u and l resized back, r = f*mask - l with the mask,
+        add-back and clamp.  The callback sees the resized volume's (u, l, r)."""
+        return _downsampled(True, input_image, downsampling_factor, lambda_, order, alpha,
+                            iteration_count, mask, set_negative_values_to_zero,
+                            add_background_back, paint_iteration_interval,
+                            iteration_finished_callback, calculate_div_v, kw)
+
+
+class TGVKDeshadeDownsampledNotMaskedProcessor:
+    """Reference tgvk_deshade_downsampled/TGVKDeshadeDownsampledNotMaskedProcessor.{h,cpp}."""
+
+    @staticmethod
+    def processTGVKL1Cuda(input_image, downsampling_factor, lambda_, order, alpha,
+                          iteration_count, mask=None, set_negative_values_to_zero=True,
+                          add_background_back=False, paint_iteration_interval=0,
+                          iteration_finished_callback=None, calculate_div_v=False, **kw):
+        """TGVKDeshadeDownsampledNotMaskedProcessor.cpp:29-120: as the masked
+        form, with the input itself resized, the unmasked TGV-k processor and
+        r = f - l (times the mask)."""
+        return _downsampled(False, input_image, downsampling_factor, lambda_, order, alpha,
+                            iteration_count, mask, set_negative_values_to_zero,
+                            add_background_back, paint_iteration_interval,
+                            iteration_finished_callback, calculate_div_v, kw)
+
+
 def _aux_ctx() -> native.Context:
     """Second context for work issued from inside an iteration callback."""
     c = getattr(_tls, "aux", None)

@@ -309,6 +309,42 @@ AMB_API amb_status amb_tgvk_deshade_pipeline(amb_context* ctx, amb_memory mem, c
                                              float* shading, float* deshaded,
                                              amb_iteration_callback callback, void* user);
 
+/* ---- resampling and the downsampled deshade ----------------------------------
+ * ResizeProcessor::process(image, width, height, depth, BSpline3)
+ * (reference resize/ResizeProcessor.cpp:54-152, through ITK 4.8.1's
+ * ResampleImageFilter + BSplineInterpolateImageFunction of order 3): cubic
+ * B-spline resampling of in (width x height x depth) to out (ow x oh x od), FP64
+ * coefficients with mirror boundaries, output rounded to float.  As the
+ * reference, an upsampling resize resamples with 2 more voxels per axis and
+ * keeps the first ow x oh x od; samples outside the input are 0. */
+AMB_API amb_status amb_resize_bspline3(amb_context* ctx, amb_memory mem, const float* in,
+                                       uint32_t width, uint32_t height, uint32_t depth,
+                                       float* out, uint32_t ow, uint32_t oh, uint32_t od);
+/* ResizeProcessor::process(image, size_factor, ...) target extents
+ * (ResizeProcessor.cpp:41-52): max(1, ceil(float(extent) * factor)) per axis. */
+AMB_API void amb_resize_extents(uint32_t width, uint32_t height, uint32_t depth, float factor,
+                                uint32_t out[3]);
+
+/* TGVKDeshadeDownsampledProcessor::processTGVKL1Cuda
+ * (tgvk_deshade_downsampled/TGVKDeshadeDownsampledProcessor.cpp:29-129, the
+ * widget's and tgvk_deshade_downsampled_main.cpp's path) with
+ * AMB_FLAG_MASKED_ITERATION: the input times the mask is resized by `factor`
+ * (cubic B-splines; factor 1 skips the resizes), so is the mask; the masked TGV-k
+ * processor (amb_tgvk_deshade_pipeline) runs there; u and l are resized back;
+ * r = f*mask - l, l and r times the mask, plus f*(1 - mask) with
+ * AMB_FLAG_ADD_BACKGROUND_BACK, then the clamp (AMB_FLAG_SET_NEGATIVE_ZERO).
+ * Without the flag: TGVKDeshadeDownsampledNotMaskedProcessor.cpp:29-120 - f
+ * itself goes down, the unmasked TGV-k processor, r = f - l (times the mask),
+ * the add-back and the clamp.  mask may be NULL.  Writes u, shading and
+ * deshaded at the input's extents; the callback sees the resized volume. */
+AMB_API amb_status amb_tgvk_deshade_downsampled(amb_context* ctx, amb_memory mem, const float* f,
+                                                uint32_t width, uint32_t height, uint32_t depth,
+                                                const amb_tgv_params* params, float factor,
+                                                uint32_t order, const float* alpha,
+                                                const float* mask, float* u, float* shading,
+                                                float* deshaded, amb_iteration_callback callback,
+                                                void* user);
+
 /* ---- timing ----------------------------------------------------------------
  * With profiling on, every kernel launch of the context is bracketed by HIP
  * events on the context's stream; amb_profile_read returns the summed device
@@ -316,7 +352,7 @@ AMB_API amb_status amb_tgvk_deshade_pipeline(amb_context* ctx, amb_memory mem, c
  *   0 dual sweep, 1 primal sweep, 2 DCT passes, 3 Poisson divide,
  *   4 divergence, 5 epilogue, 6 init, 7 energy, 8 adjoint (whole call),
  *   9 fused sweep (primal half of one iteration + dual half of the next),
- *   10 TGV-k dual sweep, 11 TGV-k primal sweep. */
+ *   10 TGV-k dual sweep, 11 TGV-k primal sweep, 12 B-spline resampling. */
 AMB_API amb_status amb_profile_enable(amb_context* ctx, int enable);
 AMB_API amb_status amb_profile_read(amb_context* ctx, int kernel_class, double* total_ms,
                                     uint64_t* launches);

@@ -295,6 +295,78 @@ def deshade_k_masked(f, iterations, lam, alpha, tau, sigma, mask=None, theta=1.0
     return st.u.copy(), l2, r, st
 
 
+def spline3_coefficients(a):
+    """Cubic B-spline coefficients (FP64) of a float32 (D, H, W) volume, the
+    itk::BSplineDecompositionImageFilter restatement."""
+    a = _c(a, np.float32)
+    c = np.empty(a.shape, np.float64)
+    lib().orc_spline3_coefficients(_p(a), _p(c), *_dims(a))
+    return c
+
+
+def resize_spline3(a, out_shape):
+    """ResizeProcessor::process(image, width, height, depth, BSpline3): cubic
+    B-spline resampling of a (D, H, W) volume to out_shape (D', H', W')."""
+    a = _c(a, np.float32)
+    d, h, w = (int(v) for v in out_shape)
+    out = np.empty((d, h, w), np.float32)
+    lib().orc_resize_spline3(_p(a), *_dims(a), _p(out), ctypes.c_longlong(w),
+                             ctypes.c_longlong(h), ctypes.c_longlong(d))
+    return out
+
+
+def resize_extents(shape, factor):
+    """ResizeProcessor::process(image, size_factor, ...) target extents
+    (resize/ResizeProcessor.cpp:41-52): max(1, ceil(float(extent) * factor))."""
+    f32 = np.float32(factor)
+    return tuple(max(1, int(np.ceil(np.float32(n) * f32))) for n in shape)
+
+
+def deshade_k_downsampled(f, factor, iterations, lam, alpha, tau, sigma, mask=None, theta=1.0,
+                          set_negative_zero=True, add_background_back=False, masked=True):
+    """TGVKDeshadeDownsampledProcessor::processTGVKL1Cuda
+    (tgvk_deshade_downsampled/TGVKDeshadeDownsampledProcessor.cpp:29-129; masked=False:
+    TGVKDeshadeDownsampledNotMaskedProcessor.cpp:29-120): resize f (and the mask) by
+    the factor with cubic B-splines, deshade there (the masked TGV-k processor, or the
+    unmasked one), resize u and l back, r = f - l with the mask, add-back and clamp.
+    Returns (u, l, r)."""
+    f = _c(f, np.float32)
+    m = None if mask is None else _c(mask, np.float32)
+    down = (lambda a: a) if factor == 1 else \
+        (lambda a: resize_spline3(a, resize_extents(a.shape, factor)))
+    up = (lambda a: a) if factor == 1 else (lambda a: resize_spline3(a, f.shape))
+    if masked:
+        background = None
+        if add_background_back and m is not None:
+            background = f * (np.float32(1) - m)   # invert.cu: 1 - mask, multiply.cu
+        fm = f if m is None else f * m
+        fd = down(fm)
+        md = None if m is None else down(m)
+        ud, ld, _, _ = deshade_k_masked(fd, iterations, lam, alpha, tau, sigma, md, theta,
+                                        set_negative_zero, add_background_back)
+        u, l = up(ud), up(ld)
+        r = fm - l
+        if m is not None:
+            l = l * m
+            r = r * m
+        if background is not None:
+            r = r + background
+    else:
+        fd = down(f)
+        md = None if m is None else down(m)
+        ud, ld, _, _ = deshade_k(fd, iterations, lam, alpha, tau, sigma, theta, md,
+                                 set_negative_zero, add_background_back)
+        u, l = up(ud), up(ld)
+        r = f - l
+        if m is not None:
+            r = r * m
+        if add_background_back and m is not None:
+            r = r + f * (np.float32(1) - m)
+    if set_negative_zero:
+        r = np.where(r < 0, np.float32(0), r)
+    return u, l, r.astype(np.float32)
+
+
 def divergence(vx, vy, vz, dtype=np.float64):
     vx, vy, vz = (_c(a, dtype) for a in (vx, vy, vz))
     out = np.empty_like(vx)

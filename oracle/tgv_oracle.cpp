@@ -377,6 +377,153 @@ void tgvk_iteration(State<T>& st, std::vector<Level<T>>& lv, const T* f, const D
 }
 
 // ---------------------------------------------------------------------------
+// Cubic B-spline resampling as ResizeProcessor::process(image, w, h, d, BSpline3)
+// runs it (resize/ResizeProcessor.cpp:41-150) through ITK 4.8.1 (README.md:37):
+// itk::ResampleImageFilter with itk::BSplineInterpolateImageFunction (order 3)
+// on coefficients from itk::BSplineDecompositionImageFilter.  ITK is not part of
+// the reference tree; its published algorithm is restated, in FP64:
+//  - decomposition (M. Unser, "Splines: a perfect fit for signal and image
+//    processing", IEEE SPM 1999, Box 2): along x, then y, then z (an axis of
+//    length 1 is left alone); gain (1 - z)(1 - 1/z), pole z = sqrt(3) - 2;
+//    mirror-boundary causal initialisation, truncated to the first
+//    ceil(log(1e-10) / log|z|) = 18 terms when the line is longer and exact
+//    otherwise; c[n] += z c[n-1]; c[N-1] = z / (z^2 - 1) (z c[N-2] + c[N-1]);
+//    c[n] = z (c[n+1] - c[n]);
+//  - geometry: origins 0, unit input spacing, output spacing 1 / (float(new) /
+//    float(old)) per axis: output voxel j samples the continuous input index
+//    j * spacing; samples outside [-0.5, N - 0.5) on any axis are 0;
+//  - evaluation: support floor(x) - 1 .. floor(x) + 2 with the cubic B-spline
+//    weights of w = x - floor(x), indices mirrored about 0 and N - 1 (all 0
+//    when N = 1; see mirror_index), the 64 products (w_x w_y w_z) c summed x
+//    fastest, the sum rounded to float;
+//  - ResizeProcessor's upsampling quirk (:57-66, :146-151): when any extent
+//    grows, every extent gets 2 more voxels in the resampling geometry and the
+//    first (w, h, d) voxels are kept.
+// ---------------------------------------------------------------------------
+struct Spline3 {
+    double z, gain, pole_factor;
+    i64 horizon;
+    Spline3() {
+        z = std::sqrt(3.0) - 2.0;
+        gain = (1.0 - z) * (1.0 - 1.0 / z);
+        pole_factor = z / (z * z - 1.0);
+        horizon = i64(std::ceil(std::log(1e-10) / std::log(std::fabs(z))));
+    }
+};
+
+// One line of length N >= 2 (stride st) in place.
+void spline3_decompose_line(double* c, i64 N, i64 st, const Spline3& sp) {
+    const double z = sp.z;
+    for (i64 n = 0; n < N; ++n) c[n * st] *= sp.gain;
+    if (sp.horizon < N) {
+        double zn = z, sum = c[0];
+        for (i64 n = 1; n < sp.horizon; ++n) {
+            sum += zn * c[n * st];
+            zn *= z;
+        }
+        c[0] = sum;
+    } else {
+        double zn = z;
+        const double iz = 1.0 / z;
+        double z2n = std::pow(z, double(N - 1));
+        double sum = c[0] + z2n * c[(N - 1) * st];
+        z2n *= z2n * iz;
+        for (i64 n = 1; n <= N - 2; ++n) {
+            sum += (zn + z2n) * c[n * st];
+            zn *= z;
+            z2n *= iz;
+        }
+        c[0] = sum / (1.0 - zn * zn);
+    }
+    for (i64 n = 1; n < N; ++n) c[n * st] += z * c[(n - 1) * st];
+    c[(N - 1) * st] = sp.pole_factor * (z * c[(N - 2) * st] + c[(N - 1) * st]);
+    for (i64 n = N - 2; n >= 0; --n) c[n * st] = z * (c[(n + 1) * st] - c[n * st]);
+}
+
+void spline3_coefficients(const float* in, double* c, const Dims& s) {
+    const Spline3 sp;
+    const i64 n = s.n();
+    for (i64 i = 0; i < n; ++i) c[i] = double(in[i]);
+    for (int axis = 0; axis < 3; ++axis) {
+        const i64 N = s.extent(axis), st = s.stride(axis);
+        if (N == 1) continue;
+        const i64 lines = n / N;
+#pragma omp parallel for schedule(static)
+        for (i64 l = 0; l < lines; ++l) {
+            i64 base;
+            if (axis == 0) base = l * s.w;
+            else if (axis == 1) base = (l / s.w) * s.w * s.h + (l % s.w);
+            else base = l;
+            spline3_decompose_line(c + base, N, st, sp);
+        }
+    }
+}
+
+inline void spline3_weights(double x, i64& i1, double w4[4]) {
+    i1 = i64(std::floor(x));
+    const double w = x - double(i1);
+    w4[3] = (1.0 / 6.0) * w * w * w;
+    w4[0] = (1.0 / 6.0) + 0.5 * w * (w - 1.0) - w4[3];
+    w4[2] = w + w4[0] - 2.0 * w4[3];
+    w4[1] = 1.0 - w4[0] - w4[2] - w4[3];
+}
+
+// Whole-sample mirror about 0 and N - 1 (period 2N - 2).  ITK 4.8 reflects once
+// at each end, which is the same for N >= 3; for N = 2 its upper reflection
+// sends index 3 (x in [1, 1.5)) to -1, outside the coefficient buffer, so the
+// folded mirror (the spline's own extension) is used for every N.
+inline i64 mirror_index(i64 k, i64 N) {
+    if (N == 1) return 0;
+    const i64 period = 2 * (N - 1);
+    k %= period;
+    if (k < 0) k += period;
+    return k >= N ? period - k : k;
+}
+
+// Output extents out (the voxels kept); the resampling geometry per the quirk.
+void resize_spline3(const float* in, const Dims& si, float* out, const Dims& so) {
+    std::vector<double> c(size_t(si.n()));
+    spline3_coefficients(in, c.data(), si);
+    const bool up = so.w > si.w || so.h > si.h || so.d > si.d;
+    const i64 pad = up ? 2 : 0;
+    const i64 geo[3] = {so.w + pad, so.h + pad, so.d + pad};
+    const i64 ext_in[3] = {si.w, si.h, si.d};
+    double spacing[3];
+    for (int a = 0; a < 3; ++a) {
+        const float sf = float(geo[a]) / float(ext_in[a]);
+        spacing[a] = 1.0 / double(sf);
+    }
+#pragma omp parallel for schedule(static)
+    for (i64 o = 0; o < so.n(); ++o) {
+        const i64 oc[3] = {o % so.w, (o / so.w) % so.h, o / (so.w * so.h)};
+        double x[3];
+        bool inside = true;
+        for (int a = 0; a < 3; ++a) {
+            x[a] = double(oc[a]) * spacing[a];
+            if (!(x[a] >= -0.5 && x[a] < double(ext_in[a]) - 0.5)) inside = false;
+        }
+        if (!inside) { out[o] = 0.0f; continue; }
+        i64 idx[3][4];
+        double wt[3][4];
+        for (int a = 0; a < 3; ++a) {
+            i64 i1;
+            spline3_weights(x[a], i1, wt[a]);
+            for (int k = 0; k < 4; ++k) idx[a][k] = mirror_index(i1 - 1 + k, ext_in[a]);
+        }
+        double acc = 0.0;
+        for (int p = 0; p < 64; ++p) {
+            const int k0 = p & 3, k1 = (p >> 2) & 3, k2 = p >> 4;
+            double w = 1.0;
+            w *= wt[0][k0];
+            w *= wt[1][k1];
+            w *= wt[2][k2];
+            acc += w * c[size_t((idx[2][k2] * si.h + idx[1][k1]) * si.w + idx[0][k0])];
+        }
+        out[o] = float(acc);
+    }
+}
+
+// ---------------------------------------------------------------------------
 // cos(pi * t) from the mathematics: exact reduction of t modulo 2 in double,
 // octant folding, then libm cos/sin of pi*r with |r| <= 1/4.
 // ---------------------------------------------------------------------------
@@ -682,6 +829,13 @@ ORC_EXPORT void orc_tensor_op_f64(double** in, double** out, i64 w, i64 h, i64 d
 }
 
 ORC_EXPORT double orc_cospi(double t) { return cospi_d(t); }
+ORC_EXPORT void orc_spline3_coefficients(const float* in, double* c, i64 w, i64 h, i64 d) {
+    spline3_coefficients(in, c, Dims{w, h, d});
+}
+ORC_EXPORT void orc_resize_spline3(const float* in, i64 w, i64 h, i64 d, float* out, i64 ow,
+                                   i64 oh, i64 od) {
+    resize_spline3(in, Dims{w, h, d}, out, Dims{ow, oh, od});
+}
 ORC_EXPORT int orc_num_threads(void) { return omp_get_max_threads(); }
 ORC_EXPORT void orc_set_num_threads(int n) { omp_set_num_threads(n); }
 
