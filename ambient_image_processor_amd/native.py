@@ -317,6 +317,16 @@ class Context:
         del keep
         return u, l, r
 
+    def tgvk_deshade_downsampled_device(self, f_dev: int, shape, params: Params, factor, alpha,
+                                        mask_dev=None, u_dev=None, l_dev=None, r_dev=None):
+        al = np.ascontiguousarray(alpha, dtype=np.float32)
+        self.shape = resize_extents(shape, factor) if factor != 1 else tuple(shape)
+        _check(lib().amb_tgvk_deshade_downsampled(self._h, MEM_DEVICE, c_void_p(f_dev),
+                                                  *_dims(shape), ctypes.byref(params),
+                                                  float(factor), al.size, _ptr(al),
+                                                  _ptr(mask_dev), _ptr(u_dev), _ptr(l_dev),
+                                                  _ptr(r_dev), CALLBACK(), None))
+
     def tgvk_deshade_pipeline(self, f, params: Params, alpha, mask=None, callback=None):
         """amb_tgvk_deshade_pipeline (order = len(alpha) >= 2) on host arrays
         -> (u, shading, deshaded)."""
