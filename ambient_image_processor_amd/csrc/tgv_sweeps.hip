@@ -390,7 +390,10 @@ struct VecIO<StoreF32, 4> {
     }
     template <class I>
     static __device__ __forceinline__ void st(float* p, I i, const float (&v)[4]) {
-        *reinterpret_cast<float4*>(elem(p, i)) = make_float4(v[0], v[1], v[2], v[3]);
+        // written once per launch and not read back in it: a streaming store
+        typedef float f4 __attribute__((ext_vector_type(4)));
+        const f4 t = {v[0], v[1], v[2], v[3]};
+        __builtin_nontemporal_store(t, reinterpret_cast<f4*>(elem(p, i)));
     }
 };
 
@@ -420,7 +423,9 @@ struct VecIO<StoreF32, 2> {
     }
     template <class I>
     static __device__ __forceinline__ void st(float* p, I i, const float (&v)[2]) {
-        *reinterpret_cast<float2*>(elem(p, i)) = make_float2(v[0], v[1]);
+        typedef float f2 __attribute__((ext_vector_type(2)));
+        const f2 t = {v[0], v[1]};
+        __builtin_nontemporal_store(t, reinterpret_cast<f2*>(elem(p, i)));
     }
 };
 
