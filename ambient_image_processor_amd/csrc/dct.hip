@@ -54,39 +54,98 @@ __global__ void divergence_kernel(const float* __restrict__ vx, const float* __r
     }
 }
 
-// A15/A17 in the reference's numerics: one thread per output element;
-// term = (double)x[m] * cos64 added into the FP32 accumulator (fma in double,
-// rounded to float each term), reference term order.
+// A15/A17 in the reference's numerics: every output is its own sequence of
+// terms (double)x[m] * cos64 added into an FP32 accumulator (fma in double,
+// rounded to float each term), in the reference's term order.
 //   forward:  acc = sum_{m=0}^{n-1} in[m] * T[m][k]; last pass: acc *= 8
 //   inverse:  acc = sum_{m=1}^{n-1} in[m] * T[m][k]; acc *= 2; acc += in[0];
 //             last pass: acc = acc / (W*H*D*8.0) in double
 // T[m][k] = cospi((m + 0.5) k / n) forward, cospi((k + 0.5) m / n) inverse.
-template <bool INVERSE>
-__global__ void dct_reference_kernel(const float* __restrict__ in, float* __restrict__ out,
-                                     const double* __restrict__ table, Geometry g, int axis,
-                                     int last) {
+// A block owns 64 lines x 64 outputs; chunks of 16 terms of the input lines
+// (FP32) and of the table (FP64) are staged in LDS and every thread carries a
+// 4 x 4 block of accumulators through them.  The per-output sequence is
+// unchanged, so the result does not depend on the tiling.  LINE_CONTIG: the x
+// axis, whose lines are contiguous (threads then walk the outputs along k).
+constexpr int DT_L = 64, DT_K = 64, DT_M = 16;
+
+__device__ __forceinline__ size_t line_base(const Geometry& g, int axis, long l, int n) {
+    if (axis == 0) return size_t(l) * size_t(n);
+    if (axis == 1) return size_t(l / g.w) * size_t(g.w) * size_t(g.h) + size_t(l % g.w);
+    return size_t(l);
+}
+
+template <bool INVERSE, bool LINE_CONTIG>
+__global__ __launch_bounds__(256) void dct_reference_kernel(
+    const float* __restrict__ in, float* __restrict__ out, const double* __restrict__ table,
+    Geometry g, int axis, int last, long lines) {
+    __shared__ float s_in[DT_M][DT_L];
+    __shared__ double s_t[DT_M][DT_K];
     size_t st;
     int n;
     axis_info(g, axis, st, n);
-    const size_t total = g.n();
-    const double vol8 = double(size_t(g.w) * g.h * g.d) * 8.0;
-    const size_t stride = size_t(gridDim.x) * blockDim.x;
-    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < total; i += stride) {
-        const int k = int((i / st) % size_t(n));
-        const size_t base = i - size_t(k) * st;
-        float acc = 0.0f;
-        if (!INVERSE) {
-            for (int m = 0; m < n; ++m)
-                acc = float(fma(double(in[base + size_t(m) * st]), table[size_t(m) * n + k], double(acc)));
-            if (last) acc = acc * 8.0f;
-        } else {
-            for (int m = 1; m < n; ++m)
-                acc = float(fma(double(in[base + size_t(m) * st]), table[size_t(m) * n + k], double(acc)));
-            acc = acc * 2.0f;
-            acc = acc + in[base];
-            if (last) acc = float(double(acc) / vol8);
+    const int nkt = (n + DT_K - 1) / DT_K;
+    const long l0 = long(blockIdx.x / unsigned(nkt)) * DT_L;
+    const int k0 = int(blockIdx.x % unsigned(nkt)) * DT_K;
+    const int t = int(threadIdx.x);
+    const int tl = LINE_CONTIG ? (t / 16) * 4 : (t % 16) * 4;
+    const int tk = LINE_CONTIG ? (t % 16) * 4 : (t / 16) * 4;
+    float acc[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) acc[a][b] = 0.0f;
+
+    for (int m0 = INVERSE ? 1 : 0; m0 < n; m0 += DT_M) {
+        const int mc = min(DT_M, n - m0);
+        __syncthreads();
+        for (int e = t; e < DT_M * DT_L; e += 256) {
+            const int m = LINE_CONTIG ? e % DT_M : e / DT_L;
+            const int l = LINE_CONTIG ? e / DT_M : e % DT_L;
+            float v = 0.0f;
+            if (m < mc && l0 + l < lines) v = in[line_base(g, axis, l0 + l, n) + size_t(m0 + m) * st];
+            s_in[m][l] = v;
         }
-        out[i] = acc;
+        for (int e = t; e < DT_M * DT_K; e += 256) {
+            const int m = e / DT_K, k = e % DT_K;
+            double v = 0.0;
+            if (m < mc && k0 + k < n) v = table[size_t(m0 + m) * size_t(n) + size_t(k0 + k)];
+            s_t[m][k] = v;
+        }
+        __syncthreads();
+        for (int m = 0; m < mc; ++m) {
+            double xin[4], tv[4];
+#pragma unroll
+            for (int a = 0; a < 4; ++a) xin[a] = double(s_in[m][tl + a]);
+#pragma unroll
+            for (int b = 0; b < 4; ++b) tv[b] = s_t[m][tk + b];
+#pragma unroll
+            for (int a = 0; a < 4; ++a)
+#pragma unroll
+                for (int b = 0; b < 4; ++b)
+                    acc[a][b] = float(fma(xin[a], tv[b], double(acc[a][b])));
+        }
+    }
+
+    const double vol8 = double(size_t(g.w) * g.h * g.d) * 8.0;
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+        const long l = l0 + tl + a;
+        if (l >= lines) continue;
+        const size_t base = line_base(g, axis, l, n);
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const int k = k0 + tk + b;
+            if (k >= n) continue;
+            float r = acc[a][b];
+            if (!INVERSE) {
+                if (last) r = r * 8.0f;
+            } else {
+                r = r * 2.0f;
+                r = r + in[base];
+                if (last) r = float(double(r) / vol8);
+            }
+            out[base + size_t(k) * st] = r;
+        }
     }
 }
 
@@ -181,12 +240,25 @@ hipError_t launch_divergence(hipStream_t s, const float* vx, const float* vy, co
 hipError_t launch_dct_reference_pass(hipStream_t s, const float* in, float* out,
                                      const double* table, Geometry g, int axis, int inverse,
                                      int last) {
-    if (inverse)
-        hipLaunchKernelGGL(dct_reference_kernel<true>, dim3(elem_blocks(g.n())), dim3(256), 0, s,
-                           in, out, table, g, axis, last);
-    else
-        hipLaunchKernelGGL(dct_reference_kernel<false>, dim3(elem_blocks(g.n())), dim3(256), 0, s,
-                           in, out, table, g, axis, last);
+    const int n = axis == 0 ? g.w : (axis == 1 ? g.h : g.d);
+    const long lines = long(g.n() / size_t(n));
+    const long blocks = ((lines + DT_L - 1) / DT_L) * long((n + DT_K - 1) / DT_K);
+    const dim3 grid{static_cast<unsigned>(blocks), 1, 1};
+    if (axis == 0) {
+        if (inverse)
+            hipLaunchKernelGGL((dct_reference_kernel<true, true>), grid, dim3(256), 0, s, in, out,
+                               table, g, axis, last, lines);
+        else
+            hipLaunchKernelGGL((dct_reference_kernel<false, true>), grid, dim3(256), 0, s, in, out,
+                               table, g, axis, last, lines);
+    } else {
+        if (inverse)
+            hipLaunchKernelGGL((dct_reference_kernel<true, false>), grid, dim3(256), 0, s, in, out,
+                               table, g, axis, last, lines);
+        else
+            hipLaunchKernelGGL((dct_reference_kernel<false, false>), grid, dim3(256), 0, s, in, out,
+                               table, g, axis, last, lines);
+    }
     return hipGetLastError();
 }
 
