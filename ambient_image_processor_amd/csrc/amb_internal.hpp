@@ -160,6 +160,21 @@ hipError_t launch_tgvk_dual(hipStream_t s, float* const* fields_dev, float* cons
 hipError_t launch_tgvk_primal(hipStream_t s, float* const* fields_dev, float* const* levels_dev,
                               const float* f, const uint8_t* active, Geometry g, int levels,
                               float tau, float lambda, float theta);
+// Fused TGV-k sweep for one level (k = 3): primal half of iteration k and
+// dual half of iteration k + 1, reading the 25 fields `src` (order u, vx, vy,
+// vz, px, py, pz, qxx, qyy, qzz, qxy, qxz, qyz, wx, wy, wz, wxy, wxz, wyz, rx,
+// ry, rz, rxy, rxz, ryz) and writing the same fields to `dst`; u_bar, v_bar
+// and w_bar are neither read nor written.  alpha: the three weights (host).
+struct TgvkFusedPlan {
+    int ty;       // owned y rows per block
+    int zchunk;   // planes per block
+};
+constexpr int N_TGVK_FUSED_FIELDS = 25;
+bool tgvk_fused_plan(const Geometry& g, int levels, int device, TgvkFusedPlan* plan);
+hipError_t launch_tgvk_fused(hipStream_t s, const float* const* src, float* const* dst,
+                             const float* f, const uint8_t* active, Geometry g, float tau,
+                             float sigma, float lambda, float theta, const float* alpha,
+                             int planar, const TgvkFusedPlan& plan);
 
 // Cubic B-spline resampling (resize.hip): in (gi) to out (go, the kept voxels);
 // coef: gi.n() doubles of scratch.

@@ -52,10 +52,10 @@ amb_status fail_hip(hipError_t e, const char* what, int line) {
     } while (0)
 
 constexpr int N_SCRATCH = 5;
-constexpr int N_PROF = 13;
+constexpr int N_PROF = 14;
 enum { P_DUAL = 0, P_PRIMAL = 1, P_DCT = 2, P_POISSON = 3, P_DIV = 4, P_EPI = 5, P_INIT = 6,
        P_ENERGY = 7, P_ADJOINT = 8, P_FUSED = 9, P_TGVK_DUAL = 10, P_TGVK_PRIMAL = 11,
-       P_RESIZE = 12 };
+       P_RESIZE = 12, P_TGVK_FUSED = 13 };
 
 struct AxisTables {
     int n = 0;
@@ -126,6 +126,13 @@ struct amb_context {
     std::vector<float*> klv_host;
     std::vector<float> kalpha_host;
     int klevels = 0;
+    // fused TGV-k sweep (one level): second copy of w_0 and r_0 (grow-only),
+    // the launch plan
+    char* kalt_block = nullptr;
+    size_t kalt_cap = 0;
+    float* kalt[12] = {};
+    bool kfused_ok = false;
+    TgvkFusedPlan kfplan{0, 0};
     // resampling and the downsampled deshade: one grow-only workspace
     char* rblock = nullptr;
     size_t rcap = 0;
@@ -618,26 +625,109 @@ amb_status tgvk_begin(amb_context* c, amb_memory mem, const float* f, uint32_t w
     AMB_HIP(hipMemcpyAsync(c->fields_dev, c->fields, sizeof(void*) * N_FIELDS,
                            hipMemcpyHostToDevice, c->stream));
     c->klevels = L;
+    c->kfused_ok = tgvk_fused_plan(c->g, L, c->device, &c->kfplan);
+    return AMB_OK;
+}
+
+// Level-table slots of w_0 (0..5) and r_0 (12..17), the level fields the
+// fused TGV-k sweep ping-pongs; c->kalt[m] is the other copy of slot KLV_PING[m].
+constexpr int KLV_PING[12] = {0, 1, 2, 3, 4, 5, 12, 13, 14, 15, 16, 17};
+
+amb_status ensure_kalt(amb_context* c) {
+    const size_t fstride = align_up(c->g.n() * 4, 256);
+    const size_t need = fstride * 12;
+    auto main_at = [&](int j) { return reinterpret_cast<float*>(c->kblock + size_t(j) * fstride); };
+    if (need > c->kalt_cap) {
+        for (int m = 0; m < 12; ++m)   // state must be in the level block
+            if (c->klv_host[KLV_PING[m]] != main_at(KLV_PING[m]))
+                return fail(AMB_ERR_NOT_SUPPORTED, "fused TGV-k buffers out of step with the state");
+        if (c->kalt_block) AMB_HIP(hipFree(c->kalt_block));
+        c->kalt_block = nullptr;
+        c->kalt_cap = 0;
+        AMB_HIP(hipMalloc(reinterpret_cast<void**>(&c->kalt_block), need));
+        c->kalt_cap = need;
+    }
+    for (int m = 0; m < 12; ++m) {
+        float* a = reinterpret_cast<float*>(c->kalt_block + size_t(m) * fstride);
+        const int j = KLV_PING[m];
+        c->kalt[m] = c->klv_host[j] == main_at(j) ? a : main_at(j);
+    }
+    return AMB_OK;
+}
+
+// Primal half of TGV-k iteration k + dual half of iteration k + 1 into the
+// second copies, then swap the copies in the host tables.
+amb_status tgvk_fused_step(amb_context* c) {
+    const float* src[N_TGVK_FUSED_FIELDS];
+    float* dst[N_TGVK_FUSED_FIELDS];
+    for (int k = 0; k < N_FUSED_FIELDS; ++k) {
+        src[k] = static_cast<const float*>(c->fields[FUSED_FIELD_MAP[k]]);
+        dst[k] = static_cast<float*>(c->alt[k]);
+    }
+    for (int m = 0; m < 12; ++m) {
+        src[N_FUSED_FIELDS + m] = c->klv_host[KLV_PING[m]];
+        dst[N_FUSED_FIELDS + m] = c->kalt[m];
+    }
+    {
+        ProfScope ps(c, P_TGVK_FUSED);
+        AMB_HIP(launch_tgvk_fused(c->stream, src, dst, c->f, c->sp.active, c->g, c->sp.tau,
+                                  c->sp.sigma, c->sp.lambda, c->sp.theta, c->kalpha_host.data(),
+                                  c->sp.planar, c->kfplan));
+    }
+    for (int k = 0; k < N_FUSED_FIELDS; ++k) std::swap(c->fields[FUSED_FIELD_MAP[k]], c->alt[k]);
+    for (int m = 0; m < 12; ++m) std::swap(c->klv_host[KLV_PING[m]], c->kalt[m]);
+    return AMB_OK;
+}
+
+// Device copies of the field and level tables after fused steps moved the
+// state; synchronises, so the host tables may change again afterwards.
+amb_status tgvk_upload_tables(amb_context* c) {
+    AMB_HIP(hipMemcpyAsync(c->fields_dev, c->fields, sizeof(void*) * N_FIELDS,
+                           hipMemcpyHostToDevice, c->stream));
+    AMB_HIP(hipMemcpyAsync(c->klv_dev, c->klv_host.data(), c->klv_host.size() * sizeof(float*),
+                           hipMemcpyHostToDevice, c->stream));
+    AMB_HIP(hipStreamSynchronize(c->stream));
     return AMB_OK;
 }
 
 // The TGV-k loop: per iteration one dual launch (p, q, every r_i) and one
-// primal launch (u, v, every w_i); callback rule as in run().
+// primal launch (u, v, every w_i); callback rule as in run().  With the fused
+// sweep (one level) the loop runs dual_0, fused(primal_k + dual_k+1) for
+// k = 0 .. N - 2, primal_N-1, and an observed iteration ends with the primal
+// launch alone, exactly as run() does for TGV2.  The dual and primal launches
+// read the state through the device tables, which are brought up to date
+// after fused steps have swapped the copies.
 amb_status tgvk_run(amb_context* c, uint32_t iterations, uint32_t interval,
                     amb_iteration_callback cb, void* user) {
     float* const* F = reinterpret_cast<float* const*>(c->fields_dev);
+    const bool fused = c->kfused_ok && iterations > 1;
+    if (fused) {
+        AMB_TRY(ensure_alt(c));
+        AMB_TRY(ensure_kalt(c));
+        // earlier uploads from the host tables have run before they change
+        AMB_HIP(hipStreamSynchronize(c->stream));
+    }
+    bool need_dual = true, stale = false;
     for (uint32_t it = 0; it < iterations; ++it) {
-        {
+        const bool observed = cb && interval > 0 && it > 0 && it % interval == 0;
+        if (need_dual) {
             ProfScope ps(c, P_TGVK_DUAL);
             AMB_HIP(launch_tgvk_dual(c->stream, F, c->klv_dev, c->kalpha_dev, c->sp.active, c->g,
                                      c->klevels, c->sp.sigma, c->sp.planar));
         }
-        {
+        if (fused && !observed && it + 1 < iterations) {
+            AMB_TRY(tgvk_fused_step(c));
+            need_dual = false;
+            stale = true;
+        } else {
+            if (stale) AMB_TRY(tgvk_upload_tables(c));
+            stale = false;
             ProfScope ps(c, P_TGVK_PRIMAL);
             AMB_HIP(launch_tgvk_primal(c->stream, F, c->klv_dev, c->f, c->sp.active, c->g,
                                        c->klevels, c->sp.tau, c->sp.lambda, c->sp.theta));
+            need_dual = true;
         }
-        if (cb && interval > 0 && it > 0 && it % interval == 0) {
+        if (observed) {
             const size_t n = c->g.n();
             c->cb_u.resize(n);
             AMB_TRY(copy_field_out(c, AMB_MEM_HOST, c->cb_u.data(), F_U));
@@ -834,6 +924,7 @@ AMB_API amb_status amb_context_destroy(amb_context* c) {
     if (c->kblock) (void)hipFree(c->kblock);
     if (c->klv_dev) (void)hipFree(c->klv_dev);
     if (c->kalpha_dev) (void)hipFree(c->kalpha_dev);
+    if (c->kalt_block) (void)hipFree(c->kalt_block);
     if (c->rblock) (void)hipFree(c->rblock);
     (void)hipStreamDestroy(c->stream);
     delete c;

@@ -637,6 +637,365 @@ __global__ __launch_bounds__(KT) void tgvk_primal_fixed(
     }
 }
 
+// ---------------------------------------------------------------------------
+// Fused sweep for one level (k = 3): the primal half of iteration k (u, v, w)
+// and the dual half of iteration k + 1 (p, q, r) in one z-march, with the
+// per-voxel arithmetic of tgvk_primal_fixed<1> and tgvk_dual_fixed<1> in the
+// same order.
+//
+// The dual half at a voxel reads u_bar at its +x, +y, +z neighbours and
+// v_bar, w_bar at its -x, -y, -z neighbours, i.e. primal results of
+// neighbouring voxels.  A block owns whole x rows (one voxel per thread) and a
+// band of `ty` y rows; it computes the primal half also on one row above and
+// below the band and one plane before and after its z chunk, and runs the
+// dual half one plane behind the primal half, so u_bar, v_bar and w_bar never
+// leave the CU.  Neighbouring blocks recompute the halo rows from the same
+// inputs, so the iteration state is read from one copy (src) and written to
+// another (dst).  Per voxel-iteration: reads u, f, v x3, p x3, q x6, w x6,
+// r x6 = 26, writes u, v x3, p x3, q x6, w x6, r x6 = 25: 51 element accesses
+// (204 B) instead of the 86 (344 B) of the two launches.
+//
+// Thread row r holds y = y0 - 1 + r.  Rows 1..ty are owned; row 0 feeds only
+// v_bar and w_bar (the dual half's backward differences), row ty + 1 only
+// u_bar (its forward differences).  Step s computes the primal half at plane
+// s and the dual half at plane s - 1.
+//
+// LDS rows (RT x W floats each): P region, plane s, the dual fields whose x
+// or y neighbours the primal half reads; D region, plane s - 1, u_bar, v_bar
+// and w_bar.  z neighbours stay in registers: p_z and the dual fields of
+// plane s - 1, q and r z-components of plane s + 1 (loaded one plane ahead),
+// v_bar and w_bar z-components of plane s - 2.
+// ---------------------------------------------------------------------------
+enum KFusedField { KF_U = 0, KF_VX, KF_VY, KF_VZ, KF_PX, KF_PY, KF_PZ, KF_QXX, KF_QYY, KF_QZZ,
+                   KF_QXY, KF_QXZ, KF_QYZ, KF_WX, KF_WY, KF_WZ, KF_WXY, KF_WXZ, KF_WYZ, KF_RX,
+                   KF_RY, KF_RZ, KF_RXY, KF_RXZ, KF_RYZ, N_KF };
+static_assert(N_KF == N_TGVK_FUSED_FIELDS, "fused TGV-k field count");
+
+struct KFusedFields {
+    const float* src[N_KF];
+    float* dst[N_KF];
+};
+
+enum KFusedRow { KR_PX = 0, KR_PY, KR_QXX, KR_QXY, KR_QYY, KR_QXZ, KR_QYZ, KR_RX, KR_RY, KR_RXY,
+                 KR_RXZ, KR_RYZ,                                            // P region
+                 KR_UB, KR_VBX, KR_VBY, KR_VBZ, KR_WBX, KR_WBY, KR_WBZ, KR_WBXY, KR_WBXZ,
+                 KR_WBYZ,                                                   // D region
+                 N_KROWS };
+constexpr int KFUSED_MAX_THREADS = 1024;
+
+template <class I>
+__device__ __forceinline__ float kf_ld(const float* p, I i) {
+    if constexpr (sizeof(I) == 4) return LDO(p, uint32_t(i) * 4u);
+    else return p[i];
+}
+template <class I>
+__device__ __forceinline__ void kf_st(float* p, I i, float v) {
+    if constexpr (sizeof(I) == 4)
+        __builtin_nontemporal_store(v, reinterpret_cast<float*>(reinterpret_cast<char*>(p) +
+                                                                uint32_t(i) * 4u));
+    else
+        __builtin_nontemporal_store(v, p + i);
+}
+
+template <class I, int MK>
+__global__ __launch_bounds__(KFUSED_MAX_THREADS) void tgvk_fused_kernel(
+    KFusedFields ff, const float* __restrict__ f_, const uint8_t* __restrict__ act, Geometry g,
+    float tau, float sigma, float lambda, float theta, float a_p, float a_q, float a_r, int ty,
+    int zchunk, int planar) {
+    extern __shared__ __align__(16) float lds[];
+    const int W = g.w, H = g.h, D = g.d;
+    const int RT = ty + 2, RW = RT * W;
+    const int x = int(threadIdx.x) % W, r = int(threadIdx.x) / W;
+    const int nb = (H + ty - 1) / ty;
+    const int t = kxcd_tile(int(blockIdx.x), int(gridDim.x));
+    const int y0 = (t % nb) * ty;
+    const int z0 = (t / nb) * zchunk, z1 = min(z0 + zchunk, D);
+    const int y = y0 - 1 + r;
+    const bool rowin = y >= 0 && y < H;
+    const bool own = rowin && r >= 1 && r <= ty;
+    const bool xp = x > 0, xn = x < W - 1, yp = y > 0, yn = y < H - 1;
+    const I plane = I(W) * I(H);
+    const I col = rowin ? I(y) * I(W) + I(x) : I(0);
+    const float tl = tau * lambda;
+    const int me = r * W + x;
+    auto R = [&](int k) { return lds + k * RW; };
+    const float* const* src = ff.src;
+    float* const* dst = ff.dst;
+
+    // D region starts at zero (read before its first write only into unused carries)
+    for (int k = int(threadIdx.x); k < (N_KROWS - KR_UB) * RW; k += int(blockDim.x))
+        lds[KR_UB * RW + k] = 0.f;
+
+    float p_c[3], q_c[6], r_c[6];     // dual fields at plane s - 1
+    float qn[3], rn[3];               // q_zz, q_xz, q_yz and r_z, r_xz, r_yz at plane s
+    float vb_m2[3], wb_m2[3];         // v_bar, and w_bar z, xz, yz at plane s - 2
+#pragma unroll
+    for (int k = 0; k < 3; ++k) p_c[k] = qn[k] = rn[k] = vb_m2[k] = wb_m2[k] = 0.f;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) q_c[k] = r_c[k] = 0.f;
+    bool act_c = true;
+
+    const int zs = z0 > 0 ? z0 - 1 : 0;
+    if (rowin && zs < D) {
+        const I i = I(zs) * plane + col;
+        qn[0] = kf_ld(src[KF_QZZ], i); qn[1] = kf_ld(src[KF_QXZ], i); qn[2] = kf_ld(src[KF_QYZ], i);
+        rn[0] = kf_ld(src[KF_RZ], i); rn[1] = kf_ld(src[KF_RXZ], i); rn[2] = kf_ld(src[KF_RYZ], i);
+    }
+
+    for (int s = zs; s <= z1; ++s) {
+        const bool pin = s < D;
+        const bool zn = s < D - 1;
+        const I i = I(s) * plane + col;
+        // ---- loads at plane s (q and r z-components at s + 1)
+        float px = 0.f, py = 0.f, pz = 0.f, qxx = 0.f, qxy = 0.f, qyy = 0.f, u = 0.f, fv = 0.f;
+        float v[3] = {0.f, 0.f, 0.f}, qnn[3] = {0.f, 0.f, 0.f}, rnn[3] = {0.f, 0.f, 0.f};
+        float rx = 0.f, ry = 0.f, rxy = 0.f;
+        float w[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+        bool actv = true;
+        if (rowin && pin) {
+            px = kf_ld(src[KF_PX], i); py = kf_ld(src[KF_PY], i); pz = kf_ld(src[KF_PZ], i);
+            qxx = kf_ld(src[KF_QXX], i); qxy = kf_ld(src[KF_QXY], i); qyy = kf_ld(src[KF_QYY], i);
+            u = kf_ld(src[KF_U], i); fv = kf_ld(f_, i);
+#pragma unroll
+            for (int k = 0; k < 3; ++k) v[k] = kf_ld(src[KF_VX + k], i);
+            rx = kf_ld(src[KF_RX], i); ry = kf_ld(src[KF_RY], i); rxy = kf_ld(src[KF_RXY], i);
+#pragma unroll
+            for (int k = 0; k < 6; ++k) w[k] = kf_ld(src[KF_WX + k], i);
+            if (zn) {
+                const I j = i + plane;
+                qnn[0] = kf_ld(src[KF_QZZ], j); qnn[1] = kf_ld(src[KF_QXZ], j);
+                qnn[2] = kf_ld(src[KF_QYZ], j);
+                rnn[0] = kf_ld(src[KF_RZ], j); rnn[1] = kf_ld(src[KF_RXZ], j);
+                rnn[2] = kf_ld(src[KF_RYZ], j);
+            }
+            if (MK) actv = act[i] != 0;
+        }
+
+        // ---- P region: plane s
+        R(KR_PX)[me] = px; R(KR_PY)[me] = py;
+        R(KR_QXX)[me] = qxx; R(KR_QXY)[me] = qxy; R(KR_QYY)[me] = qyy;
+        R(KR_QXZ)[me] = qn[1]; R(KR_QYZ)[me] = qn[2];
+        R(KR_RX)[me] = rx; R(KR_RY)[me] = ry; R(KR_RXY)[me] = rxy;
+        R(KR_RXZ)[me] = rn[1]; R(KR_RYZ)[me] = rn[2];
+        __syncthreads();
+
+        // ---- primal half at plane s
+        const bool need_u = rowin && pin && r > 0 && s >= z0;
+        const bool need_v = rowin && pin && r < RT - 1 && s < z1;
+        float ubn = 0.f, vbn[3] = {0.f, 0.f, 0.f}, wbn[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+        float uo = 0.f, vo[3], wo[6];
+        if (need_u) {
+            // u update, L1 prox, over-relaxation (tgv2_l1.cu:27-60)
+            const float pxm = xp ? R(KR_PX)[me - 1] : 0.f;
+            const float pym = yp ? R(KR_PY)[me - W] : 0.f;
+            const float bx = xp ? -px + pxm : -px;
+            const float by = yp ? -py + pym : -py;
+            const float bz = s > 0 ? -pz + p_c[2] : -pz;
+            const float div = (bx + by) + bz;
+            float un = fmaf(-tau, div, u);
+            const float umf = un - fv;
+            if (umf > tl) un = un - tl;
+            else if (umf < -tl) un = un + tl;
+            else un = fv;
+            if (MK && !actv) un = u;
+            uo = un;
+            ubn = fmaf(theta, un - u, un);
+        }
+        if (need_v) {
+            // div2(q) with forward differences, v update (tgv2_common.cu:279-354)
+            const float qxx_x = xn ? R(KR_QXX)[me + 1] : 0.f;
+            const float qxy_x = xn ? R(KR_QXY)[me + 1] : 0.f;
+            const float qxz_x = xn ? R(KR_QXZ)[me + 1] : 0.f;
+            const float qxy_y = yn ? R(KR_QXY)[me + W] : 0.f;
+            const float qyy_y = yn ? R(KR_QYY)[me + W] : 0.f;
+            const float qyz_y = yn ? R(KR_QYZ)[me + W] : 0.f;
+            const float dxx = xn ? qxx_x - qxx : 0.0f;
+            const float dyxy = yn ? qxy_y - qxy : 0.0f;
+            const float dzxz = zn ? qnn[1] - qn[1] : 0.0f;
+            const float dyy = yn ? qyy_y - qyy : 0.0f;
+            const float dxxy = xn ? qxy_x - qxy : 0.0f;
+            const float dzyz = zn ? qnn[2] - qn[2] : 0.0f;
+            const float dzz = zn ? qnn[0] - qn[0] : 0.0f;
+            const float dxxz = xn ? qxz_x - qn[1] : 0.0f;
+            const float dyyz = yn ? qyz_y - qn[2] : 0.0f;
+            const float w3[3] = {(dxx + dyxy) + dzxz, (dyy + dxxy) + dzyz, (dzz + dxxz) + dyyz};
+            const float p0[3] = {px, py, pz};
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                float vn = fmaf(-tau, w3[k] - p0[k], v[k]);
+                if (MK && !actv) vn = v[k];
+                vo[k] = vn;
+                vbn[k] = fmaf(theta, vn - v[k], vn);
+            }
+            // w from H(r) (forward differences) - q (tgv3_common.cu:362-405,
+            // tgvk_common.cu:138-182)
+            const float rx_x = xn ? R(KR_RX)[me + 1] : 0.f;
+            const float rxy_x = xn ? R(KR_RXY)[me + 1] : 0.f;
+            const float rxz_x = xn ? R(KR_RXZ)[me + 1] : 0.f;
+            const float ry_y = yn ? R(KR_RY)[me + W] : 0.f;
+            const float rxy_y = yn ? R(KR_RXY)[me + W] : 0.f;
+            const float ryz_y = yn ? R(KR_RYZ)[me + W] : 0.f;
+            float h[6];
+            h[0] = xn ? rx_x - rx : 0.0f;
+            h[1] = yn ? ry_y - ry : 0.0f;
+            h[2] = zn ? rnn[0] - rn[0] : 0.0f;
+            h[3] = (xn ? rxy_x - rxy : 0.0f) + (yn ? rxy_y - rxy : 0.0f);
+            h[4] = (xn ? rxz_x - rn[1] : 0.0f) + (zn ? rnn[1] - rn[1] : 0.0f);
+            h[5] = (yn ? ryz_y - rn[2] : 0.0f) + (zn ? rnn[2] - rn[2] : 0.0f);
+            const float q0[6] = {qxx, qyy, qn[0], qxy, qn[1], qn[2]};
+#pragma unroll
+            for (int k = 0; k < 6; ++k) {
+                float wn = fmaf(-tau, h[k] - q0[k], w[k]);
+                if (MK && !actv) wn = w[k];
+                wo[k] = wn;
+                wbn[k] = fmaf(theta, wn - w[k], wn);
+            }
+        }
+        if (own && pin && s >= z0 && s < z1) {
+            kf_st(dst[KF_U], i, uo);
+#pragma unroll
+            for (int k = 0; k < 3; ++k) kf_st(dst[KF_VX + k], i, vo[k]);
+#pragma unroll
+            for (int k = 0; k < 6; ++k) kf_st(dst[KF_WX + k], i, wo[k]);
+        }
+
+        // ---- dual half at plane zd = s - 1 (D region holds its u_bar, v_bar, w_bar)
+        const int zd = s - 1;
+        float vbc[3], wbc[6];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) vbc[k] = R(KR_VBX + k)[me];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) wbc[k] = R(KR_WBX + k)[me];
+        if (own && zd >= z0) {
+            // p update + projection onto |p| <= alpha[0] (tgv2_common.cu:96-131)
+            const float ubc = R(KR_UB)[me];
+            const float uxn = xn ? R(KR_UB)[me + 1] : 0.f;
+            const float uyn = yn ? R(KR_UB)[me + W] : 0.f;
+            const float dx = xn ? uxn - ubc : 0.0f;
+            const float dy = yn ? uyn - ubc : 0.0f;
+            const float dz = zd < D - 1 ? ubn - ubc : 0.0f;
+            float a3[3];
+            a3[0] = fmaf(sigma, dx - vbc[0], p_c[0]);
+            a3[1] = fmaf(sigma, dy - vbc[1], p_c[1]);
+            a3[2] = fmaf(sigma, dz - vbc[2], p_c[2]);
+            float n2 = a3[0] * a3[0];
+            n2 = fmaf(a3[1], a3[1], n2);
+            n2 = fmaf(a3[2], a3[2], n2);
+            const float tp = sqrtf(n2) / a_p;
+            const bool sp = tp > 1.0f;
+
+            // q update with eps_B(v_bar) - w_bar (tgv2_common.cu:174-219, tgv3_common.cu:28-86)
+            float bxv[3], byv[3], bzv[3];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                bxv[k] = xp ? -vbc[k] + R(KR_VBX + k)[me - 1] : -vbc[k];
+                byv[k] = yp ? -vbc[k] + R(KR_VBX + k)[me - W] : -vbc[k];
+                bzv[k] = planar ? 0.0f : (zd > 0 ? -vbc[k] + vb_m2[k] : -vbc[k]);
+            }
+            float e[6];
+            e[0] = bxv[0];
+            e[1] = byv[1];
+            e[2] = bzv[2];
+            e[3] = (bxv[1] + byv[0]) * 0.5f;
+            e[4] = (bxv[2] + bzv[0]) * 0.5f;
+            e[5] = (byv[2] + bzv[1]) * 0.5f;
+            float b6[6];
+#pragma unroll
+            for (int k = 0; k < 6; ++k) b6[k] = fmaf(sigma, e[k] - wbc[k], q_c[k]);
+            float m2 = b6[0] * b6[0];
+            m2 = fmaf(b6[1], b6[1], m2);
+            m2 = fmaf(2.0f * b6[3], b6[3], m2);
+            m2 = fmaf(b6[2], b6[2], m2);
+            m2 = fmaf(2.0f * b6[4], b6[4], m2);
+            m2 = fmaf(2.0f * b6[5], b6[5], m2);
+            const float tq = sqrtf(m2) / a_q;
+            const bool sq = tq > 1.0f;
+
+            // r update from G(w_bar) (backward differences), projection with
+            // alpha[2] (tgv3_common.cu:214-262, 311-360)
+            float gg[6];
+            gg[0] = xp ? -wbc[0] + R(KR_WBX)[me - 1] : -wbc[0];
+            gg[1] = yp ? -wbc[1] + R(KR_WBY)[me - W] : -wbc[1];
+            gg[2] = zd > 0 ? -wbc[2] + wb_m2[0] : -wbc[2];
+            {
+                const float d = xp ? -wbc[3] + R(KR_WBXY)[me - 1] : -wbc[3];
+                const float d2 = yp ? -wbc[3] + R(KR_WBXY)[me - W] : -wbc[3];
+                gg[3] = (d + d2) * 0.5f;
+            }
+            {
+                const float d = xp ? -wbc[4] + R(KR_WBXZ)[me - 1] : -wbc[4];
+                const float d2 = zd > 0 ? -wbc[4] + wb_m2[1] : -wbc[4];
+                gg[4] = (d + d2) * 0.5f;
+            }
+            {
+                const float d = yp ? -wbc[5] + R(KR_WBYZ)[me - W] : -wbc[5];
+                const float d2 = zd > 0 ? -wbc[5] + wb_m2[2] : -wbc[5];
+                gg[5] = (d + d2) * 0.5f;
+            }
+            float rc[6];
+#pragma unroll
+            for (int k = 0; k < 6; ++k) rc[k] = fmaf(sigma, gg[k], r_c[k]);
+            float r2 = rc[0] * rc[0];
+            r2 = fmaf(rc[1], rc[1], r2);
+            r2 = fmaf(2.0f * rc[3], rc[3], r2);
+            r2 = fmaf(rc[2], rc[2], r2);
+            r2 = fmaf(2.0f * rc[4], rc[4], r2);
+            r2 = fmaf(2.0f * rc[5], rc[5], r2);
+            const float tr = sqrtf(r2) / a_r;
+            const bool sr = tr > 1.0f;
+
+            const I id = I(zd) * plane + col;
+            const bool keep = MK && !act_c;   // masked loop: out-of-mask voxels keep their values
+#pragma unroll
+            for (int k = 0; k < 3; ++k)
+                kf_st(dst[KF_PX + k], id, keep ? p_c[k] : (sp ? a3[k] / tp : a3[k]));
+#pragma unroll
+            for (int k = 0; k < 6; ++k)
+                kf_st(dst[KF_QXX + k], id, keep ? q_c[k] : (sq ? b6[k] / tq : b6[k]));
+#pragma unroll
+            for (int k = 0; k < 6; ++k)
+                kf_st(dst[KF_RX + k], id, keep ? r_c[k] : (sr ? rc[k] / tr : rc[k]));
+        }
+        __syncthreads();
+
+        // ---- D region <- plane s; carry plane s to the next step
+        R(KR_UB)[me] = ubn;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) R(KR_VBX + k)[me] = vbn[k];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) R(KR_WBX + k)[me] = wbn[k];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) vb_m2[k] = vbc[k];
+        wb_m2[0] = wbc[2]; wb_m2[1] = wbc[4]; wb_m2[2] = wbc[5];
+        p_c[0] = px; p_c[1] = py; p_c[2] = pz;
+        q_c[0] = qxx; q_c[1] = qyy; q_c[2] = qn[0]; q_c[3] = qxy; q_c[4] = qn[1]; q_c[5] = qn[2];
+        r_c[0] = rx; r_c[1] = ry; r_c[2] = rn[0]; r_c[3] = rxy; r_c[4] = rn[1]; r_c[5] = rn[2];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) { qn[k] = qnn[k]; rn[k] = rnn[k]; }
+        act_c = actv;
+    }
+}
+
+template <class I, int MK>
+hipError_t tgvk_fused_launch(hipStream_t s, const KFusedFields& ff, const float* f,
+                             const uint8_t* act, Geometry g, float tau, float sigma, float lambda,
+                             float theta, const float* alpha, int planar, const TgvkFusedPlan& plan) {
+    const int RT = plan.ty + 2;
+    const int nb = (g.h + plan.ty - 1) / plan.ty, nc = (g.d + plan.zchunk - 1) / plan.zchunk;
+    const size_t lds = size_t(N_KROWS) * RT * g.w * sizeof(float);
+    auto kern = tgvk_fused_kernel<I, MK>;
+    static size_t lds_set = 0;   // per instantiation: raise the dynamic LDS limit once
+    if (lds > lds_set) {
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
+        if (e != hipSuccess) return e;
+        lds_set = lds;
+    }
+    hipLaunchKernelGGL(kern, dim3(nb * nc), dim3(g.w * RT), lds, s, ff, f, act, g, tau, sigma,
+                       lambda, theta, alpha[0], alpha[1], alpha[2], plan.ty, plan.zchunk, planar);
+    return hipGetLastError();
+}
+
 // Planes per block: enough z-chunks for about 4096 blocks (AMB_TGVK_ZCHUNK
 // overrides, for tests of the chunk edges).
 int tgvk_blocks(const Geometry& g, int* zchunk) {
@@ -655,6 +1014,65 @@ int tgvk_blocks(const Geometry& g, int* zchunk) {
 }
 
 }  // namespace
+
+// Plan of the fused TGV-k sweep: one level, blocks of W x (ty + 2) threads
+// (at most 1024, ty >= 1), all 22 LDS rows within the device's limit.  Cost
+// model: rounds of resident blocks (one per CU: 1024 threads hold all its
+// registers) x the rows and planes a block marches, the two halo rows
+// counting half.  AMB_TGVK_FUSED=0 disables the fused sweep; AMB_TGVK_FUSED_TY
+// and AMB_TGVK_FUSED_ZCHUNK override.
+bool tgvk_fused_plan(const Geometry& g, int levels, int device, TgvkFusedPlan* plan) {
+    // 32-bit offsets only (volumes under 2^30 voxels): the 64-bit form spills
+    if (levels != 1 || g.n() * 4 >= (size_t(1) << 32)) return false;
+    const char* en = std::getenv("AMB_TGVK_FUSED");
+    if (en && en[0] == '0') return false;
+    int max_lds = 0, cus = 0;
+    if (hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) !=
+            hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess)
+        return false;
+    auto env_int = [](const char* name) {
+        const char* e = std::getenv(name);
+        return e ? std::atoi(e) : 0;
+    };
+    const int want_ty = env_int("AMB_TGVK_FUSED_TY"), want_zc = env_int("AMB_TGVK_FUSED_ZCHUNK");
+    double best = 1e300;
+    TgvkFusedPlan bp{0, 0};
+    for (int rt = 3; rt * g.w <= KFUSED_MAX_THREADS && rt - 2 <= g.h; ++rt) {
+        const int ty = rt - 2;
+        if (want_ty && ty != want_ty) continue;
+        if (size_t(N_KROWS) * rt * g.w * sizeof(float) > size_t(max_lds)) break;
+        const long nb = (g.h + ty - 1) / ty;
+        for (int nc = 1; nc <= g.d; ++nc) {
+            const int zc = want_zc > 0 ? std::min(want_zc, g.d) : (g.d + nc - 1) / nc;
+            const long blocks = nb * ((g.d + zc - 1) / zc);
+            const long rounds = (blocks + cus - 1) / cus;
+            const double cost = double(rounds) * (ty + 1.0) * (zc + 1);
+            if (cost < best) { best = cost; bp = TgvkFusedPlan{ty, zc}; }
+            if (want_zc > 0) break;
+        }
+    }
+    if (bp.ty == 0) return false;
+    *plan = bp;
+    return true;
+}
+
+hipError_t launch_tgvk_fused(hipStream_t s, const float* const* src, float* const* dst,
+                             const float* f, const uint8_t* active, Geometry g, float tau,
+                             float sigma, float lambda, float theta, const float* alpha,
+                             int planar, const TgvkFusedPlan& plan) {
+    KFusedFields ff;
+    for (int k = 0; k < N_KF; ++k) {
+        ff.src[k] = src[k];
+        ff.dst[k] = dst[k];
+    }
+    if (g.n() * 4 >= (size_t(1) << 32)) return hipErrorInvalidValue;   // see tgvk_fused_plan
+    if (active)
+        return tgvk_fused_launch<uint32_t, 1>(s, ff, f, active, g, tau, sigma, lambda, theta, alpha,
+                                              planar, plan);
+    return tgvk_fused_launch<uint32_t, 0>(s, ff, f, active, g, tau, sigma, lambda, theta, alpha,
+                                          planar, plan);
+}
 
 hipError_t launch_tgvk_dual(hipStream_t s, float* const* fields_dev, float* const* levels_dev,
                             const float* alpha_dev, const uint8_t* active, Geometry g, int levels,

@@ -30,7 +30,7 @@ FIELD_NAMES = ["u", "ubar", "vx", "vy", "vz", "vbx", "vby", "vbz", "px", "py", "
                "qxx", "qyy", "qzz", "qxy", "qxz", "qyz"]
 PROFILE_CLASSES = ["dual_sweep", "primal_sweep", "dct", "poisson", "divergence", "epilogue",
                    "init", "energy", "adjoint", "fused_sweep", "tgvk_dual", "tgvk_primal",
-                   "resize"]
+                   "resize", "tgvk_fused"]
 
 # Every function include/ambient_amd.h declares (checked by the CPU test suite).
 EXPORTS = [

@@ -66,9 +66,34 @@ def tgvk_bytes(kind, levels, n):
       dual    read u_bar, v_bar x3, p x3, q x6, per level w_bar x6, r x6;
               write p x3, q x6, per level r x6                    (22 + 18 L) x 4
       primal  read u, f, p x3, q x6, v x3, per level r x6, w x6;
-              write u, u_bar, v x3, v_bar x3, per level w x6, w_bar x6  (22 + 24 L) x 4"""
-    per = 22 + 18 * levels if kind == "tgvk_dual" else 22 + 24 * levels
+              write u, u_bar, v x3, v_bar x3, per level w x6, w_bar x6  (22 + 24 L) x 4
+      fused   (primal half of one iteration + dual half of the next; u_bar,
+              v_bar, w_bar stay on chip) read u, f, v x3, p x3, q x6, per level
+              w x6, r x6; write u, v x3, p x3, q x6, per level w x6, r x6
+                                                                  (27 + 24 L) x 4"""
+    per = {"tgvk_dual": 22 + 18 * levels, "tgvk_primal": 22 + 24 * levels,
+           "tgvk_fused": 27 + 24 * levels}[kind]
     return 4 * per * n
+
+
+def tgvk_kernels(prof, levels, n):
+    """Per-kernel averages of the TGV-k launches that ran, and the one with the
+    most device time.  For the fused sweep `two_launch_equivalent_gbs` is the
+    two launches' bytes over its time (one fused launch does one iteration's
+    work)."""
+    kernels = {}
+    for name in ("tgvk_dual", "tgvk_primal", "tgvk_fused"):
+        ms, cnt = prof[name]
+        if cnt == 0:
+            continue
+        b = tgvk_bytes(name, levels, n)
+        kernels[name] = {"avg_us": 1000.0 * ms / cnt, "launches": cnt, "bytes_per_launch": b,
+                         "gbs": b / (ms / cnt * 1e-3) / 1e9}
+        if name == "tgvk_fused":
+            b2 = tgvk_bytes("tgvk_dual", levels, n) + tgvk_bytes("tgvk_primal", levels, n)
+            kernels[name]["two_launch_equivalent_gbs"] = b2 / (ms / cnt * 1e-3) / 1e9
+    dom = max(kernels, key=lambda k: kernels[k]["avg_us"] * kernels[k]["launches"])
+    return kernels, dom
 
 
 def sweep_bytes(kind, storage_bytes, n):
@@ -444,13 +469,7 @@ def tgvk3_256(args):
     elapsed, prof = timed(ctx, lambda: ctx.tgvk_deshade_pipeline_device(
         f_dev, shape, p, alpha, u_dev, l_dev, r_dev), args.steps, args.warmup)
     value = float(n) * args.iterations * args.steps / elapsed
-    kernels = {}
-    for name in ("tgvk_dual", "tgvk_primal"):
-        ms, cnt = prof[name]
-        b = tgvk_bytes(name, levels, n)
-        kernels[name] = {"avg_us": 1000.0 * ms / cnt, "launches": cnt, "bytes_per_launch": b,
-                         "gbs": b / (ms / cnt * 1e-3) / 1e9}
-    dom = max(kernels, key=lambda k: kernels[k]["avg_us"])
+    kernels, dom = tgvk_kernels(prof, levels, n)
     achieved = kernels[dom]["gbs"]
     traffic = MEASURED_TRAFFIC.get((dom, levels, n_side))
     cpu = None
@@ -521,13 +540,7 @@ def downsampled_512(args):
     ns = int(np.prod(small))
     value = float(ns) * iters * args.steps / elapsed
     resize_ms, resize_n = prof["resize"]
-    kernels = {}
-    for name in ("tgvk_dual", "tgvk_primal"):
-        ms, cnt = prof[name]
-        b = tgvk_bytes(name, 1, ns)
-        kernels[name] = {"avg_us": 1000.0 * ms / cnt, "launches": cnt, "bytes_per_launch": b,
-                         "gbs": b / (ms / cnt * 1e-3) / 1e9}
-    dom = max(kernels, key=lambda k: kernels[k]["avg_us"])
+    kernels, dom = tgvk_kernels(prof, 1, ns)
     print(json.dumps({
         "metric": "voxel-iters/s (resized volume) downsampled TGV-k deshade 512^3 -> 256^3",
         "value": value, "unit": "voxel-iters/s", "n_gpus": 1, "steps": args.steps,

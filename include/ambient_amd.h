@@ -352,7 +352,8 @@ AMB_API amb_status amb_tgvk_deshade_downsampled(amb_context* ctx, amb_memory mem
  *   0 dual sweep, 1 primal sweep, 2 DCT passes, 3 Poisson divide,
  *   4 divergence, 5 epilogue, 6 init, 7 energy, 8 adjoint (whole call),
  *   9 fused sweep (primal half of one iteration + dual half of the next),
- *   10 TGV-k dual sweep, 11 TGV-k primal sweep, 12 B-spline resampling. */
+ *   10 TGV-k dual sweep, 11 TGV-k primal sweep, 12 B-spline resampling,
+ *   13 fused TGV-k sweep (primal half of one iteration + dual half of the next). */
 AMB_API amb_status amb_profile_enable(amb_context* ctx, int enable);
 AMB_API amb_status amb_profile_read(amb_context* ctx, int kernel_class, double* total_ms,
                                     uint64_t* launches);

@@ -116,6 +116,104 @@ def test_tgvk_launch_counts(ctx):
     ctx.profile(False)
     assert prof["tgvk_dual"][1] == 13 and prof["tgvk_primal"][1] == 13
     assert prof["dual_sweep"][1] == 0 and prof["fused_sweep"][1] == 0
+    assert prof["tgvk_fused"][1] == 0
+
+
+# ---------------------------------------------------------------------------
+# fused sweep (k = 3: primal half of iteration k + dual half of iteration k + 1)
+# ---------------------------------------------------------------------------
+FUSED_SHAPES = [(9, 13, 21), (12, 10, 64), (5, 70, 33), (17, 4, 130), (3, 3, 3), (6, 9, 256),
+                (2, 5, 341)]
+
+
+@pytest.mark.parametrize("shape", FUSED_SHAPES)
+@pytest.mark.parametrize("band,zchunk", [("0", "0"), ("1", "1"), ("2", "3"), ("5", "7")])
+def test_tgvk_fused_bitwise_against_oracle(ctx, orc, monkeypatch, shape, band, zchunk):
+    """Band and chunk edges of the fused sweep: the halo rows and planes, the
+    +z loads one plane ahead and the -z carries.  A band that does not fit
+    (more rows than the volume, or (band + 2) x W above 1024 threads) leaves
+    the two launches."""
+    from ambient_image_processor_amd import native
+    _say("fused tgvk", shape, "band", band, "zchunk", zchunk)
+    d, h, w = shape
+    fits = band == "0" or (int(band) <= h and (int(band) + 2) * w <= 1024)
+    if band != "0":
+        monkeypatch.setenv("AMB_TGVK_FUSED_TY", band)
+    if zchunk != "0":
+        monkeypatch.setenv("AMB_TGVK_FUSED_ZCHUNK", zchunk)
+    f = _f(shape, 30)
+    alpha = (1.0, 2.0, 0.6)
+    ctx.profile(True)
+    ctx.profile_reset()
+    u, vx, vy, vz = ctx.tgvk_deshade(f, _params(native, 18), alpha)
+    prof = ctx.profile_read()
+    ctx.profile(False)
+    assert prof["tgvk_fused"][1] == (17 if fits else 0), prof
+    st = orc.tgvk(f, 18, alpha=alpha, dtype=np.float32, **KPRM)
+    np.testing.assert_array_equal(u, st.u)
+    np.testing.assert_array_equal(vx, st.v[0])
+    np.testing.assert_array_equal(vy, st.v[1])
+    np.testing.assert_array_equal(vz, st.v[2])
+
+
+@pytest.mark.parametrize("shape", [(7, 11, 48), (1, 24, 36)])
+def test_tgvk_fused_equals_two_launches(ctx, monkeypatch, shape):
+    """u and v after the fused loop equal the two-launch loop's, with the
+    projection of r binding (alpha[2] = 0.05), also in the 2-D form."""
+    from ambient_image_processor_amd import native
+    f = _f(shape, 31)
+    alpha = (1.0, 2.0, 0.05)   # a binding projection of r
+    p = _params(native, 40, flags=native.FLAG_PLANAR if shape[0] == 1 else 0)
+    fused = ctx.tgvk_deshade(f, p, alpha)
+    monkeypatch.setenv("AMB_TGVK_FUSED", "0")
+    two = ctx.tgvk_deshade(f, p, alpha)
+    for a, b in zip(fused, two):
+        np.testing.assert_array_equal(a, b)
+
+
+def test_tgvk_fused_callback_and_stop(ctx, orc):
+    """Observed iterations end with the primal launch alone; the state the
+    callback sees and an early stop match the oracle."""
+    from ambient_image_processor_amd import native
+    f = _f((7, 9, 20), 32)
+    alpha = (1.0, 2.0, 0.6)
+    seen = []
+
+    def cb(it, count, u, vx, vy, vz):
+        st = orc.tgvk(f, it + 1, alpha=alpha, dtype=np.float32, **KPRM)
+        seen.append((it, np.array_equal(u, st.u) and np.array_equal(vz, st.v[2])))
+        return it >= 14
+
+    ctx.profile(True)
+    ctx.profile_reset()
+    u, vx, vy, vz = ctx.tgvk_deshade(f, _params(native, 100, callback_interval=7), alpha, cb)
+    prof = ctx.profile_read()
+    ctx.profile(False)
+    assert seen == [(7, True), (14, True)]
+    # iterations 0..14: dual launches at 0, 8 (after the observed 7), primal
+    # launches at 7 and 14, fused steps for the rest
+    assert prof["tgvk_dual"][1] == 2 and prof["tgvk_primal"][1] == 2
+    assert prof["tgvk_fused"][1] == 13
+    st = orc.tgvk(f, 15, alpha=alpha, dtype=np.float32, **KPRM)
+    np.testing.assert_array_equal(u, st.u)
+    np.testing.assert_array_equal(vy, st.v[1])
+
+
+def test_tgvk_fused_wide_rows_fall_back(ctx, orc):
+    """Rows wider than 341 voxels do not fit a 3-row band of 1024 threads: the
+    loop runs the two launches."""
+    from ambient_image_processor_amd import native
+    f = _f((3, 5, 400), 33)
+    alpha = (1.0, 2.0, 0.6)
+    ctx.profile(True)
+    ctx.profile_reset()
+    u, vx, vy, vz = ctx.tgvk_deshade(f, _params(native, 10), alpha)
+    prof = ctx.profile_read()
+    ctx.profile(False)
+    assert prof["tgvk_fused"][1] == 0 and prof["tgvk_dual"][1] == 10
+    st = orc.tgvk(f, 10, alpha=alpha, dtype=np.float32, **KPRM)
+    np.testing.assert_array_equal(u, st.u)
+    np.testing.assert_array_equal(vx, st.v[0])
 
 
 @pytest.mark.parametrize("with_mask", [False, True])
