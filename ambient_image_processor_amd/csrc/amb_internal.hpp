@@ -164,8 +164,11 @@ hipError_t launch_tgvk_primal(hipStream_t s, float* const* fields_dev, float* co
 // dual half of iteration k + 1, reading the 25 fields `src` (order u, vx, vy,
 // vz, px, py, pz, qxx, qyy, qzz, qxy, qxz, qyz, wx, wy, wz, wxy, wxz, wyz, rx,
 // ry, rz, rxy, rxz, ryz) and writing the same fields to `dst`; u_bar, v_bar
-// and w_bar are neither read nor written.  alpha: the three weights (host).
+// and w_bar are neither read nor written.  dst[k] - src[k] must be the same
+// for the first 13 fields and the same for the last 12.  alpha: the three
+// weights (host).
 struct TgvkFusedPlan {
+    int tx;       // owned x columns per block (W: whole rows)
     int ty;       // owned y rows per block
     int zchunk;   // planes per block
 };

@@ -132,7 +132,7 @@ struct amb_context {
     size_t kalt_cap = 0;
     float* kalt[12] = {};
     bool kfused_ok = false;
-    TgvkFusedPlan kfplan{0, 0};
+    TgvkFusedPlan kfplan{0, 0, 0};
     // resampling and the downsampled deshade: one grow-only workspace
     char* rblock = nullptr;
     size_t rcap = 0;
@@ -451,13 +451,15 @@ amb_status begin(amb_context* c, amb_memory mem, const float* f, uint32_t w, uin
     return AMB_OK;
 }
 
-// Second copy of the 13 ping-ponged fields for the fused sweep.  Field k of
+// Second copy of the 13 ping-ponged fields for the fused sweeps.  Field k of
 // the state lives either in the state block or in the second block; c->alt[k]
-// is set to the other one.
+// is set to the other one.  The second block has the state block's layout
+// (field F at F x stride; the u_bar and v_bar slots stay unused), so the two
+// copies of every field are the same distance apart.
 amb_status ensure_alt(amb_context* c) {
     const size_t elem = c->storage == 1 ? 2 : 4;
     const size_t fstride = align_up(c->g.n() * elem, 256);
-    const size_t need = fstride * N_FUSED_FIELDS;
+    const size_t need = fstride * N_FIELDS;
     auto main_at = [&](int k) { return static_cast<void*>(c->block + size_t(FUSED_FIELD_MAP[k]) * fstride); };
     if (need > c->alt_cap) {
         for (int k = 0; k < N_FUSED_FIELDS; ++k)   // state must be in the state block
@@ -470,7 +472,7 @@ amb_status ensure_alt(amb_context* c) {
         c->alt_cap = need;
     }
     for (int k = 0; k < N_FUSED_FIELDS; ++k) {
-        void* a = c->alt_block + size_t(k) * fstride;
+        void* a = c->alt_block + size_t(FUSED_FIELD_MAP[k]) * fstride;
         c->alt[k] = c->fields[FUSED_FIELD_MAP[k]] == main_at(k) ? a : main_at(k);
     }
     return AMB_OK;
@@ -631,11 +633,12 @@ amb_status tgvk_begin(amb_context* c, amb_memory mem, const float* f, uint32_t w
 
 // Level-table slots of w_0 (0..5) and r_0 (12..17), the level fields the
 // fused TGV-k sweep ping-pongs; c->kalt[m] is the other copy of slot KLV_PING[m].
+// The second block has the first level's layout (slot j at j x stride).
 constexpr int KLV_PING[12] = {0, 1, 2, 3, 4, 5, 12, 13, 14, 15, 16, 17};
 
 amb_status ensure_kalt(amb_context* c) {
     const size_t fstride = align_up(c->g.n() * 4, 256);
-    const size_t need = fstride * 12;
+    const size_t need = fstride * 18;
     auto main_at = [&](int j) { return reinterpret_cast<float*>(c->kblock + size_t(j) * fstride); };
     if (need > c->kalt_cap) {
         for (int m = 0; m < 12; ++m)   // state must be in the level block
@@ -648,8 +651,8 @@ amb_status ensure_kalt(amb_context* c) {
         c->kalt_cap = need;
     }
     for (int m = 0; m < 12; ++m) {
-        float* a = reinterpret_cast<float*>(c->kalt_block + size_t(m) * fstride);
         const int j = KLV_PING[m];
+        float* a = reinterpret_cast<float*>(c->kalt_block + size_t(j) * fstride);
         c->kalt[m] = c->klv_host[j] == main_at(j) ? a : main_at(j);
     }
     return AMB_OK;
@@ -667,6 +670,13 @@ amb_status tgvk_fused_step(amb_context* c) {
     for (int m = 0; m < 12; ++m) {
         src[N_FUSED_FIELDS + m] = c->klv_host[KLV_PING[m]];
         dst[N_FUSED_FIELDS + m] = c->kalt[m];
+    }
+    // every field's other copy lies at one distance per block (see ensure_alt, ensure_kalt)
+    for (int k = 1; k < N_TGVK_FUSED_FIELDS; ++k) {
+        const int k0 = k < N_FUSED_FIELDS ? 0 : N_FUSED_FIELDS;
+        if (reinterpret_cast<const char*>(dst[k]) - reinterpret_cast<const char*>(src[k]) !=
+            reinterpret_cast<const char*>(dst[k0]) - reinterpret_cast<const char*>(src[k0]))
+            return fail(AMB_ERR_NOT_SUPPORTED, "fused TGV-k buffers out of step with the state");
     }
     {
         ProfScope ps(c, P_TGVK_FUSED);

@@ -645,22 +645,24 @@ __global__ __launch_bounds__(KT) void tgvk_primal_fixed(
 //
 // The dual half at a voxel reads u_bar at its +x, +y, +z neighbours and
 // v_bar, w_bar at its -x, -y, -z neighbours, i.e. primal results of
-// neighbouring voxels.  A block owns whole x rows (one voxel per thread) and a
-// band of `ty` y rows; it computes the primal half also on one row above and
-// below the band and one plane before and after its z chunk, and runs the
-// dual half one plane behind the primal half, so u_bar, v_bar and w_bar never
-// leave the CU.  Neighbouring blocks recompute the halo rows from the same
+// neighbouring voxels.  A block owns a tile of `tx` x columns (one voxel per
+// thread; tx = W: whole rows) and `ty` y rows; it computes the primal half
+// also on one column left and right of the tile (tx < W), one row above and
+// below it and one plane before and after its z chunk, and runs the dual half
+// one plane behind the primal half, so u_bar, v_bar and w_bar never leave the
+// CU.  Neighbouring blocks recompute the halo rows from the same
 // inputs, so the iteration state is read from one copy (src) and written to
 // another (dst).  Per voxel-iteration: reads u, f, v x3, p x3, q x6, w x6,
 // r x6 = 26, writes u, v x3, p x3, q x6, w x6, r x6 = 25: 51 element accesses
 // (204 B) instead of the 86 (344 B) of the two launches.
 //
-// Thread row r holds y = y0 - 1 + r.  Rows 1..ty are owned; row 0 feeds only
-// v_bar and w_bar (the dual half's backward differences), row ty + 1 only
-// u_bar (its forward differences).  Step s computes the primal half at plane
-// s and the dual half at plane s - 1.
+// Thread (c, r) holds y = y0 - 1 + r and x = x0 - 1 + c (tiled) or x = c.
+// Rows 1..ty and columns 1..tx are owned; row and column 0 feed only v_bar and
+// w_bar (the dual half's backward differences), row ty + 1 and column tx + 1
+// only u_bar (its forward differences).  Step s computes the primal half at
+// plane s and the dual half at plane s - 1.
 //
-// LDS rows (RT x W floats each): P region, plane s, the dual fields whose x
+// LDS arrays (one float per thread each): P region, plane s, the dual fields whose x
 // or y neighbours the primal half reads; D region, plane s - 1, u_bar, v_bar
 // and w_bar.  z neighbours stay in registers: p_z and the dual fields of
 // plane s - 1, q and r z-components of plane s + 1 (loaded one plane ahead),
@@ -671,9 +673,12 @@ enum KFusedField { KF_U = 0, KF_VX, KF_VY, KF_VZ, KF_PX, KF_PY, KF_PZ, KF_QXX, K
                    KF_RY, KF_RZ, KF_RXY, KF_RXZ, KF_RYZ, N_KF };
 static_assert(N_KF == N_TGVK_FUSED_FIELDS, "fused TGV-k field count");
 
+// dst field k = src field k + dt bytes (k < 13) or + dl bytes (k >= 13): the
+// two copies of the TGV2 fields and of the level fields are one distance
+// apart, which keeps the 25 destination pointers out of the scalar registers.
 struct KFusedFields {
     const float* src[N_KF];
-    float* dst[N_KF];
+    long long dt, dl;
 };
 
 enum KFusedRow { KR_PX = 0, KR_PY, KR_QXX, KR_QXY, KR_QYY, KR_QXZ, KR_QYZ, KR_RX, KR_RY, KR_RXY,
@@ -700,27 +705,36 @@ __device__ __forceinline__ void kf_st(float* p, I i, float v) {
 template <class I, int MK>
 __global__ __launch_bounds__(KFUSED_MAX_THREADS) void tgvk_fused_kernel(
     KFusedFields ff, const float* __restrict__ f_, const uint8_t* __restrict__ act, Geometry g,
-    float tau, float sigma, float lambda, float theta, float a_p, float a_q, float a_r, int ty,
-    int zchunk, int planar) {
+    float tau, float sigma, float lambda, float theta, float a_p, float a_q, float a_r, int tx,
+    int ty, int zchunk, int planar) {
     extern __shared__ __align__(16) float lds[];
     const int W = g.w, H = g.h, D = g.d;
-    const int RT = ty + 2, RW = RT * W;
-    const int x = int(threadIdx.x) % W, r = int(threadIdx.x) / W;
-    const int nb = (H + ty - 1) / ty;
+    // tx < W: tiles of tx columns with one halo column on each side
+    const bool tiled = tx < W;
+    const int TXH = tiled ? tx + 2 : W, RT = ty + 2, RW = RT * TXH;
+    const int c = int(threadIdx.x) % TXH, r = int(threadIdx.x) / TXH;
+    const int nbx = (W + tx - 1) / tx, nb = (H + ty - 1) / ty;
     const int t = kxcd_tile(int(blockIdx.x), int(gridDim.x));
-    const int y0 = (t % nb) * ty;
-    const int z0 = (t / nb) * zchunk, z1 = min(z0 + zchunk, D);
+    const int tb = t / nbx;
+    const int x = tiled ? (t % nbx) * tx - 1 + c : c;
+    const int y0 = (tb % nb) * ty;
+    const int z0 = (tb / nb) * zchunk, z1 = min(z0 + zchunk, D);
     const int y = y0 - 1 + r;
-    const bool rowin = y >= 0 && y < H;
-    const bool own = rowin && r >= 1 && r <= ty;
+    const bool in = y >= 0 && y < H && x >= 0 && x < W;
+    // column 0 feeds only v_bar and w_bar, column tx + 1 only u_bar (tiled)
+    const bool cl = !tiled || c > 0, cr = !tiled || c < TXH - 1;
+    const bool own = in && r >= 1 && r <= ty && cl && cr;
     const bool xp = x > 0, xn = x < W - 1, yp = y > 0, yn = y < H - 1;
     const I plane = I(W) * I(H);
-    const I col = rowin ? I(y) * I(W) + I(x) : I(0);
+    const I col = in ? I(y) * I(W) + I(x) : I(0);
     const float tl = tau * lambda;
-    const int me = r * W + x;
+    const int me = r * TXH + c;
     auto R = [&](int k) { return lds + k * RW; };
     const float* const* src = ff.src;
-    float* const* dst = ff.dst;
+    auto dst = [&](int k) {
+        return reinterpret_cast<float*>(const_cast<char*>(reinterpret_cast<const char*>(src[k])) +
+                                        (k < KF_WX ? ff.dt : ff.dl));
+    };
 
     // D region starts at zero (read before its first write only into unused carries)
     for (int k = int(threadIdx.x); k < (N_KROWS - KR_UB) * RW; k += int(blockDim.x))
@@ -736,7 +750,7 @@ __global__ __launch_bounds__(KFUSED_MAX_THREADS) void tgvk_fused_kernel(
     bool act_c = true;
 
     const int zs = z0 > 0 ? z0 - 1 : 0;
-    if (rowin && zs < D) {
+    if (in && zs < D) {
         const I i = I(zs) * plane + col;
         qn[0] = kf_ld(src[KF_QZZ], i); qn[1] = kf_ld(src[KF_QXZ], i); qn[2] = kf_ld(src[KF_QYZ], i);
         rn[0] = kf_ld(src[KF_RZ], i); rn[1] = kf_ld(src[KF_RXZ], i); rn[2] = kf_ld(src[KF_RYZ], i);
@@ -752,7 +766,7 @@ __global__ __launch_bounds__(KFUSED_MAX_THREADS) void tgvk_fused_kernel(
         float rx = 0.f, ry = 0.f, rxy = 0.f;
         float w[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
         bool actv = true;
-        if (rowin && pin) {
+        if (in && pin) {
             px = kf_ld(src[KF_PX], i); py = kf_ld(src[KF_PY], i); pz = kf_ld(src[KF_PZ], i);
             qxx = kf_ld(src[KF_QXX], i); qxy = kf_ld(src[KF_QXY], i); qyy = kf_ld(src[KF_QYY], i);
             u = kf_ld(src[KF_U], i); fv = kf_ld(f_, i);
@@ -780,14 +794,14 @@ __global__ __launch_bounds__(KFUSED_MAX_THREADS) void tgvk_fused_kernel(
         __syncthreads();
 
         // ---- primal half at plane s
-        const bool need_u = rowin && pin && r > 0 && s >= z0;
-        const bool need_v = rowin && pin && r < RT - 1 && s < z1;
+        const bool need_u = in && pin && r > 0 && cl && s >= z0;
+        const bool need_v = in && pin && r < RT - 1 && cr && s < z1;
         float ubn = 0.f, vbn[3] = {0.f, 0.f, 0.f}, wbn[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
         float uo = 0.f, vo[3], wo[6];
         if (need_u) {
             // u update, L1 prox, over-relaxation (tgv2_l1.cu:27-60)
             const float pxm = xp ? R(KR_PX)[me - 1] : 0.f;
-            const float pym = yp ? R(KR_PY)[me - W] : 0.f;
+            const float pym = yp ? R(KR_PY)[me - TXH] : 0.f;
             const float bx = xp ? -px + pxm : -px;
             const float by = yp ? -py + pym : -py;
             const float bz = s > 0 ? -pz + p_c[2] : -pz;
@@ -806,9 +820,9 @@ __global__ __launch_bounds__(KFUSED_MAX_THREADS) void tgvk_fused_kernel(
             const float qxx_x = xn ? R(KR_QXX)[me + 1] : 0.f;
             const float qxy_x = xn ? R(KR_QXY)[me + 1] : 0.f;
             const float qxz_x = xn ? R(KR_QXZ)[me + 1] : 0.f;
-            const float qxy_y = yn ? R(KR_QXY)[me + W] : 0.f;
-            const float qyy_y = yn ? R(KR_QYY)[me + W] : 0.f;
-            const float qyz_y = yn ? R(KR_QYZ)[me + W] : 0.f;
+            const float qxy_y = yn ? R(KR_QXY)[me + TXH] : 0.f;
+            const float qyy_y = yn ? R(KR_QYY)[me + TXH] : 0.f;
+            const float qyz_y = yn ? R(KR_QYZ)[me + TXH] : 0.f;
             const float dxx = xn ? qxx_x - qxx : 0.0f;
             const float dyxy = yn ? qxy_y - qxy : 0.0f;
             const float dzxz = zn ? qnn[1] - qn[1] : 0.0f;
@@ -832,9 +846,9 @@ __global__ __launch_bounds__(KFUSED_MAX_THREADS) void tgvk_fused_kernel(
             const float rx_x = xn ? R(KR_RX)[me + 1] : 0.f;
             const float rxy_x = xn ? R(KR_RXY)[me + 1] : 0.f;
             const float rxz_x = xn ? R(KR_RXZ)[me + 1] : 0.f;
-            const float ry_y = yn ? R(KR_RY)[me + W] : 0.f;
-            const float rxy_y = yn ? R(KR_RXY)[me + W] : 0.f;
-            const float ryz_y = yn ? R(KR_RYZ)[me + W] : 0.f;
+            const float ry_y = yn ? R(KR_RY)[me + TXH] : 0.f;
+            const float rxy_y = yn ? R(KR_RXY)[me + TXH] : 0.f;
+            const float ryz_y = yn ? R(KR_RYZ)[me + TXH] : 0.f;
             float h[6];
             h[0] = xn ? rx_x - rx : 0.0f;
             h[1] = yn ? ry_y - ry : 0.0f;
@@ -852,11 +866,11 @@ __global__ __launch_bounds__(KFUSED_MAX_THREADS) void tgvk_fused_kernel(
             }
         }
         if (own && pin && s >= z0 && s < z1) {
-            kf_st(dst[KF_U], i, uo);
+            kf_st(dst(KF_U), i, uo);
 #pragma unroll
-            for (int k = 0; k < 3; ++k) kf_st(dst[KF_VX + k], i, vo[k]);
+            for (int k = 0; k < 3; ++k) kf_st(dst(KF_VX + k), i, vo[k]);
 #pragma unroll
-            for (int k = 0; k < 6; ++k) kf_st(dst[KF_WX + k], i, wo[k]);
+            for (int k = 0; k < 6; ++k) kf_st(dst(KF_WX + k), i, wo[k]);
         }
 
         // ---- dual half at plane zd = s - 1 (D region holds its u_bar, v_bar, w_bar)
@@ -870,7 +884,7 @@ __global__ __launch_bounds__(KFUSED_MAX_THREADS) void tgvk_fused_kernel(
             // p update + projection onto |p| <= alpha[0] (tgv2_common.cu:96-131)
             const float ubc = R(KR_UB)[me];
             const float uxn = xn ? R(KR_UB)[me + 1] : 0.f;
-            const float uyn = yn ? R(KR_UB)[me + W] : 0.f;
+            const float uyn = yn ? R(KR_UB)[me + TXH] : 0.f;
             const float dx = xn ? uxn - ubc : 0.0f;
             const float dy = yn ? uyn - ubc : 0.0f;
             const float dz = zd < D - 1 ? ubn - ubc : 0.0f;
@@ -882,14 +896,17 @@ __global__ __launch_bounds__(KFUSED_MAX_THREADS) void tgvk_fused_kernel(
             n2 = fmaf(a3[1], a3[1], n2);
             n2 = fmaf(a3[2], a3[2], n2);
             const float tp = sqrtf(n2) / a_p;
-            const bool sp = tp > 1.0f;
+            if (tp > 1.0f) {   // wave-uniform branch: the divisions only where a lane projects
+#pragma unroll
+                for (int k = 0; k < 3; ++k) a3[k] = a3[k] / tp;
+            }
 
             // q update with eps_B(v_bar) - w_bar (tgv2_common.cu:174-219, tgv3_common.cu:28-86)
             float bxv[3], byv[3], bzv[3];
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
                 bxv[k] = xp ? -vbc[k] + R(KR_VBX + k)[me - 1] : -vbc[k];
-                byv[k] = yp ? -vbc[k] + R(KR_VBX + k)[me - W] : -vbc[k];
+                byv[k] = yp ? -vbc[k] + R(KR_VBX + k)[me - TXH] : -vbc[k];
                 bzv[k] = planar ? 0.0f : (zd > 0 ? -vbc[k] + vb_m2[k] : -vbc[k]);
             }
             float e[6];
@@ -909,17 +926,20 @@ __global__ __launch_bounds__(KFUSED_MAX_THREADS) void tgvk_fused_kernel(
             m2 = fmaf(2.0f * b6[4], b6[4], m2);
             m2 = fmaf(2.0f * b6[5], b6[5], m2);
             const float tq = sqrtf(m2) / a_q;
-            const bool sq = tq > 1.0f;
+            if (tq > 1.0f) {
+#pragma unroll
+                for (int k = 0; k < 6; ++k) b6[k] = b6[k] / tq;
+            }
 
             // r update from G(w_bar) (backward differences), projection with
             // alpha[2] (tgv3_common.cu:214-262, 311-360)
             float gg[6];
             gg[0] = xp ? -wbc[0] + R(KR_WBX)[me - 1] : -wbc[0];
-            gg[1] = yp ? -wbc[1] + R(KR_WBY)[me - W] : -wbc[1];
+            gg[1] = yp ? -wbc[1] + R(KR_WBY)[me - TXH] : -wbc[1];
             gg[2] = zd > 0 ? -wbc[2] + wb_m2[0] : -wbc[2];
             {
                 const float d = xp ? -wbc[3] + R(KR_WBXY)[me - 1] : -wbc[3];
-                const float d2 = yp ? -wbc[3] + R(KR_WBXY)[me - W] : -wbc[3];
+                const float d2 = yp ? -wbc[3] + R(KR_WBXY)[me - TXH] : -wbc[3];
                 gg[3] = (d + d2) * 0.5f;
             }
             {
@@ -928,7 +948,7 @@ __global__ __launch_bounds__(KFUSED_MAX_THREADS) void tgvk_fused_kernel(
                 gg[4] = (d + d2) * 0.5f;
             }
             {
-                const float d = yp ? -wbc[5] + R(KR_WBYZ)[me - W] : -wbc[5];
+                const float d = yp ? -wbc[5] + R(KR_WBYZ)[me - TXH] : -wbc[5];
                 const float d2 = zd > 0 ? -wbc[5] + wb_m2[2] : -wbc[5];
                 gg[5] = (d + d2) * 0.5f;
             }
@@ -942,19 +962,19 @@ __global__ __launch_bounds__(KFUSED_MAX_THREADS) void tgvk_fused_kernel(
             r2 = fmaf(2.0f * rc[4], rc[4], r2);
             r2 = fmaf(2.0f * rc[5], rc[5], r2);
             const float tr = sqrtf(r2) / a_r;
-            const bool sr = tr > 1.0f;
+            if (tr > 1.0f) {
+#pragma unroll
+                for (int k = 0; k < 6; ++k) rc[k] = rc[k] / tr;
+            }
 
             const I id = I(zd) * plane + col;
             const bool keep = MK && !act_c;   // masked loop: out-of-mask voxels keep their values
 #pragma unroll
-            for (int k = 0; k < 3; ++k)
-                kf_st(dst[KF_PX + k], id, keep ? p_c[k] : (sp ? a3[k] / tp : a3[k]));
+            for (int k = 0; k < 3; ++k) kf_st(dst(KF_PX + k), id, keep ? p_c[k] : a3[k]);
 #pragma unroll
-            for (int k = 0; k < 6; ++k)
-                kf_st(dst[KF_QXX + k], id, keep ? q_c[k] : (sq ? b6[k] / tq : b6[k]));
+            for (int k = 0; k < 6; ++k) kf_st(dst(KF_QXX + k), id, keep ? q_c[k] : b6[k]);
 #pragma unroll
-            for (int k = 0; k < 6; ++k)
-                kf_st(dst[KF_RX + k], id, keep ? r_c[k] : (sr ? rc[k] / tr : rc[k]));
+            for (int k = 0; k < 6; ++k) kf_st(dst(KF_RX + k), id, keep ? r_c[k] : rc[k]);
         }
         __syncthreads();
 
@@ -980,9 +1000,10 @@ template <class I, int MK>
 hipError_t tgvk_fused_launch(hipStream_t s, const KFusedFields& ff, const float* f,
                              const uint8_t* act, Geometry g, float tau, float sigma, float lambda,
                              float theta, const float* alpha, int planar, const TgvkFusedPlan& plan) {
-    const int RT = plan.ty + 2;
+    const int RT = plan.ty + 2, TXH = plan.tx < g.w ? plan.tx + 2 : g.w;
+    const int nbx = (g.w + plan.tx - 1) / plan.tx;
     const int nb = (g.h + plan.ty - 1) / plan.ty, nc = (g.d + plan.zchunk - 1) / plan.zchunk;
-    const size_t lds = size_t(N_KROWS) * RT * g.w * sizeof(float);
+    const size_t lds = size_t(N_KROWS) * RT * TXH * sizeof(float);
     auto kern = tgvk_fused_kernel<I, MK>;
     static size_t lds_set = 0;   // per instantiation: raise the dynamic LDS limit once
     if (lds > lds_set) {
@@ -991,8 +1012,9 @@ hipError_t tgvk_fused_launch(hipStream_t s, const KFusedFields& ff, const float*
         if (e != hipSuccess) return e;
         lds_set = lds;
     }
-    hipLaunchKernelGGL(kern, dim3(nb * nc), dim3(g.w * RT), lds, s, ff, f, act, g, tau, sigma,
-                       lambda, theta, alpha[0], alpha[1], alpha[2], plan.ty, plan.zchunk, planar);
+    hipLaunchKernelGGL(kern, dim3(nbx * nb * nc), dim3(TXH * RT), lds, s, ff, f, act, g, tau,
+                       sigma, lambda, theta, alpha[0], alpha[1], alpha[2], plan.tx, plan.ty,
+                       plan.zchunk, planar);
     return hipGetLastError();
 }
 
@@ -1015,12 +1037,18 @@ int tgvk_blocks(const Geometry& g, int* zchunk) {
 
 }  // namespace
 
-// Plan of the fused TGV-k sweep: one level, blocks of W x (ty + 2) threads
-// (at most 1024, ty >= 1), all 22 LDS rows within the device's limit.  Cost
-// model: rounds of resident blocks (one per CU: 1024 threads hold all its
-// registers) x the rows and planes a block marches, the two halo rows
-// counting half.  AMB_TGVK_FUSED=0 disables the fused sweep; AMB_TGVK_FUSED_TY
-// and AMB_TGVK_FUSED_ZCHUNK override.
+// Plan of the fused TGV-k sweep (one level): blocks of up to 1024 threads
+// (one per CU: they hold all its registers), all 22 LDS arrays within the
+// device's limit.  Whole rows (tx = W) whenever a band of at least one row
+// fits (W <= 341), with the tallest band that fits: rows then start on cache
+// lines and halo rows fall on whole waves, which skip the dual half.  Wider
+// rows run in tiles of at most 128 columns plus a halo column on each side.
+// Measured on MI355X at 256^3: whole rows, band 2, 803 us per launch; 64- and
+// 128-column tiles with bands of 4 to 13 rows, 1027-1385 us (their rows are
+// 264- or 520-B pieces off cache-line boundaries, and every wave then holds
+// owned lanes and runs the dual half).  The z chunk minimises rounds of
+// resident blocks x (planes + 1).  AMB_TGVK_FUSED=0 disables the fused
+// sweep; AMB_TGVK_FUSED_TX, _TY and _ZCHUNK override.
 bool tgvk_fused_plan(const Geometry& g, int levels, int device, TgvkFusedPlan* plan) {
     // 32-bit offsets only (volumes under 2^30 voxels): the 64-bit form spills
     if (levels != 1 || g.n() * 4 >= (size_t(1) << 32)) return false;
@@ -1035,25 +1063,30 @@ bool tgvk_fused_plan(const Geometry& g, int levels, int device, TgvkFusedPlan* p
         const char* e = std::getenv(name);
         return e ? std::atoi(e) : 0;
     };
-    const int want_ty = env_int("AMB_TGVK_FUSED_TY"), want_zc = env_int("AMB_TGVK_FUSED_ZCHUNK");
-    double best = 1e300;
-    TgvkFusedPlan bp{0, 0};
-    for (int rt = 3; rt * g.w <= KFUSED_MAX_THREADS && rt - 2 <= g.h; ++rt) {
-        const int ty = rt - 2;
-        if (want_ty && ty != want_ty) continue;
-        if (size_t(N_KROWS) * rt * g.w * sizeof(float) > size_t(max_lds)) break;
-        const long nb = (g.h + ty - 1) / ty;
-        for (int nc = 1; nc <= g.d; ++nc) {
-            const int zc = want_zc > 0 ? std::min(want_zc, g.d) : (g.d + nc - 1) / nc;
-            const long blocks = nb * ((g.d + zc - 1) / zc);
-            const long rounds = (blocks + cus - 1) / cus;
-            const double cost = double(rounds) * (ty + 1.0) * (zc + 1);
-            if (cost < best) { best = cost; bp = TgvkFusedPlan{ty, zc}; }
-            if (want_zc > 0) break;
-        }
+    const int want_tx = env_int("AMB_TGVK_FUSED_TX"), want_ty = env_int("AMB_TGVK_FUSED_TY");
+    const int want_zc = env_int("AMB_TGVK_FUSED_ZCHUNK");
+    int tx = g.w;
+    if (want_tx > 0) tx = std::min(want_tx, g.w);
+    else if (3 * g.w > KFUSED_MAX_THREADS) tx = (g.w + (g.w + 127) / 128 - 1) / ((g.w + 127) / 128);
+    const int txh = tx < g.w ? tx + 2 : g.w;
+    int ty = std::min(KFUSED_MAX_THREADS / txh - 2, g.h);
+    while (ty >= 1 && size_t(N_KROWS) * (ty + 2) * txh * sizeof(float) > size_t(max_lds)) --ty;
+    if (want_ty > 0) {
+        if (want_ty > ty) return false;   // does not fit
+        ty = want_ty;
     }
-    if (bp.ty == 0) return false;
-    *plan = bp;
+    if (ty < 1) return false;
+    const long nb = long((g.w + tx - 1) / tx) * ((g.h + ty - 1) / ty);
+    int best_zc = g.d;
+    long best = -1;
+    for (int nc = 1; nc <= g.d; ++nc) {
+        const int zc = (g.d + nc - 1) / nc;
+        const long rounds = (nb * ((g.d + zc - 1) / zc) + cus - 1) / cus;
+        const long cost = rounds * (zc + 1);
+        if (best < 0 || cost < best) { best = cost; best_zc = zc; }
+    }
+    if (want_zc > 0) best_zc = std::min(want_zc, g.d);
+    *plan = TgvkFusedPlan{tx, ty, best_zc};
     return true;
 }
 
@@ -1062,10 +1095,9 @@ hipError_t launch_tgvk_fused(hipStream_t s, const float* const* src, float* cons
                              float sigma, float lambda, float theta, const float* alpha,
                              int planar, const TgvkFusedPlan& plan) {
     KFusedFields ff;
-    for (int k = 0; k < N_KF; ++k) {
-        ff.src[k] = src[k];
-        ff.dst[k] = dst[k];
-    }
+    for (int k = 0; k < N_KF; ++k) ff.src[k] = src[k];
+    ff.dt = reinterpret_cast<const char*>(dst[0]) - reinterpret_cast<const char*>(src[0]);
+    ff.dl = reinterpret_cast<const char*>(dst[KF_WX]) - reinterpret_cast<const char*>(src[KF_WX]);
     if (g.n() * 4 >= (size_t(1) << 32)) return hipErrorInvalidValue;   // see tgvk_fused_plan
     if (active)
         return tgvk_fused_launch<uint32_t, 1>(s, ff, f, active, g, tau, sigma, lambda, theta, alpha,

@@ -127,19 +127,24 @@ FUSED_SHAPES = [(9, 13, 21), (12, 10, 64), (5, 70, 33), (17, 4, 130), (3, 3, 3),
 
 
 @pytest.mark.parametrize("shape", FUSED_SHAPES)
-@pytest.mark.parametrize("band,zchunk", [("0", "0"), ("1", "1"), ("2", "3"), ("5", "7")])
-def test_tgvk_fused_bitwise_against_oracle(ctx, orc, monkeypatch, shape, band, zchunk):
-    """Band and chunk edges of the fused sweep: the halo rows and planes, the
-    +z loads one plane ahead and the -z carries.  A band that does not fit
-    (more rows than the volume, or (band + 2) x W above 1024 threads) leaves
-    the two launches."""
+@pytest.mark.parametrize("tile,band,zchunk", [("0", "0", "0"), ("1000", "1", "1"),
+                                              ("1000", "2", "3"), ("8", "1", "1"),
+                                              ("16", "2", "3"), ("64", "5", "7"),
+                                              ("30", "13", "2")])
+def test_tgvk_fused_bitwise_against_oracle(ctx, orc, monkeypatch, shape, tile, band, zchunk):
+    """Tile, band and chunk edges of the fused sweep: the halo columns, rows
+    and planes, the +z loads one plane ahead and the -z carries (tile 1000:
+    whole rows).  A tile that does not fit (more rows than the volume, or more
+    than 1024 threads) leaves the two launches."""
     from ambient_image_processor_amd import native
-    _say("fused tgvk", shape, "band", band, "zchunk", zchunk)
+    _say("fused tgvk", shape, "tile", tile, "band", band, "zchunk", zchunk)
     d, h, w = shape
-    fits = band == "0" or (int(band) <= h and (int(band) + 2) * w <= 1024)
-    if band != "0":
+    tx = min(int(tile), w)
+    txh = tx + 2 if tx < w else w
+    fits = tile == "0" or (int(band) <= h and (int(band) + 2) * txh <= 1024)
+    if tile != "0":
+        monkeypatch.setenv("AMB_TGVK_FUSED_TX", tile)
         monkeypatch.setenv("AMB_TGVK_FUSED_TY", band)
-    if zchunk != "0":
         monkeypatch.setenv("AMB_TGVK_FUSED_ZCHUNK", zchunk)
     f = _f(shape, 30)
     alpha = (1.0, 2.0, 0.6)
@@ -199,19 +204,27 @@ def test_tgvk_fused_callback_and_stop(ctx, orc):
     np.testing.assert_array_equal(vy, st.v[1])
 
 
-def test_tgvk_fused_wide_rows_fall_back(ctx, orc):
-    """Rows wider than 341 voxels do not fit a 3-row band of 1024 threads: the
-    loop runs the two launches."""
+def test_tgvk_fused_wide_rows_and_fallback(ctx, orc, monkeypatch):
+    """Rows wider than 341 voxels run in x tiles; whole rows of 400 voxels do
+    not fit a 3-row band of 1024 threads, so that forced tile leaves the two
+    launches."""
     from ambient_image_processor_amd import native
     f = _f((3, 5, 400), 33)
     alpha = (1.0, 2.0, 0.6)
+    st = orc.tgvk(f, 10, alpha=alpha, dtype=np.float32, **KPRM)
     ctx.profile(True)
+    ctx.profile_reset()
+    u, vx, vy, vz = ctx.tgvk_deshade(f, _params(native, 10), alpha)
+    prof = ctx.profile_read()
+    assert prof["tgvk_fused"][1] == 9 and prof["tgvk_dual"][1] == 1
+    np.testing.assert_array_equal(u, st.u)
+    np.testing.assert_array_equal(vx, st.v[0])
+    monkeypatch.setenv("AMB_TGVK_FUSED_TX", "400")
     ctx.profile_reset()
     u, vx, vy, vz = ctx.tgvk_deshade(f, _params(native, 10), alpha)
     prof = ctx.profile_read()
     ctx.profile(False)
     assert prof["tgvk_fused"][1] == 0 and prof["tgvk_dual"][1] == 10
-    st = orc.tgvk(f, 10, alpha=alpha, dtype=np.float32, **KPRM)
     np.testing.assert_array_equal(u, st.u)
     np.testing.assert_array_equal(vx, st.v[0])
 
