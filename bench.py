@@ -57,8 +57,9 @@ SWEEPS = ("dual_sweep", "primal_sweep", "fused_sweep")
 MEASURED_TRAFFIC = {("fused_sweep", 4, 256): 1845865369,
                     ("dual_sweep", 4, 256): 1532669905,
                     ("primal_sweep", 4, 256): 1533132892,
-                    ("tgvk_dual", 1, 256): 2908351910,
-                    ("tgvk_primal", 1, 256): 3564456435}
+                    ("tgvk_dual", 1, 256): 2908697085,
+                    ("tgvk_primal", 1, 256): 3563855669,
+                    ("tgvk_fused", 1, 256): 3479840059}
 
 
 def tgvk_bytes(kind, levels, n):
