@@ -1,7 +1,8 @@
 """GPU tests at the benchmark configurations' full sizes, through checks whose
 meaning does not depend on the size: the first iterations against the FP32
-oracle bit for bit, schedule and masking equivalences bit for bit, the dual
-feasibility and the decrease of the FP64 primal-dual gap over the iterations,
+oracle bit for bit (TGV2 and TGV-k), schedule and masking equivalences bit for
+bit, the dual feasibility and the decrease of the FP64 primal-dual gap over the
+iterations,
 the Poisson solve against its closed form, and the adjoint against central
 differences.  (Thousand-iteration parity with the oracle at 256^3 would take
 the CPU minutes; the same loop is checked over many iterations at smaller
@@ -78,6 +79,32 @@ def test_256_gap_decreases_and_duals_stay_feasible(ctx, w256):
     qn = np.sqrt(q[0] ** 2 + q[1] ** 2 + q[2] ** 2 + 2 * (q[3] ** 2 + q[4] ** 2 + q[5] ** 2))
     assert pn.max() <= w256["alpha1"] * (1 + 1e-6)
     assert qn.max() <= w256["alpha0"] * (1 + 1e-6)
+
+
+def test_tgvk3_256_fused_bitwise(ctx, orc, monkeypatch, w256):
+    """The TGV-k (k = 3) configuration at 256^3: three iterations (dual, two
+    fused sweeps, primal) equal the FP32 oracle bit for bit, and 12 iterations
+    of the fused loop equal the two-launch loop's."""
+    from ambient_image_processor_amd import native
+    f = w256["f"]
+    alpha = (1.0, 4.0, 7.0)
+    tau = float(np.float32(1 / np.sqrt(20)))
+
+    def params(it):
+        return native.default_params(lam=0.5, tau=tau, sigma=tau, theta=1.0, iterations=it,
+                                     data_term=native.DATA_L1)
+
+    u, vx, vy, vz = ctx.tgvk_deshade(f, params(3), alpha)
+    _say("FP32 oracle, 3 TGV-k iterations at 256^3")
+    st = orc.tgvk(f, 3, alpha=alpha, lam=0.5, tau=tau, sigma=tau, theta=1.0, dtype=np.float32)
+    np.testing.assert_array_equal(u, st.u)
+    np.testing.assert_array_equal(vx, st.v[0])
+    np.testing.assert_array_equal(vz, st.v[2])
+    fused = ctx.tgvk_deshade(f, params(12), alpha)
+    monkeypatch.setenv("AMB_TGVK_FUSED", "0")
+    two = ctx.tgvk_deshade(f, params(12), alpha)
+    for a, b in zip(fused, two):
+        np.testing.assert_array_equal(a, b)
 
 
 @pytest.mark.parametrize("precision,tol", [(1, 1e-5), (2, 1e-4)])
