@@ -4,7 +4,8 @@
 // itk::ResampleImageFilter with itk::BSplineInterpolateImageFunction (order 3).
 // The algorithm (M. Unser, "Splines: a perfect fit for signal and image
 // processing", IEEE SPM 1999, Box 2) in FP64:
-//   coefficients - per axis x, y, z (skipped for length 1), one thread per line:
+//   coefficients - per axis x, y, z (skipped for length 1), one thread per line
+//     (x: lines staged through LDS in chunks):
 //     gain (1 - z)(1 - 1/z), pole z = sqrt(3) - 2, mirror-boundary causal start
 //     (18-term truncated sum for lines longer than 18, exact otherwise), causal
 //     and anticausal recursions;
@@ -48,37 +49,152 @@ __global__ void spline3_decompose_kernel(double* __restrict__ c, Geometry g, int
     else if (axis == 1) base = (l / W) * W * H + (l % W);
     else base = l;
     double* a = c + base;
-    const double z = sp.z;
-    for (long n = 0; n < N; ++n) a[n * st] *= sp.gain;
+    const double z = sp.z, gain = sp.gain;
+    // the gain is applied where each sample is first read (the same products as
+    // a separate scaling pass, one memory pass fewer)
+    double prev;
     if (sp.horizon < N) {
-        double zn = z, sum = a[0];
+        double zn = z, sum = a[0] * gain;
         for (long n = 1; n < sp.horizon; ++n) {
-            sum += zn * a[n * st];
+            sum += zn * (a[n * st] * gain);
             zn *= z;
         }
-        a[0] = sum;
+        prev = sum;
     } else {
         double zn = z;
         const double iz = 1.0 / z;
         double z2n = sp.z2n0;
-        double sum = a[0] + z2n * a[(N - 1) * st];
+        double sum = a[0] * gain + z2n * (a[(N - 1) * st] * gain);
         z2n *= z2n * iz;
         for (long n = 1; n <= N - 2; ++n) {
-            sum += (zn + z2n) * a[n * st];
+            sum += (zn + z2n) * (a[n * st] * gain);
             zn *= z;
             z2n *= iz;
         }
-        a[0] = sum / (1.0 - zn * zn);
+        prev = sum / (1.0 - zn * zn);
     }
-    for (long n = 1; n < N; ++n) a[n * st] += z * a[(n - 1) * st];
-    a[(N - 1) * st] = sp.pole_factor * (z * a[(N - 2) * st] + a[(N - 1) * st]);
-    for (long n = N - 2; n >= 0; --n) a[n * st] = z * (a[(n + 1) * st] - a[n * st]);
+    a[0] = prev;
+    double last = 0.0;
+    for (long n = 1; n < N; ++n) {
+        const double v = a[n * st] * gain + z * prev;
+        a[n * st] = v;
+        last = prev;
+        prev = v;
+    }
+    double next = sp.pole_factor * (z * last + prev);
+    a[(N - 1) * st] = next;
+    for (long n = N - 2; n >= 0; --n) {
+        next = z * (next - a[n * st]);
+        a[n * st] = next;
+    }
+}
+
+// The x prefilter (lines contiguous in memory): one wave per 64 lines walks
+// along x in chunks of 32 samples staged through LDS, so global loads and
+// stores are coalesced, and each lane runs its line's recursions from LDS with
+// the operations of spline3_decompose_kernel in the same order: a forward
+// pass over the chunks (gain, causal start in chunk 0 - the 18-term start and
+// the exact start of lines up to 18 samples both lie there - and the causal
+// recursion), then a backward pass (anticausal start from the last two causal
+// values, kept in registers, and the anticausal recursion).
+constexpr int XD_LINES = 64, XD_CHUNK = 32, XD_PITCH = XD_CHUNK + 1;
+
+__global__ __launch_bounds__(XD_LINES) void spline3_decompose_x_kernel(double* __restrict__ c,
+                                                                        long N, long lines,
+                                                                        Spline3Line sp) {
+    __shared__ double tile[XD_LINES * XD_PITCH];
+    const int lane = int(threadIdx.x);
+    const long l0 = long(blockIdx.x) * XD_LINES;
+    const bool has = l0 + lane < lines;
+    const long nch = (N + XD_CHUNK - 1) / XD_CHUNK;
+    const double z = sp.z;
+    double* const a = tile + lane * XD_PITCH;
+    // cooperative chunk copies: lanes 0-31 one line, 32-63 the next
+    const int cl = lane / XD_CHUNK, cx = lane % XD_CHUNK;
+    auto load = [&](long x0, int cw) {
+        for (int r = 0; r < XD_LINES; r += 2) {
+            const long gl = l0 + r + cl;
+            if (gl < lines && cx < cw) tile[(r + cl) * XD_PITCH + cx] = c[gl * N + x0 + cx];
+        }
+        __syncthreads();
+    };
+    auto store = [&](long x0, int cw) {
+        __syncthreads();
+        for (int r = 0; r < XD_LINES; r += 2) {
+            const long gl = l0 + r + cl;
+            if (gl < lines && cx < cw) c[gl * N + x0 + cx] = tile[(r + cl) * XD_PITCH + cx];
+        }
+        __syncthreads();
+    };
+
+    double prev = 0.0, last = 0.0;   // causal values at n - 1 (and the last two at the end)
+    for (long ch = 0; ch < nch; ++ch) {
+        const long x0 = ch * XD_CHUNK;
+        const int cw = int(min(long(XD_CHUNK), N - x0));
+        load(x0, cw);
+        if (has) {
+            for (int n = 0; n < cw; ++n) a[n] *= sp.gain;
+            int n0 = 0;
+            if (ch == 0) {
+                if (sp.horizon < N) {
+                    double zn = z, sum = a[0];
+                    for (long n = 1; n < sp.horizon; ++n) {
+                        sum += zn * a[n];
+                        zn *= z;
+                    }
+                    a[0] = sum;
+                } else {   // N <= 18 <= XD_CHUNK: the whole line is in this chunk
+                    double zn = z;
+                    const double iz = 1.0 / z;
+                    double z2n = sp.z2n0;
+                    double sum = a[0] + z2n * a[N - 1];
+                    z2n *= z2n * iz;
+                    for (long n = 1; n <= N - 2; ++n) {
+                        sum += (zn + z2n) * a[n];
+                        zn *= z;
+                        z2n *= iz;
+                    }
+                    a[0] = sum / (1.0 - zn * zn);
+                }
+                prev = a[0];
+                n0 = 1;
+            }
+            for (int n = n0; n < cw; ++n) {
+                a[n] += z * prev;
+                last = prev;
+                prev = a[n];
+            }
+        }
+        store(x0, cw);
+    }
+    // prev = causal c[N - 1], last = causal c[N - 2]
+    double next = 0.0;
+    for (long ch = nch - 1; ch >= 0; --ch) {
+        const long x0 = ch * XD_CHUNK;
+        const int cw = int(min(long(XD_CHUNK), N - x0));
+        load(x0, cw);
+        if (has) {
+            int n = cw - 1;
+            if (ch == nch - 1) {
+                a[n] = sp.pole_factor * (z * last + prev);
+                next = a[n];
+                --n;
+            }
+            for (; n >= 0; --n) {
+                a[n] = z * (next - a[n]);
+                next = a[n];
+            }
+        }
+        store(x0, cw);
+    }
 }
 
 // Whole-sample mirror about 0 and N - 1 (period 2N - 2); all 0 for N = 1.
-__device__ __forceinline__ long mirror_index(long k, long N) {
+// (Volumes hold fewer than 2^31 voxels, so indices fit in int.)
+__device__ __forceinline__ int mirror_index(int k, int N) {
+    if (k >= 0 && k < N) return k;   // interior taps: no reflection
     if (N == 1) return 0;
-    const long period = 2 * (N - 1);
+    const int period = 2 * (N - 1);
     k %= period;
     if (k < 0) k += period;
     return k >= N ? period - k : k;
@@ -87,13 +203,14 @@ __device__ __forceinline__ long mirror_index(long k, long N) {
 __global__ void spline3_resample_kernel(const double* __restrict__ c, Geometry gi,
                                         float* __restrict__ out, Geometry go, double sp0,
                                         double sp1, double sp2) {
-    const size_t n = go.n();
-    const size_t stride = size_t(gridDim.x) * blockDim.x;
-    const long ext[3] = {gi.w, gi.h, gi.d};
+    const uint32_t n = uint32_t(go.n());
+    const uint32_t stride = gridDim.x * blockDim.x;
+    const uint32_t OW = uint32_t(go.w), OH = uint32_t(go.h);
+    const int ext[3] = {gi.w, gi.h, gi.d};
     const double spacing[3] = {sp0, sp1, sp2};
-    for (size_t o = size_t(blockIdx.x) * blockDim.x + threadIdx.x; o < n; o += stride) {
-        const long oc[3] = {long(o % size_t(go.w)), long((o / size_t(go.w)) % size_t(go.h)),
-                            long(o / (size_t(go.w) * size_t(go.h)))};
+    for (uint32_t o = blockIdx.x * blockDim.x + threadIdx.x; o < n; o += stride) {
+        const uint32_t oyz = o / OW;
+        const uint32_t oc[3] = {o - oyz * OW, oyz % OH, oyz / OH};
         double x[3];
         bool inside = true;
         for (int a = 0; a < 3; ++a) {
@@ -104,11 +221,11 @@ __global__ void spline3_resample_kernel(const double* __restrict__ c, Geometry g
             out[o] = 0.0f;
             continue;
         }
-        long idx[3][4];
+        int idx[3][4];
         double wt[3][4];
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
-            const long i1 = long(floor(x[a]));
+            const int i1 = int(floor(x[a]));
             const double w = x[a] - double(i1);
             wt[a][3] = (1.0 / 6.0) * w * w * w;
             wt[a][0] = (1.0 / 6.0) + 0.5 * w * (w - 1.0) - wt[a][3];
@@ -125,7 +242,7 @@ __global__ void spline3_resample_kernel(const double* __restrict__ c, Geometry g
             w *= wt[0][k0];
             w *= wt[1][k1];
             w *= wt[2][k2];
-            acc += w * c[size_t((idx[2][k2] * ext[1] + idx[1][k1]) * ext[0] + idx[0][k0])];
+            acc += w * c[(idx[2][k2] * ext[1] + idx[1][k1]) * ext[0] + idx[0][k0]];
         }
         out[o] = float(acc);
     }
@@ -191,8 +308,13 @@ hipError_t launch_resize_spline3(hipStream_t s, const float* in, Geometry gi, do
         if (N == 1) continue;
         sp.z2n0 = std::pow(sp.z, double(N - 1));
         const long lines = long(n) / N;
-        hipLaunchKernelGGL(spline3_decompose_kernel, dim3(unsigned((lines + 255) / 256)), dim3(256),
-                           0, s, coef, gi, axis, lines, sp);
+        if (axis == 0)
+            hipLaunchKernelGGL(spline3_decompose_x_kernel,
+                               dim3(unsigned((lines + XD_LINES - 1) / XD_LINES)), dim3(XD_LINES), 0,
+                               s, coef, N, lines, sp);
+        else
+            hipLaunchKernelGGL(spline3_decompose_kernel, dim3(unsigned((lines + 255) / 256)),
+                               dim3(256), 0, s, coef, gi, axis, lines, sp);
         e = hipGetLastError();
         if (e != hipSuccess) return e;
     }

@@ -1357,13 +1357,11 @@ hipError_t fused_launch(hipStream_t s, const FusedFields& ff, const float* f, Ge
     const int nb = (g.h + plan.ty - 1) / plan.ty, nc = (g.d + plan.zchunk - 1) / plan.zchunk;
     const size_t lds = fused_lds_floats<VX>(g.w, plan.ty) * sizeof(float);
     auto kern = fused_sweep_kernel<S, I, VX, L2, MK, MAXT, PF>;
-    static size_t lds_set = 0;   // per instantiation: raise the dynamic LDS limit once
-    if (lds > lds_set) {
-        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
-        if (e != hipSuccess) return e;
-        lds_set = lds;
-    }
+    // the dynamic LDS limit, set on every launch: it is a property of the kernel on
+    // the current device, and a cached flag would be shared by threads and devices
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
+    if (e != hipSuccess) return e;
     hipLaunchKernelGGL(kern, dim3(nb * nc), dim3(L * RT), lds, s, ff, f, p.active, g, p.tau,
                        p.sigma, p.lambda, p.alpha0, p.alpha1, p.theta, plan.ty, plan.zchunk,
                        p.planar);

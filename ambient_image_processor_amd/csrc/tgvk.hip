@@ -1005,13 +1005,11 @@ hipError_t tgvk_fused_launch(hipStream_t s, const KFusedFields& ff, const float*
     const int nb = (g.h + plan.ty - 1) / plan.ty, nc = (g.d + plan.zchunk - 1) / plan.zchunk;
     const size_t lds = size_t(N_KROWS) * RT * TXH * sizeof(float);
     auto kern = tgvk_fused_kernel<I, MK>;
-    static size_t lds_set = 0;   // per instantiation: raise the dynamic LDS limit once
-    if (lds > lds_set) {
-        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
-        if (e != hipSuccess) return e;
-        lds_set = lds;
-    }
+    // the dynamic LDS limit, set on every launch: it is a property of the kernel on
+    // the current device, and a cached flag would be shared by threads and devices
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
+    if (e != hipSuccess) return e;
     hipLaunchKernelGGL(kern, dim3(nbx * nb * nc), dim3(TXH * RT), lds, s, ff, f, act, g, tau,
                        sigma, lambda, theta, alpha[0], alpha[1], alpha[2], plan.tx, plan.ty,
                        plan.zchunk, planar);

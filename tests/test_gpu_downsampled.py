@@ -39,7 +39,12 @@ def _mask(shape, seed):
 RESIZES = [((9, 30, 41), (5, 15, 21)), ((5, 15, 21), (9, 30, 41)), ((8, 25, 33), (3, 8, 10)),
            ((6, 7, 64), (2, 2, 16)), ((2, 2, 16), (6, 7, 64)), ((1, 40, 52), (1, 20, 26)),
            ((1, 20, 26), (1, 40, 52)), ((1, 1, 3), (1, 1, 30)), ((40, 48, 64), (20, 24, 32)),
-           ((20, 24, 32), (40, 48, 64)), ((7, 9, 11), (7, 9, 11))]
+           ((20, 24, 32), (40, 48, 64)), ((7, 9, 11), (7, 9, 11)),
+           # x lines across the 32-sample chunks of the x prefilter: a last chunk of 1
+           # sample, 3 chunks + 1, the 18-term start at 19 samples, the exact start
+           # at 18 and at 2
+           ((4, 5, 33), (2, 3, 17)), ((2, 3, 97), (5, 7, 200)), ((2, 70, 19), (1, 35, 10)),
+           ((1, 3, 18), (1, 6, 36)), ((3, 4, 2), (6, 8, 4)), ((3, 66, 130), (2, 33, 65))]
 
 
 @pytest.mark.parametrize("src,dst", RESIZES)
