@@ -105,6 +105,10 @@ constexpr int TILE_VECTOR_DEFAULT = 4;
 // Launchers (stream-ordered, asynchronous).  `fields` points at 17 arrays of
 // the storage type (void* so both storages share the signature).
 hipError_t launch_init(hipStream_t s, int storage, void* const* fields, const float* f, Geometry g);
+// Raises a kernel's dynamic LDS limit on the current device to at least
+// `bytes`, once per (device, kernel, size): setting it before every launch
+// costs a gap between back-to-back launches.  Thread-safe.
+hipError_t ensure_dynamic_lds(const void* kernel, size_t bytes);
 // Fused sweep (primal half of iteration k + dual half of iteration k + 1):
 // reads u, v, p, q through the 17-field table `src` and writes them to the 13
 // fields `dst13` (order u, vx, vy, vz, px, py, pz, qxx, qyy, qzz, qxy, qxz,

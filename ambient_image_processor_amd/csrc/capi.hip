@@ -131,6 +131,9 @@ struct amb_context {
     char* kalt_block = nullptr;
     size_t kalt_cap = 0;
     float* kalt[12] = {};
+    char* kalt13_block = nullptr;   // its own second copy of the 13 TGV2 fields
+    size_t kalt13_cap = 0;
+    void* kalt13[N_FUSED_FIELDS] = {};
     bool kfused_ok = false;
     TgvkFusedPlan kfplan{0, 0, 0};
     // resampling and the downsampled deshade: one grow-only workspace
@@ -451,15 +454,17 @@ amb_status begin(amb_context* c, amb_memory mem, const float* f, uint32_t w, uin
     return AMB_OK;
 }
 
-// Second copy of the 13 ping-ponged fields for the fused sweeps.  Field k of
-// the state lives either in the state block or in the second block; c->alt[k]
-// is set to the other one.  The second block has the state block's layout
-// (field F at F x stride; the u_bar and v_bar slots stay unused), so the two
-// copies of every field are the same distance apart.
+// Second copy of the 13 ping-ponged fields for the fused TGV2 sweep.  Field k
+// of the state lives either in the state block or in the second block;
+// c->alt[k] is set to the other one.  The second block is packed in the
+// sweep's field order, so the two copies of different fields lie at different
+// distances: a second block with the state block's layout (every field's
+// copies one distance apart) made the fused sweep 2.4% slower on MI355X at
+// 256^3 (384 vs 375 us, same box), also with the block offset by 4 KB or 1 MB.
 amb_status ensure_alt(amb_context* c) {
     const size_t elem = c->storage == 1 ? 2 : 4;
     const size_t fstride = align_up(c->g.n() * elem, 256);
-    const size_t need = fstride * N_FIELDS;
+    const size_t need = fstride * N_FUSED_FIELDS;
     auto main_at = [&](int k) { return static_cast<void*>(c->block + size_t(FUSED_FIELD_MAP[k]) * fstride); };
     if (need > c->alt_cap) {
         for (int k = 0; k < N_FUSED_FIELDS; ++k)   // state must be in the state block
@@ -472,7 +477,7 @@ amb_status ensure_alt(amb_context* c) {
         c->alt_cap = need;
     }
     for (int k = 0; k < N_FUSED_FIELDS; ++k) {
-        void* a = c->alt_block + size_t(FUSED_FIELD_MAP[k]) * fstride;
+        void* a = c->alt_block + size_t(k) * fstride;
         c->alt[k] = c->fields[FUSED_FIELD_MAP[k]] == main_at(k) ? a : main_at(k);
     }
     return AMB_OK;
@@ -633,8 +638,37 @@ amb_status tgvk_begin(amb_context* c, amb_memory mem, const float* f, uint32_t w
 
 // Level-table slots of w_0 (0..5) and r_0 (12..17), the level fields the
 // fused TGV-k sweep ping-pongs; c->kalt[m] is the other copy of slot KLV_PING[m].
-// The second block has the first level's layout (slot j at j x stride).
 constexpr int KLV_PING[12] = {0, 1, 2, 3, 4, 5, 12, 13, 14, 15, 16, 17};
+
+// The fused TGV-k sweep reaches every destination as its source pointer plus
+// one distance per buffer (25 destination pointers would not fit the scalar
+// registers), so its second copies mirror the first copies' layouts: the 13
+// TGV2 fields in a block of the state block's layout (the u_bar and v_bar
+// slots unused), the level fields in a block of the level block's layout.
+// (Measured on MI355X at 256^3, same box: 808 us per launch this way; 829 us
+// with each field's second copy in the slot of another field, which the TGV2
+// sweep prefers - see ensure_alt.)
+
+amb_status ensure_kalt13(amb_context* c) {
+    const size_t fstride = align_up(c->g.n() * 4, 256);
+    const size_t need = fstride * N_FIELDS;
+    auto main_at = [&](int k) { return static_cast<void*>(c->block + size_t(FUSED_FIELD_MAP[k]) * fstride); };
+    if (need > c->kalt13_cap) {
+        for (int k = 0; k < N_FUSED_FIELDS; ++k)   // state must be in the state block
+            if (c->fields[FUSED_FIELD_MAP[k]] != main_at(k))
+                return fail(AMB_ERR_NOT_SUPPORTED, "fused TGV-k buffers out of step with the state");
+        if (c->kalt13_block) AMB_HIP(hipFree(c->kalt13_block));
+        c->kalt13_block = nullptr;
+        c->kalt13_cap = 0;
+        AMB_HIP(hipMalloc(reinterpret_cast<void**>(&c->kalt13_block), need));
+        c->kalt13_cap = need;
+    }
+    for (int k = 0; k < N_FUSED_FIELDS; ++k) {
+        void* a = c->kalt13_block + size_t(FUSED_FIELD_MAP[k]) * fstride;
+        c->kalt13[k] = c->fields[FUSED_FIELD_MAP[k]] == main_at(k) ? a : main_at(k);
+    }
+    return AMB_OK;
+}
 
 amb_status ensure_kalt(amb_context* c) {
     const size_t fstride = align_up(c->g.n() * 4, 256);
@@ -665,13 +699,13 @@ amb_status tgvk_fused_step(amb_context* c) {
     float* dst[N_TGVK_FUSED_FIELDS];
     for (int k = 0; k < N_FUSED_FIELDS; ++k) {
         src[k] = static_cast<const float*>(c->fields[FUSED_FIELD_MAP[k]]);
-        dst[k] = static_cast<float*>(c->alt[k]);
+        dst[k] = static_cast<float*>(c->kalt13[k]);
     }
     for (int m = 0; m < 12; ++m) {
         src[N_FUSED_FIELDS + m] = c->klv_host[KLV_PING[m]];
         dst[N_FUSED_FIELDS + m] = c->kalt[m];
     }
-    // every field's other copy lies at one distance per block (see ensure_alt, ensure_kalt)
+    // every field's other copy lies at one distance per block (ensure_kalt13, ensure_kalt)
     for (int k = 1; k < N_TGVK_FUSED_FIELDS; ++k) {
         const int k0 = k < N_FUSED_FIELDS ? 0 : N_FUSED_FIELDS;
         if (reinterpret_cast<const char*>(dst[k]) - reinterpret_cast<const char*>(src[k]) !=
@@ -684,7 +718,7 @@ amb_status tgvk_fused_step(amb_context* c) {
                                   c->sp.sigma, c->sp.lambda, c->sp.theta, c->kalpha_host.data(),
                                   c->sp.planar, c->kfplan));
     }
-    for (int k = 0; k < N_FUSED_FIELDS; ++k) std::swap(c->fields[FUSED_FIELD_MAP[k]], c->alt[k]);
+    for (int k = 0; k < N_FUSED_FIELDS; ++k) std::swap(c->fields[FUSED_FIELD_MAP[k]], c->kalt13[k]);
     for (int m = 0; m < 12; ++m) std::swap(c->klv_host[KLV_PING[m]], c->kalt[m]);
     return AMB_OK;
 }
@@ -712,7 +746,7 @@ amb_status tgvk_run(amb_context* c, uint32_t iterations, uint32_t interval,
     float* const* F = reinterpret_cast<float* const*>(c->fields_dev);
     const bool fused = c->kfused_ok && iterations > 1;
     if (fused) {
-        AMB_TRY(ensure_alt(c));
+        AMB_TRY(ensure_kalt13(c));
         AMB_TRY(ensure_kalt(c));
         // earlier uploads from the host tables have run before they change
         AMB_HIP(hipStreamSynchronize(c->stream));
@@ -935,6 +969,7 @@ AMB_API amb_status amb_context_destroy(amb_context* c) {
     if (c->klv_dev) (void)hipFree(c->klv_dev);
     if (c->kalpha_dev) (void)hipFree(c->kalpha_dev);
     if (c->kalt_block) (void)hipFree(c->kalt_block);
+    if (c->kalt13_block) (void)hipFree(c->kalt13_block);
     if (c->rblock) (void)hipFree(c->rblock);
     (void)hipStreamDestroy(c->stream);
     delete c;

@@ -30,6 +30,10 @@
 // -ffp-contract=off).
 #include "amb_internal.hpp"
 
+#include <map>
+#include <mutex>
+#include <utility>
+
 #include <algorithm>
 #include <cstdlib>
 
@@ -1357,10 +1361,7 @@ hipError_t fused_launch(hipStream_t s, const FusedFields& ff, const float* f, Ge
     const int nb = (g.h + plan.ty - 1) / plan.ty, nc = (g.d + plan.zchunk - 1) / plan.zchunk;
     const size_t lds = fused_lds_floats<VX>(g.w, plan.ty) * sizeof(float);
     auto kern = fused_sweep_kernel<S, I, VX, L2, MK, MAXT, PF>;
-    // the dynamic LDS limit, set on every launch: it is a property of the kernel on
-    // the current device, and a cached flag would be shared by threads and devices
-    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
+    hipError_t e = ensure_dynamic_lds(reinterpret_cast<const void*>(kern), lds);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(kern, dim3(nb * nc), dim3(L * RT), lds, s, ff, f, p.active, g, p.tau,
                        p.sigma, p.lambda, p.alpha0, p.alpha1, p.theta, plan.ty, plan.zchunk,
@@ -1552,6 +1553,20 @@ hipError_t launch_convert(hipStream_t s, int storage, const void* src, void* dst
                            static_cast<const float*>(src), static_cast<float*>(dst), n);
     }
     return hipGetLastError();
+}
+
+hipError_t ensure_dynamic_lds(const void* kernel, size_t bytes) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    static std::mutex mu;
+    static std::map<std::pair<int, const void*>, size_t> raised;
+    std::lock_guard<std::mutex> lock(mu);
+    size_t& have = raised[{dev, kernel}];
+    if (bytes <= have) return hipSuccess;
+    e = hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, int(bytes));
+    if (e == hipSuccess) have = bytes;
+    return e;
 }
 
 }  // namespace amb

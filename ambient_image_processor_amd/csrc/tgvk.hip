@@ -674,8 +674,9 @@ enum KFusedField { KF_U = 0, KF_VX, KF_VY, KF_VZ, KF_PX, KF_PY, KF_PZ, KF_QXX, K
 static_assert(N_KF == N_TGVK_FUSED_FIELDS, "fused TGV-k field count");
 
 // dst field k = src field k + dt bytes (k < 13) or + dl bytes (k >= 13): the
-// two copies of the TGV2 fields and of the level fields are one distance
-// apart, which keeps the 25 destination pointers out of the scalar registers.
+// host lays the second copies out so, which keeps the 25 destination pointers
+// out of the scalar registers.
+
 struct KFusedFields {
     const float* src[N_KF];
     long long dt, dl;
@@ -732,8 +733,9 @@ __global__ __launch_bounds__(KFUSED_MAX_THREADS) void tgvk_fused_kernel(
     auto R = [&](int k) { return lds + k * RW; };
     const float* const* src = ff.src;
     auto dst = [&](int k) {
-        return reinterpret_cast<float*>(const_cast<char*>(reinterpret_cast<const char*>(src[k])) +
-                                        (k < KF_WX ? ff.dt : ff.dl));
+        return reinterpret_cast<float*>(
+            const_cast<char*>(reinterpret_cast<const char*>(src[k])) +
+            (k < KF_WX ? ff.dt : ff.dl));
     };
 
     // D region starts at zero (read before its first write only into unused carries)
@@ -1005,10 +1007,7 @@ hipError_t tgvk_fused_launch(hipStream_t s, const KFusedFields& ff, const float*
     const int nb = (g.h + plan.ty - 1) / plan.ty, nc = (g.d + plan.zchunk - 1) / plan.zchunk;
     const size_t lds = size_t(N_KROWS) * RT * TXH * sizeof(float);
     auto kern = tgvk_fused_kernel<I, MK>;
-    // the dynamic LDS limit, set on every launch: it is a property of the kernel on
-    // the current device, and a cached flag would be shared by threads and devices
-    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
+    hipError_t e = ensure_dynamic_lds(reinterpret_cast<const void*>(kern), lds);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(kern, dim3(nbx * nb * nc), dim3(TXH * RT), lds, s, ff, f, act, g, tau,
                        sigma, lambda, theta, alpha[0], alpha[1], alpha[2], plan.tx, plan.ty,
