@@ -54,9 +54,9 @@ SWEEPS = ("dual_sweep", "primal_sweep", "fused_sweep")
 # 2.0 and WRITE_SIZE by 1.0 (what the counters read for a plain 1 GiB copy);
 # summaries in profiles/.  Keyed by (kernel, storage bytes, edge length); the
 # TGV-k entries by (kernel, levels, edge length) (k = 3, FP32 fields).
-MEASURED_TRAFFIC = {("fused_sweep", 4, 256): 1845865369,
-                    ("dual_sweep", 4, 256): 1532669905,
-                    ("primal_sweep", 4, 256): 1533132892,
+MEASURED_TRAFFIC = {("fused_sweep", 4, 256): 1857372102,
+                    ("dual_sweep", 4, 256): 1531281680,
+                    ("primal_sweep", 4, 256): 1532567692,
                     ("tgvk_dual", 1, 256): 2908697085,
                     ("tgvk_primal", 1, 256): 3563855669,
                     ("tgvk_fused", 1, 256): 3479840059}
@@ -98,19 +98,25 @@ def tgvk_kernels(prof, levels, n):
 
 
 def sweep_bytes(kind, storage_bytes, n):
-    """Algorithmic bytes per launch."""
+    """Algorithmic bytes per launch: every field the kernel's own formulation
+    reads or writes, moved once (f is FP32):
+      dual   read u_bar, v_bar x3, p x3, q x6; write p x3, q x6         22 fields
+      primal read u, f, p x3, q x6, v x3; write u, u_bar, v x3, v_bar x3 21 + f
+      fused  (primal half of one iteration + dual half of the next; u_bar and
+             v_bar stay on chip) read u, f, v x3, p x3, q x6; write u, v x3,
+             p x3, q x6                                                26 + f
+    (108 B per voxel-iteration fused at FP32; 176 B for the two sweeps.)"""
     if kind == "dual_sweep":
         return 22 * storage_bytes * n
     if kind == "primal_sweep":
         return (21 * storage_bytes + 4) * n
-    return (43 * storage_bytes + 4) * n
+    return (26 * storage_bytes + 4) * n
 
 
-def kernel_min_bytes(kind, storage_bytes, n):
-    """Bytes each field must move once for the kernel's own formulation."""
-    if kind == "fused_sweep":
-        return (26 * storage_bytes + 4) * n
-    return sweep_bytes(kind, storage_bytes, n)
+def two_sweep_bytes(storage_bytes, n):
+    """Bytes of one iteration in the two-sweep form (dual + primal): the 176 B
+    per voxel-iteration at FP32 that the fused sweep does in 108 B."""
+    return sweep_bytes("dual_sweep", storage_bytes, n) + sweep_bytes("primal_sweep", storage_bytes, n)
 
 
 def rel_err(a, b):
@@ -175,10 +181,11 @@ def sweep_kernels(prof, sb, n):
             continue
         avg_ms = ms / cnt
         b = sweep_bytes(name, sb, n)
-        kmin = kernel_min_bytes(name, sb, n)
         kernels[name] = {"avg_us": 1000.0 * avg_ms, "launches": cnt,
-                         "bytes_per_launch": b, "gbs": b / (avg_ms * 1e-3) / 1e9,
-                         "kernel_min_bytes": kmin, "kernel_min_gbs": kmin / (avg_ms * 1e-3) / 1e9}
+                         "bytes_per_launch": b, "gbs": b / (avg_ms * 1e-3) / 1e9}
+        if name == "fused_sweep":   # one fused launch does one iteration's work
+            kernels[name]["two_sweep_equivalent_gbs"] = (two_sweep_bytes(sb, n) /
+                                                         (avg_ms * 1e-3) / 1e9)
     return kernels
 
 
@@ -292,14 +299,13 @@ def deshade_256(args):
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "bytes_per_launch": kernels[dom]["bytes_per_launch"],
-                     "kernel_min_bytes_per_launch": kernels[dom]["kernel_min_bytes"],
-                     "kernel_min_frac": kernels[dom]["kernel_min_gbs"] / HBM_PEAK_GBS,
+                     "two_sweep_equivalent_gbs": kernels[dom].get("two_sweep_equivalent_gbs"),
                      "measured_hbm_gbs": (traffic / avg_s / 1e9) if traffic else None},
         "iteration": {"ms": it_ms, "bytes": iter_bytes,
                       "gbs": iter_bytes / (it_ms * 1e-3) / 1e9, "voxel_iters_per_s": it_rate},
         "kernels": rounded(kernels),
-        "other_kernels_ms": {k: round(prof[k][0], 3) for k in ("dct", "poisson", "divergence",
-                                                                "epilogue", "init")},
+        "other_kernels_ms_per_step": {k: round(prof[k][0] / args.steps, 3)
+                                      for k in ("dct", "poisson", "divergence", "epilogue", "init")},
         "split": split,
         "host_buffers": host,
         "cpu_baseline": cpu,
@@ -501,8 +507,8 @@ def tgvk3_256(args):
                      "measured_hbm_gbs": (traffic / (kernels[dom]["avg_us"] * 1e-6) / 1e9)
                      if traffic else None},
         "kernels": rounded(kernels),
-        "other_kernels_ms": {k: round(prof[k][0], 3) for k in ("dct", "poisson", "divergence",
-                                                                "epilogue", "init")},
+        "other_kernels_ms_per_step": {k: round(prof[k][0] / args.steps, 3)
+                                      for k in ("dct", "poisson", "divergence", "epilogue", "init")},
         "cpu_baseline": cpu}))
     for ptr in (f_dev, u_dev, l_dev, r_dev):
         ctx.free(ptr)
@@ -559,8 +565,8 @@ def downsampled_512(args):
                    "note": "4 resamplings per deshade (f*mask and mask down, u and l up), "
                            "timed as 2 scopes"},
         "kernels": rounded(kernels),
-        "other_kernels_ms": {k: round(prof[k][0], 3) for k in ("dct", "poisson", "divergence",
-                                                                "epilogue", "init")},
+        "other_kernels_ms_per_step": {k: round(prof[k][0] / args.steps, 3)
+                                      for k in ("dct", "poisson", "divergence", "epilogue", "init")},
         "cpu_baseline": None}))
     for ptr in bufs:
         ctx.free(ptr)
