@@ -109,6 +109,9 @@ hipError_t launch_init(hipStream_t s, int storage, void* const* fields, const fl
 // `bytes`, once per (device, kernel, size): setting it before every launch
 // costs a gap between back-to-back launches.  Thread-safe.
 hipError_t ensure_dynamic_lds(const void* kernel, size_t bytes);
+// AMB_INDEX64=1: run the kernels' 64-bit-offset forms also on volumes whose
+// offsets fit 32 bits (tests of those paths).
+bool force_index64();
 // Fused sweep (primal half of iteration k + dual half of iteration k + 1):
 // reads u, v, p, q through the 17-field table `src` and writes them to the 13
 // fields `dst13` (order u, vx, vy, vz, px, py, pz, qxx, qyy, qzz, qxy, qxz,

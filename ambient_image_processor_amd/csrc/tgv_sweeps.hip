@@ -1279,7 +1279,11 @@ hipError_t primal_vec_pf(hipStream_t s, void* const* fv, const float* f, Geometr
 }
 
 // 32-bit offsets when every byte offset of a 4-byte field fits in 32 bits.
-inline bool small_index(const Geometry& g) { return g.n() * 4 < (size_t(1) << 32); }
+// 32-bit element offsets fit volumes under 2^30 voxels; AMB_INDEX64=1 forces
+// the 64-bit forms (tests of those paths on small volumes).
+inline bool small_index(const Geometry& g) {
+    return g.n() * 4 < (size_t(1) << 32) && !force_index64();
+}
 
 template <class S, class I, int TX, int TY, int MK = 0>
 hipError_t dual_launch(hipStream_t s, void* const* fv, Geometry g, const SweepParams& p) {
@@ -1562,6 +1566,11 @@ hipError_t launch_convert(hipStream_t s, int storage, const void* src, void* dst
                            static_cast<const float*>(src), static_cast<float*>(dst), n);
     }
     return hipGetLastError();
+}
+
+bool force_index64() {
+    const char* e = std::getenv("AMB_INDEX64");
+    return e && e[0] == '1';
 }
 
 hipError_t ensure_dynamic_lds(const void* kernel, size_t bytes) {

@@ -1048,7 +1048,7 @@ int tgvk_blocks(const Geometry& g, int* zchunk) {
 // sweep; AMB_TGVK_FUSED_TX, _TY and _ZCHUNK override.
 bool tgvk_fused_plan(const Geometry& g, int levels, int device, TgvkFusedPlan* plan) {
     // 32-bit offsets only (volumes under 2^30 voxels): the 64-bit form spills
-    if (levels != 1 || g.n() * 4 >= (size_t(1) << 32)) return false;
+    if (levels != 1 || g.n() * 4 >= (size_t(1) << 32) || force_index64()) return false;
     const char* en = std::getenv("AMB_TGVK_FUSED");
     if (en && en[0] == '0') return false;
     int max_lds = 0, cus = 0;
@@ -1109,7 +1109,7 @@ hipError_t launch_tgvk_dual(hipStream_t s, float* const* fields_dev, float* cons
     int zc = 1;
     const int nb = tgvk_blocks(g, &zc);
     // fixed-level kernels address with 32-bit byte offsets
-    switch (g.n() <= (size_t(1) << 29) ? levels : 0) {
+    switch (g.n() <= (size_t(1) << 29) && !force_index64() ? levels : 0) {
     case 1: hipLaunchKernelGGL(tgvk_dual_fixed<1>, dim3(nb), dim3(KT), 0, s, fields_dev,
                                levels_dev, alpha_dev, active, g, sigma, planar, zc); break;
     case 2: hipLaunchKernelGGL(tgvk_dual_fixed<2>, dim3(nb), dim3(KT), 0, s, fields_dev,
@@ -1129,7 +1129,7 @@ hipError_t launch_tgvk_primal(hipStream_t s, float* const* fields_dev, float* co
     int zc = 1;
     const int nb = tgvk_blocks(g, &zc);
     // fixed-level kernels address with 32-bit byte offsets
-    switch (g.n() <= (size_t(1) << 29) ? levels : 0) {
+    switch (g.n() <= (size_t(1) << 29) && !force_index64() ? levels : 0) {
     case 1: hipLaunchKernelGGL(tgvk_primal_fixed<1>, dim3(nb), dim3(KT), 0, s, fields_dev,
                                levels_dev, f, active, g, tau, lambda, theta, zc); break;
     case 2: hipLaunchKernelGGL(tgvk_primal_fixed<2>, dim3(nb), dim3(KT), 0, s, fields_dev,

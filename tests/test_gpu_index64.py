@@ -1,0 +1,111 @@
+"""GPU tests of the 64-bit-offset kernel forms, which volumes of 2^30 voxels and
+more run (every field then spans 4 GiB or more).  AMB_INDEX64=1 forces those
+forms on small volumes: they perform the same per-voxel arithmetic, so their
+results equal the 32-bit forms' bit for bit (TGV2 fused and two-sweep, L1 and
+L2, FP32 and BF16 fields, masked, 2-D; TGV-k through the generic kernels).
+One test runs a real 2049 x 1024 x 512 volume and checks a block at the far
+end of the volume, where byte offsets exceed 2^32, against the FP32 oracle run
+on a crop around it (after N iterations a voxel depends only on voxels within
+2N of it, so the crop's own edges do not reach the compared block)."""
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+TAU = float(np.float32(1 / np.sqrt(20)))
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _f(shape, seed):
+    from ambient_image_processor_amd import phantoms
+    return phantoms.deshade_phantom(shape, seed=seed)[0]
+
+
+def _p(native, iterations, data_term, **kw):
+    return native.default_params(lam=0.7, alpha0=2.0, alpha1=1.0, tau=TAU, sigma=TAU, theta=1.0,
+                                 iterations=iterations, data_term=data_term, **kw)
+
+
+def _both(monkeypatch, run):
+    a = run()
+    monkeypatch.setenv("AMB_INDEX64", "1")
+    b = run()
+    monkeypatch.delenv("AMB_INDEX64")
+    return a, b
+
+
+@pytest.mark.parametrize("shape", [(8, 12, 64), (7, 9, 30), (5, 6, 41)])
+@pytest.mark.parametrize("data_term,storage", [(0, 0), (1, 0), (0, 1)])
+@pytest.mark.parametrize("fused", ["1", "0"])
+def test_tgv2_index64_equals_index32(ctx, monkeypatch, shape, data_term, storage, fused):
+    from ambient_image_processor_amd import native
+    _say("index64 tgv2", shape, data_term, storage, fused)
+    monkeypatch.setenv("AMB_FUSED", fused)
+    f = _f(shape, 40)
+    p = _p(native, 12, data_term, storage=storage)
+    a, b = _both(monkeypatch, lambda: ctx.tgv2_deshade(f, p))
+    for x, y in zip(a, b):
+        np.testing.assert_array_equal(x, y)
+
+
+def test_tgv2_index64_masked_and_planar(ctx, monkeypatch):
+    from ambient_image_processor_amd import native
+    f = _f((6, 10, 48), 41)
+    act = np.random.default_rng(42).random(f.shape) > 0.3
+    p = _p(native, 12, 0)
+    a, b = _both(monkeypatch, lambda: ctx.tgv2_deshade_masked(f, p, act))
+    for x, y in zip(a, b):
+        np.testing.assert_array_equal(x, y)
+    f1 = _f((1, 20, 36), 43)
+    p1 = _p(native, 12, 0, flags=native.FLAG_PLANAR)
+    a, b = _both(monkeypatch, lambda: ctx.tgv2_deshade(f1, p1))
+    for x, y in zip(a, b):
+        np.testing.assert_array_equal(x, y)
+
+
+@pytest.mark.parametrize("alpha", [(1.0, 2.0, 0.6), (1.0, 2.0, 0.8, 0.5)])
+def test_tgvk_index64_equals_index32(ctx, monkeypatch, alpha):
+    """k = 3 runs the fused sweep with 32-bit offsets and the generic kernels
+    with 64-bit ones; k = 4 the fixed-level and the generic kernels."""
+    from ambient_image_processor_amd import native
+    f = _f((7, 11, 40), 44)
+    p = native.default_params(lam=0.7, tau=TAU, sigma=TAU, theta=1.0, iterations=15,
+                              data_term=native.DATA_L1)
+    a, b = _both(monkeypatch, lambda: ctx.tgvk_deshade(f, p, alpha))
+    for x, y in zip(a, b):
+        np.testing.assert_array_equal(x, y)
+
+
+def test_tgv2_over_2_30_voxels_against_cropped_oracle(ctx, orc):
+    """2049 x 1024 x 512 (4.3 GB per field), three TGV2-L1 iterations (dual
+    sweep, two fused sweeps in the four-voxel form, primal sweep, all with
+    64-bit offsets); the last 24 planes of a 24 x 24 column window at the x
+    end equal the FP32 oracle run on the crop."""
+    from ambient_image_processor_amd import native
+    d, h, w = 2049, 1024, 512
+    iters, m = 3, 8
+    _say("generating the 2049 x 1024 x 512 input")
+    f = np.random.default_rng(45).random((d, h, w), dtype=np.float32)
+    _say("GPU: 3 iterations at 2049 x 1024 x 512")
+    ctx.profile(True)
+    ctx.profile_reset()
+    u, vx, vy, vz = ctx.tgv2_deshade(f, _p(native, iters, 0))
+    prof = ctx.profile_read()
+    ctx.profile(False)
+    assert prof["fused_sweep"][1] == iters - 1, prof
+    z0, y0, x0, n = d - 24, 500, w - 24, 24
+    crop = f[z0 - m:, y0 - m:y0 + n + m, x0 - m:]
+    _say("oracle on the crop", crop.shape)
+    st = orc.tgv2(crop, iters, lam=0.7, alpha0=2.0, alpha1=1.0, tau=TAU, sigma=TAU,
+                  data_term="l1", dtype=np.float32)
+    sl = (slice(z0, d), slice(y0, y0 + n), slice(x0, w))
+    cs = (slice(m, None), slice(m, m + n), slice(m, None))
+    np.testing.assert_array_equal(u[sl], st.u[cs])
+    np.testing.assert_array_equal(vx[sl], st.v[0][cs])
+    np.testing.assert_array_equal(vz[sl], st.v[2][cs])
