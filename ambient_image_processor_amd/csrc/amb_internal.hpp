@@ -115,12 +115,13 @@ bool force_index64();
 // Fused sweep (primal half of iteration k + dual half of iteration k + 1):
 // reads u, v, p, q through the 17-field table `src` and writes them to the 13
 // fields `dst13` (order u, vx, vy, vz, px, py, pz, qxx, qyy, qzz, qxy, qxz,
-// qyz); u_bar and v_bar are neither read nor written.  Needs W % 4 == 0.
+// qyz); u_bar and v_bar are neither read nor written.  Needs an even W.
 struct FusedPlan {
     int ty;       // owned y rows per block
     int zchunk;   // planes per block
     int pf;       // 1: next-plane loads issued one step ahead
     int vx;       // voxels per thread along x (2 or 4)
+    int tx;       // 0: whole rows; else owned x columns per block (x tiles, vx 2)
 };
 bool fused_plan(const Geometry& g, int storage, int device, FusedPlan* plan);
 size_t fused_lds_bytes(const Geometry& g, const FusedPlan& plan);

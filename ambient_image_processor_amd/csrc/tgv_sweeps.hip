@@ -882,10 +882,11 @@ struct FusedFields {
 
 constexpr int FUSED_MAX_THREADS = 768;
 
+// LDS floats of a fused block: `row` floats per row (W, or tx + 2 VX for tiles).
 template <int VX>
-inline size_t fused_lds_floats(int W, int ty) {
-    const int L = W / VX, RT = ty + 2;
-    return size_t(8) * RT * W + size_t(8) * RT * (L + 1);
+inline size_t fused_lds_floats(int row, int ty) {
+    const int L = row / VX, RT = ty + 2;
+    return size_t(8) * RT * row + size_t(8) * RT * (L + 1);
 }
 
 template <int VX>
@@ -897,34 +898,41 @@ struct FusedPlane {
 
 // PF = 1 loads plane s + 1 right after the first barrier of step s, so the
 // loads are in flight during both halves of step s.
-template <class S, class I, int VX, int L2, int MK, int MAXT, int PF>
+// TL = 1: the block owns `tx` columns (tx % VX == 0) instead of whole rows, with
+// one halo lane (VX voxels) on each side: lane 0 feeds only v_bar at x0 - 1,
+// the last lane only u_bar at x0 + tx (rows wider than whole-row bands allow).
+template <class S, class I, int VX, int L2, int MK, int MAXT, int PF, int TL>
 __global__ __launch_bounds__(MAXT) void fused_sweep_kernel(
     FusedFields ff, const float* __restrict__ f_, const uint8_t* __restrict__ act, Geometry g,
     float tau, float sigma, float lambda, float alpha0, float alpha1, float theta, int ty,
-    int zchunk, int planar) {
+    int zchunk, int planar, int tx) {
     typedef typename S::T T;
     typedef VecIO<S, VX> V;
     typedef VecIO<StoreF32, VX> VF;
     extern __shared__ __align__(16) float lds[];
 
     const int W = g.w, H = g.h, D = g.d;
-    const int L = W / VX, RT = ty + 2;
+    const int L = TL ? tx / VX + 2 : W / VX, RT = ty + 2;
+    const int RWD = L * VX;               // floats per LDS row
     const int lane = int(threadIdx.x) % L, r = int(threadIdx.x) / L;
-    const int nb = (H + ty - 1) / ty;
+    const int nbx = TL ? (W + tx - 1) / tx : 1, nb = (H + ty - 1) / ty;
     const int t = xcd_tile(blockIdx.x, gridDim.x);
-    const int y0 = (t % nb) * ty;
-    const int z0 = (t / nb) * zchunk, z1 = min(z0 + zchunk, D);
+    const int tb = TL ? t / nbx : t;
+    const int y0 = (tb % nb) * ty;
+    const int z0 = (tb / nb) * zchunk, z1 = min(z0 + zchunk, D);
     const int y = y0 - 1 + r;
-    const int xb = lane * VX;
-    const bool rowin = y >= 0 && y < H;
-    const bool own = rowin && r >= 1 && r <= ty;
+    const int xb = TL ? (t % nbx) * tx - VX + lane * VX : lane * VX;   // global x of element 0
+    // (W % VX == 0 and x0 % VX == 0: a lane lies wholly inside or outside the volume)
+    const bool rowin = y >= 0 && y < H && (!TL || (xb >= 0 && xb < W));
+    const bool cl = !TL || lane > 0, cr = !TL || lane < L - 1;
+    const bool own = rowin && r >= 1 && r <= ty && cl && cr;
     const I plane = I(W) * I(H);
     const I col = rowin ? I(y) * I(W) + I(xb) : I(0);
     const bool use_bz = !L2 || D > 1;
     const float tl = tau * lambda;
 
     // LDS carve-up
-    const int RW = RT * W, RE = RT * (L + 1);
+    const int RW = RT * RWD, RE = RT * (L + 1);
     float* const sPY = lds;               // P region: p_y, q_xy, q_yy, q_yz rows
     float* const sQXY = sPY + RW;
     float* const sQYY = sQXY + RW;
@@ -934,18 +942,19 @@ __global__ __launch_bounds__(MAXT) void fused_sweep_kernel(
     float* const sQXYp = sQXXp + RE;
     float* const sQXZp = sQXYp + RE;
     float* const sUB = sQXZp + RE;        // D region: u_bar, v_bar x3 rows
-    float* const sVB = sUB + RW;          // [3][RT][W]
+    float* const sVB = sUB + RW;          // [3][RT][RWD]
     float* const sUBp = sVB + 3 * RW;     // u_bar element 0 at [r][lane]
     float* const sVBm = sUBp + RE;        // [3][RT][L+1]: v_bar element VX-1 at [r][lane+1]
 
-    // zero everything once: edge slots at x = -1 / x = W and rows outside the
-    // volume are never written afterwards and read only under false flags
+    // zero everything once: edge slots at x = -1 / x = W (and beyond the halo
+    // lanes) and rows outside the volume are never written afterwards and read
+    // only under false flags
     for (int k = int(threadIdx.x); k < 8 * RW + 8 * RE; k += int(blockDim.x)) lds[k] = 0.f;
     __syncthreads();
 
     const T* const* src = reinterpret_cast<const T* const*>(ff.src);
     T* const* dst = reinterpret_cast<T* const*>(ff.dst);
-    const int rw = r * W + xb;            // this thread's offset in a row array
+    const int rw = r * RWD + lane * VX;   // this thread's offset in a row array
     const int re = r * (L + 1) + lane;    // this thread's edge slot
 
     float p_c[3][VX], q_c[6][VX];         // p, q at plane s - 1 (dual base; p_z for B_z)
@@ -1022,8 +1031,8 @@ __global__ __launch_bounds__(MAXT) void fused_sweep_kernel(
         if (PF && s < z1) load_plane(s + 1, Pn);
 
         // ---- primal half at plane s
-        const bool need_u = rowin && pin && r > 0 && s >= z0;
-        const bool need_v = rowin && pin && r < RT - 1 && s < z1;
+        const bool need_u = rowin && pin && r > 0 && cl && s >= z0;
+        const bool need_v = rowin && pin && r < RT - 1 && cr && s < z1;
         float ubn[VX], vbn[3][VX];
         zero(ubn);
 #pragma unroll
@@ -1031,7 +1040,7 @@ __global__ __launch_bounds__(MAXT) void fused_sweep_kernel(
         float uo[VX], vo[3][VX];
         if (need_u) {
             float pym[VX];
-            lds_get<VX>(&sPY[rw - W], pym);
+            lds_get<VX>(&sPY[rw - RWD], pym);
             const float pxm = sPXm[re];
 #pragma unroll
             for (int e = 0; e < VX; ++e) {
@@ -1057,9 +1066,9 @@ __global__ __launch_bounds__(MAXT) void fused_sweep_kernel(
         }
         if (need_v) {
             float qxyn_y[VX], qyyn_y[VX], qyzn_y[VX];
-            lds_get<VX>(&sQXY[rw + W], qxyn_y);
-            lds_get<VX>(&sQYY[rw + W], qyyn_y);
-            lds_get<VX>(&sQYZ[rw + W], qyzn_y);
+            lds_get<VX>(&sQXY[rw + RWD], qxyn_y);
+            lds_get<VX>(&sQYY[rw + RWD], qyyn_y);
+            lds_get<VX>(&sQYZ[rw + RWD], qyzn_y);
             const float qxxn_x = sQXXp[re + 1];
             const float qxyn_x = sQXYp[re + 1];
             const float qxzn_x = sQXZp[re + 1];
@@ -1104,11 +1113,11 @@ __global__ __launch_bounds__(MAXT) void fused_sweep_kernel(
         if (own && zd >= z0) {
             float ubc[VX], uyn[VX], vym[3][VX], vxm[3];
             lds_get<VX>(&sUB[rw], ubc);
-            lds_get<VX>(&sUB[rw + W], uyn);
+            lds_get<VX>(&sUB[rw + RWD], uyn);
             const float uxn = sUBp[re + 1];
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
-                lds_get<VX>(&sVB[k * RW + rw - W], vym[k]);
+                lds_get<VX>(&sVB[k * RW + rw - RWD], vym[k]);
                 vxm[k] = sVBm[k * RE + re];
             }
             const I id = I(zd) * plane + col;
@@ -1358,18 +1367,21 @@ hipError_t primal_tiles(hipStream_t s, void* const* fv, const float* f, Geometry
     }
 }
 
-template <class S, class I, int VX, int L2, int MK, int MAXT, int PF>
+template <class S, class I, int VX, int L2, int MK, int MAXT, int PF, int TL = 0>
 hipError_t fused_launch(hipStream_t s, const FusedFields& ff, const float* f, Geometry g,
                         const SweepParams& p, const FusedPlan& plan) {
-    const int L = g.w / VX, RT = plan.ty + 2;
+    const int row = TL ? plan.tx + 2 * VX : g.w;
+    const int L = row / VX, RT = plan.ty + 2;
+    if (L * RT > MAXT) return hipErrorInvalidConfiguration;
+    const int nbx = TL ? (g.w + plan.tx - 1) / plan.tx : 1;
     const int nb = (g.h + plan.ty - 1) / plan.ty, nc = (g.d + plan.zchunk - 1) / plan.zchunk;
-    const size_t lds = fused_lds_floats<VX>(g.w, plan.ty) * sizeof(float);
-    auto kern = fused_sweep_kernel<S, I, VX, L2, MK, MAXT, PF>;
+    const size_t lds = fused_lds_floats<VX>(row, plan.ty) * sizeof(float);
+    auto kern = fused_sweep_kernel<S, I, VX, L2, MK, MAXT, PF, TL>;
     hipError_t e = ensure_dynamic_lds(reinterpret_cast<const void*>(kern), lds);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(kern, dim3(nb * nc), dim3(L * RT), lds, s, ff, f, p.active, g, p.tau,
+    hipLaunchKernelGGL(kern, dim3(nbx * nb * nc), dim3(L * RT), lds, s, ff, f, p.active, g, p.tau,
                        p.sigma, p.lambda, p.alpha0, p.alpha1, p.theta, plan.ty, plan.zchunk,
-                       p.planar);
+                       p.planar, TL ? plan.tx : 0);
     return hipGetLastError();
 }
 
@@ -1395,6 +1407,14 @@ hipError_t fused_pf(hipStream_t s, const FusedFields& ff, const float* f, Geomet
     if constexpr (sizeof(I) == 8) {
         return fused_shape<S, I, MK, 0>(s, ff, f, g, p, plan);
     } else {
+        if (plan.tx > 0) {   // x tiles, VX = 2
+            if (plan.pf && (plan.tx / 2 + 2) * (plan.ty + 2) <= 768)
+                return p.data_term == 1
+                           ? fused_launch<S, I, 2, 1, MK, 768, 1, 1>(s, ff, f, g, p, plan)
+                           : fused_launch<S, I, 2, 0, MK, 768, 1, 1>(s, ff, f, g, p, plan);
+            return p.data_term == 1 ? fused_launch<S, I, 2, 1, MK, 1024, 0, 1>(s, ff, f, g, p, plan)
+                                    : fused_launch<S, I, 2, 0, MK, 1024, 0, 1>(s, ff, f, g, p, plan);
+        }
         // VX = 2 with the prefetch: blocks of up to 768 threads (three waves per SIMD)
         if (plan.pf && plan.vx == 2 && (g.w / 2) * (plan.ty + 2) <= 768)
             return p.data_term == 1 ? fused_launch<S, I, 2, 1, MK, 768, 1>(s, ff, f, g, p, plan)
@@ -1424,7 +1444,7 @@ hipError_t fused_dispatch(hipStream_t s, const FusedFields& ff, const float* f, 
 // halo rows count half; with FP32 fields the halo traffic dominates.  Ties go
 // to vx = 2 and to the lower band.
 // AMB_FUSED=0 disables the fused sweep; AMB_FUSED_VX / _TY / _ZCHUNK / _PF
-// override.
+// override, and AMB_FUSED_TX the tile width of rows too wide for whole rows.
 bool fused_plan(const Geometry& g, int storage, int device, FusedPlan* plan) {
     const char* en = std::getenv("AMB_FUSED");
     if (en && en[0] == '0') return false;
@@ -1438,12 +1458,13 @@ bool fused_plan(const Geometry& g, int storage, int device, FusedPlan* plan) {
         return e ? std::atoi(e) : 0;
     };
     const int want_vx = env_int("AMB_FUSED_VX"), want_ty = env_int("AMB_FUSED_TY");
-    const int want_zc = env_int("AMB_FUSED_ZCHUNK");
+    const int want_zc = env_int("AMB_FUSED_ZCHUNK"), want_tx = env_int("AMB_FUSED_TX");
     const bool big_index = !small_index(g);
     double best = 1e300;
-    FusedPlan bp{0, 0, 1, 0};
+    FusedPlan bp{0, 0, 1, 0, 0};
     for (int vx : {2, 4}) {
         if (want_vx && vx != want_vx) continue;
+        if (want_tx > 0) break;   // x tiles forced
         if (g.w % vx != 0 || (vx == 2 && big_index)) continue;
         const int L = g.w / vx;
         const int max_threads = vx == 4 ? 512 : 1024;
@@ -1472,6 +1493,46 @@ bool fused_plan(const Geometry& g, int storage, int device, FusedPlan* plan) {
             }
         }
     }
+    if (bp.ty == 0 && !big_index && g.w % 2 == 0 && (!want_vx || want_vx == 2)) {
+        // rows too wide for whole-row bands: tiles of tx columns (even) plus one
+        // two-voxel halo lane on each side, FP32 with the prefetch (<= 768
+        // threads), BF16 without (<= 1024); the halo lanes count in the cost
+        const int maxt = storage == 0 ? 768 : 1024, cap = storage == 0 ? 3 : 4;
+        for (int txc : {252, 124, 380, 508, 256, 128}) {
+            const int tx = want_tx ? want_tx : txc;
+            if (tx % 2 != 0 || tx >= g.w) { if (want_tx) break; continue; }
+            const int lanes = tx / 2 + 2;
+            const long nbx = (g.w + tx - 1) / tx;
+            for (int rt = 3; rt * lanes <= maxt && rt - 2 <= g.h + 1; ++rt) {
+                const int ty = rt - 2;
+                if (want_ty && ty != want_ty) continue;
+                const size_t lds = fused_lds_floats<2>(tx + 4, ty) * sizeof(float);
+                if (lds > size_t(max_lds)) break;
+                const int waves_simd = ((rt * lanes + 63) / 64 + 3) / 4;
+                const int per_cu = std::max(1, std::min(int((160 * 1024) / lds), cap / waves_simd));
+                const long conc = long(cus) * per_cu;
+                const long nb = nbx * ((g.h + ty - 1) / ty);
+                for (int nc = 1; nc <= g.d; ++nc) {
+                    const int zc = want_zc ? want_zc : (g.d + nc - 1) / nc;
+                    const long blocks = nb * ((g.d + zc - 1) / zc);
+                    const long rounds = (blocks + conc - 1) / conc;
+                    const double cost = double(rounds) * per_cu * (ty + 1.0) * (zc + 1) *
+                                        (2.0 * lanes / tx);
+                    if (cost < best) {
+                        best = cost;
+                        bp = FusedPlan{ty, zc, storage == 0 ? 1 : 0, 2, tx};
+                    }
+                    if (want_zc) break;
+                }
+            }
+            if (want_tx) break;
+        }
+        if (bp.ty == 0) return false;
+        const char* epf = std::getenv("AMB_FUSED_PF");
+        if (epf && (epf[0] == '0' || epf[0] == '1')) bp.pf = epf[0] == '1';
+        *plan = bp;
+        return true;
+    }
     if (bp.ty == 0) return false;
     // FP32 fields, two voxels per thread, at most 768 threads: the next plane's
     // loads issued one step ahead fit the 168 VGPRs of three waves per SIMD
@@ -1485,7 +1546,8 @@ bool fused_plan(const Geometry& g, int storage, int device, FusedPlan* plan) {
 }
 
 size_t fused_lds_bytes(const Geometry& g, const FusedPlan& plan) {
-    return (plan.vx == 2 ? fused_lds_floats<2>(g.w, plan.ty) : fused_lds_floats<4>(g.w, plan.ty)) *
+    const int row = plan.tx > 0 ? plan.tx + 2 * plan.vx : g.w;
+    return (plan.vx == 2 ? fused_lds_floats<2>(row, plan.ty) : fused_lds_floats<4>(row, plan.ty)) *
            sizeof(float);
 }
 

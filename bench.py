@@ -16,6 +16,8 @@ the same workload on the host cores.
     python bench.py --workload adjoint_4x128    # unrolled TGV2-L2 + adjoint, 4 x 128^3
     python bench.py --workload tgvk3_256        # TGV-k (k = 3) deshade 256^3
     python bench.py --workload downsampled_512  # 512^3 resized to 256^3, masked TGV-k, back
+    python bench.py --workload masked_256       # masked TGV2-L1 deshade 256^3, spherical mask
+    python bench.py --workload planar_4096      # 2-D TGV2-L1 deshade of a 4096^2 image
     python bench.py --copy-calibration          # plain 1 GiB device copies (PMC scale)
 
 Unless --no-split, the default workload also runs in two split forms -- FP32
@@ -316,6 +318,103 @@ def deshade_256(args):
     ctx.close()
 
 
+def _tgv2_variant(args, f, mask, flags, workload, metric, data, cpu_note):
+    """One TGV2-L1 deshade pipeline per step on the given volume (masked and/or
+    2-D per `flags`): throughput, the dominant sweep's roofline, per-kernel
+    times and the oracle's loop on a bounded sample as the CPU baseline."""
+    from ambient_image_processor_amd import native
+    shape = f.shape
+    n = f.size
+    tau = float(1.0 / np.sqrt(20.0))
+    p = native.default_params(lam=0.5, alpha0=2.0, alpha1=1.0, tau=tau, sigma=tau, theta=1.0,
+                              iterations=args.iterations, data_term=native.DATA_L1, flags=flags)
+    ctx = native.Context(0)
+    bufs = [ctx.alloc(n * 4) for _ in range(4 if mask is None else 5)]
+    f_dev, u_dev, l_dev, r_dev = bufs[:4]
+    m_dev = bufs[4] if mask is not None else None
+    ctx.upload(f_dev, f)
+    if mask is not None:
+        ctx.upload(m_dev, mask)
+    elapsed, prof = timed(ctx, lambda: ctx.deshade_pipeline_device(
+        f_dev, shape, p, u_dev, l_dev, r_dev, mask_dev=m_dev), args.steps, args.warmup)
+    value = float(n) * args.iterations * args.steps / elapsed
+    kernels = sweep_kernels(prof, 4, n)
+    dom = max(kernels, key=lambda k: kernels[k]["avg_us"] * kernels[k]["launches"])
+    cpu = None
+    if not args.no_cpu_baseline:
+        from oracle import oracle
+        oracle.build()
+        planar = bool(flags & native.FLAG_PLANAR)
+        m_it = mask if (flags & native.FLAG_MASKED_ITERATION) else None
+        st = oracle.TGVState(f, np.float32)
+        kw = dict(lam=0.5, alpha0=2.0, alpha1=1.0, tau=tau, sigma=tau, data_term="l1",
+                  mask=m_it, planar=planar)
+        st.iterate(1, **kw)
+        t1 = time.perf_counter()
+        st.iterate(args.cpu_iters, **kw)
+        cpu_s = time.perf_counter() - t1
+        cpu = {"value": float(n) * args.cpu_iters / cpu_s, "unit": "voxel-iters/s",
+               "cores": oracle.num_threads(), "kind": "port",
+               "sample": f"{args.cpu_iters} {cpu_note} (FP32 reference op order, OpenMP), "
+                         f"{cpu_s:.1f} s"}
+    print(json.dumps({
+        "metric": metric, "value": value, "unit": "voxel-iters/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1000.0 * elapsed / args.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": data,
+        "config": {"workload": workload, "shape": list(shape), "iterations": args.iterations,
+                   "lambda": 0.5, "alpha0": 2.0, "alpha1": 1.0, "parallelism": "single-gpu"},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": kernels[dom]["gbs"],
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": kernels[dom]["gbs"] / HBM_PEAK_GBS, "traffic": None,
+                     "bytes_per_launch": kernels[dom]["bytes_per_launch"]},
+        "kernels": rounded(kernels),
+        "other_kernels_ms_per_step": {k: round(prof[k][0] / args.steps, 3)
+                                      for k in ("dct", "poisson", "divergence", "epilogue", "init")},
+        "cpu_baseline": cpu}))
+    for ptr in bufs:
+        ctx.free(ptr)
+    ctx.close()
+
+
+def masked_256(args):
+    """Masked TGV2-L1 deshade (tgv_deshade_masked/, TGVDeshadeMaskedProcessor) of
+    the 256^3 volume with a spherical mask (mask 1 inside a ball holding about
+    half of the voxels, 0.5 outside): the masked iteration (updates only where
+    mask >= 1), the Poisson solve, the shading times the mask, the clamp and the
+    background added back.  The masked sweeps also read one active byte per
+    voxel (not counted in the algorithmic bytes)."""
+    from ambient_image_processor_amd import native, phantoms
+    n_side = args.size
+    shape = (n_side, n_side, n_side)
+    f, _, _ = phantoms.deshade_phantom(shape, seed=2, kind="spheres", noise=0.03, bias_order=3,
+                                       bias_sigma=0.05)
+    c = (n_side - 1) / 2.0
+    z, y, x = np.ogrid[0:n_side, 0:n_side, 0:n_side]
+    ball = ((z - c) ** 2 + (y - c) ** 2 + (x - c) ** 2) < (0.495 * n_side) ** 2
+    mask = np.where(ball, 1.0, 0.5).astype(np.float32)
+    flags = (native.FLAG_MASKED_ITERATION | native.FLAG_SET_NEGATIVE_ZERO
+             | native.FLAG_ADD_BACKGROUND_BACK)
+    _tgv2_variant(args, f, mask, flags, f"tgv2_l1_masked_deshade_{n_side}^3",
+                  f"voxel-iters/s masked TGV2-L1 deshade {n_side}^3 + %HBM roofline",
+                  "synthetic (32 seeded spheres + DCT bias + N(0,0.03^2) noise, spherical mask)",
+                  f"masked TGV2-L1 iterations of the same {n_side}^3 volume")
+
+
+def planar_4096(args):
+    """2-D TGV2-L1 deshade (tgv2_l1_deshade_2d, the reference's depth-1 loop) of
+    a 4096 x 4096 image, as many pixels as the 256^3 volume has voxels."""
+    from ambient_image_processor_amd import native, phantoms
+    side = 4096
+    f, _, _ = phantoms.deshade_phantom((1, side, side), seed=2, kind="spheres", noise=0.03,
+                                       bias_order=3, bias_sigma=0.05)
+    _tgv2_variant(args, f, None, native.FLAG_PLANAR | native.FLAG_SET_NEGATIVE_ZERO,
+                  f"tgv2_l1_deshade_2d_{side}^2",
+                  f"pixel-iters/s 2-D TGV2-L1 deshade {side}^2 + %HBM roofline",
+                  "synthetic (seeded discs + DCT bias + N(0,0.03^2) noise), depth 1",
+                  f"2-D TGV2-L1 iterations of the same {side}^2 image")
+
+
 def poisson_512(args):
     """DCT Poisson integration of v = D+ phi at 512 x 512 x 256, phi a sum of
     64 cosine modes: B-divergence, forward DCT with the eigenvalue divide, and
@@ -587,7 +686,8 @@ def main():
     ap.add_argument("--no-split", action="store_true")
     ap.add_argument("--no-host", action="store_true")
     ap.add_argument("--workload", choices=["deshade_256", "poisson_512", "bf16_384",
-                                           "adjoint_4x128", "tgvk3_256", "downsampled_512"],
+                                           "adjoint_4x128", "tgvk3_256", "downsampled_512",
+                                           "masked_256", "planar_4096"],
                     default="deshade_256")
     ap.add_argument("--copy-calibration", action="store_true")
     args = ap.parse_args()
@@ -602,7 +702,8 @@ def main():
         return
     {"deshade_256": deshade_256, "poisson_512": poisson_512, "bf16_384": bf16_384,
      "adjoint_4x128": adjoint_4x128, "tgvk3_256": tgvk3_256,
-     "downsampled_512": downsampled_512}[args.workload](args)
+     "downsampled_512": downsampled_512, "masked_256": masked_256,
+     "planar_4096": planar_4096}[args.workload](args)
 
 
 if __name__ == "__main__":

@@ -186,3 +186,88 @@ def test_fused_state_continues_and_energy(ctx, monkeypatch):
     state = ctx.get_state()
     np.testing.assert_array_equal(state, ref_state)
     np.testing.assert_array_equal(ctx.energy(), ref_e)
+
+
+# ---------------------------------------------------------------------------
+# x tiles (rows too wide for whole-row bands; forced on small volumes)
+# ---------------------------------------------------------------------------
+TILE_CASES = [((9, 13, 64), "16", "2", "3"), ((9, 13, 64), "30", "5", "0"),
+              ((5, 20, 130), "62", "0", "2"), ((7, 9, 34), "8", "1", "1"),
+              ((6, 7, 40), "38", "3", "4")]
+
+
+@pytest.mark.parametrize("shape,tile,band,zchunk", TILE_CASES)
+@pytest.mark.parametrize("data_term", ["l1", "l2"])
+def test_fused_x_tiles_against_oracle(ctx, orc, monkeypatch, shape, tile, band, zchunk,
+                                      data_term):
+    """Halo lanes on both sides of a tile, partial last tiles, band and chunk
+    edges: u and v equal the FP32 oracle bit for bit."""
+    from ambient_image_processor_amd import native
+    monkeypatch.setenv("AMB_FUSED_TX", tile)
+    if band != "0":
+        monkeypatch.setenv("AMB_FUSED_TY", band)
+    if zchunk != "0":
+        monkeypatch.setenv("AMB_FUSED_ZCHUNK", zchunk)
+    f = _f(shape, 50)
+    dt = native.DATA_L1 if data_term == "l1" else native.DATA_L2
+    p = native.default_params(lam=0.7, alpha0=2.0, alpha1=1.0, tau=TAU, sigma=TAU, theta=1.0,
+                              iterations=14, data_term=dt)
+    ctx.profile(True)
+    ctx.profile_reset()
+    u, vx, vy, vz = ctx.tgv2_deshade(f, p)
+    prof = ctx.profile_read()
+    ctx.profile(False)
+    assert prof["fused_sweep"][1] == 13, prof
+    st = orc.tgv2(f, 14, lam=0.7, alpha0=2.0, alpha1=1.0, tau=TAU, sigma=TAU,
+                  data_term=data_term, dtype=np.float32)
+    np.testing.assert_array_equal(u, st.u)
+    np.testing.assert_array_equal(vx, st.v[0])
+    np.testing.assert_array_equal(vy, st.v[1])
+    np.testing.assert_array_equal(vz, st.v[2])
+
+
+@pytest.mark.parametrize("tile", ["30", "62"])
+def test_fused_x_tiles_bf16_masked_planar(ctx, orc, monkeypatch, tile):
+    """BF16 fields, the masked loop and the 2-D loop in x tiles equal the
+    two-sweep loop (and, for the FP32 forms, the oracle) bit for bit."""
+    from ambient_image_processor_amd import native
+    f = _f((6, 11, 96), 51)
+    pb = native.default_params(lam=0.7, alpha0=2.0, alpha1=1.0, tau=TAU, sigma=TAU, theta=1.0,
+                               iterations=12, data_term=native.DATA_L1,
+                               storage=native.STORAGE_BF16)
+    act = np.random.default_rng(52).random(f.shape) > 0.3
+    pm = native.default_params(lam=0.7, alpha0=2.0, alpha1=1.0, tau=TAU, sigma=TAU, theta=1.0,
+                               iterations=12, data_term=native.DATA_L1)
+    f2 = _f((1, 30, 96), 53)
+    p2 = native.default_params(lam=0.7, alpha0=2.0, alpha1=1.0, tau=TAU, sigma=TAU, theta=1.0,
+                               iterations=12, data_term=native.DATA_L1, flags=native.FLAG_PLANAR)
+    monkeypatch.setenv("AMB_FUSED_TX", tile)
+    tiled = [ctx.tgv2_deshade(f, pb), ctx.tgv2_deshade_masked(f, pm, act),
+             ctx.tgv2_deshade(f2, p2)]
+    monkeypatch.setenv("AMB_FUSED", "0")
+    two = [ctx.tgv2_deshade(f, pb), ctx.tgv2_deshade_masked(f, pm, act), ctx.tgv2_deshade(f2, p2)]
+    for a, b in zip(tiled, two):
+        for x, y in zip(a, b):
+            np.testing.assert_array_equal(x, y)
+    st = orc.tgv2(f2, 12, lam=0.7, alpha0=2.0, alpha1=1.0, tau=TAU, sigma=TAU, data_term="l1",
+                  dtype=np.float32, planar=True)
+    np.testing.assert_array_equal(tiled[2][0], st.u)
+
+
+def test_fused_wide_rows_run_tiled(ctx, orc):
+    """Rows of 1400 voxels (too wide for whole-row bands) take x tiles by
+    themselves."""
+    from ambient_image_processor_amd import native
+    f = _f((3, 6, 1400), 54)
+    p = native.default_params(lam=0.7, alpha0=2.0, alpha1=1.0, tau=TAU, sigma=TAU, theta=1.0,
+                              iterations=10, data_term=native.DATA_L1)
+    ctx.profile(True)
+    ctx.profile_reset()
+    u, vx, vy, vz = ctx.tgv2_deshade(f, p)
+    prof = ctx.profile_read()
+    ctx.profile(False)
+    assert prof["fused_sweep"][1] == 9, prof
+    st = orc.tgv2(f, 10, lam=0.7, alpha0=2.0, alpha1=1.0, tau=TAU, sigma=TAU, data_term="l1",
+                  dtype=np.float32)
+    np.testing.assert_array_equal(u, st.u)
+    np.testing.assert_array_equal(vz, st.v[2])
