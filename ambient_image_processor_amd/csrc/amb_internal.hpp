@@ -81,6 +81,9 @@ struct SweepParams {
     // depth 1 only: the reference's 2-D algebra (no z backward differences in
     // eps_B(v_bar); the z components of v, p, q stay 0)
     int planar;
+    // planar, and every copy of v_z, p_z, q_zz, q_xz, q_yz holds +0: the fused
+    // sweep may take them as +0 without loading or storing them
+    int planar_zero_z;
 };
 
 // Column-tile shapes of the sweep kernels: TX x TY threads, each thread owning

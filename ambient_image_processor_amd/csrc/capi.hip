@@ -90,6 +90,7 @@ struct amb_context {
     size_t partials_cap = 0;
     double* energy_out = nullptr;
     bool have_state = false;
+    bool z_zero = false;   // v_z, p_z, q_zz, q_xz, q_yz are +0 (from begin(), not set_state)
     SweepParams sp{};
     amb_tgv_params params{};
     // cosine tables per axis, keyed by extent
@@ -451,6 +452,7 @@ amb_status begin(amb_context* c, amb_memory mem, const float* f, uint32_t w, uin
     c->sp.planar = (p->flags & AMB_FLAG_PLANAR) ? 1 : 0;
     c->fused_ok = fused_plan(g, c->storage, c->device, &c->fplan);
     c->have_state = true;
+    c->z_zero = true;
     return AMB_OK;
 }
 
@@ -545,6 +547,13 @@ amb_status run(amb_context* c, uint32_t iterations, uint32_t interval, amb_itera
                void* user, bool with_v) {
     const bool fused = c->fused_ok && iterations > 1;
     if (fused) AMB_TRY(ensure_alt(c));
+    // the 2-D loop keeps v_z, p_z, q_zz, q_xz, q_yz at +0 in the state; with
+    // their second copies cleared too, the fused sweep skips them
+    c->sp.planar_zero_z = fused && c->sp.planar && c->storage == 0 && c->z_zero ? 1 : 0;
+    if (c->sp.planar_zero_z) {
+        const int zf[5] = {3, 6, 9, 11, 12};   // vz, pz, qzz, qxz, qyz in the fused order
+        for (int k : zf) AMB_HIP(hipMemsetAsync(c->alt[k], 0, c->g.n() * 4, c->stream));
+    }
     bool need_dual = true;
     for (uint32_t it = 0; it < iterations; ++it) {
         const bool observed = cb && interval > 0 && it > 0 && it % interval == 0;
@@ -1023,6 +1032,7 @@ AMB_API amb_status amb_tgv2_set_state(amb_context* c, amb_memory mem, const floa
     if (!c->have_state) return fail(AMB_ERR_INVALID_ARGUMENT, "no iteration state: call amb_tgv2_begin");
     AMB_TRY(set_device(c));
     const size_t n = c->g.n();
+    c->z_zero = false;
     for (int k = 0; k < N_FIELDS; ++k) {
         if (!fields[k]) return fail(AMB_ERR_INVALID_ARGUMENT, "field pointer is NULL");
         if (c->storage == 0) {

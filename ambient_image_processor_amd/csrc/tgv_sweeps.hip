@@ -32,6 +32,7 @@
 
 #include <map>
 #include <mutex>
+#include <type_traits>
 #include <utility>
 
 #include <algorithm>
@@ -901,7 +902,10 @@ struct FusedPlane {
 // TL = 1: the block owns `tx` columns (tx % VX == 0) instead of whole rows, with
 // one halo lane (VX voxels) on each side: lane 0 feeds only v_bar at x0 - 1,
 // the last lane only u_bar at x0 + tx (rows wider than whole-row bands allow).
-template <class S, class I, int VX, int L2, int MK, int MAXT, int PF, int TL>
+// P2 = 1 (the 2-D loop, depth 1): v_z, p_z, q_zz, q_xz and q_yz stay +0 through
+// every update there, so they are taken as +0 instead of loaded and are not
+// stored (both copies of them must hold +0); the arithmetic is unchanged.
+template <class S, class I, int VX, int L2, int MK, int MAXT, int PF, int TL, int P2>
 __global__ __launch_bounds__(MAXT) void fused_sweep_kernel(
     FusedFields ff, const float* __restrict__ f_, const uint8_t* __restrict__ act, Geometry g,
     float tau, float sigma, float lambda, float alpha0, float alpha1, float theta, int ty,
@@ -967,7 +971,7 @@ __global__ __launch_bounds__(MAXT) void fused_sweep_kernel(
     for (int k = 0; k < 6; ++k) zero(q_c[k]);
 
     const int zs = z0 > 0 ? z0 - 1 : 0;
-    if (rowin && zs < D) {
+    if (!P2 && rowin && zs < D) {
         const I i = I(zs) * plane + col;
         V::ld(src[G_QZZ], i, qn[0]); V::ld(src[G_QXZ], i, qn[1]); V::ld(src[G_QYZ], i, qn[2]);
     }
@@ -976,11 +980,13 @@ __global__ __launch_bounds__(MAXT) void fused_sweep_kernel(
         P.act = 0xffffffffu;
         if (rowin && zz < D) {
             const I i = I(zz) * plane + col;
-            V::ld(src[G_PX], i, P.px); V::ld(src[G_PY], i, P.py); V::ld(src[G_PZ], i, P.pz);
+            V::ld(src[G_PX], i, P.px); V::ld(src[G_PY], i, P.py);
+            if (P2) zero(P.pz); else V::ld(src[G_PZ], i, P.pz);
             V::ld(src[G_QXX], i, P.qxx); V::ld(src[G_QXY], i, P.qxy); V::ld(src[G_QYY], i, P.qyy);
             V::ld(src[G_U], i, P.u); VF::ld(f_, i, P.f);
-            V::ld(src[G_VX], i, P.v[0]); V::ld(src[G_VY], i, P.v[1]); V::ld(src[G_VZ], i, P.v[2]);
-            if (zz + 1 < D) {
+            V::ld(src[G_VX], i, P.v[0]); V::ld(src[G_VY], i, P.v[1]);
+            if (P2) zero(P.v[2]); else V::ld(src[G_VZ], i, P.v[2]);
+            if (!P2 && zz + 1 < D) {
                 const I j = i + plane;
                 V::ld(src[G_QZZ], j, P.qnn[0]); V::ld(src[G_QXZ], j, P.qnn[1]);
                 V::ld(src[G_QYZ], j, P.qnn[2]);
@@ -1102,7 +1108,8 @@ __global__ __launch_bounds__(MAXT) void fused_sweep_kernel(
         }
         if (own && pin && s >= z0 && s < z1) {
             V::st(dst[G_U], i, uo);
-            V::st(dst[G_VX], i, vo[0]); V::st(dst[G_VY], i, vo[1]); V::st(dst[G_VZ], i, vo[2]);
+            V::st(dst[G_VX], i, vo[0]); V::st(dst[G_VY], i, vo[1]);
+            if (!P2) V::st(dst[G_VZ], i, vo[2]);
         }
 
         // ---- dual half at plane zd = s - 1 (D region holds u_bar, v_bar at zd)
@@ -1179,10 +1186,13 @@ __global__ __launch_bounds__(MAXT) void fused_sweep_kernel(
                     for (int k = 0; k < 6; ++k) qo[k][e] = q_c[k][e];
                 }
             }
-            V::st(dst[G_PX], id, po[0]); V::st(dst[G_PY], id, po[1]); V::st(dst[G_PZ], id, po[2]);
-            V::st(dst[G_QXX], id, qo[0]); V::st(dst[G_QYY], id, qo[1]);
-            V::st(dst[G_QZZ], id, qo[2]); V::st(dst[G_QXY], id, qo[3]);
-            V::st(dst[G_QXZ], id, qo[4]); V::st(dst[G_QYZ], id, qo[5]);
+            V::st(dst[G_PX], id, po[0]); V::st(dst[G_PY], id, po[1]);
+            V::st(dst[G_QXX], id, qo[0]); V::st(dst[G_QYY], id, qo[1]); V::st(dst[G_QXY], id, qo[3]);
+            if (!P2) {
+                V::st(dst[G_PZ], id, po[2]);
+                V::st(dst[G_QZZ], id, qo[2]); V::st(dst[G_QXZ], id, qo[4]);
+                V::st(dst[G_QYZ], id, qo[5]);
+            }
         }
         __syncthreads();
 
@@ -1367,7 +1377,7 @@ hipError_t primal_tiles(hipStream_t s, void* const* fv, const float* f, Geometry
     }
 }
 
-template <class S, class I, int VX, int L2, int MK, int MAXT, int PF, int TL = 0>
+template <class S, class I, int VX, int L2, int MK, int MAXT, int PF, int TL = 0, int P2 = 0>
 hipError_t fused_launch(hipStream_t s, const FusedFields& ff, const float* f, Geometry g,
                         const SweepParams& p, const FusedPlan& plan) {
     const int row = TL ? plan.tx + 2 * VX : g.w;
@@ -1376,7 +1386,7 @@ hipError_t fused_launch(hipStream_t s, const FusedFields& ff, const float* f, Ge
     const int nbx = TL ? (g.w + plan.tx - 1) / plan.tx : 1;
     const int nb = (g.h + plan.ty - 1) / plan.ty, nc = (g.d + plan.zchunk - 1) / plan.zchunk;
     const size_t lds = fused_lds_floats<VX>(row, plan.ty) * sizeof(float);
-    auto kern = fused_sweep_kernel<S, I, VX, L2, MK, MAXT, PF, TL>;
+    auto kern = fused_sweep_kernel<S, I, VX, L2, MK, MAXT, PF, TL, P2>;
     hipError_t e = ensure_dynamic_lds(reinterpret_cast<const void*>(kern), lds);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(kern, dim3(nbx * nb * nc), dim3(L * RT), lds, s, ff, f, p.active, g, p.tau,
@@ -1407,6 +1417,28 @@ hipError_t fused_pf(hipStream_t s, const FusedFields& ff, const float* f, Geomet
     if constexpr (sizeof(I) == 8) {
         return fused_shape<S, I, MK, 0>(s, ff, f, g, p, plan);
     } else {
+        if constexpr (std::is_same<S, StoreF32>::value) {
+            if (p.planar_zero_z && plan.vx == 2) {   // the 2-D loop: z fields taken as +0
+                const bool pf = plan.pf && (plan.tx > 0 ? (plan.tx / 2 + 2) : g.w / 2) *
+                                               (plan.ty + 2) <= 768;
+                if (plan.tx > 0) {
+                    if (pf)
+                        return p.data_term == 1
+                                   ? fused_launch<S, I, 2, 1, MK, 768, 1, 1, 1>(s, ff, f, g, p, plan)
+                                   : fused_launch<S, I, 2, 0, MK, 768, 1, 1, 1>(s, ff, f, g, p, plan);
+                    return p.data_term == 1
+                               ? fused_launch<S, I, 2, 1, MK, 1024, 0, 1, 1>(s, ff, f, g, p, plan)
+                               : fused_launch<S, I, 2, 0, MK, 1024, 0, 1, 1>(s, ff, f, g, p, plan);
+                }
+                if (pf)
+                    return p.data_term == 1
+                               ? fused_launch<S, I, 2, 1, MK, 768, 1, 0, 1>(s, ff, f, g, p, plan)
+                               : fused_launch<S, I, 2, 0, MK, 768, 1, 0, 1>(s, ff, f, g, p, plan);
+                return p.data_term == 1
+                           ? fused_launch<S, I, 2, 1, MK, 1024, 0, 0, 1>(s, ff, f, g, p, plan)
+                           : fused_launch<S, I, 2, 0, MK, 1024, 0, 0, 1>(s, ff, f, g, p, plan);
+            }
+        }
         if (plan.tx > 0) {   // x tiles, VX = 2
             if (plan.pf && (plan.tx / 2 + 2) * (plan.ty + 2) <= 768)
                 return p.data_term == 1
@@ -1462,6 +1494,24 @@ bool fused_plan(const Geometry& g, int storage, int device, FusedPlan* plan) {
     const bool big_index = !small_index(g);
     double best = 1e300;
     FusedPlan bp{0, 0, 1, 0, 0};
+    // depth 1 (images) with rows too wide for whole-row bands: nothing along z
+    // amortises the halo rows of a band, so such images run in tiles of 124
+    // columns (64 two-voxel lanes) and bands of 6 rows, not the tile the cost
+    // model below would pick.  Measured per iteration: 4096^2 456 us (the cost
+    // model's tile 681 us, bands of 4 / 10 rows 543 / 501 us, 252-column tiles
+    // 536 us, the two sweeps 564 us); 1024^2 38 us (the two sweeps 52 us).
+    // Images of at most 682 columns keep whole rows (512^2: 11.5 us, tiles 14.2).
+    const bool image = g.d == 1 && !big_index && g.w % 2 == 0 && !want_vx && want_tx == 0;
+    if (image && 3 * (g.w / 2) > 1024) {
+        const int ty = std::min(want_ty ? want_ty : 6, g.h);
+        if (size_t(fused_lds_floats<2>(128, ty)) * sizeof(float) <= size_t(max_lds)) {
+            FusedPlan ip{ty, 1, storage == 0 ? 1 : 0, 2, 124};
+            const char* epf = std::getenv("AMB_FUSED_PF");
+            if (epf && (epf[0] == '0' || epf[0] == '1')) ip.pf = epf[0] == '1';
+            *plan = ip;
+            return true;
+        }
+    }
     for (int vx : {2, 4}) {
         if (want_vx && vx != want_vx) continue;
         if (want_tx > 0) break;   // x tiles forced

@@ -107,7 +107,12 @@ def sweep_bytes(kind, storage_bytes, n):
       fused  (primal half of one iteration + dual half of the next; u_bar and
              v_bar stay on chip) read u, f, v x3, p x3, q x6; write u, v x3,
              p x3, q x6                                                26 + f
-    (108 B per voxel-iteration fused at FP32; 176 B for the two sweeps.)"""
+    (108 B per voxel-iteration fused at FP32; 176 B for the two sweeps.)  The
+    fused 2-D sweep (depth 1) takes v_z, p_z, q_zz, q_xz, q_yz as 0 without
+    moving them: reads u, f, v x2, p x2, q x3, writes u, v x2, p x2, q x3 =
+    16 fields + f (68 B per pixel-iteration at FP32)."""
+    if kind == "fused_sweep_2d":
+        return (16 * storage_bytes + 4) * n
     if kind == "dual_sweep":
         return 22 * storage_bytes * n
     if kind == "primal_sweep":
@@ -175,14 +180,14 @@ def timed(ctx, step, steps, warmup):
     return elapsed, prof
 
 
-def sweep_kernels(prof, sb, n):
+def sweep_kernels(prof, sb, n, planar=False):
     kernels = {}
     for name in SWEEPS:
         ms, cnt = prof[name]
         if cnt == 0:
             continue
         avg_ms = ms / cnt
-        b = sweep_bytes(name, sb, n)
+        b = sweep_bytes("fused_sweep_2d" if planar and name == "fused_sweep" else name, sb, n)
         kernels[name] = {"avg_us": 1000.0 * avg_ms, "launches": cnt,
                          "bytes_per_launch": b, "gbs": b / (avg_ms * 1e-3) / 1e9}
         if name == "fused_sweep":   # one fused launch does one iteration's work
@@ -338,7 +343,7 @@ def _tgv2_variant(args, f, mask, flags, workload, metric, data, cpu_note):
     elapsed, prof = timed(ctx, lambda: ctx.deshade_pipeline_device(
         f_dev, shape, p, u_dev, l_dev, r_dev, mask_dev=m_dev), args.steps, args.warmup)
     value = float(n) * args.iterations * args.steps / elapsed
-    kernels = sweep_kernels(prof, 4, n)
+    kernels = sweep_kernels(prof, 4, n, planar=bool(flags & native.FLAG_PLANAR))
     dom = max(kernels, key=lambda k: kernels[k]["avg_us"] * kernels[k]["launches"])
     cpu = None
     if not args.no_cpu_baseline:

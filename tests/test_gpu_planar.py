@@ -254,3 +254,44 @@ def test_reference_launcher_caller_2d(orc, tmp_path, masked):
     np.testing.assert_array_equal(u, uo)
     np.testing.assert_array_equal(l, lo)
     np.testing.assert_array_equal(rr, ro)
+
+
+def test_planar_state_with_nonzero_z_fields(ctx, monkeypatch):
+    """The fused 2-D sweep skips v_z, p_z, q_zz, q_xz, q_yz only while they hold
+    +0 (the state from begin()); a state set with other values there is swept
+    with them, as the two-sweep loop does."""
+    from ambient_image_processor_amd import native
+    f = _f((1, 24, 64), 60)
+    p = _params(native, 0)
+    rng = np.random.default_rng(61)
+    state = (rng.standard_normal((native.N_FIELDS,) + f.shape) * 0.1).astype(np.float32)
+    runs = []
+    for fused in ("1", "0"):
+        monkeypatch.setenv("AMB_FUSED", fused)
+        ctx.begin(f, p)
+        ctx.set_state(state)
+        ctx.iterate(9)
+        runs.append(ctx.get_state())
+    np.testing.assert_array_equal(runs[0], runs[1])
+
+
+@pytest.mark.parametrize("shape,tile", [((1, 40, 1400), "0"), ((1, 36, 200), "62"),
+                                        ((1, 17, 96), "30")])
+def test_planar_wide_and_tiled_images(ctx, orc, monkeypatch, shape, tile):
+    """Rows too wide for whole-row bands (1400 pixels) and forced tiles run the
+    fused 2-D sweep, z fields skipped, bit for bit with the oracle."""
+    from ambient_image_processor_amd import native
+    if tile != "0":
+        monkeypatch.setenv("AMB_FUSED_TX", tile)
+    f = _f(shape, 62)
+    ctx.profile(True)
+    ctx.profile_reset()
+    u, vx, vy, vz = ctx.tgv2_deshade(f, _params(native, 16))
+    prof = ctx.profile_read()
+    ctx.profile(False)
+    assert prof["fused_sweep"][1] == 15, prof
+    st = orc.tgv2(f, 16, dtype=np.float32, data_term="l1", planar=True, **PRM)
+    np.testing.assert_array_equal(u, st.u)
+    np.testing.assert_array_equal(vx, st.v[0])
+    np.testing.assert_array_equal(vy, st.v[1])
+    assert not np.any(vz)
