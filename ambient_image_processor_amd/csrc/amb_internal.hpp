@@ -133,6 +133,12 @@ hipError_t launch_fused_sweep(hipStream_t s, int storage, void* const* src, void
                               const FusedPlan& plan);
 constexpr int N_FUSED_FIELDS = 13;
 extern const int FUSED_FIELD_MAP[N_FUSED_FIELDS];
+// The 2-D loop (planar, FP32 fields, z components +0 in both copies) as a
+// fused sweep marched along the image's rows; same src / dst13 convention as
+// launch_fused_sweep.
+bool fused_image_plan(const Geometry& g, int device, FusedPlan* plan);
+hipError_t launch_fused_image(hipStream_t s, void* const* src, void* const* dst13, const float* f,
+                              Geometry g, const SweepParams& p, const FusedPlan& plan);
 
 hipError_t launch_dual_sweep(hipStream_t s, int storage, void* const* fields, Geometry g,
                              const SweepParams& p);

@@ -116,6 +116,8 @@ struct amb_context {
     void* alt[N_FUSED_FIELDS] = {};
     bool fused_ok = false;
     FusedPlan fplan{0, 0};
+    bool iplan_ok = false;   // the 2-D loop marched along the image's rows
+    FusedPlan iplan{0, 0};
     // TGV-k: 18 fields per level (w[6], w_bar[6], r[6]; grow-only), their
     // device pointer table and the k weights
     char* kblock = nullptr;
@@ -451,6 +453,7 @@ amb_status begin(amb_context* c, amb_memory mem, const float* f, uint32_t w, uin
     c->sp.active = nullptr;
     c->sp.planar = (p->flags & AMB_FLAG_PLANAR) ? 1 : 0;
     c->fused_ok = fused_plan(g, c->storage, c->device, &c->fplan);
+    c->iplan_ok = c->sp.planar && c->storage == 0 && fused_image_plan(g, c->device, &c->iplan);
     c->have_state = true;
     c->z_zero = true;
     return AMB_OK;
@@ -490,8 +493,11 @@ amb_status ensure_alt(amb_context* c) {
 amb_status fused_step(amb_context* c) {
     {
         ProfScope ps(c, P_FUSED);
-        AMB_HIP(launch_fused_sweep(c->stream, c->storage, c->fields, c->alt, c->f, c->g, c->sp,
-                                   c->fplan));
+        if (c->sp.planar_zero_z && c->iplan_ok)
+            AMB_HIP(launch_fused_image(c->stream, c->fields, c->alt, c->f, c->g, c->sp, c->iplan));
+        else
+            AMB_HIP(launch_fused_sweep(c->stream, c->storage, c->fields, c->alt, c->f, c->g, c->sp,
+                                       c->fplan));
     }
     for (int k = 0; k < N_FUSED_FIELDS; ++k) std::swap(c->fields[FUSED_FIELD_MAP[k]], c->alt[k]);
     return AMB_OK;
@@ -545,7 +551,8 @@ amb_status sweep(amb_context* c, int which) {
 // only by the primal sweep; the last iteration is always a primal sweep.
 amb_status run(amb_context* c, uint32_t iterations, uint32_t interval, amb_iteration_callback cb,
                void* user, bool with_v) {
-    const bool fused = c->fused_ok && iterations > 1;
+    const bool image = c->iplan_ok && c->z_zero;
+    const bool fused = (c->fused_ok || image) && iterations > 1;
     if (fused) AMB_TRY(ensure_alt(c));
     // the 2-D loop keeps v_z, p_z, q_zz, q_xz, q_yz at +0 in the state; with
     // their second copies cleared too, the fused sweep skips them
@@ -1276,7 +1283,10 @@ AMB_API amb_status amb_tgv2_deshade_pipeline(amb_context* c, amb_memory mem, con
     const bool masked = mask && (p->flags & AMB_FLAG_MASKED_ITERATION);
     // the masked processor takes the 2-D loop for depth-1 images
     // (TGVDeshadeMaskedProcessor.cpp:93-98)
-    if (masked && d == 1) c->sp.planar = 1;
+    if (masked && d == 1 && !c->sp.planar) {
+        c->sp.planar = 1;
+        c->iplan_ok = c->storage == 0 && fused_image_plan(c->g, c->device, &c->iplan);
+    }
     if (masked) {
         // active set = mask >= 1 (TGVDeshadeMaskedProcessor.cpp:348-353); f stays unmasked
         // (:99 clones it before the mask multiply at :109)

@@ -905,6 +905,13 @@ struct FusedPlane {
 // P2 = 1 (the 2-D loop, depth 1): v_z, p_z, q_zz, q_xz and q_yz stay +0 through
 // every update there, so they are taken as +0 instead of loaded and are not
 // stored (both copies of them must hold +0); the arithmetic is unchanged.
+// P2 = 2 (the 2-D loop marched along the image's rows): the host passes the
+// 1 x H x W image as an H x 1 x W volume with the fields' y and z roles
+// exchanged (v_y <-> v_z, p_y <-> p_z, q_yy <-> q_zz, q_xy <-> q_xz); the
+// kernel's y components are then the image's zero z components (taken as +0,
+// not stored), its y backward differences of eps_B(v_bar) are 0 (the 2-D
+// algebra), bands have no halo rows (height 1), and every nonzero term of
+// every update keeps its place in the reference's operation order.
 template <class S, class I, int VX, int L2, int MK, int MAXT, int PF, int TL, int P2>
 __global__ __launch_bounds__(MAXT) void fused_sweep_kernel(
     FusedFields ff, const float* __restrict__ f_, const uint8_t* __restrict__ act, Geometry g,
@@ -916,7 +923,8 @@ __global__ __launch_bounds__(MAXT) void fused_sweep_kernel(
     extern __shared__ __align__(16) float lds[];
 
     const int W = g.w, H = g.h, D = g.d;
-    const int L = TL ? tx / VX + 2 : W / VX, RT = ty + 2;
+    constexpr bool SKZ = P2 == 1, IMG = P2 == 2;
+    const int L = TL ? tx / VX + 2 : W / VX, RT = IMG ? ty : ty + 2;
     const int RWD = L * VX;               // floats per LDS row
     const int lane = int(threadIdx.x) % L, r = int(threadIdx.x) / L;
     const int nbx = TL ? (W + tx - 1) / tx : 1, nb = (H + ty - 1) / ty;
@@ -924,12 +932,12 @@ __global__ __launch_bounds__(MAXT) void fused_sweep_kernel(
     const int tb = TL ? t / nbx : t;
     const int y0 = (tb % nb) * ty;
     const int z0 = (tb / nb) * zchunk, z1 = min(z0 + zchunk, D);
-    const int y = y0 - 1 + r;
+    const int y = IMG ? y0 + r : y0 - 1 + r;
     const int xb = TL ? (t % nbx) * tx - VX + lane * VX : lane * VX;   // global x of element 0
     // (W % VX == 0 and x0 % VX == 0: a lane lies wholly inside or outside the volume)
     const bool rowin = y >= 0 && y < H && (!TL || (xb >= 0 && xb < W));
     const bool cl = !TL || lane > 0, cr = !TL || lane < L - 1;
-    const bool own = rowin && r >= 1 && r <= ty && cl && cr;
+    const bool own = rowin && (IMG || (r >= 1 && r <= ty)) && cl && cr;
     const I plane = I(W) * I(H);
     const I col = rowin ? I(y) * I(W) + I(xb) : I(0);
     const bool use_bz = !L2 || D > 1;
@@ -971,25 +979,30 @@ __global__ __launch_bounds__(MAXT) void fused_sweep_kernel(
     for (int k = 0; k < 6; ++k) zero(q_c[k]);
 
     const int zs = z0 > 0 ? z0 - 1 : 0;
-    if (!P2 && rowin && zs < D) {
+    if (!SKZ && rowin && zs < D) {
         const I i = I(zs) * plane + col;
-        V::ld(src[G_QZZ], i, qn[0]); V::ld(src[G_QXZ], i, qn[1]); V::ld(src[G_QYZ], i, qn[2]);
+        V::ld(src[G_QZZ], i, qn[0]); V::ld(src[G_QXZ], i, qn[1]);
+        if (!IMG) V::ld(src[G_QYZ], i, qn[2]);
     }
 
     auto load_plane = [&](int zz, FusedPlane<VX>& P) {
         P.act = 0xffffffffu;
         if (rowin && zz < D) {
             const I i = I(zz) * plane + col;
-            V::ld(src[G_PX], i, P.px); V::ld(src[G_PY], i, P.py);
-            if (P2) zero(P.pz); else V::ld(src[G_PZ], i, P.pz);
-            V::ld(src[G_QXX], i, P.qxx); V::ld(src[G_QXY], i, P.qxy); V::ld(src[G_QYY], i, P.qyy);
+            V::ld(src[G_PX], i, P.px);
+            if (IMG) zero(P.py); else V::ld(src[G_PY], i, P.py);
+            if (SKZ) zero(P.pz); else V::ld(src[G_PZ], i, P.pz);
+            V::ld(src[G_QXX], i, P.qxx);
+            if (IMG) { zero(P.qxy); zero(P.qyy); }
+            else { V::ld(src[G_QXY], i, P.qxy); V::ld(src[G_QYY], i, P.qyy); }
             V::ld(src[G_U], i, P.u); VF::ld(f_, i, P.f);
-            V::ld(src[G_VX], i, P.v[0]); V::ld(src[G_VY], i, P.v[1]);
-            if (P2) zero(P.v[2]); else V::ld(src[G_VZ], i, P.v[2]);
-            if (!P2 && zz + 1 < D) {
+            V::ld(src[G_VX], i, P.v[0]);
+            if (IMG) zero(P.v[1]); else V::ld(src[G_VY], i, P.v[1]);
+            if (SKZ) zero(P.v[2]); else V::ld(src[G_VZ], i, P.v[2]);
+            if (!SKZ && zz + 1 < D) {
                 const I j = i + plane;
                 V::ld(src[G_QZZ], j, P.qnn[0]); V::ld(src[G_QXZ], j, P.qnn[1]);
-                V::ld(src[G_QYZ], j, P.qnn[2]);
+                if (IMG) zero(P.qnn[2]); else V::ld(src[G_QYZ], j, P.qnn[2]);
             } else {
                 zero(P.qnn[0]); zero(P.qnn[1]); zero(P.qnn[2]);
             }
@@ -1037,8 +1050,8 @@ __global__ __launch_bounds__(MAXT) void fused_sweep_kernel(
         if (PF && s < z1) load_plane(s + 1, Pn);
 
         // ---- primal half at plane s
-        const bool need_u = rowin && pin && r > 0 && cl && s >= z0;
-        const bool need_v = rowin && pin && r < RT - 1 && cr && s < z1;
+        const bool need_u = rowin && pin && (IMG || r > 0) && cl && s >= z0;
+        const bool need_v = rowin && pin && (IMG || r < RT - 1) && cr && s < z1;
         float ubn[VX], vbn[3][VX];
         zero(ubn);
 #pragma unroll
@@ -1046,7 +1059,7 @@ __global__ __launch_bounds__(MAXT) void fused_sweep_kernel(
         float uo[VX], vo[3][VX];
         if (need_u) {
             float pym[VX];
-            lds_get<VX>(&sPY[rw - RWD], pym);
+            if (IMG) zero(pym); else lds_get<VX>(&sPY[rw - RWD], pym);
             const float pxm = sPXm[re];
 #pragma unroll
             for (int e = 0; e < VX; ++e) {
@@ -1072,9 +1085,13 @@ __global__ __launch_bounds__(MAXT) void fused_sweep_kernel(
         }
         if (need_v) {
             float qxyn_y[VX], qyyn_y[VX], qyzn_y[VX];
-            lds_get<VX>(&sQXY[rw + RWD], qxyn_y);
-            lds_get<VX>(&sQYY[rw + RWD], qyyn_y);
-            lds_get<VX>(&sQYZ[rw + RWD], qyzn_y);
+            if (IMG) {   // no y neighbours (height 1)
+                zero(qxyn_y); zero(qyyn_y); zero(qyzn_y);
+            } else {
+                lds_get<VX>(&sQXY[rw + RWD], qxyn_y);
+                lds_get<VX>(&sQYY[rw + RWD], qyyn_y);
+                lds_get<VX>(&sQYZ[rw + RWD], qyzn_y);
+            }
             const float qxxn_x = sQXXp[re + 1];
             const float qxyn_x = sQXYp[re + 1];
             const float qxzn_x = sQXZp[re + 1];
@@ -1108,8 +1125,9 @@ __global__ __launch_bounds__(MAXT) void fused_sweep_kernel(
         }
         if (own && pin && s >= z0 && s < z1) {
             V::st(dst[G_U], i, uo);
-            V::st(dst[G_VX], i, vo[0]); V::st(dst[G_VY], i, vo[1]);
-            if (!P2) V::st(dst[G_VZ], i, vo[2]);
+            V::st(dst[G_VX], i, vo[0]);
+            if (!IMG) V::st(dst[G_VY], i, vo[1]);
+            if (!SKZ) V::st(dst[G_VZ], i, vo[2]);
         }
 
         // ---- dual half at plane zd = s - 1 (D region holds u_bar, v_bar at zd)
@@ -1120,11 +1138,11 @@ __global__ __launch_bounds__(MAXT) void fused_sweep_kernel(
         if (own && zd >= z0) {
             float ubc[VX], uyn[VX], vym[3][VX], vxm[3];
             lds_get<VX>(&sUB[rw], ubc);
-            lds_get<VX>(&sUB[rw + RWD], uyn);
+            if (IMG) zero(uyn); else lds_get<VX>(&sUB[rw + RWD], uyn);
             const float uxn = sUBp[re + 1];
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
-                lds_get<VX>(&sVB[k * RW + rw - RWD], vym[k]);
+                if (IMG) zero(vym[k]); else lds_get<VX>(&sVB[k * RW + rw - RWD], vym[k]);
                 vxm[k] = sVBm[k * RE + re];
             }
             const I id = I(zd) * plane + col;
@@ -1153,7 +1171,7 @@ __global__ __launch_bounds__(MAXT) void fused_sweep_kernel(
                 for (int k = 0; k < 3; ++k) {
                     const float vc = vbc[k][e];
                     bxv[k] = bwd(vc, e > 0 ? vbc[k][e > 0 ? e - 1 : 0] : vxm[k], x > 0);
-                    byv[k] = bwd(vc, vym[k][e], y > 0);
+                    byv[k] = IMG ? 0.0f : bwd(vc, vym[k][e], y > 0);
                     bzv[k] = planar ? 0.0f : bwd(vc, vb_m2[k][e], zd > 0);
                 }
                 const float exy = (bxv[1] + byv[0]) * 0.5f;
@@ -1186,13 +1204,16 @@ __global__ __launch_bounds__(MAXT) void fused_sweep_kernel(
                     for (int k = 0; k < 6; ++k) qo[k][e] = q_c[k][e];
                 }
             }
-            V::st(dst[G_PX], id, po[0]); V::st(dst[G_PY], id, po[1]);
-            V::st(dst[G_QXX], id, qo[0]); V::st(dst[G_QYY], id, qo[1]); V::st(dst[G_QXY], id, qo[3]);
-            if (!P2) {
+            V::st(dst[G_PX], id, po[0]); V::st(dst[G_QXX], id, qo[0]);
+            if (!IMG) {
+                V::st(dst[G_PY], id, po[1]); V::st(dst[G_QYY], id, qo[1]);
+                V::st(dst[G_QXY], id, qo[3]);
+            }
+            if (!SKZ) {
                 V::st(dst[G_PZ], id, po[2]);
                 V::st(dst[G_QZZ], id, qo[2]); V::st(dst[G_QXZ], id, qo[4]);
-                V::st(dst[G_QYZ], id, qo[5]);
             }
+            if (!SKZ && !IMG) V::st(dst[G_QYZ], id, qo[5]);
         }
         __syncthreads();
 
@@ -1381,11 +1402,11 @@ template <class S, class I, int VX, int L2, int MK, int MAXT, int PF, int TL = 0
 hipError_t fused_launch(hipStream_t s, const FusedFields& ff, const float* f, Geometry g,
                         const SweepParams& p, const FusedPlan& plan) {
     const int row = TL ? plan.tx + 2 * VX : g.w;
-    const int L = row / VX, RT = plan.ty + 2;
+    const int L = row / VX, RT = P2 == 2 ? plan.ty : plan.ty + 2;
     if (L * RT > MAXT) return hipErrorInvalidConfiguration;
     const int nbx = TL ? (g.w + plan.tx - 1) / plan.tx : 1;
     const int nb = (g.h + plan.ty - 1) / plan.ty, nc = (g.d + plan.zchunk - 1) / plan.zchunk;
-    const size_t lds = fused_lds_floats<VX>(row, plan.ty) * sizeof(float);
+    const size_t lds = (size_t(8) * RT * row + size_t(8) * RT * (L + 1)) * sizeof(float);
     auto kern = fused_sweep_kernel<S, I, VX, L2, MK, MAXT, PF, TL, P2>;
     hipError_t e = ensure_dynamic_lds(reinterpret_cast<const void*>(kern), lds);
     if (e != hipSuccess) return e;
@@ -1619,6 +1640,65 @@ hipError_t launch_fused_sweep(hipStream_t s, int storage, void* const* src, void
                      : fused_dispatch<StoreBF16, size_t>(s, ff, f, g, p, plan);
     return small ? fused_dispatch<StoreF32, uint32_t>(s, ff, f, g, p, plan)
                  : fused_dispatch<StoreF32, size_t>(s, ff, f, g, p, plan);
+}
+
+// The 2-D loop marched along the image's rows (fused_sweep_kernel with P2 = 2):
+// the 1 x H x W image as an H x 1 x W volume, the fields' y and z roles
+// exchanged.  Used for images wider than 682 pixels (narrower ones run
+// whole-row bands, which measured faster: 512^2 11.7 vs 15.0 us per launch).
+// Plan: blocks of one row of two-voxel tiles of 124 columns (images up to 2048
+// wide) or 252 columns, a halo lane on each side, and chunks of image rows for
+// about 24 blocks per CU, 4 to 64 rows.  Measured per launch: 1024^2 27.6 us
+// (tiled depth-1 bands 38.0, two sweeps 43), 2048^2 79-91 us, 4096^2
+// 293-324 us (tiled depth-1 bands 454, two sweeps 545).
+bool fused_image_plan(const Geometry& g, int device, FusedPlan* plan) {
+    if (g.d != 1 || g.w % 2 != 0 || !small_index(g)) return false;
+    const char* en = std::getenv("AMB_FUSED_IMAGE");
+    if (en && en[0] == '0') return false;
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess)
+        return false;
+    const char* etx = std::getenv("AMB_FUSED_IMAGE_TX");
+    const char* ezc = std::getenv("AMB_FUSED_IMAGE_ZCHUNK");
+    if (!etx && 3 * (g.w / 2) <= 1024) return false;   // whole-row bands fit
+    int tx = g.w <= 2048 ? 124 : 252;
+    if (etx) tx = std::atoi(etx) >= g.w ? 0 : std::atoi(etx);
+    if (tx < 0 || tx % 2 != 0 || (tx == 0 && g.w / 2 > 768)) return false;
+    const long nbx = tx ? (g.w + tx - 1) / tx : 1;
+    int zc = int(std::min(64L, std::max(4L, long(g.h) * nbx / (24L * cus))));
+    zc = std::min(zc, g.h);
+    if (ezc) zc = std::max(1, std::min(std::atoi(ezc), g.h));
+    *plan = FusedPlan{1, zc, 1, 2, tx};
+    return true;
+}
+
+hipError_t launch_fused_image(hipStream_t s, void* const* src, void* const* dst13, const float* f,
+                              Geometry g, const SweepParams& p, const FusedPlan& plan) {
+    // fused order u, vx, vy, vz, px, py, pz, qxx, qyy, qzz, qxy, qxz, qyz with
+    // y and z exchanged: the kernel's y slots get the image's (zero) z fields
+    static const int swap[N_GFIELDS] = {0, 1, 3, 2, 4, 6, 5, 7, 9, 8, 11, 10, 12};
+    FusedFields ff;
+    for (int k = 0; k < N_GFIELDS; ++k) {
+        ff.src[k] = src[FUSED_FIELD_MAP[swap[k]]];
+        ff.dst[k] = dst13[swap[k]];
+    }
+    const Geometry gi{g.w, 1, g.h};
+    SweepParams pi = p;
+    pi.planar = 0;
+    typedef StoreF32 S;
+    typedef uint32_t I;
+    if (plan.tx > 0) {
+        if (p.active)
+            return p.data_term == 1 ? fused_launch<S, I, 2, 1, 1, 768, 1, 1, 2>(s, ff, f, gi, pi, plan)
+                                    : fused_launch<S, I, 2, 0, 1, 768, 1, 1, 2>(s, ff, f, gi, pi, plan);
+        return p.data_term == 1 ? fused_launch<S, I, 2, 1, 0, 768, 1, 1, 2>(s, ff, f, gi, pi, plan)
+                                : fused_launch<S, I, 2, 0, 0, 768, 1, 1, 2>(s, ff, f, gi, pi, plan);
+    }
+    if (p.active)
+        return p.data_term == 1 ? fused_launch<S, I, 2, 1, 1, 768, 1, 0, 2>(s, ff, f, gi, pi, plan)
+                                : fused_launch<S, I, 2, 0, 1, 768, 1, 0, 2>(s, ff, f, gi, pi, plan);
+    return p.data_term == 1 ? fused_launch<S, I, 2, 1, 0, 768, 1, 0, 2>(s, ff, f, gi, pi, plan)
+                            : fused_launch<S, I, 2, 0, 0, 768, 1, 0, 2>(s, ff, f, gi, pi, plan);
 }
 
 hipError_t launch_dual_sweep(hipStream_t s, int storage, void* const* fields, Geometry g,

@@ -295,3 +295,39 @@ def test_planar_wide_and_tiled_images(ctx, orc, monkeypatch, shape, tile):
     np.testing.assert_array_equal(vx, st.v[0])
     np.testing.assert_array_equal(vy, st.v[1])
     assert not np.any(vz)
+
+
+ROW_MARCH = [((1, 37, 96), "100000", "5"), ((1, 50, 130), "30", "7"), ((1, 23, 64), "16", "1"),
+             ((1, 9, 40), "8", "3"), ((1, 40, 1400), "0", "0")]
+
+
+@pytest.mark.parametrize("shape,tx,zc", ROW_MARCH)
+@pytest.mark.parametrize("data_term", ["l1", "l2"])
+@pytest.mark.parametrize("masked", [False, True])
+def test_planar_row_march(ctx, orc, monkeypatch, shape, tx, zc, data_term, masked):
+    """The 2-D loop marched along the image's rows (the image as an H x 1 x W
+    volume with y and z exchanged): whole rows (tile 100000) and tiles, chunk
+    edges, masked and unmasked, bit for bit with the oracle's planar loop."""
+    from ambient_image_processor_amd import native
+    _say("row march", shape, tx, zc, data_term, masked)
+    if tx != "0":
+        monkeypatch.setenv("AMB_FUSED_IMAGE_TX", tx)
+    if zc != "0":
+        monkeypatch.setenv("AMB_FUSED_IMAGE_ZCHUNK", zc)
+    f = _f(shape, 63)
+    p = _params(native, 16, data_term)
+    m = _mask(shape, 64) if masked else None
+    ctx.profile(True)
+    ctx.profile_reset()
+    if masked:
+        u, vx, vy, vz = ctx.tgv2_deshade_masked(f, p, m >= 1)
+    else:
+        u, vx, vy, vz = ctx.tgv2_deshade(f, p)
+    prof = ctx.profile_read()
+    ctx.profile(False)
+    assert prof["fused_sweep"][1] == 15, prof
+    st = orc.tgv2(f, 16, dtype=np.float32, data_term=data_term, planar=True, mask=m, **PRM)
+    np.testing.assert_array_equal(u, st.u)
+    np.testing.assert_array_equal(vx, st.v[0])
+    np.testing.assert_array_equal(vy, st.v[1])
+    assert not np.any(vz)
