@@ -1646,11 +1646,13 @@ hipError_t launch_fused_sweep(hipStream_t s, int storage, void* const* src, void
 // the 1 x H x W image as an H x 1 x W volume, the fields' y and z roles
 // exchanged.  Used for images wider than 682 pixels (narrower ones run
 // whole-row bands, which measured faster: 512^2 11.7 vs 15.0 us per launch).
-// Plan: blocks of one row of two-voxel tiles of 124 columns (images up to 2048
-// wide) or 252 columns, a halo lane on each side, and chunks of image rows for
-// about 24 blocks per CU, 4 to 64 rows.  Measured per launch: 1024^2 27.6 us
-// (tiled depth-1 bands 38.0, two sweeps 43), 2048^2 79-91 us, 4096^2
-// 293-324 us (tiled depth-1 bands 454, two sweeps 545).
+// Plan: blocks of one row of two-voxel lanes, a whole number of waves: tiles of
+// 124 columns (64 lanes with the halo lane on each side; images up to 2048
+// wide) or 508 columns (256 lanes), and chunks of image rows for about 24
+// blocks per CU, 4 to 64 rows.  Measured per launch: 1024^2 26-28 us (tiled
+// depth-1 bands 38, two sweeps 43); 2048^2 79-91 us; 4096^2 284 us with 508
+// columns (380: 285, 252: 302, 224: 291, 1020: 322; 512, i.e. 258 lanes: 390;
+// tiled depth-1 bands 454, two sweeps 545).
 bool fused_image_plan(const Geometry& g, int device, FusedPlan* plan) {
     if (g.d != 1 || g.w % 2 != 0 || !small_index(g)) return false;
     const char* en = std::getenv("AMB_FUSED_IMAGE");
@@ -1661,7 +1663,7 @@ bool fused_image_plan(const Geometry& g, int device, FusedPlan* plan) {
     const char* etx = std::getenv("AMB_FUSED_IMAGE_TX");
     const char* ezc = std::getenv("AMB_FUSED_IMAGE_ZCHUNK");
     if (!etx && 3 * (g.w / 2) <= 1024) return false;   // whole-row bands fit
-    int tx = g.w <= 2048 ? 124 : 252;
+    int tx = g.w <= 2048 ? 124 : 508;
     if (etx) tx = std::atoi(etx) >= g.w ? 0 : std::atoi(etx);
     if (tx < 0 || tx % 2 != 0 || (tx == 0 && g.w / 2 > 768)) return false;
     const long nbx = tx ? (g.w + tx - 1) / tx : 1;

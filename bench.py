@@ -56,9 +56,9 @@ SWEEPS = ("dual_sweep", "primal_sweep", "fused_sweep")
 # 2.0 and WRITE_SIZE by 1.0 (what the counters read for a plain 1 GiB copy);
 # summaries in profiles/.  Keyed by (kernel, storage bytes, edge length); the
 # TGV-k entries by (kernel, levels, edge length) (k = 3, FP32 fields).
-MEASURED_TRAFFIC = {("fused_sweep", 4, 256): 1857372102,
-                    ("dual_sweep", 4, 256): 1531281680,
-                    ("primal_sweep", 4, 256): 1532567692,
+MEASURED_TRAFFIC = {("fused_sweep", 4, 256): 1857559048,
+                    ("dual_sweep", 4, 256): 1532122086,
+                    ("primal_sweep", 4, 256): 1532759954,
                     ("tgvk_dual", 1, 256): 2908697085,
                     ("tgvk_primal", 1, 256): 3563855669,
                     ("tgvk_fused", 1, 256): 3479840059}
