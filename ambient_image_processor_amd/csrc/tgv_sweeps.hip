@@ -1436,6 +1436,9 @@ template <class S, class I, int MK>
 hipError_t fused_pf(hipStream_t s, const FusedFields& ff, const float* f, Geometry g,
                     const SweepParams& p, const FusedPlan& plan) {
     if constexpr (sizeof(I) == 8) {
+        if (plan.tx > 0)   // x tiles, VX = 4 (64-bit offsets)
+            return p.data_term == 1 ? fused_launch<S, I, 4, 1, MK, 512, 0, 1, 0>(s, ff, f, g, p, plan)
+                                    : fused_launch<S, I, 4, 0, MK, 512, 0, 1, 0>(s, ff, f, g, p, plan);
         return fused_shape<S, I, MK, 0>(s, ff, f, g, p, plan);
     } else {
         if constexpr (std::is_same<S, StoreF32>::value) {
@@ -1564,19 +1567,48 @@ bool fused_plan(const Geometry& g, int storage, int device, FusedPlan* plan) {
             }
         }
     }
+    if (bp.ty == 0 && big_index && g.w % 4 == 0 && (!want_vx || want_vx == 4)) {
+        // 64-bit offsets and rows too wide for whole-row bands: tiles of 248
+        // columns (64 four-voxel lanes with a halo lane on each side) and bands
+        // of up to 6 rows in 512-thread blocks; the cost model picks the z chunk
+        const int tx = want_tx > 0 ? want_tx : 248;
+        if (tx % 4 == 0 && tx < g.w) {
+            const int lanes = tx / 4 + 2, ty = std::min(want_ty ? want_ty : 6, g.h);
+            const size_t lds = fused_lds_floats<4>(tx + 8, ty) * sizeof(float);
+            if (lanes * (ty + 2) <= 512 && lds <= size_t(max_lds)) {
+                const long nb = long((g.w + tx - 1) / tx) * ((g.h + ty - 1) / ty);
+                // one block per CU: the four-voxel form's VGPRs allow two waves
+                // per SIMD, which one 512-thread block fills
+                for (int nc = 1; nc <= g.d; ++nc) {
+                    const int zc = want_zc ? want_zc : (g.d + nc - 1) / nc;
+                    const long blocks = nb * ((g.d + zc - 1) / zc);
+                    const long rounds = (blocks + cus - 1) / cus;
+                    const double cost = double(rounds) * (ty + 1.0) * (zc + 1);
+                    if (cost < best) { best = cost; bp = FusedPlan{ty, zc, 0, 4, tx}; }
+                    if (want_zc) break;
+                }
+            }
+        }
+        if (bp.ty == 0) return false;
+        *plan = bp;
+        return true;
+    }
     if (bp.ty == 0 && !big_index && g.w % 2 == 0 && (!want_vx || want_vx == 2)) {
-        // rows too wide for whole-row bands: tiles of tx columns (even) plus one
-        // two-voxel halo lane on each side, FP32 with the prefetch (<= 768
-        // threads), BF16 without (<= 1024); the halo lanes count in the cost
+        // rows too wide for whole-row bands: tiles of 124 columns (64 two-voxel
+        // lanes, one wave, with a halo lane on each side) and bands of up to 8
+        // rows, FP32 with the prefetch (<= 768 threads), BF16 without (<= 1024);
+        // the cost model picks the z chunk.  Measured at 768^3 per launch: about
+        // 12-13 ms for 124 x 8, 124 x 10, 188 x 6 and 252 x 4 (10% run to run),
+        // 13.4 ms for 380 x 2, 15.9 ms for 508 x 1; the two sweeps take 16.3 ms.
         const int maxt = storage == 0 ? 768 : 1024, cap = storage == 0 ? 3 : 4;
-        for (int txc : {252, 124, 380, 508, 256, 128}) {
+        for (int txc : {124}) {
             const int tx = want_tx ? want_tx : txc;
             if (tx % 2 != 0 || tx >= g.w) { if (want_tx) break; continue; }
             const int lanes = tx / 2 + 2;
             const long nbx = (g.w + tx - 1) / tx;
             for (int rt = 3; rt * lanes <= maxt && rt - 2 <= g.h + 1; ++rt) {
                 const int ty = rt - 2;
-                if (want_ty && ty != want_ty) continue;
+                if (want_ty ? ty != want_ty : ty != std::min(8, g.h)) continue;
                 const size_t lds = fused_lds_floats<2>(tx + 4, ty) * sizeof(float);
                 if (lds > size_t(max_lds)) break;
                 const int waves_simd = ((rt * lanes + 63) / 64 + 3) / 4;

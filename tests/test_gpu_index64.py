@@ -3,8 +3,8 @@ more run (every field then spans 4 GiB or more).  AMB_INDEX64=1 forces those
 forms on small volumes: they perform the same per-voxel arithmetic, so their
 results equal the 32-bit forms' bit for bit (TGV2 fused and two-sweep, L1 and
 L2, FP32 and BF16 fields, masked, 2-D; TGV-k through the generic kernels).
-One test runs a real 2049 x 1024 x 512 volume and checks a block at the far
-end of the volume, where byte offsets exceed 2^32, against the FP32 oracle run
+Real 2049 x 1024 x 512 and 1025 x 1024 x 1024 volumes (whole-row bands and x
+tiles of the fused sweep) are checked in a block at the far end of the volume, where byte offsets exceed 2^32, against the FP32 oracle run
 on a crop around it (after N iterations a voxel depends only on voxels within
 2N of it, so the crop's own edges do not reach the compared block)."""
 import sys
@@ -54,6 +54,28 @@ def test_tgv2_index64_equals_index32(ctx, monkeypatch, shape, data_term, storage
         np.testing.assert_array_equal(x, y)
 
 
+@pytest.mark.parametrize("shape,tile", [((6, 9, 96), "24"), ((5, 7, 72), "40")])
+def test_tgv2_index64_x_tiles(ctx, orc, monkeypatch, shape, tile):
+    """x tiles of the four-voxel 64-bit form (rows too wide for whole-row
+    bands at 2^30 voxels and more), forced on small volumes, against the
+    oracle bit for bit."""
+    from ambient_image_processor_amd import native
+    monkeypatch.setenv("AMB_INDEX64", "1")
+    monkeypatch.setenv("AMB_FUSED_TX", tile)
+    f = _f(shape, 46)
+    ctx.profile(True)
+    ctx.profile_reset()
+    u, vx, vy, vz = ctx.tgv2_deshade(f, _p(native, 12, 0))
+    prof = ctx.profile_read()
+    ctx.profile(False)
+    assert prof["fused_sweep"][1] == 11, prof
+    st = orc.tgv2(f, 12, lam=0.7, alpha0=2.0, alpha1=1.0, tau=TAU, sigma=TAU, data_term="l1",
+                  dtype=np.float32)
+    np.testing.assert_array_equal(u, st.u)
+    for k, v in enumerate((vx, vy, vz)):
+        np.testing.assert_array_equal(v, st.v[k])
+
+
 def test_tgv2_index64_masked_and_planar(ctx, monkeypatch):
     from ambient_image_processor_amd import native
     f = _f((6, 10, 48), 41)
@@ -82,17 +104,19 @@ def test_tgvk_index64_equals_index32(ctx, monkeypatch, alpha):
         np.testing.assert_array_equal(x, y)
 
 
-def test_tgv2_over_2_30_voxels_against_cropped_oracle(ctx, orc):
-    """2049 x 1024 x 512 (4.3 GB per field), three TGV2-L1 iterations (dual
-    sweep, two fused sweeps in the four-voxel form, primal sweep, all with
-    64-bit offsets); the last 24 planes of a 24 x 24 column window at the x
-    end equal the FP32 oracle run on the crop."""
+@pytest.mark.parametrize("dims", [(2049, 1024, 512), (1025, 1024, 1024)])
+def test_tgv2_over_2_30_voxels_against_cropped_oracle(ctx, orc, dims):
+    """2049 x 1024 x 512 and 1025 x 1024 x 1024 (4.3 GB per field), three
+    TGV2-L1 iterations (dual sweep, two fused sweeps in the four-voxel form -
+    whole rows of 512, x tiles for rows of 1024 - and the primal sweep, all
+    with 64-bit offsets); the last 24 planes of a 24 x 24 column window at the
+    x end equal the FP32 oracle run on the crop."""
     from ambient_image_processor_amd import native
-    d, h, w = 2049, 1024, 512
+    d, h, w = dims
     iters, m = 3, 8
-    _say("generating the 2049 x 1024 x 512 input")
+    _say("generating the input", dims)
     f = np.random.default_rng(45).random((d, h, w), dtype=np.float32)
-    _say("GPU: 3 iterations at 2049 x 1024 x 512")
+    _say("GPU: 3 iterations at", dims)
     ctx.profile(True)
     ctx.profile_reset()
     u, vx, vy, vz = ctx.tgv2_deshade(f, _p(native, iters, 0))
