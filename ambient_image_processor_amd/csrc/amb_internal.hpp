@@ -108,6 +108,8 @@ constexpr int TILE_VECTOR_DEFAULT = 4;
 // Launchers (stream-ordered, asynchronous).  `fields` points at 17 arrays of
 // the storage type (void* so both storages share the signature).
 hipError_t launch_init(hipStream_t s, int storage, void* const* fields, const float* f, Geometry g);
+// zero `bytes` (a multiple of 4, 4-byte aligned) at p with a fill kernel
+hipError_t launch_zero_fill(hipStream_t s, void* p, size_t bytes);
 // Raises a kernel's dynamic LDS limit on the current device to at least
 // `bytes`, once per (device, kernel, size): setting it before every launch
 // costs a gap between back-to-back launches.  Thread-safe.

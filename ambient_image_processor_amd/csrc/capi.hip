@@ -559,7 +559,7 @@ amb_status run(amb_context* c, uint32_t iterations, uint32_t interval, amb_itera
     c->sp.planar_zero_z = fused && c->sp.planar && c->storage == 0 && c->z_zero ? 1 : 0;
     if (c->sp.planar_zero_z) {
         const int zf[5] = {3, 6, 9, 11, 12};   // vz, pz, qzz, qxz, qyz in the fused order
-        for (int k : zf) AMB_HIP(hipMemsetAsync(c->alt[k], 0, c->g.n() * 4, c->stream));
+        for (int k : zf) AMB_HIP(launch_zero_fill(c->stream, c->alt[k], c->g.n() * 4));
     }
     bool need_dual = true;
     for (uint32_t it = 0; it < iterations; ++it) {
@@ -619,7 +619,7 @@ amb_status tgvk_begin(amb_context* c, amb_memory mem, const float* f, uint32_t w
         AMB_HIP(hipMalloc(reinterpret_cast<void**>(&c->kblock), need));
         c->kcap = need;
     }
-    AMB_HIP(hipMemsetAsync(c->kblock, 0, need, c->stream));
+    AMB_HIP(launch_zero_fill(c->stream, c->kblock, need));
     const size_t nt = 18 * size_t(L);
     if (nt > c->klv_cap) {
         if (c->klv_dev) AMB_HIP(hipFree(c->klv_dev));
@@ -1600,10 +1600,10 @@ AMB_API amb_status amb_tgv2_denoise_adjoint(amb_context* c, amb_memory mem, cons
     // reverse: adjoint of x_N = (dL/du, 0, ...)
     void* adj[N_FIELDS];
     state_at(adjs, adj);
-    AMB_HIP(hipMemsetAsync(adjs, 0, state_bytes, c->stream));
+    AMB_HIP(launch_zero_fill(c->stream, adjs, state_bytes));
     AMB_HIP(hipMemcpyAsync(adj[F_U], dL_du, n * 4, kind_in(mem), c->stream));
-    AMB_HIP(hipMemsetAsync(gradf, 0, n * 4, c->stream));
-    AMB_HIP(hipMemsetAsync(acc, 0, 3 * sizeof(double), c->stream));
+    AMB_HIP(launch_zero_fill(c->stream, gradf, n * 4));
+    AMB_HIP(launch_zero_fill(c->stream, acc, 3 * sizeof(double)));
     for (int seg = int(nseg) - 1; seg >= 0; --seg) {
         const uint32_t k0 = uint32_t(seg) * K, k1 = std::min(k0 + K, N), L = k1 - k0;
         void* tabs[2][N_FIELDS];

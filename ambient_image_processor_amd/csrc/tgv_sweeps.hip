@@ -1776,6 +1776,41 @@ hipError_t launch_init(hipStream_t s, int storage, void* const* fields, const fl
     return hipGetLastError();
 }
 
+// Zero fill by a kernel, not hipMemsetAsync, for the clears inside the
+// iteration calls (the 2-D loop's second z copies, the TGV-k block, the
+// adjoint's accumulators): their work on the context's stream is then kernel
+// launches and copies only, with no memset nodes should it ever be captured
+// into a HIP graph (the calls synchronise at their end, so none is today).
+template <bool VEC>
+__global__ void zero_words_kernel(uint32_t* p, size_t nwords) {
+    const size_t stride = size_t(gridDim.x) * blockDim.x;
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    size_t head = 0;
+    if (VEC) {
+        const size_t n4 = nwords / 4;
+        uint4* p4 = reinterpret_cast<uint4*>(p);
+        for (size_t k = i; k < n4; k += stride) p4[k] = make_uint4(0u, 0u, 0u, 0u);
+        head = n4 * 4;
+    }
+    for (size_t k = head + i; k < nwords; k += stride) p[k] = 0u;
+}
+
+hipError_t launch_zero_fill(hipStream_t s, void* p, size_t bytes) {
+    if (bytes == 0) return hipSuccess;
+    if (bytes % 4 != 0 || reinterpret_cast<uintptr_t>(p) % 4 != 0) return hipErrorInvalidValue;
+    const size_t nwords = bytes / 4;
+    const bool vec = reinterpret_cast<uintptr_t>(p) % 16 == 0;
+    const size_t units = vec ? (nwords + 3) / 4 : nwords;
+    const int blocks = int(std::min<size_t>((units + 255) / 256, 4096));
+    if (vec)
+        hipLaunchKernelGGL(zero_words_kernel<true>, dim3(blocks), dim3(256), 0, s,
+                           static_cast<uint32_t*>(p), nwords);
+    else
+        hipLaunchKernelGGL(zero_words_kernel<false>, dim3(blocks), dim3(256), 0, s,
+                           static_cast<uint32_t*>(p), nwords);
+    return hipGetLastError();
+}
+
 // to_storage = 1: float -> storage ; 0: storage -> float
 hipError_t launch_convert(hipStream_t s, int storage, const void* src, void* dst, size_t n,
                           int to_storage) {
