@@ -48,11 +48,19 @@ struct StoreBF16 {
     static __device__ __forceinline__ float ld(const uint16_t* p, Idx i) {
         return __uint_as_float(uint32_t(*elem(p, i)) << 16);
     }
+    // Round to nearest even with gfx950's v_cvt_pk_bf16_f32.  For every
+    // non-NaN input (denormals and overflow to infinity included) it gives the
+    // bits of the integer form (u + 0x7fff + bit 16 of u) >> 16, checked over
+    // all 2^32 patterns; a NaN stays a NaN (its payload bits may differ).
     static __device__ __forceinline__ uint16_t rne(float v) {
-        uint32_t u = __float_as_uint(v);
-        if ((u & 0x7f800000u) == 0x7f800000u) return uint16_t((u >> 16) | ((u & 0xffffu) ? 0x40u : 0u));
-        u += 0x7fffu + ((u >> 16) & 1u);
-        return uint16_t(u >> 16);
+        return __builtin_bit_cast(uint16_t, static_cast<__bf16>(v));
+    }
+    // two values in one instruction: a in the low half, b in the high half
+    static __device__ __forceinline__ uint32_t rne2(float a, float b) {
+        typedef float f2 __attribute__((ext_vector_type(2)));
+        typedef __bf16 h2 __attribute__((ext_vector_type(2)));
+        const f2 v = {a, b};
+        return __builtin_bit_cast(uint32_t, __builtin_convertvector(v, h2));
     }
     template <class Idx>
     static __device__ __forceinline__ void st(uint16_t* p, Idx i, float v) { *elem(p, i) = rne(v); }

@@ -413,8 +413,8 @@ struct VecIO<StoreBF16, 4> {
     template <class I>
     static __device__ __forceinline__ void st(uint16_t* p, I i, const float (&v)[4]) {
         uint2 t;
-        t.x = uint32_t(StoreBF16::rne(v[0])) | (uint32_t(StoreBF16::rne(v[1])) << 16);
-        t.y = uint32_t(StoreBF16::rne(v[2])) | (uint32_t(StoreBF16::rne(v[3])) << 16);
+        t.x = StoreBF16::rne2(v[0], v[1]);
+        t.y = StoreBF16::rne2(v[2], v[3]);
         *reinterpret_cast<uint2*>(elem(p, i)) = t;
     }
 };
@@ -443,8 +443,7 @@ struct VecIO<StoreBF16, 2> {
     }
     template <class I>
     static __device__ __forceinline__ void st(uint16_t* p, I i, const float (&v)[2]) {
-        *reinterpret_cast<uint32_t*>(elem(p, i)) =
-            uint32_t(StoreBF16::rne(v[0])) | (uint32_t(StoreBF16::rne(v[1])) << 16);
+        *reinterpret_cast<uint32_t*>(elem(p, i)) = StoreBF16::rne2(v[0], v[1]);
     }
 };
 
