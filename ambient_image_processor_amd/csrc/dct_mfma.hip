@@ -37,12 +37,6 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef short i16x8 __attribute__((ext_vector_type(8)));
 
-__device__ __forceinline__ uint16_t bf16_rne(float v) {
-    uint32_t u = __float_as_uint(v);
-    u += 0x7fffu + ((u >> 16) & 1u);
-    return uint16_t(u >> 16);
-}
-__device__ __forceinline__ float bf16_f(uint16_t h) { return __uint_as_float(uint32_t(h) << 16); }
 
 // Poisson eigenvalue divide on the output element (x, y, z) of the forward z
 // pass, i = z and j = y W + x (the same arithmetic as poisson_kernel in
@@ -210,16 +204,15 @@ __device__ __forceinline__ void load_table(TableRegs& T, const GemmArgs& a, int 
     }
 }
 
-// 4 floats -> packed bf16 hi and lo (2 x 64 bits)
+// 4 floats -> packed bf16 hi and lo (2 x 64 bits): hi = rne(x), lo = rne(x - hi),
+// two values per conversion instruction
 __device__ __forceinline__ void split4(const float (&x)[4], uint2& hi, uint2& lo) {
-    uint16_t h[4], l[4];
-#pragma unroll
-    for (int c = 0; c < 4; ++c) {
-        h[c] = bf16_rne(x[c]);
-        l[c] = bf16_rne(x[c] - bf16_f(h[c]));
-    }
-    hi = make_uint2(uint32_t(h[0]) | (uint32_t(h[1]) << 16), uint32_t(h[2]) | (uint32_t(h[3]) << 16));
-    lo = make_uint2(uint32_t(l[0]) | (uint32_t(l[1]) << 16), uint32_t(l[2]) | (uint32_t(l[3]) << 16));
+    const uint32_t h01 = StoreBF16::rne2(x[0], x[1]), h23 = StoreBF16::rne2(x[2], x[3]);
+    hi = make_uint2(h01, h23);
+    lo = make_uint2(StoreBF16::rne2(x[0] - __uint_as_float(h01 << 16),
+                                    x[1] - __uint_as_float(h01 & 0xffff0000u)),
+                    StoreBF16::rne2(x[2] - __uint_as_float(h23 << 16),
+                                    x[3] - __uint_as_float(h23 & 0xffff0000u)));
 }
 
 // 4 K-consecutive values of row r of operand tile X (isx) or Y, at column k
