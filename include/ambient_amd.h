@@ -50,7 +50,8 @@ typedef enum { AMB_MEM_HOST = 0, AMB_MEM_DEVICE = 1 } amb_memory;
 typedef enum { AMB_DATA_L1 = 0, AMB_DATA_L2 = 1 } amb_data_term;
 
 /* Storage type of the iteration fields u, u_bar, v, v_bar, p, q.  Arithmetic
- * is FP32 in both cases; f is always stored FP32. */
+ * is FP32 in both cases; f is always stored FP32.  BF16 stores round to
+ * nearest, ties to even. */
 typedef enum { AMB_STORAGE_F32 = 0, AMB_STORAGE_BF16 = 1 } amb_storage;
 
 /* Arithmetic of the separable DCT-II/III passes.
