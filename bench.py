@@ -698,8 +698,8 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py: this build measures one GPU only (the TGV sweeps would shard into "
-              "z-slabs exchanging one-plane halos, which is not built). Run with --gpus 1.",
+        print("bench.py measures one GPU only (the TGV sweeps would shard into z-slabs "
+              "exchanging one-plane halos, which is not implemented). Run with --gpus 1.",
               file=sys.stderr)
         sys.exit(2)
     if args.copy_calibration:
