@@ -58,7 +58,7 @@ def test_tgv2_index64_equals_index32(ctx, monkeypatch, shape, data_term, storage
 def test_tgv2_index64_x_tiles(ctx, orc, monkeypatch, shape, tile):
     """x tiles of the four-voxel 64-bit form (rows too wide for whole-row
     bands at 2^30 voxels and more), forced on small volumes, against the
-    oracle bit for bit."""
+    oracle bit for bit (FP32 fields, L1)."""
     from ambient_image_processor_amd import native
     monkeypatch.setenv("AMB_INDEX64", "1")
     monkeypatch.setenv("AMB_FUSED_TX", tile)
@@ -74,6 +74,36 @@ def test_tgv2_index64_x_tiles(ctx, orc, monkeypatch, shape, tile):
     np.testing.assert_array_equal(u, st.u)
     for k, v in enumerate((vx, vy, vz)):
         np.testing.assert_array_equal(v, st.v[k])
+
+
+@pytest.mark.parametrize("data_term,storage,masked", [(1, 0, False), (0, 1, False), (0, 0, True),
+                                                       (1, 1, True)])
+def test_tgv2_index64_x_tiles_equal_two_sweeps(ctx, monkeypatch, data_term, storage, masked):
+    """The same 64-bit x tiles with L2, BF16 fields and the masked iteration
+    equal the two-sweep loop of the 64-bit kernels bit for bit."""
+    from ambient_image_processor_amd import native
+    _say("index64 x tiles", data_term, storage, masked)
+    monkeypatch.setenv("AMB_INDEX64", "1")
+    f = _f((5, 9, 88), 47)
+    act = np.random.default_rng(48).random(f.shape) > 0.3
+    p = _p(native, 12, data_term, storage=storage)
+
+    def run():
+        ctx.profile(True)
+        ctx.profile_reset()
+        out = ctx.tgv2_deshade_masked(f, p, act) if masked else ctx.tgv2_deshade(f, p)
+        prof = ctx.profile_read()
+        ctx.profile(False)
+        return out, prof["fused_sweep"][1]
+
+    monkeypatch.setenv("AMB_FUSED_TX", "32")
+    a, n_fused = run()
+    assert n_fused == 11
+    monkeypatch.setenv("AMB_FUSED", "0")
+    b, n_two = run()
+    assert n_two == 0
+    for x, y in zip(a, b):
+        np.testing.assert_array_equal(x, y)
 
 
 def test_tgv2_index64_masked_and_planar(ctx, monkeypatch):
