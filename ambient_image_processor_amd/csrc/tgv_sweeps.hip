@@ -1510,7 +1510,7 @@ bool fused_plan(const Geometry& g, int storage, int device, FusedPlan* plan) {
         return false;
     auto env_int = [](const char* name) {
         const char* e = std::getenv(name);
-        return e ? std::atoi(e) : 0;
+        return e ? std::max(0, std::atoi(e)) : 0;   // 0 (or negative): not set
     };
     const int want_vx = env_int("AMB_FUSED_VX"), want_ty = env_int("AMB_FUSED_TY");
     const int want_zc = env_int("AMB_FUSED_ZCHUNK"), want_tx = env_int("AMB_FUSED_TX");

@@ -1058,7 +1058,7 @@ bool tgvk_fused_plan(const Geometry& g, int levels, int device, TgvkFusedPlan* p
         return false;
     auto env_int = [](const char* name) {
         const char* e = std::getenv(name);
-        return e ? std::atoi(e) : 0;
+        return e ? std::max(0, std::atoi(e)) : 0;   // 0 (or negative): not set
     };
     const int want_tx = env_int("AMB_TGVK_FUSED_TX"), want_ty = env_int("AMB_TGVK_FUSED_TY");
     const int want_zc = env_int("AMB_TGVK_FUSED_ZCHUNK");
