@@ -254,11 +254,13 @@ def test_fused_x_tiles_bf16_masked_planar(ctx, orc, monkeypatch, tile):
     np.testing.assert_array_equal(tiled[2][0], st.u)
 
 
-def test_fused_wide_rows_run_tiled(ctx, orc):
-    """Rows of 1400 voxels (too wide for whole-row bands) take x tiles by
-    themselves."""
+@pytest.mark.parametrize("shape", [(3, 6, 1400), (4, 19, 760)])
+def test_fused_wide_rows_run_tiled(ctx, orc, shape):
+    """Rows too wide for whole-row bands take x tiles of 124 columns and bands
+    of up to 8 rows by themselves (760: six full tiles and one of 16 columns,
+    bands of 8, 8 and 3 rows)."""
     from ambient_image_processor_amd import native
-    f = _f((3, 6, 1400), 54)
+    f = _f(shape, 54)
     p = native.default_params(lam=0.7, alpha0=2.0, alpha1=1.0, tau=TAU, sigma=TAU, theta=1.0,
                               iterations=10, data_term=native.DATA_L1)
     ctx.profile(True)
@@ -270,4 +272,5 @@ def test_fused_wide_rows_run_tiled(ctx, orc):
     st = orc.tgv2(f, 10, lam=0.7, alpha0=2.0, alpha1=1.0, tau=TAU, sigma=TAU, data_term="l1",
                   dtype=np.float32)
     np.testing.assert_array_equal(u, st.u)
-    np.testing.assert_array_equal(vz, st.v[2])
+    for k, v in enumerate((vx, vy, vz)):
+        np.testing.assert_array_equal(v, st.v[k])
