@@ -80,6 +80,15 @@ def cospi(t: float) -> float:
     return lib().orc_cospi(float(t))
 
 
+def bf16_rne(a):
+    """The oracle's bfloat16 store rounding (nearest even) of float32 values,
+    returned as float32."""
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    out = np.empty_like(a)
+    lib().orc_bf16_rne(_p(a), _p(out), ctypes.c_longlong(a.size))
+    return out
+
+
 def forward_difference(a, axis, dtype=np.float64):
     a = _c(a, dtype)
     out = np.empty_like(a)
@@ -97,12 +106,18 @@ def backward_difference(a, axis, dtype=np.float64):
 class TGVState:
     """The 17 iteration fields as one contiguous (17, D, H, W) array."""
 
-    def __init__(self, f, dtype=np.float64):
+    def __init__(self, f, dtype=np.float64, bf16=False):
+        """bf16: BF16 field storage - FP32 arithmetic, every field store rounded
+        to bfloat16 (nearest even) where the reference's kernels store it."""
         self.dtype = np.dtype(dtype)
+        if bf16 and self.dtype != np.float32:
+            raise TypeError("BF16 storage runs FP32 arithmetic (dtype float32)")
+        self.bf16 = bool(bf16)
         self.f = _c(f, dtype)
         self.fields = np.zeros((N_FIELDS,) + self.f.shape, dtype=dtype)
         self._ptrs()
-        getattr(lib(), "orc_tgv2_init_" + _suf(dtype))(self._pp, _p(self.f), *_dims(self.f))
+        suf = "bf16" if self.bf16 else _suf(dtype)
+        getattr(lib(), "orc_tgv2_init_" + suf)(self._pp, _p(self.f), *_dims(self.f))
 
     def _ptrs(self):
         n = self.fields[0].size * self.dtype.itemsize
@@ -131,6 +146,7 @@ class TGVState:
     def copy(self):
         c = TGVState.__new__(TGVState)
         c.dtype = self.dtype
+        c.bf16 = self.bf16
         c.f = self.f.copy()
         c.fields = self.fields.copy()
         c._ptrs()
@@ -151,7 +167,8 @@ class TGVState:
         act = None
         if mask is not None:
             act = np.ascontiguousarray(np.asarray(mask) >= 1, dtype=np.uint8)
-        getattr(lib(), "orc_tgv2_iterate_" + _suf(self.dtype))(
+        suf = "bf16" if self.bf16 else _suf(self.dtype)
+        getattr(lib(), "orc_tgv2_iterate_" + suf)(
             self._pp, _p(self.f), *_dims(self.f), prm, dt, int(iterations),
             _p(act) if act is not None else None, pl)
         return self
@@ -168,11 +185,11 @@ class TGVState:
 
 
 def tgv2(f, iterations, lam, alpha0, alpha1, tau, sigma, theta=1.0, data_term="l2",
-         dtype=np.float64, mask=None, planar=False):
+         dtype=np.float64, mask=None, planar=False, bf16=False):
     """Run the reference loop from the reference initialisation; returns the state.
     With a mask: the masked loop (updates only where mask >= 1).  planar: the
-    2-D loop at depth 1."""
-    st = TGVState(f, dtype)
+    2-D loop at depth 1.  bf16: BF16 field storage (dtype float32)."""
+    st = TGVState(f, dtype, bf16=bf16)
     st.iterate(iterations, lam, alpha0, alpha1, tau, sigma, theta, data_term, mask, planar)
     return st
 

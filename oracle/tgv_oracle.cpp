@@ -55,6 +55,22 @@ inline i64 coord(const Dims& s, i64 i, int axis) {
     return i / (s.w * s.h);
 }
 
+// BF16 field storage (the GPU's amb_storage BF16): a field store keeps the
+// value rounded to bfloat16, nearest with ties to even, the integer form
+// (bits + 0x7fff + bit 16) >> 16 (NaN passed through).  Arithmetic stays FP32.
+inline float bf16_rne(float x) {
+    uint32_t b;
+    std::memcpy(&b, &x, 4);
+    if ((b & 0x7fffffffu) > 0x7f800000u) return x;
+    b = (b + 0x7fffu + ((b >> 16) & 1u)) & 0xffff0000u;
+    std::memcpy(&x, &b, 4);
+    return x;
+}
+// The value a field holds after a store: FP32 / FP64 modes keep it, the BF16
+// mode (float only) rounds it.
+inline float stored(float x, int bf16) { return bf16 ? bf16_rne(x) : x; }
+inline double stored(double x, int) { return x; }
+
 inline float fma_t(float a, float b, float c) { return std::fmaf(a, b, c); }
 inline double fma_t(double a, double b, double c) { return std::fma(a, b, c); }
 inline float sqrt_t(float a) { return std::sqrt(a); }
@@ -101,6 +117,7 @@ struct Params {
     double lambda, alpha0, alpha1, tau, sigma, theta;
     int data_term;  // 0 = L1, 1 = L2
     int planar;     // depth 1: the 2-D loop (no z backward differences in eps_B(v_bar))
+    int bf16 = 0;   // float only: fields stored as bfloat16 (every field store rounded)
 };
 
 template <typename T>
@@ -113,11 +130,12 @@ void fill_state_pointers(State<T>& st, T** fields) {
 // A3: initialisation, reference tgv_cuda/tgv_common.cu:36-63,
 // tgv_cuda/tgv2_common.cu:357-371, launched tgv_deshade/tgv2_l1_deshade.cu:153-167
 //   u = u_bar = f ; p = v = v_bar = q = 0.
+// bf16: u = u_bar = f rounded to bfloat16 (f itself stays FP32).
 template <typename T>
-void tgv2_init(State<T>& st, const T* f, const Dims& s) {
+void tgv2_init(State<T>& st, const T* f, const Dims& s, int bf16 = 0) {
 #pragma omp parallel for schedule(static)
     for (i64 i = 0; i < s.n(); ++i) {
-        st.u[i] = st.ubar[i] = f[i];
+        st.u[i] = st.ubar[i] = stored(f[i], bf16);
         for (int c = 0; c < 3; ++c) st.v[c][i] = st.vbar[c][i] = st.p[c][i] = T(0);
         for (int c = 0; c < 6; ++c) st.q[c][i] = T(0);
     }
@@ -133,6 +151,10 @@ void tgv2_init(State<T>& st, const T* f, const Dims& s) {
 // initial values and are read as neighbours.
 // wbar0 (may be null): TGV-k's q-update q + sigma*(eps_B(v_bar) - w_bar_0)
 // (tgv3/tgv3_common.cu:28-86, launched tgvk_deshade/tgvk_l1_deshade.cu:200-214).
+// prm.bf16: the stores of the reference's update kernels - p (tgv2_common.cu:124-128),
+// q and v_prev (:262-275), v and v_bar (:338-347), u and u_bar (tgv2_l1.cu:39-50,
+// tgv2_l2.cu:44-46) - keep the bfloat16 rounding of the value; u_bar and v_bar are
+// formed from the unrounded new u and v, and u_prev / v_prev are the stored u, v.
 template <typename T>
 void tgv2_iteration(State<T>& st, const T* f, const Dims& s, const Params& prm,
                     std::vector<T>* tmp /* 9 arrays of n */, const unsigned char* active,
@@ -140,6 +162,7 @@ void tgv2_iteration(State<T>& st, const T* f, const Dims& s, const Params& prm,
     const i64 n = s.n();
     const T tau = T(prm.tau), sigma = T(prm.sigma), theta = T(prm.theta);
     const T lambda = T(prm.lambda), alpha0 = T(prm.alpha0), alpha1 = T(prm.alpha1);
+    const int rb = prm.bf16;
     T* gx = tmp[0].data(); T* gy = tmp[1].data(); T* gz = tmp[2].data();
     T* uprev = tmp[3].data();
     T* vprev[3] = {tmp[4].data(), tmp[5].data(), tmp[6].data()};
@@ -162,7 +185,8 @@ void tgv2_iteration(State<T>& st, const T* f, const Dims& s, const Params& prm,
         n2 = fma_t(pz, pz, n2);
         const T norm = sqrt_t(n2);
         const T nrm = std::max(T(1), norm / alpha1);
-        st.p[0][i] = px / nrm; st.p[1][i] = py / nrm; st.p[2][i] = pz / nrm;
+        st.p[0][i] = stored(px / nrm, rb); st.p[1][i] = stored(py / nrm, rb);
+        st.p[2][i] = stored(pz / nrm, rb);
         uprev[i] = st.u[i];
     }
 
@@ -187,8 +211,8 @@ void tgv2_iteration(State<T>& st, const T* f, const Dims& s, const Params& prm,
         } else {
             u = fma_t(tl, f[i], u) / (T(1) + tl);
         }
-        st.u[i] = u;
-        st.ubar[i] = fma_t(theta, u - uprev[i], u);
+        st.u[i] = stored(u, rb);
+        st.ubar[i] = stored(fma_t(theta, u - uprev[i], u), rb);
     }
 
     // A9 tgv_launch_gradient2, tgv_cuda/tgv2_common.cu:174-219 (+ addAndHalf :31-40)
@@ -241,8 +265,9 @@ void tgv2_iteration(State<T>& st, const T* f, const Dims& s, const Params& prm,
         n2 = fma_t(T(2) * qxz, qxz, n2);
         n2 = fma_t(T(2) * qyz, qyz, n2);
         const T nrm = std::max(T(1), sqrt_t(n2) / alpha0);
-        st.q[0][i] = qxx / nrm; st.q[1][i] = qyy / nrm; st.q[3][i] = qxy / nrm;
-        st.q[2][i] = qzz / nrm; st.q[4][i] = qxz / nrm; st.q[5][i] = qyz / nrm;
+        st.q[0][i] = stored(qxx / nrm, rb); st.q[1][i] = stored(qyy / nrm, rb);
+        st.q[3][i] = stored(qxy / nrm, rb); st.q[2][i] = stored(qzz / nrm, rb);
+        st.q[4][i] = stored(qxz / nrm, rb); st.q[5][i] = stored(qyz / nrm, rb);
         vprev[2][i] = st.v[2][i]; vprev[0][i] = st.v[0][i]; vprev[1][i] = st.v[1][i];
     }
 
@@ -279,8 +304,8 @@ void tgv2_iteration(State<T>& st, const T* f, const Dims& s, const Params& prm,
         if (active && !active[i]) continue;
         for (int c = 0; c < 3; ++c) {
             const T vn = fma_t(-tau, w[c][i] - st.p[c][i], st.v[c][i]);
-            st.v[c][i] = vn;
-            st.vbar[c][i] = fma_t(theta, vn - vprev[c][i], vn);
+            st.v[c][i] = stored(vn, rb);
+            st.vbar[c][i] = stored(fma_t(theta, vn - vprev[c][i], vn), rb);
         }
     }
 }
@@ -791,6 +816,22 @@ ORC_EXPORT void orc_tgv2_energy_##SUF(const T* f, T** fields, i64 w, i64 h, i64 
 
 ORC_DEFINE_TYPED(float, f32)
 ORC_DEFINE_TYPED(double, f64)
+
+// BF16 field storage (FP32 arithmetic, every field store rounded to bfloat16).
+ORC_EXPORT void orc_bf16_rne(const float* in, float* out, i64 n) {
+    for (i64 i = 0; i < n; ++i) out[i] = bf16_rne(in[i]); }
+ORC_EXPORT void orc_tgv2_init_bf16(float** fields, const float* f, i64 w, i64 h, i64 d) {
+    State<float> st; fill_state_pointers(st, fields); tgv2_init(st, f, Dims{w, h, d}, 1); }
+ORC_EXPORT int orc_tgv2_iterate_bf16(float** fields, const float* f, i64 w, i64 h, i64 d,
+        const double* prm6, int data_term, int iterations, const unsigned char* active,
+        int planar) {
+    State<float> st; fill_state_pointers(st, fields);
+    Params prm{prm6[0], prm6[1], prm6[2], prm6[3], prm6[4], prm6[5], data_term, planar, 1};
+    const Dims s{w, h, d};
+    std::vector<float> tmp[9];
+    for (auto& t : tmp) t.assign(size_t(s.n()), 0.0f);
+    for (int it = 0; it < iterations; ++it) tgv2_iteration(st, f, s, prm, tmp, active);
+    return iterations; }
 
 // TGV-k: fields = the 17 TGV2 fields, then per level w[6], w_bar[6], r[6].
 // prm4 = (lambda, tau, sigma, theta); alpha[order].
