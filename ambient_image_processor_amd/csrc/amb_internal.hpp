@@ -67,6 +67,16 @@ struct StoreBF16 {
     static __device__ __forceinline__ float rt(float v) { return __uint_as_float(uint32_t(rne(v)) << 16); }
 };
 
+// Block id -> tile id such that each of the 8 XCDs (workgroups are dealt to
+// them round-robin: block b runs on XCD b % 8) receives one contiguous range
+// of tiles, so tiles that share an input panel share that XCD's L2.  A
+// bijection on [0, nblocks) for any nblocks: XCD x holds the
+// nblocks / 8 + (x < nblocks % 8) tiles that follow those of XCDs 0 .. x - 1.
+__device__ __forceinline__ unsigned xcd_contiguous(unsigned bid, unsigned nblocks) {
+    const unsigned x = bid & 7u, k = bid >> 3, q = nblocks >> 3, r = nblocks & 7u;
+    return x * q + (x < r ? x : r) + k;
+}
+
 // Field indices inside the 17-field state block (C ABI order).
 enum Field {
     F_U = 0, F_UBAR = 1, F_VX = 2, F_VY = 3, F_VZ = 4, F_VBX = 5, F_VBY = 6, F_VBZ = 7,
@@ -78,6 +88,31 @@ struct Geometry {
     int w, h, d;
     __host__ __device__ size_t n() const { return size_t(w) * size_t(h) * size_t(d); }
 };
+
+// Launch-plan overrides for tests and tuning (amb_context_set_plan_override):
+// held by the context, resolved into its plans when an iteration begins; the
+// product defaults when unset (0, or -1 for the switches).  Results do not
+// depend on them: every plan computes the same values.
+struct PlanOverrides {
+    int tile = -1;               // sweep tile shape id 0..7 (tile_shape)
+    int zchunk = 0;              // planes per block of the two sweeps
+    int sweep_pf = -1;           // next-plane prefetch of the vector sweeps (0/1)
+    int fused = -1;              // 0: no fused TGV2 sweep
+    int fused_vx = 0, fused_ty = 0, fused_zchunk = 0, fused_tx = 0;
+    int fused_pf = -1;           // 0/1
+    int fused_image = -1;        // 0: no row-marched 2-D sweep
+    int fused_image_tx = -1, fused_image_zchunk = 0;
+    int index64 = 0;             // 1: 64-bit-offset kernel forms on any volume
+    int tgvk_zchunk = 0;
+    int tgvk_fused = -1;         // 0: no fused TGV-k sweep
+    int tgvk_fused_tx = 0, tgvk_fused_ty = 0, tgvk_fused_zchunk = 0;
+};
+
+// 64-bit offsets: volumes of 2^30 voxels and more (a 4-byte field's byte
+// offsets no longer fit 32 bits), or forced by the overrides.
+inline bool needs_index64(const Geometry& g, const PlanOverrides& ov) {
+    return g.n() * 4 >= (size_t(1) << 32) || ov.index64 > 0;
+}
 
 struct SweepParams {
     float lambda, alpha0, alpha1, tau, sigma, theta;
@@ -92,6 +127,18 @@ struct SweepParams {
     // planar, and every copy of v_z, p_z, q_zz, q_xz, q_yz holds +0: the fused
     // sweep may take them as +0 without loading or storing them
     int planar_zero_z;
+    // fused sweep only: non-null = the launch also reduces the FP64 energy and
+    // gap of the finished iteration into one row of 10 doubles per block here
+    // (the plan from fused_energy_plan)
+    double* energy_partials;
+    // resolved when the iteration begins: 64-bit-offset kernel forms, and the
+    // vector sweeps' next-plane prefetch (-1: the storage's default)
+    int index64;
+    int sweep_pf;
+    // masked fused sweep only: one byte per tile of its plan, 0 = no voxel the
+    // tile owns is active, so the tile skips (valid once both copies of the
+    // state agree outside the active set); nullptr = no skipping
+    const uint8_t* tile_active;
 };
 
 // Column-tile shapes of the sweep kernels: TX x TY threads, each thread owning
@@ -122,9 +169,6 @@ hipError_t launch_zero_fill(hipStream_t s, void* p, size_t bytes);
 // `bytes`, once per (device, kernel, size): setting it before every launch
 // costs a gap between back-to-back launches.  Thread-safe.
 hipError_t ensure_dynamic_lds(const void* kernel, size_t bytes);
-// AMB_INDEX64=1: run the kernels' 64-bit-offset forms also on volumes whose
-// offsets fit 32 bits (tests of those paths).
-bool force_index64();
 // Fused sweep (primal half of iteration k + dual half of iteration k + 1):
 // reads u, v, p, q through the 17-field table `src` and writes them to the 13
 // fields `dst13` (order u, vx, vy, vz, px, py, pz, qxx, qyy, qzz, qxy, qxz,
@@ -136,8 +180,18 @@ struct FusedPlan {
     int vx;       // voxels per thread along x (2 or 4)
     int tx;       // 0: whole rows; else owned x columns per block (x tiles, vx 2)
 };
-bool fused_plan(const Geometry& g, int storage, int device, FusedPlan* plan);
+bool fused_plan(const Geometry& g, int storage, int device, const PlanOverrides& ov,
+                FusedPlan* plan);
 size_t fused_lds_bytes(const Geometry& g, const FusedPlan& plan);
+// Plan of the fused sweep's energy form (SweepParams::energy_partials) derived
+// from the iteration plan; false when the form does not cover the geometry.
+bool fused_energy_plan(const Geometry& g, int device, const PlanOverrides& ov,
+                       const FusedPlan& base, FusedPlan* plan);
+// blocks of a fused launch with this plan (= rows of energy partials)
+size_t fused_grid_blocks(const Geometry& g, const FusedPlan& plan);
+// out[t] = 1 if tile t of the fused plan owns an active voxel (fused_grid_blocks bytes)
+hipError_t launch_tile_activity(hipStream_t s, const uint8_t* act, Geometry g,
+                                const FusedPlan& plan, uint8_t* out);
 hipError_t launch_fused_sweep(hipStream_t s, int storage, void* const* src, void* const* dst13,
                               const float* f, Geometry g, const SweepParams& p,
                               const FusedPlan& plan);
@@ -146,7 +200,7 @@ extern const int FUSED_FIELD_MAP[N_FUSED_FIELDS];
 // The 2-D loop (planar, FP32 fields, z components +0 in both copies) as a
 // fused sweep marched along the image's rows; same src / dst13 convention as
 // launch_fused_sweep.
-bool fused_image_plan(const Geometry& g, int device, FusedPlan* plan);
+bool fused_image_plan(const Geometry& g, int device, const PlanOverrides& ov, FusedPlan* plan);
 hipError_t launch_fused_image(hipStream_t s, void* const* src, void* const* dst13, const float* f,
                               Geometry g, const SweepParams& p, const FusedPlan& plan);
 
@@ -181,12 +235,13 @@ hipError_t launch_active_from_mask(hipStream_t s, const float* mask, uint8_t* ac
 // fields; levels_dev: device table of 18 pointers per level (w[6], w_bar[6],
 // r[6]); alpha_dev: the k weights; active (may be null): one byte per voxel,
 // updates only where nonzero.  FP32 storage.
+// ov: the z chunk and 64-bit-offset overrides (resolved when the iteration begins).
 hipError_t launch_tgvk_dual(hipStream_t s, float* const* fields_dev, float* const* levels_dev,
                             const float* alpha_dev, const uint8_t* active, Geometry g, int levels,
-                            float sigma, int planar);
+                            float sigma, int planar, const PlanOverrides& ov);
 hipError_t launch_tgvk_primal(hipStream_t s, float* const* fields_dev, float* const* levels_dev,
                               const float* f, const uint8_t* active, Geometry g, int levels,
-                              float tau, float lambda, float theta);
+                              float tau, float lambda, float theta, const PlanOverrides& ov);
 // Fused TGV-k sweep for one level (k = 3): primal half of iteration k and
 // dual half of iteration k + 1, reading the 25 fields `src` (order u, vx, vy,
 // vz, px, py, pz, qxx, qyy, qzz, qxy, qxz, qyz, wx, wy, wz, wxy, wxz, wyz, rx,
@@ -200,7 +255,8 @@ struct TgvkFusedPlan {
     int zchunk;   // planes per block
 };
 constexpr int N_TGVK_FUSED_FIELDS = 25;
-bool tgvk_fused_plan(const Geometry& g, int levels, int device, TgvkFusedPlan* plan);
+bool tgvk_fused_plan(const Geometry& g, int levels, int device, const PlanOverrides& ov,
+                     TgvkFusedPlan* plan);
 hipError_t launch_tgvk_fused(hipStream_t s, const float* const* src, float* const* dst,
                              const float* f, const uint8_t* active, Geometry g, float tau,
                              float sigma, float lambda, float theta, const float* alpha,
@@ -216,11 +272,16 @@ hipError_t launch_downsampled_tail(hipStream_t s, const float* f, const float* m
                                    int set_negative_zero);
 hipError_t launch_multiply(hipStream_t s, const float* a, const float* b, float* out, size_t n);
 
-// FP64 energy / gap (energy.hip).  partials: at least energy_partials_size(g) doubles.
+// FP64 energy / gap (energy.hip).  fields: HOST array of the 17 device field
+// pointers (passed to the kernel by value); partials: at least
+// energy_partials_size(g) doubles.
 size_t energy_partials_size(Geometry g);
-hipError_t launch_energy(hipStream_t s, int storage, void* const* fields, const float* f,
+hipError_t launch_energy(hipStream_t s, int storage, const void* const* fields, const float* f,
                          Geometry g, double lambda, double alpha0, double alpha1, int data_term,
                          int planar, double* partials, double* out10);
+// Final fixed-order FP64 sum of `rows` rows of 10 partial sums into out10
+// (slot 9 is the square root of its sum).
+hipError_t launch_energy_finish(hipStream_t s, const double* partials, int rows, double* out10);
 
 // Adjoint of the unrolled L2 iteration (adjoint.hip).  adj, xk, xk1: 17-field
 // tables (adjoint of x_{k+1} updated in place to the adjoint of x_k); acc: device

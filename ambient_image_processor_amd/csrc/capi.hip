@@ -14,6 +14,8 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <exception>
+#include <new>
 #include <algorithm>
 #include <cstdlib>
 #include <string>
@@ -52,10 +54,10 @@ amb_status fail_hip(hipError_t e, const char* what, int line) {
     } while (0)
 
 constexpr int N_SCRATCH = 5;
-constexpr int N_PROF = 14;
+constexpr int N_PROF = 15;
 enum { P_DUAL = 0, P_PRIMAL = 1, P_DCT = 2, P_POISSON = 3, P_DIV = 4, P_EPI = 5, P_INIT = 6,
        P_ENERGY = 7, P_ADJOINT = 8, P_FUSED = 9, P_TGVK_DUAL = 10, P_TGVK_PRIMAL = 11,
-       P_RESIZE = 12, P_TGVK_FUSED = 13 };
+       P_RESIZE = 12, P_TGVK_FUSED = 13, P_FUSED_ENERGY = 14 };
 
 struct AxisTables {
     int n = 0;
@@ -92,6 +94,10 @@ struct amb_context {
     bool have_state = false;
     bool z_zero = false;   // v_z, p_z, q_zz, q_xz, q_yz are +0 (from begin(), not set_state)
     SweepParams sp{};
+    // launch-plan overrides (tests, tuning); read when an iteration begins.
+    // kov: the ones the TGV-k loop of the current state began with
+    PlanOverrides ov{};
+    PlanOverrides kov{};
     amb_tgv_params params{};
     // cosine tables per axis, keyed by extent
     AxisTables axes[3];
@@ -107,9 +113,12 @@ struct amb_context {
     // A*, W*, dL/df, partials, scalar accumulators
     char* adj_block = nullptr;
     size_t adj_cap = 0;
-    // masked iteration: one byte per voxel, nonzero = update (grow-only)
+    // masked iteration: one byte per voxel, nonzero = update (grow-only), and
+    // one byte per tile of the fused plan, nonzero = the tile owns an active voxel
     uint8_t* act = nullptr;
     size_t act_cap = 0;
+    uint8_t* tact = nullptr;
+    size_t tact_cap = 0;
     // fused sweep: second copy of u, v, p, q (grow-only) and the launch plan
     char* alt_block = nullptr;
     size_t alt_cap = 0;
@@ -118,6 +127,16 @@ struct amb_context {
     FusedPlan fplan{0, 0};
     bool iplan_ok = false;   // the 2-D loop marched along the image's rows
     FusedPlan iplan{0, 0};
+    // energy / gap trace: every energy_interval iterations of a call (0 = off)
+    // and after its last one; in the fused sweep's energy form where that
+    // iteration's primal half runs fused, else the standalone pass
+    uint32_t energy_interval = 0;
+    bool fen_ok = false;
+    FusedPlan fen_plan{0, 0};
+    double* etrace_dev = nullptr;     // rows of 10 doubles
+    size_t etrace_cap = 0;            // rows
+    std::vector<double> etrace;       // rows of 11: iteration count, 10 values
+    std::vector<uint32_t> etrace_it;
     // TGV-k: 18 fields per level (w[6], w_bar[6], r[6]; grow-only), their
     // device pointer table and the k weights
     char* kblock = nullptr;
@@ -253,7 +272,11 @@ amb_status ensure_buffers(amb_context* c, Geometry g, int storage) {
     if (!c->fields_dev) AMB_HIP(hipMalloc(reinterpret_cast<void**>(&c->fields_dev), sizeof(void*) * N_FIELDS));
     AMB_HIP(hipMemcpyAsync(c->fields_dev, c->fields, sizeof(void*) * N_FIELDS,
                            hipMemcpyHostToDevice, c->stream));
-    const size_t pn = energy_partials_size(g);
+    size_t pn = energy_partials_size(g);
+    FusedPlan base{0, 0}, en{0, 0};
+    if (fused_plan(g, storage, c->device, c->ov, &base) &&
+        fused_energy_plan(g, c->device, c->ov, base, &en))
+        pn = std::max(pn, fused_grid_blocks(g, en) * 10 + 16);
     if (pn > c->partials_cap) {
         if (c->partials) AMB_HIP(hipFree(c->partials));
         c->partials = nullptr;
@@ -286,56 +309,86 @@ float host_bf16_to_float(uint16_t h) {
     return v;
 }
 
-// Cosine tables for one axis of extent n (built once per extent, host FP64).
+// Largest extent of the DCT passes: the reference-numerics passes read an
+// n x n FP64 table per direction (16 n^2 bytes of device memory per axis,
+// 4 GiB at n = 16384); the reference evaluates its cosines on the fly and has
+// no such limit.
+constexpr int DCT_MAX_EXTENT = 16384;
+
+// Cosine tables for one axis of extent n (built once per extent, host FP64,
+// uploaded in row chunks so that host memory stays O(n)): the FP64 [m][k]
+// tables of the reference-numerics passes always, the FP32 and bf16-split
+// [k][m] tables of the MFMA passes only for n <= dct_mfma_max_extent().
 amb_status ensure_axis(amb_context* c, int axis, int n) {
     AxisTables& t = c->axes[axis];
     if (t.n == n) return AMB_OK;
+    if (n > DCT_MAX_EXTENT)
+        return fail(AMB_ERR_NOT_SUPPORTED,
+                    "DCT Poisson solve: extent " + std::to_string(n) +
+                        " exceeds 16384 (the cosine tables would exceed 4 GiB per axis)");
     free_axis(t);
     const size_t nn = size_t(n) * n;
-    std::vector<double> f64(nn), i64v(nn);
-    std::vector<float> f32(nn), i32(nn);
-    std::vector<uint16_t> fh(nn), fl(nn), ih(nn), il(nn);
+    const bool mfma = n <= dct_mfma_max_extent();
     const double size = double(float(n));
-    for (int m = 0; m < n; ++m)
-        for (int k = 0; k < n; ++k) {
-            f64[size_t(m) * n + k] = host_cospi((double(m) + 0.5) * double(k) / size);
-            i64v[size_t(m) * n + k] = host_cospi((double(k) + 0.5) * double(m) / size);
-        }
-    // MFMA tables, FP32 and bf16 splits: [k][m] (contiguous over the summed index)
-    for (int k = 0; k < n; ++k)
-        for (int m = 0; m < n; ++m) {
-            const double cf = f64[size_t(m) * n + k];
-            // inverse: out[k] = in[0] + 2 sum_{m>=1} in[m] cos  -> fold the factor 2 and the
-            // m = 0 term into the matrix (weights 1 and 2)
-            const double ci = (m == 0 ? 1.0 : 2.0) * i64v[size_t(m) * n + k];
-            f32[size_t(k) * n + m] = float(cf);
-            i32[size_t(k) * n + m] = float(ci);
-            const uint16_t h1 = host_bf16_rne(float(cf));
-            fh[size_t(k) * n + m] = h1;
-            fl[size_t(k) * n + m] = host_bf16_rne(float(cf - double(host_bf16_to_float(h1))));
-            const uint16_t h2 = host_bf16_rne(float(ci));
-            ih[size_t(k) * n + m] = h2;
-            il[size_t(k) * n + m] = host_bf16_rne(float(ci - double(host_bf16_to_float(h2))));
-        }
-    std::vector<float> eig(size_t(n) + 1);
-    for (int k = 0; k <= n; ++k) eig[k] = float(host_cospi(double(float(k) / float(n))));
     AMB_HIP(hipMalloc(reinterpret_cast<void**>(&t.fwd64), nn * 8));
     AMB_HIP(hipMalloc(reinterpret_cast<void**>(&t.inv64), nn * 8));
-    AMB_HIP(hipMalloc(reinterpret_cast<void**>(&t.fwd32), nn * 4));
-    AMB_HIP(hipMalloc(reinterpret_cast<void**>(&t.inv32), nn * 4));
-    AMB_HIP(hipMalloc(reinterpret_cast<void**>(&t.fwd_hi), nn * 2));
-    AMB_HIP(hipMalloc(reinterpret_cast<void**>(&t.fwd_lo), nn * 2));
-    AMB_HIP(hipMalloc(reinterpret_cast<void**>(&t.inv_hi), nn * 2));
-    AMB_HIP(hipMalloc(reinterpret_cast<void**>(&t.inv_lo), nn * 2));
+    if (mfma) {
+        AMB_HIP(hipMalloc(reinterpret_cast<void**>(&t.fwd32), nn * 4));
+        AMB_HIP(hipMalloc(reinterpret_cast<void**>(&t.inv32), nn * 4));
+        AMB_HIP(hipMalloc(reinterpret_cast<void**>(&t.fwd_hi), nn * 2));
+        AMB_HIP(hipMalloc(reinterpret_cast<void**>(&t.fwd_lo), nn * 2));
+        AMB_HIP(hipMalloc(reinterpret_cast<void**>(&t.inv_hi), nn * 2));
+        AMB_HIP(hipMalloc(reinterpret_cast<void**>(&t.inv_lo), nn * 2));
+    }
     AMB_HIP(hipMalloc(reinterpret_cast<void**>(&t.eig), (size_t(n) + 1) * 4));
-    AMB_HIP(hipMemcpy(t.fwd64, f64.data(), nn * 8, hipMemcpyHostToDevice));
-    AMB_HIP(hipMemcpy(t.inv64, i64v.data(), nn * 8, hipMemcpyHostToDevice));
-    AMB_HIP(hipMemcpy(t.fwd32, f32.data(), nn * 4, hipMemcpyHostToDevice));
-    AMB_HIP(hipMemcpy(t.inv32, i32.data(), nn * 4, hipMemcpyHostToDevice));
-    AMB_HIP(hipMemcpy(t.fwd_hi, fh.data(), nn * 2, hipMemcpyHostToDevice));
-    AMB_HIP(hipMemcpy(t.fwd_lo, fl.data(), nn * 2, hipMemcpyHostToDevice));
-    AMB_HIP(hipMemcpy(t.inv_hi, ih.data(), nn * 2, hipMemcpyHostToDevice));
-    AMB_HIP(hipMemcpy(t.inv_lo, il.data(), nn * 2, hipMemcpyHostToDevice));
+    const size_t rows = std::max<size_t>(1, (size_t(1) << 21) / size_t(n));   // ~2M entries
+    std::vector<double> a64, b64;
+    std::vector<float> a32, b32;
+    std::vector<uint16_t> ah, al, bh, bl;
+    for (size_t r0 = 0; r0 < size_t(n); r0 += rows) {
+        const size_t nr = std::min(rows, size_t(n) - r0), cnt = nr * size_t(n);
+        // FP64 reference tables, row r = m: [m][k] cospi((m+0.5)k/n), cospi((k+0.5)m/n)
+        a64.resize(cnt);
+        b64.resize(cnt);
+        for (size_t r = 0; r < nr; ++r) {
+            const int m = int(r0 + r);
+            for (int k = 0; k < n; ++k) {
+                a64[r * n + k] = host_cospi((double(m) + 0.5) * double(k) / size);
+                b64[r * n + k] = host_cospi((double(k) + 0.5) * double(m) / size);
+            }
+        }
+        AMB_HIP(hipMemcpy(t.fwd64 + r0 * n, a64.data(), cnt * 8, hipMemcpyHostToDevice));
+        AMB_HIP(hipMemcpy(t.inv64 + r0 * n, b64.data(), cnt * 8, hipMemcpyHostToDevice));
+        if (!mfma) continue;
+        // MFMA tables, row r = k: [k][m] (contiguous over the summed index); the
+        // inverse folds the reference's "X[0] + 2 sum_{m>=1}" into weights 1, 2
+        a32.resize(cnt); b32.resize(cnt);
+        ah.resize(cnt); al.resize(cnt); bh.resize(cnt); bl.resize(cnt);
+        for (size_t r = 0; r < nr; ++r) {
+            const int k = int(r0 + r);
+            for (int m = 0; m < n; ++m) {
+                const double cf = host_cospi((double(m) + 0.5) * double(k) / size);
+                const double ci = (m == 0 ? 1.0 : 2.0) * host_cospi((double(k) + 0.5) * double(m) / size);
+                const size_t o = r * n + size_t(m);
+                a32[o] = float(cf);
+                b32[o] = float(ci);
+                const uint16_t h1 = host_bf16_rne(float(cf));
+                ah[o] = h1;
+                al[o] = host_bf16_rne(float(cf - double(host_bf16_to_float(h1))));
+                const uint16_t h2 = host_bf16_rne(float(ci));
+                bh[o] = h2;
+                bl[o] = host_bf16_rne(float(ci - double(host_bf16_to_float(h2))));
+            }
+        }
+        AMB_HIP(hipMemcpy(t.fwd32 + r0 * n, a32.data(), cnt * 4, hipMemcpyHostToDevice));
+        AMB_HIP(hipMemcpy(t.inv32 + r0 * n, b32.data(), cnt * 4, hipMemcpyHostToDevice));
+        AMB_HIP(hipMemcpy(t.fwd_hi + r0 * n, ah.data(), cnt * 2, hipMemcpyHostToDevice));
+        AMB_HIP(hipMemcpy(t.fwd_lo + r0 * n, al.data(), cnt * 2, hipMemcpyHostToDevice));
+        AMB_HIP(hipMemcpy(t.inv_hi + r0 * n, bh.data(), cnt * 2, hipMemcpyHostToDevice));
+        AMB_HIP(hipMemcpy(t.inv_lo + r0 * n, bl.data(), cnt * 2, hipMemcpyHostToDevice));
+    }
+    std::vector<float> eig(size_t(n) + 1);
+    for (int k = 0; k <= n; ++k) eig[k] = float(host_cospi(double(float(k) / float(n))));
     AMB_HIP(hipMemcpy(t.eig, eig.data(), (size_t(n) + 1) * 4, hipMemcpyHostToDevice));
     t.n = n;
     return AMB_OK;
@@ -355,27 +408,19 @@ hipMemcpyKind kind_out(amb_memory mem) {
     return mem == AMB_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
 }
 
-// Sweep tile shape: 64x4 threads unless AMB_TILE selects another shape id
-// (tile_shape() in amb_internal.hpp), for tuning.
-int choose_tile(const Geometry& g) {
+// Sweep tile shape: 64x4 threads unless the tile override selects another
+// shape id (tile_shape() in amb_internal.hpp), for tuning.
+int choose_tile(const Geometry& g, const PlanOverrides& ov) {
     int t = g.w % 4 == 0 ? TILE_VECTOR_DEFAULT : TILE_SCALAR_DEFAULT;
-    const char* env = std::getenv("AMB_TILE");
-    if (env) {
-        const int v = std::atoi(env);
-        if (v >= 0 && v <= 7) t = v;
-    }
+    if (ov.tile >= 0 && ov.tile <= 7) t = ov.tile;
     if (t >= 4 && g.w % 4 != 0) t = TILE_SCALAR_DEFAULT;   // vector tiles need W % 4 == 0
     return t;
 }
 
 // Planes per block: 4 for FP32 vector tiles and 16 for BF16 vector tiles
 // (measured on MI355X at 256^3), else >= 4096 blocks with chunks of 8..64.
-int choose_zchunk(const Geometry& g, int tile, int storage) {
-    const char* env = std::getenv("AMB_ZCHUNK");
-    if (env) {
-        const int v = std::atoi(env);
-        if (v > 0) return v;
-    }
+int choose_zchunk(const Geometry& g, int tile, int storage, const PlanOverrides& ov) {
+    if (ov.zchunk > 0) return ov.zchunk;
     if (tile >= 4) return std::min(storage == 1 ? 16 : 4, g.d);
     int tx, ty, vx;
     tile_shape(tile, tx, ty, vx);
@@ -448,12 +493,20 @@ amb_status begin(amb_context* c, amb_memory mem, const float* f, uint32_t w, uin
     c->sp.lambda = p->lambda; c->sp.alpha0 = p->alpha0; c->sp.alpha1 = p->alpha1;
     c->sp.tau = p->tau; c->sp.sigma = p->sigma; c->sp.theta = p->theta;
     c->sp.data_term = int(p->data_term);
-    c->sp.tile = choose_tile(g);
-    c->sp.zchunk = choose_zchunk(g, c->sp.tile, int(p->storage));
+    c->sp.tile = choose_tile(g, c->ov);
+    c->sp.zchunk = choose_zchunk(g, c->sp.tile, int(p->storage), c->ov);
+    c->sp.index64 = needs_index64(g, c->ov) ? 1 : 0;
+    c->sp.sweep_pf = c->ov.sweep_pf;
+    c->sp.energy_partials = nullptr;
+    c->sp.tile_active = nullptr;
     c->sp.active = nullptr;
     c->sp.planar = (p->flags & AMB_FLAG_PLANAR) ? 1 : 0;
-    c->fused_ok = fused_plan(g, c->storage, c->device, &c->fplan);
-    c->iplan_ok = c->sp.planar && c->storage == 0 && fused_image_plan(g, c->device, &c->iplan);
+    c->fused_ok = fused_plan(g, c->storage, c->device, c->ov, &c->fplan);
+    c->iplan_ok = c->sp.planar && c->storage == 0 &&
+                  fused_image_plan(g, c->device, c->ov, &c->iplan);
+    c->fen_ok = c->fused_ok && !c->sp.planar &&
+                fused_energy_plan(g, c->device, c->ov, c->fplan, &c->fen_plan) &&
+                fused_grid_blocks(g, c->fen_plan) * 10 + 16 <= c->partials_cap;
     c->have_state = true;
     c->z_zero = true;
     return AMB_OK;
@@ -490,8 +543,18 @@ amb_status ensure_alt(amb_context* c) {
 
 // Primal half of iteration k + dual half of iteration k + 1 into the second
 // copy, then swap the copies (the state table points at the new values).
-amb_status fused_step(amb_context* c) {
-    {
+// energy_out (device, 10 doubles; may be null): the launch also reduces the
+// energy and gap of the state after iteration k (the fused sweep's energy form).
+amb_status fused_step(amb_context* c, double* energy_out = nullptr) {
+    if (energy_out) {
+        ProfScope ps(c, P_FUSED_ENERGY);
+        SweepParams sp = c->sp;
+        sp.energy_partials = c->partials;
+        AMB_HIP(launch_fused_sweep(c->stream, c->storage, c->fields, c->alt, c->f, c->g, sp,
+                                   c->fen_plan));
+        AMB_HIP(launch_energy_finish(c->stream, c->partials,
+                                     int(fused_grid_blocks(c->g, c->fen_plan)), energy_out));
+    } else {
         ProfScope ps(c, P_FUSED);
         if (c->sp.planar_zero_z && c->iplan_ok)
             AMB_HIP(launch_fused_image(c->stream, c->fields, c->alt, c->f, c->g, c->sp, c->iplan));
@@ -500,6 +563,53 @@ amb_status fused_step(amb_context* c) {
                                        c->fplan));
     }
     for (int k = 0; k < N_FUSED_FIELDS; ++k) std::swap(c->fields[FUSED_FIELD_MAP[k]], c->alt[k]);
+    return AMB_OK;
+}
+
+// The standalone FP64 energy pass over the current state into out (device).
+amb_status energy_pass(amb_context* c, double* out) {
+    ProfScope ps(c, P_ENERGY);
+    AMB_HIP(launch_energy(c->stream, c->storage, c->fields, c->f, c->g, c->params.lambda,
+                          c->params.alpha0, c->params.alpha1, int(c->params.data_term),
+                          c->sp.planar, c->partials, out));
+    return AMB_OK;
+}
+
+// Device slot of the next energy trace row (grow-only buffer; rows already
+// written are copied into the larger one).
+amb_status etrace_slot(amb_context* c, double** slot) {
+    const size_t row = c->etrace_it.size();
+    if (row >= c->etrace_cap) {
+        const size_t cap = std::max<size_t>(64, 2 * c->etrace_cap);
+        double* nb = nullptr;
+        AMB_HIP(hipMalloc(reinterpret_cast<void**>(&nb), cap * 10 * sizeof(double)));
+        if (c->etrace_dev) {
+            AMB_HIP(hipMemcpyAsync(nb, c->etrace_dev, row * 10 * sizeof(double),
+                                   hipMemcpyDeviceToDevice, c->stream));
+            AMB_HIP(hipStreamSynchronize(c->stream));
+            AMB_HIP(hipFree(c->etrace_dev));
+        }
+        c->etrace_dev = nb;
+        c->etrace_cap = cap;
+    }
+    *slot = c->etrace_dev + row * 10;
+    return AMB_OK;
+}
+
+// Copies the trace rows of the call to the host (iteration count + 10 values each).
+amb_status etrace_collect(amb_context* c) {
+    const size_t rows = c->etrace_it.size();
+    std::vector<double> vals(rows * 10);
+    if (rows) {
+        AMB_HIP(hipMemcpyAsync(vals.data(), c->etrace_dev, rows * 10 * sizeof(double),
+                               hipMemcpyDeviceToHost, c->stream));
+        AMB_HIP(hipStreamSynchronize(c->stream));
+    }
+    c->etrace.assign(rows * 11, 0.0);
+    for (size_t r = 0; r < rows; ++r) {
+        c->etrace[r * 11] = double(c->etrace_it[r]);
+        for (int k = 0; k < 10; ++k) c->etrace[r * 11 + 1 + k] = vals[r * 10 + k];
+    }
     return AMB_OK;
 }
 
@@ -523,7 +633,7 @@ amb_status run_masked(amb_context* c, uint32_t iterations, uint32_t interval,
                       amb_iteration_callback cb, void* user) {
     struct Reset {
         amb_context* c;
-        ~Reset() { c->sp.active = nullptr; }
+        ~Reset() { c->sp.active = nullptr; c->sp.tile_active = nullptr; }
     } reset{c};
     c->sp.active = c->act;
     return run(c, iterations, interval, cb, user, true);
@@ -561,17 +671,50 @@ amb_status run(amb_context* c, uint32_t iterations, uint32_t interval, amb_itera
         const int zf[5] = {3, 6, 9, 11, 12};   // vz, pz, qzz, qxz, qyz in the fused order
         for (int k : zf) AMB_HIP(launch_zero_fill(c->stream, c->alt[k], c->g.n() * 4));
     }
-    bool need_dual = true;
+    // masked 3-D loop: tiles of the fused plan that own no active voxel skip,
+    // from the second fused launch of the call on (the first one writes every
+    // voxel of the second copy, after which both copies agree outside the
+    // active set: the masked sweeps store only active voxels)
+    uint8_t* tflags = nullptr;
+    c->sp.tile_active = nullptr;
+    if (fused && c->sp.active && !image && !c->sp.planar_zero_z) {
+        const size_t nt = fused_grid_blocks(c->g, c->fplan);
+        if (nt > c->tact_cap) {
+            if (c->tact) AMB_HIP(hipFree(c->tact));
+            c->tact = nullptr;
+            c->tact_cap = 0;
+            AMB_HIP(hipMalloc(reinterpret_cast<void**>(&c->tact), nt));
+            c->tact_cap = nt;
+        }
+        AMB_HIP(launch_tile_activity(c->stream, c->sp.active, c->g, c->fplan, c->tact));
+        tflags = c->tact;
+    }
+    bool need_dual = true, fused_ran = false;
+    const uint32_t every = c->energy_interval;
+    const bool en_fused = fused && !image && c->fen_ok && !c->sp.planar_zero_z;
+    c->etrace_it.clear();
     for (uint32_t it = 0; it < iterations; ++it) {
         const bool observed = cb && interval > 0 && it > 0 && it % interval == 0;
+        // energy after this iteration: every `every` iterations and after the last
+        const bool want_e = every > 0 && ((it + 1) % every == 0 || it + 1 == iterations);
         if (need_dual) AMB_TRY(sweep(c, 0));
-        if (fused && !observed && it + 1 < iterations) {
-            AMB_TRY(fused_step(c));
+        if (fused && !observed && it + 1 < iterations && (!want_e || en_fused)) {
+            double* slot = nullptr;
+            if (want_e) AMB_TRY(etrace_slot(c, &slot));
+            c->sp.tile_active = fused_ran ? tflags : nullptr;
+            AMB_TRY(fused_step(c, slot));
+            fused_ran = true;
             need_dual = false;
         } else {
             AMB_TRY(sweep(c, 1));
             need_dual = true;
+            if (want_e) {
+                double* slot = nullptr;
+                AMB_TRY(etrace_slot(c, &slot));
+                AMB_TRY(energy_pass(c, slot));
+            }
         }
+        if (want_e) c->etrace_it.push_back(it + 1);
         if (observed) {
             const size_t n = c->g.n();
             c->cb_u.resize(n);
@@ -585,9 +728,19 @@ amb_status run(amb_context* c, uint32_t iterations, uint32_t interval, amb_itera
             const int stop = cb(it, iterations, c->cb_u.data(), with_v ? c->cb_v[0].data() : nullptr,
                                 with_v ? c->cb_v[1].data() : nullptr,
                                 with_v ? c->cb_v[2].data() : nullptr, user);
-            if (stop) break;
+            if (stop) {
+                if (every > 0 && !want_e) {   // the energy where the loop stopped
+                    double* slot = nullptr;
+                    AMB_TRY(etrace_slot(c, &slot));
+                    AMB_TRY(energy_pass(c, slot));
+                    c->etrace_it.push_back(it + 1);
+                }
+                break;
+            }
         }
     }
+    c->sp.tile_active = nullptr;
+    if (every > 0) AMB_TRY(etrace_collect(c));
     return AMB_OK;
 }
 
@@ -648,7 +801,8 @@ amb_status tgvk_begin(amb_context* c, amb_memory mem, const float* f, uint32_t w
     AMB_HIP(hipMemcpyAsync(c->fields_dev, c->fields, sizeof(void*) * N_FIELDS,
                            hipMemcpyHostToDevice, c->stream));
     c->klevels = L;
-    c->kfused_ok = tgvk_fused_plan(c->g, L, c->device, &c->kfplan);
+    c->kov = c->ov;
+    c->kfused_ok = tgvk_fused_plan(c->g, L, c->device, c->kov, &c->kfplan);
     return AMB_OK;
 }
 
@@ -773,7 +927,7 @@ amb_status tgvk_run(amb_context* c, uint32_t iterations, uint32_t interval,
         if (need_dual) {
             ProfScope ps(c, P_TGVK_DUAL);
             AMB_HIP(launch_tgvk_dual(c->stream, F, c->klv_dev, c->kalpha_dev, c->sp.active, c->g,
-                                     c->klevels, c->sp.sigma, c->sp.planar));
+                                     c->klevels, c->sp.sigma, c->sp.planar, c->kov));
         }
         if (fused && !observed && it + 1 < iterations) {
             AMB_TRY(tgvk_fused_step(c));
@@ -784,7 +938,8 @@ amb_status tgvk_run(amb_context* c, uint32_t iterations, uint32_t interval,
             stale = false;
             ProfScope ps(c, P_TGVK_PRIMAL);
             AMB_HIP(launch_tgvk_primal(c->stream, F, c->klv_dev, c->f, c->sp.active, c->g,
-                                       c->klevels, c->sp.tau, c->sp.lambda, c->sp.theta));
+                                       c->klevels, c->sp.tau, c->sp.lambda, c->sp.theta,
+                                       c->kov));
             need_dual = true;
         }
         if (observed) {
@@ -919,7 +1074,24 @@ amb_status geometry_only(amb_context* c, uint32_t w, uint32_t h, uint32_t d) {
     return AMB_OK;
 }
 
+// Every amb_status entry point is a function-try-block ending in
+// guard_exception(): no C++ exception (a std::bad_alloc of a host vector, say)
+// crosses the C ABI; it becomes a status code and amb_last_error() text.
+amb_status guard_exception() noexcept {
+    try {
+        throw;
+    } catch (const std::bad_alloc&) {
+        return fail(AMB_ERR_OUT_OF_MEMORY, "host memory allocation failed");
+    } catch (const std::exception& e) {
+        return fail(AMB_ERR_HIP, std::string("internal error: ") + e.what());
+    } catch (...) {
+        return fail(AMB_ERR_HIP, "internal error: unknown C++ exception");
+    }
+}
+
 }  // namespace
+
+
 
 // ===========================================================================
 extern "C" {
@@ -944,7 +1116,7 @@ AMB_API void amb_tgv_params_default(amb_tgv_params* p) {
     p->flags = AMB_FLAG_SET_NEGATIVE_ZERO;
 }
 
-AMB_API amb_status amb_context_create(int device, amb_context** out) {
+AMB_API amb_status amb_context_create(int device, amb_context** out) try {
     if (!out) return fail(AMB_ERR_INVALID_ARGUMENT, "ctx is NULL");
     *out = nullptr;
     int count = 0;
@@ -960,9 +1132,11 @@ AMB_API amb_status amb_context_create(int device, amb_context** out) {
     }
     *out = c;
     return AMB_OK;
+} catch (...) {
+    return guard_exception();
 }
 
-AMB_API amb_status amb_context_destroy(amb_context* c) {
+AMB_API amb_status amb_context_destroy(amb_context* c) try {
     if (!c) return AMB_OK;
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
@@ -978,8 +1152,10 @@ AMB_API amb_status amb_context_destroy(amb_context* c) {
     if (c->fields_dev) (void)hipFree(c->fields_dev);
     if (c->partials) (void)hipFree(c->partials);
     if (c->energy_out) (void)hipFree(c->energy_out);
+    if (c->etrace_dev) (void)hipFree(c->etrace_dev);
     if (c->adj_block) (void)hipFree(c->adj_block);
     if (c->act) (void)hipFree(c->act);
+    if (c->tact) (void)hipFree(c->tact);
     if (c->alt_block) (void)hipFree(c->alt_block);
     if (c->kblock) (void)hipFree(c->kblock);
     if (c->klv_dev) (void)hipFree(c->klv_dev);
@@ -990,24 +1166,30 @@ AMB_API amb_status amb_context_destroy(amb_context* c) {
     (void)hipStreamDestroy(c->stream);
     delete c;
     return AMB_OK;
+} catch (...) {
+    return guard_exception();
 }
 
-AMB_API amb_status amb_device_alloc(amb_context* c, size_t bytes, void** ptr) {
+AMB_API amb_status amb_device_alloc(amb_context* c, size_t bytes, void** ptr) try {
     if (!c || !ptr) return fail(AMB_ERR_INVALID_ARGUMENT, "NULL argument");
     AMB_TRY(set_device(c));
     AMB_HIP(hipMalloc(ptr, bytes));
     return AMB_OK;
+} catch (...) {
+    return guard_exception();
 }
 
-AMB_API amb_status amb_device_free(amb_context* c, void* ptr) {
+AMB_API amb_status amb_device_free(amb_context* c, void* ptr) try {
     if (!c) return fail(AMB_ERR_INVALID_ARGUMENT, "ctx is NULL");
     AMB_TRY(set_device(c));
     AMB_HIP(hipStreamSynchronize(c->stream));
     AMB_HIP(hipFree(ptr));
     return AMB_OK;
+} catch (...) {
+    return guard_exception();
 }
 
-AMB_API amb_status amb_memcpy(amb_context* c, void* dst, const void* src, size_t bytes, int kind) {
+AMB_API amb_status amb_memcpy(amb_context* c, void* dst, const void* src, size_t bytes, int kind) try {
     if (!c || (!dst && bytes) || (!src && bytes)) return fail(AMB_ERR_INVALID_ARGUMENT, "NULL argument");
     AMB_TRY(set_device(c));
     const hipMemcpyKind k = kind == 0 ? hipMemcpyHostToDevice
@@ -1015,26 +1197,72 @@ AMB_API amb_status amb_memcpy(amb_context* c, void* dst, const void* src, size_t
     AMB_HIP(hipMemcpyAsync(dst, src, bytes, k, c->stream));
     AMB_HIP(hipStreamSynchronize(c->stream));
     return AMB_OK;
+} catch (...) {
+    return guard_exception();
 }
 
-AMB_API amb_status amb_synchronize(amb_context* c) {
+AMB_API amb_status amb_synchronize(amb_context* c) try {
     if (!c) return fail(AMB_ERR_INVALID_ARGUMENT, "ctx is NULL");
     AMB_TRY(set_device(c));
     AMB_HIP(hipStreamSynchronize(c->stream));
     return AMB_OK;
+} catch (...) {
+    return guard_exception();
 }
 
 AMB_API void* amb_context_stream(amb_context* c) { return c ? static_cast<void*>(c->stream) : nullptr; }
 
+AMB_API amb_status amb_context_set_plan_override(amb_context* c, const char* name,
+                                                 int32_t value) try {
+    if (!c || !name) return fail(AMB_ERR_INVALID_ARGUMENT, "NULL argument");
+    PlanOverrides& o = c->ov;
+    const PlanOverrides unset{};
+    struct Key {
+        const char* name;
+        int PlanOverrides::*field;
+    };
+    static const Key keys[] = {
+        {"tile", &PlanOverrides::tile}, {"zchunk", &PlanOverrides::zchunk},
+        {"sweep_pf", &PlanOverrides::sweep_pf}, {"fused", &PlanOverrides::fused},
+        {"fused_vx", &PlanOverrides::fused_vx}, {"fused_ty", &PlanOverrides::fused_ty},
+        {"fused_zchunk", &PlanOverrides::fused_zchunk}, {"fused_tx", &PlanOverrides::fused_tx},
+        {"fused_pf", &PlanOverrides::fused_pf}, {"fused_image", &PlanOverrides::fused_image},
+        {"fused_image_tx", &PlanOverrides::fused_image_tx},
+        {"fused_image_zchunk", &PlanOverrides::fused_image_zchunk},
+        {"index64", &PlanOverrides::index64}, {"tgvk_zchunk", &PlanOverrides::tgvk_zchunk},
+        {"tgvk_fused", &PlanOverrides::tgvk_fused},
+        {"tgvk_fused_tx", &PlanOverrides::tgvk_fused_tx},
+        {"tgvk_fused_ty", &PlanOverrides::tgvk_fused_ty},
+        {"tgvk_fused_zchunk", &PlanOverrides::tgvk_fused_zchunk}};
+    for (const Key& k : keys)
+        if (std::strcmp(k.name, name) == 0) {
+            o.*k.field = value < 0 ? unset.*k.field : value;
+            return AMB_OK;
+        }
+    return fail(AMB_ERR_INVALID_ARGUMENT, std::string("unknown plan override: ") + name);
+} catch (...) {
+    return guard_exception();
+}
+
+AMB_API amb_status amb_context_clear_plan_overrides(amb_context* c) try {
+    if (!c) return fail(AMB_ERR_INVALID_ARGUMENT, "ctx is NULL");
+    c->ov = PlanOverrides{};
+    return AMB_OK;
+} catch (...) {
+    return guard_exception();
+}
+
 AMB_API amb_status amb_tgv2_begin(amb_context* c, amb_memory mem, const float* f, uint32_t w,
-                                  uint32_t h, uint32_t d, const amb_tgv_params* p) {
+                                  uint32_t h, uint32_t d, const amb_tgv_params* p) try {
     if (!c) return fail(AMB_ERR_INVALID_ARGUMENT, "ctx is NULL");
     AMB_TRY(begin(c, mem, f, w, h, d, p));
     AMB_HIP(hipStreamSynchronize(c->stream));
     return AMB_OK;
+} catch (...) {
+    return guard_exception();
 }
 
-AMB_API amb_status amb_tgv2_set_state(amb_context* c, amb_memory mem, const float* const* fields) {
+AMB_API amb_status amb_tgv2_set_state(amb_context* c, amb_memory mem, const float* const* fields) try {
     if (!c || !fields) return fail(AMB_ERR_INVALID_ARGUMENT, "NULL argument");
     if (!c->have_state) return fail(AMB_ERR_INVALID_ARGUMENT, "no iteration state: call amb_tgv2_begin");
     AMB_TRY(set_device(c));
@@ -1055,9 +1283,11 @@ AMB_API amb_status amb_tgv2_set_state(amb_context* c, amb_memory mem, const floa
     }
     AMB_HIP(hipStreamSynchronize(c->stream));
     return AMB_OK;
+} catch (...) {
+    return guard_exception();
 }
 
-AMB_API amb_status amb_tgv2_get_state(amb_context* c, amb_memory mem, float* const* fields) {
+AMB_API amb_status amb_tgv2_get_state(amb_context* c, amb_memory mem, float* const* fields) try {
     if (!c || !fields) return fail(AMB_ERR_INVALID_ARGUMENT, "NULL argument");
     if (!c->have_state) return fail(AMB_ERR_INVALID_ARGUMENT, "no iteration state");
     AMB_TRY(set_device(c));
@@ -1067,18 +1297,22 @@ AMB_API amb_status amb_tgv2_get_state(amb_context* c, amb_memory mem, float* con
     }
     AMB_HIP(hipStreamSynchronize(c->stream));
     return AMB_OK;
+} catch (...) {
+    return guard_exception();
 }
 
-AMB_API amb_status amb_tgv2_iterate(amb_context* c, uint32_t iterations) {
+AMB_API amb_status amb_tgv2_iterate(amb_context* c, uint32_t iterations) try {
     if (!c) return fail(AMB_ERR_INVALID_ARGUMENT, "ctx is NULL");
     if (!c->have_state) return fail(AMB_ERR_INVALID_ARGUMENT, "no iteration state: call amb_tgv2_begin");
     AMB_TRY(set_device(c));
     AMB_TRY(run(c, iterations, 0, nullptr, nullptr, false));
     AMB_HIP(hipStreamSynchronize(c->stream));
     return AMB_OK;
+} catch (...) {
+    return guard_exception();
 }
 
-AMB_API amb_status amb_tgv2_sweep(amb_context* c, int which) {
+AMB_API amb_status amb_tgv2_sweep(amb_context* c, int which) try {
     if (!c) return fail(AMB_ERR_INVALID_ARGUMENT, "ctx is NULL");
     if (!c->have_state) return fail(AMB_ERR_INVALID_ARGUMENT, "no iteration state: call amb_tgv2_begin");
     if (which != 0 && which != 1) return fail(AMB_ERR_INVALID_ARGUMENT, "which must be 0 or 1");
@@ -1086,12 +1320,14 @@ AMB_API amb_status amb_tgv2_sweep(amb_context* c, int which) {
     AMB_TRY(sweep(c, which));
     AMB_HIP(hipStreamSynchronize(c->stream));
     return AMB_OK;
+} catch (...) {
+    return guard_exception();
 }
 
 AMB_API amb_status amb_tgv2_deshade(amb_context* c, amb_memory mem, const float* f, uint32_t w,
                                     uint32_t h, uint32_t d, const amb_tgv_params* p, float* u,
                                     float* vx, float* vy, float* vz, amb_iteration_callback cb,
-                                    void* user) {
+                                    void* user) try {
     if (!c) return fail(AMB_ERR_INVALID_ARGUMENT, "ctx is NULL");
     AMB_TRY(begin(c, mem, f, w, h, d, p));
     AMB_TRY(run(c, p->iterations, p->callback_interval, cb, user, true));
@@ -1101,13 +1337,15 @@ AMB_API amb_status amb_tgv2_deshade(amb_context* c, amb_memory mem, const float*
     AMB_TRY(copy_field_out(c, mem, vz, F_VZ));
     AMB_HIP(hipStreamSynchronize(c->stream));
     return AMB_OK;
+} catch (...) {
+    return guard_exception();
 }
 
 AMB_API amb_status amb_tgv2_deshade_masked(amb_context* c, amb_memory mem, const float* f,
                                            uint32_t w, uint32_t h, uint32_t d,
                                            const amb_tgv_params* p, const uint8_t* active, float* u,
                                            float* vx, float* vy, float* vz,
-                                           amb_iteration_callback cb, void* user) {
+                                           amb_iteration_callback cb, void* user) try {
     if (!c) return fail(AMB_ERR_INVALID_ARGUMENT, "ctx is NULL");
     if (!active) return fail(AMB_ERR_INVALID_ARGUMENT, "active is NULL");
     AMB_TRY(begin(c, mem, f, w, h, d, p));
@@ -1120,21 +1358,25 @@ AMB_API amb_status amb_tgv2_deshade_masked(amb_context* c, amb_memory mem, const
     AMB_TRY(copy_field_out(c, mem, vz, F_VZ));
     AMB_HIP(hipStreamSynchronize(c->stream));
     return AMB_OK;
+} catch (...) {
+    return guard_exception();
 }
 
 AMB_API amb_status amb_tgv2_denoise(amb_context* c, amb_memory mem, const float* f, uint32_t w,
                                     uint32_t h, uint32_t d, const amb_tgv_params* p, float* u,
-                                    amb_iteration_callback cb, void* user) {
+                                    amb_iteration_callback cb, void* user) try {
     if (!c) return fail(AMB_ERR_INVALID_ARGUMENT, "ctx is NULL");
     AMB_TRY(begin(c, mem, f, w, h, d, p));
     AMB_TRY(run(c, p->iterations, p->callback_interval, cb, user, false));
     AMB_TRY(copy_field_out(c, mem, u, F_U));
     AMB_HIP(hipStreamSynchronize(c->stream));
     return AMB_OK;
+} catch (...) {
+    return guard_exception();
 }
 
 AMB_API amb_status amb_divergence(amb_context* c, amb_memory mem, const float* dx, const float* dy,
-                                  const float* dz, uint32_t w, uint32_t h, uint32_t d, float* out) {
+                                  const float* dz, uint32_t w, uint32_t h, uint32_t d, float* out) try {
     if (!c || !out) return fail(AMB_ERR_INVALID_ARGUMENT, "NULL argument");
     AMB_TRY(geometry_only(c, w, h, d));
     const float* v[3];
@@ -1146,6 +1388,8 @@ AMB_API amb_status amb_divergence(amb_context* c, amb_memory mem, const float* d
     AMB_TRY(copy_out(c, mem, out, c->scratch[0]));
     AMB_HIP(hipStreamSynchronize(c->stream));
     return AMB_OK;
+} catch (...) {
+    return guard_exception();
 }
 
 static amb_status transform_entry(amb_context* c, amb_memory mem, const float* image, uint32_t w,
@@ -1163,19 +1407,23 @@ static amb_status transform_entry(amb_context* c, amb_memory mem, const float* i
 
 AMB_API amb_status amb_cosine_transform(amb_context* c, amb_memory mem, const float* image,
                                         uint32_t w, uint32_t h, uint32_t d, uint32_t precision,
-                                        float* out) {
+                                        float* out) try {
     return transform_entry(c, mem, image, w, h, d, precision, out, 0);
+} catch (...) {
+    return guard_exception();
 }
 
 AMB_API amb_status amb_inverse_cosine_transform(amb_context* c, amb_memory mem, const float* image,
                                                 uint32_t w, uint32_t h, uint32_t d,
-                                                uint32_t precision, float* out) {
+                                                uint32_t precision, float* out) try {
     return transform_entry(c, mem, image, w, h, d, precision, out, 1);
+} catch (...) {
+    return guard_exception();
 }
 
 AMB_API amb_status amb_solve_poisson_in_cosine_domain(amb_context* c, amb_memory mem,
                                                       const float* image, uint32_t w, uint32_t h,
-                                                      uint32_t d, uint32_t flags, float* out) {
+                                                      uint32_t d, uint32_t flags, float* out) try {
     if (!c || !image || !out) return fail(AMB_ERR_INVALID_ARGUMENT, "NULL argument");
     AMB_TRY(geometry_only(c, w, h, d));
     AMB_HIP(hipMemcpyAsync(c->scratch[0], image, c->g.n() * 4, kind_in(mem), c->stream));
@@ -1183,12 +1431,14 @@ AMB_API amb_status amb_solve_poisson_in_cosine_domain(amb_context* c, amb_memory
     AMB_TRY(copy_out(c, mem, out, c->scratch[1]));
     AMB_HIP(hipStreamSynchronize(c->stream));
     return AMB_OK;
+} catch (...) {
+    return guard_exception();
 }
 
 AMB_API amb_status amb_integrate_gradients_poisson(amb_context* c, amb_memory mem, const float* vx,
                                                    const float* vy, const float* vz, uint32_t w,
                                                    uint32_t h, uint32_t d, uint32_t precision,
-                                                   uint32_t flags, float* shading) {
+                                                   uint32_t flags, float* shading) try {
     if (!c || !shading) return fail(AMB_ERR_INVALID_ARGUMENT, "NULL argument");
     if (precision > 2) return fail(AMB_ERR_INVALID_ARGUMENT, "unknown dct_precision");
     AMB_TRY(geometry_only(c, w, h, d));
@@ -1199,12 +1449,14 @@ AMB_API amb_status amb_integrate_gradients_poisson(amb_context* c, amb_memory me
     AMB_TRY(copy_out(c, mem, shading, l));
     AMB_HIP(hipStreamSynchronize(c->stream));
     return AMB_OK;
+} catch (...) {
+    return guard_exception();
 }
 
 AMB_API amb_status amb_deshade_epilogue(amb_context* c, amb_memory mem, const float* u,
                                         const float* shading, const float* mask, uint32_t w,
                                         uint32_t h, uint32_t d, uint32_t flags, float* shading_out,
-                                        float* deshaded) {
+                                        float* deshaded) try {
     if (!c || !u || !shading || !deshaded) return fail(AMB_ERR_INVALID_ARGUMENT, "NULL argument");
     AMB_TRY(geometry_only(c, w, h, d));
     const size_t n = c->g.n();
@@ -1224,6 +1476,8 @@ AMB_API amb_status amb_deshade_epilogue(amb_context* c, amb_memory mem, const fl
     AMB_TRY(copy_out(c, mem, deshaded, c->scratch[4]));
     AMB_HIP(hipStreamSynchronize(c->stream));
     return AMB_OK;
+} catch (...) {
+    return guard_exception();
 }
 
 // Poisson integration of the state's v, then the epilogue (A18, A19): shared by
@@ -1277,7 +1531,7 @@ AMB_API amb_status amb_tgv2_deshade_pipeline(amb_context* c, amb_memory mem, con
                                              uint32_t w, uint32_t h, uint32_t d,
                                              const amb_tgv_params* p, const float* mask, float* u,
                                              float* shading, float* deshaded,
-                                             amb_iteration_callback cb, void* user) {
+                                             amb_iteration_callback cb, void* user) try {
     if (!c) return fail(AMB_ERR_INVALID_ARGUMENT, "ctx is NULL");
     AMB_TRY(begin(c, mem, f, w, h, d, p));
     const bool masked = mask && (p->flags & AMB_FLAG_MASKED_ITERATION);
@@ -1285,7 +1539,7 @@ AMB_API amb_status amb_tgv2_deshade_pipeline(amb_context* c, amb_memory mem, con
     // (TGVDeshadeMaskedProcessor.cpp:93-98)
     if (masked && d == 1 && !c->sp.planar) {
         c->sp.planar = 1;
-        c->iplan_ok = c->storage == 0 && fused_image_plan(c->g, c->device, &c->iplan);
+        c->iplan_ok = c->storage == 0 && fused_image_plan(c->g, c->device, c->ov, &c->iplan);
     }
     if (masked) {
         // active set = mask >= 1 (TGVDeshadeMaskedProcessor.cpp:348-353); f stays unmasked
@@ -1302,12 +1556,14 @@ AMB_API amb_status amb_tgv2_deshade_pipeline(amb_context* c, amb_memory mem, con
         AMB_TRY(run(c, p->iterations, p->callback_interval, cb, user, true));
     }
     return deshade_tail(c, mem, mask, p->flags, p->dct_precision, masked, u, shading, deshaded);
+} catch (...) {
+    return guard_exception();
 }
 
 AMB_API amb_status amb_tgvk_deshade(amb_context* c, amb_memory mem, const float* f, uint32_t w,
                                     uint32_t h, uint32_t d, const amb_tgv_params* p,
                                     uint32_t order, const float* alpha, float* u, float* vx,
-                                    float* vy, float* vz, amb_iteration_callback cb, void* user) {
+                                    float* vy, float* vz, amb_iteration_callback cb, void* user) try {
     if (!c) return fail(AMB_ERR_INVALID_ARGUMENT, "ctx is NULL");
     AMB_TRY(tgvk_begin(c, mem, f, w, h, d, p, order, alpha));
     AMB_TRY(tgvk_run(c, p->iterations, p->callback_interval, cb, user));
@@ -1317,6 +1573,8 @@ AMB_API amb_status amb_tgvk_deshade(amb_context* c, amb_memory mem, const float*
     AMB_TRY(copy_field_out(c, mem, vz, F_VZ));
     AMB_HIP(hipStreamSynchronize(c->stream));
     return AMB_OK;
+} catch (...) {
+    return guard_exception();
 }
 
 AMB_API amb_status amb_tgvk_deshade_masked(amb_context* c, amb_memory mem, const float* f,
@@ -1324,7 +1582,7 @@ AMB_API amb_status amb_tgvk_deshade_masked(amb_context* c, amb_memory mem, const
                                            const amb_tgv_params* p, uint32_t order,
                                            const float* alpha, const uint8_t* active, float* u,
                                            float* vx, float* vy, float* vz,
-                                           amb_iteration_callback cb, void* user) {
+                                           amb_iteration_callback cb, void* user) try {
     if (!c) return fail(AMB_ERR_INVALID_ARGUMENT, "ctx is NULL");
     if (!active) return fail(AMB_ERR_INVALID_ARGUMENT, "active is NULL");
     AMB_TRY(tgvk_begin(c, mem, f, w, h, d, p, order, alpha));
@@ -1337,6 +1595,8 @@ AMB_API amb_status amb_tgvk_deshade_masked(amb_context* c, amb_memory mem, const
     AMB_TRY(copy_field_out(c, mem, vz, F_VZ));
     AMB_HIP(hipStreamSynchronize(c->stream));
     return AMB_OK;
+} catch (...) {
+    return guard_exception();
 }
 
 AMB_API amb_status amb_tgvk_deshade_pipeline(amb_context* c, amb_memory mem, const float* f,
@@ -1344,7 +1604,7 @@ AMB_API amb_status amb_tgvk_deshade_pipeline(amb_context* c, amb_memory mem, con
                                              const amb_tgv_params* p, uint32_t order,
                                              const float* alpha, const float* mask, float* u,
                                              float* shading, float* deshaded,
-                                             amb_iteration_callback cb, void* user) {
+                                             amb_iteration_callback cb, void* user) try {
     if (!c) return fail(AMB_ERR_INVALID_ARGUMENT, "ctx is NULL");
     if (!p) return fail(AMB_ERR_INVALID_ARGUMENT, "params is NULL");
     if (!alpha) return fail(AMB_ERR_INVALID_ARGUMENT, "alpha is NULL");
@@ -1379,6 +1639,8 @@ AMB_API amb_status amb_tgvk_deshade_pipeline(amb_context* c, amb_memory mem, con
         AMB_TRY(tgvk_run(c, pp.iterations, pp.callback_interval, cb, user));
     }
     return deshade_tail(c, mem, mask, pp.flags, pp.dct_precision, masked, u, shading, deshaded);
+} catch (...) {
+    return guard_exception();
 }
 
 // ---- resampling -------------------------------------------------------------
@@ -1420,7 +1682,7 @@ AMB_API void amb_resize_extents(uint32_t w, uint32_t h, uint32_t d, float factor
 
 AMB_API amb_status amb_resize_bspline3(amb_context* c, amb_memory mem, const float* in, uint32_t w,
                                        uint32_t h, uint32_t d, float* out, uint32_t ow,
-                                       uint32_t oh, uint32_t od) {
+                                       uint32_t oh, uint32_t od) try {
     if (!c || !in || !out) return fail(AMB_ERR_INVALID_ARGUMENT, "NULL argument");
     AMB_TRY(check_geometry(w, h, d));
     AMB_TRY(check_geometry(ow, oh, od));
@@ -1444,6 +1706,8 @@ AMB_API amb_status amb_resize_bspline3(amb_context* c, amb_memory mem, const flo
         AMB_HIP(hipMemcpyAsync(out, dst, go.n() * 4, hipMemcpyDeviceToHost, c->stream));
     AMB_HIP(hipStreamSynchronize(c->stream));
     return AMB_OK;
+} catch (...) {
+    return guard_exception();
 }
 
 AMB_API amb_status amb_tgvk_deshade_downsampled(amb_context* c, amb_memory mem, const float* f,
@@ -1452,7 +1716,7 @@ AMB_API amb_status amb_tgvk_deshade_downsampled(amb_context* c, amb_memory mem, 
                                                 uint32_t order, const float* alpha,
                                                 const float* mask, float* u, float* shading,
                                                 float* deshaded, amb_iteration_callback cb,
-                                                void* user) {
+                                                void* user) try {
     if (!c || !f || !p || !alpha) return fail(AMB_ERR_INVALID_ARGUMENT, "NULL argument");
     if (!(factor > 0.f)) return fail(AMB_ERR_INVALID_ARGUMENT, "downsampling factor must be positive");
     AMB_TRY(check_geometry(w, h, d));
@@ -1522,32 +1786,48 @@ AMB_API amb_status amb_tgvk_deshade_downsampled(amb_context* c, amb_memory mem, 
     if (deshaded) AMB_HIP(hipMemcpyAsync(deshaded, R, n * 4, kout, c->stream));
     AMB_HIP(hipStreamSynchronize(c->stream));
     return AMB_OK;
+} catch (...) {
+    return guard_exception();
 }
 
-AMB_API amb_status amb_tgv2_energy(amb_context* c, double out[10]) {
+AMB_API amb_status amb_tgv2_energy(amb_context* c, double out[10]) try {
     if (!c || !out) return fail(AMB_ERR_INVALID_ARGUMENT, "NULL argument");
     if (!c->have_state) return fail(AMB_ERR_INVALID_ARGUMENT, "no iteration state");
     AMB_TRY(set_device(c));
-    // the field table moves with the fused sweep's buffer swaps: refresh the
-    // device copy here (c->fields does not change before the sync below)
-    AMB_HIP(hipMemcpyAsync(c->fields_dev, c->fields, sizeof(void*) * N_FIELDS,
-                           hipMemcpyHostToDevice, c->stream));
-    {
-        ProfScope ps(c, P_ENERGY);
-        AMB_HIP(launch_energy(c->stream, c->storage, c->fields_dev, c->f, c->g, c->params.lambda,
-                              c->params.alpha0, c->params.alpha1, int(c->params.data_term),
-                              c->sp.planar, c->partials, c->energy_out));
-    }
+    AMB_TRY(energy_pass(c, c->energy_out));
     AMB_HIP(hipMemcpyAsync(out, c->energy_out, 10 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     AMB_HIP(hipStreamSynchronize(c->stream));
     return AMB_OK;
+} catch (...) {
+    return guard_exception();
+}
+
+AMB_API amb_status amb_tgv2_set_energy_interval(amb_context* c, uint32_t interval) try {
+    if (!c) return fail(AMB_ERR_INVALID_ARGUMENT, "NULL context");
+    c->energy_interval = interval;
+    c->etrace.clear();
+    c->etrace_it.clear();
+    return AMB_OK;
+} catch (...) {
+    return guard_exception();
+}
+
+AMB_API amb_status amb_tgv2_energy_trace(amb_context* c, double* out, uint32_t max_rows,
+                                         uint32_t* rows) try {
+    if (!c || !rows) return fail(AMB_ERR_INVALID_ARGUMENT, "NULL argument");
+    const size_t n = c->etrace.size() / 11;
+    *rows = uint32_t(n);
+    if (out) std::memcpy(out, c->etrace.data(), std::min<size_t>(n, max_rows) * 11 * sizeof(double));
+    return AMB_OK;
+} catch (...) {
+    return guard_exception();
 }
 
 AMB_API amb_status amb_tgv2_denoise_adjoint(amb_context* c, amb_memory mem, const float* f,
                                             uint32_t w, uint32_t h, uint32_t d,
                                             const amb_tgv_params* p, const float* dL_du,
                                             float* u_out, float* dL_df, double dL_dparams[3],
-                                            uint32_t checkpoint_every) {
+                                            uint32_t checkpoint_every) try {
     if (!c || !dL_du || !dL_df || !dL_dparams) return fail(AMB_ERR_INVALID_ARGUMENT, "NULL argument");
     AMB_TRY(check_params(p));
     if (p->data_term != AMB_DATA_L2)
@@ -1631,29 +1911,37 @@ AMB_API amb_status amb_tgv2_denoise_adjoint(amb_context* c, amb_memory mem, cons
     AMB_HIP(hipMemcpyAsync(dL_dparams, acc, 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     AMB_HIP(hipStreamSynchronize(c->stream));
     return AMB_OK;
+} catch (...) {
+    return guard_exception();
 }
 
-AMB_API amb_status amb_profile_enable(amb_context* c, int enable) {
+AMB_API amb_status amb_profile_enable(amb_context* c, int enable) try {
     if (!c) return fail(AMB_ERR_INVALID_ARGUMENT, "ctx is NULL");
     c->prof_on = enable != 0;
     return AMB_OK;
+} catch (...) {
+    return guard_exception();
 }
 
-AMB_API amb_status amb_profile_read(amb_context* c, int cls, double* ms, uint64_t* n) {
+AMB_API amb_status amb_profile_read(amb_context* c, int cls, double* ms, uint64_t* n) try {
     if (!c || cls < 0 || cls >= N_PROF) return fail(AMB_ERR_INVALID_ARGUMENT, "bad argument");
     AMB_TRY(set_device(c));
     AMB_TRY(drain_profile(c));
     if (ms) *ms = c->prof_ms[cls];
     if (n) *n = c->prof_n[cls];
     return AMB_OK;
+} catch (...) {
+    return guard_exception();
 }
 
-AMB_API amb_status amb_profile_reset(amb_context* c) {
+AMB_API amb_status amb_profile_reset(amb_context* c) try {
     if (!c) return fail(AMB_ERR_INVALID_ARGUMENT, "ctx is NULL");
     AMB_TRY(set_device(c));
     AMB_TRY(drain_profile(c));
     for (int k = 0; k < N_PROF; ++k) { c->prof_ms[k] = 0.0; c->prof_n[k] = 0; }
     return AMB_OK;
+} catch (...) {
+    return guard_exception();
 }
 
 }  // extern "C"

@@ -84,8 +84,10 @@ __global__ __launch_bounds__(256) void dct_reference_kernel(
     int n;
     axis_info(g, axis, st, n);
     const int nkt = (n + DT_K - 1) / DT_K;
-    const long l0 = long(blockIdx.x / unsigned(nkt)) * DT_L;
-    const int k0 = int(blockIdx.x % unsigned(nkt)) * DT_K;
+    // the nkt blocks of one line panel are adjacent tiles on one XCD
+    const unsigned tile = xcd_contiguous(blockIdx.x, gridDim.x);
+    const long l0 = long(tile / unsigned(nkt)) * DT_L;
+    const int k0 = int(tile % unsigned(nkt)) * DT_K;
     const int t = int(threadIdx.x);
     const int tl = LINE_CONTIG ? (t / 16) * 4 : (t % 16) * 4;
     const int tk = LINE_CONTIG ? (t % 16) * 4 : (t / 16) * 4;

@@ -24,8 +24,8 @@
 // read per lane and split for the BF16 operands).
 //   column pass (y, z): X = C[k][m] (table), Y = in[o][m][s], D = out[o][k][s]
 //   row pass (x):       X = in[r][m],       Y[m][k] = C[k][m], D = out[r][k]
-// The data tile is read once per K chunk with 16-byte loads; blocks that share
-// a data panel are adjacent in the grid (same XCD L2).  Operand k-slots are
+// The data tile is read once per K chunk with 16-byte loads; tiles that share
+// a data panel are adjacent and run on one XCD (xcd_contiguous), sharing its L2.  Operand k-slots are
 // assigned identically for X and Y, so the products do not depend on the
 // hardware's k order within a slot group.
 #include "amb_internal.hpp"
@@ -281,8 +281,10 @@ __global__ __launch_bounds__(256) void dct_gemm_kernel(GemmArgs a) {
     __shared__ __align__(16) GemmLds<PREC> L;
     const int M = a.M, N = a.N, K = a.K;
     const int tm = (M + BM - 1) / BM, tn = (N + BN - 1) / BN;
-    // grid order: the tiles sharing a data panel are adjacent
-    int bid = blockIdx.x;
+    // tile order: the tiles sharing a data panel are adjacent, and each XCD
+    // takes a contiguous range of tiles, so a panel is fetched from HBM once
+    // into one XCD's L2 instead of once per XCD its tiles land on
+    int bid = int(xcd_contiguous(blockIdx.x, gridDim.x));
     int it, jt, o;
     if (ROW) {                  // data panel = rows it; j tiles fastest
         jt = bid % tn; bid /= tn; it = bid % tm; o = bid / tm;

@@ -27,10 +27,17 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
+// the 17 state field pointers, passed by value
+struct EnergyFields {
+    const void* p[N_FIELDS];
+};
+
 template <class S>
 __global__ __launch_bounds__(EB) void energy_kernel(
-    const typename S::T* const* __restrict__ F, const float* __restrict__ f, Geometry g,
+    EnergyFields EF, const float* __restrict__ f, Geometry g,
     double lam, double a0, double a1, int l2, int planar, double* __restrict__ partials) {
+    typedef typename S::T T;
+    const T* const* F = reinterpret_cast<const T* const*>(EF.p);
     const size_t n = g.n();
     const size_t plane = size_t(g.w) * g.h;
     const size_t st[3] = {1, size_t(g.w), plane};
@@ -149,13 +156,14 @@ inline int energy_blocks(const Geometry& g) {
 }
 
 template <class S>
-hipError_t energy_impl(hipStream_t s, void* const* fields_dev_table, const float* f, Geometry g,
+hipError_t energy_impl(hipStream_t s, const void* const* fields, const float* f, Geometry g,
                        double lam, double a0, double a1, int l2, int planar, double* partials,
                        double* out) {
     const int nb = energy_blocks(g);
-    hipLaunchKernelGGL(energy_kernel<S>, dim3(nb), dim3(EB), 0, s,
-                       reinterpret_cast<const typename S::T* const*>(fields_dev_table), f, g, lam,
-                       a0, a1, l2, planar, partials);
+    EnergyFields EF;
+    for (int k = 0; k < N_FIELDS; ++k) EF.p[k] = fields[k];
+    hipLaunchKernelGGL(energy_kernel<S>, dim3(nb), dim3(EB), 0, s, EF, f, g, lam, a0, a1, l2,
+                       planar, partials);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(energy_finish_kernel, dim3(1), dim3(64 * NQ), 0, s, partials, nb, out);
@@ -166,8 +174,13 @@ hipError_t energy_impl(hipStream_t s, void* const* fields_dev_table, const float
 
 size_t energy_partials_size(Geometry g) { return size_t(energy_blocks(g)) * NQ + 16; }
 
-// `fields` here is a DEVICE array of the 17 field pointers.
-hipError_t launch_energy(hipStream_t s, int storage, void* const* fields, const float* f,
+hipError_t launch_energy_finish(hipStream_t s, const double* partials, int rows, double* out10) {
+    hipLaunchKernelGGL(energy_finish_kernel, dim3(1), dim3(64 * NQ), 0, s, partials, rows, out10);
+    return hipGetLastError();
+}
+
+// `fields`: host array of the 17 device field pointers.
+hipError_t launch_energy(hipStream_t s, int storage, const void* const* fields, const float* f,
                          Geometry g, double lambda, double alpha0, double alpha1, int data_term,
                          int planar, double* partials, double* out10) {
     if (storage == 1)

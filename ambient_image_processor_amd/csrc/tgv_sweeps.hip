@@ -882,11 +882,23 @@ struct FusedFields {
 
 constexpr int FUSED_MAX_THREADS = 768;
 
-// LDS floats of a fused block: `row` floats per row (W, or tx + 2 VX for tiles).
+// LDS floats of a fused block: `row` floats per row (W, or tx + 2 VX for tiles);
+// the energy form keeps u and v of the finished iteration in 4 more rows.
 template <int VX>
-inline size_t fused_lds_floats(int row, int ty) {
+inline size_t fused_lds_floats(int row, int ty, int en = 0) {
     const int L = row / VX, RT = ty + 2;
-    return size_t(8) * RT * row + size_t(8) * RT * (L + 1);
+    return size_t(en ? 12 : 8) * (size_t(RT) * row + size_t(RT) * (L + 1));
+}
+
+// Sum over the first nact lanes of a wave (lanes beyond nact do not exist in a
+// partial last wave; their shuffled values are ignored).
+__device__ __forceinline__ double wave_sum_n(double v, int lane, int nact) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const double t = __shfl_down(v, o, 64);
+        if (lane + o < nact) v += t;
+    }
+    return v;
 }
 
 template <int VX>
@@ -911,11 +923,22 @@ struct FusedPlane {
 // not stored), its y backward differences of eps_B(v_bar) are 0 (the 2-D
 // algebra), bands have no halo rows (height 1), and every nonzero term of
 // every update keeps its place in the reference's operation order.
-template <class S, class I, int VX, int L2, int MK, int MAXT, int PF, int TL, int P2>
+// EN = 1 (3-D loop, P2 = 0, not planar): the launch also reduces, in FP64, the
+// energy and primal-dual gap of the state after the primal half - u and v just
+// computed, p and q read - with the per-voxel formulas of the standalone
+// energy pass (energy.hip): the data term, the -(D+)^T p bracket, its L1
+// infeasibility and the v residual in the primal half at plane s (the p and q
+// neighbours are the ones the updates read), the alpha1 and alpha0 terms and
+// brackets one plane behind, with the dual half (u at z + 1 is then in
+// registers; u and v of the plane in 4 more LDS rows).  One row of ten sums
+// per block goes to epart[blockIdx.x * 10 ..] (slots as the standalone pass).
+template <class S, class I, int VX, int L2, int MK, int MAXT, int PF, int TL, int P2, int EN = 0>
 __global__ __launch_bounds__(MAXT) void fused_sweep_kernel(
     FusedFields ff, const float* __restrict__ f_, const uint8_t* __restrict__ act, Geometry g,
     float tau, float sigma, float lambda, float alpha0, float alpha1, float theta, int ty,
-    int zchunk, int planar, int tx) {
+    int zchunk, int planar, int tx, double* __restrict__ epart,
+    const uint8_t* __restrict__ tile_active) {
+    static_assert(!EN || P2 == 0, "the in-sweep energy covers the 3-D loop");
     typedef typename S::T T;
     typedef VecIO<S, VX> V;
     typedef VecIO<StoreF32, VX> VF;
@@ -928,6 +951,10 @@ __global__ __launch_bounds__(MAXT) void fused_sweep_kernel(
     const int lane = int(threadIdx.x) % L, r = int(threadIdx.x) / L;
     const int nbx = TL ? (W + tx - 1) / tx : 1, nb = (H + ty - 1) / ty;
     const int t = xcd_tile(blockIdx.x, gridDim.x);
+    // masked loop: a tile without an active voxel among the voxels it owns would
+    // store its inputs unchanged; once both copies of the state agree outside
+    // the active set (the host passes tile_active only then), it can skip
+    if (MK && !EN && P2 == 0 && tile_active && !tile_active[t]) return;
     const int tb = TL ? t / nbx : t;
     const int y0 = (tb % nb) * ty;
     const int z0 = (tb / nb) * zchunk, z1 = min(z0 + zchunk, D);
@@ -956,11 +983,17 @@ __global__ __launch_bounds__(MAXT) void fused_sweep_kernel(
     float* const sVB = sUB + RW;          // [3][RT][RWD]
     float* const sUBp = sVB + 3 * RW;     // u_bar element 0 at [r][lane]
     float* const sVBm = sUBp + RE;        // [3][RT][L+1]: v_bar element VX-1 at [r][lane+1]
+    // EN: u and v of the finished iteration at plane s - 1 (stored values)
+    float* const sUK = sVBm + 3 * RE;     // u rows
+    float* const sUKp = sUK + RW;         // u element 0 at [r][lane]
+    float* const sVK = sUKp + RE;         // [3][RT][RWD] v rows
+    float* const sVKm = sVK + 3 * RW;     // [3][RT][L+1]: v element VX-1 at [r][lane+1]
 
     // zero everything once: edge slots at x = -1 / x = W (and beyond the halo
     // lanes) and rows outside the volume are never written afterwards and read
     // only under false flags
-    for (int k = int(threadIdx.x); k < 8 * RW + 8 * RE; k += int(blockDim.x)) lds[k] = 0.f;
+    for (int k = int(threadIdx.x); k < (EN ? 12 : 8) * (RW + RE); k += int(blockDim.x))
+        lds[k] = 0.f;
     __syncthreads();
 
     const T* const* src = reinterpret_cast<const T* const*>(ff.src);
@@ -972,8 +1005,14 @@ __global__ __launch_bounds__(MAXT) void fused_sweep_kernel(
     float qn[3][VX];                      // q_zz, q_xz, q_yz at plane s
     float vb_m2[3][VX];                   // v_bar at plane s - 2
     uint32_t act_c = 0xffffffffu;         // active bytes at plane s - 1
+    // EN: FP64 sums (slots of the standalone energy pass; 0 and 4 formed at the end)
+    double acc[10];
+    float vk_m[3][VX];                    // v (stored value) at plane s - 2
+    const double lamd = double(lambda), a0d = double(alpha0), a1d = double(alpha1);
 #pragma unroll
-    for (int k = 0; k < 3; ++k) { zero(p_c[k]); zero(qn[k]); zero(vb_m2[k]); }
+    for (int k = 0; k < 10; ++k) acc[k] = 0.0;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) { zero(p_c[k]); zero(qn[k]); zero(vb_m2[k]); zero(vk_m[k]); }
 #pragma unroll
     for (int k = 0; k < 6; ++k) zero(q_c[k]);
 
@@ -1056,6 +1095,11 @@ __global__ __launch_bounds__(MAXT) void fused_sweep_kernel(
 #pragma unroll
         for (int k = 0; k < 3; ++k) zero(vbn[k]);
         float uo[VX], vo[3][VX];
+        if (EN) {
+            zero(uo);
+#pragma unroll
+            for (int k = 0; k < 3; ++k) zero(vo[k]);
+        }
         if (need_u) {
             float pym[VX];
             if (IMG) zero(pym); else lds_get<VX>(&sPY[rw - RWD], pym);
@@ -1122,6 +1166,82 @@ __global__ __launch_bounds__(MAXT) void fused_sweep_kernel(
                 vbn[2][e] = S::rt(fmaf(theta, vn2 - v[2][e], vn2));
             }
         }
+        if constexpr (EN) {
+            // primal-half terms at plane s: data term with s = -(D+)^T p (true
+            // transpose: p at the -x/-y/-z neighbours), L1 infeasibility, and
+            // the v residual (eps_B^T q - p with B^T = D+ - e_last e_last^T:
+            // q at the +x/+y/+z neighbours)
+            if (own && pin && s >= z0 && s < z1) {
+                float pym[VX], qxyn_y[VX], qyyn_y[VX], qyzn_y[VX];
+                lds_get<VX>(&sPY[rw - RWD], pym);
+                lds_get<VX>(&sQXY[rw + RWD], qxyn_y);
+                lds_get<VX>(&sQYY[rw + RWD], qyyn_y);
+                lds_get<VX>(&sQYZ[rw + RWD], qyzn_y);
+                const float pxm = sPXm[re];
+                const float qxxn_x = sQXXp[re + 1], qxyn_x = sQXYp[re + 1], qxzn_x = sQXZp[re + 1];
+#pragma unroll
+                for (int e = 0; e < VX; ++e) {
+                    const int x = xb + e;
+                    const int en = e + 1 < VX ? e + 1 : e;
+                    const bool xl = e < VX - 1;
+                    const bool hx = x > 0, hy = y > 0, hz = s > 0;
+                    const bool nx = x < W - 1, ny = y < H - 1, nz = s < D - 1;
+                    double sdiv = 0.0;
+                    {
+                        double t = 0.0;
+                        if (hx) t += double(e > 0 ? px[e > 0 ? e - 1 : 0] : pxm);
+                        if (nx) t -= double(px[e]);
+                        sdiv -= t;
+                    }
+                    {
+                        double t = 0.0;
+                        if (hy) t += double(pym[e]);
+                        if (ny) t -= double(py[e]);
+                        sdiv -= t;
+                    }
+                    {
+                        double t = 0.0;
+                        if (hz) t += double(p_c[2][e]);
+                        if (nz) t -= double(pz[e]);
+                        sdiv -= t;
+                    }
+                    const double umf = double(S::rt(uo[e])) - double(fv[e]);
+                    if (!L2) {
+                        acc[1] += lamd * fabs(umf);
+                        acc[5] += lamd * fabs(umf) - umf * sdiv;
+                        acc[8] += fmax(0.0, fabs(sdiv) - lamd);
+                    } else {
+                        acc[1] += 0.5 * lamd * umf * umf;
+                        const double rr = lamd * umf - sdiv;
+                        acc[5] += rr * rr / (2.0 * lamd);
+                    }
+                    // (eps_B^T q)_c: sum over axes a of -q_ac + q_ac(+a) (last index: -q_ac)
+                    const float qxz_c = qn[1][e], qyz_c = qn[2][e], qzz_c = qn[0][e];
+                    const float qxx_x = xl ? qxx[en] : qxxn_x, qxy_x = xl ? qxy[en] : qxyn_x;
+                    const float qxz_x = xl ? qn[1][en] : qxzn_x;
+                    const float cq[3][3] = {{qxx[e], qxy[e], qxz_c},     // c = x: (xx, xy, xz)
+                                            {qxy[e], qyy[e], qyz_c},     // c = y: (xy, yy, yz)
+                                            {qxz_c, qyz_c, qzz_c}};      // c = z: (xz, yz, zz)
+                    const float nq[3][3] = {{qxx_x, qxyn_y[e], qnn[1][e]},
+                                            {qxy_x, qyyn_y[e], qnn[2][e]},
+                                            {qxz_x, qyzn_y[e], qnn[0][e]}};
+                    const bool na[3] = {nx, ny, nz};
+                    const float pc[3] = {px[e], py[e], pz[e]};
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) {
+                        double t = 0.0;
+#pragma unroll
+                        for (int a = 0; a < 3; ++a) {
+                            double bt = -double(cq[c][a]);
+                            if (na[a]) bt += double(nq[c][a]);
+                            t += bt;
+                        }
+                        const double rr = t - double(pc[c]);
+                        acc[9] += rr * rr;
+                    }
+                }
+            }
+        }
         if (own && pin && s >= z0 && s < z1) {
             V::st(dst[G_U], i, uo);
             V::st(dst[G_VX], i, vo[0]);
@@ -1134,6 +1254,11 @@ __global__ __launch_bounds__(MAXT) void fused_sweep_kernel(
         float vbc[3][VX];
 #pragma unroll
         for (int k = 0; k < 3; ++k) lds_get<VX>(&sVB[k * RW + rw], vbc[k]);
+        float vkc[3][VX];                 // EN: v at plane zd
+        if (EN) {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) lds_get<VX>(&sVK[k * RW + rw], vkc[k]);
+        }
         if (own && zd >= z0) {
             float ubc[VX], uyn[VX], vym[3][VX], vxm[3];
             lds_get<VX>(&sUB[rw], ubc);
@@ -1203,6 +1328,64 @@ __global__ __launch_bounds__(MAXT) void fused_sweep_kernel(
                     for (int k = 0; k < 6; ++k) qo[k][e] = q_c[k][e];
                 }
             }
+            if constexpr (EN) {
+                // terms of the finished iteration at plane zd: alpha1 |D+u - v|
+                // and its bracket with p, alpha0 |eps_B v|_w and its bracket with q
+                float ukc[VX], uky[VX], vky[3][VX], vkxm[3];
+                lds_get<VX>(&sUK[rw], ukc);
+                lds_get<VX>(&sUK[rw + RWD], uky);
+                const float ukx = sUKp[re + 1];
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    lds_get<VX>(&sVK[k * RW + rw - RWD], vky[k]);
+                    vkxm[k] = sVKm[k * RE + re];
+                }
+#pragma unroll
+                for (int e = 0; e < VX; ++e) {
+                    const int x = xb + e;
+                    const int en = e + 1 < VX ? e + 1 : e;
+                    const bool nx = x < W - 1, ny = y < H - 1, nz = zd < D - 1;
+                    const bool hx = x > 0, hy = y > 0, hz = zd > 0;
+                    const double uc = ukc[e];
+                    const double du[3] = {
+                        nx ? double(e < VX - 1 ? ukc[en] : ukx) - uc : 0.0,
+                        ny ? double(uky[e]) - uc : 0.0,
+                        nz ? double(S::rt(uo[e])) - uc : 0.0};   // u at zd + 1 = s
+                    double bv[3][3];
+#pragma unroll
+                    for (int a = 0; a < 3; ++a)
+#pragma unroll
+                        for (int c = 0; c < 3; ++c) {
+                            const double vc = vkc[c][e];
+                            const float nb = a == 0 ? (e > 0 ? vkc[c][e > 0 ? e - 1 : 0] : vkxm[c])
+                                           : a == 1 ? vky[c][e] : vk_m[c][e];
+                            const bool has = a == 0 ? hx : a == 1 ? hy : hz;
+                            bv[a][c] = has ? double(nb) - vc : -vc;
+                        }
+                    double gn = 0.0, gp = 0.0;
+#pragma unroll
+                    for (int a = 0; a < 3; ++a) {
+                        const double ga = du[a] - double(vkc[a][e]);
+                        gn += ga * ga;
+                        gp += ga * double(p_c[a][e]);
+                    }
+                    gn = sqrt(gn);
+                    acc[2] += a1d * gn;
+                    acc[6] += a1d * gn - gp;
+                    const double ev[6] = {bv[0][0], bv[1][1], bv[2][2], 0.5 * (bv[0][1] + bv[1][0]),
+                                          0.5 * (bv[0][2] + bv[2][0]), 0.5 * (bv[1][2] + bv[2][1])};
+                    double enr = 0.0, eq = 0.0;
+#pragma unroll
+                    for (int k = 0; k < 6; ++k) {
+                        const double wgt = k < 3 ? 1.0 : 2.0;
+                        enr += wgt * ev[k] * ev[k];
+                        eq += wgt * ev[k] * double(q_c[k][e]);
+                    }
+                    enr = sqrt(enr);
+                    acc[3] += a0d * enr;
+                    acc[7] += a0d * enr - eq;
+                }
+            }
             V::st(dst[G_PX], id, po[0]); V::st(dst[G_QXX], id, qo[0]);
             if (!IMG) {
                 V::st(dst[G_PY], id, po[1]); V::st(dst[G_QYY], id, qo[1]);
@@ -1225,6 +1408,21 @@ __global__ __launch_bounds__(MAXT) void fused_sweep_kernel(
             sVBm[k * RE + re + 1] = vbn[k][VX - 1];
             copyv(vb_m2[k], vbc[k]);
         }
+        if constexpr (EN) {   // u, v of plane s as stored
+            float t[VX];
+#pragma unroll
+            for (int e = 0; e < VX; ++e) t[e] = S::rt(uo[e]);
+            lds_put<VX>(&sUK[rw], t);
+            sUKp[re] = t[0];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+#pragma unroll
+                for (int e = 0; e < VX; ++e) t[e] = S::rt(vo[k][e]);
+                lds_put<VX>(&sVK[k * RW + rw], t);
+                sVKm[k * RE + re + 1] = t[VX - 1];
+                copyv(vk_m[k], vkc[k]);
+            }
+        }
         copyv(p_c[0], px); copyv(p_c[1], py); copyv(p_c[2], pz);
         copyv(q_c[0], qxx); copyv(q_c[1], qyy); copyv(q_c[2], qn[0]);
         copyv(q_c[3], qxy); copyv(q_c[4], qn[1]); copyv(q_c[5], qn[2]);
@@ -1232,6 +1430,26 @@ __global__ __launch_bounds__(MAXT) void fused_sweep_kernel(
         for (int k = 0; k < 3; ++k) copyv(qn[k], qnn[k]);
         act_c = actv;
         if (PF && s < z1) P = Pn;
+    }
+    if constexpr (EN) {
+        acc[0] = acc[1] + acc[2] + acc[3];
+        acc[4] = acc[5] + acc[6] + acc[7];
+        __syncthreads();   // the LDS rows are free again: [waves][10] doubles
+        double* red = reinterpret_cast<double*>(lds);
+        const int wl = int(threadIdx.x) & 63, wv = int(threadIdx.x) >> 6;
+        const int nact = min(64, int(blockDim.x) - wv * 64);
+#pragma unroll
+        for (int k = 0; k < 10; ++k) {
+            const double t = wave_sum_n(acc[k], wl, nact);
+            if (wl == 0) red[wv * 10 + k] = t;
+        }
+        __syncthreads();
+        if (threadIdx.x < 10) {
+            const int nw = (int(blockDim.x) + 63) / 64;
+            double t = 0.0;
+            for (int w = 0; w < nw; ++w) t += red[w * 10 + threadIdx.x];
+            epart[size_t(blockIdx.x) * 10 + threadIdx.x] = t;
+        }
     }
 }
 
@@ -1263,12 +1481,11 @@ int grid_blocks(const Geometry& g, int zchunk) {
 }
 
 // Next-plane prefetch of the vector sweeps: on for FP32 fields, off for BF16
-// (measured on MI355X at 256^3); AMB_SWEEP_PF=0/1 overrides for tuning.
+// (measured on MI355X at 256^3); SweepParams::sweep_pf = 0/1 sets it (tuning).
 // Both variants compute identical results.
 template <class S>
-inline int sweep_prefetch() {
-    const char* e = std::getenv("AMB_SWEEP_PF");
-    if (e && (e[0] == '0' || e[0] == '1')) return e[0] == '1';
+inline int sweep_prefetch(const SweepParams& p) {
+    if (p.sweep_pf == 0 || p.sweep_pf == 1) return p.sweep_pf;
     return sizeof(typename S::T) == 4 ? 1 : 0;
 }
 
@@ -1307,22 +1524,16 @@ hipError_t primal_vec_launch(hipStream_t s, void* const* fv, const float* f, Geo
 
 template <class S, class I, int TX, int TY>
 hipError_t dual_vec_pf(hipStream_t s, void* const* fv, Geometry g, const SweepParams& p) {
-    return sweep_prefetch<S>() ? dual_vec_launch<S, I, TX, TY, 1>(s, fv, g, p)
+    return sweep_prefetch<S>(p) ? dual_vec_launch<S, I, TX, TY, 1>(s, fv, g, p)
                             : dual_vec_launch<S, I, TX, TY, 0>(s, fv, g, p);
 }
 template <class S, class I, int TX, int TY>
 hipError_t primal_vec_pf(hipStream_t s, void* const* fv, const float* f, Geometry g,
                          const SweepParams& p) {
-    return sweep_prefetch<S>() ? primal_vec_launch<S, I, TX, TY, 1>(s, fv, f, g, p)
+    return sweep_prefetch<S>(p) ? primal_vec_launch<S, I, TX, TY, 1>(s, fv, f, g, p)
                             : primal_vec_launch<S, I, TX, TY, 0>(s, fv, f, g, p);
 }
 
-// 32-bit offsets when every byte offset of a 4-byte field fits in 32 bits.
-// 32-bit element offsets fit volumes under 2^30 voxels; AMB_INDEX64=1 forces
-// the 64-bit forms (tests of those paths on small volumes).
-inline bool small_index(const Geometry& g) {
-    return g.n() * 4 < (size_t(1) << 32) && !force_index64();
-}
 
 template <class S, class I, int TX, int TY, int MK = 0>
 hipError_t dual_launch(hipStream_t s, void* const* fv, Geometry g, const SweepParams& p) {
@@ -1397,7 +1608,8 @@ hipError_t primal_tiles(hipStream_t s, void* const* fv, const float* f, Geometry
     }
 }
 
-template <class S, class I, int VX, int L2, int MK, int MAXT, int PF, int TL = 0, int P2 = 0>
+template <class S, class I, int VX, int L2, int MK, int MAXT, int PF, int TL = 0, int P2 = 0,
+          int EN = 0>
 hipError_t fused_launch(hipStream_t s, const FusedFields& ff, const float* f, Geometry g,
                         const SweepParams& p, const FusedPlan& plan) {
     const int row = TL ? plan.tx + 2 * VX : g.w;
@@ -1405,14 +1617,42 @@ hipError_t fused_launch(hipStream_t s, const FusedFields& ff, const float* f, Ge
     if (L * RT > MAXT) return hipErrorInvalidConfiguration;
     const int nbx = TL ? (g.w + plan.tx - 1) / plan.tx : 1;
     const int nb = (g.h + plan.ty - 1) / plan.ty, nc = (g.d + plan.zchunk - 1) / plan.zchunk;
-    const size_t lds = (size_t(8) * RT * row + size_t(8) * RT * (L + 1)) * sizeof(float);
-    auto kern = fused_sweep_kernel<S, I, VX, L2, MK, MAXT, PF, TL, P2>;
+    size_t lds = (size_t(EN ? 12 : 8) * RT * row + size_t(EN ? 12 : 8) * RT * (L + 1)) * sizeof(float);
+    if (EN) lds = std::max(lds, size_t(16 * 10 * sizeof(double)));   // the block reduction
+    auto kern = fused_sweep_kernel<S, I, VX, L2, MK, MAXT, PF, TL, P2, EN>;
     hipError_t e = ensure_dynamic_lds(reinterpret_cast<const void*>(kern), lds);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(kern, dim3(nbx * nb * nc), dim3(L * RT), lds, s, ff, f, p.active, g, p.tau,
                        p.sigma, p.lambda, p.alpha0, p.alpha1, p.theta, plan.ty, plan.zchunk,
-                       p.planar, TL ? plan.tx : 0);
+                       p.planar, TL ? plan.tx : 0, EN ? p.energy_partials : nullptr,
+                       MK && !EN && P2 == 0 ? p.tile_active : nullptr);
     return hipGetLastError();
+}
+
+// The energy form (EN = 1): 32-bit offsets, the 3-D loop, two voxels per
+// thread, no next-plane prefetch, blocks of at most EN_THREADS threads (two
+// waves per SIMD: the FP64 sums and the carried u, v need the registers); the
+// plan comes from fused_energy_plan.
+constexpr int EN_THREADS = 512;
+
+template <class S, int MK>
+hipError_t fused_energy_mk(hipStream_t s, const FusedFields& ff, const float* f, Geometry g,
+                           const SweepParams& p, const FusedPlan& plan) {
+    typedef uint32_t I;
+    constexpr int T = EN_THREADS;
+    if (plan.vx != 2) return hipErrorInvalidValue;
+    if (plan.tx > 0)
+        return p.data_term == 1 ? fused_launch<S, I, 2, 1, MK, T, 0, 1, 0, 1>(s, ff, f, g, p, plan)
+                                : fused_launch<S, I, 2, 0, MK, T, 0, 1, 0, 1>(s, ff, f, g, p, plan);
+    return p.data_term == 1 ? fused_launch<S, I, 2, 1, MK, T, 0, 0, 0, 1>(s, ff, f, g, p, plan)
+                            : fused_launch<S, I, 2, 0, MK, T, 0, 0, 0, 1>(s, ff, f, g, p, plan);
+}
+
+template <class S>
+hipError_t fused_energy_dispatch(hipStream_t s, const FusedFields& ff, const float* f, Geometry g,
+                                 const SweepParams& p, const FusedPlan& plan) {
+    return p.active ? fused_energy_mk<S, 1>(s, ff, f, g, p, plan)
+                    : fused_energy_mk<S, 0>(s, ff, f, g, p, plan);
 }
 
 // VX = 4: blocks of up to 512 threads (two waves per SIMD, ~200-250 VGPRs).
@@ -1498,23 +1738,20 @@ hipError_t fused_dispatch(hipStream_t s, const FusedFields& ff, const float* f, 
 // at 256^3), so the cost is scaled by 16 / (waves per CU, at most 16) and the
 // halo rows count half; with FP32 fields the halo traffic dominates.  Ties go
 // to vx = 2 and to the lower band.
-// AMB_FUSED=0 disables the fused sweep; AMB_FUSED_VX / _TY / _ZCHUNK / _PF
-// override, and AMB_FUSED_TX the tile width of rows too wide for whole rows.
-bool fused_plan(const Geometry& g, int storage, int device, FusedPlan* plan) {
-    const char* en = std::getenv("AMB_FUSED");
-    if (en && en[0] == '0') return false;
+// The overrides: fused = 0 disables the fused sweep; fused_vx / _ty / _zchunk /
+// _pf set the plan, and fused_tx the tile width of rows too wide for whole rows.
+bool fused_plan(const Geometry& g, int storage, int device, const PlanOverrides& ov,
+                FusedPlan* plan) {
+    if (ov.fused == 0) return false;
     int max_lds = 0, cus = 0;
     if (hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) !=
             hipSuccess ||
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess)
         return false;
-    auto env_int = [](const char* name) {
-        const char* e = std::getenv(name);
-        return e ? std::max(0, std::atoi(e)) : 0;   // 0 (or negative): not set
-    };
-    const int want_vx = env_int("AMB_FUSED_VX"), want_ty = env_int("AMB_FUSED_TY");
-    const int want_zc = env_int("AMB_FUSED_ZCHUNK"), want_tx = env_int("AMB_FUSED_TX");
-    const bool big_index = !small_index(g);
+    const int want_vx = std::max(0, ov.fused_vx), want_ty = std::max(0, ov.fused_ty);
+    const int want_zc = std::max(0, ov.fused_zchunk), want_tx = std::max(0, ov.fused_tx);
+    const bool big_index = needs_index64(g, ov);
+    const int want_pf = ov.fused_pf;   // 0/1, else the plan's own choice
     double best = 1e300;
     FusedPlan bp{0, 0, 1, 0, 0};
     // depth 1 (images) with rows too wide for whole-row bands: nothing along z
@@ -1529,8 +1766,7 @@ bool fused_plan(const Geometry& g, int storage, int device, FusedPlan* plan) {
         const int ty = std::min(want_ty ? want_ty : 6, g.h);
         if (size_t(fused_lds_floats<2>(128, ty)) * sizeof(float) <= size_t(max_lds)) {
             FusedPlan ip{ty, 1, storage == 0 ? 1 : 0, 2, 124};
-            const char* epf = std::getenv("AMB_FUSED_PF");
-            if (epf && (epf[0] == '0' || epf[0] == '1')) ip.pf = epf[0] == '1';
+            if (want_pf == 0 || want_pf == 1) ip.pf = want_pf;
             *plan = ip;
             return true;
         }
@@ -1630,8 +1866,7 @@ bool fused_plan(const Geometry& g, int storage, int device, FusedPlan* plan) {
             if (want_tx) break;
         }
         if (bp.ty == 0) return false;
-        const char* epf = std::getenv("AMB_FUSED_PF");
-        if (epf && (epf[0] == '0' || epf[0] == '1')) bp.pf = epf[0] == '1';
+        if (want_pf == 0 || want_pf == 1) bp.pf = want_pf;
         *plan = bp;
         return true;
     }
@@ -1640,11 +1875,77 @@ bool fused_plan(const Geometry& g, int storage, int device, FusedPlan* plan) {
     // loads issued one step ahead fit the 168 VGPRs of three waves per SIMD
     // (measured at 256^3, band 4: 366.5 vs 373.7 us; with BF16 fields it is
     // slower, 528 vs 316 us)
-    if (storage == 0 && bp.vx == 2 && small_index(g) && (g.w / 2) * (bp.ty + 2) <= 768) bp.pf = 1;
-    const char* epf = std::getenv("AMB_FUSED_PF");
-    if (epf && (epf[0] == '0' || epf[0] == '1')) bp.pf = epf[0] == '1';
+    if (storage == 0 && bp.vx == 2 && !big_index && (g.w / 2) * (bp.ty + 2) <= 768) bp.pf = 1;
+    if (want_pf == 0 || want_pf == 1) bp.pf = want_pf;
     *plan = bp;
     return true;
+}
+
+// The energy form's plan (results do not depend on the plan): two voxels per
+// thread, no prefetch, the iteration plan's z chunk; whole rows in bands of up
+// to the iteration plan's height while three rows fit EN_THREADS threads, else
+// x tiles of 124 columns (64 lanes) in bands of up to 6 rows.  None for 64-bit
+// offsets or an odd width.
+bool fused_energy_plan(const Geometry& g, int device, const PlanOverrides& ov,
+                       const FusedPlan& base, FusedPlan* plan) {
+    if (needs_index64(g, ov) || base.ty < 1 || g.w % 2 != 0) return false;
+    FusedPlan e = base;
+    e.pf = 0;
+    e.vx = 2;
+    if (base.tx == 0 && 3 * (g.w / 2) <= EN_THREADS) {
+        e.tx = 0;
+        e.ty = std::min(base.ty, EN_THREADS / (g.w / 2) - 2);
+    } else {
+        e.tx = base.tx > 0 && base.vx == 2 ? base.tx : 124;
+        if (e.tx >= g.w) e.tx = 124;
+        e.ty = std::min(std::min(base.tx > 0 ? base.ty : 6, 6), EN_THREADS / (e.tx / 2 + 2) - 2);
+    }
+    e.ty = std::min(e.ty, g.h);
+    if (e.ty < 1) return false;
+    int max_lds = 0;
+    if (hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) !=
+        hipSuccess)
+        return false;
+    const int row = e.tx > 0 ? e.tx + 4 : g.w;
+    if (fused_lds_floats<2>(row, e.ty, 1) * sizeof(float) > size_t(max_lds)) return false;
+    *plan = e;
+    return true;
+}
+
+size_t fused_grid_blocks(const Geometry& g, const FusedPlan& plan) {
+    const size_t nbx = plan.tx > 0 ? size_t((g.w + plan.tx - 1) / plan.tx) : 1;
+    return nbx * size_t((g.h + plan.ty - 1) / plan.ty) * size_t((g.d + plan.zchunk - 1) / plan.zchunk);
+}
+
+// Per tile of a fused plan (the kernel's tile index t): 1 if any voxel the
+// tile owns is active.  One block per tile, an OR over its rows and planes.
+__global__ void tile_activity_kernel(const uint8_t* __restrict__ act, Geometry g, int ty,
+                                     int zchunk, int tx, uint8_t* __restrict__ out) {
+    const int t = int(blockIdx.x);
+    const int nbx = tx > 0 ? (g.w + tx - 1) / tx : 1, nb = (g.h + ty - 1) / ty;
+    const int tb = tx > 0 ? t / nbx : t;
+    const int x0 = tx > 0 ? (t % nbx) * tx : 0, x1 = tx > 0 ? min(x0 + tx, g.w) : g.w;
+    const int y0 = (tb % nb) * ty, y1 = min(y0 + ty, g.h);
+    const int z0 = (tb / nb) * zchunk, z1 = min(z0 + zchunk, g.d);
+    const int rw = x1 - x0;
+    const long cnt = long(y1 - y0) * (z1 - z0) * rw;
+    int any = 0;
+    for (long k = threadIdx.x; k < cnt && !any; k += blockDim.x) {
+        const int x = x0 + int(k % rw);
+        const long r = k / rw;
+        const int y = y0 + int(r % (y1 - y0)), z = z0 + int(r / (y1 - y0));
+        any = act[(size_t(z) * g.h + y) * g.w + x] != 0;
+    }
+    any = __syncthreads_or(any);
+    if (threadIdx.x == 0) out[t] = uint8_t(any ? 1 : 0);
+}
+
+hipError_t launch_tile_activity(hipStream_t s, const uint8_t* act, Geometry g,
+                                const FusedPlan& plan, uint8_t* out) {
+    const size_t nt = fused_grid_blocks(g, plan);
+    hipLaunchKernelGGL(tile_activity_kernel, dim3(unsigned(nt)), dim3(256), 0, s, act, g, plan.ty,
+                       plan.zchunk, plan.tx, out);
+    return hipGetLastError();
 }
 
 size_t fused_lds_bytes(const Geometry& g, const FusedPlan& plan) {
@@ -1665,7 +1966,12 @@ hipError_t launch_fused_sweep(hipStream_t s, int storage, void* const* src, void
         ff.src[k] = src[FUSED_FIELD_MAP[k]];
         ff.dst[k] = dst13[k];
     }
-    const bool small = small_index(g);
+    const bool small = !p.index64;
+    if (p.energy_partials) {
+        if (!small || p.planar || p.planar_zero_z || plan.pf) return hipErrorInvalidValue;
+        return storage == 1 ? fused_energy_dispatch<StoreBF16>(s, ff, f, g, p, plan)
+                            : fused_energy_dispatch<StoreF32>(s, ff, f, g, p, plan);
+    }
     if (storage == 1)
         return small ? fused_dispatch<StoreBF16, uint32_t>(s, ff, f, g, p, plan)
                      : fused_dispatch<StoreBF16, size_t>(s, ff, f, g, p, plan);
@@ -1684,23 +1990,21 @@ hipError_t launch_fused_sweep(hipStream_t s, int storage, void* const* src, void
 // depth-1 bands 38, two sweeps 43); 2048^2 79-91 us; 4096^2 284 us with 508
 // columns (380: 285, 252: 302, 224: 291, 1020: 322; 512, i.e. 258 lanes: 390;
 // tiled depth-1 bands 454, two sweeps 545).
-bool fused_image_plan(const Geometry& g, int device, FusedPlan* plan) {
-    if (g.d != 1 || g.w % 2 != 0 || !small_index(g)) return false;
-    const char* en = std::getenv("AMB_FUSED_IMAGE");
-    if (en && en[0] == '0') return false;
+bool fused_image_plan(const Geometry& g, int device, const PlanOverrides& ov, FusedPlan* plan) {
+    if (g.d != 1 || g.w % 2 != 0 || needs_index64(g, ov)) return false;
+    if (ov.fused_image == 0) return false;
     int cus = 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess)
         return false;
-    const char* etx = std::getenv("AMB_FUSED_IMAGE_TX");
-    const char* ezc = std::getenv("AMB_FUSED_IMAGE_ZCHUNK");
-    if (!etx && 3 * (g.w / 2) <= 1024) return false;   // whole-row bands fit
+    const bool want_tx = ov.fused_image_tx >= 0;   // 0: whole rows
+    if (!want_tx && 3 * (g.w / 2) <= 1024) return false;   // whole-row bands fit
     int tx = g.w <= 2048 ? 124 : 508;
-    if (etx) tx = std::atoi(etx) >= g.w ? 0 : std::atoi(etx);
+    if (want_tx) tx = ov.fused_image_tx >= g.w ? 0 : ov.fused_image_tx;
     if (tx < 0 || tx % 2 != 0 || (tx == 0 && g.w / 2 > 768)) return false;
     const long nbx = tx ? (g.w + tx - 1) / tx : 1;
     int zc = int(std::min(64L, std::max(4L, long(g.h) * nbx / (24L * cus))));
     zc = std::min(zc, g.h);
-    if (ezc) zc = std::max(1, std::min(std::atoi(ezc), g.h));
+    if (ov.fused_image_zchunk > 0) zc = std::min(ov.fused_image_zchunk, g.h);
     *plan = FusedPlan{1, zc, 1, 2, tx};
     return true;
 }
@@ -1736,7 +2040,7 @@ hipError_t launch_fused_image(hipStream_t s, void* const* src, void* const* dst1
 
 hipError_t launch_dual_sweep(hipStream_t s, int storage, void* const* fields, Geometry g,
                              const SweepParams& p) {
-    const bool small = small_index(g);
+    const bool small = !p.index64;
     if (storage == 1)
         return small ? dual_tiles<StoreBF16, uint32_t>(s, fields, g, p)
                      : dual_tiles<StoreBF16, size_t>(s, fields, g, p);
@@ -1746,7 +2050,7 @@ hipError_t launch_dual_sweep(hipStream_t s, int storage, void* const* fields, Ge
 
 hipError_t launch_primal_sweep(hipStream_t s, int storage, void* const* fields, const float* f,
                                Geometry g, const SweepParams& p) {
-    const bool small = small_index(g);
+    const bool small = !p.index64;
     if (storage == 1)
         return small ? primal_tiles<StoreBF16, uint32_t>(s, fields, f, g, p)
                      : primal_tiles<StoreBF16, size_t>(s, fields, f, g, p);
@@ -1826,11 +2130,6 @@ hipError_t launch_convert(hipStream_t s, int storage, const void* src, void* dst
                            static_cast<const float*>(src), static_cast<float*>(dst), n);
     }
     return hipGetLastError();
-}
-
-bool force_index64() {
-    const char* e = std::getenv("AMB_INDEX64");
-    return e && e[0] == '1';
 }
 
 hipError_t ensure_dynamic_lds(const void* kernel, size_t bytes) {

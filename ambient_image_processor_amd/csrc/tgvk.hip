@@ -1015,19 +1015,16 @@ hipError_t tgvk_fused_launch(hipStream_t s, const KFusedFields& ff, const float*
     return hipGetLastError();
 }
 
-// Planes per block: enough z-chunks for about 4096 blocks (AMB_TGVK_ZCHUNK
-// overrides, for tests of the chunk edges).
-int tgvk_blocks(const Geometry& g, int* zchunk) {
+// Planes per block: enough z-chunks for about 4096 blocks (the tgvk_zchunk
+// override sets it, for tests of the chunk edges).
+int tgvk_blocks(const Geometry& g, const PlanOverrides& ov, int* zchunk) {
     const int nbx = (g.w + KTX - 1) / KTX, nby = (g.h + KTY - 1) / KTY;
     const long cols = long(nbx) * nby;
     long nz = 4096 / (cols > 0 ? cols : 1);
     if (nz < 1) nz = 1;
     if (nz > g.d) nz = g.d;
     int zc = int((g.d + nz - 1) / nz);
-    if (const char* e = std::getenv("AMB_TGVK_ZCHUNK")) {
-        const int v = std::atoi(e);
-        if (v > 0) zc = std::min(v, g.d);
-    }
+    if (ov.tgvk_zchunk > 0) zc = std::min(ov.tgvk_zchunk, g.d);
     *zchunk = zc;
     return int(cols * ((g.d + zc - 1) / zc));
 }
@@ -1044,24 +1041,20 @@ int tgvk_blocks(const Geometry& g, int* zchunk) {
 // 128-column tiles with bands of 4 to 13 rows, 1027-1385 us (their rows are
 // 264- or 520-B pieces off cache-line boundaries, and every wave then holds
 // owned lanes and runs the dual half).  The z chunk minimises rounds of
-// resident blocks x (planes + 1).  AMB_TGVK_FUSED=0 disables the fused
-// sweep; AMB_TGVK_FUSED_TX, _TY and _ZCHUNK override.
-bool tgvk_fused_plan(const Geometry& g, int levels, int device, TgvkFusedPlan* plan) {
+// resident blocks x (planes + 1).  The overrides tgvk_fused = 0 disable the
+// fused sweep, tgvk_fused_tx, _ty and _zchunk set the plan.
+bool tgvk_fused_plan(const Geometry& g, int levels, int device, const PlanOverrides& ov,
+                     TgvkFusedPlan* plan) {
     // 32-bit offsets only (volumes under 2^30 voxels): the 64-bit form spills
-    if (levels != 1 || g.n() * 4 >= (size_t(1) << 32) || force_index64()) return false;
-    const char* en = std::getenv("AMB_TGVK_FUSED");
-    if (en && en[0] == '0') return false;
+    if (levels != 1 || needs_index64(g, ov)) return false;
+    if (ov.tgvk_fused == 0) return false;
     int max_lds = 0, cus = 0;
     if (hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) !=
             hipSuccess ||
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess)
         return false;
-    auto env_int = [](const char* name) {
-        const char* e = std::getenv(name);
-        return e ? std::max(0, std::atoi(e)) : 0;   // 0 (or negative): not set
-    };
-    const int want_tx = env_int("AMB_TGVK_FUSED_TX"), want_ty = env_int("AMB_TGVK_FUSED_TY");
-    const int want_zc = env_int("AMB_TGVK_FUSED_ZCHUNK");
+    const int want_tx = std::max(0, ov.tgvk_fused_tx), want_ty = std::max(0, ov.tgvk_fused_ty);
+    const int want_zc = std::max(0, ov.tgvk_fused_zchunk);
     int tx = g.w;
     if (want_tx > 0) tx = std::min(want_tx, g.w);
     else if (3 * g.w > KFUSED_MAX_THREADS) tx = (g.w + (g.w + 127) / 128 - 1) / ((g.w + 127) / 128);
@@ -1105,11 +1098,11 @@ hipError_t launch_tgvk_fused(hipStream_t s, const float* const* src, float* cons
 
 hipError_t launch_tgvk_dual(hipStream_t s, float* const* fields_dev, float* const* levels_dev,
                             const float* alpha_dev, const uint8_t* active, Geometry g, int levels,
-                            float sigma, int planar) {
+                            float sigma, int planar, const PlanOverrides& ov) {
     int zc = 1;
-    const int nb = tgvk_blocks(g, &zc);
+    const int nb = tgvk_blocks(g, ov, &zc);
     // fixed-level kernels address with 32-bit byte offsets
-    switch (g.n() <= (size_t(1) << 29) && !force_index64() ? levels : 0) {
+    switch (g.n() <= (size_t(1) << 29) && ov.index64 <= 0 ? levels : 0) {
     case 1: hipLaunchKernelGGL(tgvk_dual_fixed<1>, dim3(nb), dim3(KT), 0, s, fields_dev,
                                levels_dev, alpha_dev, active, g, sigma, planar, zc); break;
     case 2: hipLaunchKernelGGL(tgvk_dual_fixed<2>, dim3(nb), dim3(KT), 0, s, fields_dev,
@@ -1125,11 +1118,11 @@ hipError_t launch_tgvk_dual(hipStream_t s, float* const* fields_dev, float* cons
 
 hipError_t launch_tgvk_primal(hipStream_t s, float* const* fields_dev, float* const* levels_dev,
                               const float* f, const uint8_t* active, Geometry g, int levels,
-                              float tau, float lambda, float theta) {
+                              float tau, float lambda, float theta, const PlanOverrides& ov) {
     int zc = 1;
-    const int nb = tgvk_blocks(g, &zc);
+    const int nb = tgvk_blocks(g, ov, &zc);
     // fixed-level kernels address with 32-bit byte offsets
-    switch (g.n() <= (size_t(1) << 29) && !force_index64() ? levels : 0) {
+    switch (g.n() <= (size_t(1) << 29) && ov.index64 <= 0 ? levels : 0) {
     case 1: hipLaunchKernelGGL(tgvk_primal_fixed<1>, dim3(nb), dim3(KT), 0, s, fields_dev,
                                levels_dev, f, active, g, tau, lambda, theta, zc); break;
     case 2: hipLaunchKernelGGL(tgvk_primal_fixed<2>, dim3(nb), dim3(KT), 0, s, fields_dev,

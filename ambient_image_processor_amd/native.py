@@ -30,7 +30,7 @@ FIELD_NAMES = ["u", "ubar", "vx", "vy", "vz", "vbx", "vby", "vbz", "px", "py", "
                "qxx", "qyy", "qzz", "qxy", "qxz", "qyz"]
 PROFILE_CLASSES = ["dual_sweep", "primal_sweep", "dct", "poisson", "divergence", "epilogue",
                    "init", "energy", "adjoint", "fused_sweep", "tgvk_dual", "tgvk_primal",
-                   "resize", "tgvk_fused"]
+                   "resize", "tgvk_fused", "fused_sweep_energy"]
 
 # Every function include/ambient_amd.h declares (checked by the CPU test suite).
 EXPORTS = [
@@ -44,7 +44,8 @@ EXPORTS = [
     "amb_profile_read", "amb_profile_reset", "amb_tgv2_denoise_adjoint",
     "amb_tgv2_deshade_masked", "amb_deshade_epilogue", "amb_tgvk_deshade",
     "amb_tgvk_deshade_pipeline", "amb_tgvk_deshade_masked", "amb_resize_bspline3",
-    "amb_resize_extents", "amb_tgvk_deshade_downsampled",
+    "amb_resize_extents", "amb_tgvk_deshade_downsampled", "amb_tgv2_set_energy_interval",
+    "amb_tgv2_energy_trace", "amb_context_set_plan_override", "amb_context_clear_plan_overrides",
 ]
 
 
@@ -112,6 +113,10 @@ def lib():
         "amb_tgvk_deshade_downsampled": [vp, mem, fp, u32, u32, u32, POINTER(Params), c_float,
                                          u32, fp, fp, fp, fp, fp, CALLBACK, vp],
         "amb_tgv2_energy": [vp, POINTER(c_double)],
+        "amb_tgv2_set_energy_interval": [vp, u32],
+        "amb_context_set_plan_override": [vp, ctypes.c_char_p, ctypes.c_int32],
+        "amb_context_clear_plan_overrides": [vp],
+        "amb_tgv2_energy_trace": [vp, vp, u32, POINTER(c_uint32)],
         "amb_tgv2_begin": [vp, mem, fp, u32, u32, u32, POINTER(Params)],
         "amb_tgv2_set_state": [vp, mem, POINTER(c_void_p)],
         "amb_tgv2_get_state": [vp, mem, POINTER(c_void_p)],
@@ -419,6 +424,31 @@ class Context:
         out = (c_double * 10)()
         _check(lib().amb_tgv2_energy(self._h, out))
         return np.array(out[:])
+
+    def set_override(self, name: str, value: int):
+        """amb_context_set_plan_override (tests, tuning): value < 0 unsets; applies
+        from the next call that begins an iteration."""
+        _check(lib().amb_context_set_plan_override(self._h, name.encode(), int(value)))
+
+    def clear_overrides(self):
+        _check(lib().amb_context_clear_plan_overrides(self._h))
+
+    def set_energy_interval(self, interval: int):
+        """amb_tgv2_set_energy_interval: energy/gap every `interval` iterations of
+        each iteration call (and after its last), reduced inside the fused sweep
+        where it runs; 0 = off."""
+        _check(lib().amb_tgv2_set_energy_interval(self._h, c_uint32(int(interval))))
+
+    def energy_trace(self) -> np.ndarray:
+        """Rows of the last iteration call: [iteration count, the ten amb_tgv2_energy
+        values] -> array (rows, 11)."""
+        rows = c_uint32(0)
+        _check(lib().amb_tgv2_energy_trace(self._h, None, c_uint32(0), ctypes.byref(rows)))
+        out = np.zeros((rows.value, 11), dtype=np.float64)
+        if rows.value:
+            _check(lib().amb_tgv2_energy_trace(self._h, out.ctypes.data_as(c_void_p),
+                                               c_uint32(rows.value), ctypes.byref(rows)))
+        return out
 
     # ---- DCT Poisson ------------------------------------------------------
     def divergence(self, vx, vy, vz):

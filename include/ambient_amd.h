@@ -125,6 +125,21 @@ AMB_API amb_status amb_memcpy(amb_context* ctx, void* dst, const void* src, size
 AMB_API amb_status amb_synchronize(amb_context* ctx);
 /* The context's HIP stream (a hipStream_t), for callers that order their own work after ours. */
 AMB_API void* amb_context_stream(amb_context* ctx);
+/* Launch-plan overrides for tests and tuning (no counterpart in the reference,
+ * whose launchers take a cuda_block_dimension that changes only the block
+ * size).  Results never depend on them: every plan computes the same values.
+ * They belong to the context and apply from the next call that begins an
+ * iteration (deshade, denoise, begin, TGV-k); a value < 0 unsets one.  Names:
+ *   tile (0-7), zchunk, sweep_pf (0/1)        the two-sweep kernels
+ *   fused (0 = off), fused_vx (2/4), fused_ty, fused_zchunk, fused_tx, fused_pf
+ *                                             the fused TGV2 sweep
+ *   fused_image (0 = off), fused_image_tx, fused_image_zchunk   the 2-D row march
+ *   index64 (1 = the 64-bit-offset kernel forms on any volume)
+ *   tgvk_zchunk, tgvk_fused (0 = off), tgvk_fused_tx, tgvk_fused_ty, tgvk_fused_zchunk
+ * An unknown name returns AMB_ERR_INVALID_ARGUMENT. */
+AMB_API amb_status amb_context_set_plan_override(amb_context* ctx, const char* name,
+                                                 int32_t value);
+AMB_API amb_status amb_context_clear_plan_overrides(amb_context* ctx);
 
 /* ---- A13/A19: TGV2 deshade iteration ------------------------------------
  * Replaces tgv2_l1_deshade_launch<float>(f_host, width, height, depth, lambda,
@@ -240,6 +255,23 @@ AMB_API amb_status amb_tgv2_deshade_pipeline(amb_context* ctx, amb_memory mem, c
  *   out[9] ||eps_B^T q - p||_2  (v stationarity residual) */
 AMB_API amb_status amb_tgv2_energy(amb_context* ctx, double out[10]);
 
+/* ---- A21 in the iteration (no counterpart in the reference) ----------------
+ * With interval > 0, every TGV2 iteration call of ctx (deshade, denoise,
+ * masked, pipeline, iterate) evaluates amb_tgv2_energy's ten quantities after
+ * every iteration count c (1-based, within the call) with c % interval == 0,
+ * after its last iteration and where a callback stops it.  Where iteration c's
+ * primal half runs in the fused sweep (3-D loop, volumes under 2^30 voxels)
+ * the sweep reduces them itself, in FP64 from the values it holds (wave64
+ * shuffles, one row of partial sums per block, a fixed-order final sum): no
+ * extra pass over the state.  Elsewhere the standalone pass runs.  0 = off
+ * (the default).  The setting stays with the context. */
+AMB_API amb_status amb_tgv2_set_energy_interval(amb_context* ctx, uint32_t interval);
+/* The rows of the last call: *rows = their number; out (may be NULL) receives
+ * min(rows, max_rows) rows of 11 doubles - the iteration count c, then
+ * out[0..9] of amb_tgv2_energy for the state after iteration c. */
+AMB_API amb_status amb_tgv2_energy_trace(amb_context* ctx, double* out, uint32_t max_rows,
+                                         uint32_t* rows);
+
 /* ---- adjoint of the unrolled TGV2-L2 denoise (no counterpart in the reference)
  * Runs the amb_tgv2_denoise iteration (data_term must be AMB_DATA_L2, storage
  * AMB_STORAGE_F32) with a checkpoint of the state every `checkpoint_every`
@@ -354,7 +386,8 @@ AMB_API amb_status amb_tgvk_deshade_downsampled(amb_context* ctx, amb_memory mem
  *   4 divergence, 5 epilogue, 6 init, 7 energy, 8 adjoint (whole call),
  *   9 fused sweep (primal half of one iteration + dual half of the next),
  *   10 TGV-k dual sweep, 11 TGV-k primal sweep, 12 B-spline resampling,
- *   13 fused TGV-k sweep (primal half of one iteration + dual half of the next). */
+ *   13 fused TGV-k sweep (primal half of one iteration + dual half of the next),
+ *   14 fused sweep with the in-sweep FP64 energy/gap (and its final sum). */
 AMB_API amb_status amb_profile_enable(amb_context* ctx, int enable);
 AMB_API amb_status amb_profile_read(amb_context* ctx, int kernel_class, double* total_ms,
                                     uint64_t* launches);

@@ -33,6 +33,14 @@ def ctx():
     c.close()
 
 
+@pytest.fixture(autouse=True)
+def _clear_plan_overrides(request):
+    """Launch-plan overrides a test sets on the shared context end with it."""
+    yield
+    if "ctx" in request.fixturenames:
+        request.getfixturevalue("ctx").clear_overrides()
+
+
 def rel_err(a, b):
     """max |a - b| / max |b| (norm-wise relative error)."""
     a = np.asarray(a, dtype=np.float64)

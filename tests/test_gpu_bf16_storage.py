@@ -24,12 +24,12 @@ pytestmark = pytest.mark.gpu
 TAU = float(1 / np.sqrt(20))
 PRM = dict(lam=1.0, alpha0=2.0, alpha1=1.0, tau=TAU, sigma=TAU, theta=1.0)
 
-# launch forms: (name, environment overrides)
+# launch forms: (name, plan overrides)
 FORMS = [
     ("fused_vx2", {}),                                   # default plan, packed rounding
-    ("fused_vx4", {"AMB_FUSED_VX": "4"}),                # four-voxel fused lanes
-    ("two_sweeps_vec", {"AMB_FUSED": "0"}),              # vector tiles (W % 4 == 0)
-    ("two_sweeps_scalar", {"AMB_FUSED": "0", "AMB_TILE": "0"}),   # one voxel per thread
+    ("fused_vx4", {"fused_vx": 4}),                      # four-voxel fused lanes
+    ("two_sweeps_vec", {"fused": 0}),                    # vector tiles (W % 4 == 0)
+    ("two_sweeps_scalar", {"fused": 0, "tile": 0}),      # one voxel per thread
 ]
 
 
@@ -52,15 +52,14 @@ def _c4_generator(shape):
 
 
 @pytest.mark.parametrize("form", FORMS, ids=[f[0] for f in FORMS])
-def test_c4_generator_bitwise(ctx, orc, monkeypatch, form):
+def test_c4_generator_bitwise(ctx, orc, form):
     """64 x 48 x 64 (W % 4 == 0), 150 iterations: all 17 fields equal the oracle's
     BF16 mode bit for bit."""
     from ambient_image_processor_amd import native
-    name, env = form
-    for k in ("AMB_FUSED", "AMB_FUSED_VX", "AMB_TILE"):
-        monkeypatch.delenv(k, raising=False)
-    for k, v in env.items():
-        monkeypatch.setenv(k, v)
+    name, ov = form
+    ctx.clear_overrides()
+    for k, v in ov.items():
+        ctx.set_override(k, v)
     shape = (64, 48, 64)
     f = _c4_generator(shape)
     _say("case", name, shape, "150 iterations")
@@ -72,12 +71,11 @@ def test_c4_generator_bitwise(ctx, orc, monkeypatch, form):
         np.testing.assert_array_equal(got[k], ref.fields[k], err_msg=f"{name} {fname}")
 
 
-def test_c4_generator_odd_width_and_callback(ctx, orc, monkeypatch):
+def test_c4_generator_odd_width_and_callback(ctx, orc):
     """Odd width (one-voxel tiles, scalar rounding) and a callback every 40
     iterations: u and v seen by the callback and returned equal the oracle."""
     from ambient_image_processor_amd import native
-    for k in ("AMB_FUSED", "AMB_FUSED_VX", "AMB_TILE"):
-        monkeypatch.delenv(k, raising=False)
+    ctx.clear_overrides()
     f = _c4_generator((20, 33, 45))
     seen = {}
 

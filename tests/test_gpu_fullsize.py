@@ -54,7 +54,7 @@ def test_256_schedules_and_all_ones_mask_agree(ctx, monkeypatch, w256):
     p = _p256(native, w256, 12)
     fused = ctx.tgv2_deshade(f, p)
     masked = ctx.tgv2_deshade_masked(f, p, np.ones(f.shape, np.uint8))
-    monkeypatch.setenv("AMB_FUSED", "0")
+    ctx.set_override("fused", 0)
     two = ctx.tgv2_deshade(f, p)
     for a, b, c in zip(fused, two, masked):
         np.testing.assert_array_equal(a, b)
@@ -101,7 +101,7 @@ def test_tgvk3_256_fused_bitwise(ctx, orc, monkeypatch, w256):
     np.testing.assert_array_equal(vx, st.v[0])
     np.testing.assert_array_equal(vz, st.v[2])
     fused = ctx.tgvk_deshade(f, params(12), alpha)
-    monkeypatch.setenv("AMB_TGVK_FUSED", "0")
+    ctx.set_override("tgvk_fused", 0)
     two = ctx.tgvk_deshade(f, params(12), alpha)
     for a, b in zip(fused, two):
         np.testing.assert_array_equal(a, b)
@@ -132,7 +132,7 @@ def test_384_bf16_storage_schedules_agree(ctx, monkeypatch):
                               tau=w["tau"], sigma=w["tau"], theta=1.0, iterations=6,
                               data_term=native.DATA_L2, storage=native.STORAGE_BF16)
     fused = ctx.tgv2_deshade(w["f"], p)
-    monkeypatch.setenv("AMB_FUSED", "0")
+    ctx.set_override("fused", 0)
     two = ctx.tgv2_deshade(w["f"], p)
     for a, b in zip(fused, two):
         np.testing.assert_array_equal(a, b)

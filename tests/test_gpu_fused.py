@@ -3,9 +3,10 @@ iteration k + 1 in one launch, u/v/p/q ping-ponged between two copies).
 
 The fused schedule performs the same per-voxel operations in the same order
 as the two-sweep loop, so every result is compared bit for bit: with the two
-sweeps (AMB_FUSED=0) and with the FP32 oracle.  Cases cover band and z-chunk
-edges (voxels per thread, band heights, chunk lengths and prefetch forced
-through AMB_FUSED_VX / _TY / _ZCHUNK / _PF, including 1024-thread blocks),
+sweeps (plan override fused = 0) and with the FP32 oracle.  Cases cover band and
+z-chunk edges (voxels per thread, band heights, chunk lengths and prefetch
+forced through the overrides fused_vx / _ty / _zchunk / _pf, including
+1024-thread blocks),
 partial bands,
 depth 1, L1/L2, BF16 storage, the masked loop, callbacks (which split the
 fused schedule) and continuing from a state the fused loop left in the
@@ -58,9 +59,9 @@ def test_fused_equals_two_sweeps_and_oracle(ctx, orc, shape, data_term, monkeypa
     _say("fused", shape, data_term)
     f = _f(shape, 3)
     p = _params(native, 23, data_term)
-    monkeypatch.setenv("AMB_FUSED", "0")
+    ctx.set_override("fused", 0)
     ref = ctx.tgv2_deshade(f, p)
-    monkeypatch.delenv("AMB_FUSED")
+    ctx.set_override("fused", -1)
     out, prof = _profile_counts(ctx, lambda: ctx.tgv2_deshade(f, p))
     assert prof["fused_sweep"][1] == 22 and prof["dual_sweep"][1] == 1
     assert prof["primal_sweep"][1] == 1
@@ -83,11 +84,11 @@ def test_fused_forced_plans(ctx, monkeypatch, vx, ty, zchunk, pf, shape):
     _say("fused plan vx", vx, "ty", ty, "zchunk", zchunk, "pf", pf, shape)
     f = _f(shape, 5)
     p = _params(native, 9)
-    monkeypatch.setenv("AMB_FUSED", "0")
+    ctx.set_override("fused", 0)
     ref = ctx.tgv2_deshade(f, p)
-    monkeypatch.delenv("AMB_FUSED")
+    ctx.set_override("fused", -1)
     for k, v in (("VX", vx), ("TY", ty), ("ZCHUNK", zchunk), ("PF", pf)):
-        monkeypatch.setenv("AMB_FUSED_" + k, v)
+        ctx.set_override("fused_" + k.lower(), int(v))
     out, prof = _profile_counts(ctx, lambda: ctx.tgv2_deshade(f, p))
     assert prof["fused_sweep"][1] == 8
     for a, b in zip(out, ref):
@@ -102,12 +103,12 @@ def test_fused_masked_and_bf16_per_width(ctx, monkeypatch, vx):
     m = (np.random.default_rng(13).random(shape) > 0.4).astype(np.float32)
     for storage in (native.STORAGE_F32, native.STORAGE_BF16):
         p = _params(native, 14, storage=storage)
-        monkeypatch.setenv("AMB_FUSED", "0")
+        ctx.set_override("fused", 0)
         ref = ctx.tgv2_deshade_masked(f, p, m >= 1)
-        monkeypatch.delenv("AMB_FUSED")
-        monkeypatch.setenv("AMB_FUSED_VX", vx)
+        ctx.set_override("fused", -1)
+        ctx.set_override("fused_vx", int(vx))
         out, prof = _profile_counts(ctx, lambda: ctx.tgv2_deshade_masked(f, p, m >= 1))
-        monkeypatch.delenv("AMB_FUSED_VX")
+        ctx.set_override("fused_vx", -1)
         assert prof["fused_sweep"][1] == 13
         for a, b in zip(out, ref):
             np.testing.assert_array_equal(a, b)
@@ -117,9 +118,9 @@ def test_fused_bf16_storage(ctx, monkeypatch):
     from ambient_image_processor_amd import native
     f = _f((12, 18, 64), 6)
     p = _params(native, 30, storage=native.STORAGE_BF16)
-    monkeypatch.setenv("AMB_FUSED", "0")
+    ctx.set_override("fused", 0)
     ref = ctx.tgv2_deshade(f, p)
-    monkeypatch.delenv("AMB_FUSED")
+    ctx.set_override("fused", -1)
     out = ctx.tgv2_deshade(f, p)
     for a, b in zip(out, ref):
         np.testing.assert_array_equal(a, b)
@@ -131,9 +132,9 @@ def test_fused_masked(ctx, orc, monkeypatch):
     f = _f(shape, 7)
     m = (np.random.default_rng(8).random(shape) > 0.3).astype(np.float32)
     p = _params(native, 25)
-    monkeypatch.setenv("AMB_FUSED", "0")
+    ctx.set_override("fused", 0)
     ref = ctx.tgv2_deshade_masked(f, p, m >= 1)
-    monkeypatch.delenv("AMB_FUSED")
+    ctx.set_override("fused", -1)
     out, prof = _profile_counts(ctx, lambda: ctx.tgv2_deshade_masked(f, p, m >= 1))
     assert prof["fused_sweep"][1] == 24
     for a, b in zip(out, ref):
@@ -154,10 +155,10 @@ def test_fused_with_callback(ctx, monkeypatch):
             return it >= 28
         return cb
 
-    monkeypatch.setenv("AMB_FUSED", "0")
+    ctx.set_override("fused", 0)
     ref_seen = {}
     ref = ctx.tgv2_deshade(f, p, make(ref_seen))
-    monkeypatch.delenv("AMB_FUSED")
+    ctx.set_override("fused", -1)
     out = ctx.tgv2_deshade(f, p, make(seen))
     assert sorted(seen) == sorted(ref_seen) == [7, 14, 21, 28]
     for it in seen:
@@ -174,12 +175,12 @@ def test_fused_state_continues_and_energy(ctx, monkeypatch):
     from ambient_image_processor_amd import native
     f = _f((9, 10, 64), 10)
     p = _params(native, 0)
-    monkeypatch.setenv("AMB_FUSED", "0")
+    ctx.set_override("fused", 0)
     ctx.begin(f, p)
     ctx.iterate(13)
     ref_state = ctx.get_state()
     ref_e = ctx.energy()
-    monkeypatch.delenv("AMB_FUSED")
+    ctx.set_override("fused", -1)
     ctx.begin(f, p)
     ctx.iterate(7)
     ctx.iterate(6)
@@ -203,11 +204,11 @@ def test_fused_x_tiles_against_oracle(ctx, orc, monkeypatch, shape, tile, band, 
     """Halo lanes on both sides of a tile, partial last tiles, band and chunk
     edges: u and v equal the FP32 oracle bit for bit."""
     from ambient_image_processor_amd import native
-    monkeypatch.setenv("AMB_FUSED_TX", tile)
+    ctx.set_override("fused_tx", int(tile))
     if band != "0":
-        monkeypatch.setenv("AMB_FUSED_TY", band)
+        ctx.set_override("fused_ty", int(band))
     if zchunk != "0":
-        monkeypatch.setenv("AMB_FUSED_ZCHUNK", zchunk)
+        ctx.set_override("fused_zchunk", int(zchunk))
     f = _f(shape, 50)
     dt = native.DATA_L1 if data_term == "l1" else native.DATA_L2
     p = native.default_params(lam=0.7, alpha0=2.0, alpha1=1.0, tau=TAU, sigma=TAU, theta=1.0,
@@ -241,10 +242,10 @@ def test_fused_x_tiles_bf16_masked_planar(ctx, orc, monkeypatch, tile):
     f2 = _f((1, 30, 96), 53)
     p2 = native.default_params(lam=0.7, alpha0=2.0, alpha1=1.0, tau=TAU, sigma=TAU, theta=1.0,
                                iterations=12, data_term=native.DATA_L1, flags=native.FLAG_PLANAR)
-    monkeypatch.setenv("AMB_FUSED_TX", tile)
+    ctx.set_override("fused_tx", int(tile))
     tiled = [ctx.tgv2_deshade(f, pb), ctx.tgv2_deshade_masked(f, pm, act),
              ctx.tgv2_deshade(f2, p2)]
-    monkeypatch.setenv("AMB_FUSED", "0")
+    ctx.set_override("fused", 0)
     two = [ctx.tgv2_deshade(f, pb), ctx.tgv2_deshade_masked(f, pm, act), ctx.tgv2_deshade(f2, p2)]
     for a, b in zip(tiled, two):
         for x, y in zip(a, b):

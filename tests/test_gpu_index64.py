@@ -1,6 +1,6 @@
 """GPU tests of the 64-bit-offset kernel forms, which volumes of 2^30 voxels and
-more run (every field then spans 4 GiB or more).  AMB_INDEX64=1 forces those
-forms on small volumes: they perform the same per-voxel arithmetic, so their
+more run (every field then spans 4 GiB or more).  The plan override index64 = 1
+(amb_context_set_plan_override) forces those forms on small volumes: they perform the same per-voxel arithmetic, so their
 results equal the 32-bit forms' bit for bit (TGV2 fused and two-sweep, L1 and
 L2, FP32 and BF16 fields, masked, 2-D; TGV-k through the generic kernels).
 Real 2049 x 1024 x 512 and 1025 x 1024 x 1024 volumes (whole-row bands and x
@@ -32,11 +32,11 @@ def _p(native, iterations, data_term, **kw):
                                  iterations=iterations, data_term=data_term, **kw)
 
 
-def _both(monkeypatch, run):
+def _both(ctx, run):
     a = run()
-    monkeypatch.setenv("AMB_INDEX64", "1")
+    ctx.set_override("index64", 1)
     b = run()
-    monkeypatch.delenv("AMB_INDEX64")
+    ctx.set_override("index64", -1)
     return a, b
 
 
@@ -46,10 +46,10 @@ def _both(monkeypatch, run):
 def test_tgv2_index64_equals_index32(ctx, monkeypatch, shape, data_term, storage, fused):
     from ambient_image_processor_amd import native
     _say("index64 tgv2", shape, data_term, storage, fused)
-    monkeypatch.setenv("AMB_FUSED", fused)
+    ctx.set_override("fused", int(fused))
     f = _f(shape, 40)
     p = _p(native, 12, data_term, storage=storage)
-    a, b = _both(monkeypatch, lambda: ctx.tgv2_deshade(f, p))
+    a, b = _both(ctx, lambda: ctx.tgv2_deshade(f, p))
     for x, y in zip(a, b):
         np.testing.assert_array_equal(x, y)
 
@@ -60,8 +60,8 @@ def test_tgv2_index64_x_tiles(ctx, orc, monkeypatch, shape, tile):
     bands at 2^30 voxels and more), forced on small volumes, against the
     oracle bit for bit (FP32 fields, L1)."""
     from ambient_image_processor_amd import native
-    monkeypatch.setenv("AMB_INDEX64", "1")
-    monkeypatch.setenv("AMB_FUSED_TX", tile)
+    ctx.set_override("index64", 1)
+    ctx.set_override("fused_tx", int(tile))
     f = _f(shape, 46)
     ctx.profile(True)
     ctx.profile_reset()
@@ -83,7 +83,7 @@ def test_tgv2_index64_x_tiles_equal_two_sweeps(ctx, monkeypatch, data_term, stor
     equal the two-sweep loop of the 64-bit kernels bit for bit."""
     from ambient_image_processor_amd import native
     _say("index64 x tiles", data_term, storage, masked)
-    monkeypatch.setenv("AMB_INDEX64", "1")
+    ctx.set_override("index64", 1)
     f = _f((5, 9, 88), 47)
     act = np.random.default_rng(48).random(f.shape) > 0.3
     p = _p(native, 12, data_term, storage=storage)
@@ -96,10 +96,10 @@ def test_tgv2_index64_x_tiles_equal_two_sweeps(ctx, monkeypatch, data_term, stor
         ctx.profile(False)
         return out, prof["fused_sweep"][1]
 
-    monkeypatch.setenv("AMB_FUSED_TX", "32")
+    ctx.set_override("fused_tx", 32)
     a, n_fused = run()
     assert n_fused == 11
-    monkeypatch.setenv("AMB_FUSED", "0")
+    ctx.set_override("fused", 0)
     b, n_two = run()
     assert n_two == 0
     for x, y in zip(a, b):
@@ -111,12 +111,12 @@ def test_tgv2_index64_masked_and_planar(ctx, monkeypatch):
     f = _f((6, 10, 48), 41)
     act = np.random.default_rng(42).random(f.shape) > 0.3
     p = _p(native, 12, 0)
-    a, b = _both(monkeypatch, lambda: ctx.tgv2_deshade_masked(f, p, act))
+    a, b = _both(ctx, lambda: ctx.tgv2_deshade_masked(f, p, act))
     for x, y in zip(a, b):
         np.testing.assert_array_equal(x, y)
     f1 = _f((1, 20, 36), 43)
     p1 = _p(native, 12, 0, flags=native.FLAG_PLANAR)
-    a, b = _both(monkeypatch, lambda: ctx.tgv2_deshade(f1, p1))
+    a, b = _both(ctx, lambda: ctx.tgv2_deshade(f1, p1))
     for x, y in zip(a, b):
         np.testing.assert_array_equal(x, y)
 
@@ -129,7 +129,7 @@ def test_tgvk_index64_equals_index32(ctx, monkeypatch, alpha):
     f = _f((7, 11, 40), 44)
     p = native.default_params(lam=0.7, tau=TAU, sigma=TAU, theta=1.0, iterations=15,
                               data_term=native.DATA_L1)
-    a, b = _both(monkeypatch, lambda: ctx.tgvk_deshade(f, p, alpha))
+    a, b = _both(ctx, lambda: ctx.tgvk_deshade(f, p, alpha))
     for x, y in zip(a, b):
         np.testing.assert_array_equal(x, y)
 

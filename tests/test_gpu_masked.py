@@ -243,3 +243,43 @@ def test_masked_rejects_bad_arguments(ctx):
             ctx._h, native.MEM_HOST, native._ptr(f), 4, 4, 4,
             native.ctypes.byref(_params(native, 2)), None, None, None, None, None,
             native.CALLBACK(), None))
+
+
+@pytest.mark.parametrize("zchunk,storage,cb_every", [(0, 0, 0), (4, 0, 7), (3, 1, 0), (5, 0, 0)])
+def test_masked_fused_tiles_without_active_voxels_skip(ctx, orc, zchunk, storage, cb_every):
+    """A small ball mask leaves whole bands and z chunks of the fused sweep
+    inactive; from the second fused launch of a call those tiles skip (both
+    copies of the state then agree outside the mask).  u and v equal the two-sweep
+    masked loop bit for bit (and the FP32 oracle's masked loop for FP32 fields),
+    also with callbacks, which end iterations with the primal sweep alone."""
+    from ambient_image_processor_amd import native
+    shape = (40, 36, 64)
+    d, h, w = shape
+    z, y, x = np.meshgrid(np.arange(d), np.arange(h), np.arange(w), indexing="ij")
+    ball = ((z - 17) ** 2 + (y - 20) ** 2 + (x - 30) ** 2 <= 9 ** 2).astype(np.float32)
+    f = (0.4 + 0.2 * np.random.default_rng(21).standard_normal(shape)).astype(np.float32)
+    _say("tile skip zchunk", zchunk, "storage", storage, "callback every", cb_every)
+    p = _params(native, 26, storage=storage, callback_interval=cb_every)
+    seen = []
+
+    def cb(it, count, u, vx, vy, vz):
+        seen.append((it, u.copy()))
+        return False
+
+    if zchunk:
+        ctx.set_override("fused_zchunk", zchunk)
+    out = ctx.tgv2_deshade_masked(f, p, ball >= 1, cb if cb_every else None)
+    seen_fused = list(seen)
+    seen.clear()
+    ctx.clear_overrides()
+    ctx.set_override("fused", 0)
+    ref = ctx.tgv2_deshade_masked(f, p, ball >= 1, cb if cb_every else None)
+    for a, b in zip(out, ref):
+        np.testing.assert_array_equal(a, b)
+    assert [s[0] for s in seen_fused] == [s[0] for s in seen]
+    for (_, a), (_, b) in zip(seen_fused, seen):
+        np.testing.assert_array_equal(a, b)
+    if storage == 0:
+        st = orc.tgv2(f, 26, data_term="l1", dtype=np.float32, mask=ball, **PRM)
+        np.testing.assert_array_equal(out[0], st.u)
+        np.testing.assert_array_equal(np.stack(out[1:]), st.v)

@@ -58,14 +58,14 @@ def test_one_iteration_from_random_state(ctx, orc, shape, data_term, zchunk, var
     from ambient_image_processor_amd import native
     _say("case", shape, data_term, zchunk, variant)
     if zchunk != "auto":
-        monkeypatch.setenv("AMB_ZCHUNK", zchunk)
+        ctx.set_override("zchunk", int(zchunk))
     else:
-        monkeypatch.delenv("AMB_ZCHUNK", raising=False)
+        ctx.set_override("zchunk", -1)
     if variant[0] != "auto":
-        monkeypatch.setenv("AMB_TILE", variant[0])
+        ctx.set_override("tile", int(variant[0]))
     else:
-        monkeypatch.delenv("AMB_TILE", raising=False)
-    monkeypatch.setenv("AMB_SWEEP_PF", variant[1])
+        ctx.set_override("tile", -1)
+    ctx.set_override("sweep_pf", int(variant[1]))
     f, st = _random_state(shape, seed=sum(shape))
     p = _params(native, PRM12, 1, data_term)
     ctx.begin(f, p)
@@ -386,3 +386,17 @@ def test_cli_deshade_metaimage(orc, tmp_path):
     np.testing.assert_array_equal(u, st.u)
     np.testing.assert_array_equal(l, lo)
     np.testing.assert_array_equal(rr, ro)
+
+
+def test_dct_extent_beyond_table_budget_is_rejected(ctx):
+    """An extent whose cosine tables would exceed the stated budget (16384 per
+    axis) returns AMB_ERR_NOT_SUPPORTED instead of allocating them; the context
+    stays usable."""
+    from ambient_image_processor_amd import native
+    a = np.zeros((1, 1, 16385), np.float32)
+    with pytest.raises(native.AmbError) as ei:
+        ctx.cosine_transform(a)
+    assert ei.value.status == 4 and "16384" in str(ei.value)
+    b = np.random.default_rng(2).standard_normal((3, 4, 5)).astype(np.float32)
+    from oracle import oracle
+    np.testing.assert_array_equal(ctx.cosine_transform(b), oracle.cosine_transform(b, dtype=np.float32))

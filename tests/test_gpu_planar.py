@@ -58,7 +58,7 @@ def test_planar_iteration_bitwise(ctx, orc, shape, data_term, fused, monkeypatch
     from ambient_image_processor_amd import native
     _say("planar", shape, data_term, "fused", fused)
     f = _f(shape, 3)
-    monkeypatch.setenv("AMB_FUSED", fused)
+    ctx.set_override("fused", int(fused))
     u, vx, vy, vz = ctx.tgv2_deshade(f, _params(native, 31, data_term))
     st = orc.tgv2(f, 31, data_term=data_term, dtype=np.float32, planar=True, **PRM)
     np.testing.assert_array_equal(u, st.u)
@@ -77,7 +77,7 @@ def test_planar_forced_fused_plans(ctx, orc, monkeypatch, vx, ty, zchunk, pf):
     _say("planar fused plan", vx, ty, zchunk, pf, shape)
     f = _f(shape, 4)
     for k, v in (("VX", vx), ("TY", ty), ("ZCHUNK", zchunk), ("PF", pf)):
-        monkeypatch.setenv("AMB_FUSED_" + k, v)
+        ctx.set_override("fused_" + k.lower(), int(v))
     u, vxo, vyo, _ = ctx.tgv2_deshade(f, _params(native, 17))
     st = orc.tgv2(f, 17, data_term="l1", dtype=np.float32, planar=True, **PRM)
     np.testing.assert_array_equal(u, st.u)
@@ -108,7 +108,7 @@ def test_planar_masked_bitwise(ctx, orc, fused, monkeypatch):
     _say("planar masked fused", fused)
     f = _f(shape, 6)
     m = _mask(shape, 7)
-    monkeypatch.setenv("AMB_FUSED", fused)
+    ctx.set_override("fused", int(fused))
     u, vx, vy, vz = ctx.tgv2_deshade_masked(f, _params(native, 25), m >= 1)
     st = orc.tgv2(f, 25, data_term="l1", dtype=np.float32, mask=m, planar=True, **PRM)
     np.testing.assert_array_equal(u, st.u)
@@ -267,7 +267,7 @@ def test_planar_state_with_nonzero_z_fields(ctx, monkeypatch):
     state = (rng.standard_normal((native.N_FIELDS,) + f.shape) * 0.1).astype(np.float32)
     runs = []
     for fused in ("1", "0"):
-        monkeypatch.setenv("AMB_FUSED", fused)
+        ctx.set_override("fused", int(fused))
         ctx.begin(f, p)
         ctx.set_state(state)
         ctx.iterate(9)
@@ -282,7 +282,7 @@ def test_planar_wide_and_tiled_images(ctx, orc, monkeypatch, shape, tile):
     fused 2-D sweep, z fields skipped, bit for bit with the oracle."""
     from ambient_image_processor_amd import native
     if tile != "0":
-        monkeypatch.setenv("AMB_FUSED_TX", tile)
+        ctx.set_override("fused_tx", int(tile))
     f = _f(shape, 62)
     ctx.profile(True)
     ctx.profile_reset()
@@ -311,9 +311,9 @@ def test_planar_row_march(ctx, orc, monkeypatch, shape, tx, zc, data_term, maske
     from ambient_image_processor_amd import native
     _say("row march", shape, tx, zc, data_term, masked)
     if tx != "0":
-        monkeypatch.setenv("AMB_FUSED_IMAGE_TX", tx)
+        ctx.set_override("fused_image_tx", int(tx))
     if zc != "0":
-        monkeypatch.setenv("AMB_FUSED_IMAGE_ZCHUNK", zc)
+        ctx.set_override("fused_image_zchunk", int(zc))
     f = _f(shape, 63)
     p = _params(native, 16, data_term)
     m = _mask(shape, 64) if masked else None

@@ -59,7 +59,7 @@ def test_tgvk_z_chunks(ctx, orc, monkeypatch, zchunk, alpha):
     3 levels) and the generic kernels (4 levels)."""
     from ambient_image_processor_amd import native
     _say("tgvk zchunk", zchunk, "order", len(alpha))
-    monkeypatch.setenv("AMB_TGVK_ZCHUNK", zchunk)
+    ctx.set_override("tgvk_zchunk", int(zchunk))
     f = _f((17, 9, 70), 17)
     u, vx, vy, vz = ctx.tgvk_deshade(f, _params(native, 15), alpha)
     st = orc.tgvk(f, 15, alpha=alpha, dtype=np.float32, **KPRM)
@@ -143,9 +143,9 @@ def test_tgvk_fused_bitwise_against_oracle(ctx, orc, monkeypatch, shape, tile, b
     txh = tx + 2 if tx < w else w
     fits = tile == "0" or (int(band) <= h and (int(band) + 2) * txh <= 1024)
     if tile != "0":
-        monkeypatch.setenv("AMB_TGVK_FUSED_TX", tile)
-        monkeypatch.setenv("AMB_TGVK_FUSED_TY", band)
-        monkeypatch.setenv("AMB_TGVK_FUSED_ZCHUNK", zchunk)
+        ctx.set_override("tgvk_fused_tx", int(tile))
+        ctx.set_override("tgvk_fused_ty", int(band))
+        ctx.set_override("tgvk_fused_zchunk", int(zchunk))
     f = _f(shape, 30)
     alpha = (1.0, 2.0, 0.6)
     ctx.profile(True)
@@ -170,7 +170,7 @@ def test_tgvk_fused_equals_two_launches(ctx, monkeypatch, shape):
     alpha = (1.0, 2.0, 0.05)   # a binding projection of r
     p = _params(native, 40, flags=native.FLAG_PLANAR if shape[0] == 1 else 0)
     fused = ctx.tgvk_deshade(f, p, alpha)
-    monkeypatch.setenv("AMB_TGVK_FUSED", "0")
+    ctx.set_override("tgvk_fused", 0)
     two = ctx.tgvk_deshade(f, p, alpha)
     for a, b in zip(fused, two):
         np.testing.assert_array_equal(a, b)
@@ -219,7 +219,7 @@ def test_tgvk_fused_wide_rows_and_fallback(ctx, orc, monkeypatch):
     assert prof["tgvk_fused"][1] == 9 and prof["tgvk_dual"][1] == 1
     np.testing.assert_array_equal(u, st.u)
     np.testing.assert_array_equal(vx, st.v[0])
-    monkeypatch.setenv("AMB_TGVK_FUSED_TX", "400")
+    ctx.set_override("tgvk_fused_tx", 400)
     ctx.profile_reset()
     u, vx, vy, vz = ctx.tgvk_deshade(f, _params(native, 10), alpha)
     prof = ctx.profile_read()
@@ -372,7 +372,7 @@ def test_masked_tgvk_bitwise(ctx, orc, monkeypatch, alpha, zchunk):
     from ambient_image_processor_amd import native
     _say("masked tgvk order", len(alpha), "zchunk", zchunk)
     if zchunk != "0":
-        monkeypatch.setenv("AMB_TGVK_ZCHUNK", zchunk)
+        ctx.set_override("tgvk_zchunk", int(zchunk))
     shape = (11, 13, 68)
     f = _f(shape, 18)
     m = _mask(shape, 19)
