@@ -13,22 +13,33 @@ The backward pass re-runs the forward iteration with a checkpoint every
 """
 from __future__ import annotations
 
+import contextlib
 import math
+import threading
 
 import numpy as np
 import torch
 
 from . import native
 
+# One library context per device, shared by every thread; each holds a lock
+# around the calls of one forward or backward (a context keeps the iteration
+# state, scratch and adjoint workspace of the call in progress, and autograd
+# may run a backward on its own thread while another thread runs a forward).
 _contexts = {}
+_contexts_lock = threading.Lock()
 
 
-def _context(device_index: int) -> native.Context:
-    c = _contexts.get(device_index)
-    if c is None:
-        c = native.Context(device_index)
-        _contexts[device_index] = c
-    return c
+@contextlib.contextmanager
+def _context(device_index: int):
+    with _contexts_lock:
+        entry = _contexts.get(device_index)
+        if entry is None:
+            entry = (native.Context(device_index), threading.Lock())
+            _contexts[device_index] = entry
+    c, lock = entry
+    with lock:
+        yield c
 
 
 def _scalar(x) -> float:
@@ -55,16 +66,16 @@ class TGV2L2DenoiseFunction(torch.autograd.Function):
         u = torch.empty_like(vols)
         if vols.is_cuda:
             torch.cuda.current_stream(vols.device).synchronize()
-            c = _context(vols.device.index or 0)
-            for b in range(vols.shape[0]):
-                native._check(native.lib().amb_tgv2_denoise(
-                    c._h, native.MEM_DEVICE, native.c_void_p(vols[b].data_ptr()),
-                    *native._dims(shape), native.ctypes.byref(p),
-                    native.c_void_p(u[b].data_ptr()), native.CALLBACK(), None))
+            with _context(vols.device.index or 0) as c:
+                for b in range(vols.shape[0]):
+                    native._check(native.lib().amb_tgv2_denoise(
+                        c._h, native.MEM_DEVICE, native.c_void_p(vols[b].data_ptr()),
+                        *native._dims(shape), native.ctypes.byref(p),
+                        native.c_void_p(u[b].data_ptr()), native.CALLBACK(), None))
         else:
-            c = _context(0)
-            for b in range(vols.shape[0]):
-                u[b] = torch.from_numpy(c.tgv2_denoise(vols[b].numpy(), p))
+            with _context(0) as c:
+                for b in range(vols.shape[0]):
+                    u[b] = torch.from_numpy(c.tgv2_denoise(vols[b].numpy(), p))
         ctx.save_for_backward(vols)
         ctx.meta = (p, shape, int(checkpoint_every), f.dim(),
                     [torch.is_tensor(a) for a in (alpha0, alpha1, lam)],
@@ -81,16 +92,17 @@ class TGV2L2DenoiseFunction(torch.autograd.Function):
         gp = np.zeros(3)
         if vols.is_cuda:
             torch.cuda.current_stream(vols.device).synchronize()
-            c = _context(vols.device.index or 0)
-            for b in range(vols.shape[0]):
-                gp += c.tgv2_denoise_adjoint_device(vols[b].data_ptr(), shape, p,
-                                                    g[b].data_ptr(), None, grad_f[b].data_ptr(), k)
+            with _context(vols.device.index or 0) as c:
+                for b in range(vols.shape[0]):
+                    gp += c.tgv2_denoise_adjoint_device(vols[b].data_ptr(), shape, p,
+                                                        g[b].data_ptr(), None,
+                                                        grad_f[b].data_ptr(), k)
         else:
-            c = _context(0)
-            for b in range(vols.shape[0]):
-                _, gf, gpb = c.tgv2_denoise_adjoint(vols[b].numpy(), p, g[b].numpy(), k)
-                grad_f[b] = torch.from_numpy(gf)
-                gp += gpb
+            with _context(0) as c:
+                for b in range(vols.shape[0]):
+                    _, gf, gpb = c.tgv2_denoise_adjoint(vols[b].numpy(), p, g[b].numpy(), k)
+                    grad_f[b] = torch.from_numpy(gf)
+                    gp += gpb
         outs = []
         for j in range(3):
             outs.append(torch.tensor(gp[j], dtype=torch.float32, device=devs[j]) if is_t[j] else None)

@@ -12,7 +12,9 @@
 
 #include <zlib.h>
 
+#include <algorithm>
 #include <cctype>
+#include <climits>
 #include <cstdint>
 #include <cstring>
 #include <fstream>
@@ -83,19 +85,40 @@ inline float convert(const unsigned char* p, const std::string& t, bool msb) {
     return float(load_le<int64_t>(p, msb));
 }
 
-inline std::vector<unsigned char> inflate_all(const std::vector<unsigned char>& in, size_t out_size) {
+// zlib counts bytes per call in 32 bits (uInt): payloads of 4 GiB and more
+// are fed and drained in pieces of at most `chunk` bytes.
+inline std::vector<unsigned char> inflate_all(const std::vector<unsigned char>& in, size_t out_size,
+                                              size_t chunk = size_t(UINT_MAX)) {
     std::vector<unsigned char> out(out_size);
     z_stream zs;
     std::memset(&zs, 0, sizeof(zs));
     if (inflateInit(&zs) != Z_OK) throw std::runtime_error("zlib inflateInit failed");
-    zs.next_in = const_cast<unsigned char*>(in.data());
-    zs.avail_in = uInt(in.size());
-    zs.next_out = out.data();
-    zs.avail_out = uInt(out.size());
-    const int r = inflate(&zs, Z_FINISH);
+    unsigned char spare = 0;   // next_out target once the output is full
+    zs.next_out = out_size ? out.data() : &spare;
+    size_t in_pos = 0, out_pos = 0;
+    int r = Z_OK;
+    while (r != Z_STREAM_END) {
+        if (zs.avail_in == 0 && in_pos < in.size()) {
+            const size_t n = std::min(chunk, in.size() - in_pos);
+            zs.next_in = const_cast<unsigned char*>(in.data()) + in_pos;
+            zs.avail_in = uInt(n);
+            in_pos += n;
+        }
+        if (zs.avail_out == 0 && out_pos < out_size) {
+            const size_t n = std::min(chunk, out_size - out_pos);
+            zs.next_out = out.data() + out_pos;
+            zs.avail_out = uInt(n);
+            out_pos += n;
+        }
+        // with the output full inflate still reads the stream's trailer; it
+        // returns Z_BUF_ERROR when no progress is possible (truncated input,
+        // or more output than the header declares)
+        r = inflate(&zs, Z_NO_FLUSH);
+        if (r != Z_OK && r != Z_STREAM_END) break;
+    }
+    const bool ok = r == Z_STREAM_END && size_t(zs.total_out) == out_size;
     inflateEnd(&zs);
-    if (r != Z_STREAM_END || zs.total_out != out_size)
-        throw std::runtime_error("compressed element data is corrupt or truncated");
+    if (!ok) throw std::runtime_error("compressed element data is corrupt or truncated");
     return out;
 }
 
