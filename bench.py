@@ -24,9 +24,19 @@ Unless --no-split, the default workload also runs in two split forms -- FP32
 fields with the BF16x3 MFMA DCT, and BF16 field storage with the BF16x3 DCT --
 and the line carries, for each, its throughput, its speed-up over the plain
 form (FP32 fields, reference-numerics DCT) and the max- and RMS-norm relative
-errors of its u and shading against the plain form's.  Unless --no-host, one more
-deshade runs from host buffers (PCIe transfers included) and is reported as
-`host_buffers`.
+errors of its u and shading against the plain form's.  Unless --no-energy, one
+more deshade runs with the FP64 energy / gap reduced inside the fused sweep
+every 100 iterations (`energy`: the final gap and brackets, the energy launch's
+time against the plain fused launch).  Unless --no-host, one more deshade runs
+from host buffers (PCIe transfers included) and is reported as `host_buffers`.
+Unless --no-configs, the other benchmark configurations run briefly under the
+same clock (`configs`): the 512x512x256 DCT Poisson solve in the three DCT
+precisions (3 solves each), the 384^3 TGV2-L2 deshade with FP32 and with BF16
+fields (800 iterations each), and the unrolled TGV2-L2 denoise + adjoint of
+4 x 128^3.  Where tests/golden/longrun.npz (the oracle's full-size runs) is
+present, the line also states whether the plain form equals the FP32 oracle bit
+for bit (per-plane sums of u and the shading) and the FP32 oracle's distance to
+FP64 at this configuration.
 """
 import argparse
 import json
@@ -126,6 +136,30 @@ def two_sweep_bytes(storage_bytes, n):
     return sweep_bytes("dual_sweep", storage_bytes, n) + sweep_bytes("primal_sweep", storage_bytes, n)
 
 
+PLAN_OVERRIDES = []   # (name, value) from --plan, set on every context of the run
+
+
+def new_context():
+    """A library context on GPU 0 with the --plan launch-plan overrides (tuning;
+    results do not depend on them)."""
+    from ambient_image_processor_amd import native
+    ctx = native.Context(0)
+    for name, value in PLAN_OVERRIDES:
+        ctx.set_override(name, value)
+    return ctx
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for ln in fh:
+                if ln.startswith("model name"):
+                    return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def rel_err(a, b):
     """max |a - b| / max |b|"""
     return float(np.max(np.abs(a.astype(np.float64) - b)) / max(np.max(np.abs(b)), 1e-30))
@@ -221,7 +255,7 @@ def deshade_256(args):
                                      dct_precision=dct_, flags=native.FLAG_SET_NEGATIVE_ZERO)
 
     p = params(storage, dct)
-    ctx = native.Context(0)
+    ctx = new_context()
     f_dev = ctx.alloc(n * 4)
     u_dev, l_dev, r_dev = ctx.alloc(n * 4), ctx.alloc(n * 4), ctx.alloc(n * 4)
     ctx.upload(f_dev, f)
@@ -261,6 +295,32 @@ def deshade_256(args):
                           "shading_rel_err": rel_err(l_s, l_plain),
                           "shading_rms_err": rms_err(l_s, l_plain)}
 
+    # in-sweep FP64 energy / gap every 100 iterations (the fused sweep's energy
+    # form) on one more deshade: its launch time against the plain fused sweep
+    energy = None
+    if not args.no_energy and storage == native.STORAGE_F32 and dct == native.DCT_REFERENCE:
+        every = 100
+        ctx.set_energy_interval(every)
+        e_el, e_prof = timed(ctx, step_with(p), 1, 0)
+        trace = ctx.energy_trace()
+        ctx.set_energy_interval(0)
+        ms_e, n_e = e_prof["fused_sweep_energy"]
+        ms_f, n_f = e_prof["fused_sweep"]
+        fused_us = 1000.0 * ms_f / max(n_f, 1)
+        en_us = 1000.0 * ms_e / max(n_e, 1)
+        energy = {"every": every, "launches": n_e, "energy_launch_us": en_us,
+                  "fused_launch_us": fused_us,
+                  "energy_launch_over_fused": en_us / fused_us if n_e else None,
+                  "loop_overhead": (ms_e + ms_f) / (fused_us * 1e-3 * (n_e + n_f)) - 1.0
+                  if n_e else None,
+                  "ms_per_step": 1000.0 * e_el,
+                  "final": {"iteration": int(trace[-1, 0]), "E": trace[-1, 1],
+                            "gap": trace[-1, 5], "brackets": [trace[-1, 6], trace[-1, 7],
+                                                               trace[-1, 8]],
+                            "l1_infeasibility": trace[-1, 9], "v_residual": trace[-1, 10]},
+                  "gap_trace": [[int(r[0]), r[5]] for r in trace]}
+        print("energy trace done", file=sys.stderr, flush=True)
+
     host = None
     if not args.no_host:
         t0 = time.perf_counter()
@@ -281,8 +341,38 @@ def deshade_256(args):
         cpu_s = time.perf_counter() - t1
         cpu = {"value": float(n) * args.cpu_iters / cpu_s, "unit": "voxel-iters/s",
                "cores": oracle.num_threads(), "kind": "port",
+               "cpu_model": cpu_model(), "nproc": os.cpu_count(),
                "sample": f"{args.cpu_iters} TGV2-L1 iterations of the same {n_side}^3 volume "
                          f"(FP32 reference op order, OpenMP), {cpu_s:.1f} s"}
+
+    # the other benchmark configurations, short runs under the same clock
+    configs = None
+    if not args.no_configs and n_side == 256:
+        import types
+        sub = types.SimpleNamespace(steps=3, warmup=1, iterations=1000)
+        configs = {}
+        for name, fn in (("poisson_512x512x256", poisson_512), ("bf16_storage_384", bf16_384),
+                         ("adjoint_4x128", adjoint_4x128)):
+            t0 = time.perf_counter()
+            r_ = fn(sub, emit=False)
+            r_ = {k: v for k, v in r_.items()
+                  if k not in ("n_gpus", "higher_is_better", "scaling", "vs_baseline")}
+            r_["wall_s"] = time.perf_counter() - t0
+            configs[name] = r_
+            print(f"{name} done", file=sys.stderr, flush=True)
+
+    gold = longrun_golden()
+    if split is not None and gold is not None and "c2_fp64_u_max" in gold:
+        split["fp32_oracle_vs_fp64"] = {
+            "note": "the plain form's oracle (FP32 mirror order) against the FP64 oracle at "
+                    "this configuration, offline (tests/golden/longrun.npz)",
+            "u_max": float(gold["c2_fp64_u_max"]), "u_rms": float(gold["c2_fp64_u_rms"]),
+            "shading_max": float(gold["c2_fp64_shading_max"]),
+            "shading_rms": float(gold["c2_fp64_shading_rms"])}
+        if args.iterations == 1000:
+            split["plain_equals_fp32_oracle"] = bool(
+                np.array_equal(plane_sums(u_plain), gold["c2_u_sums"]) and
+                np.array_equal(plane_sums(l_plain), gold["c2_shading_sums"]))
 
     traffic = MEASURED_TRAFFIC.get((dom, sb, n_side))
     avg_s = kernels[dom]["avg_us"] * 1e-6
@@ -314,8 +404,10 @@ def deshade_256(args):
         "other_kernels_ms_per_step": {k: round(prof[k][0] / args.steps, 3)
                                       for k in ("dct", "poisson", "divergence", "epilogue", "init")},
         "split": split,
+        "energy": energy,
         "host_buffers": host,
         "cpu_baseline": cpu,
+        "configs": configs,
     }
     print(json.dumps(line))
     for ptr in (f_dev, u_dev, l_dev, r_dev):
@@ -333,7 +425,7 @@ def _tgv2_variant(args, f, mask, flags, workload, metric, data, cpu_note):
     tau = float(1.0 / np.sqrt(20.0))
     p = native.default_params(lam=0.5, alpha0=2.0, alpha1=1.0, tau=tau, sigma=tau, theta=1.0,
                               iterations=args.iterations, data_term=native.DATA_L1, flags=flags)
-    ctx = native.Context(0)
+    ctx = new_context()
     bufs = [ctx.alloc(n * 4) for _ in range(4 if mask is None else 5)]
     f_dev, u_dev, l_dev, r_dev = bufs[:4]
     m_dev = bufs[4] if mask is not None else None
@@ -420,7 +512,7 @@ def planar_4096(args):
                   f"2-D TGV2-L1 iterations of the same {side}^2 image")
 
 
-def poisson_512(args):
+def poisson_512(args, emit=True):
     """DCT Poisson integration of v = D+ phi at 512 x 512 x 256, phi a sum of
     64 cosine modes: B-divergence, forward DCT with the eigenvalue divide, and
     inverse DCT; the exact answer is phi - mean(phi) = phi."""
@@ -429,7 +521,7 @@ def poisson_512(args):
     n = int(np.prod(shape))
     _, phi = phantoms.poisson_field(shape, seed=3)
     v = [forward_difference(phi, a).astype(np.float32) for a in range(3)]
-    ctx = native.Context(0)
+    ctx = new_context()
     vd = [ctx.alloc(n * 4) for _ in range(3)]
     for d, a in zip(vd, v):
         ctx.upload(d, a)
@@ -455,7 +547,7 @@ def poisson_512(args):
                      "rms_rel_err_vs_closed_form": rms_err(l, phi)}
         print(f"{name} done", file=sys.stderr, flush=True)
     best = res["bf16x3"]
-    print(json.dumps({
+    line = {
         "metric": "DCT Poisson solves/s 512x512x256 (B-div + DCT-II + eigen divide + DCT-III)",
         "value": 1000.0 / best["ms_per_solve"], "unit": "solves/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": best["ms_per_solve"],
@@ -464,22 +556,42 @@ def poisson_512(args):
         "config": {"workload": "poisson_512x512x256", "shape": list(shape),
                    "flops_per_solve_fp32_equiv": flops},
         "variants": {k: {kk: (round(vv, 6) if isinstance(vv, float) else vv)
-                         for kk, vv in v.items()} for k, v in res.items()}}))
+                         for kk, vv in v.items()} for k, v in res.items()}}
+    if emit:
+        print(json.dumps(line))
     for d in vd + [out]:
         ctx.free(d)
     ctx.close()
+    return line
 
 
-def bf16_384(args):
+def longrun_golden():
+    """The oracle's full-size results (tests/golden/longrun.npz, written by
+    tests/golden/gen_longrun.py), or None."""
+    path = os.path.join(ROOT, "tests", "golden", "longrun.npz")
+    if not os.path.exists(path):
+        return None
+    with np.load(path) as z:
+        return {k: z[k] for k in z.files}
+
+
+def plane_sums(a):
+    return np.sum(np.asarray(a, dtype=np.float32), axis=(1, 2), dtype=np.float64)
+
+
+def bf16_384(args, emit=True):
     """TGV2-L2 deshade at 384^3, 800 iterations, BF16 field storage against the
-    FP32 path (errors of u and of the shading l)."""
+    FP32 path (errors of u and of the shading l), and, where the oracle's
+    full-size results are committed, whether the BF16 run equals the oracle's
+    BF16 mode bit for bit (per-plane sums) together with that mode's offline
+    errors against the FP64 oracle."""
     from ambient_image_processor_amd import native, phantoms
     w = phantoms.workload("deshade_384")
     f = w["f"]
     shape = f.shape
     n = f.size
     iters = args.iterations if args.iterations != 1000 else w["iterations"]
-    ctx = native.Context(0)
+    ctx = new_context()
     f_dev = ctx.alloc(n * 4)
     u_dev, l_dev, r_dev = ctx.alloc(n * 4), ctx.alloc(n * 4), ctx.alloc(n * 4)
     ctx.upload(f_dev, f)
@@ -497,7 +609,23 @@ def bf16_384(args):
                      "u": ctx.download(u_dev, shape), "l": ctx.download(l_dev, shape)}
     res = {k: {kk: vv for kk, vv in v.items() if kk not in ("u", "l")} for k, v in out.items()}
     b, r = out["bf16"], out["f32"]
-    print(json.dumps({
+    gold = longrun_golden()
+    vs64 = None
+    if gold is not None and "c4_u_sums" in gold and iters == 800:
+        # the shading of this run comes from the F32 MFMA DCT; the oracle's from the
+        # reference-numerics DCT, so only u is compared bit for bit
+        same = bool(np.array_equal(plane_sums(b["u"]), gold["c4_u_sums"]))
+        vs64 = {"u_equals_oracle_bf16_mode": same,
+                "note": "oracle BF16 mode and FP32 run against the FP64 oracle at this "
+                        "configuration (offline, tests/golden/longrun.npz)",
+                "bf16_u_max": float(gold["c4_bf16_u_max"]),
+                "bf16_u_rms": float(gold["c4_bf16_u_rms"]),
+                "bf16_shading_max": float(gold["c4_bf16_shading_max"]),
+                "bf16_shading_rms": float(gold["c4_bf16_shading_rms"]),
+                "f32_u_max": float(gold["c4_f32_u_max"]), "f32_u_rms": float(gold["c4_f32_u_rms"]),
+                "f32_shading_max": float(gold["c4_f32_shading_max"]),
+                "f32_shading_rms": float(gold["c4_f32_shading_rms"])}
+    line = {
         "metric": "voxel-iters/s TGV2-L2 deshade 384^3 800 it, BF16 field storage",
         "value": b["voxel_iters_per_s"], "unit": "voxel-iters/s", "n_gpus": 1, "steps": 1,
         "warmup": 0, "ms_per_step": 1000.0 * b["s"], "higher_is_better": True,
@@ -508,13 +636,17 @@ def bf16_384(args):
         "u_rel_err_vs_f32": rel_err(b["u"], r["u"]), "u_rms_err_vs_f32": rms_err(b["u"], r["u"]),
         "shading_rel_err_vs_f32": rel_err(b["l"], r["l"]),
         "shading_rms_err_vs_f32": rms_err(b["l"], r["l"]),
-        "runs": res}))
+        "vs_fp64": vs64,
+        "runs": res}
+    if emit:
+        print(json.dumps(line))
     for ptr in (f_dev, u_dev, l_dev, r_dev):
         ctx.free(ptr)
     ctx.close()
+    return line
 
 
-def adjoint_4x128(args):
+def adjoint_4x128(args, emit=True):
     """Unrolled TGV2-L2 denoise (200 iterations) and its hand-written adjoint
     (checkpoints every 20) on 4 volumes of 128^3; MSE loss to the clean volumes."""
     from ambient_image_processor_amd import native, phantoms
@@ -526,7 +658,7 @@ def adjoint_4x128(args):
     p = native.default_params(lam=w["lam"], alpha0=w["alpha0"], alpha1=w["alpha1"],
                               tau=w["tau"], sigma=w["tau"], theta=1.0, iterations=iters,
                               data_term=native.DATA_L2)
-    ctx = native.Context(0)
+    ctx = new_context()
     f_dev, g_dev, u_dev, gf_dev = (ctx.alloc(n * 4) for _ in range(4))
     t_fwd = t_adj = 0.0
     for b in range(vols.shape[0]):
@@ -543,7 +675,7 @@ def adjoint_4x128(args):
         t_adj += time.perf_counter() - t0
         print(f"volume {b} done", file=sys.stderr, flush=True)
     vi = float(n) * iters * vols.shape[0]
-    print(json.dumps({
+    line = {
         "metric": "voxel-iters/s unrolled TGV2-L2 denoise forward + adjoint, 4 x 128^3, 200 it",
         "value": vi / t_adj, "unit": "voxel-iters/s (forward + reverse)", "n_gpus": 1,
         "steps": vols.shape[0], "warmup": 0, "ms_per_step": 1000.0 * t_adj / vols.shape[0],
@@ -552,10 +684,13 @@ def adjoint_4x128(args):
         "config": {"workload": "tgv2_l2_adjoint_4x128^3", "iterations": iters,
                    "checkpoint_every": w["checkpoint_every"]},
         "forward_only_voxel_iters_per_s": vi / t_fwd,
-        "adjoint_over_forward_time": t_adj / t_fwd}))
+        "adjoint_over_forward_time": t_adj / t_fwd}
+    if emit:
+        print(json.dumps(line))
     for ptr in (f_dev, g_dev, u_dev, gf_dev):
         ctx.free(ptr)
     ctx.close()
+    return line
 
 
 def tgvk3_256(args):
@@ -573,7 +708,7 @@ def tgvk3_256(args):
     tau = float(1.0 / np.sqrt(20.0))
     p = native.default_params(lam=0.5, tau=tau, sigma=tau, theta=1.0, iterations=args.iterations,
                               data_term=native.DATA_L1, flags=native.FLAG_SET_NEGATIVE_ZERO)
-    ctx = native.Context(0)
+    ctx = new_context()
     f_dev = ctx.alloc(n * 4)
     u_dev, l_dev, r_dev = ctx.alloc(n * 4), ctx.alloc(n * 4), ctx.alloc(n * 4)
     ctx.upload(f_dev, f)
@@ -639,7 +774,7 @@ def downsampled_512(args):
     p = native.default_params(lam=0.5, tau=tau, sigma=tau, theta=1.0, iterations=iters,
                               flags=native.FLAG_MASKED_ITERATION | native.FLAG_SET_NEGATIVE_ZERO
                               | native.FLAG_ADD_BACKGROUND_BACK)
-    ctx = native.Context(0)
+    ctx = new_context()
     bufs = [ctx.alloc(n * 4) for _ in range(5)]
     f_dev, m_dev, u_dev, l_dev, r_dev = bufs
     ctx.upload(f_dev, f)
@@ -690,12 +825,19 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-split", action="store_true")
     ap.add_argument("--no-host", action="store_true")
+    ap.add_argument("--no-energy", action="store_true")
+    ap.add_argument("--no-configs", action="store_true")
     ap.add_argument("--workload", choices=["deshade_256", "poisson_512", "bf16_384",
                                            "adjoint_4x128", "tgvk3_256", "downsampled_512",
                                            "masked_256", "planar_4096"],
                     default="deshade_256")
     ap.add_argument("--copy-calibration", action="store_true")
+    ap.add_argument("--plan", action="append", default=[], metavar="NAME=VALUE",
+                    help="launch-plan override (amb_context_set_plan_override), repeatable")
     args = ap.parse_args()
+    for item in args.plan:
+        name, _, value = item.partition("=")
+        PLAN_OVERRIDES.append((name.strip(), int(value)))
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only (the TGV sweeps would shard into z-slabs "
