@@ -18,6 +18,7 @@
 #include <new>
 #include <algorithm>
 #include <cstdlib>
+#include <mutex>
 #include <string>
 #include <utility>
 #include <vector>
@@ -1093,6 +1094,39 @@ amb_status guard_exception() noexcept {
 
 
 
+// ---------------------------------------------------------------------------
+// amb_tgv2_energy_noalloc: the energy pass for callers that cannot allocate.
+// Its block partials live in `scratch`, a fixed device array: 4096 rows of 10
+// doubles (the energy grid is capped at 4096 blocks, so every volume fits) plus
+// 16 spare, 327,808 bytes per device, held until the process ends.  Each block
+// writes its whole row and the final sum reads only the rows written, so
+// nothing has to start at zero.  In host code `scratch` names a host-side
+// stand-in: its device address is taken once per device with
+// hipGetSymbolAddress (that device current) and kept in g_scratch_ptr.
+// ---------------------------------------------------------------------------
+constexpr size_t NOALLOC_SCRATCH_DOUBLES = 4096 * 10 + 16;
+static __device__ __align__(256) double scratch[NOALLOC_SCRATCH_DOUBLES];
+
+namespace {
+
+std::mutex g_scratch_mutex;
+std::vector<double*> g_scratch_ptr;   // per device; null until resolved
+
+amb_status scratch_ptr(int device, double** out) {
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (device >= int(g_scratch_ptr.size())) g_scratch_ptr.resize(size_t(device) + 1, nullptr);
+    if (!g_scratch_ptr[device]) {
+        void* p = nullptr;
+        AMB_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return fail(AMB_ERR_HIP, "hipGetSymbolAddress returned no address for scratch");
+        g_scratch_ptr[device] = static_cast<double*>(p);
+    }
+    *out = g_scratch_ptr[device];
+    return AMB_OK;
+}
+
+}  // namespace
+
 // ===========================================================================
 extern "C" {
 
@@ -1797,6 +1831,32 @@ AMB_API amb_status amb_tgv2_energy(amb_context* c, double out[10]) try {
     AMB_TRY(energy_pass(c, c->energy_out));
     AMB_HIP(hipMemcpyAsync(out, c->energy_out, 10 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     AMB_HIP(hipStreamSynchronize(c->stream));
+    return AMB_OK;
+} catch (...) {
+    return guard_exception();
+}
+
+AMB_API amb_status amb_tgv2_energy_noalloc(const float* f, const void* const* fields,
+                                           uint32_t w, uint32_t h, uint32_t d,
+                                           const amb_tgv_params* p, void* stream,
+                                           double* out_dev) try {
+    if (!f || !fields || !p || !out_dev) return fail(AMB_ERR_INVALID_ARGUMENT, "NULL argument");
+    for (int k = 0; k < N_FIELDS; ++k)
+        if (!fields[k]) return fail(AMB_ERR_INVALID_ARGUMENT, "NULL field pointer");
+    AMB_TRY(check_geometry(w, h, d));
+    AMB_TRY(check_params(p));
+    if ((p->flags & AMB_FLAG_PLANAR) && d != 1)
+        return fail(AMB_ERR_INVALID_ARGUMENT, "AMB_FLAG_PLANAR needs depth 1");
+    const Geometry g{int(w), int(h), int(d)};
+    if (energy_partials_size(g) > NOALLOC_SCRATCH_DOUBLES)
+        return fail(AMB_ERR_NOT_SUPPORTED, "energy partials exceed the 4096-row scratch array");
+    int device = 0;
+    AMB_HIP(hipGetDevice(&device));
+    double* part = nullptr;
+    AMB_TRY(scratch_ptr(device, &part));
+    AMB_HIP(launch_energy(static_cast<hipStream_t>(stream), int(p->storage), fields, f, g,
+                          p->lambda, p->alpha0, p->alpha1, int(p->data_term),
+                          (p->flags & AMB_FLAG_PLANAR) ? 1 : 0, part, out_dev));
     return AMB_OK;
 } catch (...) {
     return guard_exception();

@@ -46,6 +46,7 @@ EXPORTS = [
     "amb_tgvk_deshade_pipeline", "amb_tgvk_deshade_masked", "amb_resize_bspline3",
     "amb_resize_extents", "amb_tgvk_deshade_downsampled", "amb_tgv2_set_energy_interval",
     "amb_tgv2_energy_trace", "amb_context_set_plan_override", "amb_context_clear_plan_overrides",
+    "amb_tgv2_energy_noalloc",
 ]
 
 
@@ -117,6 +118,7 @@ def lib():
         "amb_context_set_plan_override": [vp, ctypes.c_char_p, ctypes.c_int32],
         "amb_context_clear_plan_overrides": [vp],
         "amb_tgv2_energy_trace": [vp, vp, u32, POINTER(c_uint32)],
+        "amb_tgv2_energy_noalloc": [vp, POINTER(c_void_p), u32, u32, u32, POINTER(Params), vp, vp],
         "amb_tgv2_begin": [vp, mem, fp, u32, u32, u32, POINTER(Params)],
         "amb_tgv2_set_state": [vp, mem, POINTER(c_void_p)],
         "amb_tgv2_get_state": [vp, mem, POINTER(c_void_p)],

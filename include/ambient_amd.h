@@ -272,6 +272,26 @@ AMB_API amb_status amb_tgv2_set_energy_interval(amb_context* ctx, uint32_t inter
 AMB_API amb_status amb_tgv2_energy_trace(amb_context* ctx, double* out, uint32_t max_rows,
                                          uint32_t* rows);
 
+/* ---- amb_tgv2_energy without allocation (no counterpart in the reference) --
+ * For callers that cannot allocate device memory: code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc.  The ten
+ * quantities of amb_tgv2_energy for a state the caller holds in device memory,
+ * enqueued on `stream` (a hipStream_t of the calling thread's current device;
+ * NULL = the null stream) with no context and no hipMalloc / hipFree:
+ *   f        W*H*D FP32 input (device);
+ *   fields   17 device pointers in the amb_tgv2_get_state order (FP32, or
+ *            bfloat16 values when params->storage is AMB_STORAGE_BF16);
+ *   params   lambda, alpha0, alpha1, data_term, storage and AMB_FLAG_PLANAR;
+ *   out_dev  10 doubles in device memory, written by the stream's work.
+ * Scratch: one fixed device array per device of 4096 rows of 10 FP64 partial
+ * sums (327,808 bytes, held until the process ends); any volume the library
+ * accepts (at most 2^31 - 1 voxels) fits it.  Calls that overlap on one device
+ * (on different streams or threads) share that array and race: order them. */
+AMB_API amb_status amb_tgv2_energy_noalloc(const float* f, const void* const* fields,
+                                           uint32_t width, uint32_t height, uint32_t depth,
+                                           const amb_tgv_params* params, void* stream,
+                                           double* out_dev);
+
 /* ---- adjoint of the unrolled TGV2-L2 denoise (no counterpart in the reference)
  * Runs the amb_tgv2_denoise iteration (data_term must be AMB_DATA_L2, storage
  * AMB_STORAGE_F32) with a checkpoint of the state every `checkpoint_every`
