@@ -104,6 +104,8 @@ struct amb_context {
     AxisTables axes[3];
     // profiling
     bool prof_on = false;
+    uint32_t prof_every = 1;          // time one launch in prof_every per class
+    uint64_t prof_seen[N_PROF] = {};  // launches while profiling, timed or not
     std::vector<hipEvent_t> pool;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> pending[N_PROF];
     double prof_ms[N_PROF] = {};
@@ -189,7 +191,7 @@ struct ProfScope {
     int cls;
     hipEvent_t a = nullptr;
     ProfScope(amb_context* c_, int cls_) : c(c_), cls(cls_) {
-        if (c->prof_on) {
+        if (c->prof_on && (c->prof_seen[cls]++ % c->prof_every) == 0) {
             a = take_event(c);
             if (a) (void)hipEventRecord(a, c->stream);
         }
@@ -1978,6 +1980,7 @@ AMB_API amb_status amb_tgv2_denoise_adjoint(amb_context* c, amb_memory mem, cons
 AMB_API amb_status amb_profile_enable(amb_context* c, int enable) try {
     if (!c) return fail(AMB_ERR_INVALID_ARGUMENT, "ctx is NULL");
     c->prof_on = enable != 0;
+    c->prof_every = enable > 1 ? uint32_t(enable) : 1u;
     return AMB_OK;
 } catch (...) {
     return guard_exception();
@@ -1994,11 +1997,20 @@ AMB_API amb_status amb_profile_read(amb_context* c, int cls, double* ms, uint64_
     return guard_exception();
 }
 
+AMB_API amb_status amb_profile_count(amb_context* c, int cls, uint64_t* launches) try {
+    if (!c || !launches || cls < 0 || cls >= N_PROF)
+        return fail(AMB_ERR_INVALID_ARGUMENT, "bad argument");
+    *launches = c->prof_seen[cls];
+    return AMB_OK;
+} catch (...) {
+    return guard_exception();
+}
+
 AMB_API amb_status amb_profile_reset(amb_context* c) try {
     if (!c) return fail(AMB_ERR_INVALID_ARGUMENT, "ctx is NULL");
     AMB_TRY(set_device(c));
     AMB_TRY(drain_profile(c));
-    for (int k = 0; k < N_PROF; ++k) { c->prof_ms[k] = 0.0; c->prof_n[k] = 0; }
+    for (int k = 0; k < N_PROF; ++k) { c->prof_ms[k] = 0.0; c->prof_n[k] = 0; c->prof_seen[k] = 0; }
     return AMB_OK;
 } catch (...) {
     return guard_exception();

@@ -46,7 +46,7 @@ EXPORTS = [
     "amb_tgvk_deshade_pipeline", "amb_tgvk_deshade_masked", "amb_resize_bspline3",
     "amb_resize_extents", "amb_tgvk_deshade_downsampled", "amb_tgv2_set_energy_interval",
     "amb_tgv2_energy_trace", "amb_context_set_plan_override", "amb_context_clear_plan_overrides",
-    "amb_tgv2_energy_noalloc",
+    "amb_tgv2_energy_noalloc", "amb_profile_count",
 ]
 
 
@@ -127,6 +127,7 @@ def lib():
         "amb_profile_enable": [vp, c_int],
         "amb_profile_read": [vp, c_int, POINTER(c_double), POINTER(c_uint64)],
         "amb_profile_reset": [vp],
+        "amb_profile_count": [vp, c_int, POINTER(c_uint64)],
         "amb_tgv2_denoise_adjoint": [vp, mem, fp, u32, u32, u32, POINTER(Params), fp, fp, fp,
                                      POINTER(c_double), u32],
     }
@@ -517,16 +518,27 @@ class Context:
         return np.array(gp[:])
 
     # ---- profiling ----------------------------------------------------------
-    def profile(self, enable=True):
-        _check(lib().amb_profile_enable(self._h, int(enable)))
+    def profile(self, enable=True, every=1):
+        """Per-launch HIP-event timing; every = k >= 2 times one launch in k per class."""
+        _check(lib().amb_profile_enable(self._h, (int(every) if every > 1 else 1) if enable else 0))
 
     def profile_reset(self):
         _check(lib().amb_profile_reset(self._h))
 
     def profile_read(self):
+        """{class: (ms of the timed launches, timed launches)}"""
         res = {}
         for k, name in enumerate(PROFILE_CLASSES):
             ms, n = c_double(), c_uint64()
             _check(lib().amb_profile_read(self._h, k, ctypes.byref(ms), ctypes.byref(n)))
             res[name] = (ms.value, n.value)
+        return res
+
+    def profile_counts(self):
+        """{class: launches since the last reset, timed or not}"""
+        res = {}
+        for k, name in enumerate(PROFILE_CLASSES):
+            n = c_uint64()
+            _check(lib().amb_profile_count(self._h, k, ctypes.byref(n)))
+            res[name] = n.value
         return res

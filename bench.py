@@ -195,13 +195,20 @@ def copy_calibration(nbytes=1 << 30, reps=3):
     print(json.dumps({"copy_bytes": nbytes, "reps": reps}))
 
 
-def timed(ctx, step, steps, warmup):
-    """Warm-up, then `steps` timed calls of step(); returns (seconds, profile)."""
+PROFILE_EVERY = 8   # HIP events around one launch in 8 of each kernel class
+
+
+def timed(ctx, step, steps, warmup, every=PROFILE_EVERY):
+    """Warm-up, then `steps` timed calls of step(); returns (seconds, profile).
+    Inside the timed region HIP events on the library's stream time one launch
+    in `every` of each kernel class (the events' own cost, about 2% of a step
+    when every launch is bracketed, stays out of the others); profile[class] =
+    (average of the timed launches x all launches in ms, all launches)."""
     for _ in range(warmup):
         step()
         print("warmup step done", file=sys.stderr, flush=True)
     ctx.synchronize()
-    ctx.profile(True)
+    ctx.profile(True, every=every)
     ctx.profile_reset()
     t0 = time.perf_counter()
     for k in range(steps):
@@ -209,8 +216,10 @@ def timed(ctx, step, steps, warmup):
         print(f"timed step {k + 1}/{steps} done", file=sys.stderr, flush=True)
     ctx.synchronize()
     elapsed = time.perf_counter() - t0
-    prof = ctx.profile_read()
+    measured = ctx.profile_read()
+    counts = ctx.profile_counts()
     ctx.profile(False)
+    prof = {k: ((ms / n * counts[k]) if n else 0.0, counts[k]) for k, (ms, n) in measured.items()}
     return elapsed, prof
 
 
@@ -301,7 +310,7 @@ def deshade_256(args):
     if not args.no_energy and storage == native.STORAGE_F32 and dct == native.DCT_REFERENCE:
         every = 100
         ctx.set_energy_interval(every)
-        e_el, e_prof = timed(ctx, step_with(p), 1, 0)
+        e_el, e_prof = timed(ctx, step_with(p), 1, 0, every=1)
         trace = ctx.energy_trace()
         ctx.set_energy_interval(0)
         ms_e, n_e = e_prof["fused_sweep_energy"]
@@ -661,12 +670,17 @@ def adjoint_4x128(args, emit=True):
     ctx = new_context()
     f_dev, g_dev, u_dev, gf_dev = (ctx.alloc(n * 4) for _ in range(4))
     t_fwd = t_adj = 0.0
+    import ctypes
     for b in range(vols.shape[0]):
         ctx.upload(f_dev, vols[b])
         ctx.synchronize()
-        t0 = time.perf_counter()
-        u = ctx.tgv2_denoise(vols[b], p)
+        t0 = time.perf_counter()   # the forward alone, device-resident like the adjoint
+        native._check(native.lib().amb_tgv2_denoise(
+            ctx._h, native.MEM_DEVICE, ctypes.c_void_p(f_dev), *native._dims(shape),
+            ctypes.byref(p), ctypes.c_void_p(u_dev), native.CALLBACK(), None))
+        ctx.synchronize()
         t_fwd += time.perf_counter() - t0
+        u = ctx.download(u_dev, shape)
         ctx.upload(g_dev, ((2.0 / (vols.shape[0] * n)) * (u - cleans[b])).astype(np.float32))
         ctx.synchronize()
         t0 = time.perf_counter()

@@ -411,6 +411,10 @@ AMB_API amb_status amb_tgvk_deshade_downsampled(amb_context* ctx, amb_memory mem
 AMB_API amb_status amb_profile_enable(amb_context* ctx, int enable);
 AMB_API amb_status amb_profile_read(amb_context* ctx, int kernel_class, double* total_ms,
                                     uint64_t* launches);
+/* enable = k >= 2 times one launch in k of each class (the events' own cost
+ * then stays out of k - 1 launches in k); amb_profile_read reports the timed
+ * launches, amb_profile_count every launch of the class since the last reset. */
+AMB_API amb_status amb_profile_count(amb_context* ctx, int kernel_class, uint64_t* launches);
 AMB_API amb_status amb_profile_reset(amb_context* ctx);
 
 #ifdef __cplusplus

@@ -1,0 +1,70 @@
+"""The benchmark configurations' long runs at their full sizes against the
+oracle's own full-size runs (tests/golden/longrun.npz, written by
+tests/golden/gen_longrun.py from the oracle in this repository).
+
+A volume equal to the oracle's bit for bit has the same FP64 sum over every z
+plane, so the committed per-plane sums (plus a few whole planes) check the GPU
+result at every voxel without shipping the volumes:
+
+* TGV2-L1 deshade, 256^3, 1000 iterations (FP32 fields, reference-numerics DCT,
+  clamp): u, v, shading and deshaded (reference tgv_deshade/tgv2_l1_deshade.cu:169-272,
+  TGVDeshadeProcessor.cpp:122-166);
+* TGV2-L2 deshade, 384^3, 800 iterations with BF16 field storage: u, v and the
+  shading against the oracle's BF16 mode (tgv_cuda/tgv2_l2.cu:27-56 with every
+  field store rounded to bfloat16);
+* the unrolled TGV2-L2 denoise of 4 x 128^3 volumes, 200 iterations: the
+  adjoint's dL/dalpha0, dL/dalpha1 and dL/dlambda, summed over the four
+  volumes, against FP64 central differences of the loss.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "longrun.npz")
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def gold():
+    with np.load(GOLD) as z:
+        return {k: z[k] for k in z.files}
+
+
+def _sums(a):
+    return np.sum(np.asarray(a, dtype=np.float32), axis=(1, 2), dtype=np.float64)
+
+
+def _check(name, a, ref_sums):
+    got = _sums(a)
+    bad = np.nonzero(got != ref_sums)[0]
+    assert bad.size == 0, f"{name}: plane sums differ at z = {bad[:10]} ({bad.size} planes)"
+
+
+def test_deshade_256_1000_iterations_bitwise(ctx, gold):
+    from ambient_image_processor_amd import native, phantoms
+    _say("generating the 256^3 workload")
+    w = phantoms.workload("deshade_256")
+    f = w["f"]
+    p = native.default_params(lam=w["lam"], alpha0=w["alpha0"], alpha1=w["alpha1"],
+                              tau=w["tau"], sigma=w["tau"], theta=1.0, iterations=1000,
+                              data_term=native.DATA_L1, flags=native.FLAG_SET_NEGATIVE_ZERO)
+    _say("256^3 TGV2-L1 deshade, 1000 iterations")
+    u, l, r = ctx.deshade_pipeline(f, p)
+    np.testing.assert_array_equal(u[128], gold["c2_u_z128"])
+    np.testing.assert_array_equal(l[128], gold["c2_shading_z128"])
+    _check("u", u, gold["c2_u_sums"])
+    _check("shading", l, gold["c2_shading_sums"])
+    _check("deshaded", r, gold["c2_deshaded_sums"])
+    _, vx, vy, vz = ctx.tgv2_deshade(f, p)
+    for name, a in (("vx", vx), ("vy", vy), ("vz", vz)):
+        _check(name, a, gold[f"c2_{name}_sums"])
+    _say("FP32 oracle vs FP64 at this size: u", float(gold["c2_fp64_u_max"]),
+         "shading", float(gold["c2_fp64_shading_max"]))
