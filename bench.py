@@ -444,6 +444,11 @@ def _tgv2_variant(args, f, mask, flags, workload, metric, data, cpu_note):
     elapsed, prof = timed(ctx, lambda: ctx.deshade_pipeline_device(
         f_dev, shape, p, u_dev, l_dev, r_dev, mask_dev=m_dev), args.steps, args.warmup)
     value = float(n) * args.iterations * args.steps / elapsed
+    in_mask = None
+    if mask is not None and (flags & native.FLAG_MASKED_ITERATION):
+        act = int(np.count_nonzero(mask >= 1))
+        in_mask = {"active_voxels": act, "fraction": act / n,
+                   "voxel_iters_per_s": float(act) * args.iterations * args.steps / elapsed}
     kernels = sweep_kernels(prof, 4, n, planar=bool(flags & native.FLAG_PLANAR))
     dom = max(kernels, key=lambda k: kernels[k]["avg_us"] * kernels[k]["launches"])
     cpu = None
@@ -474,6 +479,7 @@ def _tgv2_variant(args, f, mask, flags, workload, metric, data, cpu_note):
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": kernels[dom]["gbs"] / HBM_PEAK_GBS, "traffic": None,
                      "bytes_per_launch": kernels[dom]["bytes_per_launch"]},
+        "in_mask": in_mask,
         "kernels": rounded(kernels),
         "other_kernels_ms_per_step": {k: round(prof[k][0] / args.steps, 3)
                                       for k in ("dct", "poisson", "divergence", "epilogue", "init")},
@@ -485,11 +491,13 @@ def _tgv2_variant(args, f, mask, flags, workload, metric, data, cpu_note):
 
 def masked_256(args):
     """Masked TGV2-L1 deshade (tgv_deshade_masked/, TGVDeshadeMaskedProcessor) of
-    the 256^3 volume with a spherical mask (mask 1 inside a ball holding about
-    half of the voxels, 0.5 outside): the masked iteration (updates only where
-    mask >= 1), the Poisson solve, the shading times the mask, the clamp and the
-    background added back.  The masked sweeps also read one active byte per
-    voxel (not counted in the algorithmic bytes)."""
+    the 256^3 volume with a spherical mask (mask 1 inside a ball of radius
+    --mask-radius x 256, by default 0.495, about half of the voxels; 0.5
+    outside): the masked iteration (updates only where mask >= 1), the Poisson
+    solve, the shading times the mask, the clamp and the background added back.
+    The masked sweeps also read one active byte per voxel (not counted in the
+    algorithmic bytes); fused-sweep tiles that own no active voxel skip, so the
+    line also gives the rate over the in-mask voxels."""
     from ambient_image_processor_amd import native, phantoms
     n_side = args.size
     shape = (n_side, n_side, n_side)
@@ -497,7 +505,7 @@ def masked_256(args):
                                        bias_sigma=0.05)
     c = (n_side - 1) / 2.0
     z, y, x = np.ogrid[0:n_side, 0:n_side, 0:n_side]
-    ball = ((z - c) ** 2 + (y - c) ** 2 + (x - c) ** 2) < (0.495 * n_side) ** 2
+    ball = ((z - c) ** 2 + (y - c) ** 2 + (x - c) ** 2) < (args.mask_radius * n_side) ** 2
     mask = np.where(ball, 1.0, 0.5).astype(np.float32)
     flags = (native.FLAG_MASKED_ITERATION | native.FLAG_SET_NEGATIVE_ZERO
              | native.FLAG_ADD_BACKGROUND_BACK)
@@ -840,6 +848,8 @@ def main():
     ap.add_argument("--no-split", action="store_true")
     ap.add_argument("--no-host", action="store_true")
     ap.add_argument("--no-energy", action="store_true")
+    ap.add_argument("--mask-radius", type=float, default=0.495,
+                    help="masked_256: ball radius as a fraction of the edge")
     ap.add_argument("--no-configs", action="store_true")
     ap.add_argument("--workload", choices=["deshade_256", "poisson_512", "bf16_384",
                                            "adjoint_4x128", "tgvk3_256", "downsampled_512",
