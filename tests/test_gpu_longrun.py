@@ -68,3 +68,35 @@ def test_deshade_256_1000_iterations_bitwise(ctx, gold):
         _check(name, a, gold[f"c2_{name}_sums"])
     _say("FP32 oracle vs FP64 at this size: u", float(gold["c2_fp64_u_max"]),
          "shading", float(gold["c2_fp64_shading_max"]))
+
+
+def test_adjoint_4x128_scalar_gradients_vs_fp64_differences(ctx, gold):
+    """The differentiable configuration at full size: 4 volumes of 128^3, 200
+    unrolled TGV2-L2 iterations, checkpoints every 20, L = (1/4) sum_b
+    mean((u_b - clean_b)^2).  The adjoint's dL/dalpha1 and dL/dlambda (the
+    four volumes' sum) agree with FP64 central differences of L (eps = 1e-3) to
+    1e-3; dL/dalpha0 (about 1e-6: few q reach their ball) to 1e-2 of itself.
+    The differences at eps = 1e-3 and 1e-5 agree to 1.3e-4 (alpha1), 6e-5
+    (lambda) and 3e-3 (alpha0), so the kinks of the projections do not bias them."""
+    from ambient_image_processor_amd import native, phantoms
+    w = phantoms.workload("denoise_128x4")
+    vols, cleans = w["f"], w["clean"]
+    B, n = vols.shape[0], vols[0].size
+    p = native.default_params(lam=w["lam"], alpha0=w["alpha0"], alpha1=w["alpha1"],
+                              tau=w["tau"], sigma=w["tau"], theta=1.0,
+                              iterations=w["iterations"], data_term=native.DATA_L2)
+    total = np.zeros(3)
+    loss = 0.0
+    for b in range(B):
+        _say("adjoint volume", b)
+        u = ctx.tgv2_denoise(vols[b], p)
+        loss += float(np.mean((u.astype(np.float64) - cleans[b]) ** 2)) / B
+        g = ((2.0 / (B * n)) * (u.astype(np.float64) - cleans[b])).astype(np.float32)
+        _, _, gp = ctx.tgv2_denoise_adjoint(vols[b], p, g, w["checkpoint_every"])
+        total += gp
+    fd = gold["c5_fd"][0]   # eps = 1e-3: alpha0, alpha1, lambda
+    _say("adjoint", total, "fd", fd, "loss", loss, float(gold["c5_loss"]))
+    assert abs(loss - float(gold["c5_loss"])) <= 1e-5 * float(gold["c5_loss"])
+    assert abs(total[1] - fd[1]) <= 1e-3 * abs(fd[1])
+    assert abs(total[2] - fd[2]) <= 1e-3 * abs(fd[2])
+    assert abs(total[0] - fd[0]) <= 1e-2 * abs(fd[0])
