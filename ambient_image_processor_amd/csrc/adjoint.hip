@@ -16,15 +16,19 @@
 //            set); writes the p/q adjoints; FP64 partials of dL/dalpha1, dL/dalpha0.
 //   phase C (stencil): u_bar adjoint = sigma (D+)^T P~, v_bar adjoint =
 //            sigma eps_B^T Q~ - sigma P~.
-// Scalar partials are reduced per block (wave64 shuffles) and summed in a fixed
-// order by a single-block kernel (deterministic).
+// The kernels run one block per row piece (x fastest; row index -> y, z once
+// per block, 32-bit voxel indices, no per-voxel division).  Scalar gradients:
+// each block reduces its voxels' terms (wave64 shuffles) and adds the sum to
+// its own FP64 slot, which accumulates over all steps; one fixed-order sum of
+// the slots at the end (deterministic).
 #include "amb_internal.hpp"
+
+#include <algorithm>
 
 namespace amb {
 namespace {
 
-constexpr int AB = 256;
-constexpr int MAXB = 2048;
+constexpr int AB = 256;   // largest block: one row piece of up to 256 voxels
 
 struct Fields17 {
     float* f[N_FIELDS];
@@ -36,10 +40,13 @@ __device__ __forceinline__ double wsum(double v) {
     return v;
 }
 
-template <int NQ>
-__device__ __forceinline__ void block_partials(double (&acc)[NQ], double* partials) {
+// Block sums of the terms in acc (blockDim.x a multiple of 64) added to this
+// block's slots: slot[blockIdx.x * 3 + OFF + k] += sum (each block owns its
+// slots; the phases run in stream order, so no two blocks touch one slot).
+template <int NQ, int OFF>
+__device__ __forceinline__ void block_accumulate(double (&acc)[NQ], double* slots) {
     __shared__ double red[AB / 64][NQ];
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = int(blockDim.x) >> 6;
 #pragma unroll
     for (int k = 0; k < NQ; ++k) {
         const double s = wsum(acc[k]);
@@ -48,31 +55,37 @@ __device__ __forceinline__ void block_partials(double (&acc)[NQ], double* partia
     __syncthreads();
     if (threadIdx.x < NQ) {
         double s = 0.0;
-        for (int i = 0; i < AB / 64; ++i) s += red[i][threadIdx.x];
-        partials[size_t(blockIdx.x) * 4 + threadIdx.x] = s;
+        for (int i = 0; i < nw; ++i) s += red[i][threadIdx.x];
+        slots[size_t(blockIdx.x) * 3 + OFF + threadIdx.x] += s;
     }
 }
 
-struct Coord {
+// Voxel of this thread: block b covers row r = b / nbx (y = r % H, z = r / H),
+// columns (b % nbx) * blockDim.x + threadIdx.x.
+struct RowVoxel {
     int x, y, z;
+    uint32_t i;
+    bool in;
+    __device__ __forceinline__ RowVoxel(const Geometry& g) {
+        const uint32_t nbx = (uint32_t(g.w) + blockDim.x - 1) / blockDim.x;
+        const uint32_t r = blockIdx.x / nbx;
+        x = int((blockIdx.x - r * nbx) * blockDim.x + threadIdx.x);
+        y = int(r % uint32_t(g.h));
+        z = int(r / uint32_t(g.h));
+        in = x < g.w;
+        i = (uint32_t(z) * uint32_t(g.h) + uint32_t(y)) * uint32_t(g.w) + uint32_t(x);
+    }
 };
-__device__ __forceinline__ Coord coord_of(size_t i, const Geometry& g) {
-    Coord c;
-    c.x = int(i % size_t(g.w));
-    c.y = int((i / size_t(g.w)) % size_t(g.h));
-    c.z = int(i / (size_t(g.w) * g.h));
-    return c;
-}
 
 // (D+)^T b [j] = b[j-1] [j >= 1] - b[j] [j < n-1]
-__device__ __forceinline__ float dplus_t(const float* b, size_t i, size_t st, int c, int n) {
+__device__ __forceinline__ float dplus_t(const float* b, uint32_t i, uint32_t st, int c, int n) {
     float r = 0.0f;
     if (c >= 1) r += b[i - st];
     if (c < n - 1) r -= b[i];
     return r;
 }
 // B^T b [j] = b[j+1] [j < n-1] - b[j]
-__device__ __forceinline__ float bm_t(const float* b, size_t i, size_t st, int c, int n) {
+__device__ __forceinline__ float bm_t(const float* b, uint32_t i, uint32_t st, int c, int n) {
     float r = -b[i];
     if (c < n - 1) r += b[i + st];
     return r;
@@ -84,11 +97,13 @@ __global__ __launch_bounds__(AB) void adjoint_phase_a(Fields17 A, const float* _
                                                       float* __restrict__ wst,
                                                       float* __restrict__ grad_f, Geometry g,
                                                       float tau, float lambda, float theta,
-                                                      double* __restrict__ partials) {
-    const size_t n = g.n();
+                                                      double* __restrict__ slots) {
+    const uint32_t n = uint32_t(g.n());
     const float tl = tau * lambda;
     double acc[1] = {0.0};
-    for (size_t i = size_t(blockIdx.x) * AB + threadIdx.x; i < n; i += size_t(gridDim.x) * AB) {
+    const RowVoxel rv(g);
+    if (rv.in) {
+        const uint32_t i = rv.i;
         const float ub = A.f[F_UBAR][i];
         const float un = A.f[F_U][i] + (1.0f + theta) * ub;
         const float as = un / (1.0f + tl);
@@ -105,21 +120,22 @@ __global__ __launch_bounds__(AB) void adjoint_phase_a(Fields17 A, const float* _
             A.f[F_PX + c][i] += tau * vn;
         }
     }
-    block_partials<1>(acc, partials);
+    block_accumulate<1, 2>(acc, slots);   // slot 2: lambda
 }
 
 __global__ __launch_bounds__(AB) void adjoint_phase_b(Fields17 A, Fields17 S,
                                                       const float* __restrict__ ast,
                                                       const float* __restrict__ wst, Geometry g,
                                                       float tau, float sigma, float alpha0,
-                                                      float alpha1, double* __restrict__ partials) {
-    const size_t n = g.n();
-    const size_t st[3] = {1, size_t(g.w), size_t(g.w) * g.h};
+                                                      float alpha1, double* __restrict__ slots) {
+    const uint32_t n = uint32_t(g.n());
+    const uint32_t st[3] = {1u, uint32_t(g.w), uint32_t(g.w) * uint32_t(g.h)};
     const int ext[3] = {g.w, g.h, g.d};
     double acc[2] = {0.0, 0.0};   // alpha0, alpha1
-    for (size_t i = size_t(blockIdx.x) * AB + threadIdx.x; i < n; i += size_t(gridDim.x) * AB) {
-        const Coord cc = coord_of(i, g);
-        const int c3[3] = {cc.x, cc.y, cc.z};
+    const RowVoxel rv(g);
+    if (rv.in) {
+        const uint32_t i = rv.i;
+        const int c3[3] = {rv.x, rv.y, rv.z};
         // P' = A.p - tau B^T A*   (component a uses B_a)
         float P[3];
 #pragma unroll
@@ -193,16 +209,16 @@ __global__ __launch_bounds__(AB) void adjoint_phase_b(Fields17 A, Fields17 S,
 #pragma unroll
         for (int k = 0; k < 6; ++k) A.f[F_QXX + k][i] = Q[k];
     }
-    block_partials<2>(acc, partials);
+    block_accumulate<2, 0>(acc, slots);   // slots 0, 1: alpha0, alpha1
 }
 
 __global__ __launch_bounds__(AB) void adjoint_phase_c(Fields17 A, Geometry g, float sigma) {
-    const size_t n = g.n();
-    const size_t st[3] = {1, size_t(g.w), size_t(g.w) * g.h};
+    const uint32_t st[3] = {1u, uint32_t(g.w), uint32_t(g.w) * uint32_t(g.h)};
     const int ext[3] = {g.w, g.h, g.d};
-    for (size_t i = size_t(blockIdx.x) * AB + threadIdx.x; i < n; i += size_t(gridDim.x) * AB) {
-        const Coord cc = coord_of(i, g);
-        const int c3[3] = {cc.x, cc.y, cc.z};
+    const RowVoxel rv(g);
+    if (rv.in) {
+        const uint32_t i = rv.i;
+        const int c3[3] = {rv.x, rv.y, rv.z};
         // u_bar adjoint = sigma (D+)^T P~
         float ub = 0.0f;
 #pragma unroll
@@ -227,15 +243,15 @@ __global__ __launch_bounds__(AB) void adjoint_phase_c(Fields17 A, Geometry g, fl
     }
 }
 
-// out[k] += sum over blocks of partials[b][k], fixed order, k < nq
-__global__ void adjoint_accumulate(const double* __restrict__ partials, int nblocks, int nq,
-                                   int offset, double* __restrict__ acc) {
+// out[k] = sum over blocks of slots[b][k], fixed order (one wave per k < 3)
+__global__ void adjoint_reduce(const double* __restrict__ slots, int nblocks,
+                               double* __restrict__ out) {
     const int k = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    if (k >= nq) return;
+    if (k >= 3) return;
     double s = 0.0;
-    for (int b = lane; b < nblocks; b += 64) s += partials[size_t(b) * 4 + k];
+    for (int b = lane; b < nblocks; b += 64) s += slots[size_t(b) * 3 + k];
     s = wsum(s);
-    if (lane == 0) acc[offset + k] += s;
+    if (lane == 0) out[k] = s;
 }
 
 __global__ void adjoint_finish_f(const float* __restrict__ au, const float* __restrict__ aub,
@@ -244,9 +260,16 @@ __global__ void adjoint_finish_f(const float* __restrict__ au, const float* __re
         grad_f[i] += au[i] + aub[i];
 }
 
+// threads per block: the row length rounded up to whole waves, at most AB
+inline int row_threads(const Geometry& g) { return std::min(AB, (g.w + 63) / 64 * 64); }
+inline int row_blocks(const Geometry& g) {
+    const int t = row_threads(g);
+    return ((g.w + t - 1) / t) * g.h * g.d;
+}
+
 inline int blocks_for(size_t n) {
     size_t b = (n + AB - 1) / AB;
-    if (b > MAXB) b = MAXB;
+    if (b > 2048) b = 2048;
     return int(b ? b : 1);
 }
 
@@ -258,27 +281,30 @@ Fields17 to_fields(void* const* p) {
 
 }  // namespace
 
-size_t adjoint_partials_size() { return size_t(MAXB) * 4; }
+size_t adjoint_slots_size(Geometry g) { return size_t(row_blocks(g)) * 3; }
 
-// acc: device double[3] = (alpha0, alpha1, lambda) gradients, accumulated.
+// slots: adjoint_slots_size(g) doubles, zeroed before the first step; each
+// step adds every block's (alpha0, alpha1, lambda) terms to its own slots.
 hipError_t launch_adjoint_step(hipStream_t s, void* const* adj, void* const* xk, void* const* xk1,
                                const float* f, float* ast, float* wst, float* grad_f, Geometry g,
                                float tau, float sigma, float theta, float lambda, float alpha0,
-                               float alpha1, double* partials, double* acc) {
-    const int nb = blocks_for(g.n());
+                               float alpha1, double* slots) {
+    const int nb = row_blocks(g), nt = row_threads(g);
     const Fields17 A = to_fields(adj), S = to_fields(xk), T = to_fields(xk1);
-    hipLaunchKernelGGL(adjoint_phase_a, dim3(nb), dim3(AB), 0, s, A, f, T.f[F_U], ast, wst,
-                       grad_f, g, tau, lambda, theta, partials);
+    hipLaunchKernelGGL(adjoint_phase_a, dim3(nb), dim3(nt), 0, s, A, f, T.f[F_U], ast, wst,
+                       grad_f, g, tau, lambda, theta, slots);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(adjoint_accumulate, dim3(1), dim3(64), 0, s, partials, nb, 1, 2, acc);
-    hipLaunchKernelGGL(adjoint_phase_b, dim3(nb), dim3(AB), 0, s, A, S, ast, wst, g, tau, sigma,
-                       alpha0, alpha1, partials);
+    hipLaunchKernelGGL(adjoint_phase_b, dim3(nb), dim3(nt), 0, s, A, S, ast, wst, g, tau, sigma,
+                       alpha0, alpha1, slots);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
-    // partial slots 0: alpha0, 1: alpha1 -> acc[0], acc[1]
-    hipLaunchKernelGGL(adjoint_accumulate, dim3(1), dim3(128), 0, s, partials, nb, 2, 0, acc);
-    hipLaunchKernelGGL(adjoint_phase_c, dim3(nb), dim3(AB), 0, s, A, g, sigma);
+    hipLaunchKernelGGL(adjoint_phase_c, dim3(nb), dim3(nt), 0, s, A, g, sigma);
+    return hipGetLastError();
+}
+
+hipError_t launch_adjoint_reduce(hipStream_t s, const double* slots, Geometry g, double* out3) {
+    hipLaunchKernelGGL(adjoint_reduce, dim3(1), dim3(192), 0, s, slots, row_blocks(g), out3);
     return hipGetLastError();
 }
 

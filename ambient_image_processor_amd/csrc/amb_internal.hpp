@@ -40,6 +40,8 @@ struct StoreF32 {
     static __device__ __forceinline__ void st(float* p, Idx i, float v) { *elem(p, i) = v; }
     // value after a store/load round trip
     static __device__ __forceinline__ float rt(float v) { return v; }
+    template <int VX>
+    static __device__ __forceinline__ void rt_all(float (&)[VX]) {}
 };
 
 struct StoreBF16 {
@@ -65,6 +67,17 @@ struct StoreBF16 {
     template <class Idx>
     static __device__ __forceinline__ void st(uint16_t* p, Idx i, float v) { *elem(p, i) = rne(v); }
     static __device__ __forceinline__ float rt(float v) { return __uint_as_float(uint32_t(rne(v)) << 16); }
+    // rt of VX values (VX even) in pairs, one packed conversion per pair
+    template <int VX>
+    static __device__ __forceinline__ void rt_all(float (&a)[VX]) {
+        static_assert(VX % 2 == 0, "pairs");
+#pragma unroll
+        for (int e = 0; e < VX; e += 2) {
+            const uint32_t pk = rne2(a[e], a[e + 1]);
+            a[e] = __uint_as_float(pk << 16);
+            a[e + 1] = __uint_as_float(pk & 0xffff0000u);
+        }
+    }
 };
 
 // Block id -> tile id such that each of the 8 XCDs (workgroups are dealt to
@@ -197,6 +210,15 @@ hipError_t launch_fused_sweep(hipStream_t s, int storage, void* const* src, void
                               const FusedPlan& plan);
 constexpr int N_FUSED_FIELDS = 13;
 extern const int FUSED_FIELD_MAP[N_FUSED_FIELDS];
+// The fused sweep out of place with u_bar and v_bar stored as well: src and dst
+// are full 17-field tables (state layout); dst receives u, u_bar, v, v_bar of the
+// primal half and p, q of the next dual half.  FP32 fields, 32-bit offsets, the
+// 3-D loop without a mask, the L2 data term and a two-voxel plan
+// (fused_bars_supported); for the adjoint's segment recompute.
+bool fused_bars_supported(int storage, Geometry g, const SweepParams& p, const FusedPlan& plan);
+hipError_t launch_fused_sweep_bars(hipStream_t s, void* const* src, void* const* dst,
+                                   const float* f, Geometry g, const SweepParams& p,
+                                   const FusedPlan& plan);
 // The 2-D loop (planar, FP32 fields, z components +0 in both copies) as a
 // fused sweep marched along the image's rows; same src / dst13 convention as
 // launch_fused_sweep.
@@ -284,13 +306,16 @@ hipError_t launch_energy(hipStream_t s, int storage, const void* const* fields, 
 hipError_t launch_energy_finish(hipStream_t s, const double* partials, int rows, double* out10);
 
 // Adjoint of the unrolled L2 iteration (adjoint.hip).  adj, xk, xk1: 17-field
-// tables (adjoint of x_{k+1} updated in place to the adjoint of x_k); acc: device
-// double[3] (alpha0, alpha1, lambda) accumulated; partials: adjoint_partials_size().
-size_t adjoint_partials_size();
+// tables (adjoint of x_{k+1} updated in place to the adjoint of x_k); slots:
+// adjoint_slots_size(g) doubles, zeroed before the first step, in which every
+// block accumulates its (alpha0, alpha1, lambda) terms over the steps;
+// launch_adjoint_reduce sums them (fixed order) into out3.
+size_t adjoint_slots_size(Geometry g);
 hipError_t launch_adjoint_step(hipStream_t s, void* const* adj, void* const* xk, void* const* xk1,
                                const float* f, float* ast, float* wst, float* grad_f, Geometry g,
                                float tau, float sigma, float theta, float lambda, float alpha0,
-                               float alpha1, double* partials, double* acc);
+                               float alpha1, double* slots);
+hipError_t launch_adjoint_reduce(hipStream_t s, const double* slots, Geometry g, double* out3);
 hipError_t launch_adjoint_finish(hipStream_t s, void* const* adj, float* grad_f, Geometry g);
 
 // Host-side cos(pi t) from the mathematics (exact reduction modulo 2, octant

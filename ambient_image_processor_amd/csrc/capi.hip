@@ -110,6 +110,9 @@ struct amb_context {
     std::vector<std::pair<hipEvent_t, hipEvent_t>> pending[N_PROF];
     double prof_ms[N_PROF] = {};
     uint64_t prof_n[N_PROF] = {};
+    // tiles of the 3-D fused sweep (class 9) launched while profiling: swept and in
+    // the grid (a masked run skips the tiles that own no active voxel)
+    uint64_t prof_tiles_swept = 0, prof_tiles_total = 0;
     // host staging for callbacks
     std::vector<float> cb_u, cb_v[3];
     // adjoint workspace (grow-only): checkpoints, recompute slots, adjoint state,
@@ -122,6 +125,7 @@ struct amb_context {
     size_t act_cap = 0;
     uint8_t* tact = nullptr;
     size_t tact_cap = 0;
+    size_t tact_active = 0;   // active tiles of the masked run's plan
     // fused sweep: second copy of u, v, p, q (grow-only) and the launch plan
     char* alt_block = nullptr;
     size_t alt_cap = 0;
@@ -562,9 +566,15 @@ amb_status fused_step(amb_context* c, double* energy_out = nullptr) {
         ProfScope ps(c, P_FUSED);
         if (c->sp.planar_zero_z && c->iplan_ok)
             AMB_HIP(launch_fused_image(c->stream, c->fields, c->alt, c->f, c->g, c->sp, c->iplan));
-        else
+        else {
             AMB_HIP(launch_fused_sweep(c->stream, c->storage, c->fields, c->alt, c->f, c->g, c->sp,
                                        c->fplan_run));
+            if (c->prof_on) {
+                const size_t nt = fused_grid_blocks(c->g, c->fplan_run);
+                c->prof_tiles_total += nt;
+                c->prof_tiles_swept += c->sp.tile_active ? c->tact_active : nt;
+            }
+        }
     }
     for (int k = 0; k < N_FUSED_FIELDS; ++k) std::swap(c->fields[FUSED_FIELD_MAP[k]], c->alt[k]);
     return AMB_OK;
@@ -707,24 +717,27 @@ amb_status run(amb_context* c, uint32_t iterations, uint32_t interval, amb_itera
             off[k] = k ? off[k - 1] + align_up(fused_grid_blocks(c->g, cand[k - 1]), 256) : 0;
             AMB_HIP(launch_tile_activity(c->stream, c->sp.active, c->g, cand[k], c->tact + off[k]));
         }
+        std::vector<uint8_t> h(need);
+        AMB_HIP(hipMemcpyAsync(h.data(), c->tact, need, hipMemcpyDeviceToHost, c->stream));
+        AMB_HIP(hipStreamSynchronize(c->stream));
+        size_t active[3] = {0, 0, 0};
+        for (int k = 0; k < ncand; ++k) {
+            const size_t nt = fused_grid_blocks(c->g, cand[k]);
+            for (size_t t = 0; t < nt; ++t) active[k] += h[off[k] + t] != 0;
+        }
         int best = 0;
         int cus = 0;
         if (ncand > 1 &&
             hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device) ==
                 hipSuccess && cus > 0) {
-            std::vector<uint8_t> h(need);
-            AMB_HIP(hipMemcpyAsync(h.data(), c->tact, need, hipMemcpyDeviceToHost, c->stream));
-            AMB_HIP(hipStreamSynchronize(c->stream));
             double best_cost = 0.0;
             for (int k = 0; k < ncand; ++k) {
-                const size_t nt = fused_grid_blocks(c->g, cand[k]);
-                size_t a = 0;
-                for (size_t t = 0; t < nt; ++t) a += h[off[k] + t] != 0;
-                const double rounds = double((a + size_t(cus) - 1) / size_t(cus));
+                const double rounds = double((active[k] + size_t(cus) - 1) / size_t(cus));
                 const double cost = rounds * double(cand[k].zchunk + 5);
                 if (k == 0 || cost < best_cost) { best = k; best_cost = cost; }
             }
         }
+        c->tact_active = active[best];
         c->fplan_run = cand[best];
         tflags = c->tact + off[best];
     }
@@ -1942,7 +1955,7 @@ AMB_API amb_status amb_tgv2_denoise_adjoint(amb_context* c, amb_memory mem, cons
     const size_t fstride = align_up(n * 4, 256);
     const size_t state_bytes = fstride * N_FIELDS;
     const uint32_t nseg = N ? (N + K - 1) / K : 0;
-    const size_t part_bytes = align_up(adjoint_partials_size() * sizeof(double), 256);
+    const size_t part_bytes = align_up(adjoint_slots_size(g) * sizeof(double), 256);
     const size_t need = state_bytes * (size_t(nseg) + (K + 1) + 1) + fstride * 5 + part_bytes + 256;
     if (need > c->adj_cap) {
         if (c->adj_block) AMB_HIP(hipFree(c->adj_block));
@@ -1963,16 +1976,24 @@ AMB_API amb_status amb_tgv2_denoise_adjoint(amb_context* c, amb_memory mem, cons
     float* gradf = reinterpret_cast<float*>(adjs + state_bytes + fstride * 4);
     double* partials = reinterpret_cast<double*>(adjs + state_bytes + fstride * 5);
     double* acc = reinterpret_cast<double*>(adjs + state_bytes + fstride * 5 + part_bytes);
-    char* work = static_cast<char*>(c->fields[0]);   // the working state block
-
     ProfScope ps(c, P_ADJOINT);
-    // forward with a checkpoint of x_k at every k % K == 0
-    for (uint32_t it = 0; it < N; ++it) {
-        if (it % K == 0)
-            AMB_HIP(hipMemcpyAsync(ckpt + state_bytes * (it / K), work, state_bytes,
+    // forward with a checkpoint of x_k at every k % K == 0: each segment runs the
+    // iteration loop (fused sweeps inside, ending with the two-sweep state); the
+    // fused sweep ping-pongs fields between two blocks, so the checkpoint copies
+    // field by field
+    const uint32_t every = c->energy_interval;
+    c->energy_interval = 0;
+    struct RestoreInterval {
+        amb_context* c;
+        uint32_t v;
+        ~RestoreInterval() { c->energy_interval = v; }
+    } restore{c, every};
+    for (uint32_t seg = 0; seg < nseg; ++seg) {
+        char* cp = ckpt + state_bytes * seg;
+        for (int k = 0; k < N_FIELDS; ++k)
+            AMB_HIP(hipMemcpyAsync(cp + size_t(k) * fstride, c->fields[k], n * 4,
                                    hipMemcpyDeviceToDevice, c->stream));
-        AMB_TRY(sweep(c, 0));
-        AMB_TRY(sweep(c, 1));
+        AMB_TRY(run(c, std::min(K, N - seg * K), 0, nullptr, nullptr, false));
     }
     if (u_out) AMB_TRY(copy_field_out(c, mem, u_out, F_U));
     // reverse: adjoint of x_N = (dL/du, 0, ...)
@@ -1981,13 +2002,43 @@ AMB_API amb_status amb_tgv2_denoise_adjoint(amb_context* c, amb_memory mem, cons
     AMB_HIP(launch_zero_fill(c->stream, adjs, state_bytes));
     AMB_HIP(hipMemcpyAsync(adj[F_U], dL_du, n * 4, kind_in(mem), c->stream));
     AMB_HIP(launch_zero_fill(c->stream, gradf, n * 4));
-    AMB_HIP(launch_zero_fill(c->stream, acc, 3 * sizeof(double)));
+    AMB_HIP(launch_zero_fill(c->stream, partials, adjoint_slots_size(g) * sizeof(double)));
+    // recompute of a segment: with the fused sweep's form that also stores u_bar
+    // and v_bar, out of place, one launch per iteration; else in place with the
+    // two sweeps after a copy of the state
+    const bool fb = c->fused_ok && fused_bars_supported(c->storage, g, c->sp, c->fplan);
     for (int seg = int(nseg) - 1; seg >= 0; --seg) {
         const uint32_t k0 = uint32_t(seg) * K, k1 = std::min(k0 + K, N), L = k1 - k0;
+        char* cp = ckpt + state_bytes * size_t(seg);
         void* tabs[2][N_FIELDS];
+        AMB_HIP(hipMemcpyAsync(slots, cp, state_bytes, hipMemcpyDeviceToDevice, c->stream));
+        if (fb) {
+            // slots[j] = u, u_bar, v, v_bar of x_{k0 + j} with p, q of the dual half
+            // of iteration k0 + j + 1: slot 0 from x_{k0} by one dual sweep, slot j
+            // from slot j - 1 by one fused launch (primal half k0 + j, dual half
+            // k0 + j + 1)
+            state_at(slots, tabs[0]);
+            AMB_HIP(launch_dual_sweep(c->stream, 0, tabs[0], g, c->sp));
+            for (uint32_t j = 1; j <= L; ++j) {
+                state_at(slots + state_bytes * size_t(j - 1), tabs[0]);
+                state_at(slots + state_bytes * size_t(j), tabs[1]);
+                AMB_HIP(launch_fused_sweep_bars(c->stream, tabs[0], tabs[1], c->f, g, c->sp,
+                                                c->fplan));
+            }
+            // x_{k0 + j}: u_bar, v_bar (and u, v) from slot j, p and q from slot
+            // j - 1 (the checkpoint for j = 0); u of x_{k0 + j + 1} from slot j + 1
+            for (int j = int(L) - 1; j >= 0; --j) {
+                state_at(slots + state_bytes * size_t(j), tabs[0]);
+                char* pq = j > 0 ? slots + state_bytes * size_t(j - 1) : cp;
+                for (int k = F_PX; k <= F_QYZ; ++k) tabs[0][k] = pq + size_t(k) * fstride;
+                state_at(slots + state_bytes * size_t(j + 1), tabs[1]);
+                AMB_HIP(launch_adjoint_step(c->stream, adj, tabs[0], tabs[1], c->f, ast, wst, gradf,
+                                            g, c->sp.tau, c->sp.sigma, c->sp.theta, c->sp.lambda,
+                                            c->sp.alpha0, c->sp.alpha1, partials));
+            }
+            continue;
+        }
         // slots[0] = x_{k0}; slots[j] = x_{k0 + j} recomputed with the same sweeps
-        AMB_HIP(hipMemcpyAsync(slots, ckpt + state_bytes * size_t(seg), state_bytes,
-                               hipMemcpyDeviceToDevice, c->stream));
         for (uint32_t j = 1; j <= L; ++j) {
             char* dst = slots + state_bytes * j;
             AMB_HIP(hipMemcpyAsync(dst, dst - state_bytes, state_bytes, hipMemcpyDeviceToDevice,
@@ -2001,9 +2052,10 @@ AMB_API amb_status amb_tgv2_denoise_adjoint(amb_context* c, amb_memory mem, cons
             state_at(slots + state_bytes * size_t(j + 1), tabs[1]);
             AMB_HIP(launch_adjoint_step(c->stream, adj, tabs[0], tabs[1], c->f, ast, wst, gradf, g,
                                         c->sp.tau, c->sp.sigma, c->sp.theta, c->sp.lambda,
-                                        c->sp.alpha0, c->sp.alpha1, partials, acc));
+                                        c->sp.alpha0, c->sp.alpha1, partials));
         }
     }
+    AMB_HIP(launch_adjoint_reduce(c->stream, partials, g, acc));
     AMB_HIP(launch_adjoint_finish(c->stream, adj, gradf, g));
     AMB_HIP(hipMemcpyAsync(dL_df, gradf, n * 4, kind_out(mem), c->stream));
     AMB_HIP(hipMemcpyAsync(dL_dparams, acc, 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -2042,11 +2094,22 @@ AMB_API amb_status amb_profile_count(amb_context* c, int cls, uint64_t* launches
     return guard_exception();
 }
 
+AMB_API amb_status amb_profile_fused_tiles(amb_context* c, uint64_t* swept, uint64_t* total) try {
+    if (!c || !swept || !total) return fail(AMB_ERR_INVALID_ARGUMENT, "bad argument");
+    *swept = c->prof_tiles_swept;
+    *total = c->prof_tiles_total;
+    return AMB_OK;
+} catch (...) {
+    return guard_exception();
+}
+
 AMB_API amb_status amb_profile_reset(amb_context* c) try {
     if (!c) return fail(AMB_ERR_INVALID_ARGUMENT, "ctx is NULL");
     AMB_TRY(set_device(c));
     AMB_TRY(drain_profile(c));
     for (int k = 0; k < N_PROF; ++k) { c->prof_ms[k] = 0.0; c->prof_n[k] = 0; c->prof_seen[k] = 0; }
+    c->prof_tiles_swept = 0;
+    c->prof_tiles_total = 0;
     return AMB_OK;
 } catch (...) {
     return guard_exception();

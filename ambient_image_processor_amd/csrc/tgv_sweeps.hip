@@ -878,6 +878,7 @@ enum FusedField { G_U = 0, G_VX, G_VY, G_VZ, G_PX, G_PY, G_PZ, G_QXX, G_QYY, G_Q
 struct FusedFields {
     const void* src[N_GFIELDS];
     void* dst[N_GFIELDS];
+    void* bars[4];   // SB = 1: u_bar, v_bar x, y, z of the primal half (else unused)
 };
 
 constexpr int FUSED_MAX_THREADS = 768;
@@ -932,13 +933,19 @@ struct FusedPlane {
 // brackets one plane behind, with the dual half (u at z + 1 is then in
 // registers; u and v of the plane in 4 more LDS rows).  One row of ten sums
 // per block goes to epart[blockIdx.x * 10 ..] (slots as the standalone pass).
-template <class S, class I, int VX, int L2, int MK, int MAXT, int PF, int TL, int P2, int EN = 0>
+// SB = 1 (3-D loop, P2 = 0): u_bar and v_bar of the primal half, which the
+// sweep otherwise keeps in LDS only, are also stored (to ff.bars), so that dst
+// then holds the whole state after the primal half with p and q one dual half
+// ahead (the adjoint's recompute of a segment, one launch per iteration).
+template <class S, class I, int VX, int L2, int MK, int MAXT, int PF, int TL, int P2, int EN = 0,
+          int SB = 0>
 __global__ __launch_bounds__(MAXT) void fused_sweep_kernel(
     FusedFields ff, const float* __restrict__ f_, const uint8_t* __restrict__ act, Geometry g,
     float tau, float sigma, float lambda, float alpha0, float alpha1, float theta, int ty,
     int zchunk, int planar, int tx, double* __restrict__ epart,
     const uint8_t* __restrict__ tile_active) {
     static_assert(!EN || P2 == 0, "the in-sweep energy covers the 3-D loop");
+    static_assert(!SB || P2 == 0, "u_bar / v_bar stores cover the 3-D loop");
     typedef typename S::T T;
     typedef VecIO<S, VX> V;
     typedef VecIO<StoreF32, VX> VF;
@@ -1123,8 +1130,9 @@ __global__ __launch_bounds__(MAXT) void fused_sweep_kernel(
                 }
                 if (MK && !((actv >> (8 * e)) & 0xffu)) un = uc;
                 uo[e] = un;
-                ubn[e] = S::rt(fmaf(theta, un - uc, un));
+                ubn[e] = fmaf(theta, un - uc, un);
             }
+            S::template rt_all<VX>(ubn);   // u_bar as stored (rounded in pairs for BF16)
         }
         if (need_v) {
             float qxyn_y[VX], qyyn_y[VX], qyzn_y[VX];
@@ -1161,10 +1169,12 @@ __global__ __launch_bounds__(MAXT) void fused_sweep_kernel(
                 float vn2 = fmaf(-tau, w2 - pz[e], v[2][e]);
                 if (MK && !((actv >> (8 * e)) & 0xffu)) { vn0 = v[0][e]; vn1 = v[1][e]; vn2 = v[2][e]; }
                 vo[0][e] = vn0; vo[1][e] = vn1; vo[2][e] = vn2;
-                vbn[0][e] = S::rt(fmaf(theta, vn0 - v[0][e], vn0));
-                vbn[1][e] = S::rt(fmaf(theta, vn1 - v[1][e], vn1));
-                vbn[2][e] = S::rt(fmaf(theta, vn2 - v[2][e], vn2));
+                vbn[0][e] = fmaf(theta, vn0 - v[0][e], vn0);
+                vbn[1][e] = fmaf(theta, vn1 - v[1][e], vn1);
+                vbn[2][e] = fmaf(theta, vn2 - v[2][e], vn2);
             }
+#pragma unroll
+            for (int k = 0; k < 3; ++k) S::template rt_all<VX>(vbn[k]);
         }
         if constexpr (EN) {
             // primal-half terms at plane s: data term with s = -(D+)^T p (true
@@ -1247,6 +1257,12 @@ __global__ __launch_bounds__(MAXT) void fused_sweep_kernel(
             V::st(dst[G_VX], i, vo[0]);
             if (!IMG) V::st(dst[G_VY], i, vo[1]);
             if (!SKZ) V::st(dst[G_VZ], i, vo[2]);
+            if constexpr (SB) {
+                T* const* bars = reinterpret_cast<T* const*>(ff.bars);
+                V::st(bars[0], i, ubn);
+#pragma unroll
+                for (int k = 0; k < 3; ++k) V::st(bars[1 + k], i, vbn[k]);
+            }
         }
 
         // ---- dual half at plane zd = s - 1 (D region holds u_bar, v_bar at zd)
@@ -1609,7 +1625,7 @@ hipError_t primal_tiles(hipStream_t s, void* const* fv, const float* f, Geometry
 }
 
 template <class S, class I, int VX, int L2, int MK, int MAXT, int PF, int TL = 0, int P2 = 0,
-          int EN = 0>
+          int EN = 0, int SB = 0>
 hipError_t fused_launch(hipStream_t s, const FusedFields& ff, const float* f, Geometry g,
                         const SweepParams& p, const FusedPlan& plan) {
     const int row = TL ? plan.tx + 2 * VX : g.w;
@@ -1619,7 +1635,7 @@ hipError_t fused_launch(hipStream_t s, const FusedFields& ff, const float* f, Ge
     const int nb = (g.h + plan.ty - 1) / plan.ty, nc = (g.d + plan.zchunk - 1) / plan.zchunk;
     size_t lds = (size_t(EN ? 12 : 8) * RT * row + size_t(EN ? 12 : 8) * RT * (L + 1)) * sizeof(float);
     if (EN) lds = std::max(lds, size_t(16 * 10 * sizeof(double)));   // the block reduction
-    auto kern = fused_sweep_kernel<S, I, VX, L2, MK, MAXT, PF, TL, P2, EN>;
+    auto kern = fused_sweep_kernel<S, I, VX, L2, MK, MAXT, PF, TL, P2, EN, SB>;
     hipError_t e = ensure_dynamic_lds(reinterpret_cast<const void*>(kern), lds);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(kern, dim3(nbx * nb * nc), dim3(L * RT), lds, s, ff, f, p.active, g, p.tau,
@@ -1724,6 +1740,23 @@ hipError_t fused_dispatch(hipStream_t s, const FusedFields& ff, const float* f, 
                           const SweepParams& p, const FusedPlan& plan) {
     return p.active ? fused_pf<S, I, 1>(s, ff, f, g, p, plan)
                     : fused_pf<S, I, 0>(s, ff, f, g, p, plan);
+}
+
+// The SB form (u_bar / v_bar stored too) for the adjoint's recompute: FP32
+// fields, 32-bit offsets, the 3-D loop, no mask, the L2 data term, two voxels
+// per thread; whole rows or x tiles and the prefetch as the plan says.
+hipError_t fused_bars_dispatch(hipStream_t s, const FusedFields& ff, const float* f, Geometry g,
+                               const SweepParams& p, const FusedPlan& plan) {
+    typedef StoreF32 S;
+    typedef uint32_t I;
+    if (plan.vx != 2 || p.data_term != 1) return hipErrorInvalidValue;
+    const int lanes = plan.tx > 0 ? plan.tx / 2 + 2 : g.w / 2;
+    const bool pf = plan.pf && lanes * (plan.ty + 2) <= 768;
+    if (plan.tx > 0)
+        return pf ? fused_launch<S, I, 2, 1, 0, 768, 1, 1, 0, 0, 1>(s, ff, f, g, p, plan)
+                  : fused_launch<S, I, 2, 1, 0, 1024, 0, 1, 0, 0, 1>(s, ff, f, g, p, plan);
+    return pf ? fused_launch<S, I, 2, 1, 0, 768, 1, 0, 0, 0, 1>(s, ff, f, g, p, plan)
+              : fused_launch<S, I, 2, 1, 0, 1024, 0, 0, 0, 0, 1>(s, ff, f, g, p, plan);
 }
 
 }  // namespace
@@ -1961,7 +1994,7 @@ hipError_t launch_fused_sweep(hipStream_t s, int storage, void* const* src, void
                               const float* f, Geometry g, const SweepParams& p,
                               const FusedPlan& plan) {
     static_assert(int(N_GFIELDS) == N_FUSED_FIELDS, "fused field count");
-    FusedFields ff;
+    FusedFields ff{};
     for (int k = 0; k < N_GFIELDS; ++k) {
         ff.src[k] = src[FUSED_FIELD_MAP[k]];
         ff.dst[k] = dst13[k];
@@ -1977,6 +2010,25 @@ hipError_t launch_fused_sweep(hipStream_t s, int storage, void* const* src, void
                      : fused_dispatch<StoreBF16, size_t>(s, ff, f, g, p, plan);
     return small ? fused_dispatch<StoreF32, uint32_t>(s, ff, f, g, p, plan)
                  : fused_dispatch<StoreF32, size_t>(s, ff, f, g, p, plan);
+}
+
+bool fused_bars_supported(int storage, Geometry g, const SweepParams& p, const FusedPlan& plan) {
+    return storage == 0 && !p.index64 && !p.active && !p.planar && !p.planar_zero_z &&
+           p.data_term == 1 && plan.vx == 2 && g.d > 1;
+}
+
+hipError_t launch_fused_sweep_bars(hipStream_t s, void* const* src, void* const* dst,
+                                   const float* f, Geometry g, const SweepParams& p,
+                                   const FusedPlan& plan) {
+    if (!fused_bars_supported(0, g, p, plan) || p.energy_partials) return hipErrorInvalidValue;
+    FusedFields ff{};
+    for (int k = 0; k < N_GFIELDS; ++k) {
+        ff.src[k] = src[FUSED_FIELD_MAP[k]];
+        ff.dst[k] = dst[FUSED_FIELD_MAP[k]];
+    }
+    ff.bars[0] = dst[F_UBAR];
+    for (int k = 0; k < 3; ++k) ff.bars[1 + k] = dst[F_VBX + k];
+    return fused_bars_dispatch(s, ff, f, g, p, plan);
 }
 
 // The 2-D loop marched along the image's rows (fused_sweep_kernel with P2 = 2):
@@ -2014,7 +2066,7 @@ hipError_t launch_fused_image(hipStream_t s, void* const* src, void* const* dst1
     // fused order u, vx, vy, vz, px, py, pz, qxx, qyy, qzz, qxy, qxz, qyz with
     // y and z exchanged: the kernel's y slots get the image's (zero) z fields
     static const int swap[N_GFIELDS] = {0, 1, 3, 2, 4, 6, 5, 7, 9, 8, 11, 10, 12};
-    FusedFields ff;
+    FusedFields ff{};
     for (int k = 0; k < N_GFIELDS; ++k) {
         ff.src[k] = src[FUSED_FIELD_MAP[swap[k]]];
         ff.dst[k] = dst13[swap[k]];

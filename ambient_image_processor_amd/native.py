@@ -46,7 +46,7 @@ EXPORTS = [
     "amb_tgvk_deshade_pipeline", "amb_tgvk_deshade_masked", "amb_resize_bspline3",
     "amb_resize_extents", "amb_tgvk_deshade_downsampled", "amb_tgv2_set_energy_interval",
     "amb_tgv2_energy_trace", "amb_context_set_plan_override", "amb_context_clear_plan_overrides",
-    "amb_tgv2_energy_noalloc", "amb_profile_count",
+    "amb_tgv2_energy_noalloc", "amb_profile_count", "amb_profile_fused_tiles",
 ]
 
 
@@ -128,6 +128,7 @@ def lib():
         "amb_profile_read": [vp, c_int, POINTER(c_double), POINTER(c_uint64)],
         "amb_profile_reset": [vp],
         "amb_profile_count": [vp, c_int, POINTER(c_uint64)],
+        "amb_profile_fused_tiles": [vp, POINTER(c_uint64), POINTER(c_uint64)],
         "amb_tgv2_denoise_adjoint": [vp, mem, fp, u32, u32, u32, POINTER(Params), fp, fp, fp,
                                      POINTER(c_double), u32],
     }
@@ -542,3 +543,9 @@ class Context:
             _check(lib().amb_profile_count(self._h, k, ctypes.byref(n)))
             res[name] = n.value
         return res
+
+    def profile_fused_tiles(self):
+        """(tiles swept, tiles in the grids) of the 3-D fused sweep since the last reset"""
+        a, b = c_uint64(), c_uint64()
+        _check(lib().amb_profile_fused_tiles(self._h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value

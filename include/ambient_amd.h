@@ -415,6 +415,12 @@ AMB_API amb_status amb_profile_read(amb_context* ctx, int kernel_class, double* 
  * then stays out of k - 1 launches in k); amb_profile_read reports the timed
  * launches, amb_profile_count every launch of the class since the last reset. */
 AMB_API amb_status amb_profile_count(amb_context* ctx, int kernel_class, uint64_t* launches);
+/* Tiles of the 3-D fused sweep (class 9) launched while profiling since the
+ * last reset: *swept those that ran, *total those in the launches' grids.  A
+ * masked run skips, from its second fused launch on, the tiles that own no
+ * active voxel; otherwise swept == total.  For pricing a masked launch's bytes
+ * (the reference counts launches only; this has no counterpart there). */
+AMB_API amb_status amb_profile_fused_tiles(amb_context* ctx, uint64_t* swept, uint64_t* total);
 AMB_API amb_status amb_profile_reset(amb_context* ctx);
 
 #ifdef __cplusplus

@@ -268,7 +268,17 @@ def test_masked_fused_tiles_without_active_voxels_skip(ctx, orc, zchunk, storage
 
     if zchunk:
         ctx.set_override("fused_zchunk", zchunk)
+    ctx.profile(True)
+    ctx.profile_reset()
     out = ctx.tgv2_deshade_masked(f, p, ball >= 1, cb if cb_every else None)
+    launches = ctx.profile_counts()["fused_sweep"]
+    swept, total = ctx.profile_fused_tiles()
+    ctx.profile(False)
+    # the first fused launch of the call sweeps every tile, the others the active ones
+    assert launches >= 2 and total % launches == 0
+    per = total // launches
+    assert (swept - per) % (launches - 1) == 0
+    assert 0 < (swept - per) // (launches - 1) < per
     seen_fused = list(seen)
     seen.clear()
     ctx.clear_overrides()
@@ -283,3 +293,17 @@ def test_masked_fused_tiles_without_active_voxels_skip(ctx, orc, zchunk, storage
         st = orc.tgv2(f, 26, data_term="l1", dtype=np.float32, mask=ball, **PRM)
         np.testing.assert_array_equal(out[0], st.u)
         np.testing.assert_array_equal(np.stack(out[1:]), st.v)
+
+
+def test_unmasked_fused_sweep_counts_every_tile(ctx):
+    """Without a mask every fused launch sweeps its whole grid."""
+    from ambient_image_processor_amd import native
+    f = (0.4 + 0.2 * np.random.default_rng(5).standard_normal((24, 20, 40))).astype(np.float32)
+    ctx.profile(True)
+    ctx.profile_reset()
+    ctx.tgv2_deshade(f, _params(native, 9))
+    launches = ctx.profile_counts()["fused_sweep"]
+    swept, total = ctx.profile_fused_tiles()
+    ctx.profile(False)
+    assert launches >= 2 and total > 0 and total % launches == 0
+    assert swept == total
