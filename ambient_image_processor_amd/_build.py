@@ -59,6 +59,7 @@ def build(verbose: bool = False) -> str:
     os.replace(tmp, LIB_PATH)
     _build_compat_demo()
     _build_cli()
+    _build_div_rn_check()
     if verbose:
         print("built", LIB_PATH)
     return LIB_PATH
@@ -78,6 +79,20 @@ def _build_compat_demo() -> str:
     if r.returncode != 0:
         raise RuntimeError("compat demo build failed:\n" + r.stdout + r.stderr)
     return COMPAT_DEMO
+
+
+DIV_RN_CHECK = os.path.join(BUILD_DIR, "div_rn_check")
+
+
+def _build_div_rn_check() -> str:
+    """The device check of the fused sweep's correctly rounded division
+    (tests/compat/div_rn_check.hip, same flags as the library)."""
+    src = os.path.join(ROOT, "tests", "compat", "div_rn_check.hip")
+    cmd = [HIPCC] + COMMON_FLAGS + [src, "-o", DIV_RN_CHECK]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("division check build failed:\n" + r.stdout + r.stderr)
+    return DIV_RN_CHECK
 
 
 CLI = os.path.join(BUILD_DIR, "tgv2_deshade_application")

@@ -5,17 +5,22 @@
 // operators as implemented (D+, the reference's B, eps_B, div2).
 //
 // One reverse iteration k, given x_k (S), x_{k+1} (T) and the adjoint A of
-// x_{k+1} (17 fields, the state layout), in three launches:
+// x_{k+1} (17 fields, the state layout), in three phases:
 //   phase A (pointwise): u/v over-relaxation and L2 prox adjoints; writes the new
 //            u and v adjoints in place, A* = U'/(1 + tau lambda) and
-//            W* = -tau V' into scratch, adds tau V' into the p adjoint;
-//            FP64 partial of dL/dlambda = sum U' tau (f - u') / (1 + tau lambda).
-//   phase B (stencil): P' = A.p - tau B^T A*, Q' = A.q + div2^T W*; projection
+//            W* = -tau V' into scratch, and the p adjoint plus tau V' into the
+//            p buffer (3 fields); FP64 partial of dL/dlambda =
+//            sum U' tau (f - u') / (1 + tau lambda).
+//   phase B (stencil): P' = (p buffer) - tau B^T A*, Q' = A.q + div2^T W*; projection
 //            Jacobians at p~ = p + sigma (D+ u_bar - v_bar), q~ = q + sigma eps_B(v_bar)
 //            recomputed from S with the forward's FP32 arithmetic (same active
 //            set); writes the p/q adjoints; FP64 partials of dL/dalpha1, dL/dalpha0.
 //   phase C (stencil): u_bar adjoint = sigma (D+)^T P~, v_bar adjoint =
 //            sigma eps_B^T Q~ - sigma P~.
+// Phase A of iteration k - 1 reads only its own voxel's u_bar / v_bar adjoints,
+// so it runs in the launch of phase C of iteration k (adjoint_phase_ca) on the
+// values in registers: those adjoints are stored only by the last phase C.
+// C reads the p adjoint at neighbouring voxels, hence A's separate p buffer.
 // The kernels run one block per row piece (x fastest; row index -> y, z once
 // per block, 32-bit voxel indices, no per-voxel division).  Scalar gradients:
 // each block reduces its voxels' terms (wave64 shuffles) and adds the sum to
@@ -91,39 +96,52 @@ __device__ __forceinline__ float bm_t(const float* b, uint32_t i, uint32_t st, i
     return r;
 }
 
+// Phase A at voxel i from the u_bar and v_bar adjoints ub, vb: the p adjoint
+// plus tau V' goes to pout (3 fields, stride n); the lambda term to acc.
+__device__ __forceinline__ void phase_a_voxel(const Fields17& A, uint32_t i, uint32_t n, float ub,
+                                              const float (&vb)[3], const float* __restrict__ f,
+                                              const float* __restrict__ u_next,
+                                              float* __restrict__ ast, float* __restrict__ wst,
+                                              float* __restrict__ pout,
+                                              float* __restrict__ grad_f, float tau, float tl,
+                                              float theta, double& acc) {
+    const float un = A.f[F_U][i] + (1.0f + theta) * ub;
+    const float as = un / (1.0f + tl);
+    A.f[F_U][i] = -theta * ub + as;
+    ast[i] = as;
+    grad_f[i] += tl / (1.0f + tl) * un;
+    acc += double(un) * double(tau) * (double(f[i]) - double(u_next[i])) / (1.0 + double(tl));
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const float vn = A.f[F_VX + c][i] + (1.0f + theta) * vb[c];
+        A.f[F_VX + c][i] = -theta * vb[c] + vn;
+        wst[size_t(c) * n + i] = -tau * vn;
+        pout[size_t(c) * n + i] = A.f[F_PX + c][i] + tau * vn;
+    }
+}
+
 __global__ __launch_bounds__(AB) void adjoint_phase_a(Fields17 A, const float* __restrict__ f,
                                                       const float* __restrict__ u_next,
                                                       float* __restrict__ ast,
                                                       float* __restrict__ wst,
+                                                      float* __restrict__ pout,
                                                       float* __restrict__ grad_f, Geometry g,
                                                       float tau, float lambda, float theta,
                                                       double* __restrict__ slots) {
     const uint32_t n = uint32_t(g.n());
-    const float tl = tau * lambda;
     double acc[1] = {0.0};
     const RowVoxel rv(g);
     if (rv.in) {
         const uint32_t i = rv.i;
-        const float ub = A.f[F_UBAR][i];
-        const float un = A.f[F_U][i] + (1.0f + theta) * ub;
-        const float as = un / (1.0f + tl);
-        A.f[F_U][i] = -theta * ub + as;
-        ast[i] = as;
-        grad_f[i] += tl / (1.0f + tl) * un;
-        acc[0] += double(un) * double(tau) * (double(f[i]) - double(u_next[i])) / (1.0 + double(tl));
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            const float vb = A.f[F_VBX + c][i];
-            const float vn = A.f[F_VX + c][i] + (1.0f + theta) * vb;
-            A.f[F_VX + c][i] = -theta * vb + vn;
-            wst[size_t(c) * n + i] = -tau * vn;
-            A.f[F_PX + c][i] += tau * vn;
-        }
+        const float vb[3] = {A.f[F_VBX][i], A.f[F_VBY][i], A.f[F_VBZ][i]};
+        phase_a_voxel(A, i, n, A.f[F_UBAR][i], vb, f, u_next, ast, wst, pout, grad_f, tau,
+                      tau * lambda, theta, acc[0]);
     }
     block_accumulate<1, 2>(acc, slots);   // slot 2: lambda
 }
 
 __global__ __launch_bounds__(AB) void adjoint_phase_b(Fields17 A, Fields17 S,
+                                                      const float* __restrict__ pin,
                                                       const float* __restrict__ ast,
                                                       const float* __restrict__ wst, Geometry g,
                                                       float tau, float sigma, float alpha0,
@@ -139,7 +157,7 @@ __global__ __launch_bounds__(AB) void adjoint_phase_b(Fields17 A, Fields17 S,
         // P' = A.p - tau B^T A*   (component a uses B_a)
         float P[3];
 #pragma unroll
-        for (int a = 0; a < 3; ++a) P[a] = A.f[F_PX + a][i] - tau * bm_t(ast, i, st[a], c3[a], ext[a]);
+        for (int a = 0; a < 3; ++a) P[a] = pin[size_t(a) * n + i] - tau * bm_t(ast, i, st[a], c3[a], ext[a]);
         // Q' = A.q + div2^T W*
         const float* W0 = wst;
         const float* W1 = wst + n;
@@ -212,35 +230,71 @@ __global__ __launch_bounds__(AB) void adjoint_phase_b(Fields17 A, Fields17 S,
     block_accumulate<2, 0>(acc, slots);   // slots 0, 1: alpha0, alpha1
 }
 
+// Phase C at voxel (i, c3): the u_bar and v_bar adjoints from P~, Q~ (the p
+// and q adjoints after phase B).
+__device__ __forceinline__ void phase_c_voxel(const Fields17& A, uint32_t i, const int (&c3)[3],
+                                              const uint32_t (&st)[3], const int (&ext)[3],
+                                              float sigma, float& ub_out, float (&vb)[3]) {
+    // u_bar adjoint = sigma (D+)^T P~
+    float ub = 0.0f;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) ub += dplus_t(A.f[F_PX + a], i, st[a], c3[a], ext[a]);
+    ub_out = sigma * ub;
+    // v_bar adjoint = sigma eps_B^T Q~ - sigma P~
+    //   x: B_x^T Q_xx + 1/2 B_y^T Q_xy + 1/2 B_z^T Q_xz, etc.
+    const float* Qxx = A.f[F_QXX]; const float* Qyy = A.f[F_QYY]; const float* Qzz = A.f[F_QZZ];
+    const float* Qxy = A.f[F_QXY]; const float* Qxz = A.f[F_QXZ]; const float* Qyz = A.f[F_QYZ];
+    const float ex = bm_t(Qxx, i, st[0], c3[0], ext[0]) +
+                     0.5f * bm_t(Qxy, i, st[1], c3[1], ext[1]) +
+                     0.5f * bm_t(Qxz, i, st[2], c3[2], ext[2]);
+    const float ey = bm_t(Qyy, i, st[1], c3[1], ext[1]) +
+                     0.5f * bm_t(Qxy, i, st[0], c3[0], ext[0]) +
+                     0.5f * bm_t(Qyz, i, st[2], c3[2], ext[2]);
+    const float ez = bm_t(Qzz, i, st[2], c3[2], ext[2]) +
+                     0.5f * bm_t(Qxz, i, st[0], c3[0], ext[0]) +
+                     0.5f * bm_t(Qyz, i, st[1], c3[1], ext[1]);
+    vb[0] = sigma * ex - sigma * A.f[F_PX][i];
+    vb[1] = sigma * ey - sigma * A.f[F_PY][i];
+    vb[2] = sigma * ez - sigma * A.f[F_PZ][i];
+}
+
 __global__ __launch_bounds__(AB) void adjoint_phase_c(Fields17 A, Geometry g, float sigma) {
     const uint32_t st[3] = {1u, uint32_t(g.w), uint32_t(g.w) * uint32_t(g.h)};
     const int ext[3] = {g.w, g.h, g.d};
     const RowVoxel rv(g);
     if (rv.in) {
-        const uint32_t i = rv.i;
         const int c3[3] = {rv.x, rv.y, rv.z};
-        // u_bar adjoint = sigma (D+)^T P~
-        float ub = 0.0f;
-#pragma unroll
-        for (int a = 0; a < 3; ++a) ub += dplus_t(A.f[F_PX + a], i, st[a], c3[a], ext[a]);
-        A.f[F_UBAR][i] = sigma * ub;
-        // v_bar adjoint = sigma eps_B^T Q~ - sigma P~
-        //   x: B_x^T Q_xx + 1/2 B_y^T Q_xy + 1/2 B_z^T Q_xz, etc.
-        const float* Qxx = A.f[F_QXX]; const float* Qyy = A.f[F_QYY]; const float* Qzz = A.f[F_QZZ];
-        const float* Qxy = A.f[F_QXY]; const float* Qxz = A.f[F_QXZ]; const float* Qyz = A.f[F_QYZ];
-        const float ex = bm_t(Qxx, i, st[0], c3[0], ext[0]) +
-                         0.5f * bm_t(Qxy, i, st[1], c3[1], ext[1]) +
-                         0.5f * bm_t(Qxz, i, st[2], c3[2], ext[2]);
-        const float ey = bm_t(Qyy, i, st[1], c3[1], ext[1]) +
-                         0.5f * bm_t(Qxy, i, st[0], c3[0], ext[0]) +
-                         0.5f * bm_t(Qyz, i, st[2], c3[2], ext[2]);
-        const float ez = bm_t(Qzz, i, st[2], c3[2], ext[2]) +
-                         0.5f * bm_t(Qxz, i, st[0], c3[0], ext[0]) +
-                         0.5f * bm_t(Qyz, i, st[1], c3[1], ext[1]);
-        A.f[F_VBX][i] = sigma * ex - sigma * A.f[F_PX][i];
-        A.f[F_VBY][i] = sigma * ey - sigma * A.f[F_PY][i];
-        A.f[F_VBZ][i] = sigma * ez - sigma * A.f[F_PZ][i];
+        float ub, vb[3];
+        phase_c_voxel(A, rv.i, c3, st, ext, sigma, ub, vb);
+        A.f[F_UBAR][rv.i] = ub;
+        A.f[F_VBX][rv.i] = vb[0];
+        A.f[F_VBY][rv.i] = vb[1];
+        A.f[F_VBZ][rv.i] = vb[2];
     }
+}
+
+// Phase C of iteration k and phase A of iteration k - 1 (u_next: u of x_k).
+__global__ __launch_bounds__(AB) void adjoint_phase_ca(Fields17 A, const float* __restrict__ f,
+                                                       const float* __restrict__ u_next,
+                                                       float* __restrict__ ast,
+                                                       float* __restrict__ wst,
+                                                       float* __restrict__ pout,
+                                                       float* __restrict__ grad_f, Geometry g,
+                                                       float tau, float sigma, float lambda,
+                                                       float theta, double* __restrict__ slots) {
+    const uint32_t n = uint32_t(g.n());
+    const uint32_t st[3] = {1u, uint32_t(g.w), uint32_t(g.w) * uint32_t(g.h)};
+    const int ext[3] = {g.w, g.h, g.d};
+    double acc[1] = {0.0};
+    const RowVoxel rv(g);
+    if (rv.in) {
+        const int c3[3] = {rv.x, rv.y, rv.z};
+        float ub, vb[3];
+        phase_c_voxel(A, rv.i, c3, st, ext, sigma, ub, vb);
+        phase_a_voxel(A, rv.i, n, ub, vb, f, u_next, ast, wst, pout, grad_f, tau, tau * lambda,
+                      theta, acc[0]);
+    }
+    block_accumulate<1, 2>(acc, slots);   // slot 2: lambda
 }
 
 // out[k] = sum over blocks of slots[b][k], fixed order (one wave per k < 3)
@@ -284,22 +338,34 @@ Fields17 to_fields(void* const* p) {
 size_t adjoint_slots_size(Geometry g) { return size_t(row_blocks(g)) * 3; }
 
 // slots: adjoint_slots_size(g) doubles, zeroed before the first step; each
-// step adds every block's (alpha0, alpha1, lambda) terms to its own slots.
-hipError_t launch_adjoint_step(hipStream_t s, void* const* adj, void* const* xk, void* const* xk1,
-                               const float* f, float* ast, float* wst, float* grad_f, Geometry g,
-                               float tau, float sigma, float theta, float lambda, float alpha0,
-                               float alpha1, double* slots) {
-    const int nb = row_blocks(g), nt = row_threads(g);
-    const Fields17 A = to_fields(adj), S = to_fields(xk), T = to_fields(xk1);
-    hipLaunchKernelGGL(adjoint_phase_a, dim3(nb), dim3(nt), 0, s, A, f, T.f[F_U], ast, wst,
-                       grad_f, g, tau, lambda, theta, slots);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(adjoint_phase_b, dim3(nb), dim3(nt), 0, s, A, S, ast, wst, g, tau, sigma,
-                       alpha0, alpha1, slots);
-    e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(adjoint_phase_c, dim3(nb), dim3(nt), 0, s, A, g, sigma);
+// phase adds every block's (alpha0, alpha1, lambda) terms to its own slots.
+hipError_t launch_adjoint_a(hipStream_t s, const AdjointArgs& a, const float* u_next) {
+    const Fields17 A = to_fields(a.adj);
+    hipLaunchKernelGGL(adjoint_phase_a, dim3(row_blocks(a.g)), dim3(row_threads(a.g)), 0, s, A,
+                       a.f, u_next, a.ast, a.wst, a.pbuf, a.grad_f, a.g, a.tau, a.lambda, a.theta,
+                       a.slots);
+    return hipGetLastError();
+}
+
+hipError_t launch_adjoint_b(hipStream_t s, const AdjointArgs& a, void* const* xk) {
+    const Fields17 A = to_fields(a.adj), S = to_fields(xk);
+    hipLaunchKernelGGL(adjoint_phase_b, dim3(row_blocks(a.g)), dim3(row_threads(a.g)), 0, s, A, S,
+                       a.pbuf, a.ast, a.wst, a.g, a.tau, a.sigma, a.alpha0, a.alpha1, a.slots);
+    return hipGetLastError();
+}
+
+hipError_t launch_adjoint_ca(hipStream_t s, const AdjointArgs& a, const float* u_next) {
+    const Fields17 A = to_fields(a.adj);
+    hipLaunchKernelGGL(adjoint_phase_ca, dim3(row_blocks(a.g)), dim3(row_threads(a.g)), 0, s, A,
+                       a.f, u_next, a.ast, a.wst, a.pbuf, a.grad_f, a.g, a.tau, a.sigma, a.lambda,
+                       a.theta, a.slots);
+    return hipGetLastError();
+}
+
+hipError_t launch_adjoint_c(hipStream_t s, const AdjointArgs& a) {
+    const Fields17 A = to_fields(a.adj);
+    hipLaunchKernelGGL(adjoint_phase_c, dim3(row_blocks(a.g)), dim3(row_threads(a.g)), 0, s, A,
+                       a.g, a.sigma);
     return hipGetLastError();
 }
 

@@ -216,6 +216,10 @@ extern const int FUSED_FIELD_MAP[N_FUSED_FIELDS];
 // 3-D loop without a mask, the L2 data term and a two-voxel plan
 // (fused_bars_supported); for the adjoint's segment recompute.
 bool fused_bars_supported(int storage, Geometry g, const SweepParams& p, const FusedPlan& plan);
+// The fused sweep divides |p| / alpha1 and |q| / alpha0 through host
+// reciprocals, exact for alphas within [2^-60, 2^36]; outside that range the
+// iteration runs the two sweeps.
+bool fused_divisors_ok(float alpha0, float alpha1);
 hipError_t launch_fused_sweep_bars(hipStream_t s, void* const* src, void* const* dst,
                                    const float* f, Geometry g, const SweepParams& p,
                                    const FusedPlan& plan);
@@ -311,10 +315,27 @@ hipError_t launch_energy_finish(hipStream_t s, const double* partials, int rows,
 // block accumulates its (alpha0, alpha1, lambda) terms over the steps;
 // launch_adjoint_reduce sums them (fixed order) into out3.
 size_t adjoint_slots_size(Geometry g);
-hipError_t launch_adjoint_step(hipStream_t s, void* const* adj, void* const* xk, void* const* xk1,
-                               const float* f, float* ast, float* wst, float* grad_f, Geometry g,
-                               float tau, float sigma, float theta, float lambda, float alpha0,
-                               float alpha1, double* slots);
+// The reverse pass's buffers and constants: adj (the 17 adjoint fields), pbuf
+// (3 fields: phase A's p adjoint), ast (1 field), wst (3 fields), grad_f, slots.
+struct AdjointArgs {
+    void* const* adj;
+    float* pbuf;
+    float* ast;
+    float* wst;
+    float* grad_f;
+    const float* f;
+    double* slots;
+    Geometry g;
+    float tau, sigma, theta, lambda, alpha0, alpha1;
+};
+// Iteration k in reverse: launch_adjoint_a (phase A of the last iteration only,
+// u_next = u of x_N), then per k: launch_adjoint_b (xk = x_k), then
+// launch_adjoint_ca (phase C of k with phase A of k - 1, u_next = u of x_k) or,
+// for k = 0, launch_adjoint_c.
+hipError_t launch_adjoint_a(hipStream_t s, const AdjointArgs& a, const float* u_next);
+hipError_t launch_adjoint_b(hipStream_t s, const AdjointArgs& a, void* const* xk);
+hipError_t launch_adjoint_ca(hipStream_t s, const AdjointArgs& a, const float* u_next);
+hipError_t launch_adjoint_c(hipStream_t s, const AdjointArgs& a);
 hipError_t launch_adjoint_reduce(hipStream_t s, const double* slots, Geometry g, double* out3);
 hipError_t launch_adjoint_finish(hipStream_t s, void* const* adj, float* grad_f, Geometry g);
 

@@ -509,8 +509,9 @@ amb_status begin(amb_context* c, amb_memory mem, const float* f, uint32_t w, uin
     c->sp.tile_active = nullptr;
     c->sp.active = nullptr;
     c->sp.planar = (p->flags & AMB_FLAG_PLANAR) ? 1 : 0;
-    c->fused_ok = fused_plan(g, c->storage, c->device, c->ov, &c->fplan);
-    c->iplan_ok = c->sp.planar && c->storage == 0 &&
+    const bool fdiv = fused_divisors_ok(c->sp.alpha0, c->sp.alpha1);
+    c->fused_ok = fdiv && fused_plan(g, c->storage, c->device, c->ov, &c->fplan);
+    c->iplan_ok = fdiv && c->sp.planar && c->storage == 0 &&
                   fused_image_plan(g, c->device, c->ov, &c->iplan);
     c->fen_ok = c->fused_ok && !c->sp.planar &&
                 fused_energy_plan(g, c->device, c->ov, c->fplan, &c->fen_plan) &&
@@ -1624,7 +1625,8 @@ AMB_API amb_status amb_tgv2_deshade_pipeline(amb_context* c, amb_memory mem, con
     // (TGVDeshadeMaskedProcessor.cpp:93-98)
     if (masked && d == 1 && !c->sp.planar) {
         c->sp.planar = 1;
-        c->iplan_ok = c->storage == 0 && fused_image_plan(c->g, c->device, c->ov, &c->iplan);
+        c->iplan_ok = c->storage == 0 && fused_divisors_ok(c->sp.alpha0, c->sp.alpha1) &&
+                      fused_image_plan(c->g, c->device, c->ov, &c->iplan);
     }
     if (masked) {
         // active set = mask >= 1 (TGVDeshadeMaskedProcessor.cpp:348-353); f stays unmasked
@@ -1956,7 +1958,7 @@ AMB_API amb_status amb_tgv2_denoise_adjoint(amb_context* c, amb_memory mem, cons
     const size_t state_bytes = fstride * N_FIELDS;
     const uint32_t nseg = N ? (N + K - 1) / K : 0;
     const size_t part_bytes = align_up(adjoint_slots_size(g) * sizeof(double), 256);
-    const size_t need = state_bytes * (size_t(nseg) + (K + 1) + 1) + fstride * 5 + part_bytes + 256;
+    const size_t need = state_bytes * (size_t(nseg) + (K + 1) + 1) + fstride * 8 + part_bytes + 256;
     if (need > c->adj_cap) {
         if (c->adj_block) AMB_HIP(hipFree(c->adj_block));
         c->adj_block = nullptr;
@@ -1974,8 +1976,9 @@ AMB_API amb_status amb_tgv2_denoise_adjoint(amb_context* c, amb_memory mem, cons
     float* ast = reinterpret_cast<float*>(adjs + state_bytes);
     float* wst = reinterpret_cast<float*>(adjs + state_bytes + fstride);   // 3 fields
     float* gradf = reinterpret_cast<float*>(adjs + state_bytes + fstride * 4);
-    double* partials = reinterpret_cast<double*>(adjs + state_bytes + fstride * 5);
-    double* acc = reinterpret_cast<double*>(adjs + state_bytes + fstride * 5 + part_bytes);
+    float* pbuf = reinterpret_cast<float*>(adjs + state_bytes + fstride * 5);   // 3 fields
+    double* partials = reinterpret_cast<double*>(adjs + state_bytes + fstride * 8);
+    double* acc = reinterpret_cast<double*>(adjs + state_bytes + fstride * 8 + part_bytes);
     ProfScope ps(c, P_ADJOINT);
     // forward with a checkpoint of x_k at every k % K == 0: each segment runs the
     // iteration loop (fused sweeps inside, ending with the two-sweep state); the
@@ -2007,6 +2010,21 @@ AMB_API amb_status amb_tgv2_denoise_adjoint(amb_context* c, amb_memory mem, cons
     // and v_bar, out of place, one launch per iteration; else in place with the
     // two sweeps after a copy of the state
     const bool fb = c->fused_ok && fused_bars_supported(c->storage, g, c->sp, c->fplan);
+    const AdjointArgs aa{adj, pbuf, ast, wst, gradf, c->f, partials, g, c->sp.tau, c->sp.sigma,
+                         c->sp.theta, c->sp.lambda, c->sp.alpha0, c->sp.alpha1};
+    // iteration k in reverse: phase B with x_k, then phase C of k with phase A
+    // of k - 1 (u of x_k); phase A alone for the last iteration, C alone for k = 0
+    bool first = true;
+    auto reverse_step = [&](uint32_t k, void* const* xk, void* const* xk1) -> amb_status {
+        if (first) {
+            AMB_HIP(launch_adjoint_a(c->stream, aa, static_cast<const float*>(xk1[F_U])));
+            first = false;
+        }
+        AMB_HIP(launch_adjoint_b(c->stream, aa, xk));
+        if (k > 0) AMB_HIP(launch_adjoint_ca(c->stream, aa, static_cast<const float*>(xk[F_U])));
+        else AMB_HIP(launch_adjoint_c(c->stream, aa));
+        return AMB_OK;
+    };
     for (int seg = int(nseg) - 1; seg >= 0; --seg) {
         const uint32_t k0 = uint32_t(seg) * K, k1 = std::min(k0 + K, N), L = k1 - k0;
         char* cp = ckpt + state_bytes * size_t(seg);
@@ -2032,9 +2050,7 @@ AMB_API amb_status amb_tgv2_denoise_adjoint(amb_context* c, amb_memory mem, cons
                 char* pq = j > 0 ? slots + state_bytes * size_t(j - 1) : cp;
                 for (int k = F_PX; k <= F_QYZ; ++k) tabs[0][k] = pq + size_t(k) * fstride;
                 state_at(slots + state_bytes * size_t(j + 1), tabs[1]);
-                AMB_HIP(launch_adjoint_step(c->stream, adj, tabs[0], tabs[1], c->f, ast, wst, gradf,
-                                            g, c->sp.tau, c->sp.sigma, c->sp.theta, c->sp.lambda,
-                                            c->sp.alpha0, c->sp.alpha1, partials));
+                AMB_TRY(reverse_step(k0 + uint32_t(j), tabs[0], tabs[1]));
             }
             continue;
         }
@@ -2050,9 +2066,7 @@ AMB_API amb_status amb_tgv2_denoise_adjoint(amb_context* c, amb_memory mem, cons
         for (int j = int(L) - 1; j >= 0; --j) {
             state_at(slots + state_bytes * size_t(j), tabs[0]);
             state_at(slots + state_bytes * size_t(j + 1), tabs[1]);
-            AMB_HIP(launch_adjoint_step(c->stream, adj, tabs[0], tabs[1], c->f, ast, wst, gradf, g,
-                                        c->sp.tau, c->sp.sigma, c->sp.theta, c->sp.lambda,
-                                        c->sp.alpha0, c->sp.alpha1, partials));
+            AMB_TRY(reverse_step(k0 + uint32_t(j), tabs[0], tabs[1]));
         }
     }
     AMB_HIP(launch_adjoint_reduce(c->stream, partials, g, acc));

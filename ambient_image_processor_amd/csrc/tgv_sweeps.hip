@@ -29,6 +29,7 @@
 // correctly rounded division and sqrt (this file is built with
 // -ffp-contract=off).
 #include "amb_internal.hpp"
+#include "div_rn.hpp"
 
 #include <map>
 #include <mutex>
@@ -943,7 +944,7 @@ __global__ __launch_bounds__(MAXT) void fused_sweep_kernel(
     FusedFields ff, const float* __restrict__ f_, const uint8_t* __restrict__ act, Geometry g,
     float tau, float sigma, float lambda, float alpha0, float alpha1, float theta, int ty,
     int zchunk, int planar, int tx, double* __restrict__ epart,
-    const uint8_t* __restrict__ tile_active) {
+    const uint8_t* __restrict__ tile_active, DivConsts dc) {
     static_assert(!EN || P2 == 0, "the in-sweep energy covers the 3-D loop");
     static_assert(!SB || P2 == 0, "u_bar / v_bar stores cover the 3-D loop");
     typedef typename S::T T;
@@ -1300,7 +1301,9 @@ __global__ __launch_bounds__(MAXT) void fused_sweep_kernel(
                 float n2 = ax * ax;
                 n2 = fmaf(ay, ay, n2);
                 n2 = fmaf(az, az, n2);
-                const float tp = sqrtf(n2) / alpha1;
+                // |p| / alpha1 by the host's RN(1/alpha1) and one correction
+                // (div_rn.hpp): the quotient rounded as the IEEE division rounds it
+                const float tp = div_rn(sqrtf(n2), alpha1, dc.inv_alpha1);
                 if (tp > 1.0f) {
                     po[0][e] = ax / tp; po[1][e] = ay / tp; po[2][e] = az / tp;
                 } else {
@@ -1329,7 +1332,7 @@ __global__ __launch_bounds__(MAXT) void fused_sweep_kernel(
                 m2 = fmaf(bzz, bzz, m2);
                 m2 = fmaf(2.0f * bxz, bxz, m2);
                 m2 = fmaf(2.0f * byz, byz, m2);
-                const float tq = sqrtf(m2) / alpha0;
+                const float tq = div_rn(sqrtf(m2), alpha0, dc.inv_alpha0);
                 if (tq > 1.0f) {
                     qo[0][e] = bxx / tq; qo[1][e] = byy / tq; qo[2][e] = bzz / tq;
                     qo[3][e] = bxy / tq; qo[4][e] = bxz / tq; qo[5][e] = byz / tq;
@@ -1631,6 +1634,7 @@ hipError_t fused_launch(hipStream_t s, const FusedFields& ff, const float* f, Ge
     const int row = TL ? plan.tx + 2 * VX : g.w;
     const int L = row / VX, RT = P2 == 2 ? plan.ty : plan.ty + 2;
     if (L * RT > MAXT) return hipErrorInvalidConfiguration;
+    if (!fused_divisors_ok(p.alpha0, p.alpha1)) return hipErrorInvalidValue;
     const int nbx = TL ? (g.w + plan.tx - 1) / plan.tx : 1;
     const int nb = (g.h + plan.ty - 1) / plan.ty, nc = (g.d + plan.zchunk - 1) / plan.zchunk;
     size_t lds = (size_t(EN ? 12 : 8) * RT * row + size_t(EN ? 12 : 8) * RT * (L + 1)) * sizeof(float);
@@ -1641,7 +1645,8 @@ hipError_t fused_launch(hipStream_t s, const FusedFields& ff, const float* f, Ge
     hipLaunchKernelGGL(kern, dim3(nbx * nb * nc), dim3(L * RT), lds, s, ff, f, p.active, g, p.tau,
                        p.sigma, p.lambda, p.alpha0, p.alpha1, p.theta, plan.ty, plan.zchunk,
                        p.planar, TL ? plan.tx : 0, EN ? p.energy_partials : nullptr,
-                       MK && !EN && P2 == 0 ? p.tile_active : nullptr);
+                       MK && !EN && P2 == 0 ? p.tile_active : nullptr,
+                       div_consts(p.alpha0, p.alpha1));
     return hipGetLastError();
 }
 
@@ -2010,6 +2015,10 @@ hipError_t launch_fused_sweep(hipStream_t s, int storage, void* const* src, void
                      : fused_dispatch<StoreBF16, size_t>(s, ff, f, g, p, plan);
     return small ? fused_dispatch<StoreF32, uint32_t>(s, ff, f, g, p, plan)
                  : fused_dispatch<StoreF32, size_t>(s, ff, f, g, p, plan);
+}
+
+bool fused_divisors_ok(float alpha0, float alpha1) {
+    return div_rn_divisor_ok(alpha0) && div_rn_divisor_ok(alpha1);
 }
 
 bool fused_bars_supported(int storage, Geometry g, const SweepParams& p, const FusedPlan& plan) {

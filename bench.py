@@ -299,14 +299,20 @@ def deshade_256(args):
                 ("dct", "fp32 fields + bf16x3 MFMA DCT", native.STORAGE_F32, native.DCT_BF16X3),
                 ("storage", "bf16 fields + bf16x3 MFMA DCT", native.STORAGE_BF16,
                  native.DCT_BF16X3)):
-            s_el, _ = timed(ctx, step_with(params(st_, dct_)), args.steps, args.warmup)
+            s_el, s_prof = timed(ctx, step_with(params(st_, dct_)), args.steps, args.warmup)
             s_val = voxel_iters / s_el
             u_s, l_s = ctx.download(u_dev, shape), ctx.download(l_dev, shape)
+            s_kern = sweep_kernels(s_prof, 2 if st_ == native.STORAGE_BF16 else 4, n)
             split[key] = {"form": form, "value": s_val, "ms_per_step": 1000.0 * s_el / args.steps,
                           "speedup_vs_plain": s_val / value,
                           "u_rel_err": rel_err(u_s, u_plain), "u_rms_err": rms_err(u_s, u_plain),
                           "shading_rel_err": rel_err(l_s, l_plain),
-                          "shading_rms_err": rms_err(l_s, l_plain)}
+                          "shading_rms_err": rms_err(l_s, l_plain),
+                          "fused_sweep": rounded({"k": s_kern["fused_sweep"]})["k"]
+                          if "fused_sweep" in s_kern else None}
+            if "fused_sweep" in s_kern:
+                split[key]["fused_sweep"]["hbm_frac"] = round(
+                    s_kern["fused_sweep"]["gbs"] / HBM_PEAK_GBS, 4)
 
     # in-sweep FP64 energy / gap every 100 iterations (the fused sweep's energy
     # form) on one more deshade: its launch time against the plain fused sweep
