@@ -65,16 +65,20 @@ __device__ __forceinline__ void block_accumulate(double (&acc)[NQ], double* slot
     }
 }
 
-// Voxel of this thread: block b covers row r = b / nbx (y = r % H, z = r / H),
-// columns (b % nbx) * blockDim.x + threadIdx.x.
+// Voxel of this thread: row piece b = xcd_contiguous(blockIdx.x) covers row
+// r = b / nbx (y = r % H, z = r / H), columns (b % nbx) * blockDim.x + threadIdx.x.
+// The remap gives each XCD a contiguous range of rows, so the y and z neighbour
+// rows a block reads were mostly fetched into the same XCD's L2 by its
+// neighbouring blocks (round-robin placement fetched them again from HBM).
 struct RowVoxel {
     int x, y, z;
     uint32_t i;
     bool in;
     __device__ __forceinline__ RowVoxel(const Geometry& g) {
         const uint32_t nbx = (uint32_t(g.w) + blockDim.x - 1) / blockDim.x;
-        const uint32_t r = blockIdx.x / nbx;
-        x = int((blockIdx.x - r * nbx) * blockDim.x + threadIdx.x);
+        const uint32_t b = xcd_contiguous(blockIdx.x, gridDim.x);
+        const uint32_t r = b / nbx;
+        x = int((b - r * nbx) * blockDim.x + threadIdx.x);
         y = int(r % uint32_t(g.h));
         z = int(r / uint32_t(g.h));
         in = x < g.w;
