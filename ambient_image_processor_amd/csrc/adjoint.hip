@@ -65,39 +65,102 @@ __device__ __forceinline__ void block_accumulate(double (&acc)[NQ], double* slot
     }
 }
 
-// Voxel of this thread: row piece b = xcd_contiguous(blockIdx.x) covers row
-// r = b / nbx (y = r % H, z = r / H), columns (b % nbx) * blockDim.x + threadIdx.x.
-// The remap gives each XCD a contiguous range of rows, so the y and z neighbour
-// rows a block reads were mostly fetched into the same XCD's L2 by its
-// neighbouring blocks (round-robin placement fetched them again from HBM).
-struct RowVoxel {
-    int x, y, z;
-    uint32_t i;
+// VX consecutive voxels of one row per thread (VX = 2 needs an even width):
+// lane l of row piece b = xcd_contiguous(blockIdx.x) covers x = VX l, ...,
+// VX l + VX - 1 of row r = b / nbx (y = r % H, z = r / H).  Own values are
+// loaded VX at a time; the x neighbours across the pair come from the other
+// element or one scalar load.
+template <int VX>
+struct RowVox {
+    int x, y, z;   // x of the first voxel
+    uint32_t i;    // its index
     bool in;
-    __device__ __forceinline__ RowVoxel(const Geometry& g) {
-        const uint32_t nbx = (uint32_t(g.w) + blockDim.x - 1) / blockDim.x;
+    __device__ __forceinline__ RowVox(const Geometry& g) {
+        const uint32_t lanes = uint32_t(g.w) / VX;
+        const uint32_t nbx = (lanes + blockDim.x - 1) / blockDim.x;
         const uint32_t b = xcd_contiguous(blockIdx.x, gridDim.x);
         const uint32_t r = b / nbx;
-        x = int((b - r * nbx) * blockDim.x + threadIdx.x);
+        const uint32_t lane = (b - r * nbx) * blockDim.x + threadIdx.x;
+        x = int(lane) * VX;
         y = int(r % uint32_t(g.h));
         z = int(r / uint32_t(g.h));
-        in = x < g.w;
+        in = lane < lanes;
         i = (uint32_t(z) * uint32_t(g.h) + uint32_t(y)) * uint32_t(g.w) + uint32_t(x);
     }
 };
 
-// (D+)^T b [j] = b[j-1] [j >= 1] - b[j] [j < n-1]
-__device__ __forceinline__ float dplus_t(const float* b, uint32_t i, uint32_t st, int c, int n) {
+template <int VX>
+__device__ __forceinline__ void ldv(const float* __restrict__ p, uint32_t i, float (&v)[VX]) {
+    if constexpr (VX == 2) {
+        const float2 t = *reinterpret_cast<const float2*>(p + i);
+        v[0] = t.x; v[1] = t.y;
+    } else {
+        v[0] = p[i];
+    }
+}
+template <int VX>
+__device__ __forceinline__ void stv(float* __restrict__ p, uint32_t i, const float (&v)[VX]) {
+    if constexpr (VX == 2) *reinterpret_cast<float2*>(p + i) = make_float2(v[0], v[1]);
+    else p[i] = v[0];
+}
+
+// Neighbour values of the VX voxels along axis a (0 = x): at +1 (nxt) or -1
+// (prv); 0 where the neighbour lies outside (its flag is false there and the
+// value unused).  own: the field's values at the VX voxels.
+template <int VX>
+__device__ __forceinline__ void nxt(const float* __restrict__ p, const RowVox<VX>& v, int a,
+                                    const float (&own)[VX], const Geometry& g, float (&out)[VX]) {
+    if (a == 0) {
+#pragma unroll
+        for (int e = 0; e + 1 < VX; ++e) out[e] = own[e + 1];
+        out[VX - 1] = v.x + VX < g.w ? p[v.i + VX] : 0.0f;
+    } else if (a == 1) {
+        if (v.y < g.h - 1) ldv<VX>(p, v.i + uint32_t(g.w), out);
+        else for (int e = 0; e < VX; ++e) out[e] = 0.0f;
+    } else {
+        if (v.z < g.d - 1) ldv<VX>(p, v.i + uint32_t(g.w) * uint32_t(g.h), out);
+        else for (int e = 0; e < VX; ++e) out[e] = 0.0f;
+    }
+}
+template <int VX>
+__device__ __forceinline__ void prv(const float* __restrict__ p, const RowVox<VX>& v, int a,
+                                    const float (&own)[VX], const Geometry& g, float (&out)[VX]) {
+    if (a == 0) {
+        out[0] = v.x > 0 ? p[v.i - 1] : 0.0f;
+#pragma unroll
+        for (int e = 1; e < VX; ++e) out[e] = own[e - 1];
+    } else if (a == 1) {
+        if (v.y > 0) ldv<VX>(p, v.i - uint32_t(g.w), out);
+        else for (int e = 0; e < VX; ++e) out[e] = 0.0f;
+    } else {
+        if (v.z > 0) ldv<VX>(p, v.i - uint32_t(g.w) * uint32_t(g.h), out);
+        else for (int e = 0; e < VX; ++e) out[e] = 0.0f;
+    }
+}
+
+// (D+)^T b [j] = b[j-1] [j >= 1] - b[j] [j < n-1], from own value o and
+// previous value m (has_m: j >= 1, has_p: j < n-1)
+__device__ __forceinline__ float dplus_tv(float o, float m, bool has_m, bool has_p) {
     float r = 0.0f;
-    if (c >= 1) r += b[i - st];
-    if (c < n - 1) r -= b[i];
+    if (has_m) r += m;
+    if (has_p) r -= o;
     return r;
 }
-// B^T b [j] = b[j+1] [j < n-1] - b[j]
-__device__ __forceinline__ float bm_t(const float* b, uint32_t i, uint32_t st, int c, int n) {
-    float r = -b[i];
-    if (c < n - 1) r += b[i + st];
+// B^T b [j] = b[j+1] [j < n-1] - b[j], from own value o and next value q
+__device__ __forceinline__ float bm_tv(float o, float q, bool has_p) {
+    float r = -o;
+    if (has_p) r += q;
     return r;
+}
+
+// Per-voxel flags: hp[a]: the voxel has a +a neighbour; hm[a]: a -a neighbour.
+template <int VX>
+__device__ __forceinline__ void flags(const RowVox<VX>& v, int e, const Geometry& g, bool (&hp)[3],
+                                      bool (&hm)[3]) {
+    const int c[3] = {v.x + e, v.y, v.z};
+    const int ext[3] = {g.w, g.h, g.d};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) { hp[a] = c[a] < ext[a] - 1; hm[a] = c[a] >= 1; }
 }
 
 // Phase A at voxel i from the u_bar and v_bar adjoints ub, vb: the p adjoint
@@ -134,7 +197,7 @@ __global__ __launch_bounds__(AB) void adjoint_phase_a(Fields17 A, const float* _
                                                       double* __restrict__ slots) {
     const uint32_t n = uint32_t(g.n());
     double acc[1] = {0.0};
-    const RowVoxel rv(g);
+    const RowVox<1> rv(g);
     if (rv.in) {
         const uint32_t i = rv.i;
         const float vb[3] = {A.f[F_VBX][i], A.f[F_VBY][i], A.f[F_VBZ][i]};
@@ -144,6 +207,7 @@ __global__ __launch_bounds__(AB) void adjoint_phase_a(Fields17 A, const float* _
     block_accumulate<1, 2>(acc, slots);   // slot 2: lambda
 }
 
+template <int VX>
 __global__ __launch_bounds__(AB) void adjoint_phase_b(Fields17 A, Fields17 S,
                                                       const float* __restrict__ pin,
                                                       const float* __restrict__ ast,
@@ -151,133 +215,177 @@ __global__ __launch_bounds__(AB) void adjoint_phase_b(Fields17 A, Fields17 S,
                                                       float tau, float sigma, float alpha0,
                                                       float alpha1, double* __restrict__ slots) {
     const uint32_t n = uint32_t(g.n());
-    const uint32_t st[3] = {1u, uint32_t(g.w), uint32_t(g.w) * uint32_t(g.h)};
-    const int ext[3] = {g.w, g.h, g.d};
     double acc[2] = {0.0, 0.0};   // alpha0, alpha1
-    const RowVoxel rv(g);
+    const RowVox<VX> rv(g);
     if (rv.in) {
         const uint32_t i = rv.i;
-        const int c3[3] = {rv.x, rv.y, rv.z};
-        // P' = A.p - tau B^T A*   (component a uses B_a)
-        float P[3];
+        // own values and the neighbours the transposes and the recomputed p~, q~ read
+        float pi[3][VX], as[VX], asn[3][VX], w[3][VX], wp[3][3][VX], qa[6][VX];
+        float ub[VX], ubn[3][VX], vb[3][VX], vbp[3][3][VX], sp[3][VX], sq[6][VX];
 #pragma unroll
-        for (int a = 0; a < 3; ++a) P[a] = pin[size_t(a) * n + i] - tau * bm_t(ast, i, st[a], c3[a], ext[a]);
-        // Q' = A.q + div2^T W*
-        const float* W0 = wst;
-        const float* W1 = wst + n;
-        const float* W2 = wst + 2 * n;
-        float Q[6];
-        Q[0] = A.f[F_QXX][i] + dplus_t(W0, i, st[0], c3[0], ext[0]);
-        Q[1] = A.f[F_QYY][i] + dplus_t(W1, i, st[1], c3[1], ext[1]);
-        Q[2] = A.f[F_QZZ][i] + dplus_t(W2, i, st[2], c3[2], ext[2]);
-        Q[3] = A.f[F_QXY][i] + (dplus_t(W0, i, st[1], c3[1], ext[1]) + dplus_t(W1, i, st[0], c3[0], ext[0]));
-        Q[4] = A.f[F_QXZ][i] + (dplus_t(W0, i, st[2], c3[2], ext[2]) + dplus_t(W2, i, st[0], c3[0], ext[0]));
-        Q[5] = A.f[F_QYZ][i] + (dplus_t(W1, i, st[2], c3[2], ext[2]) + dplus_t(W2, i, st[1], c3[1], ext[1]));
-
-        // p~ from x_k, forward arithmetic (tgv_sweeps.hip dual sweep)
-        const float ubc = S.f[F_UBAR][i];
-        float pt[3];
+        for (int a = 0; a < 3; ++a) ldv<VX>(pin, a * n + i, pi[a]);
+        ldv<VX>(ast, i, as);
+        ldv<VX>(S.f[F_UBAR], i, ub);
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
-            const float d = (c3[a] < ext[a] - 1) ? S.f[F_UBAR][i + st[a]] - ubc : 0.0f;
-            pt[a] = fmaf(sigma, d - S.f[F_VBX + a][i], S.f[F_PX + a][i]);
-        }
-        float n2 = pt[0] * pt[0];
-        n2 = fmaf(pt[1], pt[1], n2);
-        n2 = fmaf(pt[2], pt[2], n2);
-        const float npn = sqrtf(n2);
-        if (npn / alpha1 > 1.0f) {
-            const float dot = P[0] * pt[0] + P[1] * pt[1] + P[2] * pt[2];
-            acc[1] += double(dot) / double(npn);
-            const float k1 = alpha1 / npn, k2 = dot / (npn * npn);
-#pragma unroll
-            for (int a = 0; a < 3; ++a) P[a] = k1 * (P[a] - k2 * pt[a]);
-        }
-        // q~ from x_k
-        float bxv[3], byv[3], bzv[3];
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            const float* vb = S.f[F_VBX + c];
-            const float vc = vb[i];
-            bxv[c] = c3[0] > 0 ? -vc + vb[i - st[0]] : -vc;
-            byv[c] = c3[1] > 0 ? -vc + vb[i - st[1]] : -vc;
-            bzv[c] = c3[2] > 0 ? -vc + vb[i - st[2]] : -vc;
-        }
-        float qt[6];
-        qt[0] = fmaf(sigma, bxv[0], S.f[F_QXX][i]);
-        qt[1] = fmaf(sigma, byv[1], S.f[F_QYY][i]);
-        qt[2] = fmaf(sigma, bzv[2], S.f[F_QZZ][i]);
-        qt[3] = fmaf(sigma, (bxv[1] + byv[0]) * 0.5f, S.f[F_QXY][i]);
-        qt[4] = fmaf(sigma, (bxv[2] + bzv[0]) * 0.5f, S.f[F_QXZ][i]);
-        qt[5] = fmaf(sigma, (byv[2] + bzv[1]) * 0.5f, S.f[F_QYZ][i]);
-        float m2 = qt[0] * qt[0];
-        m2 = fmaf(qt[1], qt[1], m2);
-        m2 = fmaf(2.0f * qt[3], qt[3], m2);
-        m2 = fmaf(qt[2], qt[2], m2);
-        m2 = fmaf(2.0f * qt[4], qt[4], m2);
-        m2 = fmaf(2.0f * qt[5], qt[5], m2);
-        const float nq = sqrtf(m2);
-        if (nq / alpha0 > 1.0f) {
-            float dot = 0.0f;
-#pragma unroll
-            for (int k = 0; k < 6; ++k) dot += Q[k] * qt[k];
-            acc[0] += double(dot) / double(nq);
-            const float k1 = alpha0 / nq, k2 = dot / (nq * nq);
-#pragma unroll
-            for (int k = 0; k < 6; ++k) Q[k] = k1 * (Q[k] - k2 * (k < 3 ? 1.0f : 2.0f) * qt[k]);
+            nxt<VX>(ast, rv, a, as, g, asn[a]);
+            nxt<VX>(S.f[F_UBAR], rv, a, ub, g, ubn[a]);
         }
 #pragma unroll
-        for (int a = 0; a < 3; ++a) A.f[F_PX + a][i] = P[a];
+        for (int k = 0; k < 3; ++k) {
+            ldv<VX>(wst, k * n + i, w[k]);
+            ldv<VX>(S.f[F_VBX + k], i, vb[k]);
+            ldv<VX>(S.f[F_PX + k], i, sp[k]);
 #pragma unroll
-        for (int k = 0; k < 6; ++k) A.f[F_QXX + k][i] = Q[k];
+            for (int a = 0; a < 3; ++a) {
+                prv<VX>(wst + k * n, rv, a, w[k], g, wp[k][a]);
+                prv<VX>(S.f[F_VBX + k], rv, a, vb[k], g, vbp[k][a]);
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            ldv<VX>(A.f[F_QXX + k], i, qa[k]);
+            ldv<VX>(S.f[F_QXX + k], i, sq[k]);
+        }
+        float Po[3][VX], Qo[6][VX];
+#pragma unroll
+        for (int e = 0; e < VX; ++e) {
+            bool hp[3], hm[3];
+            flags<VX>(rv, e, g, hp, hm);
+            // P' = A.p - tau B^T A*   (component a uses B_a)
+            float P[3];
+#pragma unroll
+            for (int a = 0; a < 3; ++a) P[a] = pi[a][e] - tau * bm_tv(as[e], asn[a][e], hp[a]);
+            // Q' = A.q + div2^T W*
+            auto dw = [&](int k, int a) { return dplus_tv(w[k][e], wp[k][a][e], hm[a], hp[a]); };
+            float Q[6];
+            Q[0] = qa[0][e] + dw(0, 0);
+            Q[1] = qa[1][e] + dw(1, 1);
+            Q[2] = qa[2][e] + dw(2, 2);
+            Q[3] = qa[3][e] + (dw(0, 1) + dw(1, 0));
+            Q[4] = qa[4][e] + (dw(0, 2) + dw(2, 0));
+            Q[5] = qa[5][e] + (dw(1, 2) + dw(2, 1));
+
+            // p~ from x_k, forward arithmetic (tgv_sweeps.hip dual sweep)
+            const float ubc = ub[e];
+            float pt[3];
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                const float d = hp[a] ? ubn[a][e] - ubc : 0.0f;
+                pt[a] = fmaf(sigma, d - vb[a][e], sp[a][e]);
+            }
+            float n2 = pt[0] * pt[0];
+            n2 = fmaf(pt[1], pt[1], n2);
+            n2 = fmaf(pt[2], pt[2], n2);
+            const float npn = sqrtf(n2);
+            if (npn / alpha1 > 1.0f) {
+                const float dot = P[0] * pt[0] + P[1] * pt[1] + P[2] * pt[2];
+                acc[1] += double(dot) / double(npn);
+                const float k1 = alpha1 / npn, k2 = dot / (npn * npn);
+#pragma unroll
+                for (int a = 0; a < 3; ++a) P[a] = k1 * (P[a] - k2 * pt[a]);
+            }
+            // q~ from x_k
+            float bxv[3], byv[3], bzv[3];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const float vc = vb[c][e];
+                bxv[c] = hm[0] ? -vc + vbp[c][0][e] : -vc;
+                byv[c] = hm[1] ? -vc + vbp[c][1][e] : -vc;
+                bzv[c] = hm[2] ? -vc + vbp[c][2][e] : -vc;
+            }
+            float qt[6];
+            qt[0] = fmaf(sigma, bxv[0], sq[0][e]);
+            qt[1] = fmaf(sigma, byv[1], sq[1][e]);
+            qt[2] = fmaf(sigma, bzv[2], sq[2][e]);
+            qt[3] = fmaf(sigma, (bxv[1] + byv[0]) * 0.5f, sq[3][e]);
+            qt[4] = fmaf(sigma, (bxv[2] + bzv[0]) * 0.5f, sq[4][e]);
+            qt[5] = fmaf(sigma, (byv[2] + bzv[1]) * 0.5f, sq[5][e]);
+            float m2 = qt[0] * qt[0];
+            m2 = fmaf(qt[1], qt[1], m2);
+            m2 = fmaf(2.0f * qt[3], qt[3], m2);
+            m2 = fmaf(qt[2], qt[2], m2);
+            m2 = fmaf(2.0f * qt[4], qt[4], m2);
+            m2 = fmaf(2.0f * qt[5], qt[5], m2);
+            const float nq = sqrtf(m2);
+            if (nq / alpha0 > 1.0f) {
+                float dot = 0.0f;
+#pragma unroll
+                for (int k = 0; k < 6; ++k) dot += Q[k] * qt[k];
+                acc[0] += double(dot) / double(nq);
+                const float k1 = alpha0 / nq, k2 = dot / (nq * nq);
+#pragma unroll
+                for (int k = 0; k < 6; ++k) Q[k] = k1 * (Q[k] - k2 * (k < 3 ? 1.0f : 2.0f) * qt[k]);
+            }
+#pragma unroll
+            for (int a = 0; a < 3; ++a) Po[a][e] = P[a];
+#pragma unroll
+            for (int k = 0; k < 6; ++k) Qo[k][e] = Q[k];
+        }
+#pragma unroll
+        for (int a = 0; a < 3; ++a) stv<VX>(A.f[F_PX + a], i, Po[a]);
+#pragma unroll
+        for (int k = 0; k < 6; ++k) stv<VX>(A.f[F_QXX + k], i, Qo[k]);
     }
     block_accumulate<2, 0>(acc, slots);   // slots 0, 1: alpha0, alpha1
 }
 
-// Phase C at voxel (i, c3): the u_bar and v_bar adjoints from P~, Q~ (the p
-// and q adjoints after phase B).
-__device__ __forceinline__ void phase_c_voxel(const Fields17& A, uint32_t i, const int (&c3)[3],
-                                              const uint32_t (&st)[3], const int (&ext)[3],
-                                              float sigma, float& ub_out, float (&vb)[3]) {
-    // u_bar adjoint = sigma (D+)^T P~
-    float ub = 0.0f;
+// Phase C at the VX voxels: the u_bar and v_bar adjoints from P~, Q~ (the p
+// and q adjoints after phase B); P~ at the voxels also returned (pt).
+template <int VX>
+__device__ __forceinline__ void phase_c_vox(const Fields17& A, const RowVox<VX>& rv,
+                                            const Geometry& g, float sigma, float (&ubo)[VX],
+                                            float (&vbo)[3][VX], float (&pt)[3][VX]) {
+    const uint32_t i = rv.i;
+    float pm[3][VX], q[6][VX], qn[6][3][VX];
 #pragma unroll
-    for (int a = 0; a < 3; ++a) ub += dplus_t(A.f[F_PX + a], i, st[a], c3[a], ext[a]);
-    ub_out = sigma * ub;
-    // v_bar adjoint = sigma eps_B^T Q~ - sigma P~
-    //   x: B_x^T Q_xx + 1/2 B_y^T Q_xy + 1/2 B_z^T Q_xz, etc.
-    const float* Qxx = A.f[F_QXX]; const float* Qyy = A.f[F_QYY]; const float* Qzz = A.f[F_QZZ];
-    const float* Qxy = A.f[F_QXY]; const float* Qxz = A.f[F_QXZ]; const float* Qyz = A.f[F_QYZ];
-    const float ex = bm_t(Qxx, i, st[0], c3[0], ext[0]) +
-                     0.5f * bm_t(Qxy, i, st[1], c3[1], ext[1]) +
-                     0.5f * bm_t(Qxz, i, st[2], c3[2], ext[2]);
-    const float ey = bm_t(Qyy, i, st[1], c3[1], ext[1]) +
-                     0.5f * bm_t(Qxy, i, st[0], c3[0], ext[0]) +
-                     0.5f * bm_t(Qyz, i, st[2], c3[2], ext[2]);
-    const float ez = bm_t(Qzz, i, st[2], c3[2], ext[2]) +
-                     0.5f * bm_t(Qxz, i, st[0], c3[0], ext[0]) +
-                     0.5f * bm_t(Qyz, i, st[1], c3[1], ext[1]);
-    vb[0] = sigma * ex - sigma * A.f[F_PX][i];
-    vb[1] = sigma * ey - sigma * A.f[F_PY][i];
-    vb[2] = sigma * ez - sigma * A.f[F_PZ][i];
+    for (int a = 0; a < 3; ++a) {
+        ldv<VX>(A.f[F_PX + a], i, pt[a]);
+        prv<VX>(A.f[F_PX + a], rv, a, pt[a], g, pm[a]);
+    }
+    // Q~ next values used: xx +x; yy +y; zz +z; xy +x, +y; xz +x, +z; yz +y, +z
+    const int axes[6][2] = {{0, 0}, {1, 1}, {2, 2}, {0, 1}, {0, 2}, {1, 2}};
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        ldv<VX>(A.f[F_QXX + k], i, q[k]);
+        nxt<VX>(A.f[F_QXX + k], rv, axes[k][0], q[k], g, qn[k][axes[k][0]]);
+        if (axes[k][1] != axes[k][0]) nxt<VX>(A.f[F_QXX + k], rv, axes[k][1], q[k], g, qn[k][axes[k][1]]);
+    }
+#pragma unroll
+    for (int e = 0; e < VX; ++e) {
+        bool hp[3], hm[3];
+        flags<VX>(rv, e, g, hp, hm);
+        // u_bar adjoint = sigma (D+)^T P~
+        float ub = 0.0f;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) ub += dplus_tv(pt[a][e], pm[a][e], hm[a], hp[a]);
+        ubo[e] = sigma * ub;
+        // v_bar adjoint = sigma eps_B^T Q~ - sigma P~
+        //   x: B_x^T Q_xx + 1/2 B_y^T Q_xy + 1/2 B_z^T Q_xz, etc.
+        auto bq = [&](int k, int a) { return bm_tv(q[k][e], qn[k][a][e], hp[a]); };
+        const float ex = bq(0, 0) + 0.5f * bq(3, 1) + 0.5f * bq(4, 2);
+        const float ey = bq(1, 1) + 0.5f * bq(3, 0) + 0.5f * bq(5, 2);
+        const float ez = bq(2, 2) + 0.5f * bq(4, 0) + 0.5f * bq(5, 1);
+        vbo[0][e] = sigma * ex - sigma * pt[0][e];
+        vbo[1][e] = sigma * ey - sigma * pt[1][e];
+        vbo[2][e] = sigma * ez - sigma * pt[2][e];
+    }
 }
 
+template <int VX>
 __global__ __launch_bounds__(AB) void adjoint_phase_c(Fields17 A, Geometry g, float sigma) {
-    const uint32_t st[3] = {1u, uint32_t(g.w), uint32_t(g.w) * uint32_t(g.h)};
-    const int ext[3] = {g.w, g.h, g.d};
-    const RowVoxel rv(g);
+    const RowVox<VX> rv(g);
     if (rv.in) {
-        const int c3[3] = {rv.x, rv.y, rv.z};
-        float ub, vb[3];
-        phase_c_voxel(A, rv.i, c3, st, ext, sigma, ub, vb);
-        A.f[F_UBAR][rv.i] = ub;
-        A.f[F_VBX][rv.i] = vb[0];
-        A.f[F_VBY][rv.i] = vb[1];
-        A.f[F_VBZ][rv.i] = vb[2];
+        float ub[VX], vb[3][VX], pt[3][VX];
+        phase_c_vox<VX>(A, rv, g, sigma, ub, vb, pt);
+        stv<VX>(A.f[F_UBAR], rv.i, ub);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) stv<VX>(A.f[F_VBX + c], rv.i, vb[c]);
     }
 }
 
 // Phase C of iteration k and phase A of iteration k - 1 (u_next: u of x_k).
+template <int VX>
 __global__ __launch_bounds__(AB) void adjoint_phase_ca(Fields17 A, const float* __restrict__ f,
                                                        const float* __restrict__ u_next,
                                                        float* __restrict__ ast,
@@ -287,16 +395,47 @@ __global__ __launch_bounds__(AB) void adjoint_phase_ca(Fields17 A, const float* 
                                                        float tau, float sigma, float lambda,
                                                        float theta, double* __restrict__ slots) {
     const uint32_t n = uint32_t(g.n());
-    const uint32_t st[3] = {1u, uint32_t(g.w), uint32_t(g.w) * uint32_t(g.h)};
-    const int ext[3] = {g.w, g.h, g.d};
     double acc[1] = {0.0};
-    const RowVoxel rv(g);
+    const RowVox<VX> rv(g);
     if (rv.in) {
-        const int c3[3] = {rv.x, rv.y, rv.z};
-        float ub, vb[3];
-        phase_c_voxel(A, rv.i, c3, st, ext, sigma, ub, vb);
-        phase_a_voxel(A, rv.i, n, ub, vb, f, u_next, ast, wst, pout, grad_f, tau, tau * lambda,
-                      theta, acc[0]);
+        const uint32_t i = rv.i;
+        float ub[VX], vb[3][VX], pt[3][VX];
+        phase_c_vox<VX>(A, rv, g, sigma, ub, vb, pt);
+        // phase A of iteration k - 1 on the same voxels (phase_a_voxel's arithmetic)
+        const float tl = tau * lambda;
+        float au[VX], fv[VX], un_[VX], gf[VX], av[3][VX];
+        ldv<VX>(A.f[F_U], i, au);
+        ldv<VX>(f, i, fv);
+        ldv<VX>(u_next, i, un_);
+        ldv<VX>(grad_f, i, gf);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) ldv<VX>(A.f[F_VX + c], i, av[c]);
+        float auo[VX], aso[VX], gfo[VX], avo[3][VX], wso[3][VX], po[3][VX];
+#pragma unroll
+        for (int e = 0; e < VX; ++e) {
+            const float un = au[e] + (1.0f + theta) * ub[e];
+            const float as = un / (1.0f + tl);
+            auo[e] = -theta * ub[e] + as;
+            aso[e] = as;
+            gfo[e] = gf[e] + tl / (1.0f + tl) * un;
+            acc[0] += double(un) * double(tau) * (double(fv[e]) - double(un_[e])) / (1.0 + double(tl));
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const float vn = av[c][e] + (1.0f + theta) * vb[c][e];
+                avo[c][e] = -theta * vb[c][e] + vn;
+                wso[c][e] = -tau * vn;
+                po[c][e] = pt[c][e] + tau * vn;
+            }
+        }
+        stv<VX>(A.f[F_U], i, auo);
+        stv<VX>(ast, i, aso);
+        stv<VX>(grad_f, i, gfo);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            stv<VX>(A.f[F_VX + c], i, avo[c]);
+            stv<VX>(wst + c * n, i, wso[c]);
+            stv<VX>(pout + c * n, i, po[c]);
+        }
     }
     block_accumulate<1, 2>(acc, slots);   // slot 2: lambda
 }
@@ -318,11 +457,14 @@ __global__ void adjoint_finish_f(const float* __restrict__ au, const float* __re
         grad_f[i] += au[i] + aub[i];
 }
 
-// threads per block: the row length rounded up to whole waves, at most AB
-inline int row_threads(const Geometry& g) { return std::min(AB, (g.w + 63) / 64 * 64); }
-inline int row_blocks(const Geometry& g) {
-    const int t = row_threads(g);
-    return ((g.w + t - 1) / t) * g.h * g.d;
+// threads per block: a row's VX-voxel lanes rounded up to whole waves, at most
+// AB; blocks: the row pieces of all rows (VX = 1 gives the most)
+inline int row_threads(const Geometry& g, int vx = 1) {
+    return std::min(AB, (g.w / vx + 63) / 64 * 64);
+}
+inline int row_blocks(const Geometry& g, int vx = 1) {
+    const int t = row_threads(g, vx), lanes = g.w / vx;
+    return ((lanes + t - 1) / t) * g.h * g.d;
 }
 
 inline int blocks_for(size_t n) {
@@ -351,25 +493,42 @@ hipError_t launch_adjoint_a(hipStream_t s, const AdjointArgs& a, const float* u_
     return hipGetLastError();
 }
 
+int adjoint_vx(Geometry g, int want) {
+    if (want == 1) return 1;
+    return g.w % 2 == 0 ? 2 : 1;
+}
+
 hipError_t launch_adjoint_b(hipStream_t s, const AdjointArgs& a, void* const* xk) {
     const Fields17 A = to_fields(a.adj), S = to_fields(xk);
-    hipLaunchKernelGGL(adjoint_phase_b, dim3(row_blocks(a.g)), dim3(row_threads(a.g)), 0, s, A, S,
-                       a.pbuf, a.ast, a.wst, a.g, a.tau, a.sigma, a.alpha0, a.alpha1, a.slots);
+    const dim3 grid(row_blocks(a.g, a.vx)), block(row_threads(a.g, a.vx));
+    if (a.vx == 2)
+        hipLaunchKernelGGL(adjoint_phase_b<2>, grid, block, 0, s, A, S, a.pbuf, a.ast, a.wst, a.g,
+                           a.tau, a.sigma, a.alpha0, a.alpha1, a.slots);
+    else
+        hipLaunchKernelGGL(adjoint_phase_b<1>, grid, block, 0, s, A, S, a.pbuf, a.ast, a.wst, a.g,
+                           a.tau, a.sigma, a.alpha0, a.alpha1, a.slots);
     return hipGetLastError();
 }
 
 hipError_t launch_adjoint_ca(hipStream_t s, const AdjointArgs& a, const float* u_next) {
     const Fields17 A = to_fields(a.adj);
-    hipLaunchKernelGGL(adjoint_phase_ca, dim3(row_blocks(a.g)), dim3(row_threads(a.g)), 0, s, A,
-                       a.f, u_next, a.ast, a.wst, a.pbuf, a.grad_f, a.g, a.tau, a.sigma, a.lambda,
-                       a.theta, a.slots);
+    const dim3 grid(row_blocks(a.g, a.vx)), block(row_threads(a.g, a.vx));
+    if (a.vx == 2)
+        hipLaunchKernelGGL(adjoint_phase_ca<2>, grid, block, 0, s, A, a.f, u_next, a.ast, a.wst,
+                           a.pbuf, a.grad_f, a.g, a.tau, a.sigma, a.lambda, a.theta, a.slots);
+    else
+        hipLaunchKernelGGL(adjoint_phase_ca<1>, grid, block, 0, s, A, a.f, u_next, a.ast, a.wst,
+                           a.pbuf, a.grad_f, a.g, a.tau, a.sigma, a.lambda, a.theta, a.slots);
     return hipGetLastError();
 }
 
 hipError_t launch_adjoint_c(hipStream_t s, const AdjointArgs& a) {
     const Fields17 A = to_fields(a.adj);
-    hipLaunchKernelGGL(adjoint_phase_c, dim3(row_blocks(a.g)), dim3(row_threads(a.g)), 0, s, A,
-                       a.g, a.sigma);
+    const dim3 grid(row_blocks(a.g, a.vx)), block(row_threads(a.g, a.vx));
+    if (a.vx == 2)
+        hipLaunchKernelGGL(adjoint_phase_c<2>, grid, block, 0, s, A, a.g, a.sigma);
+    else
+        hipLaunchKernelGGL(adjoint_phase_c<1>, grid, block, 0, s, A, a.g, a.sigma);
     return hipGetLastError();
 }
 

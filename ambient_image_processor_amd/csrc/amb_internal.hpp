@@ -119,6 +119,7 @@ struct PlanOverrides {
     int tgvk_zchunk = 0;
     int tgvk_fused = -1;         // 0: no fused TGV-k sweep
     int tgvk_fused_tx = 0, tgvk_fused_ty = 0, tgvk_fused_zchunk = 0;
+    int adjoint_vx = 0;          // 1: one voxel per thread in the reverse phases
 };
 
 // 64-bit offsets: volumes of 2^30 voxels and more (a 4-byte field's byte
@@ -327,7 +328,11 @@ struct AdjointArgs {
     double* slots;
     Geometry g;
     float tau, sigma, theta, lambda, alpha0, alpha1;
+    int vx;   // voxels per thread of phases B and C (adjoint_vx)
 };
+// Voxels per thread of the reverse phases: 2 (paired 8-byte loads) when the
+// width is even, else 1; want = 1 (plan override adjoint_vx) forces 1.
+int adjoint_vx(Geometry g, int want);
 // Iteration k in reverse: launch_adjoint_a (phase A of the last iteration only,
 // u_next = u of x_N), then per k: launch_adjoint_b (xk = x_k), then
 // launch_adjoint_ca (phase C of k with phase A of k - 1, u_next = u of x_k) or,

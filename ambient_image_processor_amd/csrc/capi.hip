@@ -1311,6 +1311,7 @@ AMB_API amb_status amb_context_set_plan_override(amb_context* c, const char* nam
         {"tile", &PlanOverrides::tile}, {"zchunk", &PlanOverrides::zchunk},
         {"sweep_pf", &PlanOverrides::sweep_pf}, {"fused", &PlanOverrides::fused},
         {"fused_vx", &PlanOverrides::fused_vx}, {"fused_ty", &PlanOverrides::fused_ty},
+        {"adjoint_vx", &PlanOverrides::adjoint_vx},
         {"fused_zchunk", &PlanOverrides::fused_zchunk}, {"fused_tx", &PlanOverrides::fused_tx},
         {"fused_pf", &PlanOverrides::fused_pf}, {"fused_image", &PlanOverrides::fused_image},
         {"fused_image_tx", &PlanOverrides::fused_image_tx},
@@ -2011,7 +2012,8 @@ AMB_API amb_status amb_tgv2_denoise_adjoint(amb_context* c, amb_memory mem, cons
     // two sweeps after a copy of the state
     const bool fb = c->fused_ok && fused_bars_supported(c->storage, g, c->sp, c->fplan);
     const AdjointArgs aa{adj, pbuf, ast, wst, gradf, c->f, partials, g, c->sp.tau, c->sp.sigma,
-                         c->sp.theta, c->sp.lambda, c->sp.alpha0, c->sp.alpha1};
+                         c->sp.theta, c->sp.lambda, c->sp.alpha0, c->sp.alpha1,
+                         adjoint_vx(g, c->ov.adjoint_vx)};
     // iteration k in reverse: phase B with x_k, then phase C of k with phase A
     // of k - 1 (u of x_k); phase A alone for the last iteration, C alone for k = 0
     bool first = true;

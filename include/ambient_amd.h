@@ -136,6 +136,7 @@ AMB_API void* amb_context_stream(amb_context* ctx);
  *   fused_image (0 = off), fused_image_tx, fused_image_zchunk   the 2-D row march
  *   index64 (1 = the 64-bit-offset kernel forms on any volume)
  *   tgvk_zchunk, tgvk_fused (0 = off), tgvk_fused_tx, tgvk_fused_ty, tgvk_fused_zchunk
+ *   adjoint_vx (1 = one voxel per thread in the adjoint's reverse phases)
  * An unknown name returns AMB_ERR_INVALID_ARGUMENT. */
 AMB_API amb_status amb_context_set_plan_override(amb_context* ctx, const char* name,
                                                  int32_t value);

@@ -87,6 +87,27 @@ def test_adjoint_recompute_forms_agree(ctx, shape, form):
         np.testing.assert_array_equal(x, y)
 
 
+@pytest.mark.parametrize("shape", [(8, 12, 20), (6, 9, 130), (5, 7, 600)])
+def test_adjoint_two_voxels_per_thread_equals_one(ctx, shape):
+    """On even widths the reverse phases run two voxels per thread (paired
+    loads); the plan override adjoint_vx = 1 forces one.  Each voxel's
+    arithmetic is the same, so u and dL/df agree bit for bit; the scalar
+    gradients are FP64 sums over differently sized blocks and agree to FP64
+    rounding.  Widths: one row piece, a row piece with idle lanes, several
+    row pieces per row."""
+    from ambient_image_processor_amd import native
+    f, clean = _vol(shape, 9)
+    p = _params(native, 13, 0.05, 0.02, 1.0)
+    g = np.random.default_rng(3).standard_normal(f.shape).astype(np.float32)
+    _say("adjoint voxels per thread", shape)
+    a = ctx.tgv2_denoise_adjoint(f, p, g, checkpoint_every=5)
+    ctx.set_override("adjoint_vx", 1)
+    b = ctx.tgv2_denoise_adjoint(f, p, g, checkpoint_every=5)
+    np.testing.assert_array_equal(a[0], b[0])
+    np.testing.assert_array_equal(a[1], b[1])
+    np.testing.assert_allclose(a[2], b[2], rtol=1e-12, atol=1e-15)
+
+
 @pytest.mark.parametrize("alphas", [(2.0, 1.0), (0.02, 0.01)])
 def test_adjoint_against_central_differences(ctx, orc, alphas):
     """32^3 volume, 200 iterations, checkpoints every 20, MSE loss to the clean
