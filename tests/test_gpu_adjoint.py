@@ -32,11 +32,12 @@ def _params(native, iters, alpha0, alpha1, lam):
                                  theta=1.0, iterations=iters, data_term=native.DATA_L2)
 
 
+@pytest.mark.parametrize("shape", [(10, 14, 18), (9, 11, 19)])   # two / one voxel per thread
 @pytest.mark.parametrize("alphas", [(2.0, 1.0), (0.02, 0.01)])
-def test_adjoint_matches_fp64_cpu_adjoint(ctx, alphas):
+def test_adjoint_matches_fp64_cpu_adjoint(ctx, alphas, shape):
     from ambient_image_processor_amd import native
     from oracle import adjoint as adj
-    f, clean = _vol((10, 14, 18), 5)
+    f, clean = _vol(shape, 5)
     a0, a1 = alphas
     p = _params(native, 40, a0, a1, 1.0)
     n = f.size
