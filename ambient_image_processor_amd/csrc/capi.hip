@@ -1958,8 +1958,16 @@ AMB_API amb_status amb_tgv2_denoise_adjoint(amb_context* c, amb_memory mem, cons
     const size_t fstride = align_up(n * 4, 256);
     const size_t state_bytes = fstride * N_FIELDS;
     const uint32_t nseg = N ? (N + K - 1) / K : 0;
+    // the reverse pass rebuilds each segment with the fused sweep's form that
+    // also stores u_bar and v_bar, out of place, one launch per iteration (else
+    // in place with the two sweeps after a copy of the state).  With a checkpoint
+    // at every iteration (K = 1) that form keeps every state instead: the
+    // forward runs as those launches into N + 1 slots and nothing is recomputed.
+    const bool fb = c->fused_ok && fused_bars_supported(c->storage, g, c->sp, c->fplan);
+    const bool resident = fb && K == 1 && N > 0;
+    const size_t n_ckpt = resident ? 1 : nseg, n_slots = resident ? size_t(N) + 1 : size_t(K) + 1;
     const size_t part_bytes = align_up(adjoint_slots_size(g) * sizeof(double), 256);
-    const size_t need = state_bytes * (size_t(nseg) + (K + 1) + 1) + fstride * 8 + part_bytes + 256;
+    const size_t need = state_bytes * (n_ckpt + n_slots + 1) + fstride * 8 + part_bytes + 256;
     if (need > c->adj_cap) {
         if (c->adj_block) AMB_HIP(hipFree(c->adj_block));
         c->adj_block = nullptr;
@@ -1972,8 +1980,8 @@ AMB_API amb_status amb_tgv2_denoise_adjoint(amb_context* c, amb_memory mem, cons
         for (int k = 0; k < N_FIELDS; ++k) tab[k] = blk + size_t(k) * fstride;
     };
     char* ckpt = base;
-    char* slots = ckpt + state_bytes * nseg;
-    char* adjs = slots + state_bytes * (K + 1);
+    char* slots = ckpt + state_bytes * n_ckpt;
+    char* adjs = slots + state_bytes * n_slots;
     float* ast = reinterpret_cast<float*>(adjs + state_bytes);
     float* wst = reinterpret_cast<float*>(adjs + state_bytes + fstride);   // 3 fields
     float* gradf = reinterpret_cast<float*>(adjs + state_bytes + fstride * 4);
@@ -1992,14 +2000,39 @@ AMB_API amb_status amb_tgv2_denoise_adjoint(amb_context* c, amb_memory mem, cons
         uint32_t v;
         ~RestoreInterval() { c->energy_interval = v; }
     } restore{c, every};
-    for (uint32_t seg = 0; seg < nseg; ++seg) {
-        char* cp = ckpt + state_bytes * seg;
+    void* tabs[2][N_FIELDS];
+    // slots[j] = u, u_bar, v, v_bar of x_{k0 + j} with p, q of the dual half of
+    // iteration k0 + j + 1: slot 0 from x_{k0} (at cp) by one dual sweep, slot j
+    // from slot j - 1 by one fused launch (primal half k0 + j, dual half k0 + j + 1)
+    auto fill_slots = [&](const char* cp, uint32_t L) -> amb_status {
+        AMB_HIP(hipMemcpyAsync(slots, cp, state_bytes, hipMemcpyDeviceToDevice, c->stream));
+        state_at(slots, tabs[0]);
+        AMB_HIP(launch_dual_sweep(c->stream, 0, tabs[0], g, c->sp));
+        for (uint32_t j = 1; j <= L; ++j) {
+            state_at(slots + state_bytes * size_t(j - 1), tabs[0]);
+            state_at(slots + state_bytes * size_t(j), tabs[1]);
+            AMB_HIP(launch_fused_sweep_bars(c->stream, tabs[0], tabs[1], c->f, g, c->sp, c->fplan));
+        }
+        return AMB_OK;
+    };
+    if (resident) {
         for (int k = 0; k < N_FIELDS; ++k)
-            AMB_HIP(hipMemcpyAsync(cp + size_t(k) * fstride, c->fields[k], n * 4,
+            AMB_HIP(hipMemcpyAsync(ckpt + size_t(k) * fstride, c->fields[k], n * 4,
                                    hipMemcpyDeviceToDevice, c->stream));
-        AMB_TRY(run(c, std::min(K, N - seg * K), 0, nullptr, nullptr, false));
+        AMB_TRY(fill_slots(ckpt, N));
+        if (u_out)
+            AMB_HIP(hipMemcpyAsync(u_out, slots + state_bytes * size_t(N) + size_t(F_U) * fstride,
+                                   n * 4, kind_out(mem), c->stream));
+    } else {
+        for (uint32_t seg = 0; seg < nseg; ++seg) {
+            char* cp = ckpt + state_bytes * seg;
+            for (int k = 0; k < N_FIELDS; ++k)
+                AMB_HIP(hipMemcpyAsync(cp + size_t(k) * fstride, c->fields[k], n * 4,
+                                       hipMemcpyDeviceToDevice, c->stream));
+            AMB_TRY(run(c, std::min(K, N - seg * K), 0, nullptr, nullptr, false));
+        }
+        if (u_out) AMB_TRY(copy_field_out(c, mem, u_out, F_U));
     }
-    if (u_out) AMB_TRY(copy_field_out(c, mem, u_out, F_U));
     // reverse: adjoint of x_N = (dL/du, 0, ...)
     void* adj[N_FIELDS];
     state_at(adjs, adj);
@@ -2007,10 +2040,6 @@ AMB_API amb_status amb_tgv2_denoise_adjoint(amb_context* c, amb_memory mem, cons
     AMB_HIP(hipMemcpyAsync(adj[F_U], dL_du, n * 4, kind_in(mem), c->stream));
     AMB_HIP(launch_zero_fill(c->stream, gradf, n * 4));
     AMB_HIP(launch_zero_fill(c->stream, partials, adjoint_slots_size(g) * sizeof(double)));
-    // recompute of a segment: with the fused sweep's form that also stores u_bar
-    // and v_bar, out of place, one launch per iteration; else in place with the
-    // two sweeps after a copy of the state
-    const bool fb = c->fused_ok && fused_bars_supported(c->storage, g, c->sp, c->fplan);
     const AdjointArgs aa{adj, pbuf, ast, wst, gradf, c->f, partials, g, c->sp.tau, c->sp.sigma,
                          c->sp.theta, c->sp.lambda, c->sp.alpha0, c->sp.alpha1,
                          adjoint_vx(g, c->ov.adjoint_vx)};
@@ -2027,36 +2056,30 @@ AMB_API amb_status amb_tgv2_denoise_adjoint(amb_context* c, amb_memory mem, cons
         else AMB_HIP(launch_adjoint_c(c->stream, aa));
         return AMB_OK;
     };
-    for (int seg = int(nseg) - 1; seg >= 0; --seg) {
+    // x_{k0 + j} from the filled slots: u_bar, v_bar (and u, v) from slot j, p and
+    // q from slot j - 1 (the checkpoint at cp for j = 0); u of x_{k0 + j + 1}
+    // from slot j + 1
+    auto reverse_slots = [&](char* cp, uint32_t k0, uint32_t L) -> amb_status {
+        for (int j = int(L) - 1; j >= 0; --j) {
+            state_at(slots + state_bytes * size_t(j), tabs[0]);
+            char* pq = j > 0 ? slots + state_bytes * size_t(j - 1) : cp;
+            for (int k = F_PX; k <= F_QYZ; ++k) tabs[0][k] = pq + size_t(k) * fstride;
+            state_at(slots + state_bytes * size_t(j + 1), tabs[1]);
+            AMB_TRY(reverse_step(k0 + uint32_t(j), tabs[0], tabs[1]));
+        }
+        return AMB_OK;
+    };
+    if (resident) AMB_TRY(reverse_slots(ckpt, 0, N));
+    for (int seg = resident ? -1 : int(nseg) - 1; seg >= 0; --seg) {
         const uint32_t k0 = uint32_t(seg) * K, k1 = std::min(k0 + K, N), L = k1 - k0;
         char* cp = ckpt + state_bytes * size_t(seg);
-        void* tabs[2][N_FIELDS];
-        AMB_HIP(hipMemcpyAsync(slots, cp, state_bytes, hipMemcpyDeviceToDevice, c->stream));
         if (fb) {
-            // slots[j] = u, u_bar, v, v_bar of x_{k0 + j} with p, q of the dual half
-            // of iteration k0 + j + 1: slot 0 from x_{k0} by one dual sweep, slot j
-            // from slot j - 1 by one fused launch (primal half k0 + j, dual half
-            // k0 + j + 1)
-            state_at(slots, tabs[0]);
-            AMB_HIP(launch_dual_sweep(c->stream, 0, tabs[0], g, c->sp));
-            for (uint32_t j = 1; j <= L; ++j) {
-                state_at(slots + state_bytes * size_t(j - 1), tabs[0]);
-                state_at(slots + state_bytes * size_t(j), tabs[1]);
-                AMB_HIP(launch_fused_sweep_bars(c->stream, tabs[0], tabs[1], c->f, g, c->sp,
-                                                c->fplan));
-            }
-            // x_{k0 + j}: u_bar, v_bar (and u, v) from slot j, p and q from slot
-            // j - 1 (the checkpoint for j = 0); u of x_{k0 + j + 1} from slot j + 1
-            for (int j = int(L) - 1; j >= 0; --j) {
-                state_at(slots + state_bytes * size_t(j), tabs[0]);
-                char* pq = j > 0 ? slots + state_bytes * size_t(j - 1) : cp;
-                for (int k = F_PX; k <= F_QYZ; ++k) tabs[0][k] = pq + size_t(k) * fstride;
-                state_at(slots + state_bytes * size_t(j + 1), tabs[1]);
-                AMB_TRY(reverse_step(k0 + uint32_t(j), tabs[0], tabs[1]));
-            }
+            AMB_TRY(fill_slots(cp, L));
+            AMB_TRY(reverse_slots(cp, k0, L));
             continue;
         }
         // slots[0] = x_{k0}; slots[j] = x_{k0 + j} recomputed with the same sweeps
+        AMB_HIP(hipMemcpyAsync(slots, cp, state_bytes, hipMemcpyDeviceToDevice, c->stream));
         for (uint32_t j = 1; j <= L; ++j) {
             char* dst = slots + state_bytes * j;
             AMB_HIP(hipMemcpyAsync(dst, dst - state_bytes, state_bytes, hipMemcpyDeviceToDevice,

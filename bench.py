@@ -704,7 +704,8 @@ def adjoint_4x128(args, emit=True):
     # warm-up: one forward + adjoint of the first volume (buffers, tables, code objects)
     ctx.upload(f_dev, vols[0])
     ctx.upload(g_dev, np.zeros(shape, np.float32))
-    ctx.tgv2_denoise_adjoint_device(f_dev, shape, p, g_dev, u_dev, gf_dev, w["checkpoint_every"])
+    ckpt = getattr(args, "checkpoint_every", 0) or w["checkpoint_every"]
+    ctx.tgv2_denoise_adjoint_device(f_dev, shape, p, g_dev, u_dev, gf_dev, ckpt)
     ctx.synchronize()
     for b in range(vols.shape[0]):
         ctx.upload(f_dev, vols[b])
@@ -719,8 +720,7 @@ def adjoint_4x128(args, emit=True):
         ctx.upload(g_dev, ((2.0 / (vols.shape[0] * n)) * (u - cleans[b])).astype(np.float32))
         ctx.synchronize()
         t0 = time.perf_counter()
-        ctx.tgv2_denoise_adjoint_device(f_dev, shape, p, g_dev, u_dev, gf_dev,
-                                        w["checkpoint_every"])
+        ctx.tgv2_denoise_adjoint_device(f_dev, shape, p, g_dev, u_dev, gf_dev, ckpt)
         t_adj += time.perf_counter() - t0
         print(f"volume {b} done", file=sys.stderr, flush=True)
     vi = float(n) * iters * vols.shape[0]
@@ -731,7 +731,7 @@ def adjoint_4x128(args, emit=True):
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic (12 seeded boxes + N(0,0.05^2) per volume)",
         "config": {"workload": "tgv2_l2_adjoint_4x128^3", "iterations": iters,
-                   "checkpoint_every": w["checkpoint_every"]},
+                   "checkpoint_every": ckpt},
         "forward_only_voxel_iters_per_s": vi / t_fwd,
         "adjoint_over_forward_time": t_adj / t_fwd}
     if emit:
@@ -883,6 +883,9 @@ def main():
                                            "masked_256", "planar_4096"],
                     default="deshade_256")
     ap.add_argument("--copy-calibration", action="store_true")
+    ap.add_argument("--checkpoint-every", type=int, default=0,
+                    help="adjoint workload: checkpoint interval (default: the configuration's 20; "
+                         "1 keeps every state)")
     ap.add_argument("--plan", action="append", default=[], metavar="NAME=VALUE",
                     help="launch-plan override (amb_context_set_plan_override), repeatable")
     args = ap.parse_args()

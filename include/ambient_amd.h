@@ -299,9 +299,10 @@ AMB_API amb_status amb_tgv2_energy_noalloc(const float* f, const void* const* fi
  * iterations (0 = 20), then the hand-written reverse sweeps, recomputing each
  * segment from its checkpoint.  Given dL/du for the final u, returns dL/df and
  * dL_dparams = (dL/dalpha0, dL/dalpha1, dL/dlambda) (FP64 reductions).  u_out
- * (may be NULL) receives the forward result.  Workspace: (ceil(iterations /
- * checkpoint_every) + checkpoint_every + 2) * 17 + 8 volumes of FP32.  The
- * context's iteration state after the call is not specified. */
+ * (may be NULL) receives the forward result.  checkpoint_every = 1 keeps every
+ * state (where the fused sweep runs) and recomputes nothing.  Workspace:
+ * (ceil(iterations / checkpoint_every) + checkpoint_every + 2) * 17 + 8 volumes
+ * of FP32.  The context's iteration state after the call is not specified. */
 AMB_API amb_status amb_tgv2_denoise_adjoint(amb_context* ctx, amb_memory mem, const float* f,
                                             uint32_t width, uint32_t height, uint32_t depth,
                                             const amb_tgv_params* params, const float* dL_du,
