@@ -2029,9 +2029,20 @@ AMB_API amb_status amb_tgv2_denoise_adjoint(amb_context* c, amb_memory mem, cons
             for (int k = 0; k < N_FIELDS; ++k)
                 AMB_HIP(hipMemcpyAsync(cp + size_t(k) * fstride, c->fields[k], n * 4,
                                        hipMemcpyDeviceToDevice, c->stream));
-            AMB_TRY(run(c, std::min(K, N - seg * K), 0, nullptr, nullptr, false));
+            const uint32_t L = std::min(K, N - seg * K);
+            if (fb && seg + 1 == nseg) {
+                // the last segment straight into the slots: the reverse pass starts
+                // with it, so it needs no recompute
+                AMB_TRY(fill_slots(cp, L));
+                if (u_out)
+                    AMB_HIP(hipMemcpyAsync(u_out, slots + state_bytes * size_t(L) +
+                                                      size_t(F_U) * fstride,
+                                           n * 4, kind_out(mem), c->stream));
+            } else {
+                AMB_TRY(run(c, L, 0, nullptr, nullptr, false));
+            }
         }
-        if (u_out) AMB_TRY(copy_field_out(c, mem, u_out, F_U));
+        if (u_out && (!fb || nseg == 0)) AMB_TRY(copy_field_out(c, mem, u_out, F_U));
     }
     // reverse: adjoint of x_N = (dL/du, 0, ...)
     void* adj[N_FIELDS];
@@ -2074,7 +2085,7 @@ AMB_API amb_status amb_tgv2_denoise_adjoint(amb_context* c, amb_memory mem, cons
         const uint32_t k0 = uint32_t(seg) * K, k1 = std::min(k0 + K, N), L = k1 - k0;
         char* cp = ckpt + state_bytes * size_t(seg);
         if (fb) {
-            AMB_TRY(fill_slots(cp, L));
+            if (seg + 1 < int(nseg)) AMB_TRY(fill_slots(cp, L));   // the last one is filled
             AMB_TRY(reverse_slots(cp, k0, L));
             continue;
         }

@@ -53,6 +53,18 @@ def test_adjoint_matches_fp64_cpu_adjoint(ctx, alphas, shape):
         assert abs(gp[k] - ref[name]) <= 2e-3 * max(abs(ref[name]), 1e-9 + 1e-3 * np.abs(ref["f"]).sum()), name
 
 
+@pytest.mark.parametrize("k", [1, 20])
+def test_adjoint_of_zero_iterations_is_the_identity(ctx, k):
+    """No iterations: u = f, dL/df = dL/du, no parameter gradient."""
+    from ambient_image_processor_amd import native
+    f, _ = _vol((6, 8, 10), 4)
+    g = np.random.default_rng(5).standard_normal(f.shape).astype(np.float32)
+    u, gf, gp = ctx.tgv2_denoise_adjoint(f, _params(native, 0, 2.0, 1.0, 1.0), g, checkpoint_every=k)
+    np.testing.assert_array_equal(u, f)
+    np.testing.assert_array_equal(gf, g)
+    np.testing.assert_array_equal(gp, np.zeros(3))
+
+
 def test_adjoint_independent_of_checkpoint_interval(ctx):
     from ambient_image_processor_amd import native
     f, clean = _vol((8, 12, 20), 6)
