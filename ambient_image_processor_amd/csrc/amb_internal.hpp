@@ -67,15 +67,19 @@ struct StoreBF16 {
     template <class Idx>
     static __device__ __forceinline__ void st(uint16_t* p, Idx i, float v) { *elem(p, i) = rne(v); }
     static __device__ __forceinline__ float rt(float v) { return __uint_as_float(uint32_t(rne(v)) << 16); }
-    // rt of VX values (VX even) in pairs, one packed conversion per pair
+    // rt of VX values, in pairs (one packed conversion per pair) when VX is even
     template <int VX>
     static __device__ __forceinline__ void rt_all(float (&a)[VX]) {
-        static_assert(VX % 2 == 0, "pairs");
+        if constexpr (VX == 1) {
+            a[0] = rt(a[0]);
+        } else {
+            static_assert(VX % 2 == 0, "pairs");
 #pragma unroll
-        for (int e = 0; e < VX; e += 2) {
-            const uint32_t pk = rne2(a[e], a[e + 1]);
-            a[e] = __uint_as_float(pk << 16);
-            a[e + 1] = __uint_as_float(pk & 0xffff0000u);
+            for (int e = 0; e < VX; e += 2) {
+                const uint32_t pk = rne2(a[e], a[e + 1]);
+                a[e] = __uint_as_float(pk << 16);
+                a[e + 1] = __uint_as_float(pk & 0xffff0000u);
+            }
         }
     }
 };

@@ -420,6 +420,18 @@ struct VecIO<StoreBF16, 4> {
     }
 };
 
+template <class S>
+struct VecIO<S, 1> {   // one voxel per lane (odd widths)
+    template <class I>
+    static __device__ __forceinline__ void ld(const typename S::T* p, I i, float (&o)[1]) {
+        o[0] = S::ld(p, i);
+    }
+    template <class I>
+    static __device__ __forceinline__ void st(typename S::T* p, I i, const float (&v)[1]) {
+        S::st(p, i, v[0]);
+    }
+};
+
 template <>
 struct VecIO<StoreF32, 2> {
     template <class I>
@@ -452,7 +464,8 @@ struct VecIO<StoreBF16, 2> {
 template <int VX, class I>
 __device__ __forceinline__ uint32_t load_active(const uint8_t* act, I i) {
     if constexpr (VX == 4) return *reinterpret_cast<const uint32_t*>(elem(act, i));
-    else return uint32_t(*reinterpret_cast<const uint16_t*>(elem(act, i)));
+    else if constexpr (VX == 2) return uint32_t(*reinterpret_cast<const uint16_t*>(elem(act, i)));
+    else return uint32_t(*elem(act, i));
 }
 
 template <int VX>
@@ -1701,6 +1714,13 @@ hipError_t fused_pf(hipStream_t s, const FusedFields& ff, const float* f, Geomet
                                     : fused_launch<S, I, 4, 0, MK, 512, 0, 1, 0>(s, ff, f, g, p, plan);
         return fused_shape<S, I, MK, 0>(s, ff, f, g, p, plan);
     } else {
+        if (plan.vx == 1) {   // odd widths: one voxel per thread, 1024-thread blocks
+            if (plan.tx > 0)
+                return p.data_term == 1 ? fused_launch<S, I, 1, 1, MK, 1024, 0, 1>(s, ff, f, g, p, plan)
+                                        : fused_launch<S, I, 1, 0, MK, 1024, 0, 1>(s, ff, f, g, p, plan);
+            return p.data_term == 1 ? fused_launch<S, I, 1, 1, MK, 1024, 0>(s, ff, f, g, p, plan)
+                                    : fused_launch<S, I, 1, 0, MK, 1024, 0>(s, ff, f, g, p, plan);
+        }
         if constexpr (std::is_same<S, StoreF32>::value) {
             if (p.planar_zero_z && plan.vx == 2) {   // the 2-D loop: z fields taken as +0
                 const bool pf = plan.pf && (plan.tx > 0 ? (plan.tx / 2 + 2) : g.w / 2) *
@@ -1809,18 +1829,26 @@ bool fused_plan(const Geometry& g, int storage, int device, const PlanOverrides&
             return true;
         }
     }
-    for (int vx : {2, 4}) {
+    // odd widths: one voxel per thread (the two- and four-voxel forms need
+    // lanes that lie wholly inside the row), whole rows only while bands of two
+    // rows fit (four rows of lanes): bands of one row read each row three times
+    // (257^3 per launch: 755 us, tiles of 126 columns 549 us; 255^3 with bands
+    // of two rows 431 us, of one row 662 us, tiles 542 us)
+    for (int vx : {1, 2, 4}) {
         if (want_vx && vx != want_vx) continue;
         if (want_tx > 0) break;   // x tiles forced
-        if (g.w % vx != 0 || (vx == 2 && big_index)) continue;
+        if (g.w % vx != 0 || (vx <= 2 && big_index) || (vx == 1 && g.w % 2 == 0)) continue;
+        if (vx == 1 && 4 * g.w > 1024 && !want_ty) continue;
         const int L = g.w / vx;
         const int max_threads = vx == 4 ? 512 : 1024;
         const int waves_simd_cap = vx == 4 ? 2 : 4;
         for (int rt = 3; rt * L <= max_threads && rt - 2 <= g.h + 1; ++rt) {
             const int ty = rt - 2;
             if (want_ty && ty != want_ty) continue;
-            const size_t lds = (vx == 4 ? fused_lds_floats<4>(g.w, ty)
-                                        : fused_lds_floats<2>(g.w, ty)) * sizeof(float);
+            const size_t lds = (vx == 4   ? fused_lds_floats<4>(g.w, ty)
+                                : vx == 2 ? fused_lds_floats<2>(g.w, ty)
+                                          : fused_lds_floats<1>(g.w, ty)) *
+                               sizeof(float);
             if (lds > size_t(max_lds)) break;
             const int waves_simd = ((rt * L + 63) / 64 + 3) / 4;
             const int per_cu = std::max(
@@ -1908,6 +1936,31 @@ bool fused_plan(const Geometry& g, int storage, int device, const PlanOverrides&
         *plan = bp;
         return true;
     }
+    if (bp.ty == 0 && !big_index && g.w % 2 != 0 && g.d > 1 && (!want_vx || want_vx == 1)) {
+        // odd rows too wide for whole-row bands: tiles of 126 columns (128
+        // one-voxel lanes with a halo lane on each side) and bands of up to 6
+        // rows in 1024-thread blocks, no prefetch; the cost model picks the z
+        // chunk.  Not for images (depth 1): nothing along z amortises the halo
+        // rows there (2047^2 per iteration: FP32 158 us, the two sweeps 161 us;
+        // BF16 127 us, the two sweeps 102 us)
+        const int tx = want_tx > 0 ? want_tx : 126;
+        const int lanes = tx + 2, ty = std::min(want_ty ? want_ty : 6, g.h);
+        if (tx >= g.w || lanes * (ty + 2) > 1024) return false;
+        const size_t lds = fused_lds_floats<1>(tx + 2, ty) * sizeof(float);
+        if (lds > size_t(max_lds)) return false;
+        const long nb = long((g.w + tx - 1) / tx) * ((g.h + ty - 1) / ty);
+        for (int nc = 1; nc <= g.d; ++nc) {
+            const int zc = want_zc ? want_zc : (g.d + nc - 1) / nc;
+            const long blocks = nb * ((g.d + zc - 1) / zc);
+            const long rounds = (blocks + cus - 1) / cus;
+            const double cost = double(rounds) * (ty + 1.0) * (zc + 1);
+            if (cost < best) { best = cost; bp = FusedPlan{ty, zc, 0, 1, tx}; }
+            if (want_zc) break;
+        }
+        if (bp.ty == 0) return false;
+        *plan = bp;
+        return true;
+    }
     if (bp.ty == 0) return false;
     // FP32 fields, two voxels per thread, at most 768 threads: the next plane's
     // loads issued one step ahead fit the 168 VGPRs of three waves per SIMD
@@ -1915,6 +1968,7 @@ bool fused_plan(const Geometry& g, int storage, int device, const PlanOverrides&
     // slower, 528 vs 316 us)
     if (storage == 0 && bp.vx == 2 && !big_index && (g.w / 2) * (bp.ty + 2) <= 768) bp.pf = 1;
     if (want_pf == 0 || want_pf == 1) bp.pf = want_pf;
+    if (bp.vx == 1) bp.pf = 0;   // the one-voxel form has no prefetch variant
     *plan = bp;
     return true;
 }
@@ -1988,7 +2042,9 @@ hipError_t launch_tile_activity(hipStream_t s, const uint8_t* act, Geometry g,
 
 size_t fused_lds_bytes(const Geometry& g, const FusedPlan& plan) {
     const int row = plan.tx > 0 ? plan.tx + 2 * plan.vx : g.w;
-    return (plan.vx == 2 ? fused_lds_floats<2>(row, plan.ty) : fused_lds_floats<4>(row, plan.ty)) *
+    return (plan.vx == 1   ? fused_lds_floats<1>(row, plan.ty)
+            : plan.vx == 2 ? fused_lds_floats<2>(row, plan.ty)
+                           : fused_lds_floats<4>(row, plan.ty)) *
            sizeof(float);
 }
 

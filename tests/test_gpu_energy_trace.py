@@ -45,7 +45,8 @@ def _close(a, b, scale):
 
 
 # (D, H, W): whole-row bands, x tiles (rows wider than 510), odd tile and band
-# edges; widths odd -> no fused sweep (the standalone pass every time)
+# edges; widths odd -> the one-voxel fused sweep, which has no energy form (the
+# iterations with an energy row run the two sweeps and the standalone pass)
 SHAPES = [(20, 24, 32), (9, 13, 130), (6, 11, 600), (7, 10, 33)]
 
 

@@ -7,7 +7,7 @@ sweeps (plan override fused = 0) and with the FP32 oracle.  Cases cover band and
 z-chunk edges (voxels per thread, band heights, chunk lengths and prefetch
 forced through the overrides fused_vx / _ty / _zchunk / _pf, including
 1024-thread blocks),
-partial bands,
+partial bands, odd widths (one voxel per thread, whole rows and x tiles),
 depth 1, L1/L2, BF16 storage, the masked loop, callbacks (which split the
 fused schedule) and continuing from a state the fused loop left in the
 second buffer copy."""
@@ -48,8 +48,10 @@ def _profile_counts(ctx, fn):
     return out, prof
 
 
+# odd widths run one voxel per thread: whole rows up to 341 columns, x tiles of
+# 126 columns beyond (401: three full tiles and one of 23 columns)
 SHAPES = [(9, 13, 32), (17, 20, 64), (6, 7, 128), (5, 11, 256), (3, 4, 8), (1, 12, 36),
-          (20, 3, 16)]
+          (20, 3, 16), (9, 13, 21), (6, 5, 1), (1, 7, 33), (4, 9, 401)]
 
 
 @pytest.mark.parametrize("shape", SHAPES)
@@ -78,7 +80,8 @@ def test_fused_equals_two_sweeps_and_oracle(ctx, orc, shape, data_term, monkeypa
     ("4", "4", "7", "0", (1, 9, 64)), ("4", "4", "64", "1", (70, 13, 256)),
     ("2", "6", "4", "0", (9, 23, 256)),      # 128 x 8 = 1024 threads per block
     ("2", "1", "1", "0", (7, 9, 30)), ("2", "3", "5", "0", (6, 13, 66)),
-    ("2", "2", "3", "0", (5, 4, 8))])
+    ("2", "2", "3", "0", (5, 4, 8)),
+    ("1", "2", "3", "0", (7, 9, 31)), ("1", "5", "4", "0", (5, 6, 255))])
 def test_fused_forced_plans(ctx, monkeypatch, vx, ty, zchunk, pf, shape):
     from ambient_image_processor_amd import native
     _say("fused plan vx", vx, "ty", ty, "zchunk", zchunk, "pf", pf, shape)
@@ -95,10 +98,10 @@ def test_fused_forced_plans(ctx, monkeypatch, vx, ty, zchunk, pf, shape):
         np.testing.assert_array_equal(a, b)
 
 
-@pytest.mark.parametrize("vx", ["2", "4"])
+@pytest.mark.parametrize("vx", ["1", "2", "4"])
 def test_fused_masked_and_bf16_per_width(ctx, monkeypatch, vx):
     from ambient_image_processor_amd import native
-    shape = (9, 11, 48)
+    shape = (9, 11, 47) if vx == "1" else (9, 11, 48)
     f = _f(shape, 12)
     m = (np.random.default_rng(13).random(shape) > 0.4).astype(np.float32)
     for storage in (native.STORAGE_F32, native.STORAGE_BF16):
@@ -194,7 +197,8 @@ def test_fused_state_continues_and_energy(ctx, monkeypatch):
 # ---------------------------------------------------------------------------
 TILE_CASES = [((9, 13, 64), "16", "2", "3"), ((9, 13, 64), "30", "5", "0"),
               ((5, 20, 130), "62", "0", "2"), ((7, 9, 34), "8", "1", "1"),
-              ((6, 7, 40), "38", "3", "4")]
+              ((6, 7, 40), "38", "3", "4"), ((7, 9, 35), "8", "1", "1"),
+              ((6, 7, 41), "15", "3", "4"), ((5, 20, 131), "63", "0", "2")]
 
 
 @pytest.mark.parametrize("shape,tile,band,zchunk", TILE_CASES)
