@@ -1937,23 +1937,33 @@ bool fused_plan(const Geometry& g, int storage, int device, const PlanOverrides&
         return true;
     }
     if (bp.ty == 0 && !big_index && g.w % 2 != 0 && g.d > 1 && (!want_vx || want_vx == 1)) {
-        // odd rows too wide for whole-row bands: tiles of 126 columns (128
-        // one-voxel lanes with a halo lane on each side) and bands of up to 6
-        // rows in 1024-thread blocks, no prefetch; the cost model picks the z
-        // chunk.  Not for images (depth 1): nothing along z amortises the halo
+        // odd rows too wide for whole-row bands: x tiles of 126 columns (128
+        // one-voxel lanes with a halo lane on each side) and bands of up to 4
+        // rows, 768-thread blocks (the kernel's ~70 VGPRs fit two per CU), no
+        // prefetch; the cost model picks the z chunk.  Measured per launch
+        // (FP32 / BF16 us; the best of 62, 94, 126 columns x 2..8 rows; the two
+        // sweeps): 257^3 553 / 374 (522 / 328; 853 / 676); 128x256x511 529 /
+        // 311 (489 / 311; 766 / 495); 200x200x383 473 / 298 (best; 733 / 512);
+        // 100x300x765 685 / 449 (best; 1061 / 755); 48x128x2047 369 / 221 (340 /
+        // 221; 554 / 379); 64x300x301 209 / 129 (182 / 118; 258 / 121).
+        // Choosing the width by the fewest lanes over the row did worse (up to
+        // 1.26x).  Not for images (depth 1): nothing along z amortises the halo
         // rows there (2047^2 per iteration: FP32 158 us, the two sweeps 161 us;
-        // BF16 127 us, the two sweeps 102 us)
+        // BF16 127 us, the two sweeps 102 us).
         const int tx = want_tx > 0 ? want_tx : 126;
-        const int lanes = tx + 2, ty = std::min(want_ty ? want_ty : 6, g.h);
+        const int lanes = tx + 2, ty = std::min(want_ty ? want_ty : 4, g.h);
         if (tx >= g.w || lanes * (ty + 2) > 1024) return false;
         const size_t lds = fused_lds_floats<1>(tx + 2, ty) * sizeof(float);
         if (lds > size_t(max_lds)) return false;
+        const int waves_simd = ((lanes * (ty + 2) + 63) / 64 + 3) / 4;
+        const int per_cu = std::max(1, std::min(int((160 * 1024) / lds), 7 / waves_simd));
+        const long conc = long(cus) * per_cu;
         const long nb = long((g.w + tx - 1) / tx) * ((g.h + ty - 1) / ty);
         for (int nc = 1; nc <= g.d; ++nc) {
             const int zc = want_zc ? want_zc : (g.d + nc - 1) / nc;
             const long blocks = nb * ((g.d + zc - 1) / zc);
-            const long rounds = (blocks + cus - 1) / cus;
-            const double cost = double(rounds) * (ty + 1.0) * (zc + 1);
+            const long rounds = (blocks + conc - 1) / conc;
+            const double cost = double(rounds) * per_cu * (ty + 1.0) * (zc + 1);
             if (cost < best) { best = cost; bp = FusedPlan{ty, zc, 0, 1, tx}; }
             if (want_zc) break;
         }
