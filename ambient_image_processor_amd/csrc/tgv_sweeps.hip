@@ -2113,10 +2113,12 @@ hipError_t launch_fused_sweep_bars(hipStream_t s, void* const* src, void* const*
 // Plan: blocks of one row of two-voxel lanes, a whole number of waves: tiles of
 // 124 columns (64 lanes with the halo lane on each side; images up to 2048
 // wide) or 508 columns (256 lanes), and chunks of image rows for about 24
-// blocks per CU, 4 to 64 rows.  Measured per launch: 1024^2 26-28 us (tiled
-// depth-1 bands 38, two sweeps 43); 2048^2 79-91 us; 4096^2 284 us with 508
-// columns (380: 285, 252: 302, 224: 291, 1020: 322; 512, i.e. 258 lanes: 390;
-// tiled depth-1 bands 454, two sweeps 545).
+// blocks per CU, 4 to 6 rows.  Measured per launch: 1024^2 26-28 us (tiled
+// depth-1 bands 38, two sweeps 43); 2048^2 73 us (best of 252..1020 columns x
+// 4..64 rows: 71); 4096^2 261 us with 508 columns x 6 rows (the best; 380:
+// 267, 252: 271, 1020: 268; 12 rows 277; 512, i.e. 258 lanes: 390; tiled
+// depth-1 bands 454, two sweeps 545); 8192^2 1000 us with 6 rows (the best;
+// 22 rows, 24 blocks per CU: 1123).
 bool fused_image_plan(const Geometry& g, int device, const PlanOverrides& ov, FusedPlan* plan) {
     if (g.d != 1 || g.w % 2 != 0 || needs_index64(g, ov)) return false;
     if (ov.fused_image == 0) return false;
@@ -2129,7 +2131,7 @@ bool fused_image_plan(const Geometry& g, int device, const PlanOverrides& ov, Fu
     if (want_tx) tx = ov.fused_image_tx >= g.w ? 0 : ov.fused_image_tx;
     if (tx < 0 || tx % 2 != 0 || (tx == 0 && g.w / 2 > 768)) return false;
     const long nbx = tx ? (g.w + tx - 1) / tx : 1;
-    int zc = int(std::min(64L, std::max(4L, long(g.h) * nbx / (24L * cus))));
+    int zc = int(std::min(6L, std::max(4L, long(g.h) * nbx / (24L * cus))));
     zc = std::min(zc, g.h);
     if (ov.fused_image_zchunk > 0) zc = std::min(ov.fused_image_zchunk, g.h);
     *plan = FusedPlan{1, zc, 1, 2, tx};
