@@ -218,7 +218,7 @@ extern const int FUSED_FIELD_MAP[N_FUSED_FIELDS];
 // The fused sweep out of place with u_bar and v_bar stored as well: src and dst
 // are full 17-field tables (state layout); dst receives u, u_bar, v, v_bar of the
 // primal half and p, q of the next dual half.  FP32 fields, 32-bit offsets, the
-// 3-D loop without a mask, the L2 data term and a two-voxel plan
+// 3-D loop without a mask, the L2 data term and a two- or one-voxel plan
 // (fused_bars_supported); for the adjoint's segment recompute.
 bool fused_bars_supported(int storage, Geometry g, const SweepParams& p, const FusedPlan& plan);
 // The fused sweep divides |p| / alpha1 and |q| / alpha0 through host

@@ -1769,12 +1769,17 @@ hipError_t fused_dispatch(hipStream_t s, const FusedFields& ff, const float* f, 
 
 // The SB form (u_bar / v_bar stored too) for the adjoint's recompute: FP32
 // fields, 32-bit offsets, the 3-D loop, no mask, the L2 data term, two voxels
-// per thread; whole rows or x tiles and the prefetch as the plan says.
+// per thread (one on odd widths); whole rows or x tiles and the prefetch as the
+// plan says.
 hipError_t fused_bars_dispatch(hipStream_t s, const FusedFields& ff, const float* f, Geometry g,
                                const SweepParams& p, const FusedPlan& plan) {
     typedef StoreF32 S;
     typedef uint32_t I;
-    if (plan.vx != 2 || p.data_term != 1) return hipErrorInvalidValue;
+    if (p.data_term != 1) return hipErrorInvalidValue;
+    if (plan.vx == 1)
+        return plan.tx > 0 ? fused_launch<S, I, 1, 1, 0, 1024, 0, 1, 0, 0, 1>(s, ff, f, g, p, plan)
+                           : fused_launch<S, I, 1, 1, 0, 1024, 0, 0, 0, 0, 1>(s, ff, f, g, p, plan);
+    if (plan.vx != 2) return hipErrorInvalidValue;
     const int lanes = plan.tx > 0 ? plan.tx / 2 + 2 : g.w / 2;
     const bool pf = plan.pf && lanes * (plan.ty + 2) <= 768;
     if (plan.tx > 0)
@@ -2089,7 +2094,7 @@ bool fused_divisors_ok(float alpha0, float alpha1) {
 
 bool fused_bars_supported(int storage, Geometry g, const SweepParams& p, const FusedPlan& plan) {
     return storage == 0 && !p.index64 && !p.active && !p.planar && !p.planar_zero_z &&
-           p.data_term == 1 && plan.vx == 2 && g.d > 1;
+           p.data_term == 1 && (plan.vx == 2 || plan.vx == 1) && g.d > 1;
 }
 
 hipError_t launch_fused_sweep_bars(hipStream_t s, void* const* src, void* const* dst,

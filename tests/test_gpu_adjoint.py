@@ -65,9 +65,10 @@ def test_adjoint_of_zero_iterations_is_the_identity(ctx, k):
     np.testing.assert_array_equal(gp, np.zeros(3))
 
 
-def test_adjoint_independent_of_checkpoint_interval(ctx):
+@pytest.mark.parametrize("shape", [(8, 12, 20), (7, 9, 21)])   # two / one voxel per thread
+def test_adjoint_independent_of_checkpoint_interval(ctx, shape):
     from ambient_image_processor_amd import native
-    f, clean = _vol((8, 12, 20), 6)
+    f, clean = _vol(shape, 6)
     p = _params(native, 23, 0.05, 0.02, 1.0)
     g = np.random.default_rng(1).standard_normal(f.shape).astype(np.float32)
     outs = [ctx.tgv2_denoise_adjoint(f, p, g, checkpoint_every=k) for k in (1, 5, 23, 40)]
@@ -79,12 +80,14 @@ def test_adjoint_independent_of_checkpoint_interval(ctx):
 
 @pytest.mark.parametrize("shape,form", [
     ((8, 12, 20), {}), ((8, 12, 20), {"fused_pf": 0}), ((7, 10, 40), {"fused_tx": 12}),
-    ((7, 10, 40), {"fused_tx": 12, "fused_pf": 0}), ((6, 9, 34), {"fused_ty": 3})])
+    ((7, 10, 40), {"fused_tx": 12, "fused_pf": 0}), ((6, 9, 34), {"fused_ty": 3}),
+    ((7, 10, 41), {}), ((5, 6, 301), {}), ((6, 7, 45), {"fused_tx": 13})])
 def test_adjoint_recompute_forms_agree(ctx, shape, form):
     """The reverse pass recomputes each segment with the fused sweep out of
     place (u_bar, v_bar stored too); with the fused sweep off it copies the
     state and runs the two sweeps in place.  Both give the same states, so u,
-    dL/df and the scalar gradients agree bit for bit."""
+    dL/df and the scalar gradients agree bit for bit.  Odd widths take the
+    one-voxel form: whole rows (41), 126-column tiles (301), forced tiles (45)."""
     from ambient_image_processor_amd import native
     f, clean = _vol(shape, 8)
     p = _params(native, 17, 0.05, 0.02, 1.0)
