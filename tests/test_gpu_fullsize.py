@@ -1,6 +1,7 @@
 """GPU tests at the benchmark configurations' full sizes, through checks whose
 meaning does not depend on the size: the first iterations against the FP32
-oracle bit for bit (TGV2 and TGV-k), schedule and masking equivalences bit for
+oracle bit for bit (TGV2, TGV-k, and the one-voxel fused sweep at odd sizes
+next to the benchmark's 256^3), schedule and masking equivalences bit for
 bit, the dual feasibility and the decrease of the FP64 primal-dual gap over the
 iterations,
 the Poisson solve against its closed form, and the adjoint against central
@@ -136,6 +137,40 @@ def test_384_bf16_storage_schedules_agree(ctx, monkeypatch):
     two = ctx.tgv2_deshade(w["f"], p)
     for a, b in zip(fused, two):
         np.testing.assert_array_equal(a, b)
+
+
+@pytest.mark.parametrize("n", [255, 257])
+def test_odd_width_fused_sweep_full_size(ctx, orc, n):
+    """A sphere phantom at 255^3 (one voxel per thread, whole rows) and 257^3
+    (126-column x tiles): three FP32 iterations equal the FP32 oracle, and over
+    6 iterations the fused sweep equals the two-sweep loop bit for bit, FP32
+    and BF16 fields."""
+    from ambient_image_processor_amd import native, phantoms
+    f = phantoms.deshade_phantom((n, n, n), seed=2)[0]
+    tau = float(np.float32(1 / np.sqrt(20)))
+    p3 = native.default_params(lam=0.5, alpha0=2.0, alpha1=1.0, tau=tau, sigma=tau, theta=1.0,
+                               iterations=3, data_term=native.DATA_L1)
+    u, vx, vy, vz = ctx.tgv2_deshade(f, p3)
+    st = orc.tgv2(f, 3, lam=0.5, alpha0=2.0, alpha1=1.0, tau=tau, sigma=tau, data_term="l1",
+                  dtype=np.float32)
+    np.testing.assert_array_equal(u, st.u)
+    np.testing.assert_array_equal(vx, st.v[0])
+    for storage in (native.STORAGE_F32, native.STORAGE_BF16):
+        _say("odd width", n, "storage", storage)
+        p = native.default_params(lam=0.5, alpha0=2.0, alpha1=1.0, tau=tau, sigma=tau, theta=1.0,
+                                  iterations=6, data_term=native.DATA_L1, storage=storage)
+        ctx.clear_overrides()
+        ctx.profile(True)
+        ctx.profile_reset()
+        fused = ctx.tgv2_deshade(f, p)
+        prof = ctx.profile_read()
+        ctx.profile(False)
+        assert prof["fused_sweep"][1] == 5, prof
+        ctx.set_override("fused", 0)
+        two = ctx.tgv2_deshade(f, p)
+        ctx.clear_overrides()
+        for a, b in zip(fused, two):
+            np.testing.assert_array_equal(a, b)
 
 
 def test_adjoint_128_against_central_differences(ctx, orc):
