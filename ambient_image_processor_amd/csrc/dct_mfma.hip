@@ -114,26 +114,29 @@ constexpr int PYK = BN + 16;
 static_assert(BK * PYK == BN * PF32, "column-pass FP32 Y tile reuses the Y array");
 __device__ __forceinline__ int yswz(int j) { return ((j >> 4) & 3) << 3; }
 
-// Staging.  Each thread moves one 16-float piece of the data tile and 16
-// table entries per K chunk; the global loads of chunk c + 1 are issued into
+// Staging.  A block has NT threads (256 for F32, 512 for BF16x3); each moves
+// NQ = 1024 / NT rows of 4 data values and 16 (F32) or 8 (BF16, hi and lo)
+// table entries per K chunk.  The global loads of chunk c + 1 are issued into
 // registers (DataRegs / TableRegs) before the MFMAs of chunk c and written to
 // LDS after them.
+template <int NT>
 struct DataRegs {
-    float v[4][4];    // [piece row][4 consecutive elements]
+    float v[1024 / NT][4];   // [piece row][4 consecutive elements]
 };
 struct TableRegs {
     float f[4][4];    // FP32 table: 4 x float4
-    i16x8 h[2], l[2]; // BF16 table: 2 x 8 hi / lo
+    i16x8 h[2], l[2]; // BF16 table: up to 2 x 8 hi / lo
 };
 
 // Data operand, K contiguous in memory (row pass X = in[r][m]): thread e
-// loads rows r0 + (e / 8) + 32 q (q < 4), columns m0 + 4 (e % 8) .. + 4.
-__device__ __forceinline__ void load_data_kc(DataRegs& R, const float* src, int rows, int r0,
+// loads rows r0 + (e / 8) + (NT / 8) q (q < NQ), columns m0 + 4 (e % 8) .. + 4.
+template <int NT>
+__device__ __forceinline__ void load_data_kc(DataRegs<NT>& R, const float* src, int rows, int r0,
                                              int m0, int K) {
     const int e = threadIdx.x, c4 = (e % 8) * 4;
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        const int rg = r0 + e / 8 + 32 * q, m = m0 + c4;
+    for (int q = 0; q < 1024 / NT; ++q) {
+        const int rg = r0 + e / 8 + (NT / 8) * q, m = m0 + c4;
         if (rg < rows && m + 3 < K && (K & 3) == 0) {
             const float4 t = *reinterpret_cast<const float4*>(src + size_t(rg) * K + m);
             R.v[q][0] = t.x; R.v[q][1] = t.y; R.v[q][2] = t.z; R.v[q][3] = t.w;
@@ -146,13 +149,15 @@ __device__ __forceinline__ void load_data_kc(DataRegs& R, const float* src, int 
 }
 
 // Data operand, N contiguous in memory (column pass Y = in[m][s]): thread e
-// loads rows m0 + 4 (e / 32) .. + 4 at columns j0 + 4 (e % 32) .. + 4.
-__device__ __forceinline__ void load_data_nc(DataRegs& R, const float* src, int N, int K, int j0,
-                                             int m0) {
-    const int e = threadIdx.x, k4 = (e / 32) * 4, j = j0 + (e % 32) * 4;
+// loads rows m0 + NQ (e / 32) .. + NQ at columns j0 + 4 (e % 32) .. + 4.
+template <int NT>
+__device__ __forceinline__ void load_data_nc(DataRegs<NT>& R, const float* src, int N, int K,
+                                             int j0, int m0) {
+    constexpr int NQ = 1024 / NT;
+    const int e = threadIdx.x, kq = (e / 32) * NQ, j = j0 + (e % 32) * 4;
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        const int m = m0 + k4 + q;
+    for (int q = 0; q < NQ; ++q) {
+        const int m = m0 + kq + q;
         if (m < K && j + 3 < N && (N & 3) == 0) {
             const float4 t = *reinterpret_cast<const float4*>(src + size_t(m) * N + j);
             R.v[q][0] = t.x; R.v[q][1] = t.y; R.v[q][2] = t.z; R.v[q][3] = t.w;
@@ -163,10 +168,10 @@ __device__ __forceinline__ void load_data_nc(DataRegs& R, const float* src, int 
     }
 }
 
-// Table [k][m] (K contiguous): FP32 thread e loads rows t0 + e / 8 + 32 q,
-// columns m0 + 4 (e % 8); BF16 thread e loads rows t0 + e / 4 + 64 q,
-// columns m0 + 8 (e % 4).
-template <int PREC>
+// Table [k][m] (K contiguous): FP32 (256 threads) thread e loads rows
+// t0 + e / 8 + 32 q, columns m0 + 4 (e % 8); BF16 thread e loads rows
+// t0 + e / 4 + (NT / 4) q (q < 512 / NT), columns m0 + 8 (e % 4).
+template <int PREC, int NT>
 __device__ __forceinline__ void load_table(TableRegs& T, const GemmArgs& a, int t0, int nrows,
                                            int m0) {
     const int K = a.K, e = threadIdx.x;
@@ -187,8 +192,8 @@ __device__ __forceinline__ void load_table(TableRegs& T, const GemmArgs& a, int 
     } else {
         const int c8 = (e % 4) * 8;
 #pragma unroll
-        for (int q = 0; q < 2; ++q) {
-            const int tr = t0 + e / 4 + 64 * q, m = m0 + c8;
+        for (int q = 0; q < 512 / NT; ++q) {
+            const int tr = t0 + e / 4 + (NT / 4) * q, m = m0 + c8;
             if (tr < nrows && m + 7 < K && (K & 7) == 0) {
                 T.h[q] = *reinterpret_cast<const i16x8*>(a.thi + size_t(tr) * K + m);
                 T.l[q] = *reinterpret_cast<const i16x8*>(a.tlo + size_t(tr) * K + m);
@@ -229,33 +234,49 @@ __device__ __forceinline__ void put4(GemmLds<PREC>& L, bool isx, int r, int k, c
     }
 }
 
-template <int PREC>
-__device__ __forceinline__ void store_data_kc(GemmLds<PREC>& L, const DataRegs& R) {
+template <int PREC, int NT>
+__device__ __forceinline__ void store_data_kc(GemmLds<PREC>& L, const DataRegs<NT>& R) {
     const int e = threadIdx.x, c4 = (e % 8) * 4;
 #pragma unroll
-    for (int q = 0; q < 4; ++q) put4<PREC>(L, true, e / 8 + 32 * q, c4, R.v[q]);
+    for (int q = 0; q < 1024 / NT; ++q) put4<PREC>(L, true, e / 8 + (NT / 8) * q, c4, R.v[q]);
 }
 
-// column pass: FP32 rows k as loaded; BF16 transposed (4 x 4 piece) and swizzled
-template <int PREC>
-__device__ __forceinline__ void store_data_nc(GemmLds<PREC>& L, const DataRegs& R) {
-    const int e = threadIdx.x, k4 = (e / 32) * 4, j = (e % 32) * 4;
+// 2 floats -> packed bf16 hi and lo (32 bits each)
+__device__ __forceinline__ void split2(float x0, float x1, uint32_t& hi, uint32_t& lo) {
+    hi = StoreBF16::rne2(x0, x1);
+    lo = StoreBF16::rne2(x0 - __uint_as_float(hi << 16), x1 - __uint_as_float(hi & 0xffff0000u));
+}
+
+// column pass: FP32 rows k as loaded; BF16 transposed (NQ x 4 piece) and swizzled
+template <int PREC, int NT>
+__device__ __forceinline__ void store_data_nc(GemmLds<PREC>& L, const DataRegs<NT>& R) {
+    constexpr int NQ = 1024 / NT;
+    const int e = threadIdx.x, k4 = (e / 32) * NQ, j = (e % 32) * 4;
     if constexpr (PREC == 1) {
+        static_assert(NT == 256, "the F32 column pass stages 4 x 4 pieces");
         float* y = &L.y[0][0];
 #pragma unroll
         for (int q = 0; q < 4; ++q)
             *reinterpret_cast<float4*>(y + (k4 + q) * PYK + j) =
                 make_float4(R.v[q][0], R.v[q][1], R.v[q][2], R.v[q][3]);
-    } else {
+    } else if constexpr (NQ == 4) {
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
             const float x[4] = {R.v[0][c], R.v[1][c], R.v[2][c], R.v[3][c]};
             put4<PREC>(L, false, j + c, k4 ^ yswz(j + c), x);
         }
+    } else {   // 2 k values per row (the swizzle moves whole 8-element groups)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            uint32_t h, l;
+            split2(R.v[0][c], R.v[1][c], h, l);
+            *reinterpret_cast<uint32_t*>(&L.yh[j + c][k4 ^ yswz(j + c)]) = h;
+            *reinterpret_cast<uint32_t*>(&L.yl[j + c][k4 ^ yswz(j + c)]) = l;
+        }
     }
 }
 
-template <int PREC>
+template <int PREC, int NT>
 __device__ __forceinline__ void store_table(GemmLds<PREC>& L, bool isx, const TableRegs& T) {
     const int e = threadIdx.x;
     if constexpr (PREC == 1) {
@@ -268,16 +289,25 @@ __device__ __forceinline__ void store_table(GemmLds<PREC>& L, bool isx, const Ta
     } else {
         const int c8 = (e % 4) * 8;
 #pragma unroll
-        for (int q = 0; q < 2; ++q) {
-            const int r = e / 4 + 64 * q;
+        for (int q = 0; q < 512 / NT; ++q) {
+            const int r = e / 4 + (NT / 4) * q;
             *reinterpret_cast<i16x8*>(isx ? &L.xh[r][c8] : &L.yh[r][c8]) = T.h[q];
             *reinterpret_cast<i16x8*>(isx ? &L.xl[r][c8] : &L.yl[r][c8]) = T.l[q];
         }
     }
 }
 
+// Threads per block: F32 4 waves of 64 x 64 outputs; BF16x3 8 waves of 32 x 64
+// (half the accumulators per wave, so twice the waves per SIMD to hide the
+// loads' latency).
+template <int PREC>
+constexpr int gemm_threads() { return PREC == 2 ? 512 : 256; }
+
 template <int PREC, int ROW>
-__global__ __launch_bounds__(256) void dct_gemm_kernel(GemmArgs a) {
+__global__ __launch_bounds__(gemm_threads<PREC>()) void dct_gemm_kernel(GemmArgs a) {
+    constexpr int NT = gemm_threads<PREC>();
+    constexpr int WR = BM / (NT / 128);   // output rows per wave (64 or 32)
+    constexpr int TT = WR / 16;           // 16-row MFMA tiles per wave
     __shared__ __align__(16) GemmLds<PREC> L;
     const int M = a.M, N = a.N, K = a.K;
     const int tm = (M + BM - 1) / BM, tn = (N + BN - 1) / BN;
@@ -294,12 +324,12 @@ __global__ __launch_bounds__(256) void dct_gemm_kernel(GemmArgs a) {
     const int i0 = it * BM, j0 = jt * BN;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int li = lane & 15, lg = lane >> 4;
-    const int wi = (wave >> 1) * 64, wj = (wave & 1) * 64;
+    const int wi = (wave >> 1) * WR, wj = (wave & 1) * 64;
     const size_t obase = size_t(o) * size_t(M) * size_t(N);
 
-    f32x4 acc[4][4];
+    f32x4 acc[TT][4];
 #pragma unroll
-    for (int t = 0; t < 4; ++t)
+    for (int t = 0; t < TT; ++t)
 #pragma unroll
         for (int u = 0; u < 4; ++u) acc[t][u] = f32x4{0.f, 0.f, 0.f, 0.f};
 
@@ -308,24 +338,24 @@ __global__ __launch_bounds__(256) void dct_gemm_kernel(GemmArgs a) {
     // multiplied; BF16x3: one set (chunk c + 1), which keeps two waves per
     // SIMD (measured faster on MI355X at 512 x 512 x 256)
     constexpr int DEPTH = PREC == 1 ? 2 : 1;
-    DataRegs R0, R1;
+    DataRegs<NT> R0, R1;
     TableRegs T0, T1;
-    auto load = [&](DataRegs& R, TableRegs& T, int m0) {
+    auto load = [&](DataRegs<NT>& R, TableRegs& T, int m0) {
         if (ROW) {
-            load_data_kc(R, dsrc, M, i0, m0, K);
-            load_table<PREC>(T, a, j0, N, m0);
+            load_data_kc<NT>(R, dsrc, M, i0, m0, K);
+            load_table<PREC, NT>(T, a, j0, N, m0);
         } else {
-            load_table<PREC>(T, a, i0, M, m0);
-            load_data_nc(R, dsrc, N, K, j0, m0);
+            load_table<PREC, NT>(T, a, i0, M, m0);
+            load_data_nc<NT>(R, dsrc, N, K, j0, m0);
         }
     };
-    auto chunk = [&](DataRegs& R, TableRegs& T, int m0) {
+    auto chunk = [&](DataRegs<NT>& R, TableRegs& T, int m0) {
         if (ROW) {
-            store_data_kc<PREC>(L, R);
-            store_table<PREC>(L, false, T);
+            store_data_kc<PREC, NT>(L, R);
+            store_table<PREC, NT>(L, false, T);
         } else {
-            store_table<PREC>(L, true, T);
-            store_data_nc<PREC>(L, R);
+            store_table<PREC, NT>(L, true, T);
+            store_data_nc<PREC, NT>(L, R);
         }
         __syncthreads();
         if (m0 + DEPTH * BK < K) load(R, T, m0 + DEPTH * BK);
@@ -346,9 +376,9 @@ __global__ __launch_bounds__(256) void dct_gemm_kernel(GemmArgs a) {
                         acc[t][u] = __builtin_amdgcn_mfma_f32_16x16x4f32(xf[t], yf[u], acc[t][u], 0, 0, 0);
             }
         } else {
-            bf16x8 xh[4], xl[4], yh[4], yl[4];
+            bf16x8 xh[TT], xl[TT], yh[4], yl[4];
 #pragma unroll
-            for (int t = 0; t < 4; ++t) {
+            for (int t = 0; t < TT; ++t) {
                 xh[t] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const i16x8*>(&L.xh[wi + t * 16 + li][8 * lg]));
                 xl[t] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const i16x8*>(&L.xl[wi + t * 16 + li][8 * lg]));
             }
@@ -362,17 +392,17 @@ __global__ __launch_bounds__(256) void dct_gemm_kernel(GemmArgs a) {
             // each product over all 16 tiles before the next: consecutive MFMAs
             // are independent (same per-accumulator order hi*hi, hi*lo, lo*hi)
 #pragma unroll
-            for (int t = 0; t < 4; ++t)
+            for (int t = 0; t < TT; ++t)
 #pragma unroll
                 for (int u = 0; u < 4; ++u)
                     acc[t][u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xh[t], yh[u], acc[t][u], 0, 0, 0);
 #pragma unroll
-            for (int t = 0; t < 4; ++t)
+            for (int t = 0; t < TT; ++t)
 #pragma unroll
                 for (int u = 0; u < 4; ++u)
                     acc[t][u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xh[t], yl[u], acc[t][u], 0, 0, 0);
 #pragma unroll
-            for (int t = 0; t < 4; ++t)
+            for (int t = 0; t < TT; ++t)
 #pragma unroll
                 for (int u = 0; u < 4; ++u)
                     acc[t][u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xl[t], yh[u], acc[t][u], 0, 0, 0);
@@ -391,7 +421,7 @@ __global__ __launch_bounds__(256) void dct_gemm_kernel(GemmArgs a) {
     }
     // lane holds D rows 4*lg + r, column li of each 16 x 16 tile
 #pragma unroll
-    for (int t = 0; t < 4; ++t)
+    for (int t = 0; t < TT; ++t)
 #pragma unroll
         for (int u = 0; u < 4; ++u)
 #pragma unroll
@@ -433,7 +463,8 @@ hipError_t launch_dct_mfma_pass(hipStream_t s, const float* in, float* out, cons
     const long tm = (a.M + BM - 1) / BM, tn = (a.N + BN - 1) / BN;
     const long blocks = tm * tn * a.O;
     if (blocks > 0x7fffffffL) return hipErrorInvalidValue;
-    const dim3 grid{static_cast<unsigned>(blocks), 1, 1}, block{256, 1, 1};
+    const dim3 grid{static_cast<unsigned>(blocks), 1, 1};
+    const dim3 block{unsigned(precision == 2 ? gemm_threads<2>() : gemm_threads<1>()), 1, 1};
     if (precision == 2) {
         if (a.rowpass) hipLaunchKernelGGL((dct_gemm_kernel<2, 1>), grid, block, 0, s, a);
         else hipLaunchKernelGGL((dct_gemm_kernel<2, 0>), grid, block, 0, s, a);
