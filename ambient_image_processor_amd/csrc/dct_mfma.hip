@@ -335,8 +335,9 @@ __global__ __launch_bounds__(gemm_threads<PREC>()) void dct_gemm_kernel(GemmArgs
 
     const float* dsrc = ROW ? a.data : a.data + size_t(o) * size_t(K) * size_t(N);
     // FP32: two register sets, chunk c + 2 is loaded while chunk c is
-    // multiplied; BF16x3: one set (chunk c + 1), which keeps two waves per
-    // SIMD (measured faster on MI355X at 512 x 512 x 256)
+    // multiplied; BF16x3: one set (chunk c + 1), which keeps four waves per
+    // SIMD (a second set costs the column pass its second block; in the row
+    // pass alone it measured no gain at 512 x 512 x 256)
     constexpr int DEPTH = PREC == 1 ? 2 : 1;
     DataRegs<NT> R0, R1;
     TableRegs T0, T1;
