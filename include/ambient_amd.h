@@ -131,7 +131,8 @@ AMB_API void* amb_context_stream(amb_context* ctx);
  * They belong to the context and apply from the next call that begins an
  * iteration (deshade, denoise, begin, TGV-k); a value < 0 unsets one.  Names:
  *   tile (0-7), zchunk, sweep_pf (0/1)        the two-sweep kernels
- *   fused (0 = off), fused_vx (2/4), fused_ty, fused_zchunk, fused_tx, fused_pf
+ *   fused (0 = off), fused_vx (1 on odd widths, 2, 4), fused_ty, fused_zchunk,
+ *   fused_tx, fused_pf
  *                                             the fused TGV2 sweep
  *   fused_image (0 = off), fused_image_tx, fused_image_zchunk   the 2-D row march
  *   index64 (1 = the 64-bit-offset kernel forms on any volume)
