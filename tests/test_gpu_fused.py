@@ -259,6 +259,34 @@ def test_fused_x_tiles_bf16_masked_planar(ctx, orc, monkeypatch, tile):
     np.testing.assert_array_equal(tiled[2][0], st.u)
 
 
+def test_fused_odd_tiles_bf16_and_masked(ctx, orc):
+    """Odd rows wider than whole-row bands allow (263: three 126-column tiles,
+    the last of 11 columns), one voxel per thread: BF16 fields and the masked
+    loop equal the two-sweep loop bit for bit, and the masked FP32 run equals
+    the oracle."""
+    from ambient_image_processor_amd import native
+    f = _f((5, 9, 263), 55)
+    act = np.random.default_rng(56).random(f.shape) > 0.35
+    kw = dict(lam=0.7, alpha0=2.0, alpha1=1.0, tau=TAU, sigma=TAU, theta=1.0, iterations=11,
+              data_term=native.DATA_L1)
+    pb = native.default_params(storage=native.STORAGE_BF16, **kw)
+    pm = native.default_params(**kw)
+    ctx.profile(True)
+    ctx.profile_reset()
+    fused = [ctx.tgv2_deshade(f, pb), ctx.tgv2_deshade_masked(f, pm, act)]
+    prof = ctx.profile_read()
+    ctx.profile(False)
+    assert prof["fused_sweep"][1] == 20, prof
+    ctx.set_override("fused", 0)
+    two = [ctx.tgv2_deshade(f, pb), ctx.tgv2_deshade_masked(f, pm, act)]
+    for a, b in zip(fused, two):
+        for x, y in zip(a, b):
+            np.testing.assert_array_equal(x, y)
+    st = orc.tgv2(f, 11, lam=0.7, alpha0=2.0, alpha1=1.0, tau=TAU, sigma=TAU, data_term="l1",
+                  dtype=np.float32, mask=act.astype(np.float32))
+    np.testing.assert_array_equal(fused[1][0], st.u)
+
+
 @pytest.mark.parametrize("shape", [(3, 6, 1400), (4, 19, 760)])
 def test_fused_wide_rows_run_tiled(ctx, orc, shape):
     """Rows too wide for whole-row bands take x tiles of 124 columns and bands
