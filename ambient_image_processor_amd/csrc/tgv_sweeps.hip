@@ -1694,6 +1694,13 @@ hipError_t fused_energy_mk(hipStream_t s, const FusedFields& ff, const float* f,
                            const SweepParams& p, const FusedPlan& plan) {
     typedef uint32_t I;
     constexpr int T = EN_THREADS;
+    if (plan.vx == 1) {   // odd widths
+        if (plan.tx > 0)
+            return p.data_term == 1 ? fused_launch<S, I, 1, 1, MK, T, 0, 1, 0, 1>(s, ff, f, g, p, plan)
+                                    : fused_launch<S, I, 1, 0, MK, T, 0, 1, 0, 1>(s, ff, f, g, p, plan);
+        return p.data_term == 1 ? fused_launch<S, I, 1, 1, MK, T, 0, 0, 0, 1>(s, ff, f, g, p, plan)
+                                : fused_launch<S, I, 1, 0, MK, T, 0, 0, 0, 1>(s, ff, f, g, p, plan);
+    }
     if (plan.vx != 2) return hipErrorInvalidValue;
     if (plan.tx > 0)
         return p.data_term == 1 ? fused_launch<S, I, 2, 1, MK, T, 0, 1, 0, 1>(s, ff, f, g, p, plan)
@@ -2009,15 +2016,37 @@ bool fused_plan(const Geometry& g, int storage, int device, const PlanOverrides&
 }
 
 // The energy form's plan (results do not depend on the plan): two voxels per
-// thread, no prefetch, the iteration plan's z chunk; whole rows in bands of up
-// to the iteration plan's height while three rows fit EN_THREADS threads, else
-// x tiles of 124 columns (64 lanes) in bands of up to 6 rows.  None for 64-bit
-// offsets or an odd width.
+// thread (one on odd widths), no prefetch, the iteration plan's z chunk; whole
+// rows in bands of up to the iteration plan's height while three rows fit
+// EN_THREADS threads, else x tiles of 124 columns (64 lanes) in bands of up to
+// 6 rows (odd widths: 126 columns, 128 lanes, bands of 2 rows).  None for
+// 64-bit offsets.
 bool fused_energy_plan(const Geometry& g, int device, const PlanOverrides& ov,
                        const FusedPlan& base, FusedPlan* plan) {
-    if (needs_index64(g, ov) || base.ty < 1 || g.w % 2 != 0) return false;
+    if (needs_index64(g, ov) || base.ty < 1) return false;
     FusedPlan e = base;
     e.pf = 0;
+    int max_lds = 0;
+    if (hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) !=
+        hipSuccess)
+        return false;
+    if (g.w % 2 != 0) {
+        e.vx = 1;
+        if (base.tx == 0 && 3 * g.w <= EN_THREADS) {
+            e.tx = 0;
+            e.ty = std::min(base.ty, EN_THREADS / g.w - 2);
+        } else {
+            e.tx = 126;
+            e.ty = EN_THREADS / (e.tx + 2) - 2;
+            if (e.tx >= g.w) return false;
+        }
+        e.ty = std::min(e.ty, g.h);
+        if (e.ty < 1) return false;
+        const int row = e.tx > 0 ? e.tx + 2 : g.w;
+        if (fused_lds_floats<1>(row, e.ty, 1) * sizeof(float) > size_t(max_lds)) return false;
+        *plan = e;
+        return true;
+    }
     e.vx = 2;
     if (base.tx == 0 && 3 * (g.w / 2) <= EN_THREADS) {
         e.tx = 0;
@@ -2029,10 +2058,6 @@ bool fused_energy_plan(const Geometry& g, int device, const PlanOverrides& ov,
     }
     e.ty = std::min(e.ty, g.h);
     if (e.ty < 1) return false;
-    int max_lds = 0;
-    if (hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) !=
-        hipSuccess)
-        return false;
     const int row = e.tx > 0 ? e.tx + 4 : g.w;
     if (fused_lds_floats<2>(row, e.ty, 1) * sizeof(float) > size_t(max_lds)) return false;
     *plan = e;

@@ -45,9 +45,9 @@ def _close(a, b, scale):
 
 
 # (D, H, W): whole-row bands, x tiles (rows wider than 510), odd tile and band
-# edges; widths odd -> the one-voxel fused sweep, which has no energy form (the
-# iterations with an energy row run the two sweeps and the standalone pass)
-SHAPES = [(20, 24, 32), (9, 13, 130), (6, 11, 600), (7, 10, 33)]
+# edges; odd widths run the one-voxel energy form (whole rows up to 170, then
+# 126-column tiles)
+SHAPES = [(20, 24, 32), (9, 13, 130), (6, 11, 600), (7, 10, 33), (5, 9, 263)]
 
 
 @pytest.mark.parametrize("shape", SHAPES)
@@ -72,8 +72,8 @@ def test_trace_matches_standalone_pass(ctx, orc, shape, data_term, storage):
         np.testing.assert_array_equal(a, b)
     counts = [int(c) for c in trace[:, 0]]
     assert counts == [6, 12, 18, 24, 30, 36, 37]
-    if shape[2] % 2 == 0:   # the fused sweep ran, with its energy form for the rows
-        assert prof["fused_sweep_energy"][1] == len(counts) - 1, prof
+    # the fused sweep ran, with its energy form for the rows
+    assert prof["fused_sweep_energy"][1] == len(counts) - 1, prof
     # the standalone pass on the same states
     ctx.begin(f, _params(native, 0, data_term, storage))
     done = 0
